@@ -1,0 +1,277 @@
+/*
+ * simple355 — C ABI of the MI355X SIMPLE / k-epsilon hot path.
+ *
+ * The boundary replaces, one for one, the member surface of the reference's
+ * NavierStokesSolver (NS/navier.h:173-206) and the solve surface of its
+ * DataPartition (NS/MPIStructure.h:224-228).  Every entry point takes plain
+ * pointers and sizes; device memory is owned by the context.  Return codes:
+ *   S355_OK (0)               success
+ *   S355_NOT_CONVERGED (1)    a Krylov solve hit its cap or diverged; the
+ *                             s355_solve_info records say where (this is the
+ *                             reference's ConvergeError, BasicType.h:62-68)
+ *   < 0                       bad argument, bad mesh, HIP error;
+ *                             s355_last_error() holds the message.
+ * Calls on one context are made from one host thread; contexts are
+ * independent.  Field arrays are in the reference's cell / face / boundary
+ * order (geometry.cpp:222-358), so host arrays copy in and out unpermuted.
+ */
+#ifndef SIMPLE355_H
+#define SIMPLE355_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define S355_OK 0
+#define S355_NOT_CONVERGED 1
+#define S355_E_ARG (-1)
+#define S355_E_HIP (-2)
+#define S355_E_MESH (-3)
+#define S355_E_STATE (-4)
+#define S355_E_NOMEM (-5)
+
+#define S355_MAX_CELL_FACES 12
+
+/* BdRegion (BasicType.h:247-310): type1 1 wall, 2 inlet, 3 outlet,
+ * 4 symmetry, 5 volume.  init = [u,v,w,rho,T,mu,k,eps] for a volume
+ * (navier.cpp:621-633) and [u,v,w,p,rho,T,k,eps] for an inlet
+ * (readparamfile.cpp:238-247). */
+typedef struct s355_region {
+  int rid;
+  int type1;
+  int type2;
+  double fixed_value;
+  double init[10];
+} s355_region;
+
+/* Element stream as RootProcess::read hands it on (RootStructure.cpp:428-487):
+ * gmsh type codes 2 tri, 3 quad, 4 tet, 5 hex, 6 prism, 7 pyramid; tag =
+ * physical region id; vertex ids 0-based. */
+typedef struct s355_mesh_desc {
+  int n_vert;
+  const double* xyz; /* 3 * n_vert */
+  int n_elem;
+  const int* elem_type;
+  const int* elem_tag;
+  const int* elem_vptr; /* n_elem + 1 */
+  const int* elem_vidx;
+  int n_region;
+  const s355_region* regions;
+} s355_mesh_desc;
+
+enum { S355_KSP_GMRES = 0, S355_KSP_CG = 1 };
+enum { S355_PC_NONE = 0, S355_PC_JACOBI = 1, S355_PC_ILU0 = 2 };
+
+/* One KSP configuration (KSPSetType / KSPGMRESSetRestart / PCSetType /
+ * KSPSetTolerances, MPIStructure.cpp:248-262, 394-410, 460-481). */
+typedef struct s355_ksp_opts {
+  int type;
+  int restart;
+  int pc;
+  int maxit;
+  double rtol;
+  double abstol;
+  double dtol;
+} s355_ksp_opts;
+
+enum { S355_FP64 = 0, S355_MIXED = 1 };
+
+enum { S355_KSP_VELOCITY = 0, S355_KSP_PRESSURE = 1, S355_KSP_SCALAR = 2 };
+
+typedef struct s355_params {
+  int turb_model;        /* TurModel: 0 laminar, 1 k-epsilon */
+  int density_model;     /* DensityModel: only 0 (constant density) */
+  int steady;            /* IfSteady: only 1 */
+  int cell_pressure_ref; /* 0-based cell of the pressure reference */
+  double urf[8];         /* URF[0..7] (navier.cpp:500-507) */
+  double gravity[3];
+  double residual_steady;
+  s355_ksp_opts ksp[3]; /* velocity, pressure correction, scalars */
+  int precision;        /* S355_FP64 or S355_MIXED */
+  int ir_max;           /* mixed: cap on FP64 refinement steps per solve */
+  double ir_inner_rtol; /* mixed: relative tolerance of each FP32 inner solve */
+} s355_params;
+
+typedef struct s355_solve_info {
+  int iters;      /* Krylov iterations (all refinement steps together) */
+  int reason;     /* >0 converged, <0 diverged (KSPConvergedReason codes) */
+  double rnorm;   /* final residual norm used by the test */
+  double rnorm0;  /* reference norm of the test */
+  int refinements;/* FP64 refinement steps taken (mixed precision) */
+  int pad;
+} s355_solve_info;
+
+/* Field ids for s355_set_field / s355_get_field. */
+enum s355_field {
+  /* per cell, n = Ncel */
+  S355_F_R = 0, S355_F_U, S355_F_V, S355_F_W, S355_F_P, S355_F_T, S355_F_TE, S355_F_ED,
+  S355_F_DELTAP, S355_F_VISLAM, S355_F_VISTUR, S355_F_APR,
+  /* per cell, n = 3 * Ncel, [cell][xyz] */
+  S355_F_DUDX, S355_F_DVDX, S355_F_DWDX, S355_F_DPDX, S355_F_DPHIDX,
+  /* per boundary face, n = Nbnd */
+  S355_F_BRO, S355_F_BU, S355_F_BV, S355_F_BW, S355_F_BPRE, S355_F_BTE, S355_F_BED,
+  /* per boundary face, n = 3 * Nbnd */
+  S355_F_SHEAR,
+  /* per face, n = Nfac */
+  S355_F_RUFACE,
+  /* matrix values in CSR order, n = nnz */
+  S355_F_AU, S355_F_AP, S355_F_AS,
+  /* right-hand sides, n = Ncel */
+  S355_F_BU_RHS, S355_F_BV_RHS, S355_F_BW_RHS, S355_F_BP_RHS, S355_F_BS_RHS,
+  S355_F_COUNT
+};
+
+/* Integer mesh arrays for s355_get_mesh_int. */
+enum s355_mesh_int {
+  S355_M_ELEM_PERM = 0, /* n_elem: element order after the partition sort */
+  S355_M_FACE_C1,       /* Nfac */
+  S355_M_FACE_C2,       /* Nfac: -10 on boundary faces */
+  S355_M_FACE_BND,      /* Nfac: boundary index or -10 */
+  S355_M_FACE_VERT,     /* 4 * Nfac */
+  S355_M_CELL_NFACE,    /* Ncel */
+  S355_M_CELL_FACE,     /* S355_MAX_CELL_FACES * Ncel, -1 padded */
+  S355_M_CELL_CELL,     /* S355_MAX_CELL_FACES * Ncel, -1 padded */
+  S355_M_BND_FACE,      /* Nbnd */
+  S355_M_BND_RID,       /* Nbnd */
+  S355_M_CELL_RID,      /* Ncel */
+  S355_M_CSR_ROWPTR,    /* Ncel + 1 */
+  S355_M_CSR_COL,       /* nnz */
+  S355_M_CSR_DIAG,      /* Ncel */
+  S355_M_INT_COUNT
+};
+
+/* Floating-point mesh metrics for s355_get_mesh_dbl. */
+enum s355_mesh_dbl {
+  S355_G_CELL_VOL = 0, /* Ncel */
+  S355_G_CELL_X,       /* 3 * Ncel */
+  S355_G_FACE_X,       /* 3 * Nfac */
+  S355_G_FACE_N,       /* 3 * Nfac */
+  S355_G_FACE_AREA,    /* Nfac */
+  S355_G_FACE_LAMBDA,  /* Nfac */
+  S355_G_FACE_RLENCOS, /* Nfac */
+  S355_G_FACE_XPAC,    /* 3 * Nfac */
+  S355_G_FACE_XNAC,    /* 3 * Nfac */
+  S355_G_BND_DIST,     /* Nbnd */
+  S355_G_COUNT
+};
+
+typedef struct s355_sizes {
+  int n_vert, n_cell, n_face, n_bnd, n_interior, nnz, n_elem, n_warped;
+} s355_sizes;
+
+typedef struct s355_ctx s355_ctx;
+
+/* Message of the last failing call on this thread. */
+const char* s355_last_error(void);
+
+/* Reference defaults: initSolverParam (navier.cpp:468-537) and the in-code
+ * solve settings (navier_velocity.cpp:28, navier_pressure.cpp:28,
+ * navier_scalar.cpp:98/158) on PETSc's defaults (GMRES(30), ILU(0)). */
+void s355_default_params(s355_params* p);
+
+/* Case generators (synthetic meshes in element-stream form).  kind:
+ *   0 cavity  ip = {n, tets, seed}          dp = {Re, lid_u, jitter}
+ *   1 channel ip = {nx,ny,nz, turbulent, seed} dp = {lx,ly,lz, jitter, split, Re, u_in, intensity}
+ *   2 step    ip = {n_per_h, tets, turbulent, seed} dp = {upstream, downstream, span, jitter, Re, u_in, intensity}
+ * s355_case_build returns an opaque handle; s355_case_sizes fills
+ * {n_vert, n_elem, n_vidx, n_region}; s355_case_copy copies the arrays out. */
+typedef struct s355_case s355_case;
+int s355_case_build(int kind, const int* ip, const double* dp, s355_case** out);
+int s355_case_sizes(const s355_case* c, int* sizes4);
+int s355_case_copy(const s355_case* c, double* xyz, int* type, int* tag, int* vptr, int* vidx,
+                   s355_region* regions);
+void s355_case_free(s355_case* c);
+
+/* --- context: set-up (ReadGridFile / CreateFaces / CellFaceInfo /
+ *     CheckAndAllocate / initPetsc / InitFlowField) --------------------- */
+int s355_ctx_create(const s355_mesh_desc* mesh, const s355_params* params, s355_ctx** out);
+int s355_ctx_destroy(s355_ctx* ctx);
+int s355_ctx_sizes(const s355_ctx* ctx, s355_sizes* out);
+int s355_ctx_set_params(s355_ctx* ctx, const s355_params* params);
+int s355_get_mesh_int(const s355_ctx* ctx, int which, int* out, size_t n);
+int s355_get_mesh_dbl(const s355_ctx* ctx, int which, double* out, size_t n);
+int s355_set_field(s355_ctx* ctx, int field, const double* host, size_t n);
+int s355_get_field(s355_ctx* ctx, int field, double* host, size_t n);
+/* InitFlowField (navier.cpp:588-697) */
+int s355_init_flow_field(s355_ctx* ctx);
+int s355_synchronize(s355_ctx* ctx);
+
+/* --- phase entries (NavierStokesSolver members) -------------------------- */
+int s355_bc_velocity(s355_ctx* ctx);                       /* SetBCVelocity  navier_bc.cpp:8-119   */
+int s355_bc_pressure(s355_ctx* ctx);                       /* SetBCPressure  navier_bc.cpp:121-155 */
+int s355_bc_deltap(s355_ctx* ctx);                         /* SetBCDeltaP    navier_bc.cpp:157-181 */
+/* Gradient (navier_gradient.cpp:12-74): cell field `field` with boundary
+ * values `bfield` into 3-vector field `out`. */
+int s355_gradient(s355_ctx* ctx, int field, int bfield, int out);
+int s355_build_velocity_matrix(s355_ctx* ctx);             /* navier_velocity.cpp:63-393 */
+int s355_solve_velocity(s355_ctx* ctx, double rtol, int maxit, s355_solve_info info[3]); /* MPIStructure.cpp:234-375 */
+int s355_cal_ru_face(s355_ctx* ctx);                       /* CalRUFace2 navier_velocity.cpp:425-476 */
+int s355_build_pressure_matrix(s355_ctx* ctx);             /* navier_pressure.cpp:99-232 */
+int s355_solve_pressure(s355_ctx* ctx, double rtol, int maxit, s355_solve_info* info); /* MPIStructure.cpp:449-531 */
+/* correction block navier_pressure.cpp:39-78 (SetBCDeltaP, Gradient(dp),
+ * P/U/V/W update); adds sum |dp - dp_ref| vol into residual slot 3 and, when
+ * res_p is not NULL, returns this call's value (synchronous). */
+int s355_correct_fields(s355_ctx* ctx, double* res_p);
+int s355_correct_ru_face(s355_ctx* ctx);                   /* CorrectRUFace2 navier_pressure.cpp:235-278 */
+/* residual bookkeeping of CalculateVelocity (navier_velocity.cpp:19-24, 48-52):
+ * snapshot U,V,W before the solve, then add sum |old - new| vol into
+ * residual slots 0..2. */
+int s355_save_velocity(s355_ctx* ctx);
+int s355_velocity_residual(s355_ctx* ctx);
+/* localRes -> Residual (navier.cpp:64-65, 94): read the 10 residual slots;
+ * reset != 0 zeroes them afterwards. Synchronous. */
+int s355_residuals(s355_ctx* ctx, double res[10], int reset);
+
+/* k-epsilon pieces of UpdateTurKEpsilon (navier_scalar.cpp:11-197). */
+int s355_kepsilon_sources(s355_ctx* ctx);                  /* 33-64: mu_k, mu_eps, P_k, S_k, Ap_k */
+int s355_bc_kepsilon(s355_ctx* ctx);                       /* SetBCKEpsilon navier_bc.cpp:220-345 */
+int s355_build_scalar_matrix(s355_ctx* ctx, int isca);     /* BuildScalarMatrix 407-592; 2 = k, 3 = eps */
+int s355_eps_wall_rows(s355_ctx* ctx);                     /* 129-154: wall-cell rows -> identity */
+int s355_solve_scalar(s355_ctx* ctx, int isca, double rtol, int maxit, s355_solve_info* info); /* MPIStructure.cpp:378-446 */
+int s355_kepsilon_post_k(s355_ctx* ctx);                   /* 105-121: clip k, eps sources */
+int s355_kepsilon_post_eps(s355_ctx* ctx);                 /* 165-178: clip eps, mu_t */
+
+/* Device-timed log stages, the counterpart of PetscLogStagePush/Pop around
+ * the phases (navier.cpp:26-37): 1 BuildVelocity, 2 SolveVelocity,
+ * 3 BuildPressure, 4 SolvePressure, 5 other (turbulence, bookkeeping). */
+int s355_log_stage_push(s355_ctx* ctx, int stage);
+int s355_log_stage_pop(s355_ctx* ctx);
+/* ms[0..5]: accumulated device time per stage (index = stage, ms[0] = total
+ * of all stages); reset != 0 clears the accumulators.  Synchronous. */
+int s355_log_stage_times(s355_ctx* ctx, double ms[6], int reset);
+
+/* Phase-level entries, as NSSolve calls them (navier.cpp:68, 70, 78). */
+/* These run the host NavierStokesSolver mirror owned by the context; the
+ * residual contributions go to the device residual slots (s355_residuals). */
+int s355_calculate_velocity(s355_ctx* ctx, s355_solve_info info[3]);
+int s355_calculate_pressure(s355_ctx* ctx, s355_solve_info* info);
+int s355_update_tur_kepsilon(s355_ctx* ctx, s355_solve_info info[2]);
+
+/* NSSolve steady loop body (navier.cpp:52-141) for n_steps steps.  history
+ * receives 11 doubles per step: Residual[0..9] and ResMax.  *steps_done is
+ * the number of completed steps (stops early when ResMax < residual_steady
+ * or a solve fails). */
+int s355_simple_iterate(s355_ctx* ctx, int n_steps, double* history, int* steps_done);
+
+/* Times the production SpMV kernel y = A x on A in {0 Au, 1 Ap, 2 As} (as
+ * last assembled): precision 0 FP64, 1 FP32 copy.  *ms_per_call is the mean
+ * device time over `repeats` launches (HIP events on the context stream). */
+int s355_spmv_time(s355_ctx* ctx, int which, int precision, int repeats, double* ms_per_call);
+
+/* Host-side set-up only (no device): the integer and metric output of the
+ * face builder, for inspection and for tests on machines without a GPU. */
+typedef struct s355_mesh s355_mesh;
+int s355_mesh_build(const s355_mesh_desc* mesh, s355_mesh** out);
+int s355_mesh_sizes(const s355_mesh* m, s355_sizes* out);
+int s355_mesh_get_int(const s355_mesh* m, int which, int* out, size_t n);
+int s355_mesh_get_dbl(const s355_mesh* m, int which, double* out, size_t n);
+void s355_mesh_free(s355_mesh* m);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SIMPLE355_H */

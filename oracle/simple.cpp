@@ -1,0 +1,787 @@
+// ORACLE — test infrastructure only (see oracle.hpp).
+//
+// The per-iteration SIMPLE / k-epsilon path restated from the reference.
+// Arithmetic follows the reference's expressions term by term and in the
+// same evaluation order (built with -ffp-contract=off), so that, for the
+// same inputs, every assembly kernel of the HIP library can be compared to
+// rounding.  Cell loops are written as gathers over a cell's faces in
+// ascending face order, which reproduces the reference's face-ordered
+// scatter (navier_gradient.cpp:22-39) operation for operation.
+//   InitFlowField        navier.cpp:588-697
+//   SetBCVelocity        navier_bc.cpp:8-119     SetBCPressure 121-155
+//   SetBCDeltaP          navier_bc.cpp:157-181   SetBCKEpsilon 220-345
+//   Gradient             navier_gradient.cpp:12-74 (limiter 0)
+//   BuildVelocityMatrix  navier_velocity.cpp:63-393 (steady)
+//   CalRUFace2           navier_velocity.cpp:425-476
+//   CalculateVelocity    navier_velocity.cpp:10-60
+//   CalculatePressure    navier_pressure.cpp:9-97
+//   BuildPressureMatrix  navier_pressure.cpp:99-232 (DensityModel 0)
+//   CorrectRUFace2       navier_pressure.cpp:235-278
+//   UpdateTurKEpsilon    navier_scalar.cpp:11-197 (steady)
+//   BuildScalarMatrix    navier_scalar.cpp:407-592
+//   NSSolve loop         navier.cpp:52-141 (steady: one pass per step)
+// Pins: closed-form assembly at iteration 1 of a uniform cavity, M-matrix
+// sign structure, A_p * 1 = 0 and symmetry to rounding, linear-field
+// gradients, zero Rhie-Chow correction for uniform u and linear p, the
+// continuity identity after CorrectRUFace2, pointwise k-epsilon formulas.
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+
+#include "oracle.hpp"
+
+namespace orc {
+
+namespace kev {  // TurKEpsilonVar (navier.h:228-239)
+const double kappa = 0.419, Cmu = 0.09, Ceps1 = 1.44, Ceps2 = 1.92, Sigma_k = 1.0, Sigma_e = 1.3;
+}
+
+Oracle::Oracle(const s355_mesh_desc& mesh, const s355_params& params) : par(params) {
+  if (par.density_model != 0) throw std::runtime_error("only DensityModel 0 is on the hot path");
+  if (!par.steady) throw std::runtime_error("only steady runs are on the hot path");
+  ingest(mesh);
+  create_faces();
+  cell_face_info();
+  check_faces();
+  build_csr();
+  allocate();
+}
+
+void Oracle::allocate() {
+  for (auto* v : {&R, &U, &V, &W, &P, &T, &TE, &ED, &deltaP, &VisLam, &VisTur, &Apr, &bu, &bv, &bw, &bp, &bs})
+    v->assign(Ncel, 0.0);
+  for (auto* v : {&dPdX, &dUdX, &dVdX, &dWdX, &dPhidX}) v->assign(3 * static_cast<size_t>(Ncel), 0.0);
+  for (auto* v : {&BRo, &BU, &BV, &BW, &BPre, &BTE, &BED, &BTem}) v->assign(Nbnd, 0.0);
+  shear.assign(3 * static_cast<size_t>(Nbnd), 0.0);
+  RUFace.assign(Nfac, 0.0);
+  for (auto* v : {&Au, &Ap, &As}) v->assign(csr.nnz(), 0.0);
+}
+
+std::vector<double>* Oracle::field(int id) {
+  switch (id) {
+    case S355_F_R: return &R;
+    case S355_F_U: return &U;
+    case S355_F_V: return &V;
+    case S355_F_W: return &W;
+    case S355_F_P: return &P;
+    case S355_F_T: return &T;
+    case S355_F_TE: return &TE;
+    case S355_F_ED: return &ED;
+    case S355_F_DELTAP: return &deltaP;
+    case S355_F_VISLAM: return &VisLam;
+    case S355_F_VISTUR: return &VisTur;
+    case S355_F_APR: return &Apr;
+    case S355_F_DUDX: return &dUdX;
+    case S355_F_DVDX: return &dVdX;
+    case S355_F_DWDX: return &dWdX;
+    case S355_F_DPDX: return &dPdX;
+    case S355_F_DPHIDX: return &dPhidX;
+    case S355_F_BRO: return &BRo;
+    case S355_F_BU: return &BU;
+    case S355_F_BV: return &BV;
+    case S355_F_BW: return &BW;
+    case S355_F_BPRE: return &BPre;
+    case S355_F_BTE: return &BTE;
+    case S355_F_BED: return &BED;
+    case S355_F_SHEAR: return &shear;
+    case S355_F_RUFACE: return &RUFace;
+    case S355_F_AU: return &Au;
+    case S355_F_AP: return &Ap;
+    case S355_F_AS: return &As;
+    case S355_F_BU_RHS: return &bu;
+    case S355_F_BV_RHS: return &bv;
+    case S355_F_BW_RHS: return &bw;
+    case S355_F_BP_RHS: return &bp;
+    case S355_F_BS_RHS: return &bs;
+    default: return nullptr;
+  }
+}
+
+void Oracle::InitFlowField() {
+  auto apply = [&](int i, const s355_region& r) {
+    U[i] = r.init[0];
+    V[i] = r.init[1];
+    W[i] = r.init[2];
+    R[i] = r.init[3];
+    T[i] = r.init[4];
+    P[i] = 0.;
+    VisLam[i] = r.init[5];
+    VisTur[i] = 0.;
+    if (par.turb_model == 1) {
+      TE[i] = r.init[6];
+      ED[i] = r.init[7];
+      VisTur[i] = R[i] * kev::Cmu * TE[i] * TE[i] / (ED[i] + kSmall);
+    }
+  };
+  auto it0 = region.find(0);
+  if (it0 != region.end())
+    for (int i = 0; i < Ncel; ++i) apply(i, it0->second);
+  for (int i = 0; i < Ncel; ++i) apply(i, region.at(cell[i].rid));
+  std::fill(deltaP.begin(), deltaP.end(), 0.0);
+  for (int i = 0; i < Nbnd; ++i) {
+    const s355_region& reg = region.at(bnd[i].rid);
+    if (reg.type1 == 1) {
+      if (reg.type2 == 0) BTem[i] = reg.fixed_value;
+    } else if (reg.type1 == 2) {
+      BU[i] = reg.init[0];
+      BV[i] = reg.init[1];
+      BW[i] = reg.init[2];
+      BRo[i] = reg.init[4];
+      BTem[i] = reg.init[5];
+      BTE[i] = reg.init[6];
+      BED[i] = reg.init[7];
+    }
+  }
+  std::fill(RUFace.begin(), RUFace.end(), 0.0);
+}
+
+void Oracle::SetBCVelocity() {
+  for (int i = 0; i < Nbnd; ++i) {
+    int iface = bnd[i].face;
+    int ic = face[iface].c1;
+    switch (type1_of(bnd[i].rid)) {
+      case 1:
+        BRo[i] = R[ic];
+        BU[i] = 0.;
+        BV[i] = 0.;
+        BW[i] = 0.;
+        break;
+      case 2:
+        break;
+      case 3:
+        BRo[i] = R[ic];
+        BU[i] = U[ic];
+        BV[i] = V[ic];
+        BW[i] = W[ic];
+        break;
+      case 4: {
+        const Face& f = face[iface];
+        double s1 = f.n[0] / f.area, s2 = f.n[1] / f.area, s3 = f.n[2] / f.area;
+        double un = U[ic] * s1 + V[ic] * s2 + W[ic] * s3;
+        BU[i] = U[ic] - un * s1;
+        BV[i] = V[ic] - un * s2;
+        BW[i] = W[ic] - un * s3;
+        BRo[i] = R[ic];
+        break;
+      }
+      default:
+        throw std::runtime_error("no such boundary type in velocity bc");
+    }
+  }
+  // outlet mass-flow correction (DensityModel 0)
+  double min = 0.0, mout = 0.0, aout = 0.0;
+  for (int i = 0; i < Nbnd; ++i) {
+    int iface = bnd[i].face;
+    int t = type1_of(bnd[i].rid);
+    const double* n = face[iface].n;
+    if (t == 2)
+      min += BRo[i] * (BU[i] * n[0] + BV[i] * n[1] + BW[i] * n[2]);
+    else if (t == 3) {
+      mout += BRo[i] * (BU[i] * n[0] + BV[i] * n[1] + BW[i] * n[2]);
+      aout += BRo[i] * face[i].area;  // Face[i] with Bnd[i].face == i
+    }
+  }
+  if (std::fabs(mout) > kSmall) {
+    double rate = -min / mout;
+    for (int i = 0; i < Nbnd; ++i) {
+      int iface = bnd[i].face;
+      if (type1_of(bnd[i].rid) == 3) {
+        BU[iface] *= rate;
+        BV[iface] *= rate;
+        BW[iface] *= rate;
+      }
+    }
+  } else {
+    double rate = (-min - mout) / aout;
+    for (int i = 0; i < Nbnd; ++i) {
+      int iface = bnd[i].face;
+      if (type1_of(bnd[i].rid) == 3) {
+        const Face& f = face[iface];
+        BU[iface] += rate * f.n[0] / f.area;
+        BV[iface] += rate * f.n[1] / f.area;
+        BW[iface] += rate * f.n[2] / f.area;
+      }
+    }
+  }
+}
+
+void Oracle::SetBCPressure() {
+  for (int i = 0; i < Nbnd; ++i) BPre[i] = P[face[bnd[i].face].c1];
+}
+
+void Oracle::SetBCDeltaP() {
+  for (int i = 0; i < Nbnd; ++i) BPre[i] = deltaP[face[bnd[i].face].c1];
+}
+
+void Oracle::Gradient(const std::vector<double>& phi, const std::vector<double>& bphi, std::vector<double>& g) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < Ncel; ++i) {
+    double a[3] = {0., 0., 0.};
+    const int* fl = &cell_faces_sorted[static_cast<size_t>(i) * kMaxFace];
+    for (int k = 0; k < kMaxFace && fl[k] >= 0; ++k) {
+      const Face& f = face[fl[k]];
+      if (f.c2 >= 0) {
+        double pf = f.lambda * phi[f.c1] + (1. - f.lambda) * phi[f.c2];
+        if (f.c1 == i)
+          for (int d = 0; d < 3; ++d) a[d] += pf * f.n[d];
+        else
+          for (int d = 0; d < 3; ++d) a[d] -= pf * f.n[d];
+      } else {
+        double pf = bphi[f.bnd];
+        for (int d = 0; d < 3; ++d) a[d] += pf * f.n[d];
+      }
+    }
+    for (int d = 0; d < 3; ++d) g[3 * static_cast<size_t>(i) + d] = a[d] / cell[i].vol;
+  }
+}
+
+void Oracle::BuildVelocityMatrix() {
+  SetBCVelocity();
+  SetBCPressure();
+  Gradient(U, BU, dUdX);
+  Gradient(V, BV, dVdX);
+  Gradient(W, BW, dWdX);
+  Gradient(P, BPre, dPdX);
+  const double urf = par.urf[0];
+  int bad_rid = 0;
+#pragma omp parallel for schedule(static) reduction(+ : bad_rid)
+  for (int i = 0; i < Ncel; ++i) {
+    double app = 0., su = 0., sv = 0., sw = 0.;
+    double apn[kMaxFace];
+    int ani[kMaxFace];
+    int nj = 0;
+    for (int j = 0; j < kMaxFace; ++j) apn[j] = 0.;
+    const Cell& C = cell[i];
+    for (int j = 0; j < C.nface; ++j) {
+      int iface = C.face[j];
+      const Face& f = face[iface];
+      int ip = f.c1, in = f.c2;
+      double fde[3], fdi[3], fcs[3] = {0., 0., 0.};
+      if (in < 0) {
+        int b = f.bnd;
+        double s1 = f.n[0], s2 = f.n[1], s3 = f.n[2];
+        double RUn = RUFace[iface];
+        double Visc = VisLam[i] + VisTur[i];
+        const double* gu = &dUdX[3 * static_cast<size_t>(i)];
+        const double* gv = &dVdX[3 * static_cast<size_t>(i)];
+        const double* gw = &dWdX[3 * static_cast<size_t>(i)];
+        double dudx = gu[0], dudy = gu[1], dudz = gu[2];
+        double dvdx = gv[0], dvdy = gv[1], dvdz = gv[2];
+        double dwdx = gw[0], dwdy = gw[1], dwdz = gw[2];
+        double dxc[3] = {f.x[0] - C.x[0], f.x[1] - C.x[1], f.x[2] - C.x[2]};
+        int t = type1_of(bnd[b].rid);
+        if (t == 1) {
+          double val = Visc * f.rlencos;
+          app += val;
+          double* sh = &shear[3 * static_cast<size_t>(b)];
+          sh[0] = (dudx + dudx) * s1 + (dudy + dvdx) * s2 + (dudz + dwdx) * s3;
+          sh[1] = (dvdx + dudy) * s1 + (dvdy + dvdy) * s2 + (dvdz + dwdy) * s3;
+          sh[2] = (dwdx + dudz) * s1 + (dwdy + dvdz) * s2 + (dwdz + dwdz) * s3;
+          sh[0] *= Visc;
+          sh[1] *= Visc;
+          sh[2] *= Visc;
+          sh[0] += P[i] * s1;
+          sh[1] += P[i] * s2;
+          sh[2] += P[i] * s3;
+          fde[0] = val * U[i];
+          fde[1] = val * V[i];
+          fde[2] = val * W[i];
+          fdi[0] = val * U[i];
+          fdi[1] = val * V[i];
+          fdi[2] = val * W[i];
+        } else if (t >= 2 && t <= 4) {
+          double val = Visc * f.rlencos;
+          app += val;
+          fde[0] = (dudx + dudx) * s1 + (dudy + dvdx) * s2 + (dudz + dwdx) * s3;
+          fde[1] = (dvdx + dudy) * s1 + (dvdy + dvdy) * s2 + (dvdz + dwdy) * s3;
+          fde[2] = (dwdx + dudz) * s1 + (dwdy + dvdz) * s2 + (dwdz + dwdz) * s3;
+          fde[0] *= Visc;
+          fde[1] *= Visc;
+          fde[2] *= Visc;
+          fdi[0] = val * (dudx * dxc[0] + dudy * dxc[1] + dudz * dxc[2] - BU[b]);
+          fdi[1] = val * (dvdx * dxc[0] + dvdy * dxc[1] + dvdz * dxc[2] - BV[b]);
+          fdi[2] = val * (dwdx * dxc[0] + dwdy * dxc[1] + dwdz * dxc[2] - BW[b]);
+        } else {
+          bad_rid++;
+          fde[0] = fde[1] = fde[2] = fdi[0] = fdi[1] = fdi[2] = 0.;
+        }
+        if (t == 2 || t == 3) {
+          double fl = std::min(RUn, 0.0);
+          app -= fl;
+          fcs[0] = fl * BU[b];
+          fcs[1] = fl * BV[b];
+          fcs[2] = fl * BW[b];
+        } else {
+          fcs[0] = fcs[1] = fcs[2] = 0.;
+        }
+      } else {
+        double s1, s2, s3, RUn, lambda;
+        if (i != ip) {
+          in = ip;
+          s1 = -f.n[0];
+          s2 = -f.n[1];
+          s3 = -f.n[2];
+          RUn = -RUFace[iface];
+          lambda = 1. - f.lambda;
+        } else {
+          s1 = f.n[0];
+          s2 = f.n[1];
+          s3 = f.n[2];
+          RUn = RUFace[iface];
+          lambda = f.lambda;
+        }
+        double lambda2 = 1. - lambda;
+        const double* gu = &dUdX[3 * static_cast<size_t>(i)];
+        const double* gv = &dVdX[3 * static_cast<size_t>(i)];
+        const double* gw = &dWdX[3 * static_cast<size_t>(i)];
+        const double* hu = &dUdX[3 * static_cast<size_t>(in)];
+        const double* hv = &dVdX[3 * static_cast<size_t>(in)];
+        const double* hw = &dWdX[3 * static_cast<size_t>(in)];
+        double Visc = lambda * (VisLam[i] + VisTur[i]) + lambda2 * (VisLam[in] + VisTur[in]);
+        double dudx = lambda * gu[0] + lambda2 * hu[0];
+        double dudy = lambda * gu[1] + lambda2 * hu[1];
+        double dudz = lambda * gu[2] + lambda2 * hu[2];
+        double dvdx = lambda * gv[0] + lambda2 * hv[0];
+        double dvdy = lambda * gv[1] + lambda2 * hv[1];
+        double dvdz = lambda * gv[2] + lambda2 * hv[2];
+        double dwdx = lambda * gw[0] + lambda2 * hw[0];
+        double dwdy = lambda * gw[1] + lambda2 * hw[1];
+        double dwdz = lambda * gw[2] + lambda2 * hw[2];
+        if (RUn < 0.) {
+          apn[nj] += RUn;
+          app -= RUn;
+        }
+        double val = Visc * f.rlencos;
+        app += val;
+        apn[nj] += -val;
+        ani[nj] = j;
+        nj++;
+        double ufi, vfi, wfi, ufh, vfh, wfh;
+        if (RUn > 0.) {
+          ufi = U[i];
+          vfi = V[i];
+          wfi = W[i];
+          double dx[3] = {f.x[0] - C.x[0], f.x[1] - C.x[1], f.x[2] - C.x[2]};
+          ufh = U[i] + (gu[0] * dx[0] + gu[1] * dx[1] + gu[2] * dx[2]);
+          vfh = V[i] + (gv[0] * dx[0] + gv[1] * dx[1] + gv[2] * dx[2]);
+          wfh = W[i] + (gw[0] * dx[0] + gw[1] * dx[1] + gw[2] * dx[2]);
+        } else {
+          ufi = U[in];
+          vfi = V[in];
+          wfi = W[in];
+          const double* xn = cell[in].x;
+          double dx[3] = {f.x[0] - xn[0], f.x[1] - xn[1], f.x[2] - xn[2]};
+          ufh = U[in] + (hu[0] * dx[0] + hu[1] * dx[1] + hu[2] * dx[2]);
+          vfh = V[in] + (hv[0] * dx[0] + hv[1] * dx[1] + hv[2] * dx[2]);
+          wfh = W[in] + (hw[0] * dx[0] + hw[1] * dx[1] + hw[2] * dx[2]);
+        }
+        fcs[0] = RUn * (ufh - ufi);
+        fcs[1] = RUn * (vfh - vfi);
+        fcs[2] = RUn * (wfh - wfi);
+        fde[0] = (dudx + dudx) * s1 + (dudy + dvdx) * s2 + (dudz + dwdx) * s3;
+        fde[1] = (dvdx + dudy) * s1 + (dvdy + dvdy) * s2 + (dvdz + dwdy) * s3;
+        fde[2] = (dwdx + dudz) * s1 + (dwdy + dvdz) * s2 + (dwdz + dwdz) * s3;
+        fde[0] *= Visc;
+        fde[1] *= Visc;
+        fde[2] *= Visc;
+        const double* xn = cell[in].x;
+        double dxc[3] = {xn[0] - C.x[0], xn[1] - C.x[1], xn[2] - C.x[2]};
+        fdi[0] = val * (dudx * dxc[0] + dudy * dxc[1] + dudz * dxc[2]);
+        fdi[1] = val * (dvdx * dxc[0] + dvdy * dxc[1] + dvdz * dxc[2]);
+        fdi[2] = val * (dwdx * dxc[0] + dwdy * dxc[1] + dwdz * dxc[2]);
+      }
+      su += fde[0] - fdi[0] - fcs[0];
+      sv += fde[1] - fdi[1] - fcs[1];
+      sw += fde[2] - fdi[2] - fcs[2];
+    }
+    app /= urf;
+    Apr[i] = 1. / (app);
+    Au[csr.diag[i]] = app;
+    for (int k = 0; k < nj; ++k) Au[csr.slot[static_cast<size_t>(i) * kMaxFace + ani[k]]] = apn[k];
+    const double vol = C.vol;
+    const double* gp = &dPdX[3 * static_cast<size_t>(i)];
+    bu[i] = su + (1. - urf) * app * U[i] + (-gp[0] + R[i] * par.gravity[0]) * vol;
+    bv[i] = sv + (1. - urf) * app * V[i] + (-gp[1] + R[i] * par.gravity[1]) * vol;
+    bw[i] = sw + (1. - urf) * app * W[i] + (-gp[2] + R[i] * par.gravity[2]) * vol;
+  }
+  if (bad_rid) throw std::runtime_error("unknown boundary type in BuildVelocityMatrix");
+}
+
+void Oracle::CalRUFace2() {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < Nfac; ++i) {
+    const Face& f = face[i];
+    int c1 = f.c1, c2 = f.c2;
+    if (c2 < 0) {
+      int b = f.bnd;
+      double ruf = BU[b] * BRo[b], rvf = BV[b] * BRo[b], rwf = BW[b] * BRo[b];
+      RUFace[i] = ruf * f.n[0] + rvf * f.n[1] + rwf * f.n[2];
+    } else {
+      double lambda = f.lambda, lambda2 = 1. - lambda;
+      double rf = lambda * R[c1] + lambda2 * R[c2];
+      double ruf = lambda * R[c1] * U[c1] + lambda2 * R[c2] * U[c2];
+      double rvf = lambda * R[c1] * V[c1] + lambda2 * R[c2] * V[c2];
+      double rwf = lambda * R[c1] * W[c1] + lambda2 * R[c2] * W[c2];
+      double ru = ruf * f.n[0] + rvf * f.n[1] + rwf * f.n[2];
+      const double* g1 = &dPdX[3 * static_cast<size_t>(c1)];
+      const double* g2 = &dPdX[3 * static_cast<size_t>(c2)];
+      const double *x1 = cell[c1].x, *x2 = cell[c2].x;
+      double dx[3];
+      for (int d = 0; d < 3; ++d) dx[d] = f.xpac[d] - x1[d];
+      double P1 = P[c1] + (g1[0] * dx[0] + g1[1] * dx[1] + g1[2] * dx[2]);
+      for (int d = 0; d < 3; ++d) dx[d] = f.xnac[d] - x2[d];
+      double P2 = P[c2] + (g2[0] * dx[0] + g2[1] * dx[1] + g2[2] * dx[2]);
+      for (int d = 0; d < 3; ++d) dx[d] = f.xnac[d] - f.xpac[d];
+      double d12 = std::sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+      double aprf = lambda * Apr[c1] + lambda2 * Apr[c2];
+      double vol = lambda * cell[c1].vol + lambda2 * cell[c2].vol;
+      double dpx = lambda * g1[0] + lambda2 * g2[0];
+      double dpy = lambda * g1[1] + lambda2 * g2[1];
+      double dpz = lambda * g1[2] + lambda2 * g2[2];
+      double dpn = dpx * dx[0] + dpy * dx[1] + dpz * dx[2];
+      ru -= rf * f.area * aprf / d12 * vol * ((P2 - P1) - dpn);
+      RUFace[i] = ru;
+    }
+  }
+}
+
+void Oracle::BuildPressureMatrix() {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < Ncel; ++i) {
+    const Cell& C = cell[i];
+    double valcen = 0., bpv = 0.;
+    for (int j = 0; j < C.nface; ++j) {
+      int iface = C.face[j];
+      const Face& f = face[iface];
+      if (i == f.c1)
+        bpv += RUFace[iface];
+      else
+        bpv -= RUFace[iface];
+      int in = C.cell[j];
+      if (in < 0) continue;
+      double lambda = (i == f.c1) ? f.lambda : 1. - f.lambda;
+      double lambda2 = 1. - lambda;
+      double roapf = R[i] * Apr[i] * lambda + R[in] * Apr[in] * lambda2;
+      double vol = C.vol * lambda + cell[in].vol * lambda2;
+      double tmp = roapf * vol * f.rlencos;
+      valcen += -tmp;
+      Ap[csr.slot[static_cast<size_t>(i) * kMaxFace + j]] = tmp;
+    }
+    Ap[csr.diag[i]] = valcen;
+    bp[i] = bpv;
+  }
+}
+
+void Oracle::CorrectFields(double* res_p) {
+  SetBCDeltaP();
+  Gradient(deltaP, BPre, dPdX);
+  double ref = deltaP[par.cell_pressure_ref];
+  double res = 0.;
+  for (int i = 0; i < Ncel; ++i) res += std::fabs(deltaP[i] - ref) * cell[i].vol;
+  const double urf = par.urf[3];
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < Ncel; ++i) {
+    P[i] += urf * (deltaP[i] - ref);
+    double coef = cell[i].vol * Apr[i];
+    U[i] -= coef * dPdX[3 * static_cast<size_t>(i)];
+    V[i] -= coef * dPdX[3 * static_cast<size_t>(i) + 1];
+    W[i] -= coef * dPdX[3 * static_cast<size_t>(i) + 2];
+  }
+  *res_p += res;
+}
+
+void Oracle::CorrectRUFace2() {
+  const double urf = par.urf[3];
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < Nfac; ++i) {
+    const Face& f = face[i];
+    int c1 = f.c1, c2 = f.c2;
+    if (c2 < 0) {
+      int b = f.bnd;
+      double ruf = BU[b] * BRo[b], rvf = BV[b] * BRo[b], rwf = BW[b] * BRo[b];
+      RUFace[i] = ruf * f.n[0] + rvf * f.n[1] + rwf * f.n[2];
+      continue;
+    }
+    double lambda = f.lambda, lambda2 = 1. - lambda;
+    const double *x1 = cell[c1].x, *x2 = cell[c2].x;
+    double dx1[3], dx2[3];
+    for (int d = 0; d < 3; ++d) {
+      dx1[d] = f.xpac[d] - x1[d];
+      dx2[d] = f.xnac[d] - x2[d];
+    }
+    double roapf = R[c1] * Apr[c1] * lambda + R[c2] * Apr[c2] * lambda2;
+    double vol = cell[c1].vol * lambda + cell[c2].vol * lambda2;
+    double coef = roapf * vol * f.rlencos;
+    const double* g1 = &dPdX[3 * static_cast<size_t>(c1)];
+    const double* g2 = &dPdX[3 * static_cast<size_t>(c2)];
+    double dp1 = deltaP[c1] + urf * (g1[0] * dx1[0] + g1[1] * dx1[1] + g1[2] * dx1[2]);
+    double dp2 = deltaP[c2] + urf * (g2[0] * dx2[0] + g2[1] * dx2[1] + g2[2] * dx2[2]);
+    RUFace[i] += -coef * (dp2 - dp1);
+  }
+}
+
+int Oracle::SolveVelocity(double rtol, int maxit) {
+  s355_ksp_opts o = par.ksp[S355_KSP_VELOCITY];
+  o.rtol = rtol;
+  o.maxit = maxit;
+  last[0] = ksp_solve(csr, Au, bu, U, o, false);
+  last[1] = ksp_solve(csr, Au, bv, V, o, false);
+  last[2] = ksp_solve(csr, Au, bw, W, o, false);
+  for (int k = 0; k < 3; ++k)
+    if (last[k].reason < 0) return S355_NOT_CONVERGED;
+  return S355_OK;
+}
+
+int Oracle::SolvePressure(double rtol, int maxit) {
+  s355_ksp_opts o = par.ksp[S355_KSP_PRESSURE];
+  o.rtol = rtol;
+  o.maxit = maxit;
+  last[3] = ksp_solve(csr, Ap, bp, deltaP, o, true);
+  return last[3].reason < 0 ? S355_NOT_CONVERGED : S355_OK;
+}
+
+int Oracle::CalculateVelocity() {
+  BuildVelocityMatrix();
+  std::vector<double> u0(U), v0(V), w0(W);
+  const s355_ksp_opts& o = par.ksp[S355_KSP_VELOCITY];
+  int rc = SolveVelocity(o.rtol, o.maxit);
+  if (rc != S355_OK) return rc;
+  for (int i = 0; i < Ncel; ++i) {
+    localRes[0] += std::fabs(u0[i] - U[i]) * cell[i].vol;
+    localRes[1] += std::fabs(v0[i] - V[i]) * cell[i].vol;
+    localRes[2] += std::fabs(w0[i] - W[i]) * cell[i].vol;
+  }
+  return S355_OK;
+}
+
+int Oracle::CalculatePressure() {
+  SetBCVelocity();
+  CalRUFace2();
+  BuildPressureMatrix();
+  const s355_ksp_opts& o = par.ksp[S355_KSP_PRESSURE];
+  int rc = SolvePressure(o.rtol, o.maxit);
+  if (rc != S355_OK) return rc;
+  CorrectFields(&localRes[3]);
+  SetBCVelocity();
+  CorrectRUFace2();
+  return S355_OK;
+}
+
+void Oracle::SetBCKEpsilon(std::vector<double>& tes, std::vector<double>&, std::vector<double>& apte,
+                           std::vector<double>&, std::vector<double>& prod) {
+  const double Cmu25 = std::pow(kev::Cmu, 0.25);
+  const double Cmu75 = std::pow(kev::Cmu, 0.75);
+  for (int i = 0; i < Nbnd; ++i) {
+    if (type1_of(bnd[i].rid) != 1) continue;
+    int iface = bnd[i].face;
+    int ic = face[iface].c1;
+    const Face& f = face[iface];
+    BTE[i] = TE[ic];
+    BED[i] = ED[ic];
+    double vol = cell[ic].vol;
+    double s1 = f.n[0] / (f.area + 1.e-16), s2 = f.n[1] / (f.area + 1.e-16), s3 = f.n[2] / (f.area + 1.e-16);
+    double vnor = U[ic] * s1 + V[ic] * s2 + W[ic] * s3;
+    double vt1 = U[ic] - vnor * s1, vt2 = V[ic] - vnor * s2, vt3 = W[ic] - vnor * s3;
+    double vel = std::sqrt(vt1 * vt1 + vt2 * vt2 + vt3 * vt3);
+    double d = bnd[i].distance;
+    double utau = Cmu25 * std::sqrt(TE[ic]);
+    double yplus = d * utau * R[ic] / (VisLam[ic] + kSmall);
+    double tauw = VisLam[ic] * vel / d;
+    if (yplus < 11.)
+      utau = std::sqrt(tauw / R[ic]);
+    else
+      utau = Cmu25 * std::sqrt(TE[ic]);
+    prod[ic] = tauw * utau / (kev::kappa * d);
+    double eps = Cmu75 * std::pow(TE[ic], 1.5) / (kev::kappa * d);
+    tes[ic] = prod[ic] * vol;
+    apte[ic] -= R[ic] * ED[ic] / TE[ic] * vol;
+    apte[ic] += R[ic] * eps / TE[ic] * vol;
+    ED[ic] = eps;
+    BED[i] = eps;
+  }
+  for (int i = 0; i < Nbnd; ++i) {
+    int t = type1_of(bnd[i].rid);
+    if (t == 3 || t == 4) {
+      int ic = face[bnd[i].face].c1;
+      BTE[i] = TE[ic];
+      BED[i] = ED[ic];
+    }
+  }
+}
+
+void Oracle::BuildScalarMatrix(std::vector<double>& phi, std::vector<double>& bphi, const std::vector<double>& diff,
+                               const std::vector<double>& source, const std::vector<double>& app_in) {
+  Gradient(phi, bphi, dPhidX);
+  const double urf = par.urf[5];
+  int inlet_outflow = 0;
+#pragma omp parallel for schedule(static) reduction(+ : inlet_outflow)
+  for (int i = 0; i < Ncel; ++i) {
+    const Cell& C = cell[i];
+    double app = app_in[i];
+    double sphi = source[i];
+    double apn[kMaxFace];
+    int ani[kMaxFace];
+    int nj = 0;
+    for (int j = 0; j < kMaxFace; ++j) apn[j] = 0.;
+    const double* g = &dPhidX[3 * static_cast<size_t>(i)];
+    for (int j = 0; j < C.nface; ++j) {
+      int iface = C.face[j];
+      const Face& f = face[iface];
+      int ip = f.c1, in = f.c2;
+      double fde, fdi, fcs;
+      if (in < 0) {
+        int b = f.bnd;
+        int t = type1_of(bnd[b].rid);
+        double s1 = f.n[0], s2 = f.n[1], s3 = f.n[2];
+        double RUn = RUFace[iface];
+        double Visc = diff[i];
+        double dxc[3] = {f.x[0] - C.x[0], f.x[1] - C.x[1], f.x[2] - C.x[2]};
+        double val = Visc * f.rlencos;
+        app += val;
+        fde = Visc * (g[0] * s1 + g[1] * s2 + g[2] * s3);
+        fdi = val * (g[0] * dxc[0] + g[1] * dxc[1] + g[2] * dxc[2] - bphi[b]);
+        if (t == 2 || t == 3) {
+          if (t == 2 && RUn > 0) inlet_outflow++;
+          double fl = std::min(RUn, 0.0);
+          app -= fl;
+          fcs = fl * bphi[b];
+        } else {
+          fcs = 0.;
+        }
+      } else {
+        double s1, s2, s3, RUn, lambda;
+        if (i != ip) {
+          in = ip;
+          s1 = -f.n[0];
+          s2 = -f.n[1];
+          s3 = -f.n[2];
+          RUn = -RUFace[iface];
+          lambda = 1. - f.lambda;
+        } else {
+          s1 = f.n[0];
+          s2 = f.n[1];
+          s3 = f.n[2];
+          RUn = RUFace[iface];
+          lambda = f.lambda;
+        }
+        double lambda2 = 1. - lambda;
+        const double* h = &dPhidX[3 * static_cast<size_t>(in)];
+        double Visc = lambda * diff[i] + lambda2 * diff[in];
+        double dpx = lambda * g[0] + lambda2 * h[0];
+        double dpy = lambda * g[1] + lambda2 * h[1];
+        double dpz = lambda * g[2] + lambda2 * h[2];
+        if (RUn < 0.) {
+          apn[nj] += RUn;
+          app -= RUn;
+        }
+        double val = Visc * f.rlencos;
+        app += val;
+        apn[nj] += -val;
+        ani[nj] = j;
+        nj++;
+        double pfi, pfh;
+        if (RUn > 0.) {
+          pfi = phi[i];
+          double dx[3] = {f.x[0] - C.x[0], f.x[1] - C.x[1], f.x[2] - C.x[2]};
+          pfh = phi[i] + (g[0] * dx[0] + g[1] * dx[1] + g[2] * dx[2]);
+        } else {
+          pfi = phi[in];
+          const double* xn = cell[in].x;
+          double dx[3] = {f.x[0] - xn[0], f.x[1] - xn[1], f.x[2] - xn[2]};
+          pfh = phi[in] + (h[0] * dx[0] + h[1] * dx[1] + h[2] * dx[2]);
+        }
+        fcs = RUn * (pfh - pfi);
+        fde = Visc * (dpx * s1 + dpy * s2 + dpz * s3);
+        const double* xn = cell[in].x;
+        double dxc[3] = {xn[0] - C.x[0], xn[1] - C.x[1], xn[2] - C.x[2]};
+        fdi = val * (dpx * dxc[0] + dpy * dxc[1] + dpz * dxc[2]);
+      }
+      sphi += fde - fdi - fcs;
+    }
+    app /= urf;
+    As[csr.diag[i]] = app;
+    for (int k = 0; k < nj; ++k) As[csr.slot[static_cast<size_t>(i) * kMaxFace + ani[k]]] = apn[k];
+    bs[i] = sphi + (1. - urf) * app * phi[i];
+  }
+  if (inlet_outflow) throw std::runtime_error("reverse flow get out of inlet");
+}
+
+int Oracle::UpdateTurKEpsilon() {
+  std::vector<double> prod(Ncel, 0.), tes(Ncel, 0.), eds(Ncel, 0.), viste(Ncel), visted(Ncel), apte(Ncel, 0.),
+      aped(Ncel, 0.);
+  for (int i = 0; i < Ncel; ++i) {
+    double vol = cell[i].vol;
+    viste[i] = VisLam[i] + VisTur[i] / kev::Sigma_k;
+    visted[i] = VisLam[i] + VisTur[i] / kev::Sigma_e;
+    const double* du = &dUdX[3 * static_cast<size_t>(i)];
+    const double* dv = &dVdX[3 * static_cast<size_t>(i)];
+    const double* dw = &dWdX[3 * static_cast<size_t>(i)];
+    double s1 = (du[0] + du[0]) * du[0] + (du[1] + dv[0]) * du[1] + (du[2] + dw[0]) * du[2];
+    double s2 = (dv[0] + du[1]) * dv[0] + (dv[1] + dv[1]) * dv[1] + (dv[2] + dw[1]) * dv[2];
+    double s3 = (dw[0] + du[2]) * dw[0] + (dw[1] + dv[2]) * dw[1] + (dw[2] + dw[2]) * dw[2];
+    prod[i] = VisTur[i] * (s1 + s2 + s3);
+    double dis = R[i] * ED[i];
+    tes[i] = prod[i] * vol;
+    apte[i] = dis / TE[i] * vol;
+  }
+  SetBCKEpsilon(tes, eds, apte, aped, prod);
+  BuildScalarMatrix(TE, BTE, viste, tes, apte);
+  s355_ksp_opts o = par.ksp[S355_KSP_SCALAR];
+  last[4] = ksp_solve(csr, As, bs, TE, o, false);
+  if (last[4].reason < 0) return S355_NOT_CONVERGED;
+  for (int i = 0; i < Ncel; ++i)
+    if (TE[i] < 1.e-9) TE[i] = 1.e-9;
+  for (int i = 0; i < Ncel; ++i) {
+    double fact = ED[i] / (TE[i] + kSmall) * cell[i].vol;
+    eds[i] = kev::Ceps1 * fact * prod[i];
+    aped[i] += kev::Ceps2 * fact * R[i];
+  }
+  BuildScalarMatrix(ED, BED, visted, eds, aped);
+  for (int i = 0; i < Nbnd; ++i) {
+    if (type1_of(bnd[i].rid) != 1) continue;
+    int ic = face[bnd[i].face].c1;
+    As[csr.diag[ic]] = 1.0;
+    for (int j = 0; j < cell[ic].nface; ++j) {
+      int s = csr.slot[static_cast<size_t>(ic) * kMaxFace + j];
+      if (s >= 0) As[s] = 0.0;
+    }
+    bs[ic] = ED[ic];
+  }
+  last[5] = ksp_solve(csr, As, bs, ED, o, false);
+  if (last[5].reason < 0) return S355_NOT_CONVERGED;
+  for (int i = 0; i < Ncel; ++i)
+    if (ED[i] < 1.e-12) ED[i] = 1.e-12;
+  for (int i = 0; i < Ncel; ++i) {
+    if (ED[i] > 1.e-12)
+      VisTur[i] = kev::Cmu * R[i] * TE[i] * TE[i] / (ED[i] + kSmall);
+    else
+      VisTur[i] = 0.;
+  }
+  return S355_OK;
+}
+
+int Oracle::SimpleIterate(int n_steps, double* history, int* done) {
+  *done = 0;
+  for (int step = 1; step <= n_steps; ++step) {
+    std::fill(localRes, localRes + 10, 0.0);
+    int rc = CalculateVelocity();
+    if (rc != S355_OK) return rc;
+    rc = CalculatePressure();
+    if (rc != S355_OK) return rc;
+    if (par.turb_model == 1) {
+      rc = UpdateTurKEpsilon();
+      if (rc != S355_OK) return rc;
+    }
+    double resmax = 0.;
+    for (int k = 0; k < 10; ++k) resmax = std::max(resmax, localRes[k]);
+    if (history) {
+      for (int k = 0; k < 10; ++k) history[11 * (step - 1) + k] = localRes[k];
+      history[11 * (step - 1) + 10] = resmax;
+    }
+    *done = step;
+    if (resmax < par.residual_steady) break;
+  }
+  return S355_OK;
+}
+
+}  // namespace orc

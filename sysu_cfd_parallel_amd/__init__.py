@@ -1,0 +1,350 @@
+"""MI355X SIMPLE / k-epsilon hot path: Python access to the HIP library.
+
+The library (``_build/libsimple355.so``, built by ``sysu_cfd_parallel_amd.build``)
+exports the C ABI of ``include/simple355.h``; this module wraps it with
+ctypes for case generation, the device context and the SIMPLE loop.  There
+is no CPU fallback: without the built library or a GPU the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+from . import _abi
+from ._abi import (FIELD_ID, FIELD_KIND, MESH_DBL_ID, MESH_INT_ID, FP64, KSP_CG, KSP_GMRES, MIXED, PC_ILU0,
+                   PC_JACOBI, PC_NONE, KspOpts, MeshDesc, Params, Region, Sizes, SolveInfo, field_len,
+                   mesh_dbl_len, mesh_int_len)
+
+LIB_PATH = Path(__file__).resolve().parent / "_build" / "libsimple355.so"
+
+__all__ = [
+    "LIB_PATH", "library", "CaseData", "cavity", "channel", "step", "default_params", "set_all_ksp", "HostMesh",
+    "Solver", "S355Error", "NotConverged", "FP64", "MIXED", "KSP_GMRES", "KSP_CG", "PC_NONE", "PC_JACOBI",
+    "PC_ILU0",
+]
+
+
+class S355Error(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class NotConverged(S355Error):
+    pass
+
+
+_lib: Optional[C.CDLL] = None
+
+
+def library(path: Optional[Path] = None) -> C.CDLL:
+    """Load the HIP library once; raises if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise S355Error(-1, f"HIP library not built: {p} (run python -m sysu_cfd_parallel_amd.build)")
+    lib = C.CDLL(str(p))
+    _abi.declare(lib, "s355")
+    I, V = C.c_int, C.c_void_p
+    pI, pD = C.POINTER(C.c_int), C.POINTER(C.c_double)
+    more = {
+        "default_params": (None, [C.POINTER(Params)]),
+        "case_build": (I, [I, pI, pD, C.POINTER(V)]),
+        "case_sizes": (I, [V, pI]),
+        "case_copy": (I, [V, pD, pI, pI, pI, pI, C.POINTER(Region)]),
+        "case_free": (None, [V]),
+        "mesh_build": (I, [C.POINTER(MeshDesc), C.POINTER(V)]),
+        "mesh_sizes": (I, [V, C.POINTER(Sizes)]),
+        "mesh_get_int": (I, [V, I, pI, C.c_size_t]),
+        "mesh_get_dbl": (I, [V, I, pD, C.c_size_t]),
+        "mesh_free": (None, [V]),
+        "ctx_create": (I, [C.POINTER(MeshDesc), C.POINTER(Params), C.POINTER(V)]),
+        "ctx_destroy": (I, [V]),
+        "ctx_sizes": (I, [V, C.POINTER(Sizes)]),
+        "ctx_set_params": (I, [V, C.POINTER(Params)]),
+        "synchronize": (I, [V]),
+        "save_velocity": (I, [V]),
+        "velocity_residual": (I, [V]),
+        "residuals": (I, [V, pD, I]),
+        "kepsilon_sources": (I, [V]),
+        "bc_kepsilon": (I, [V]),
+        "build_scalar_matrix": (I, [V, I]),
+        "eps_wall_rows": (I, [V]),
+        "solve_scalar": (I, [V, I, C.c_double, I, C.POINTER(SolveInfo)]),
+        "kepsilon_post_k": (I, [V]),
+        "kepsilon_post_eps": (I, [V]),
+        "log_stage_push": (I, [V, I]),
+        "log_stage_pop": (I, [V]),
+        "log_stage_times": (I, [V, pD, I]),
+        "calculate_velocity": (I, [V, C.POINTER(SolveInfo)]),
+        "calculate_pressure": (I, [V, C.POINTER(SolveInfo)]),
+        "update_tur_kepsilon": (I, [V, C.POINTER(SolveInfo)]),
+        "spmv_time": (I, [V, I, I, I, pD]),
+    }
+    for name, (res, args) in more.items():
+        fn = getattr(lib, f"s355_{name}")
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(lib: C.CDLL, rc: int, what: str) -> int:
+    if rc == 0:
+        return rc
+    msg = (lib.s355_last_error() or b"").decode(errors="replace")
+    if rc == _abi.S355_NOT_CONVERGED:
+        raise NotConverged(rc, f"{what}: {msg}")
+    raise S355Error(rc, f"{what}: {msg}")
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+@dataclass
+class CaseData:
+    """A mesh in element-stream form plus its region table."""
+    xyz: np.ndarray
+    etype: np.ndarray
+    tag: np.ndarray
+    vptr: np.ndarray
+    vidx: np.ndarray
+    regions: list
+    meta: dict = field(default_factory=dict)
+    _keep: list = field(default_factory=list, repr=False)
+
+    @property
+    def n_cell_elems(self) -> int:
+        return int(np.count_nonzero(self.etype >= 4))
+
+    def desc(self) -> MeshDesc:
+        regs = (Region * len(self.regions))(*self.regions)
+        self._keep = [regs]
+        return MeshDesc(len(self.xyz) // 3, _dp(self.xyz), len(self.etype), _ip(self.etype), _ip(self.tag),
+                        _ip(self.vptr), _ip(self.vidx), len(self.regions), regs)
+
+    def region(self, rid: int) -> Region:
+        for r in self.regions:
+            if r.rid == rid:
+                return r
+        raise KeyError(rid)
+
+
+def build_case(kind: int, ip: list, dp: list, meta: Optional[dict] = None) -> CaseData:
+    lib = library()
+    ipa = np.asarray(ip, dtype=np.int32)
+    dpa = np.asarray(dp, dtype=np.float64)
+    h = C.c_void_p()
+    _check(lib, lib.s355_case_build(kind, _ip(ipa), _dp(dpa), C.byref(h)), "case_build")
+    try:
+        sz = np.zeros(4, dtype=np.int32)
+        _check(lib, lib.s355_case_sizes(h, _ip(sz)), "case_sizes")
+        nv, ne, nvi, nr = (int(x) for x in sz)
+        xyz = np.empty(3 * nv)
+        et = np.empty(ne, dtype=np.int32)
+        tg = np.empty(ne, dtype=np.int32)
+        vp = np.empty(ne + 1, dtype=np.int32)
+        vi = np.empty(nvi, dtype=np.int32)
+        regs = (Region * nr)()
+        _check(lib, lib.s355_case_copy(h, _dp(xyz), _ip(et), _ip(tg), _ip(vp), _ip(vi), regs), "case_copy")
+    finally:
+        lib.s355_case_free(h)
+    return CaseData(xyz, et, tg, vp, vi, [regs[i] for i in range(nr)], dict(meta or {}))
+
+
+def cavity(n: int, re: float = 100.0, lid_u: float = 1.0, jitter: float = 0.0, tets: bool = False,
+           seed: int = 0) -> CaseData:
+    """Lid-driven cavity in the unit cube, n^3 hexes (or 6 n^3 tets)."""
+    return build_case(0, [n, int(tets), seed], [re, lid_u, jitter],
+                      {"kind": "cavity", "n": n, "re": re, "jitter": jitter, "tets": tets})
+
+
+def channel(nx: int, ny: int, nz: int, lx: float = 2.0, ly: float = 1.0, lz: float = 2.0, jitter: float = 0.0,
+            split: float = 0.0, seed: int = 0, re: float = 1000.0, u_in: float = 1.0, turbulent: bool = False,
+            intensity: float = 0.05) -> CaseData:
+    """Channel: inlet x=0, outlet x=lx, walls y=0,ly, symmetry z=0,lz."""
+    return build_case(1, [nx, ny, nz, int(turbulent), seed], [lx, ly, lz, jitter, split, re, u_in, intensity],
+                      {"kind": "channel", "nx": nx, "ny": ny, "nz": nz, "jitter": jitter, "split": split, "re": re,
+                       "turbulent": turbulent})
+
+
+def step(n_per_h: int, upstream: float = 4.0, downstream: float = 30.0, span: float = 4.0, jitter: float = 0.10,
+         tets: bool = True, turbulent: bool = True, seed: int = 0, re: float = 5000.0, u_in: float = 1.0,
+         intensity: float = 0.05) -> CaseData:
+    """Backward-facing step, expansion ratio 2 (step height 1)."""
+    return build_case(2, [n_per_h, int(tets), int(turbulent), seed],
+                      [upstream, downstream, span, jitter, re, u_in, intensity],
+                      {"kind": "step", "n_per_h": n_per_h, "tets": tets, "turbulent": turbulent})
+
+
+def default_params() -> Params:
+    p = Params()
+    library().s355_default_params(C.byref(p))
+    return p
+
+
+def set_all_ksp(p: Params, ksp_type: int = KSP_GMRES, restart: int = 30, pc: int = PC_JACOBI,
+                rtol: Optional[float] = None, maxit: Optional[int] = None) -> Params:
+    """Apply one KSP setting to every solve, as the reference's unprefixed
+    KSPSetFromOptions does for command-line options (MPIStructure.cpp:261)."""
+    for k in range(3):
+        o = p.ksp[k]
+        o.type, o.restart, o.pc = ksp_type, restart, pc
+        if rtol is not None:
+            o.rtol = rtol
+        if maxit is not None:
+            o.maxit = maxit
+    return p
+
+
+class HostMesh:
+    """Host-only mesh set-up (no GPU needed)."""
+
+    def __init__(self, case: CaseData):
+        self.lib = library()
+        self.h = C.c_void_p()
+        d = case.desc()
+        _check(self.lib, self.lib.s355_mesh_build(C.byref(d), C.byref(self.h)), "mesh_build")
+        s = Sizes()
+        _check(self.lib, self.lib.s355_mesh_sizes(self.h, C.byref(s)), "mesh_sizes")
+        self.sizes = s
+
+    def ints(self, name: str) -> np.ndarray:
+        out = np.empty(mesh_int_len(name, self.sizes), dtype=np.int32)
+        _check(self.lib, self.lib.s355_mesh_get_int(self.h, MESH_INT_ID[name], _ip(out), out.size), name)
+        return out
+
+    def dbls(self, name: str) -> np.ndarray:
+        out = np.empty(mesh_dbl_len(name, self.sizes))
+        _check(self.lib, self.lib.s355_mesh_get_dbl(self.h, MESH_DBL_ID[name], _dp(out), out.size), name)
+        return out
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.s355_mesh_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Solver:
+    """One case resident on the GPU (an s355_ctx)."""
+
+    def __init__(self, case: CaseData, params: Params, init: bool = True):
+        self.lib = library()
+        self.case = case
+        self.params = params
+        self.ctx = C.c_void_p()
+        d = case.desc()
+        _check(self.lib, self.lib.s355_ctx_create(C.byref(d), C.byref(params), C.byref(self.ctx)), "ctx_create")
+        s = Sizes()
+        _check(self.lib, self.lib.s355_ctx_sizes(self.ctx, C.byref(s)), "ctx_sizes")
+        self.sizes = s
+        if init:
+            self.call("init_flow_field")
+
+    # --- plumbing
+    def call(self, name: str, *args) -> int:
+        fn = getattr(self.lib, f"s355_{name}")
+        return _check(self.lib, fn(self.ctx, *args), name)
+
+    def get(self, name: str) -> np.ndarray:
+        n = field_len(FIELD_KIND[name], self.sizes)
+        out = np.empty(n)
+        _check(self.lib, self.lib.s355_get_field(self.ctx, FIELD_ID[name], _dp(out), n), f"get {name}")
+        return out
+
+    def set(self, name: str, arr) -> None:
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        _check(self.lib, self.lib.s355_set_field(self.ctx, FIELD_ID[name], _dp(a), a.size), f"set {name}")
+
+    def ints(self, name: str) -> np.ndarray:
+        out = np.empty(mesh_int_len(name, self.sizes), dtype=np.int32)
+        _check(self.lib, self.lib.s355_get_mesh_int(self.ctx, MESH_INT_ID[name], _ip(out), out.size), name)
+        return out
+
+    def dbls(self, name: str) -> np.ndarray:
+        out = np.empty(mesh_dbl_len(name, self.sizes))
+        _check(self.lib, self.lib.s355_get_mesh_dbl(self.ctx, MESH_DBL_ID[name], _dp(out), out.size), name)
+        return out
+
+    def set_params(self, params: Params) -> None:
+        _check(self.lib, self.lib.s355_ctx_set_params(self.ctx, C.byref(params)), "set_params")
+        self.params = params
+
+    # --- phases
+    def gradient(self, field_name: str, bfield: str, out: str) -> None:
+        self.call("gradient", FIELD_ID[field_name], FIELD_ID[bfield], FIELD_ID[out])
+
+    def solve_velocity(self, rtol: float, maxit: int) -> list:
+        info = (SolveInfo * 3)()
+        rc = self.lib.s355_solve_velocity(self.ctx, rtol, maxit, info)
+        _check(self.lib, rc, "solve_velocity")
+        return [i.as_dict() for i in info]
+
+    def solve_pressure(self, rtol: float, maxit: int) -> dict:
+        info = SolveInfo()
+        _check(self.lib, self.lib.s355_solve_pressure(self.ctx, rtol, maxit, C.byref(info)), "solve_pressure")
+        return info.as_dict()
+
+    def correct_fields(self) -> float:
+        r = C.c_double()
+        self.call("correct_fields", C.byref(r))
+        return r.value
+
+    def residuals(self, reset: bool = False) -> np.ndarray:
+        out = np.empty(10)
+        _check(self.lib, self.lib.s355_residuals(self.ctx, _dp(out), int(reset)), "residuals")
+        return out
+
+    def iterate(self, n_steps: int) -> tuple:
+        """Run the NSSolve loop; returns (history[steps, 11], steps_done)."""
+        hist = np.zeros(11 * max(n_steps, 1))
+        done = C.c_int(0)
+        rc = self.lib.s355_simple_iterate(self.ctx, n_steps, _dp(hist), C.byref(done))
+        h = hist.reshape(-1, 11)[: done.value]
+        _check(self.lib, rc, f"simple_iterate (after {done.value} steps)")
+        return h, done.value
+
+    def stage_times(self, reset: bool = False) -> np.ndarray:
+        out = np.zeros(6)
+        _check(self.lib, self.lib.s355_log_stage_times(self.ctx, _dp(out), int(reset)), "log_stage_times")
+        return out
+
+    def spmv_time(self, which: int, precision: int, repeats: int) -> float:
+        ms = C.c_double()
+        _check(self.lib, self.lib.s355_spmv_time(self.ctx, which, precision, repeats, C.byref(ms)), "spmv_time")
+        return ms.value
+
+    def close(self) -> None:
+        if self.ctx:
+            self.lib.s355_ctx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,512 @@
+// extern "C" boundary of the library (include/simple355.h).  Each entry
+// catches every exception and turns it into a return code plus a message
+// for s355_last_error(); nothing blocks or aborts the process (the
+// reference's ErrorHandler waits on getchar() then aborts, BasicType.h:23-55).
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/simple355.h"
+#include "device/common.hpp"
+#include "device/context.hpp"
+#include "host/navier.hpp"
+#include "mesh/cases.hpp"
+#include "mesh/ingest.hpp"
+
+struct s355_ctx {
+  std::unique_ptr<s355::Context> dev;
+  std::unique_ptr<s355host::NavierStokesSolver> ns;
+};
+
+struct s355_case {
+  s355::ElementStream es;
+};
+
+struct s355_mesh {
+  s355::Mesh m;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const s355host::ConvergeError& e) {
+    g_err = e.varname + " not converged in " + std::to_string(e.iter) + " iterations, residual " +
+            std::to_string(e.residual) + ", reason " + std::to_string(e.reason);
+    return S355_NOT_CONVERGED;
+  } catch (const s355host::DeviceError& e) {
+    g_err = e.msg;
+    return e.code;
+  } catch (const s355::HipError& e) {
+    g_err = e.what();
+    return S355_E_HIP;
+  } catch (const std::invalid_argument& e) {
+    g_err = e.what();
+    return S355_E_ARG;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return S355_E_NOMEM;
+  } catch (const std::logic_error& e) {
+    g_err = e.what();
+    return S355_E_STATE;
+  } catch (const std::runtime_error& e) {
+    g_err = e.what();
+    return S355_E_MESH;
+  } catch (...) {
+    g_err = "unknown error";
+    return S355_E_STATE;
+  }
+}
+
+s355::Context& dev(s355_ctx* c) {
+  if (!c || !c->dev) throw std::invalid_argument("null context");
+  return *c->dev;
+}
+
+void sizes_of(const s355::Mesh& m, s355_sizes* s) {
+  s->n_vert = m.n_vert;
+  s->n_cell = m.n_cell;
+  s->n_face = m.n_face;
+  s->n_bnd = m.n_bnd;
+  s->n_interior = m.n_face - m.n_bnd;
+  s->nnz = m.nnz();
+  s->n_elem = m.n_elem;
+  s->n_warped = m.n_warped;
+}
+
+int mesh_int(const s355::Mesh& m, int which, int* out, size_t n) {
+  const std::vector<int>* v = nullptr;
+  std::vector<int> tmp;
+  switch (which) {
+    case S355_M_ELEM_PERM: v = &m.perm; break;
+    case S355_M_FACE_C1: v = &m.f_c1; break;
+    case S355_M_FACE_C2: v = &m.f_c2; break;
+    case S355_M_FACE_BND: v = &m.f_bnd; break;
+    case S355_M_FACE_VERT: v = &m.f_v; break;
+    case S355_M_CELL_NFACE: v = &m.c_nface; break;
+    case S355_M_CELL_FACE: v = &m.c_face; break;
+    case S355_M_CELL_CELL: v = &m.c_cell; break;
+    case S355_M_BND_FACE:
+      tmp.resize(m.n_bnd);
+      for (int b = 0; b < m.n_bnd; ++b) tmp[b] = b;  // boundary faces come first (geometry.cpp:250-265)
+      v = &tmp;
+      break;
+    case S355_M_BND_RID: v = &m.b_rid; break;
+    case S355_M_CELL_RID: v = &m.c_rid; break;
+    case S355_M_CSR_ROWPTR: v = &m.rowptr; break;
+    case S355_M_CSR_COL: v = &m.col; break;
+    case S355_M_CSR_DIAG: v = &m.diag; break;
+    default: throw std::invalid_argument("unknown mesh int id");
+  }
+  if (!out || n != v->size()) throw std::invalid_argument("mesh int " + std::to_string(which) + ": expected length " +
+                                                          std::to_string(v->size()));
+  std::memcpy(out, v->data(), n * sizeof(int));
+  return S355_OK;
+}
+
+int mesh_dbl(const s355::Mesh& m, int which, double* out, size_t n) {
+  const std::vector<double>* v = nullptr;
+  switch (which) {
+    case S355_G_CELL_VOL: v = &m.c_vol; break;
+    case S355_G_CELL_X: v = &m.c_x; break;
+    case S355_G_FACE_X: v = &m.f_x; break;
+    case S355_G_FACE_N: v = &m.f_n; break;
+    case S355_G_FACE_AREA: v = &m.f_area; break;
+    case S355_G_FACE_LAMBDA: v = &m.f_lambda; break;
+    case S355_G_FACE_RLENCOS: v = &m.f_rlencos; break;
+    case S355_G_FACE_XPAC: v = &m.f_xpac; break;
+    case S355_G_FACE_XNAC: v = &m.f_xnac; break;
+    case S355_G_BND_DIST: v = &m.b_dist; break;
+    default: throw std::invalid_argument("unknown mesh dbl id");
+  }
+  if (!out || n != v->size()) throw std::invalid_argument("mesh dbl " + std::to_string(which) + ": expected length " +
+                                                          std::to_string(v->size()));
+  std::memcpy(out, v->data(), n * sizeof(double));
+  return S355_OK;
+}
+
+void check_params(const s355_params* p) {
+  if (!p) throw std::invalid_argument("null params");
+  for (int k = 0; k < 3; ++k) {
+    const s355_ksp_opts& o = p->ksp[k];
+    if (o.type != S355_KSP_GMRES && o.type != S355_KSP_CG) throw std::invalid_argument("unknown KSP type");
+    if (o.pc < S355_PC_NONE || o.pc > S355_PC_ILU0) throw std::invalid_argument("unknown PC type");
+    if (o.restart < 1 || o.restart > s355::kMaxRestart) throw std::invalid_argument("GMRES restart must be 1..64");
+    if (o.maxit < 1 || !(o.rtol >= 0.0) || !(o.dtol > 0.0)) throw std::invalid_argument("bad KSP tolerances");
+  }
+  if (!(p->urf[0] > 0.0) || !(p->urf[5] > 0.0)) throw std::invalid_argument("relaxation factors must be positive");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* s355_last_error(void) { return g_err.c_str(); }
+
+void s355_default_params(s355_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->turb_model = 0;
+  p->density_model = 0;
+  p->steady = 1;
+  p->cell_pressure_ref = 0;
+  const double urf[8] = {0.6, 0.6, 0.6, 0.5, 0.8, 0.8, 0.8, 0.8};
+  for (int k = 0; k < 8; ++k) p->urf[k] = urf[k];
+  p->residual_steady = 1.e-6;
+  // in-code solve settings on PETSc defaults; the pressure solve asks for CG
+  // because the matrix is flagged symmetric (MPIStructure.cpp:462-465)
+  p->ksp[S355_KSP_VELOCITY] = s355_ksp_opts{S355_KSP_GMRES, 30, S355_PC_ILU0, 1000, 1.e-8, 1.e-50, 1.e5};
+  p->ksp[S355_KSP_PRESSURE] = s355_ksp_opts{S355_KSP_CG, 30, S355_PC_ILU0, 1000, 1.e-6, 1.e-50, 1.e5};
+  p->ksp[S355_KSP_SCALAR] = s355_ksp_opts{S355_KSP_GMRES, 30, S355_PC_ILU0, 500, 1.e-8, 1.e-50, 1.e5};
+  p->precision = S355_FP64;
+  p->ir_max = 30;
+  p->ir_inner_rtol = 1.e-3;
+}
+
+int s355_case_build(int kind, const int* ip, const double* dp, s355_case** out) {
+  return guarded([&] {
+    if (!ip || !dp || !out) throw std::invalid_argument("null argument");
+    auto c = std::make_unique<s355_case>();
+    if (kind == 0) {
+      s355::CavitySpec s;
+      s.n = ip[0];
+      s.tets = ip[1] != 0;
+      s.seed = static_cast<uint64_t>(ip[2]);
+      s.re = dp[0];
+      s.lid_u = dp[1];
+      s.jitter = dp[2];
+      c->es = s355::make_cavity(s);
+    } else if (kind == 1) {
+      s355::ChannelSpec s;
+      s.nx = ip[0];
+      s.ny = ip[1];
+      s.nz = ip[2];
+      s.turbulent = ip[3] != 0;
+      s.seed = static_cast<uint64_t>(ip[4]);
+      s.lx = dp[0];
+      s.ly = dp[1];
+      s.lz = dp[2];
+      s.jitter = dp[3];
+      s.split_frac = dp[4];
+      s.re = dp[5];
+      s.u_in = dp[6];
+      s.intensity = dp[7];
+      c->es = s355::make_channel(s);
+    } else if (kind == 2) {
+      s355::StepCaseSpec s;
+      s.n_per_h = ip[0];
+      s.tets = ip[1] != 0;
+      s.turbulent = ip[2] != 0;
+      s.seed = static_cast<uint64_t>(ip[3]);
+      s.upstream = dp[0];
+      s.downstream = dp[1];
+      s.span = dp[2];
+      s.jitter = dp[3];
+      s.re = dp[4];
+      s.u_in = dp[5];
+      s.intensity = dp[6];
+      c->es = s355::make_step_case(s);
+    } else {
+      throw std::invalid_argument("unknown case kind");
+    }
+    *out = c.release();
+    return S355_OK;
+  });
+}
+
+int s355_case_sizes(const s355_case* c, int* sizes4) {
+  return guarded([&] {
+    if (!c || !sizes4) throw std::invalid_argument("null argument");
+    sizes4[0] = c->es.n_vert();
+    sizes4[1] = c->es.n_elem();
+    sizes4[2] = static_cast<int>(c->es.vidx.size());
+    sizes4[3] = static_cast<int>(c->es.regions.size());
+    return S355_OK;
+  });
+}
+
+int s355_case_copy(const s355_case* c, double* xyz, int* type, int* tag, int* vptr, int* vidx,
+                   s355_region* regions) {
+  return guarded([&] {
+    if (!c) throw std::invalid_argument("null case");
+    const auto& e = c->es;
+    if (xyz) std::memcpy(xyz, e.xyz.data(), e.xyz.size() * sizeof(double));
+    if (type) std::memcpy(type, e.type.data(), e.type.size() * sizeof(int));
+    if (tag) std::memcpy(tag, e.tag.data(), e.tag.size() * sizeof(int));
+    if (vptr) std::memcpy(vptr, e.vptr.data(), e.vptr.size() * sizeof(int));
+    if (vidx) std::memcpy(vidx, e.vidx.data(), e.vidx.size() * sizeof(int));
+    if (regions)
+      for (size_t r = 0; r < e.regions.size(); ++r) {
+        const auto& g = e.regions[r];
+        s355_region o{};
+        o.rid = g.rid;
+        o.type1 = g.type1;
+        o.type2 = g.type2;
+        o.fixed_value = g.fixed_value;
+        for (int k = 0; k < 10; ++k) o.init[k] = g.init[k];
+        regions[r] = o;
+      }
+    return S355_OK;
+  });
+}
+
+void s355_case_free(s355_case* c) { delete c; }
+
+int s355_mesh_build(const s355_mesh_desc* desc, s355_mesh** out) {
+  return guarded([&] {
+    if (!desc || !out) throw std::invalid_argument("null argument");
+    auto m = std::make_unique<s355_mesh>();
+    m->m = s355::ingest_mesh(*desc);
+    *out = m.release();
+    return S355_OK;
+  });
+}
+int s355_mesh_sizes(const s355_mesh* m, s355_sizes* out) {
+  return guarded([&] {
+    if (!m || !out) throw std::invalid_argument("null argument");
+    sizes_of(m->m, out);
+    return S355_OK;
+  });
+}
+int s355_mesh_get_int(const s355_mesh* m, int which, int* out, size_t n) {
+  return guarded([&] {
+    if (!m) throw std::invalid_argument("null mesh");
+    return mesh_int(m->m, which, out, n);
+  });
+}
+int s355_mesh_get_dbl(const s355_mesh* m, int which, double* out, size_t n) {
+  return guarded([&] {
+    if (!m) throw std::invalid_argument("null mesh");
+    return mesh_dbl(m->m, which, out, n);
+  });
+}
+void s355_mesh_free(s355_mesh* m) { delete m; }
+
+int s355_ctx_create(const s355_mesh_desc* mesh, const s355_params* params, s355_ctx** out) {
+  return guarded([&] {
+    if (!mesh || !out) throw std::invalid_argument("null argument");
+    *out = nullptr;
+    check_params(params);
+    auto c = std::make_unique<s355_ctx>();
+    c->dev = std::make_unique<s355::Context>(*mesh, *params);
+    c->ns = std::make_unique<s355host::NavierStokesSolver>(c.get(), *params);
+    *out = c.release();
+    return S355_OK;
+  });
+}
+
+int s355_ctx_destroy(s355_ctx* ctx) {
+  return guarded([&] {
+    delete ctx;
+    return S355_OK;
+  });
+}
+
+int s355_ctx_sizes(const s355_ctx* ctx, s355_sizes* out) {
+  return guarded([&] {
+    if (!ctx || !out) throw std::invalid_argument("null argument");
+    sizes_of(ctx->dev->mesh, out);
+    return S355_OK;
+  });
+}
+
+int s355_ctx_set_params(s355_ctx* ctx, const s355_params* params) {
+  return guarded([&] {
+    check_params(params);
+    dev(ctx).set_params(*params);
+    ctx->ns->set_params(*params);
+    return S355_OK;
+  });
+}
+
+int s355_get_mesh_int(const s355_ctx* ctx, int which, int* out, size_t n) {
+  return guarded([&] {
+    if (!ctx) throw std::invalid_argument("null context");
+    return mesh_int(ctx->dev->mesh, which, out, n);
+  });
+}
+
+int s355_get_mesh_dbl(const s355_ctx* ctx, int which, double* out, size_t n) {
+  return guarded([&] {
+    if (!ctx) throw std::invalid_argument("null context");
+    return mesh_dbl(ctx->dev->mesh, which, out, n);
+  });
+}
+
+int s355_set_field(s355_ctx* ctx, int field, const double* host, size_t n) {
+  return guarded([&] {
+    s355::Context& d = dev(ctx);
+    s355::FieldRef f = d.field(field);
+    if (!host || n != f.n) throw std::invalid_argument("set_field: expected length " + std::to_string(f.n));
+    S355_HIP(hipMemcpyAsync(f.p, host, n * sizeof(double), hipMemcpyHostToDevice, d.stream));
+    d.sync();
+    return S355_OK;
+  });
+}
+
+int s355_get_field(s355_ctx* ctx, int field, double* host, size_t n) {
+  return guarded([&] {
+    s355::Context& d = dev(ctx);
+    s355::FieldRef f = d.field(field);
+    if (!host || n != f.n) throw std::invalid_argument("get_field: expected length " + std::to_string(f.n));
+    S355_HIP(hipMemcpyAsync(host, f.p, n * sizeof(double), hipMemcpyDeviceToHost, d.stream));
+    d.sync();
+    return S355_OK;
+  });
+}
+
+int s355_init_flow_field(s355_ctx* ctx) { return guarded([&] { dev(ctx).init_flow_field(); return S355_OK; }); }
+int s355_synchronize(s355_ctx* ctx) {
+  return guarded([&] {
+    dev(ctx).sync();
+    dev(ctx).check_flags();
+    return S355_OK;
+  });
+}
+
+int s355_bc_velocity(s355_ctx* ctx) { return guarded([&] { dev(ctx).bc_velocity(); return S355_OK; }); }
+int s355_bc_pressure(s355_ctx* ctx) { return guarded([&] { dev(ctx).bc_pressure(); return S355_OK; }); }
+int s355_bc_deltap(s355_ctx* ctx) { return guarded([&] { dev(ctx).bc_deltap(); return S355_OK; }); }
+
+int s355_gradient(s355_ctx* ctx, int field, int bfield, int out) {
+  return guarded([&] { dev(ctx).gradient(field, bfield, out); return S355_OK; });
+}
+
+int s355_build_velocity_matrix(s355_ctx* ctx) {
+  return guarded([&] { dev(ctx).build_velocity_matrix(); return S355_OK; });
+}
+
+int s355_solve_velocity(s355_ctx* ctx, double rtol, int maxit, s355_solve_info info[3]) {
+  return guarded([&] {
+    if (!info) throw std::invalid_argument("null info");
+    return dev(ctx).solve_velocity(rtol, maxit, info);
+  });
+}
+
+int s355_cal_ru_face(s355_ctx* ctx) { return guarded([&] { dev(ctx).cal_ru_face(); return S355_OK; }); }
+int s355_build_pressure_matrix(s355_ctx* ctx) {
+  return guarded([&] { dev(ctx).build_pressure_matrix(); return S355_OK; });
+}
+
+int s355_solve_pressure(s355_ctx* ctx, double rtol, int maxit, s355_solve_info* info) {
+  return guarded([&] {
+    if (!info) throw std::invalid_argument("null info");
+    return dev(ctx).solve_pressure(rtol, maxit, info);
+  });
+}
+
+int s355_correct_fields(s355_ctx* ctx, double* res_p) {
+  return guarded([&] {
+    s355::Context& d = dev(ctx);
+    if (res_p) {
+      double before[10], after[10];
+      d.read_residuals(before);
+      d.correct_fields();
+      d.read_residuals(after);
+      *res_p = after[3] - before[3];
+    } else {
+      d.correct_fields();
+    }
+    return S355_OK;
+  });
+}
+
+int s355_correct_ru_face(s355_ctx* ctx) { return guarded([&] { dev(ctx).correct_ru_face(); return S355_OK; }); }
+int s355_save_velocity(s355_ctx* ctx) { return guarded([&] { dev(ctx).save_velocity(); return S355_OK; }); }
+int s355_velocity_residual(s355_ctx* ctx) {
+  return guarded([&] { dev(ctx).velocity_residual(); return S355_OK; });
+}
+
+int s355_residuals(s355_ctx* ctx, double res[10], int reset) {
+  return guarded([&] {
+    if (!res) throw std::invalid_argument("null res");
+    s355::Context& d = dev(ctx);
+    d.read_residuals(res);
+    d.check_flags();
+    if (reset) d.zero_residuals();
+    return S355_OK;
+  });
+}
+
+int s355_kepsilon_sources(s355_ctx* ctx) { return guarded([&] { dev(ctx).kepsilon_sources(); return S355_OK; }); }
+int s355_bc_kepsilon(s355_ctx* ctx) { return guarded([&] { dev(ctx).bc_kepsilon(); return S355_OK; }); }
+int s355_build_scalar_matrix(s355_ctx* ctx, int isca) {
+  return guarded([&] { dev(ctx).build_scalar_matrix(isca); return S355_OK; });
+}
+int s355_eps_wall_rows(s355_ctx* ctx) { return guarded([&] { dev(ctx).eps_wall_rows(); return S355_OK; }); }
+int s355_solve_scalar(s355_ctx* ctx, int isca, double rtol, int maxit, s355_solve_info* info) {
+  return guarded([&] {
+    if (!info) throw std::invalid_argument("null info");
+    return dev(ctx).solve_scalar(isca, rtol, maxit, info);
+  });
+}
+int s355_kepsilon_post_k(s355_ctx* ctx) { return guarded([&] { dev(ctx).kepsilon_post_k(); return S355_OK; }); }
+int s355_kepsilon_post_eps(s355_ctx* ctx) { return guarded([&] { dev(ctx).kepsilon_post_eps(); return S355_OK; }); }
+
+int s355_log_stage_push(s355_ctx* ctx, int stage) { return guarded([&] { dev(ctx).stage_push(stage); return S355_OK; }); }
+int s355_log_stage_pop(s355_ctx* ctx) { return guarded([&] { dev(ctx).stage_pop(); return S355_OK; }); }
+int s355_log_stage_times(s355_ctx* ctx, double ms[6], int reset) {
+  return guarded([&] {
+    if (!ms) throw std::invalid_argument("null ms");
+    dev(ctx).stage_times(ms, reset != 0);
+    return S355_OK;
+  });
+}
+
+int s355_calculate_velocity(s355_ctx* ctx, s355_solve_info info[3]) {
+  return guarded([&] {
+    dev(ctx);
+    ctx->ns->CalculateVelocity();
+    if (info)
+      for (int k = 0; k < 3; ++k) info[k] = ctx->ns->last[k];
+    return S355_OK;
+  });
+}
+
+int s355_calculate_pressure(s355_ctx* ctx, s355_solve_info* info) {
+  return guarded([&] {
+    dev(ctx);
+    ctx->ns->CalculatePressure();
+    if (info) *info = ctx->ns->last[3];
+    return S355_OK;
+  });
+}
+
+int s355_update_tur_kepsilon(s355_ctx* ctx, s355_solve_info info[2]) {
+  return guarded([&] {
+    dev(ctx);
+    ctx->ns->UpdateTurKEpsilon();
+    if (info) {
+      info[0] = ctx->ns->last[4];
+      info[1] = ctx->ns->last[5];
+    }
+    return S355_OK;
+  });
+}
+
+int s355_simple_iterate(s355_ctx* ctx, int n_steps, double* history, int* steps_done) {
+  int done = 0;
+  int rc = guarded([&] {
+    dev(ctx);
+    if (n_steps < 0) throw std::invalid_argument("n_steps < 0");
+    return ctx->ns->NSSolve(n_steps, history, &done);
+  });
+  if (steps_done) *steps_done = done;
+  return rc;
+}
+
+int s355_spmv_time(s355_ctx* ctx, int which, int precision, int repeats, double* ms_per_call) {
+  return guarded([&] {
+    if (!ms_per_call || which < 0 || which > 2 || precision < 0 || precision > 1)
+      throw std::invalid_argument("spmv_time arguments");
+    *ms_per_call = dev(ctx).time_spmv(which, precision, repeats);
+    return S355_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,540 @@
+// Device context: owns the HBM copies of one case and runs the device-level
+// operations of the SIMPLE hot path on one HIP stream.
+#include "context.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+#include "common.hpp"
+
+namespace s355 {
+
+namespace {
+constexpr double kSmallInit = 1.e-16;
+constexpr double kCmuInit = 0.09;
+}  // namespace
+
+template <class T>
+T* Context::alloc(size_t n) {
+  void* p = nullptr;
+  size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+  S355_HIP(hipMalloc(&p, bytes));
+  allocs_.push_back(p);
+  return static_cast<T*>(p);
+}
+
+namespace {
+template <class T>
+T* upload(std::vector<void*>& allocs, const std::vector<T>& v, hipStream_t s) {
+  void* p = nullptr;
+  S355_HIP(hipMalloc(&p, std::max<size_t>(v.size(), 1) * sizeof(T)));
+  allocs.push_back(p);
+  if (!v.empty()) S355_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  return static_cast<T*>(p);
+}
+}  // namespace
+
+Context::Context(const s355_mesh_desc& desc, const s355_params& params) : par(params) {
+  if (par.density_model != 0) throw std::invalid_argument("only DensityModel 0 (constant density) is supported");
+  if (!par.steady) throw std::invalid_argument("only steady runs are supported");
+  if (par.precision != S355_FP64 && par.precision != S355_MIXED) throw std::invalid_argument("bad precision mode");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw HipError("no HIP device available");
+  mesh = ingest_mesh(desc);
+  if (par.cell_pressure_ref < 0 || par.cell_pressure_ref >= mesh.n_cell)
+    throw std::invalid_argument("cell_pressure_ref out of range");
+  try {
+    S355_HIP(hipStreamCreate(&stream));
+    upload_mesh();
+    allocate_fields();
+    S355_HIP(hipStreamSynchronize(stream));
+  } catch (...) {
+    free_all();
+    throw;
+  }
+}
+
+Context::~Context() { free_all(); }
+
+void Context::free_all() {
+  if (stream) hipStreamSynchronize(stream);
+  for (hipEvent_t e : all_events_) hipEventDestroy(e);
+  all_events_.clear();
+  pool_.clear();
+  open_.clear();
+  done_.clear();
+  for (void* p : allocs_) hipFree(p);
+  allocs_.clear();
+  if (host_pinned_) hipHostFree(host_pinned_);
+  host_pinned_ = nullptr;
+  if (stream) hipStreamDestroy(stream);
+  stream = nullptr;
+}
+
+void Context::upload_mesh() {
+  const Mesh& m = mesh;
+  dm.Nc = m.n_cell;
+  dm.Nf = m.n_face;
+  dm.Nb = m.n_bnd;
+  dm.nnz = m.nnz();
+  dm.f_c1 = upload(allocs_, m.f_c1, stream);
+  dm.f_c2 = upload(allocs_, m.f_c2, stream);
+  dm.f_lambda = upload(allocs_, m.f_lambda, stream);
+  dm.f_rlencos = upload(allocs_, m.f_rlencos, stream);
+  dm.f_area = upload(allocs_, m.f_area, stream);
+  dm.f_n = upload(allocs_, m.f_n, stream);
+  dm.f_x = upload(allocs_, m.f_x, stream);
+  dm.f_xpac = upload(allocs_, m.f_xpac, stream);
+  dm.f_xnac = upload(allocs_, m.f_xnac, stream);
+  dm.c_vol = upload(allocs_, m.c_vol, stream);
+  dm.c_x = upload(allocs_, m.c_x, stream);
+  dm.cf_ptr = upload(allocs_, m.cf_ptr, stream);
+  dm.cf_pat = upload(allocs_, m.cf_pat, stream);
+  dm.cf_slot = upload(allocs_, m.cf_slot, stream);
+  dm.cf_sorted = upload(allocs_, m.cf_sorted, stream);
+  dm.rowptr = upload(allocs_, m.rowptr, stream);
+  dm.col = upload(allocs_, m.col, stream);
+  dm.diag = upload(allocs_, m.diag, stream);
+  std::vector<int> btype(m.n_bnd);
+  for (int b = 0; b < m.n_bnd; ++b) {
+    btype[b] = m.type1_of_rid(m.b_rid[b]);
+    if (btype[b] == kOutlet) has_outlet_ = true;
+  }
+  dm.b_type = upload(allocs_, btype, stream);
+  dm.b_dist = upload(allocs_, m.b_dist, stream);
+  // wall-adjacent cells with their wall faces in ascending order
+  std::vector<int> wcell, wptr(1, 0), wface;
+  {
+    std::vector<std::vector<int>> per(m.n_cell);
+    for (int b = 0; b < m.n_bnd; ++b)
+      if (btype[b] == kWall) per[m.f_c1[b]].push_back(b);
+    for (int c = 0; c < m.n_cell; ++c) {
+      if (per[c].empty()) continue;
+      wcell.push_back(c);
+      for (int f : per[c]) wface.push_back(f);
+      wptr.push_back(static_cast<int>(wface.size()));
+    }
+  }
+  dm.n_wcell = static_cast<int>(wcell.size());
+  dm.wc_cell = upload(allocs_, wcell, stream);
+  dm.wc_ptr = upload(allocs_, wptr, stream);
+  dm.wc_face = upload(allocs_, wface, stream);
+}
+
+void Context::allocate_fields() {
+  const size_t Nc = dm.Nc, Nb = dm.Nb, Nf = dm.Nf, nnz = dm.nnz;
+  double** cell1[] = {&df.R, &df.U, &df.V, &df.W, &df.P, &df.T, &df.TE, &df.ED, &df.dp, &df.mu, &df.mut, &df.apr,
+                      &df.bu, &df.bv, &df.bw, &df.bp, &df.bs};
+  for (double** p : cell1) *p = alloc<double>(Nc);
+  double** cell3[] = {&df.dUdX, &df.dVdX, &df.dWdX, &df.dPdX, &df.dPhidX};
+  for (double** p : cell3) *p = alloc<double>(3 * Nc);
+  double** bnd1[] = {&df.BRo, &df.BU, &df.BV, &df.BW, &df.BPre, &df.BTE, &df.BED};
+  for (double** p : bnd1) *p = alloc<double>(Nb);
+  df.shear = alloc<double>(3 * Nb);
+  df.RUFace = alloc<double>(Nf);
+  df.Au = alloc<double>(nnz);
+  df.Ap = alloc<double>(nnz);
+  df.As = alloc<double>(nnz);
+  u0_ = alloc<double>(Nc);
+  v0_ = alloc<double>(Nc);
+  w0_ = alloc<double>(Nc);
+  kw_.prod = alloc<double>(Nc);
+  kw_.tes = alloc<double>(Nc);
+  kw_.eds = alloc<double>(Nc);
+  kw_.viste = alloc<double>(Nc);
+  kw_.visted = alloc<double>(Nc);
+  kw_.apte = alloc<double>(Nc);
+  kw_.aped = alloc<double>(Nc);
+  // zero everything once so that get_field before any phase is defined
+  double* all_cell[] = {df.R, df.U, df.V, df.W, df.P, df.T, df.TE, df.ED, df.dp, df.mu, df.mut, df.apr,
+                        df.bu, df.bv, df.bw, df.bp, df.bs};
+  for (double* p : all_cell) S355_HIP(hipMemsetAsync(p, 0, Nc * sizeof(double), stream));
+  for (double* p : {df.dUdX, df.dVdX, df.dWdX, df.dPdX, df.dPhidX})
+    S355_HIP(hipMemsetAsync(p, 0, 3 * Nc * sizeof(double), stream));
+  for (double* p : {df.BRo, df.BU, df.BV, df.BW, df.BPre, df.BTE, df.BED})
+    S355_HIP(hipMemsetAsync(p, 0, std::max<size_t>(Nb, 1) * sizeof(double), stream));
+  S355_HIP(hipMemsetAsync(df.shear, 0, std::max<size_t>(3 * Nb, 1) * sizeof(double), stream));
+  S355_HIP(hipMemsetAsync(df.RUFace, 0, Nf * sizeof(double), stream));
+  for (double* p : {df.Au, df.Ap, df.As}) S355_HIP(hipMemsetAsync(p, 0, nnz * sizeof(double), stream));
+  // scratch for reductions
+  max_restart_ = 4;
+  for (int k = 0; k < 3; ++k) max_restart_ = std::max(max_restart_, std::min(par.ksp[k].restart, kMaxRestart));
+  const size_t g = static_cast<size_t>(grid_for(static_cast<long>(std::max<size_t>(Nc, Nf))));
+  partial_len_ = std::max<size_t>(g * (max_restart_ + 1), 3 * static_cast<size_t>(grid_for(std::max<size_t>(Nb, 1))));
+  partial_ = alloc<double>(partial_len_);
+  scalars_ = alloc<double>(32);
+  flags_ = alloc<int>(8);
+  S355_HIP(hipMemsetAsync(scalars_, 0, 32 * sizeof(double), stream));
+  S355_HIP(hipMemsetAsync(flags_, 0, 8 * sizeof(int), stream));
+  // Krylov workspace
+  kwork_.n = static_cast<int>(Nc);
+  kwork_.mmax = max_restart_;
+  kwork_.V = alloc<double>((static_cast<size_t>(max_restart_) + 1) * Nc);
+  kwork_.r = alloc<double>(Nc);
+  kwork_.pcinv = alloc<double>(Nc);
+  kwork_.partial = partial_;
+  kwork_.partial_len = partial_len_;
+  kwork_.st = alloc<KState>(1);
+  S355_HIP(hipHostMalloc(&host_pinned_, 4096, hipHostMallocDefault));
+  kwork_.host = static_cast<KHeader*>(host_pinned_);
+}
+
+void Context::set_params(const s355_params& p) {
+  if (p.density_model != 0 || !p.steady) throw std::invalid_argument("unsupported model switch");
+  for (int k = 0; k < 3; ++k)
+    if (std::min(p.ksp[k].restart, kMaxRestart) > max_restart_)
+      throw std::invalid_argument("restart larger than the workspace built at context creation");
+  if (p.cell_pressure_ref < 0 || p.cell_pressure_ref >= dm.Nc) throw std::invalid_argument("cell_pressure_ref");
+  par = p;
+}
+
+FieldRef Context::field(int id) {
+  const size_t Nc = dm.Nc, Nb = dm.Nb, Nf = dm.Nf, nnz = dm.nnz;
+  switch (id) {
+    case S355_F_R: return {df.R, Nc};
+    case S355_F_U: return {df.U, Nc};
+    case S355_F_V: return {df.V, Nc};
+    case S355_F_W: return {df.W, Nc};
+    case S355_F_P: return {df.P, Nc};
+    case S355_F_T: return {df.T, Nc};
+    case S355_F_TE: return {df.TE, Nc};
+    case S355_F_ED: return {df.ED, Nc};
+    case S355_F_DELTAP: return {df.dp, Nc};
+    case S355_F_VISLAM: return {df.mu, Nc};
+    case S355_F_VISTUR: return {df.mut, Nc};
+    case S355_F_APR: return {df.apr, Nc};
+    case S355_F_DUDX: return {df.dUdX, 3 * Nc};
+    case S355_F_DVDX: return {df.dVdX, 3 * Nc};
+    case S355_F_DWDX: return {df.dWdX, 3 * Nc};
+    case S355_F_DPDX: return {df.dPdX, 3 * Nc};
+    case S355_F_DPHIDX: return {df.dPhidX, 3 * Nc};
+    case S355_F_BRO: return {df.BRo, Nb};
+    case S355_F_BU: return {df.BU, Nb};
+    case S355_F_BV: return {df.BV, Nb};
+    case S355_F_BW: return {df.BW, Nb};
+    case S355_F_BPRE: return {df.BPre, Nb};
+    case S355_F_BTE: return {df.BTE, Nb};
+    case S355_F_BED: return {df.BED, Nb};
+    case S355_F_SHEAR: return {df.shear, 3 * Nb};
+    case S355_F_RUFACE: return {df.RUFace, Nf};
+    case S355_F_AU: return {df.Au, nnz};
+    case S355_F_AP: return {df.Ap, nnz};
+    case S355_F_AS: return {df.As, nnz};
+    case S355_F_BU_RHS: return {df.bu, Nc};
+    case S355_F_BV_RHS: return {df.bv, Nc};
+    case S355_F_BW_RHS: return {df.bw, Nc};
+    case S355_F_BP_RHS: return {df.bp, Nc};
+    case S355_F_BS_RHS: return {df.bs, Nc};
+    default: throw std::invalid_argument("unknown field id " + std::to_string(id));
+  }
+}
+
+void Context::sync() { S355_HIP(hipStreamSynchronize(stream)); }
+
+// InitFlowField (navier.cpp:588-697), evaluated on the host once and uploaded.
+void Context::init_flow_field() {
+  const int Nc = dm.Nc, Nb = dm.Nb;
+  std::vector<double> U(Nc), V(Nc), W(Nc), R(Nc), T(Nc), P(Nc, 0.0), mu(Nc), mut(Nc, 0.0), k(Nc, 0.0), e(Nc, 0.0);
+  auto find = [&](int rid) -> const s355_region* {
+    for (const auto& r : mesh.regions)
+      if (r.rid == rid) return &r;
+    return nullptr;
+  };
+  auto apply = [&](int i, const s355_region& r) {
+    U[i] = r.init[0];
+    V[i] = r.init[1];
+    W[i] = r.init[2];
+    R[i] = r.init[3];
+    T[i] = r.init[4];
+    P[i] = 0.;
+    mu[i] = r.init[5];
+    mut[i] = 0.;
+    if (par.turb_model == 1) {
+      k[i] = r.init[6];
+      e[i] = r.init[7];
+      mut[i] = R[i] * kCmuInit * k[i] * k[i] / (e[i] + kSmallInit);
+    }
+  };
+  if (const s355_region* r0 = find(0))
+    for (int i = 0; i < Nc; ++i) apply(i, *r0);
+  for (int i = 0; i < Nc; ++i) {
+    const s355_region* r = find(mesh.c_rid[i]);
+    if (!r) throw std::invalid_argument("cell region missing");
+    apply(i, *r);
+  }
+  std::vector<double> BU(Nb, 0.0), BV(Nb, 0.0), BW(Nb, 0.0), BRo(Nb, 0.0), BTE(Nb, 0.0), BED(Nb, 0.0);
+  for (int b = 0; b < Nb; ++b) {
+    const s355_region* r = find(mesh.b_rid[b]);
+    if (r->type1 == kInlet) {
+      BU[b] = r->init[0];
+      BV[b] = r->init[1];
+      BW[b] = r->init[2];
+      BRo[b] = r->init[4];
+      BTE[b] = r->init[6];
+      BED[b] = r->init[7];
+    }
+  }
+  auto put = [&](double* d, const std::vector<double>& h) {
+    if (!h.empty()) S355_HIP(hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+  };
+  put(df.U, U);
+  put(df.V, V);
+  put(df.W, W);
+  put(df.R, R);
+  put(df.T, T);
+  put(df.P, P);
+  put(df.mu, mu);
+  put(df.mut, mut);
+  put(df.TE, k);
+  put(df.ED, e);
+  S355_HIP(hipMemsetAsync(df.dp, 0, Nc * sizeof(double), stream));
+  put(df.BU, BU);
+  put(df.BV, BV);
+  put(df.BW, BW);
+  put(df.BRo, BRo);
+  put(df.BTE, BTE);
+  put(df.BED, BED);
+  S355_HIP(hipMemsetAsync(df.RUFace, 0, dm.Nf * sizeof(double), stream));
+  sync();  // host staging vectors go out of scope
+}
+
+void Context::bc_velocity() { launch_bc_velocity(dm, df, has_outlet_, partial_, scalars_ + 10, stream); }
+void Context::bc_pressure() { launch_bc_copy(dm, df.P, df.BPre, stream); }
+void Context::bc_deltap() { launch_bc_copy(dm, df.dp, df.BPre, stream); }
+
+void Context::gradient(int field_id, int bfield_id, int out_id) {
+  FieldRef f = field(field_id), b = field(bfield_id), g = field(out_id);
+  if (f.n != static_cast<size_t>(dm.Nc) || b.n != static_cast<size_t>(dm.Nb) || g.n != 3 * static_cast<size_t>(dm.Nc))
+    throw std::invalid_argument("gradient: expects a cell field, a boundary field and a cell 3-vector field");
+  const double* phi[1] = {f.p};
+  const double* bphi[1] = {b.p};
+  double* out[1] = {g.p};
+  launch_gradient(dm, 1, phi, bphi, out, stream);
+}
+
+void Context::build_velocity_matrix() {
+  bc_velocity();
+  bc_pressure();
+  const double* phi[4] = {df.U, df.V, df.W, df.P};
+  const double* bphi[4] = {df.BU, df.BV, df.BW, df.BPre};
+  double* out[4] = {df.dUdX, df.dVdX, df.dWdX, df.dPdX};
+  launch_gradient(dm, 4, phi, bphi, out, stream);
+  launch_build_momentum(dm, df, par.urf[0], par.gravity, flags_, stream);
+}
+
+namespace {
+void fill_info(const KResult& r, s355_solve_info* info) {
+  if (!info) return;
+  info->iters = r.iters;
+  info->reason = r.reason;
+  info->rnorm = r.rnorm;
+  info->rnorm0 = r.rnorm0;
+  info->refinements = 0;
+  info->pad = 0;
+}
+}  // namespace
+
+int Context::run_solve(int which, const double* b, double* x, const s355_ksp_opts& o, bool zero_guess,
+                       s355_solve_info* info) {
+  const double* val = which == 0 ? df.Au : (which == 1 ? df.Ap : df.As);
+  CsrView<double> A{dm.Nc, dm.rowptr, dm.col, dm.diag, val};
+  KResult r = krylov_solve<double>(A, b, x, o, zero_guess, kwork_, stream);
+  fill_info(r, info);
+  return r.reason < 0 ? S355_NOT_CONVERGED : S355_OK;
+}
+
+int Context::solve_velocity(double rtol, int maxit, s355_solve_info info[3]) {
+  s355_ksp_opts o = par.ksp[S355_KSP_VELOCITY];
+  o.rtol = rtol;
+  o.maxit = maxit;
+  int rc = S355_OK;
+  // The reference stops at the first failing component (the throw in
+  // MPIStructure.cpp:279-283 leaves V and W unsolved).
+  const double* b[3] = {df.bu, df.bv, df.bw};
+  double* x[3] = {df.U, df.V, df.W};
+  for (int k = 0; k < 3; ++k) {
+    rc = run_solve(0, b[k], x[k], o, false, &info[k]);
+    if (rc != S355_OK) {
+      for (int j = k + 1; j < 3; ++j) info[j] = s355_solve_info{0, 0, 0.0, 0.0, 0, 0};
+      return rc;
+    }
+  }
+  return rc;
+}
+
+void Context::cal_ru_face() { launch_ruface(dm, df, stream); }
+void Context::build_pressure_matrix() { launch_build_pressure(dm, df, stream); }
+
+int Context::solve_pressure(double rtol, int maxit, s355_solve_info* info) {
+  s355_ksp_opts o = par.ksp[S355_KSP_PRESSURE];
+  o.rtol = rtol;
+  o.maxit = maxit;
+  return run_solve(1, df.bp, df.dp, o, true, info);
+}
+
+void Context::correct_fields() {
+  bc_deltap();
+  const double* phi[1] = {df.dp};
+  const double* bphi[1] = {df.BPre};
+  double* out[1] = {df.dPdX};
+  launch_gradient(dm, 1, phi, bphi, out, stream);
+  launch_correct_fields(dm, df, par.cell_pressure_ref, par.urf[3], partial_, stream);
+  launch_reduce_partials(partial_, grid_for(dm.Nc), 1, scalars_ + 3, true, stream);
+}
+
+void Context::correct_ru_face() { launch_correct_ruface(dm, df, par.urf[3], stream); }
+
+void Context::save_velocity() {
+  const size_t bytes = static_cast<size_t>(dm.Nc) * sizeof(double);
+  S355_HIP(hipMemcpyAsync(u0_, df.U, bytes, hipMemcpyDeviceToDevice, stream));
+  S355_HIP(hipMemcpyAsync(v0_, df.V, bytes, hipMemcpyDeviceToDevice, stream));
+  S355_HIP(hipMemcpyAsync(w0_, df.W, bytes, hipMemcpyDeviceToDevice, stream));
+}
+
+void Context::velocity_residual() {
+  launch_residual3(dm, u0_, v0_, w0_, df.U, df.V, df.W, partial_, stream);
+  launch_reduce_partials(partial_, grid_for(dm.Nc), 3, scalars_, true, stream);
+}
+
+void Context::kepsilon_sources() { launch_keps_sources(dm, df, kw_, stream); }
+
+void Context::bc_kepsilon() {
+  launch_keps_wall(dm, df, kw_, std::pow(0.09, 0.25), std::pow(0.09, 0.75), stream);
+  launch_keps_copy_bc(dm, df, stream);
+}
+
+void Context::build_scalar_matrix(int isca) {
+  double *phi, *bphi;
+  const double *diff, *src, *app;
+  if (isca == 2) {
+    phi = df.TE;
+    bphi = df.BTE;
+    diff = kw_.viste;
+    src = kw_.tes;
+    app = kw_.apte;
+  } else if (isca == 3) {
+    phi = df.ED;
+    bphi = df.BED;
+    diff = kw_.visted;
+    src = kw_.eds;
+    app = kw_.aped;
+  } else {
+    throw std::invalid_argument("build_scalar_matrix: isca 2 (k) or 3 (epsilon)");
+  }
+  const double* p1[1] = {phi};
+  const double* b1[1] = {bphi};
+  double* g1[1] = {df.dPhidX};
+  launch_gradient(dm, 1, p1, b1, g1, stream);
+  launch_build_scalar(dm, df, phi, bphi, diff, src, app, par.urf[5], flags_, stream);
+}
+
+void Context::eps_wall_rows() { launch_dirichlet_rows(dm, df, stream); }
+
+int Context::solve_scalar(int isca, double rtol, int maxit, s355_solve_info* info) {
+  if (isca != 2 && isca != 3) throw std::invalid_argument("solve_scalar: isca 2 or 3");
+  s355_ksp_opts o = par.ksp[S355_KSP_SCALAR];
+  o.rtol = rtol;
+  o.maxit = maxit;
+  return run_solve(2, df.bs, isca == 2 ? df.TE : df.ED, o, false, info);
+}
+
+void Context::kepsilon_post_k() {
+  launch_clip(dm.Nc, df.TE, 1.e-9, stream);
+  launch_eps_sources(dm, df, kw_, stream);
+}
+
+void Context::kepsilon_post_eps() {
+  launch_clip(dm.Nc, df.ED, 1.e-12, stream);
+  launch_turb_viscosity(dm, df, stream);
+}
+
+void Context::zero_residuals() { fill_zero(scalars_, 10 * sizeof(double), stream); }
+
+void Context::read_residuals(double out[10]) {
+  double* h = static_cast<double*>(host_pinned_) + 256;
+  S355_HIP(hipMemcpyAsync(h, scalars_, 10 * sizeof(double), hipMemcpyDeviceToHost, stream));
+  sync();
+  std::memcpy(out, h, 10 * sizeof(double));
+}
+
+void Context::check_flags() {
+  int* h = reinterpret_cast<int*>(static_cast<char*>(host_pinned_) + 3072);
+  S355_HIP(hipMemcpyAsync(h, flags_, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
+  sync();
+  if (h[0]) throw std::runtime_error("unknown boundary type met in an assembly kernel");
+  if (h[1]) throw std::runtime_error("reverse flow get out of inlet (scalar transport)");
+}
+
+double Context::time_spmv(int which, int precision, int repeats) {
+  const double* val = which == 0 ? df.Au : (which == 1 ? df.Ap : df.As);
+  if (repeats < 1) repeats = 1;
+  double* y = static_cast<double*>(kwork_.V);
+  if (precision == 0) {
+    CsrView<double> A{dm.Nc, dm.rowptr, dm.col, dm.diag, val};
+    return bench_spmv<double>(A, df.U, y, kwork_, repeats, stream);
+  }
+  // FP32 copy of the matrix and of x in the Krylov workspace
+  float* vf = reinterpret_cast<float*>(y + dm.Nc);
+  float* xf = vf + dm.nnz;
+  float* yf = xf + dm.Nc;
+  if ((static_cast<size_t>(max_restart_) + 1) * dm.Nc * sizeof(double) <
+      (static_cast<size_t>(dm.Nc) * 2 + dm.nnz) * sizeof(float) + dm.Nc * sizeof(double))
+    throw std::runtime_error("time_spmv: workspace too small for the FP32 copy");
+  convert_to_float(val, vf, dm.nnz, stream);
+  convert_to_float(df.U, xf, dm.Nc, stream);
+  CsrView<float> A{dm.Nc, dm.rowptr, dm.col, dm.diag, vf};
+  return bench_spmv<float>(A, xf, yf, kwork_, repeats, stream);
+}
+
+hipEvent_t Context::take_event() {
+  if (!pool_.empty()) {
+    hipEvent_t e = pool_.back();
+    pool_.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  S355_HIP(hipEventCreate(&e));
+  all_events_.push_back(e);
+  return e;
+}
+
+void Context::stage_push(int stage) {
+  if (stage < 1 || stage > 5) throw std::invalid_argument("log stage must be 1..5");
+  hipEvent_t a = take_event();
+  S355_HIP(hipEventRecord(a, stream));
+  open_.push_back({stage, a, nullptr});
+}
+
+void Context::stage_pop() {
+  if (open_.empty()) throw std::logic_error("log stage pop without push");
+  StageEvent ev = open_.back();
+  open_.pop_back();
+  ev.b = take_event();
+  S355_HIP(hipEventRecord(ev.b, stream));
+  done_.push_back(ev);
+  if (done_.size() > 512) resolve_stages();
+}
+
+void Context::resolve_stages() {
+  for (auto& ev : done_) {
+    S355_HIP(hipEventSynchronize(ev.b));
+    float ms = 0.f;
+    S355_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
+    stage_ms_[ev.stage] += ms;
+    stage_ms_[0] += ms;
+    pool_.push_back(ev.a);
+    pool_.push_back(ev.b);
+  }
+  done_.clear();
+}
+
+void Context::stage_times(double ms[6], bool reset) {
+  resolve_stages();
+  for (int k = 0; k < 6; ++k) ms[k] = stage_ms_[k];
+  if (reset)
+    for (double& v : stage_ms_) v = 0.0;
+}
+
+}  // namespace s355

@@ -1,0 +1,105 @@
+// Device context: HBM-resident mesh, fields and matrices of one case, and
+// the device-level operations behind the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <string>
+#include <vector>
+
+#include "../mesh/ingest.hpp"
+#include "fv.hpp"
+#include "krylov.hpp"
+
+namespace s355 {
+
+struct FieldRef {
+  double* p = nullptr;
+  size_t n = 0;
+};
+
+class Context {
+ public:
+  Context(const s355_mesh_desc& desc, const s355_params& params);
+  ~Context();
+  Context(const Context&) = delete;
+  Context& operator=(const Context&) = delete;
+
+  Mesh mesh;  // host copy (reference numbering)
+  s355_params par;
+  hipStream_t stream = nullptr;
+  DMesh dm{};
+  DFields df{};
+
+  FieldRef field(int id);
+  void set_params(const s355_params& p);
+
+  // --- device-level operations (reference members)
+  void init_flow_field();
+  void bc_velocity();
+  void bc_pressure();
+  void bc_deltap();
+  void gradient(int field, int bfield, int out);
+  void build_velocity_matrix();
+  int solve_velocity(double rtol, int maxit, s355_solve_info info[3]);
+  void cal_ru_face();
+  void build_pressure_matrix();
+  int solve_pressure(double rtol, int maxit, s355_solve_info* info);
+  void correct_fields();  // adds into residual slot 3
+  void correct_ru_face();
+  void save_velocity();
+  void velocity_residual();  // adds into residual slots 0..2
+  void kepsilon_sources();
+  void bc_kepsilon();
+  void build_scalar_matrix(int isca);
+  void eps_wall_rows();
+  int solve_scalar(int isca, double rtol, int maxit, s355_solve_info* info);
+  void kepsilon_post_k();
+  void kepsilon_post_eps();
+
+  // residual slots and error flags on the device
+  void zero_residuals();
+  void read_residuals(double out[10]);
+  void check_flags();  // throws on a flag raised by a kernel
+  void sync();
+
+  // device-timed log stages (PetscLogStagePush/Pop counterpart)
+  void stage_push(int stage);
+  void stage_pop();
+  void stage_times(double ms[6], bool reset);
+
+  // benchmark helper
+  double time_spmv(int which, int precision, int repeats);
+
+ private:
+  template <class T>
+  T* alloc(size_t n);
+  void upload_mesh();
+  void allocate_fields();
+  void free_all();
+  int run_solve(int which_matrix, const double* b, double* x, const s355_ksp_opts& o, bool zero_guess,
+                s355_solve_info* info);
+
+  std::vector<void*> allocs_;
+  double* partial_ = nullptr;
+  size_t partial_len_ = 0;
+  double* scalars_ = nullptr;  // [0..9] residuals, [10..14] outlet correction
+  int* flags_ = nullptr;       // [0] bad boundary type, [1] inlet outflow in a scalar
+  double *u0_ = nullptr, *v0_ = nullptr, *w0_ = nullptr;
+  KepsWork kw_{};
+  KWork kwork_{};
+  void* host_pinned_ = nullptr;
+  bool has_outlet_ = false;
+  int max_restart_ = 0;
+  struct StageEvent {
+    int stage;
+    hipEvent_t a, b;
+  };
+  hipEvent_t take_event();
+  void resolve_stages();
+  std::vector<StageEvent> open_, done_;
+  std::vector<hipEvent_t> pool_, all_events_;
+  double stage_ms_[6] = {0, 0, 0, 0, 0, 0};
+};
+
+}  // namespace s355

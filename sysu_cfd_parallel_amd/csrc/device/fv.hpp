@@ -1,0 +1,40 @@
+// Launchers of the finite-volume kernels (fv_kernels.hip, keps_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fv_kernels.hpp"
+
+namespace s355 {
+
+void launch_gradient(const DMesh& m, int nf, const double* const* phi, const double* const* bphi, double* const* g,
+                     hipStream_t s);
+void launch_bc_velocity(const DMesh& m, const DFields& F, bool has_outlet, double* partial, double* rate,
+                        hipStream_t s);
+void launch_bc_copy(const DMesh& m, const double* cellv, double* bval, hipStream_t s);
+void launch_build_momentum(const DMesh& m, const DFields& F, double urf, const double* grav, int* bad,
+                           hipStream_t s);
+void launch_ruface(const DMesh& m, const DFields& F, hipStream_t s);
+void launch_build_pressure(const DMesh& m, const DFields& F, hipStream_t s);
+void launch_correct_fields(const DMesh& m, const DFields& F, int ref, double urf, double* partial, hipStream_t s);
+void launch_correct_ruface(const DMesh& m, const DFields& F, double urf, hipStream_t s);
+void launch_residual3(const DMesh& m, const double* u0, const double* v0, const double* w0, const double* u,
+                      const double* v, const double* w, double* partial, hipStream_t s);
+void launch_reduce_partials(const double* partial, int nblk, int ncomp, double* out, bool accumulate,
+                            hipStream_t s);
+
+// k-epsilon (keps_kernels.hip)
+struct KepsWork {
+  double *prod, *tes, *eds, *viste, *visted, *apte, *aped;
+};
+void launch_keps_sources(const DMesh& m, const DFields& F, const KepsWork& w, hipStream_t s);
+void launch_keps_wall(const DMesh& m, const DFields& F, const KepsWork& w, double cmu25, double cmu75,
+                      hipStream_t s);
+void launch_keps_copy_bc(const DMesh& m, const DFields& F, hipStream_t s);
+void launch_build_scalar(const DMesh& m, const DFields& F, double* phi, const double* bphi, const double* diff,
+                         const double* source, const double* app_in, double urf, int* bad, hipStream_t s);
+void launch_clip(int n, double* v, double floor, hipStream_t s);
+void launch_eps_sources(const DMesh& m, const DFields& F, const KepsWork& w, hipStream_t s);
+void launch_dirichlet_rows(const DMesh& m, const DFields& F, hipStream_t s);
+void launch_turb_viscosity(const DMesh& m, const DFields& F, hipStream_t s);
+
+}  // namespace s355

@@ -1,0 +1,585 @@
+// Finite-volume kernels of the SIMPLE hot path on gfx950 (FP64).
+//
+// Data layout in HBM: structure-of-arrays streams in the reference's face and
+// cell order; 3-vectors interleaved per element ([N][3], as the reference's
+// new_Array2D blocks).  Cell kernels gather over a cell's faces through the
+// compact cell->face lists: ascending face order for the Green-Gauss sums
+// (the reference's face-ordered scatter, navier_gradient.cpp:22-39, seen from
+// each cell) and pattern order for the assembly loops (CellData::face order,
+// navier_velocity.cpp:125).  No atomics: every output element has exactly one
+// writer, so results are bitwise reproducible.  Built with
+// -ffp-contract=off; expressions keep the reference's evaluation order.
+#include "common.hpp"
+#include "fv.hpp"
+
+namespace s355 {
+
+// ---------------------------------------------------------------- gradient
+template <int NF>
+struct GradArgs {
+  const double* phi[NF];
+  const double* bphi[NF];
+  double* g[NF];
+};
+
+// Green-Gauss gradient (navier_gradient.cpp:12-48), NF fields fused so the
+// face geometry is read once for all of them.
+template <int NF>
+__global__ void __launch_bounds__(kBlock) k_gradient(DMesh m, GradArgs<NF> a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.Nc) return;
+  double acc[NF][3];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.;
+  const int e0 = m.cf_ptr[i], e1 = m.cf_ptr[i + 1];
+  for (int e = e0; e < e1; ++e) {
+    const int f = m.cf_sorted[e];
+    const double n0 = m.f_n[3 * f], n1 = m.f_n[3 * f + 1], n2 = m.f_n[3 * f + 2];
+    if (f < m.Nb) {
+#pragma unroll
+      for (int k = 0; k < NF; ++k) {
+        const double pf = a.bphi[k][f];
+        acc[k][0] += pf * n0;
+        acc[k][1] += pf * n1;
+        acc[k][2] += pf * n2;
+      }
+    } else {
+      const int c1 = m.f_c1[f], c2 = m.f_c2[f];
+      const double lam = m.f_lambda[f];
+      const double lam2 = 1. - lam;
+      if (c1 == i) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+          const double pf = lam * a.phi[k][c1] + lam2 * a.phi[k][c2];
+          acc[k][0] += pf * n0;
+          acc[k][1] += pf * n1;
+          acc[k][2] += pf * n2;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+          const double pf = lam * a.phi[k][c1] + lam2 * a.phi[k][c2];
+          acc[k][0] -= pf * n0;
+          acc[k][1] -= pf * n1;
+          acc[k][2] -= pf * n2;
+        }
+      }
+    }
+  }
+  const double vol = m.c_vol[i];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    a.g[k][3 * i] = acc[k][0] / vol;
+    a.g[k][3 * i + 1] = acc[k][1] / vol;
+    a.g[k][3 * i + 2] = acc[k][2] / vol;
+  }
+}
+
+void launch_gradient(const DMesh& m, int nf, const double* const* phi, const double* const* bphi, double* const* g,
+                     hipStream_t s) {
+  if (nf == 1) {
+    GradArgs<1> a{{phi[0]}, {bphi[0]}, {g[0]}};
+    hipLaunchKernelGGL(k_gradient<1>, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, a);
+  } else if (nf == 4) {
+    GradArgs<4> a{{phi[0], phi[1], phi[2], phi[3]}, {bphi[0], bphi[1], bphi[2], bphi[3]}, {g[0], g[1], g[2], g[3]}};
+    hipLaunchKernelGGL(k_gradient<4>, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, a);
+  } else {
+    throw std::invalid_argument("gradient: 1 or 4 fields");
+  }
+  S355_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- boundary conditions
+// SetBCVelocity first pass (navier_bc.cpp:13-83): boundary values by type and
+// per-block partial sums of inlet mass flow, outlet mass flow, outlet area.
+__global__ void __launch_bounds__(kBlock) k_bc_velocity(DMesh m, DFields F, double* partial) {
+  __shared__ double lds[3][kBlock / kWave];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double min = 0., mout = 0., aout = 0.;
+  if (i < m.Nb) {
+    const int t = m.b_type[i];
+    const int ic = m.f_c1[i];
+    if (t == kWall) {
+      F.BRo[i] = F.R[ic];
+      F.BU[i] = 0.;
+      F.BV[i] = 0.;
+      F.BW[i] = 0.;
+    } else if (t == kOutlet) {
+      F.BRo[i] = F.R[ic];
+      F.BU[i] = F.U[ic];
+      F.BV[i] = F.V[ic];
+      F.BW[i] = F.W[ic];
+    } else if (t == kSym) {
+      const double area = m.f_area[i];
+      const double s1 = m.f_n[3 * i] / area, s2 = m.f_n[3 * i + 1] / area, s3 = m.f_n[3 * i + 2] / area;
+      const double un = F.U[ic] * s1 + F.V[ic] * s2 + F.W[ic] * s3;
+      F.BU[i] = F.U[ic] - un * s1;
+      F.BV[i] = F.V[ic] - un * s2;
+      F.BW[i] = F.W[ic] - un * s3;
+      F.BRo[i] = F.R[ic];
+    }
+    const double* n = &m.f_n[3 * i];
+    if (t == kInlet)
+      min = F.BRo[i] * (F.BU[i] * n[0] + F.BV[i] * n[1] + F.BW[i] * n[2]);
+    else if (t == kOutlet) {
+      mout = F.BRo[i] * (F.BU[i] * n[0] + F.BV[i] * n[1] + F.BW[i] * n[2]);
+      aout = F.BRo[i] * m.f_area[i];
+    }
+  }
+  min = wave_sum(min);
+  mout = wave_sum(mout);
+  aout = wave_sum(aout);
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  if (lane == 0) {
+    lds[0][wid] = min;
+    lds[1][wid] = mout;
+    lds[2][wid] = aout;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double s = 0.;
+    for (int w = 0; w < kBlock / kWave; ++w) s += lds[threadIdx.x][w];
+    partial[3 * blockIdx.x + threadIdx.x] = s;
+  }
+}
+
+// Reduce the partial sums in a fixed order and decide the outlet correction
+// (navier_bc.cpp:85-117).  out = {mode, rate, m_in, m_out, a_out}.
+__global__ void __launch_bounds__(kBlock) k_bc_outlet_rate(const double* partial, int nblk, double* out) {
+  __shared__ double lds[kBlock / kWave];
+  double sum[3];
+  for (int c = 0; c < 3; ++c) {
+    double s = 0.;
+    for (int b = threadIdx.x; b < nblk; b += kBlock) s += partial[3 * b + c];
+    sum[c] = block_sum(s, lds);
+  }
+  if (threadIdx.x == 0) {
+    const double min = sum[0], mout = sum[1], aout = sum[2];
+    if (fabs(mout) > 1.e-16) {
+      out[0] = 1.;
+      out[1] = -min / mout;
+    } else {
+      out[0] = 2.;
+      out[1] = (-min - mout) / aout;
+    }
+    out[2] = min;
+    out[3] = mout;
+    out[4] = aout;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_bc_outlet_apply(DMesh m, DFields F, const double* rate) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.Nb || m.b_type[i] != kOutlet) return;
+  const double mode = rate[0], r = rate[1];
+  if (mode == 1.) {
+    F.BU[i] *= r;
+    F.BV[i] *= r;
+    F.BW[i] *= r;
+  } else {
+    const double area = m.f_area[i];
+    F.BU[i] += r * m.f_n[3 * i] / area;
+    F.BV[i] += r * m.f_n[3 * i + 1] / area;
+    F.BW[i] += r * m.f_n[3 * i + 2] / area;
+  }
+}
+
+void launch_bc_velocity(const DMesh& m, const DFields& F, bool has_outlet, double* partial, double* rate,
+                        hipStream_t s) {
+  if (m.Nb == 0) return;
+  const int g = grid_for(m.Nb);
+  hipLaunchKernelGGL(k_bc_velocity, dim3(g), dim3(kBlock), 0, s, m, F, partial);
+  S355_LAUNCH_CHECK();
+  if (!has_outlet) return;  // the correction touches outlet faces only
+  hipLaunchKernelGGL(k_bc_outlet_rate, dim3(1), dim3(kBlock), 0, s, partial, g, rate);
+  hipLaunchKernelGGL(k_bc_outlet_apply, dim3(g), dim3(kBlock), 0, s, m, F, rate);
+  S355_LAUNCH_CHECK();
+}
+
+// SetBCPressure / SetBCDeltaP (navier_bc.cpp:121-181): zero-gradient copy.
+__global__ void __launch_bounds__(kBlock) k_bc_copy(int Nb, const int* f_c1, const double* cellv, double* bval) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < Nb) bval[i] = cellv[f_c1[i]];
+}
+
+void launch_bc_copy(const DMesh& m, const double* cellv, double* bval, hipStream_t s) {
+  if (m.Nb == 0) return;
+  hipLaunchKernelGGL(k_bc_copy, dim3(grid_for(m.Nb)), dim3(kBlock), 0, s, m.Nb, m.f_c1, cellv, bval);
+  S355_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- momentum assembly
+struct MomArgs {
+  double urf;
+  double g0, g1, g2;
+  int* bad;  // set when a boundary type is unknown
+};
+
+// BuildVelocityMatrix cell loop (navier_velocity.cpp:100-383), steady form.
+__global__ void __launch_bounds__(kBlock) k_build_momentum(DMesh m, DFields F, MomArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.Nc) return;
+  double app = 0., su = 0., sv = 0., sw = 0.;
+  const double xi0 = m.c_x[3 * i], xi1 = m.c_x[3 * i + 1], xi2 = m.c_x[3 * i + 2];
+  const double Ui = F.U[i], Vi = F.V[i], Wi = F.W[i];
+  const double mui = F.mu[i] + F.mut[i];
+  const double* gu = &F.dUdX[3 * i];
+  const double* gv = &F.dVdX[3 * i];
+  const double* gw = &F.dWdX[3 * i];
+  const double gu0 = gu[0], gu1 = gu[1], gu2 = gu[2];
+  const double gv0 = gv[0], gv1 = gv[1], gv2 = gv[2];
+  const double gw0 = gw[0], gw1 = gw[1], gw2 = gw[2];
+  const int e0 = m.cf_ptr[i], e1 = m.cf_ptr[i + 1];
+  for (int e = e0; e < e1; ++e) {
+    const int f = m.cf_pat[e];
+    double fde0, fde1, fde2, fdi0, fdi1, fdi2, fcs0, fcs1, fcs2;
+    const double fx0 = m.f_x[3 * f], fx1 = m.f_x[3 * f + 1], fx2 = m.f_x[3 * f + 2];
+    if (f < m.Nb) {
+      const int t = m.b_type[f];
+      const double s1 = m.f_n[3 * f], s2 = m.f_n[3 * f + 1], s3 = m.f_n[3 * f + 2];
+      const double RUn = F.RUFace[f];
+      const double Visc = mui;
+      const double dxc0 = fx0 - xi0, dxc1 = fx1 - xi1, dxc2 = fx2 - xi2;
+      const double val = Visc * m.f_rlencos[f];
+      if (t == kWall) {
+        app += val;
+        double sh0 = (gu0 + gu0) * s1 + (gu1 + gv0) * s2 + (gu2 + gw0) * s3;
+        double sh1 = (gv0 + gu1) * s1 + (gv1 + gv1) * s2 + (gv2 + gw1) * s3;
+        double sh2 = (gw0 + gu2) * s1 + (gw1 + gv2) * s2 + (gw2 + gw2) * s3;
+        sh0 *= Visc;
+        sh1 *= Visc;
+        sh2 *= Visc;
+        const double Pi = F.P[i];
+        sh0 += Pi * s1;
+        sh1 += Pi * s2;
+        sh2 += Pi * s3;
+        F.shear[3 * f] = sh0;
+        F.shear[3 * f + 1] = sh1;
+        F.shear[3 * f + 2] = sh2;
+        fde0 = val * Ui;
+        fde1 = val * Vi;
+        fde2 = val * Wi;
+        fdi0 = val * Ui;
+        fdi1 = val * Vi;
+        fdi2 = val * Wi;
+      } else if (t >= kInlet && t <= kSym) {
+        app += val;
+        fde0 = (gu0 + gu0) * s1 + (gu1 + gv0) * s2 + (gu2 + gw0) * s3;
+        fde1 = (gv0 + gu1) * s1 + (gv1 + gv1) * s2 + (gv2 + gw1) * s3;
+        fde2 = (gw0 + gu2) * s1 + (gw1 + gv2) * s2 + (gw2 + gw2) * s3;
+        fde0 *= Visc;
+        fde1 *= Visc;
+        fde2 *= Visc;
+        fdi0 = val * (gu0 * dxc0 + gu1 * dxc1 + gu2 * dxc2 - F.BU[f]);
+        fdi1 = val * (gv0 * dxc0 + gv1 * dxc1 + gv2 * dxc2 - F.BV[f]);
+        fdi2 = val * (gw0 * dxc0 + gw1 * dxc1 + gw2 * dxc2 - F.BW[f]);
+      } else {
+        *a.bad = 1;
+        fde0 = fde1 = fde2 = fdi0 = fdi1 = fdi2 = 0.;
+      }
+      if (t == kInlet || t == kOutlet) {
+        const double fl = fmin(RUn, 0.0);
+        app -= fl;
+        fcs0 = fl * F.BU[f];
+        fcs1 = fl * F.BV[f];
+        fcs2 = fl * F.BW[f];
+      } else {
+        fcs0 = fcs1 = fcs2 = 0.;
+      }
+    } else {
+      const int c1 = m.f_c1[f];
+      const bool own = (c1 == i);
+      const int in = own ? m.f_c2[f] : c1;
+      double s1 = m.f_n[3 * f], s2 = m.f_n[3 * f + 1], s3 = m.f_n[3 * f + 2];
+      double RUn = F.RUFace[f];
+      double lambda = m.f_lambda[f];
+      if (!own) {
+        s1 = -s1;
+        s2 = -s2;
+        s3 = -s3;
+        RUn = -RUn;
+        lambda = 1. - lambda;
+      }
+      const double lambda2 = 1. - lambda;
+      const double* hu = &F.dUdX[3 * in];
+      const double* hv = &F.dVdX[3 * in];
+      const double* hw = &F.dWdX[3 * in];
+      const double hu0 = hu[0], hu1 = hu[1], hu2 = hu[2];
+      const double hv0 = hv[0], hv1 = hv[1], hv2 = hv[2];
+      const double hw0 = hw[0], hw1 = hw[1], hw2 = hw[2];
+      const double Visc = lambda * mui + lambda2 * (F.mu[in] + F.mut[in]);
+      const double dudx = lambda * gu0 + lambda2 * hu0;
+      const double dudy = lambda * gu1 + lambda2 * hu1;
+      const double dudz = lambda * gu2 + lambda2 * hu2;
+      const double dvdx = lambda * gv0 + lambda2 * hv0;
+      const double dvdy = lambda * gv1 + lambda2 * hv1;
+      const double dvdz = lambda * gv2 + lambda2 * hv2;
+      const double dwdx = lambda * gw0 + lambda2 * hw0;
+      const double dwdy = lambda * gw1 + lambda2 * hw1;
+      const double dwdz = lambda * gw2 + lambda2 * hw2;
+      double apn = 0.;
+      if (RUn < 0.) {
+        apn += RUn;
+        app -= RUn;
+      }
+      const double val = Visc * m.f_rlencos[f];
+      app += val;
+      apn += -val;
+      F.Au[m.cf_slot[e]] = apn;
+      const double xn0 = m.c_x[3 * in], xn1 = m.c_x[3 * in + 1], xn2 = m.c_x[3 * in + 2];
+      const double Un = F.U[in], Vn = F.V[in], Wn = F.W[in];
+      double ufi, vfi, wfi, ufh, vfh, wfh;
+      if (RUn > 0.) {
+        ufi = Ui;
+        vfi = Vi;
+        wfi = Wi;
+        const double dx0 = fx0 - xi0, dx1 = fx1 - xi1, dx2 = fx2 - xi2;
+        ufh = Ui + (gu0 * dx0 + gu1 * dx1 + gu2 * dx2);
+        vfh = Vi + (gv0 * dx0 + gv1 * dx1 + gv2 * dx2);
+        wfh = Wi + (gw0 * dx0 + gw1 * dx1 + gw2 * dx2);
+      } else {
+        ufi = Un;
+        vfi = Vn;
+        wfi = Wn;
+        const double dx0 = fx0 - xn0, dx1 = fx1 - xn1, dx2 = fx2 - xn2;
+        ufh = Un + (hu0 * dx0 + hu1 * dx1 + hu2 * dx2);
+        vfh = Vn + (hv0 * dx0 + hv1 * dx1 + hv2 * dx2);
+        wfh = Wn + (hw0 * dx0 + hw1 * dx1 + hw2 * dx2);
+      }
+      fcs0 = RUn * (ufh - ufi);
+      fcs1 = RUn * (vfh - vfi);
+      fcs2 = RUn * (wfh - wfi);
+      fde0 = (dudx + dudx) * s1 + (dudy + dvdx) * s2 + (dudz + dwdx) * s3;
+      fde1 = (dvdx + dudy) * s1 + (dvdy + dvdy) * s2 + (dvdz + dwdy) * s3;
+      fde2 = (dwdx + dudz) * s1 + (dwdy + dvdz) * s2 + (dwdz + dwdz) * s3;
+      fde0 *= Visc;
+      fde1 *= Visc;
+      fde2 *= Visc;
+      const double dxc0 = xn0 - xi0, dxc1 = xn1 - xi1, dxc2 = xn2 - xi2;
+      fdi0 = val * (dudx * dxc0 + dudy * dxc1 + dudz * dxc2);
+      fdi1 = val * (dvdx * dxc0 + dvdy * dxc1 + dvdz * dxc2);
+      fdi2 = val * (dwdx * dxc0 + dwdy * dxc1 + dwdz * dxc2);
+    }
+    su += fde0 - fdi0 - fcs0;
+    sv += fde1 - fdi1 - fcs1;
+    sw += fde2 - fdi2 - fcs2;
+  }
+  app /= a.urf;
+  F.apr[i] = 1. / (app);
+  F.Au[m.diag[i]] = app;
+  const double vol = m.c_vol[i];
+  const double Ri = F.R[i];
+  const double* gp = &F.dPdX[3 * i];
+  F.bu[i] = su + (1. - a.urf) * app * Ui + (-gp[0] + Ri * a.g0) * vol;
+  F.bv[i] = sv + (1. - a.urf) * app * Vi + (-gp[1] + Ri * a.g1) * vol;
+  F.bw[i] = sw + (1. - a.urf) * app * Wi + (-gp[2] + Ri * a.g2) * vol;
+}
+
+void launch_build_momentum(const DMesh& m, const DFields& F, double urf, const double* grav, int* bad,
+                           hipStream_t s) {
+  MomArgs a{urf, grav[0], grav[1], grav[2], bad};
+  hipLaunchKernelGGL(k_build_momentum, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, F, a);
+  S355_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- Rhie-Chow face flux
+// CalRUFace2 (navier_velocity.cpp:425-476).
+__global__ void __launch_bounds__(kBlock) k_ruface(DMesh m, DFields F) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= m.Nf) return;
+  const double n0 = m.f_n[3 * f], n1 = m.f_n[3 * f + 1], n2 = m.f_n[3 * f + 2];
+  if (f < m.Nb) {
+    const double ro = F.BRo[f];
+    const double ruf = F.BU[f] * ro, rvf = F.BV[f] * ro, rwf = F.BW[f] * ro;
+    F.RUFace[f] = ruf * n0 + rvf * n1 + rwf * n2;
+    return;
+  }
+  const int c1 = m.f_c1[f], c2 = m.f_c2[f];
+  const double lambda = m.f_lambda[f], lambda2 = 1. - lambda;
+  const double R1 = F.R[c1], R2 = F.R[c2];
+  const double rf = lambda * R1 + lambda2 * R2;
+  const double ruf = lambda * R1 * F.U[c1] + lambda2 * R2 * F.U[c2];
+  const double rvf = lambda * R1 * F.V[c1] + lambda2 * R2 * F.V[c2];
+  const double rwf = lambda * R1 * F.W[c1] + lambda2 * R2 * F.W[c2];
+  double ru = ruf * n0 + rvf * n1 + rwf * n2;
+  const double* g1 = &F.dPdX[3 * c1];
+  const double* g2 = &F.dPdX[3 * c2];
+  const double g10 = g1[0], g11 = g1[1], g12 = g1[2];
+  const double g20 = g2[0], g21 = g2[1], g22 = g2[2];
+  const double pa0 = m.f_xpac[3 * f], pa1 = m.f_xpac[3 * f + 1], pa2 = m.f_xpac[3 * f + 2];
+  const double na0 = m.f_xnac[3 * f], na1 = m.f_xnac[3 * f + 1], na2 = m.f_xnac[3 * f + 2];
+  double dx0 = pa0 - m.c_x[3 * c1], dx1 = pa1 - m.c_x[3 * c1 + 1], dx2 = pa2 - m.c_x[3 * c1 + 2];
+  const double P1 = F.P[c1] + (g10 * dx0 + g11 * dx1 + g12 * dx2);
+  dx0 = na0 - m.c_x[3 * c2];
+  dx1 = na1 - m.c_x[3 * c2 + 1];
+  dx2 = na2 - m.c_x[3 * c2 + 2];
+  const double P2 = F.P[c2] + (g20 * dx0 + g21 * dx1 + g22 * dx2);
+  dx0 = na0 - pa0;
+  dx1 = na1 - pa1;
+  dx2 = na2 - pa2;
+  const double d12 = sqrt(dx0 * dx0 + dx1 * dx1 + dx2 * dx2);
+  const double aprf = lambda * F.apr[c1] + lambda2 * F.apr[c2];
+  const double vol = lambda * m.c_vol[c1] + lambda2 * m.c_vol[c2];
+  const double dpx = lambda * g10 + lambda2 * g20;
+  const double dpy = lambda * g11 + lambda2 * g21;
+  const double dpz = lambda * g12 + lambda2 * g22;
+  const double dpn = dpx * dx0 + dpy * dx1 + dpz * dx2;
+  ru -= rf * m.f_area[f] * aprf / d12 * vol * ((P2 - P1) - dpn);
+  F.RUFace[f] = ru;
+}
+
+void launch_ruface(const DMesh& m, const DFields& F, hipStream_t s) {
+  hipLaunchKernelGGL(k_ruface, dim3(grid_for(m.Nf)), dim3(kBlock), 0, s, m, F);
+  S355_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- pressure-correction assembly
+// BuildPressureMatrix, DensityModel 0 (navier_pressure.cpp:106-213).
+__global__ void __launch_bounds__(kBlock) k_build_pressure(DMesh m, DFields F) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.Nc) return;
+  double valcen = 0., bpv = 0.;
+  const double rai = F.R[i] * F.apr[i];
+  const double voli = m.c_vol[i];
+  const int e0 = m.cf_ptr[i], e1 = m.cf_ptr[i + 1];
+  for (int e = e0; e < e1; ++e) {
+    const int f = m.cf_pat[e];
+    const int c1 = m.f_c1[f];
+    const double ru = F.RUFace[f];
+    if (i == c1)
+      bpv += ru;
+    else
+      bpv -= ru;
+    if (f < m.Nb) continue;
+    const int in = (i == c1) ? m.f_c2[f] : c1;
+    const double lambda = (i == c1) ? m.f_lambda[f] : 1. - m.f_lambda[f];
+    const double lambda2 = 1. - lambda;
+    const double roapf = rai * lambda + F.R[in] * F.apr[in] * lambda2;
+    const double vol = voli * lambda + m.c_vol[in] * lambda2;
+    const double tmp = roapf * vol * m.f_rlencos[f];
+    valcen += -tmp;
+    F.Ap[m.cf_slot[e]] = tmp;
+  }
+  F.Ap[m.diag[i]] = valcen;
+  F.bp[i] = bpv;
+}
+
+void launch_build_pressure(const DMesh& m, const DFields& F, hipStream_t s) {
+  hipLaunchKernelGGL(k_build_pressure, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, F);
+  S355_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- corrections
+// Field correction of CalculatePressure (navier_pressure.cpp:52-70) and the
+// per-block partial of sum |dp - dp_ref| vol.
+__global__ void __launch_bounds__(kBlock) k_correct_fields(DMesh m, DFields F, int ref, double urf, double* partial) {
+  __shared__ double lds[kBlock / kWave];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double r = 0.;
+  if (i < m.Nc) {
+    const double dref = F.dp[ref];
+    const double vol = m.c_vol[i];
+    const double d = F.dp[i] - dref;
+    r = fabs(d) * vol;
+    F.P[i] += urf * d;
+    const double coef = vol * F.apr[i];
+    F.U[i] -= coef * F.dPdX[3 * i];
+    F.V[i] -= coef * F.dPdX[3 * i + 1];
+    F.W[i] -= coef * F.dPdX[3 * i + 2];
+  }
+  const double s = block_sum(r, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+void launch_correct_fields(const DMesh& m, const DFields& F, int ref, double urf, double* partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_correct_fields, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, F, ref, urf, partial);
+  S355_LAUNCH_CHECK();
+}
+
+// CorrectRUFace2 (navier_pressure.cpp:243-278).
+__global__ void __launch_bounds__(kBlock) k_correct_ruface(DMesh m, DFields F, double urf) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= m.Nf) return;
+  const double n0 = m.f_n[3 * f], n1 = m.f_n[3 * f + 1], n2 = m.f_n[3 * f + 2];
+  if (f < m.Nb) {
+    const double ro = F.BRo[f];
+    const double ruf = F.BU[f] * ro, rvf = F.BV[f] * ro, rwf = F.BW[f] * ro;
+    F.RUFace[f] = ruf * n0 + rvf * n1 + rwf * n2;
+    return;
+  }
+  const int c1 = m.f_c1[f], c2 = m.f_c2[f];
+  const double lambda = m.f_lambda[f], lambda2 = 1. - lambda;
+  const double dx10 = m.f_xpac[3 * f] - m.c_x[3 * c1];
+  const double dx11 = m.f_xpac[3 * f + 1] - m.c_x[3 * c1 + 1];
+  const double dx12 = m.f_xpac[3 * f + 2] - m.c_x[3 * c1 + 2];
+  const double dx20 = m.f_xnac[3 * f] - m.c_x[3 * c2];
+  const double dx21 = m.f_xnac[3 * f + 1] - m.c_x[3 * c2 + 1];
+  const double dx22 = m.f_xnac[3 * f + 2] - m.c_x[3 * c2 + 2];
+  const double roapf = F.R[c1] * F.apr[c1] * lambda + F.R[c2] * F.apr[c2] * lambda2;
+  const double vol = m.c_vol[c1] * lambda + m.c_vol[c2] * lambda2;
+  const double coef = roapf * vol * m.f_rlencos[f];
+  const double* g1 = &F.dPdX[3 * c1];
+  const double* g2 = &F.dPdX[3 * c2];
+  const double dp1 = F.dp[c1] + urf * (g1[0] * dx10 + g1[1] * dx11 + g1[2] * dx12);
+  const double dp2 = F.dp[c2] + urf * (g2[0] * dx20 + g2[1] * dx21 + g2[2] * dx22);
+  F.RUFace[f] += -coef * (dp2 - dp1);
+}
+
+void launch_correct_ruface(const DMesh& m, const DFields& F, double urf, hipStream_t s) {
+  hipLaunchKernelGGL(k_correct_ruface, dim3(grid_for(m.Nf)), dim3(kBlock), 0, s, m, F, urf);
+  S355_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- residuals
+// sum |old - new| vol for u, v, w (navier_velocity.cpp:48-52): per-block partials.
+__global__ void __launch_bounds__(kBlock) k_residual3(DMesh m, const double* u0, const double* v0, const double* w0,
+                                                      const double* u, const double* v, const double* w,
+                                                      double* partial) {
+  __shared__ double lds[3][kBlock / kWave];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double r[3] = {0., 0., 0.};
+  if (i < m.Nc) {
+    const double vol = m.c_vol[i];
+    r[0] = fabs(u0[i] - u[i]) * vol;
+    r[1] = fabs(v0[i] - v[i]) * vol;
+    r[2] = fabs(w0[i] - w[i]) * vol;
+  }
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    r[c] = wave_sum(r[c]);
+    if (lane == 0) lds[c][wid] = r[c];
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double s = 0.;
+    for (int k = 0; k < kBlock / kWave; ++k) s += lds[threadIdx.x][k];
+    partial[3 * blockIdx.x + threadIdx.x] = s;
+  }
+}
+
+void launch_residual3(const DMesh& m, const double* u0, const double* v0, const double* w0, const double* u,
+                      const double* v, const double* w, double* partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_residual3, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, u0, v0, w0, u, v, w, partial);
+  S355_LAUNCH_CHECK();
+}
+
+// Sum `ncomp` interleaved partial columns in a fixed order; out[c] (+)= sum.
+__global__ void __launch_bounds__(kBlock) k_reduce_partials(const double* partial, int nblk, int ncomp, double* out,
+                                                            int accumulate) {
+  __shared__ double lds[kBlock / kWave];
+  const int c = blockIdx.x;
+  double s = 0.;
+  for (int b = threadIdx.x; b < nblk; b += kBlock) s += partial[ncomp * b + c];
+  s = block_sum(s, lds);
+  if (threadIdx.x == 0) out[c] = accumulate ? out[c] + s : s;
+}
+
+void launch_reduce_partials(const double* partial, int nblk, int ncomp, double* out, bool accumulate,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_partials, dim3(ncomp), dim3(kBlock), 0, s, partial, nblk, ncomp, out,
+                     accumulate ? 1 : 0);
+  S355_LAUNCH_CHECK();
+}
+
+}  // namespace s355

@@ -1,0 +1,52 @@
+// Device-side views of the mesh and the flow fields.  Plain structs of raw
+// device pointers, passed to kernels by value.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace s355 {
+
+// Boundary type codes (BdRegion::type1)
+enum : int { kWall = 1, kInlet = 2, kOutlet = 3, kSym = 4 };
+
+struct DMesh {
+  int Nc = 0, Nf = 0, Nb = 0, nnz = 0;
+  // faces; faces 0..Nb-1 are the boundary faces, face b <-> boundary record b
+  const int* f_c1 = nullptr;
+  const int* f_c2 = nullptr;
+  const double* f_lambda = nullptr;
+  const double* f_rlencos = nullptr;
+  const double* f_area = nullptr;
+  const double* f_n = nullptr;     // 3 per face
+  const double* f_x = nullptr;     // 3 per face
+  const double* f_xpac = nullptr;  // 3 per face
+  const double* f_xnac = nullptr;  // 3 per face
+  // cells
+  const double* c_vol = nullptr;
+  const double* c_x = nullptr;  // 3 per cell
+  const int* cf_ptr = nullptr;
+  const int* cf_pat = nullptr;
+  const int* cf_slot = nullptr;
+  const int* cf_sorted = nullptr;
+  // CSR
+  const int* rowptr = nullptr;
+  const int* col = nullptr;
+  const int* diag = nullptr;
+  // boundary records
+  const int* b_type = nullptr;
+  const double* b_dist = nullptr;
+  // cells touching a wall, with their wall faces in ascending order
+  int n_wcell = 0;
+  const int* wc_cell = nullptr;
+  const int* wc_ptr = nullptr;
+  const int* wc_face = nullptr;
+};
+
+struct DFields {
+  double *R, *U, *V, *W, *P, *T, *TE, *ED, *dp, *mu, *mut, *apr;
+  double *dUdX, *dVdX, *dWdX, *dPdX, *dPhidX;
+  double *BRo, *BU, *BV, *BW, *BPre, *BTE, *BED, *shear;
+  double* RUFace;
+  double *Au, *Ap, *As, *bu, *bv, *bw, *bp, *bs;
+};
+
+}  // namespace s355

@@ -1,0 +1,750 @@
+// Device GMRES(m) / CG for the SIMPLE solves (see krylov.hpp).
+//
+// One GMRES iteration is five launches: SpMV fused with the Jacobi apply,
+// a multi-dot against the basis, its fixed-order reduction, the Gram-Schmidt
+// update fused with the norm, and a one-block finalisation that updates the
+// Hessenberg column, applies the Givens rotations and runs the convergence
+// test.  Basis vectors are kept unnormalised (v_k = scale_k V_k); the scales
+// are folded into the coefficients, which saves a normalisation pass over N.
+// Every kernel returns at once when the device state says the solve has
+// ended, so the host may launch iterations ahead and read the state back
+// only every few iterations.
+#include "common.hpp"
+#include "krylov.hpp"
+
+namespace s355 {
+namespace {
+
+constexpr int kConvergedRtol = 2, kConvergedAtol = 3, kDivergedNull = -2, kDivergedIts = -3, kDivergedDtol = -4,
+              kDivergedBreakdown = -5, kDivergedIndefinitePc = -8, kDivergedNanorinf = -9,
+              kDivergedIndefiniteMat = -10;
+
+__device__ __forceinline__ int conv_check(const KState* st, double rnorm) {
+  if (isnan(rnorm) || isinf(rnorm)) return kDivergedNanorinf;
+  if (rnorm <= st->ttol) return rnorm < st->abstol ? kConvergedAtol : kConvergedRtol;
+  if (rnorm >= st->dtol * st->rnorm0) return kDivergedDtol;
+  return 0;
+}
+
+__device__ __forceinline__ double& Hm(KState* st, int ld, int r, int c) { return st->H[c * ld + r]; }
+
+// ---- preconditioner set-up
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_jacobi_setup(CsrView<T> A, T* inv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  const double d = static_cast<double>(A.val[A.diag[i]]);
+  inv[i] = static_cast<T>(d != 0.0 ? 1.0 / d : 1.0);
+}
+
+// ---- y = scale * M^-1 A x
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_spmv_pc(CsrView<T> A, const T* __restrict__ x, const T* __restrict__ inv,
+                                                    T* __restrict__ y, const KState* st, int it) {
+  if (st->reason) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  T s = T(0);
+  const int p1 = A.rowptr[i + 1];
+  for (int p = A.rowptr[i]; p < p1; ++p) s += A.val[p] * x[A.col[p]];
+  double z = static_cast<double>(s);
+  if (inv) z *= static_cast<double>(inv[i]);
+  y[i] = static_cast<T>(z * st->scale[it]);
+}
+
+// ---- partial[b][k] = sum over block b of w_i V_k,i, k = 0..it
+template <class T, int R>
+__global__ void __launch_bounds__(kBlock) k_mdot(const T* __restrict__ V, size_t ld, int n, int it,
+                                                 double* __restrict__ partial, int pstride, const KState* st) {
+  if (st->reason) return;
+  __shared__ double lds[kBlock / kWave][8];
+  const T* w = V + static_cast<size_t>(it + 1) * ld;
+  const size_t base = static_cast<size_t>(blockIdx.x) * (kBlock * R) + threadIdx.x;
+  double wr[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const size_t idx = base + static_cast<size_t>(r) * kBlock;
+    wr[r] = idx < static_cast<size_t>(n) ? static_cast<double>(w[idx]) : 0.0;
+  }
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  for (int k0 = 0; k0 <= it; k0 += 8) {
+    double acc[8];
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+      acc[kk] = 0.0;
+      const int k = k0 + kk;
+      if (k <= it) {
+        const T* vk = V + static_cast<size_t>(k) * ld;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const size_t idx = base + static_cast<size_t>(r) * kBlock;
+          if (idx < static_cast<size_t>(n)) acc[kk] += wr[r] * static_cast<double>(vk[idx]);
+        }
+      }
+    }
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+      acc[kk] = wave_sum(acc[kk]);
+      if (lane == 0) lds[wid][kk] = acc[kk];
+    }
+    __syncthreads();
+    if (threadIdx.x < 8 && k0 + static_cast<int>(threadIdx.x) <= it) {
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < kBlock / kWave; ++q) s += lds[q][threadIdx.x];
+      partial[static_cast<size_t>(blockIdx.x) * pstride + k0 + threadIdx.x] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// ---- hcol[k] = scale_k * sum_b partial[b][k]  (one block per k)
+__global__ void __launch_bounds__(kBlock) k_mdot_final(const double* partial, int nblk, int pstride, KState* st) {
+  if (st->reason) return;
+  __shared__ double lds[kBlock / kWave];
+  const int k = blockIdx.x;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += kBlock) s += partial[static_cast<size_t>(b) * pstride + k];
+  s = block_sum(s, lds);
+  if (threadIdx.x == 0) st->hcol[k] = s * st->scale[k];
+}
+
+// ---- w -= sum_k hcol_k scale_k V_k ; partial ||w||^2
+template <class T, int R>
+__global__ void __launch_bounds__(kBlock) k_maxpy_norm(T* __restrict__ V, size_t ld, int n, int it,
+                                                       double* __restrict__ partial, const KState* st) {
+  if (st->reason) return;
+  __shared__ double c[kMaxRestart + 1];
+  __shared__ double lds[kBlock / kWave];
+  for (int k = threadIdx.x; k <= it; k += kBlock) c[k] = st->hcol[k] * st->scale[k];
+  __syncthreads();
+  T* w = V + static_cast<size_t>(it + 1) * ld;
+  const size_t base = static_cast<size_t>(blockIdx.x) * (kBlock * R) + threadIdx.x;
+  double nrm = 0.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const size_t idx = base + static_cast<size_t>(r) * kBlock;
+    if (idx < static_cast<size_t>(n)) {
+      double s = static_cast<double>(w[idx]);
+      for (int k = 0; k <= it; ++k) s -= c[k] * static_cast<double>(V[static_cast<size_t>(k) * ld + idx]);
+      const T sv = static_cast<T>(s);
+      w[idx] = sv;
+      nrm += static_cast<double>(sv) * static_cast<double>(sv);
+    }
+  }
+  nrm = block_sum(nrm, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = nrm;
+}
+
+// ---- Hessenberg column, Givens rotations, convergence test (one block)
+__global__ void __launch_bounds__(kBlock) k_arnoldi_finalize(const double* partial, int nblk, int it, int m,
+                                                             KState* st) {
+  if (st->reason) return;
+  __shared__ double lds[kBlock / kWave];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += kBlock) s += partial[b];
+  s = block_sum(s, lds);
+  if (threadIdx.x != 0) return;
+  const int ld = m + 1;
+  const double tt = sqrt(s);
+  for (int j = 0; j <= it; ++j) Hm(st, ld, j, it) = st->hcol[j];
+  Hm(st, ld, it + 1, it) = tt;
+  st->scale[it + 1] = tt != 0.0 ? 1.0 / tt : 1.0;
+  double hapbnd = fabs(tt / st->grs[it]);
+  if (hapbnd > 1.e-30) hapbnd = 1.e-30;
+  const bool hapend = tt < hapbnd;
+  for (int j = 1; j <= it; ++j) {
+    const double t0 = Hm(st, ld, j - 1, it);
+    Hm(st, ld, j - 1, it) = st->cc[j - 1] * t0 + st->ss[j - 1] * Hm(st, ld, j, it);
+    Hm(st, ld, j, it) = st->cc[j - 1] * Hm(st, ld, j, it) - st->ss[j - 1] * t0;
+  }
+  double res;
+  int reason = 0;
+  if (!hapend) {
+    const double hv = Hm(st, ld, it, it), hn = Hm(st, ld, it + 1, it);
+    const double tr = sqrt(hv * hv + hn * hn);
+    if (tr == 0.0) {
+      reason = kDivergedNull;
+      res = st->rnorm;
+    } else {
+      st->cc[it] = hv / tr;
+      st->ss[it] = hn / tr;
+      st->grs[it + 1] = -(st->ss[it] * st->grs[it]);
+      st->grs[it] = st->cc[it] * st->grs[it];
+      Hm(st, ld, it, it) = st->cc[it] * hv + st->ss[it] * hn;
+      res = fabs(st->grs[it + 1]);
+    }
+  } else {
+    res = 0.0;
+  }
+  st->it = it + 1;
+  st->its += 1;
+  st->rnorm = res;
+  if (!reason) {
+    reason = conv_check(st, res);
+    if (hapend && !reason) reason = kDivergedBreakdown;
+  }
+  st->reason = reason;
+}
+
+// ---- back substitution; y_j = scale_j * nrs_j for the it completed vectors
+__global__ void k_build_y(KState* st, int m, int* nupd) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int ld = m + 1;
+  const int k = st->it - 1;
+  *nupd = 0;
+  if (k < 0) return;
+  double nrs[kMaxRestart + 1];
+  if (Hm(st, ld, k, k) == 0.0) {
+    st->reason = kDivergedBreakdown;
+    return;
+  }
+  nrs[k] = st->grs[k] / Hm(st, ld, k, k);
+  for (int ii = 1; ii <= k; ++ii) {
+    const int r = k - ii;
+    double tt = st->grs[r];
+    for (int j = r + 1; j <= k; ++j) tt = tt - Hm(st, ld, r, j) * nrs[j];
+    if (Hm(st, ld, r, r) == 0.0) {
+      st->reason = kDivergedBreakdown;
+      return;
+    }
+    nrs[r] = tt / Hm(st, ld, r, r);
+  }
+  for (int j = 0; j <= k; ++j) st->y[j] = nrs[j] * st->scale[j];
+  *nupd = k + 1;
+}
+
+template <class T, int R>
+__global__ void __launch_bounds__(kBlock) k_update_x(const T* __restrict__ V, size_t ld, int n, T* __restrict__ x,
+                                                     const KState* st, const int* nupd) {
+  __shared__ double y[kMaxRestart + 1];
+  const int nk = *nupd;
+  for (int k = threadIdx.x; k < nk; k += kBlock) y[k] = st->y[k];
+  __syncthreads();
+  if (nk == 0) return;
+  const size_t base = static_cast<size_t>(blockIdx.x) * (kBlock * R) + threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const size_t idx = base + static_cast<size_t>(r) * kBlock;
+    if (idx < static_cast<size_t>(n)) {
+      double s = 0.0;
+      for (int k = 0; k < nk; ++k) s += y[k] * static_cast<double>(V[static_cast<size_t>(k) * ld + idx]);
+      x[idx] = static_cast<T>(static_cast<double>(x[idx]) + s);
+    }
+  }
+}
+
+// ---- v0 = M^-1 (b - A x); partial ||v0||^2
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_residual_pc(CsrView<T> A, const T* __restrict__ b,
+                                                        const T* __restrict__ x, const T* __restrict__ inv,
+                                                        T* __restrict__ v0, double* partial, int zero_guess) {
+  __shared__ double lds[kBlock / kWave];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double nrm = 0.0;
+  if (i < A.n) {
+    double r = static_cast<double>(b[i]);
+    if (!zero_guess) {
+      T s = T(0);
+      const int p1 = A.rowptr[i + 1];
+      for (int p = A.rowptr[i]; p < p1; ++p) s += A.val[p] * x[A.col[p]];
+      r -= static_cast<double>(s);
+    }
+    if (inv) r *= static_cast<double>(inv[i]);
+    const T rv = static_cast<T>(r);
+    v0[i] = rv;
+    nrm = static_cast<double>(rv) * static_cast<double>(rv);
+  }
+  nrm = block_sum(nrm, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = nrm;
+}
+
+// ---- partial ||M^-1 b||^2 (reference norm for a nonzero initial guess)
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_pc_norm(int n, const T* __restrict__ b, const T* __restrict__ inv,
+                                                    double* partial) {
+  __shared__ double lds[kBlock / kWave];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double nrm = 0.0;
+  if (i < n) {
+    double r = static_cast<double>(b[i]);
+    if (inv) r *= static_cast<double>(inv[i]);
+    const T rv = static_cast<T>(r);
+    nrm = static_cast<double>(rv) * static_cast<double>(rv);
+  }
+  nrm = block_sum(nrm, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = nrm;
+}
+
+__global__ void __launch_bounds__(kBlock) k_snorm_final(const double* partial, int nblk, KState* st) {
+  __shared__ double lds[kBlock / kWave];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += kBlock) s += partial[b];
+  s = block_sum(s, lds);
+  if (threadIdx.x == 0) st->snorm = sqrt(s);
+}
+
+// ---- start of a GMRES cycle
+__global__ void __launch_bounds__(kBlock) k_restart_finalize(const double* partial, int nblk, KState* st, int first,
+                                                             int guess_zero) {
+  __shared__ double lds[kBlock / kWave];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += kBlock) s += partial[b];
+  s = block_sum(s, lds);
+  if (threadIdx.x != 0) return;
+  const double res = sqrt(s);
+  st->grs[0] = res;
+  st->scale[0] = res != 0.0 ? 1.0 / res : 1.0;
+  st->it = 0;
+  st->rnorm = res;
+  if (first) {
+    st->rnorm0 = guess_zero ? res : (st->snorm != 0.0 ? st->snorm : res);
+    st->ttol = fmax(st->rtol * st->rnorm0, st->abstol);
+  }
+  if (res == 0.0) {
+    st->reason = kConvergedAtol;
+    return;
+  }
+  st->reason = conv_check(st, res);
+}
+
+// ---- CG pieces (left-preconditioned, preconditioned residual norm)
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_cg_start(CsrView<T> A, const T* b, T* x, const T* inv, T* r, T* z,
+                                                     double* partial, int zero_guess) {
+  __shared__ double lds[kBlock / kWave];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double zz = 0.0, zr = 0.0;
+  if (i < A.n) {
+    double rv = static_cast<double>(b[i]);
+    if (zero_guess) {
+      x[i] = T(0);
+    } else {
+      T s = T(0);
+      for (int p = A.rowptr[i]; p < A.rowptr[i + 1]; ++p) s += A.val[p] * x[A.col[p]];
+      rv -= static_cast<double>(s);
+    }
+    const T rt = static_cast<T>(rv);
+    const T zt = inv ? static_cast<T>(static_cast<double>(rt) * static_cast<double>(inv[i])) : rt;
+    r[i] = rt;
+    z[i] = zt;
+    zz = static_cast<double>(zt) * static_cast<double>(zt);
+    zr = static_cast<double>(zt) * static_cast<double>(rt);
+  }
+  zz = block_sum(zz, lds);
+  zr = block_sum(zr, lds);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = zz;
+    partial[2 * blockIdx.x + 1] = zr;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_cg_start_final(const double* partial, int nblk, KState* st, int guess_zero) {
+  __shared__ double lds[kBlock / kWave];
+  double zz = 0.0, zr = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += kBlock) {
+    zz += partial[2 * b];
+    zr += partial[2 * b + 1];
+  }
+  zz = block_sum(zz, lds);
+  zr = block_sum(zr, lds);
+  if (threadIdx.x != 0) return;
+  const double dp = sqrt(zz);
+  st->rnorm = dp;
+  st->rnorm0 = guess_zero ? dp : (st->snorm != 0.0 ? st->snorm : dp);
+  st->ttol = fmax(st->rtol * st->rnorm0, st->abstol);
+  st->its = 0;
+  st->beta = zr;
+  st->reason = conv_check(st, dp);
+}
+
+// p = z + (beta/betaold) p  (p = z on the first iteration); checks on beta
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_cg_p(int n, const T* z, T* p, KState* st, int i_it) {
+  if (st->reason) return;
+  const double beta = st->beta, betaold = st->betaold;
+  if (beta == 0.0 || (i_it > 0 && beta * betaold < 0.0)) return;  // the finaliser records the reason
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (i_it == 0)
+    p[i] = z[i];
+  else
+    p[i] = static_cast<T>(static_cast<double>(z[i]) + (beta / betaold) * static_cast<double>(p[i]));
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_cg_spmv_dot(CsrView<T> A, const T* p, T* w, double* partial,
+                                                        const KState* st, int i_it) {
+  if (st->reason) return;
+  const double beta = st->beta;
+  if (beta == 0.0 || (i_it > 0 && beta * st->betaold < 0.0)) return;
+  __shared__ double lds[kBlock / kWave];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double d = 0.0;
+  if (i < A.n) {
+    T s = T(0);
+    for (int q = A.rowptr[i]; q < A.rowptr[i + 1]; ++q) s += A.val[q] * p[A.col[q]];
+    w[i] = s;
+    d = static_cast<double>(p[i]) * static_cast<double>(s);
+  }
+  d = block_sum(d, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = d;
+}
+
+__global__ void __launch_bounds__(kBlock) k_cg_alpha(const double* partial, int nblk, KState* st, int i_it) {
+  if (st->reason) return;
+  __shared__ double lds[kBlock / kWave];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += kBlock) s += partial[b];
+  s = block_sum(s, lds);
+  if (threadIdx.x != 0) return;
+  const double beta = st->beta;
+  if (beta == 0.0) {
+    st->reason = kConvergedAtol;
+    return;
+  }
+  if (i_it > 0 && beta * st->betaold < 0.0) {
+    st->reason = kDivergedIndefinitePc;
+    return;
+  }
+  const double dpi = s;
+  if (dpi == 0.0 || (i_it > 0 && dpi * st->dpiold <= 0.0)) {
+    st->reason = kDivergedIndefiniteMat;
+    return;
+  }
+  st->dpi = dpi;
+  st->alpha = beta / dpi;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_cg_update(int n, T* x, T* r, T* z, const T* p, const T* w, const T* inv,
+                                                      double* partial, const KState* st) {
+  if (st->reason) return;
+  __shared__ double lds[kBlock / kWave];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double a = st->alpha;
+  double zz = 0.0, zr = 0.0;
+  if (i < n) {
+    x[i] = static_cast<T>(static_cast<double>(x[i]) + a * static_cast<double>(p[i]));
+    const T rt = static_cast<T>(static_cast<double>(r[i]) - a * static_cast<double>(w[i]));
+    r[i] = rt;
+    const T zt = inv ? static_cast<T>(static_cast<double>(rt) * static_cast<double>(inv[i])) : rt;
+    z[i] = zt;
+    zz = static_cast<double>(zt) * static_cast<double>(zt);
+    zr = static_cast<double>(zt) * static_cast<double>(rt);
+  }
+  zz = block_sum(zz, lds);
+  zr = block_sum(zr, lds);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = zz;
+    partial[2 * blockIdx.x + 1] = zr;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_cg_final(const double* partial, int nblk, KState* st) {
+  if (st->reason) return;
+  __shared__ double lds[kBlock / kWave];
+  double zz = 0.0, zr = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += kBlock) {
+    zz += partial[2 * b];
+    zr += partial[2 * b + 1];
+  }
+  zz = block_sum(zz, lds);
+  zr = block_sum(zr, lds);
+  if (threadIdx.x != 0) return;
+  const double dp = sqrt(zz);
+  st->its += 1;
+  st->rnorm = dp;
+  st->dpiold = st->dpi;
+  st->betaold = st->beta;
+  st->beta = zr;
+  st->reason = conv_check(st, dp);
+}
+
+// ---- FP64 residual of a T matrix for iterative refinement
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_residual64(CsrView<T> A, const double* b, const double* x, double* r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  double s = 0.0;
+  for (int p = A.rowptr[i]; p < A.rowptr[i + 1]; ++p) s += static_cast<double>(A.val[p]) * x[A.col[p]];
+  r[i] = b[i] - s;
+}
+
+int pick_r(int n) {
+  if (n >= 8 * kBlock * 1024) return 8;
+  if (n >= 4 * kBlock * 1024) return 4;
+  if (n >= 2 * kBlock * 1024) return 2;
+  return 1;
+}
+
+template <class T, int R>
+void launch_mdot(const T* V, size_t ld, int n, int it, double* partial, int pstride, const KState* st, hipStream_t s) {
+  const int g = grid_for(n, kBlock * R);
+  hipLaunchKernelGGL((k_mdot<T, R>), dim3(g), dim3(kBlock), 0, s, V, ld, n, it, partial, pstride, st);
+}
+template <class T, int R>
+void launch_maxpy(T* V, size_t ld, int n, int it, double* partial, const KState* st, hipStream_t s) {
+  const int g = grid_for(n, kBlock * R);
+  hipLaunchKernelGGL((k_maxpy_norm<T, R>), dim3(g), dim3(kBlock), 0, s, V, ld, n, it, partial, st);
+}
+template <class T, int R>
+void launch_update(const T* V, size_t ld, int n, T* x, const KState* st, const int* nupd, hipStream_t s) {
+  const int g = grid_for(n, kBlock * R);
+  hipLaunchKernelGGL((k_update_x<T, R>), dim3(g), dim3(kBlock), 0, s, V, ld, n, x, st, nupd);
+}
+
+template <class T>
+struct Dispatch {
+  static void mdot(int R, const T* V, size_t ld, int n, int it, double* partial, int pstride, const KState* st,
+                   hipStream_t s) {
+    switch (R) {
+      case 8: launch_mdot<T, 8>(V, ld, n, it, partial, pstride, st, s); break;
+      case 4: launch_mdot<T, 4>(V, ld, n, it, partial, pstride, st, s); break;
+      case 2: launch_mdot<T, 2>(V, ld, n, it, partial, pstride, st, s); break;
+      default: launch_mdot<T, 1>(V, ld, n, it, partial, pstride, st, s); break;
+    }
+  }
+  static void maxpy(int R, T* V, size_t ld, int n, int it, double* partial, const KState* st, hipStream_t s) {
+    switch (R) {
+      case 8: launch_maxpy<T, 8>(V, ld, n, it, partial, st, s); break;
+      case 4: launch_maxpy<T, 4>(V, ld, n, it, partial, st, s); break;
+      case 2: launch_maxpy<T, 2>(V, ld, n, it, partial, st, s); break;
+      default: launch_maxpy<T, 1>(V, ld, n, it, partial, st, s); break;
+    }
+  }
+  static void update(int R, const T* V, size_t ld, int n, T* x, const KState* st, const int* nupd, hipStream_t s) {
+    switch (R) {
+      case 8: launch_update<T, 8>(V, ld, n, x, st, nupd, s); break;
+      case 4: launch_update<T, 4>(V, ld, n, x, st, nupd, s); break;
+      case 2: launch_update<T, 2>(V, ld, n, x, st, nupd, s); break;
+      default: launch_update<T, 1>(V, ld, n, x, st, nupd, s); break;
+    }
+  }
+};
+
+void read_header(KWork& w, hipStream_t s) {
+  S355_HIP(hipMemcpyAsync(w.host, w.st, sizeof(KHeader), hipMemcpyDeviceToHost, s));
+  S355_HIP(hipStreamSynchronize(s));
+}
+
+void init_state(KWork& w, const s355_ksp_opts& o, hipStream_t s) {
+  KState h{};
+  h.reason = 0;
+  h.maxit = o.maxit;
+  h.rtol = o.rtol;
+  h.abstol = o.abstol;
+  h.dtol = o.dtol;
+  // header + scalars only: the arrays are written before being read
+  S355_HIP(hipMemcpyAsync(w.st, &h, offsetof(KState, H), hipMemcpyHostToDevice, s));
+  S355_HIP(hipMemcpyAsync(&w.st->beta, &h.beta, 5 * sizeof(double), hipMemcpyHostToDevice, s));
+  // the pinned staging copy must not be reused before the copy completes
+  S355_HIP(hipStreamSynchronize(s));
+}
+
+template <class T>
+KResult gmres(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, bool zero_guess, KWork& w,
+              hipStream_t s) {
+  const int n = A.n;
+  const int m = o.restart < 1 ? 1 : (o.restart > kMaxRestart ? kMaxRestart : o.restart);
+  if (m > w.mmax) throw std::invalid_argument("gmres: restart above workspace size");
+  const size_t ld = static_cast<size_t>(n);
+  T* V = static_cast<T*>(w.V);
+  T* inv = nullptr;
+  if (o.pc == S355_PC_JACOBI) {
+    inv = static_cast<T*>(w.pcinv);
+    hipLaunchKernelGGL((k_jacobi_setup<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, A, inv);
+  } else if (o.pc != S355_PC_NONE) {
+    throw std::invalid_argument("gmres: preconditioner not available on the device path (use Jacobi or none)");
+  }
+  init_state(w, o, s);
+  const int g1 = grid_for(n);
+  const int R = pick_r(n);
+  const int gR = grid_for(n, kBlock * R);
+  const int pstride = m + 1;
+  if (static_cast<size_t>(gR) * pstride > w.partial_len || static_cast<size_t>(g1) > w.partial_len)
+    throw std::runtime_error("gmres: partial buffer too small");
+  int* nupd = reinterpret_cast<int*>(&w.st->pad);
+  if (!zero_guess) {
+    hipLaunchKernelGGL((k_pc_norm<T>), dim3(g1), dim3(kBlock), 0, s, n, b, inv, w.partial);
+    hipLaunchKernelGGL(k_snorm_final, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st);
+  } else {
+    fill_zero(x, sizeof(T) * static_cast<size_t>(n), s);
+  }
+  bool first = true, gz = zero_guess;
+  int its = 0;
+  const int check_every = n >= (1 << 20) ? 4 : 8;
+  KResult res;
+  for (;;) {
+    hipLaunchKernelGGL((k_residual_pc<T>), dim3(g1), dim3(kBlock), 0, s, A, b, x, inv, V, w.partial, gz ? 1 : 0);
+    hipLaunchKernelGGL(k_restart_finalize, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st, first ? 1 : 0,
+                       zero_guess ? 1 : 0);
+    S355_LAUNCH_CHECK();
+    first = false;
+    gz = false;
+    int it = 0;
+    bool ended = false;
+    while (it < m && its < o.maxit) {
+      hipLaunchKernelGGL((k_spmv_pc<T>), dim3(g1), dim3(kBlock), 0, s, A, V + it * ld, inv, V + (it + 1) * ld, w.st,
+                         it);
+      Dispatch<T>::mdot(R, V, ld, n, it, w.partial, pstride, w.st, s);
+      hipLaunchKernelGGL(k_mdot_final, dim3(it + 1), dim3(kBlock), 0, s, w.partial, gR, pstride, w.st);
+      Dispatch<T>::maxpy(R, V, ld, n, it, w.partial, w.st, s);
+      hipLaunchKernelGGL(k_arnoldi_finalize, dim3(1), dim3(kBlock), 0, s, w.partial, gR, it, m, w.st);
+      S355_LAUNCH_CHECK();
+      ++it;
+      ++its;
+      if (it % check_every == 0 && it < m) {
+        read_header(w, s);
+        if (w.host->reason) {
+          ended = true;
+          break;
+        }
+      }
+    }
+    (void)ended;
+    hipLaunchKernelGGL(k_build_y, dim3(1), dim3(1), 0, s, w.st, m, nupd);
+    Dispatch<T>::update(R, V, ld, n, x, w.st, nupd, s);
+    S355_LAUNCH_CHECK();
+    read_header(w, s);
+    its = w.host->its;
+    if (w.host->reason) break;
+    if (its >= o.maxit) {
+      res.reason = kDivergedIts;
+      break;
+    }
+  }
+  res.iters = w.host->its;
+  if (res.reason == 0) res.reason = w.host->reason;
+  res.rnorm = w.host->rnorm;
+  res.rnorm0 = w.host->rnorm0;
+  return res;
+}
+
+template <class T>
+KResult cg(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, bool zero_guess, KWork& w, hipStream_t s) {
+  const int n = A.n;
+  T* V = static_cast<T*>(w.V);
+  const size_t ld = static_cast<size_t>(n);
+  if (w.mmax < 3) throw std::invalid_argument("cg: workspace needs 4 vectors");
+  T *r = V, *z = V + ld, *p = V + 2 * ld, *wv = V + 3 * ld;
+  T* inv = nullptr;
+  if (o.pc == S355_PC_JACOBI) {
+    inv = static_cast<T*>(w.pcinv);
+    hipLaunchKernelGGL((k_jacobi_setup<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, A, inv);
+  } else if (o.pc != S355_PC_NONE) {
+    throw std::invalid_argument("cg: preconditioner not available on the device path (use Jacobi or none)");
+  }
+  init_state(w, o, s);
+  const int g1 = grid_for(n);
+  if (2 * static_cast<size_t>(g1) > w.partial_len) throw std::runtime_error("cg: partial buffer too small");
+  if (!zero_guess) {
+    hipLaunchKernelGGL((k_pc_norm<T>), dim3(g1), dim3(kBlock), 0, s, n, b, inv, w.partial);
+    hipLaunchKernelGGL(k_snorm_final, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st);
+  }
+  hipLaunchKernelGGL((k_cg_start<T>), dim3(g1), dim3(kBlock), 0, s, A, b, x, inv, r, z, w.partial, zero_guess ? 1 : 0);
+  hipLaunchKernelGGL(k_cg_start_final, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st, zero_guess ? 1 : 0);
+  S355_LAUNCH_CHECK();
+  const int check_every = n >= (1 << 20) ? 4 : 8;
+  KResult res;
+  int i = 0;
+  for (; i < o.maxit; ++i) {
+    hipLaunchKernelGGL((k_cg_p<T>), dim3(g1), dim3(kBlock), 0, s, n, z, p, w.st, i);
+    hipLaunchKernelGGL((k_cg_spmv_dot<T>), dim3(g1), dim3(kBlock), 0, s, A, p, wv, w.partial, w.st, i);
+    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st, i);
+    hipLaunchKernelGGL((k_cg_update<T>), dim3(g1), dim3(kBlock), 0, s, n, x, r, z, p, wv, inv, w.partial, w.st);
+    hipLaunchKernelGGL(k_cg_final, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st);
+    S355_LAUNCH_CHECK();
+    if ((i + 1) % check_every == 0) {
+      read_header(w, s);
+      if (w.host->reason) break;
+    }
+  }
+  read_header(w, s);
+  res.iters = w.host->its;
+  res.reason = w.host->reason;
+  if (res.reason == 0) res.reason = kDivergedIts;
+  res.rnorm = w.host->rnorm;
+  res.rnorm0 = w.host->rnorm0;
+  return res;
+}
+
+__global__ void k_bench_init(KState* st) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    st->reason = 0;
+    st->scale[0] = 1.0;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_to_float(const double* src, float* dst, size_t n) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = static_cast<float>(src[i]);
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+template <class T>
+KResult krylov_solve(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, bool zero_guess, KWork& w,
+                     hipStream_t s) {
+  if (A.n <= 0) return KResult{0, kConvergedAtol, 0.0, 0.0};
+  if (o.maxit < 0 || !(o.rtol >= 0.0)) throw std::invalid_argument("krylov: bad tolerances");
+  if (o.type == S355_KSP_CG) return cg(A, b, x, o, zero_guess, w, s);
+  return gmres(A, b, x, o, zero_guess, w, s);
+}
+
+template <class T>
+void residual64(const CsrView<T>& A, const double* b, const double* x, double* r, hipStream_t s) {
+  hipLaunchKernelGGL((k_residual64<T>), dim3(grid_for(A.n)), dim3(kBlock), 0, s, A, b, x, r);
+  S355_LAUNCH_CHECK();
+}
+
+template KResult krylov_solve<double>(const CsrView<double>&, const double*, double*, const s355_ksp_opts&, bool,
+                                      KWork&, hipStream_t);
+template KResult krylov_solve<float>(const CsrView<float>&, const float*, float*, const s355_ksp_opts&, bool, KWork&,
+                                     hipStream_t);
+template void residual64<double>(const CsrView<double>&, const double*, const double*, double*, hipStream_t);
+template void residual64<float>(const CsrView<float>&, const double*, const double*, double*, hipStream_t);
+
+template <class T>
+double bench_spmv(const CsrView<T>& A, const T* x, T* y, KWork& w, int repeats, hipStream_t s) {
+  hipLaunchKernelGGL(k_bench_init, dim3(1), dim3(1), 0, s, w.st);
+  const int g = grid_for(A.n);
+  hipLaunchKernelGGL((k_spmv_pc<T>), dim3(g), dim3(kBlock), 0, s, A, x, static_cast<const T*>(nullptr), y, w.st, 0);
+  S355_LAUNCH_CHECK();
+  hipEvent_t e0, e1;
+  S355_HIP(hipEventCreate(&e0));
+  S355_HIP(hipEventCreate(&e1));
+  S355_HIP(hipEventRecord(e0, s));
+  for (int r = 0; r < repeats; ++r)
+    hipLaunchKernelGGL((k_spmv_pc<T>), dim3(g), dim3(kBlock), 0, s, A, x, static_cast<const T*>(nullptr), y, w.st, 0);
+  S355_HIP(hipEventRecord(e1, s));
+  S355_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  S355_HIP(hipEventElapsedTime(&ms, e0, e1));
+  S355_HIP(hipEventDestroy(e0));
+  S355_HIP(hipEventDestroy(e1));
+  return static_cast<double>(ms) / repeats;
+}
+
+void convert_to_float(const double* src, float* dst, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_to_float, dim3(grid_for(static_cast<long>(n))), dim3(kBlock), 0, s, src, dst, n);
+  S355_LAUNCH_CHECK();
+}
+
+void fill_zero(void* p, size_t bytes, hipStream_t s) {
+  const size_t n = bytes / 4;
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(static_cast<long>(n))), dim3(kBlock), 0, s, static_cast<uint32_t*>(p),
+                     n, 0u);
+  S355_LAUNCH_CHECK();
+}
+
+template double bench_spmv<double>(const CsrView<double>&, const double*, double*, KWork&, int, hipStream_t);
+template double bench_spmv<float>(const CsrView<float>&, const float*, float*, KWork&, int, hipStream_t);
+
+}  // namespace s355

@@ -1,0 +1,149 @@
+// Host NavierStokesSolver: the reference's SIMPLE driver logic over the C ABI.
+//   CalculateVelocity   navier_velocity.cpp:10-60
+//   CalculatePressure   navier_pressure.cpp:9-97
+//   UpdateTurKEpsilon   navier_scalar.cpp:11-197
+//   NSSolve             navier.cpp:52-141 (steady: one SIMPLE pass per step)
+// Log stages follow navier.cpp:26-37 (1 BuildVelocity, 2 SolveVelocity,
+// 3 BuildPressure, 4 SolvePressure, 5 other).  A failed solve throws
+// ConvergeError as the reference does; the caller turns it into a return
+// code instead of the reference's blocking abort (BasicType.h:23-55).
+#include "navier.hpp"
+
+#include <algorithm>
+
+namespace s355host {
+
+namespace {
+void throw_if(int rc, const char* what) {
+  if (rc < 0) throw DeviceError{rc, std::string(what) + ": " + s355_last_error()};
+}
+void check_solve(int rc, const s355_solve_info& info, const char* var) {
+  throw_if(rc, var);
+  if (rc == S355_NOT_CONVERGED || info.reason < 0) throw ConvergeError{info.iters, info.rnorm, var, info.reason};
+}
+}  // namespace
+
+void NavierStokesSolver::check(int rc, const char* what) { throw_if(rc, what); }
+
+void DataPartition::solveVelocity_GMRES(double tol, int maxIter, s355_solve_info info[3]) {
+  int rc = s355_solve_velocity(ctx_, tol, maxIter, info);
+  throw_if(rc, "solveVelocity_GMRES");
+  static const char* names[3] = {"Un", "Vn", "Wn"};
+  for (int k = 0; k < 3; ++k)
+    if (info[k].reason < 0) throw ConvergeError{info[k].iters, info[k].rnorm, names[k], info[k].reason};
+}
+
+void DataPartition::solvePressureCorrection(double tol, int maxIter, s355_solve_info* info) {
+  int rc = s355_solve_pressure(ctx_, tol, maxIter, info);
+  check_solve(rc, *info, "Pn");
+}
+
+void DataPartition::solveScarlar_GMRES(double tol, int maxIter, int isca, s355_solve_info* info) {
+  int rc = s355_solve_scalar(ctx_, isca, tol, maxIter, info);
+  check_solve(rc, *info, isca == 2 ? "TE" : "ED");
+}
+
+int NavierStokesSolver::Gradient(int phi, int bphi, int phigd) {
+  check(s355_gradient(ctx_, phi, bphi, phigd), "Gradient");
+  return 0;
+}
+void NavierStokesSolver::SetBCVelocity() { check(s355_bc_velocity(ctx_), "SetBCVelocity"); }
+void NavierStokesSolver::SetBCPressure() { check(s355_bc_pressure(ctx_), "SetBCPressure"); }
+void NavierStokesSolver::SetBCDeltaP() { check(s355_bc_deltap(ctx_), "SetBCDeltaP"); }
+void NavierStokesSolver::SetBCKEpsilon() { check(s355_bc_kepsilon(ctx_), "SetBCKEpsilon"); }
+
+// BC, four fused gradients and the cell loop all happen behind this entry
+// (navier_velocity.cpp:74-79 then 100-383).
+void NavierStokesSolver::BuildVelocityMatrix() { check(s355_build_velocity_matrix(ctx_), "BuildVelocityMatrix"); }
+void NavierStokesSolver::CalRUFace2() { check(s355_cal_ru_face(ctx_), "CalRUFace2"); }
+void NavierStokesSolver::BuildPressureMatrix() { check(s355_build_pressure_matrix(ctx_), "BuildPressureMatrix"); }
+void NavierStokesSolver::CorrectRUFace2() { check(s355_correct_ru_face(ctx_), "CorrectRUFace2"); }
+void NavierStokesSolver::BuildScalarMatrix(int iSca) {
+  check(s355_build_scalar_matrix(ctx_, iSca), "BuildScalarMatrix");
+}
+
+int NavierStokesSolver::CalculateVelocity() {
+  check(s355_log_stage_push(ctx_, 1), "log");
+  BuildVelocityMatrix();
+  check(s355_log_stage_pop(ctx_), "log");
+  check(s355_log_stage_push(ctx_, 2), "log");
+  check(s355_save_velocity(ctx_), "save velocity");
+  const s355_ksp_opts& o = par_.ksp[S355_KSP_VELOCITY];
+  try {
+    dataPartition.solveVelocity_GMRES(o.rtol, o.maxit, &last[0]);
+  } catch (...) {
+    s355_log_stage_pop(ctx_);
+    throw;
+  }
+  check(s355_log_stage_pop(ctx_), "log");
+  check(s355_log_stage_push(ctx_, 1), "log");
+  check(s355_velocity_residual(ctx_), "velocity residual");
+  check(s355_log_stage_pop(ctx_), "log");
+  return 0;
+}
+
+int NavierStokesSolver::CalculatePressure() {
+  check(s355_log_stage_push(ctx_, 3), "log");
+  SetBCVelocity();
+  CalRUFace2();
+  BuildPressureMatrix();
+  check(s355_log_stage_pop(ctx_), "log");
+  check(s355_log_stage_push(ctx_, 4), "log");
+  const s355_ksp_opts& o = par_.ksp[S355_KSP_PRESSURE];
+  try {
+    dataPartition.solvePressureCorrection(o.rtol, o.maxit, &last[3]);
+  } catch (...) {
+    s355_log_stage_pop(ctx_);
+    throw;
+  }
+  check(s355_log_stage_pop(ctx_), "log");
+  check(s355_log_stage_push(ctx_, 3), "log");
+  // SetBCDeltaP, Gradient(deltaP) and the correction loop (navier_pressure.cpp:42-70)
+  check(s355_correct_fields(ctx_, nullptr), "correct fields");
+  SetBCVelocity();
+  CorrectRUFace2();
+  check(s355_log_stage_pop(ctx_), "log");
+  return 0;
+}
+
+void NavierStokesSolver::UpdateTurKEpsilon() {
+  check(s355_log_stage_push(ctx_, 5), "log");
+  const s355_ksp_opts& o = par_.ksp[S355_KSP_SCALAR];
+  try {
+    check(s355_kepsilon_sources(ctx_), "k-eps sources");
+    SetBCKEpsilon();
+    BuildScalarMatrix(2);
+    dataPartition.solveScarlar_GMRES(o.rtol, o.maxit, 2, &last[4]);
+    check(s355_kepsilon_post_k(ctx_), "k clip / eps sources");
+    BuildScalarMatrix(3);
+    check(s355_eps_wall_rows(ctx_), "eps wall rows");
+    dataPartition.solveScarlar_GMRES(o.rtol, o.maxit, 3, &last[5]);
+    check(s355_kepsilon_post_eps(ctx_), "eps clip / mu_t");
+  } catch (...) {
+    s355_log_stage_pop(ctx_);
+    throw;
+  }
+  check(s355_log_stage_pop(ctx_), "log");
+}
+
+int NavierStokesSolver::NSSolve(int max_step, double* history, int* steps_done) {
+  *steps_done = 0;
+  for (int step = 1; step <= max_step; ++step) {
+    check(s355_residuals(ctx_, Residual, 1), "residual reset");
+    CalculateVelocity();
+    CalculatePressure();
+    if (par_.turb_model == 1) UpdateTurKEpsilon();
+    check(s355_residuals(ctx_, Residual, 0), "residuals");
+    double resmax = 0.0;  // vec_max starts at 0 (tools.cpp:45-51)
+    for (double r : Residual) resmax = std::max(resmax, r);
+    if (history) {
+      std::copy(Residual, Residual + 10, history + 11 * (step - 1));
+      history[11 * (step - 1) + 10] = resmax;
+    }
+    *steps_done = step;
+    if (resmax < par_.residual_steady) break;
+  }
+  return 0;
+}
+
+}  // namespace s355host

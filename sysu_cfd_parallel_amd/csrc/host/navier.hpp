@@ -1,0 +1,79 @@
+// Host-side NavierStokesSolver with the reference's member surface
+// (navier.h:173-206) and DataPartition solve surface (MPIStructure.h:224-228).
+// It drives the device exclusively through the C ABI in simple355.h, so the
+// SIMPLE loop (navier.cpp:52-141) reads as the reference's does while every
+// field, matrix and vector stays resident in HBM.
+#pragma once
+#include <string>
+
+#include "../../../include/simple355.h"
+
+namespace s355host {
+
+// ConvergeError (BasicType.h:62-68)
+struct ConvergeError {
+  int iter;
+  double residual;
+  std::string varname;
+  int reason;
+};
+
+// Device error surfaced as an exception, carrying the C ABI return code.
+struct DeviceError {
+  int code;
+  std::string msg;
+};
+
+class DataPartition {
+ public:
+  explicit DataPartition(s355_ctx* c) : ctx_(c) {}
+  // solveVelocity_GMRES (MPIStructure.cpp:234-375): U, V, W in place.
+  void solveVelocity_GMRES(double tol, int maxIter, s355_solve_info info[3]);
+  // solvePressureCorrection (MPIStructure.cpp:449-531): zero initial guess.
+  void solvePressureCorrection(double tol, int maxIter, s355_solve_info* info);
+  // solveScarlar_GMRES (MPIStructure.cpp:378-446): isca 2 = k, 3 = eps.
+  void solveScarlar_GMRES(double tol, int maxIter, int isca, s355_solve_info* info);
+
+ private:
+  s355_ctx* ctx_;
+};
+
+class NavierStokesSolver {
+ public:
+  NavierStokesSolver(s355_ctx* ctx, const s355_params& p) : ctx_(ctx), par_(p), dataPartition(ctx) {}
+
+  void set_params(const s355_params& p) { par_ = p; }
+
+  // SIMPLE phases (navier.cpp:68, 70, 78)
+  int CalculateVelocity();
+  int CalculatePressure();
+  void UpdateTurKEpsilon();
+
+  // members they are made of
+  void BuildVelocityMatrix();
+  void CalRUFace2();
+  void BuildPressureMatrix();
+  void CorrectRUFace2();
+  void BuildScalarMatrix(int iSca);
+  int Gradient(int phi, int bphi, int phigd);
+  void SetBCVelocity();
+  void SetBCPressure();
+  void SetBCDeltaP();
+  void SetBCKEpsilon();
+
+  // NSSolve steady loop; history gets 11 doubles per step.
+  int NSSolve(int max_step, double* history, int* steps_done);
+
+  double Residual[10] = {0};
+  s355_solve_info last[6] = {};  // u, v, w, p, k, eps
+
+ private:
+  void check(int rc, const char* what);
+  s355_ctx* ctx_;
+  s355_params par_;
+
+ public:
+  DataPartition dataPartition;
+};
+
+}  // namespace s355host

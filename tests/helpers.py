@@ -1,0 +1,79 @@
+"""Shared helpers for the tests: hand-built element streams, parameter sets
+and field initialisation used by both the oracle and the HIP library."""
+from __future__ import annotations
+
+import numpy as np
+
+import sysu_cfd_parallel_amd as s
+from sysu_cfd_parallel_amd._abi import Region
+
+
+def region(rid, type1, init=None, type2=0):
+    r = Region()
+    r.rid, r.type1, r.type2 = rid, type1, type2
+    for k, v in enumerate(init or []):
+        r.init[k] = v
+    return r
+
+
+def manual_case(xyz, elems, regions) -> s.CaseData:
+    """elems: list of (gmsh_type, tag, [vertex ids])"""
+    et = np.array([e[0] for e in elems], dtype=np.int32)
+    tg = np.array([e[1] for e in elems], dtype=np.int32)
+    vp = np.zeros(len(elems) + 1, dtype=np.int32)
+    vi = []
+    for k, e in enumerate(elems):
+        vi.extend(e[2])
+        vp[k + 1] = len(vi)
+    return s.CaseData(np.asarray(xyz, dtype=np.float64).ravel(), et, tg, vp, np.asarray(vi, dtype=np.int32),
+                      list(regions), {"kind": "manual"})
+
+
+def tet_cases():
+    """A single tetrahedron and two tetrahedra sharing a face, all walls."""
+    regs = [region(1, 1), region(100, 5, [0, 0, 0, 1, 300, 0.01, 0, 0])]
+    x1 = [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]]
+    one = manual_case(x1, [(2, 1, [0, 2, 1]), (2, 1, [0, 1, 3]), (2, 1, [0, 3, 2]), (2, 1, [1, 2, 3]),
+                           (4, 100, [0, 1, 2, 3])], regs)
+    x2 = x1 + [[1, 1, 1]]
+    two = manual_case(x2, [(2, 1, [0, 2, 1]), (2, 1, [0, 1, 3]), (2, 1, [0, 3, 2]), (2, 1, [1, 4, 2]),
+                           (2, 1, [1, 3, 4]), (2, 1, [2, 4, 3]), (4, 100, [0, 1, 2, 3]), (4, 100, [1, 2, 3, 4])], regs)
+    return one, two
+
+
+def params(restart=30, pc=s.PC_JACOBI, rtol=1e-8, maxit=1000, ksp=s.KSP_GMRES, turb=0):
+    p = s.default_params()
+    s.set_all_ksp(p, ksp, restart, pc, rtol, maxit)
+    p.turb_model = turb
+    return p
+
+
+def random_state(n_cell, n_bnd, n_face, seed=0):
+    """Random but physically shaped fields for kernel-level parity checks."""
+    rng = np.random.default_rng(seed)
+    f = {
+        "U": rng.uniform(-1, 1, n_cell), "V": rng.uniform(-1, 1, n_cell), "W": rng.uniform(-1, 1, n_cell),
+        "P": rng.uniform(-1, 1, n_cell), "R": rng.uniform(0.8, 1.2, n_cell),
+        "VISLAM": rng.uniform(0.005, 0.02, n_cell), "VISTUR": rng.uniform(0.0, 0.01, n_cell),
+        "TE": rng.uniform(0.01, 0.1, n_cell), "ED": rng.uniform(0.01, 0.1, n_cell),
+        "DELTAP": rng.uniform(-0.1, 0.1, n_cell), "APR": rng.uniform(0.5, 2.0, n_cell),
+        "BU": rng.uniform(-1, 1, n_bnd), "BV": rng.uniform(-1, 1, n_bnd), "BW": rng.uniform(-1, 1, n_bnd),
+        "BRO": rng.uniform(0.8, 1.2, n_bnd), "BPRE": rng.uniform(-1, 1, n_bnd),
+        "BTE": rng.uniform(0.01, 0.1, n_bnd), "BED": rng.uniform(0.01, 0.1, n_bnd),
+        "RUFACE": rng.uniform(-0.01, 0.01, n_face),
+        "DUDX": rng.uniform(-1, 1, 3 * n_cell), "DVDX": rng.uniform(-1, 1, 3 * n_cell),
+        "DWDX": rng.uniform(-1, 1, 3 * n_cell), "DPDX": rng.uniform(-1, 1, 3 * n_cell),
+    }
+    return f
+
+
+def push(obj, state):
+    for k, v in state.items():
+        obj.set(k, v)
+
+
+def rel_err(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    den = max(np.max(np.abs(b)), 1e-300)
+    return float(np.max(np.abs(a - b)) / den)

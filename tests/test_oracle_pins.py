@@ -1,0 +1,384 @@
+"""Pins of the FP64 CPU oracle against closed forms, invariants and known
+answers.  The reference holds no tests, meshes or golden vectors, so each
+oracle function is checked against what the mathematics fixes, chosen so a
+dropped term, a wrong sign or a transposed operand fails a check."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import sysu_cfd_parallel_amd as s
+from helpers import params, push, random_state, rel_err, tet_cases
+from oracle.pyoracle import Oracle
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+PERM40 = [30, 21, 22, 23, 24, 25, 26, 27, 28, 29, 20, 31, 32, 33, 34, 35, 36, 37, 38, 39, 10, 1, 2, 3, 4, 5, 6,
+          7, 8, 9, 0, 11, 12, 13, 14, 15, 16, 17, 18, 19]
+
+
+def sort_probe(n: int, tmp_path: Path) -> list:
+    exe = tmp_path / "sort_probe"
+    if not exe.exists():
+        subprocess.run(["g++", "-O2", "-o", str(exe), str(GOLDEN / "sort_probe.cpp")], check=True)
+    out = subprocess.run([str(exe), str(n)], check=True, capture_output=True, text=True).stdout
+    return [int(x) for x in out.split()]
+
+
+# ---------------------------------------------------------------- element order
+def test_sort_permutation_probe(tmp_path):
+    assert sort_probe(40, tmp_path) == PERM40
+
+
+def test_oracle_element_order_is_the_sort_permutation(tmp_path):
+    case = s.cavity(4)
+    o = Oracle(case, params())
+    perm = o.ints("ELEM_PERM")
+    assert perm.tolist() == sort_probe(len(perm), tmp_path)
+    assert sorted(perm.tolist()) == list(range(len(perm)))
+
+
+# ---------------------------------------------------------------- face builder
+def test_face_counts_single_hex():
+    o = Oracle(s.cavity(1), params())
+    assert (o.sizes.n_cell, o.sizes.n_face, o.sizes.n_bnd, o.sizes.n_interior) == (1, 6, 6, 0)
+    assert o.ints("CELL_NFACE").tolist() == [6]
+
+
+def test_face_counts_two_hexes():
+    o = Oracle(s.channel(2, 1, 1, lx=2, ly=1, lz=1), params())
+    assert (o.sizes.n_cell, o.sizes.n_face, o.sizes.n_interior) == (2, 11, 1)
+    f = o.sizes.n_bnd
+    assert o.ints("FACE_C1")[f] == 0 and o.ints("FACE_C2")[f] == 1
+    assert o.sizes.nnz == 4
+
+
+def test_face_counts_tets():
+    one, two = tet_cases()
+    o1 = Oracle(one, params())
+    assert (o1.sizes.n_cell, o1.sizes.n_face, o1.sizes.n_interior) == (1, 4, 0)
+    o2 = Oracle(two, params())
+    assert (o2.sizes.n_cell, o2.sizes.n_face, o2.sizes.n_interior) == (2, 7, 1)
+
+
+@pytest.mark.parametrize("case", [
+    lambda: s.cavity(5), lambda: s.cavity(4, jitter=0.15, seed=2), lambda: s.cavity(3, tets=True),
+    lambda: s.channel(5, 3, 4, jitter=0.12, split=0.3, seed=5), lambda: s.step(2, upstream=2, downstream=3, span=1),
+])
+def test_face_incidence_invariants(case):
+    o = Oracle(case(), params())
+    z = o.sizes
+    nface = o.ints("CELL_NFACE")
+    assert nface.sum() == 2 * z.n_interior + z.n_bnd
+    c1, c2 = o.ints("FACE_C1"), o.ints("FACE_C2")
+    assert np.all(c2[: z.n_bnd] == -10) and np.all(c2[z.n_bnd:] >= 0)
+    assert np.all(c1 >= 0)
+    # rows sorted, one diagonal each, nnz = N + 2 Nf
+    rp, col, dg = o.ints("CSR_ROWPTR"), o.ints("CSR_COL"), o.ints("CSR_DIAG")
+    assert rp[-1] == z.nnz == z.n_cell + 2 * z.n_interior
+    for i in range(z.n_cell):
+        row = col[rp[i]:rp[i + 1]]
+        assert np.all(np.diff(row) > 0) and col[dg[i]] == i
+
+
+# ---------------------------------------------------------------- geometry
+def test_uniform_cube_closed_forms():
+    n = 4
+    h = 1.0 / n
+    o = Oracle(s.cavity(n), params())
+    z = o.sizes
+    assert np.allclose(o.dbls("CELL_VOL"), h ** 3, rtol=1e-13, atol=0)
+    assert np.allclose(o.dbls("FACE_AREA"), h ** 2, rtol=1e-13, atol=0)
+    lam = o.dbls("FACE_LAMBDA")
+    assert np.allclose(lam[z.n_bnd:], 0.5, rtol=1e-13) and np.all(lam[: z.n_bnd] == 1.0)
+    rl = o.dbls("FACE_RLENCOS")
+    assert np.allclose(rl[z.n_bnd:], h, rtol=1e-12) and np.allclose(rl[: z.n_bnd], 2 * h, rtol=1e-12)
+    assert np.allclose(o.dbls("BND_DIST"), h / 2, rtol=1e-12)
+    # Xpac / Xnac: projections of the cell centres on the face-normal line
+    # through the face centre, i.e. the centres themselves on an orthogonal mesh
+    x = o.dbls("CELL_X").reshape(-1, 3)
+    nrm = o.dbls("FACE_N").reshape(-1, 3)
+    c1, c2 = o.ints("FACE_C1"), o.ints("FACE_C2")
+    assert np.allclose(o.dbls("FACE_XPAC").reshape(-1, 3)[z.n_bnd:], x[c1[z.n_bnd:]], atol=1e-15)
+    assert np.allclose(o.dbls("FACE_XNAC").reshape(-1, 3)[z.n_bnd:], x[c2[z.n_bnd:]], atol=1e-15)
+    # normals point from cell1 to cell2
+    d = x[c2[z.n_bnd:]] - x[c1[z.n_bnd:]]
+    assert np.all(np.einsum("ij,ij->i", d, nrm[z.n_bnd:]) > 0)
+
+
+def test_tet_volume_closed_form():
+    one, two = tet_cases()
+    o = Oracle(one, params())
+    assert o.dbls("CELL_VOL")[0] == pytest.approx(1.0 / 6.0, rel=1e-15)
+    assert np.allclose(o.dbls("CELL_X"), [0.25, 0.25, 0.25], atol=1e-15)
+    o2 = Oracle(two, params())
+    # second tet (1,0,0),(0,1,0),(0,0,1),(1,1,1): |det| / 6 = 2 / 6
+    assert o2.dbls("CELL_VOL").tolist() == pytest.approx([1.0 / 6.0, 2.0 / 6.0], rel=1e-14)
+
+
+@pytest.mark.parametrize("case,vol,tol", [
+    (lambda: s.cavity(4), 1.0, 1e-13),
+    (lambda: s.cavity(4, jitter=0.2, seed=7, tets=True), 1.0, 1e-13),
+    (lambda: s.step(2, upstream=2, downstream=3, span=1, jitter=0.1), 2 * 1 + 3 * 2, 1e-13),
+    # warped hex faces: the 6-tet volume formula (geometry.cpp:432-438) implies a
+    # face diagonal per cell, and neighbours disagree, so volumes tile the box
+    # only to second order in the node jitter
+    (lambda: s.cavity(5, jitter=0.2, seed=7), 1.0, 5e-3),
+    (lambda: s.channel(6, 3, 5, lx=2, ly=1, lz=2, jitter=0.15, split=1.0, seed=9), 4.0, 1e-13),
+    (lambda: s.channel(6, 3, 5, lx=2, ly=1, lz=2, jitter=0.15, split=0.25, seed=9), 4.0, 5e-3),
+])
+def test_volume_sum_and_closed_cells(case, vol, tol):
+    o = Oracle(case(), params())
+    z = o.sizes
+    assert o.dbls("CELL_VOL").sum() == pytest.approx(vol, rel=tol)
+    # sum of outward face vector areas vanishes for every closed cell
+    nrm = o.dbls("FACE_N").reshape(-1, 3)
+    c1 = o.ints("FACE_C1")
+    acc = np.zeros((z.n_cell, 3))
+    np.add.at(acc, c1, nrm)
+    np.subtract.at(acc, o.ints("FACE_C2")[z.n_bnd:], nrm[z.n_bnd:])
+    assert np.max(np.abs(acc)) < 1e-13
+
+
+def test_affine_map_scales_volumes():
+    case = s.cavity(3, jitter=0.1, seed=4)
+    a = Oracle(case, params()).dbls("CELL_VOL")
+    M = np.array([[2.0, 0.3, 0.0], [0.0, 1.5, 0.2], [0.1, 0.0, 0.5]])
+    c2 = s.CaseData((case.xyz.reshape(-1, 3) @ M.T).ravel(), case.etype, case.tag, case.vptr, case.vidx,
+                    case.regions)
+    b = Oracle(c2, params()).dbls("CELL_VOL")
+    assert np.allclose(b, a * abs(np.linalg.det(M)), rtol=1e-12)
+
+
+# ---------------------------------------------------------------- gradient
+def test_gradient_exact_for_linear_field():
+    o = Oracle(s.cavity(5), params())
+    z = o.sizes
+    g = np.array([0.7, -1.3, 2.1])
+    x = o.dbls("CELL_X").reshape(-1, 3)
+    fx = o.dbls("FACE_X").reshape(-1, 3)
+    o.set("P", x @ g + 0.4)
+    o.set("BPRE", fx[: z.n_bnd] @ g + 0.4)
+    o.gradient("P", "BPRE", "DPDX")
+    assert np.allclose(o.get("DPDX").reshape(-1, 3), g, rtol=0, atol=1e-12)
+
+
+def test_gradient_telescoping_identity_jittered():
+    case = s.channel(5, 4, 4, jitter=0.15, split=0.2, seed=3)
+    o = Oracle(case, params())
+    z = o.sizes
+    rng = np.random.default_rng(1)
+    phi, bphi = rng.normal(size=z.n_cell), rng.normal(size=z.n_bnd)
+    o.set("U", phi)
+    o.set("BU", bphi)
+    o.gradient("U", "BU", "DUDX")
+    lhs = (o.dbls("CELL_VOL")[:, None] * o.get("DUDX").reshape(-1, 3)).sum(0)
+    rhs = (bphi[:, None] * o.dbls("FACE_N").reshape(-1, 3)[: z.n_bnd]).sum(0)
+    assert np.allclose(lhs, rhs, atol=1e-12)
+
+
+# ---------------------------------------------------------------- assembly
+def _dense(o, name):
+    z = o.sizes
+    rp, col = o.ints("CSR_ROWPTR"), o.ints("CSR_COL")
+    v = o.get(name)
+    A = np.zeros((z.n_cell, z.n_cell))
+    for i in range(z.n_cell):
+        A[i, col[rp[i]:rp[i + 1]]] = v[rp[i]:rp[i + 1]]
+    return A
+
+
+def test_momentum_first_iteration_closed_form():
+    n, re = 4, 100.0
+    h, mu, urf = 1.0 / n, 1.0 / re, 0.6
+    o = Oracle(s.cavity(n, re=re), params())
+    o.call("build_velocity_matrix")
+    A = _dense(o, "AU")
+    # cell i: interior faces -mu h off-diagonal; a_P = (k_int mu h + k_bnd 2 mu h) / URF
+    nface_int = (np.abs(A) > 0).sum(1) - 1
+    off = A - np.diag(np.diag(A))
+    assert np.allclose(off[off != 0], -mu * h, rtol=1e-12)
+    ap = (nface_int * mu * h + (6 - nface_int) * 2 * mu * h) / urf
+    assert np.allclose(np.diag(A), ap, rtol=1e-12)
+    assert np.allclose(o.get("APR"), 1.0 / ap, rtol=1e-12)
+    # at rest u varies along z only (lid at z = 1, moving along x): nothing
+    # couples into v; the transposed stress (du/dz n_x) reaches w only at the
+    # side walls x = 0, 1, where the two x-faces of a cell do not cancel
+    assert np.all(o.get("BV_RHS") == 0)
+    x = o.dbls("CELL_X").reshape(-1, 3)
+    interior_x = (x[:, 0] > h) & (x[:, 0] < 1 - h)
+    assert np.all(o.get("BW_RHS")[interior_x] == 0)
+    assert np.any(o.get("BW_RHS")[~interior_x] != 0)
+
+
+def test_momentum_m_matrix_and_pressure_structure():
+    case = s.cavity(4, jitter=0.1, seed=1)
+    o = Oracle(case, params())
+    z = o.sizes
+    push(o, {k: v for k, v in random_state(z.n_cell, z.n_bnd, z.n_face).items()
+             if k in ("U", "V", "W", "P", "RUFACE")})
+    o.set("R", np.ones(z.n_cell))
+    o.call("build_velocity_matrix")
+    A = _dense(o, "AU")
+    d = np.diag(A)
+    assert np.all(d > 0)
+    off = A - np.diag(d)
+    assert np.all(off <= 0)
+    assert np.all(d >= np.abs(off).sum(1) * (1 - 1e-12))
+    o.call("cal_ru_face")
+    o.call("build_pressure_matrix")
+    Ap = _dense(o, "AP")
+    assert np.max(np.abs(Ap.sum(1))) <= 1e-14 * np.max(np.abs(Ap))
+    assert np.max(np.abs(Ap - Ap.T)) <= 1e-12 * np.max(np.abs(Ap))
+    assert np.all(np.diag(Ap) < 0)
+    # sum of the continuity right-hand side = net boundary flux
+    ru = o.get("RUFACE")
+    assert o.get("BP_RHS").sum() == pytest.approx(ru[: z.n_bnd].sum(), abs=1e-12)
+
+
+def test_rhie_chow_zero_for_uniform_u_and_linear_p():
+    o = Oracle(s.cavity(4), params())
+    z = o.sizes
+    x = o.dbls("CELL_X").reshape(-1, 3)
+    u0 = np.array([0.3, -0.2, 0.5])
+    g = np.array([1.0, 2.0, -0.5])
+    for k, n in enumerate("UVW"):
+        o.set(n, np.full(z.n_cell, u0[k]))
+    o.set("R", np.ones(z.n_cell))
+    o.set("P", x @ g)
+    o.set("DPDX", np.tile(g, z.n_cell))
+    o.set("APR", np.random.default_rng(0).uniform(0.5, 2, z.n_cell))
+    o.call("cal_ru_face")
+    nrm = o.dbls("FACE_N").reshape(-1, 3)
+    ru = o.get("RUFACE")[z.n_bnd:]
+    assert np.allclose(ru, nrm[z.n_bnd:] @ u0, atol=1e-14)
+
+
+def test_continuity_after_flux_correction_equals_solve_residual():
+    o = Oracle(s.cavity(4, re=100), params(rtol=1e-4))
+    z = o.sizes
+    st = random_state(z.n_cell, z.n_bnd, z.n_face, seed=3)
+    push(o, {k: st[k] for k in ("U", "V", "W", "P")})
+    o.set("R", np.ones(z.n_cell))
+    o.call("build_velocity_matrix")
+    o.call("bc_velocity")
+    o.call("cal_ru_face")
+    o.call("build_pressure_matrix")
+    Ap, bp = _dense(o, "AP"), o.get("BP_RHS")
+    o.solve_pressure(1e-4, 1000)
+    dp = o.get("DELTAP")
+    o.correct_fields()
+    o.call("bc_velocity")
+    o.call("correct_ru_face")
+    ru = o.get("RUFACE")
+    c1, c2 = o.ints("FACE_C1"), o.ints("FACE_C2")
+    div = np.zeros(z.n_cell)
+    np.add.at(div, c1, ru)
+    np.subtract.at(div, c2[z.n_bnd:], ru[z.n_bnd:])
+    assert np.allclose(div, bp - Ap @ dp, atol=1e-13)
+
+
+# ---------------------------------------------------------------- Krylov
+def _set_matrix(o, A):
+    rp, col = o.ints("CSR_ROWPTR"), o.ints("CSR_COL")
+    v = np.concatenate([A[i, col[rp[i]:rp[i + 1]]] for i in range(A.shape[0])])
+    o.set("AU", v)
+
+
+@pytest.mark.parametrize("pc", [s.PC_NONE, s.PC_JACOBI, s.PC_ILU0])
+def test_gmres_matches_dense_solve(pc):
+    o = Oracle(s.cavity(4, jitter=0.1, seed=2), params(restart=30, pc=pc))
+    z = o.sizes
+    rng = np.random.default_rng(5)
+    push(o, {k: v for k, v in random_state(z.n_cell, z.n_bnd, z.n_face, 2).items() if k in ("U", "V", "W", "P")})
+    o.set("R", np.ones(z.n_cell))
+    o.call("build_velocity_matrix")
+    A = _dense(o, "AU")
+    b = rng.normal(size=(3, z.n_cell))
+    for k, n in enumerate(("BU_RHS", "BV_RHS", "BW_RHS")):
+        o.set(n, b[k])
+    info = o.solve_velocity(1e-12, 1000)
+    assert all(i["reason"] > 0 for i in info)
+    for k, n in enumerate("UVW"):
+        assert rel_err(o.get(n), np.linalg.solve(A, b[k])) < 1e-9
+
+
+def test_gmres_unrestarted_terminates_within_n():
+    o = Oracle(s.cavity(3, jitter=0.1, seed=2), params(restart=64, pc=s.PC_NONE))
+    z = o.sizes
+    rng = np.random.default_rng(0)
+    A = _dense(o, "AU") * 0
+    for i in range(z.n_cell):
+        A[i, i] = 4.0
+    rp, col = o.ints("CSR_ROWPTR"), o.ints("CSR_COL")
+    for i in range(z.n_cell):
+        for j in col[rp[i]:rp[i + 1]]:
+            if j != i:
+                A[i, j] = rng.uniform(-1, 1)
+    _set_matrix(o, A)
+    for n in ("BU_RHS", "BV_RHS", "BW_RHS"):
+        o.set(n, rng.normal(size=z.n_cell))
+    info = o.solve_velocity(1e-13, 1000)
+    assert all(0 < i["iters"] <= z.n_cell for i in info)
+
+
+def test_ilu0_is_exact_on_tridiagonal():
+    o = Oracle(s.channel(12, 1, 1, lx=12, ly=1, lz=1), params(pc=s.PC_ILU0))
+    z = o.sizes
+    rng = np.random.default_rng(1)
+    A = np.zeros((z.n_cell, z.n_cell))
+    rp, col = o.ints("CSR_ROWPTR"), o.ints("CSR_COL")
+    for i in range(z.n_cell):
+        for j in col[rp[i]:rp[i + 1]]:
+            A[i, j] = 5.0 if i == j else rng.uniform(-2, 2)
+    _set_matrix(o, A)
+    for n in ("BU_RHS", "BV_RHS", "BW_RHS"):
+        o.set(n, rng.normal(size=z.n_cell))
+    info = o.solve_velocity(1e-12, 100)
+    assert all(i["iters"] == 1 for i in info)
+
+
+def test_gmres_reports_divergence_at_the_cap():
+    o = Oracle(s.cavity(4, jitter=0.1, seed=2), params(restart=5, pc=s.PC_NONE))
+    z = o.sizes
+    push(o, {k: v for k, v in random_state(z.n_cell, z.n_bnd, z.n_face, 2).items() if k in ("U", "V", "W", "P")})
+    o.set("R", np.ones(z.n_cell))
+    o.call("build_velocity_matrix")
+    for n in ("BU_RHS", "BV_RHS", "BW_RHS"):
+        o.set(n, np.ones(z.n_cell))
+    info = o.solve_velocity(1e-14, 2)
+    assert info[0]["reason"] == -3 and info[0]["iters"] == 2
+
+
+# ---------------------------------------------------------------- k-epsilon
+def test_kepsilon_wall_function_and_mut():
+    case = s.channel(1, 1, 1, lx=1, ly=1, lz=1, turbulent=True, re=1000)
+    o = Oracle(case, params(turb=1, rtol=1e-13))
+    z = o.sizes
+    k, eps, rho, mu = 0.02, 0.05, 1.0, 1e-3
+    o.set("TE", [k])
+    o.set("ED", [eps])
+    o.set("R", [rho])
+    o.set("VISLAM", [mu])
+    o.set("VISTUR", [0.0])
+    for n in ("U", "V", "W"):
+        o.set(n, [0.3])
+    o.call("update_tur_kepsilon", (__import__("sysu_cfd_parallel_amd")._abi.SolveInfo * 2)())
+    d = 0.5  # wall distance of the only cell
+    e = o.get("ED")[0]
+    kk = o.get("TE")[0]
+    # the eps row of a wall cell is an identity row holding eps_w(k_old)
+    assert e == pytest.approx(0.09 ** 0.75 * k ** 1.5 / (0.419 * d), rel=1e-10)
+    assert o.get("VISTUR")[0] == pytest.approx(0.09 * rho * kk * kk / (e + 1e-16), rel=1e-14)
+    assert kk >= 1e-9 and e >= 1e-12
+
+
+# ---------------------------------------------------------------- end to end
+def test_cavity_converges_and_is_physical():
+    o = Oracle(s.cavity(12, re=100), params(rtol=1e-8))
+    hist, done, rc = o.iterate(150)
+    # steady stop once ResMax < ResidualSteady = 1e-6 (navier.cpp:136-138)
+    assert rc == 0 and done < 150
+    assert hist[-1, 10] < 1e-6 <= hist[-2, 10]
+    U = o.get("U")
+    assert -0.26 < U.min() < -0.15
