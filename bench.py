@@ -113,15 +113,22 @@ def main() -> int:
     z = dev.sizes
     log(f"device context: {z.n_cell} cells, {z.n_face} faces, nnz {z.nnz} ({time.perf_counter() - t:.1f}s)")
 
-    log(f"warmup {args.warmup} step(s)")
-    dev.iterate(args.warmup)
+    for k in range(args.warmup):
+        t = time.perf_counter()
+        h, _ = dev.iterate(1)
+        log(f"warmup step {k + 1}/{args.warmup}: {time.perf_counter() - t:.2f}s, ResMax {h[-1, 10]:.3e}")
     dev.call("synchronize")
     dev.stage_times(reset=True)
-    log(f"timing {args.steps} step(s)")
+    rows = []
     t0 = time.perf_counter()
-    hist, done = dev.iterate(args.steps)
+    for k in range(args.steps):
+        t = time.perf_counter()
+        h, _ = dev.iterate(1)
+        rows.append(h[-1])
+        log(f"step {k + 1}/{args.steps}: {time.perf_counter() - t:.2f}s, ResMax {h[-1, 10]:.3e}")
     dev.call("synchronize")
     t1 = time.perf_counter()
+    hist = np.array(rows)
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     stages = dev.stage_times()
     value = args.steps / (t1 - t0)

@@ -80,8 +80,8 @@ void sizes_of(const s355::Mesh& m, s355_sizes* s) {
 }
 
 int mesh_int(const s355::Mesh& m, int which, int* out, size_t n) {
-  const std::vector<int>* v = nullptr;
-  std::vector<int> tmp;
+  const s355::hvec<int>* v = nullptr;
+  s355::hvec<int> tmp;
   switch (which) {
     case S355_M_ELEM_PERM: v = &m.perm; break;
     case S355_M_FACE_C1: v = &m.f_c1; break;
@@ -110,7 +110,7 @@ int mesh_int(const s355::Mesh& m, int which, int* out, size_t n) {
 }
 
 int mesh_dbl(const s355::Mesh& m, int which, double* out, size_t n) {
-  const std::vector<double>* v = nullptr;
+  const s355::hvec<double>* v = nullptr;
   switch (which) {
     case S355_G_CELL_VOL: v = &m.c_vol; break;
     case S355_G_CELL_X: v = &m.c_x; break;

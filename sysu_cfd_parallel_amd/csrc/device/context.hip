@@ -26,13 +26,14 @@ T* Context::alloc(size_t n) {
 }
 
 namespace {
-template <class T>
-T* upload(std::vector<void*>& allocs, const std::vector<T>& v, hipStream_t s) {
+template <class Vec>
+auto upload(std::vector<void*>& allocs, const Vec& v, hipStream_t s) {
+  using T = typename Vec::value_type;
   void* p = nullptr;
   S355_HIP(hipMalloc(&p, std::max<size_t>(v.size(), 1) * sizeof(T)));
   allocs.push_back(p);
   if (!v.empty()) S355_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
-  return static_cast<T*>(p);
+  return static_cast<const T*>(p);
 }
 }  // namespace
 

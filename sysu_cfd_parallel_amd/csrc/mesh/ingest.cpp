@@ -18,7 +18,10 @@
 // quads instead of exiting.
 #include "ingest.hpp"
 
+#include "parallel.hpp"
+
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -30,54 +33,6 @@ namespace s355 {
 namespace {
 
 const int kPattern[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4}, {1, 2, 6, 5}, {3, 2, 6, 7}, {0, 3, 7, 4}};
-
-// Open-addressing table keyed on up to four sorted vertex ids.
-class KeyTable {
- public:
-  explicit KeyTable(size_t expected) {
-    size_t cap = 16;
-    while (cap < 2 * expected + 16) cap <<= 1;
-    mask_ = cap - 1;
-    keys_.assign(4 * cap, -1);
-    vals_.assign(cap, -1);
-  }
-  // returns existing value or inserts `val` and returns -1
-  int find_or_insert(const int* k, int val) {
-    size_t h = hash(k) & mask_;
-    for (;;) {
-      int* s = &keys_[4 * h];
-      if (vals_[h] < 0) {
-        std::memcpy(s, k, 4 * sizeof(int));
-        vals_[h] = val;
-        return -1;
-      }
-      if (s[0] == k[0] && s[1] == k[1] && s[2] == k[2] && s[3] == k[3]) return vals_[h];
-      h = (h + 1) & mask_;
-    }
-  }
-  bool contains(const int* k) const {
-    size_t h = hash(k) & mask_;
-    for (;;) {
-      const int* s = &keys_[4 * h];
-      if (vals_[h] < 0) return false;
-      if (s[0] == k[0] && s[1] == k[1] && s[2] == k[2] && s[3] == k[3]) return true;
-      h = (h + 1) & mask_;
-    }
-  }
-
- private:
-  static size_t hash(const int* k) {
-    uint64_t h = 0x9E3779B97F4A7C15ull;
-    for (int i = 0; i < 4; ++i) {
-      h ^= static_cast<uint32_t>(k[i]) + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
-      h *= 0xBF58476D1CE4E5B9ull;
-    }
-    return static_cast<size_t>(h ^ (h >> 31));
-  }
-  size_t mask_;
-  std::vector<int> keys_;
-  std::vector<int> vals_;
-};
 
 // sorted distinct vertex key; returns the number of distinct ids
 int make_key(int a, int b, int c, int d, int* k) {
@@ -94,6 +49,43 @@ struct InElem {
   int pid;
   int idx;
 };
+
+struct TriKey {
+  int a, b, c;
+};
+
+struct HalfFace {
+  uint64_t k0, k1;  // sorted distinct vertex ids, -1 padded
+  int seq;          // discovery sequence; boundary records are negative
+  int pad;
+};
+
+// Chunked parallel sort: sort T chunks concurrently, then merge pairs of runs
+// in parallel rounds.
+template <class Vec, class Cmp>
+void parallel_sort(Vec& v, Cmp cmp) {
+  const size_t n = v.size();
+  int nt = host_threads();
+  if (n < (1u << 16) || nt < 2) {
+    std::sort(v.begin(), v.end(), cmp);
+    return;
+  }
+  int runs = 1;
+  while (runs < nt) runs <<= 1;
+  std::vector<size_t> cut(runs + 1);
+  for (int r = 0; r <= runs; ++r) cut[r] = n * r / runs;
+  parallel_for(runs, [&](long r) { std::sort(v.begin() + cut[r], v.begin() + cut[r + 1], cmp); }, 1);
+  Vec tmp(n);
+  while (runs > 1) {
+    parallel_for(runs / 2, [&](long q) {
+      const size_t a = cut[2 * q], b = cut[2 * q + 1], c = cut[2 * q + 2];
+      std::merge(v.begin() + a, v.begin() + b, v.begin() + b, v.begin() + c, tmp.begin() + a, cmp);
+    }, 1);
+    v.swap(tmp);
+    for (int r = 0; r <= runs / 2; ++r) cut[r] = cut[2 * r];
+    runs /= 2;
+  }
+}
 struct ByPart {
   bool operator()(const InElem* l, const InElem* r) const { return l->pid < r->pid ? true : false; }
 };
@@ -238,117 +230,233 @@ Mesh ingest_mesh(const s355_mesh_desc& d) {
   clk.mark("cells/boundary");
   const int Nc = m.n_cell, Nb = m.n_bnd, K = kMaxFace;
   // ---- faces
-  KeyTable table(static_cast<size_t>(Nb) + 4 * static_cast<size_t>(Nc));
-  m.c_nface.assign(Nc, 0);
-  m.c_face.assign(static_cast<size_t>(Nc) * K, -1);
-  m.c_cell.assign(static_cast<size_t>(Nc) * K, -1);
-  m.c_poly.assign(Nc, 0);
-  m.f_c1.reserve(Nb + 3 * static_cast<size_t>(Nc));
-  auto new_face = [&](const int* q, int c1, int c2, int b) {
-    m.f_v.insert(m.f_v.end(), q, q + 4);
-    m.f_c1.push_back(c1);
-    m.f_c2.push_back(c2);
-    m.f_bnd.push_back(b);
-    return static_cast<int>(m.f_c1.size()) - 1;
-  };
-  for (int i = 0; i < Nb; ++i) {
-    int key[4];
-    make_key(bv[4 * i], bv[4 * i + 1], bv[4 * i + 2], bv[4 * i + 3], key);
-    int f = new_face(&bv[4 * static_cast<size_t>(i)], -9999999, kVoidBoundary, i);
-    if (table.find_or_insert(key, f) >= 0) throw std::runtime_error("mesh: duplicate boundary element");
-  }
-  clk.mark("boundary faces");
+  // Half-faces in discovery order (cells in order, each in pattern order,
+  // split quads expanded, degenerate faces skipped), boundary records first.
+  // Sorting (vertex key, sequence) puts the two sides of every face next to
+  // each other; an interior face takes the rank of its first (creating)
+  // half-face, which is exactly the reference's sequential numbering.
+  m.c_poly.resize(Nc);
+  parallel_for(Nc, [&](long i) { m.c_poly[i] = 0; });
   // triangles exposed by cells, for the split-quad extension
-  bool any_tri = false, any_quad = false;
-  for (int i = 0; i < Nc && !(any_tri && any_quad); ++i)
-    for (const auto& p : kPattern) {
+  std::vector<uint8_t> tri_vertex(m.n_vert, 0);
+  std::vector<TriKey> tris;
+  {
+    std::vector<int> tcount(Nc + 1, 0);
+    parallel_for(Nc, [&](long i) {
       const int* h = &cv[8 * static_cast<size_t>(i)];
-      int key[4];
-      int n = make_key(h[p[0]], h[p[1]], h[p[2]], h[p[3]], key);
-      any_tri |= n == 3;
-      any_quad |= n == 4;
-    }
-  KeyTable tris(any_tri && any_quad ? 4 * static_cast<size_t>(Nc) : 1);
-  if (any_tri && any_quad)
-    for (int i = 0; i < Nc; ++i)
+      int c = 0;
       for (const auto& p : kPattern) {
-        const int* h = &cv[8 * static_cast<size_t>(i)];
         int key[4];
-        if (make_key(h[p[0]], h[p[1]], h[p[2]], h[p[3]], key) == 3) tris.find_or_insert(key, 1);
+        if (make_key(h[p[0]], h[p[1]], h[p[2]], h[p[3]], key) == 3) ++c;
       }
-  clk.mark("triangle table");
+      tcount[i + 1] = c;
+    });
+    for (int i = 0; i < Nc; ++i) tcount[i + 1] += tcount[i];
+    bool any_quad = false;
+    for (int i = 0; i < Nc && !any_quad; ++i) {
+      const int* h = &cv[8 * static_cast<size_t>(i)];
+      for (const auto& p : kPattern) {
+        int key[4];
+        any_quad |= make_key(h[p[0]], h[p[1]], h[p[2]], h[p[3]], key) == 4;
+      }
+    }
+    if (tcount[Nc] > 0 && any_quad) {
+      tris.resize(tcount[Nc]);
+      parallel_for(Nc, [&](long i) {
+        const int* h = &cv[8 * static_cast<size_t>(i)];
+        int o = tcount[i];
+        for (const auto& p : kPattern) {
+          int key[4];
+          if (make_key(h[p[0]], h[p[1]], h[p[2]], h[p[3]], key) == 3) tris[o++] = TriKey{key[0], key[1], key[2]};
+        }
+      });
+      parallel_sort(tris, [](const TriKey& a, const TriKey& b) {
+        return a.a != b.a ? a.a < b.a : (a.b != b.b ? a.b < b.b : a.c < b.c);
+      });
+      tris.erase(std::unique(tris.begin(), tris.end(),
+                             [](const TriKey& a, const TriKey& b) { return a.a == b.a && a.b == b.b && a.c == b.c; }),
+                 tris.end());
+      for (const TriKey& t : tris) tri_vertex[t.a] = tri_vertex[t.b] = tri_vertex[t.c] = 1;
+    }
+  }
+  clk.mark("triangle set");
   auto has_tri = [&](int a, int b, int c) {
     int key[4];
     make_key(a, b, c, c, key);
-    return tris.contains(key);
+    TriKey t{key[0], key[1], key[2]};
+    return std::binary_search(tris.begin(), tris.end(), t, [](const TriKey& x, const TriKey& y) {
+      return x.a != y.a ? x.a < y.a : (x.b != y.b ? x.b < y.b : x.c < y.c);
+    });
   };
-  auto attach = [&](int ic, int n1, int n2, int n3, int n4) {
-    int irepeat = 0;
-    if (n1 == n2 || n1 == n3 || n1 == n4) irepeat++;
-    if (n2 == n3 || n2 == n4) irepeat++;
-    if (n3 == n4) irepeat++;
-    if (irepeat > 1) return;
-    int key[4];
-    make_key(n1, n2, n3, n4, key);
-    int nf = static_cast<int>(m.f_c1.size());
-    int fid = table.find_or_insert(key, nf);
-    if (m.c_nface[ic] >= K) throw std::runtime_error("mesh: cell with more than 12 faces");
-    if (fid < 0) {
-      int q[4] = {n1, n2, n3, n4};
-      new_face(q, ic, -20, -10);
-      fid = nf;
-    } else if (m.f_bnd[fid] >= 0) {
-      m.f_c1[fid] = ic;
-    } else {
-      m.f_c2[fid] = ic;
-    }
-    m.c_face[static_cast<size_t>(ic) * K + m.c_nface[ic]++] = fid;
-  };
-  for (int i = 0; i < Nc; ++i) {
+  // expand cell i's faces; emit(n1, n2, n3, n4) per non-degenerate face
+  auto expand = [&](long i, auto&& emit) {
     const int* h = &cv[8 * static_cast<size_t>(i)];
+    auto put = [&](int n1, int n2, int n3, int n4) {
+      int irepeat = 0;
+      if (n1 == n2 || n1 == n3 || n1 == n4) irepeat++;
+      if (n2 == n3 || n2 == n4) irepeat++;
+      if (n3 == n4) irepeat++;
+      if (irepeat > 1) return;
+      emit(n1, n2, n3, n4);
+    };
     for (const auto& p : kPattern) {
       int n1 = h[p[0]], n2 = h[p[1]], n3 = h[p[2]], n4 = h[p[3]];
       bool quad = n1 != n2 && n1 != n3 && n1 != n4 && n2 != n3 && n2 != n4 && n3 != n4;
-      if (quad && any_tri) {
+      if (quad && !tris.empty() && tri_vertex[n1] && tri_vertex[n2] && tri_vertex[n3] && tri_vertex[n4]) {
         if (has_tri(n1, n2, n3) && has_tri(n1, n3, n4)) {
           m.c_poly[i] = 1;
-          attach(i, n1, n2, n3, n3);
-          attach(i, n1, n3, n4, n4);
+          put(n1, n2, n3, n3);
+          put(n1, n3, n4, n4);
           continue;
         }
         if (has_tri(n1, n2, n4) && has_tri(n2, n3, n4)) {
           m.c_poly[i] = 1;
-          attach(i, n1, n2, n4, n4);
-          attach(i, n2, n3, n4, n4);
+          put(n1, n2, n4, n4);
+          put(n2, n3, n4, n4);
           continue;
         }
       }
-      attach(i, n1, n2, n3, n4);
+      put(n1, n2, n3, n4);
+    }
+  };
+  m.c_nface.resize(Nc);
+  m.cf_ptr.resize(Nc + 1);
+  m.cf_ptr[0] = 0;
+  parallel_for(Nc, [&](long i) {
+    int c = 0;
+    expand(i, [&](int, int, int, int) { ++c; });
+    m.c_nface[i] = c;
+  });
+  for (int i = 0; i < Nc; ++i) {
+    if (m.c_nface[i] > K) throw std::runtime_error("mesh: cell with more than 12 faces");
+    m.cf_ptr[i + 1] = m.cf_ptr[i] + m.c_nface[i];
+  }
+  const long H = m.cf_ptr[Nc];
+  hvec<int> hv(4 * static_cast<size_t>(H));   // vertex order of each half-face
+  hvec<int> hcell(H);                          // owning cell
+  hvec<HalfFace> rec(static_cast<size_t>(Nb) + H);
+  auto fill_key = [](HalfFace& r, int a, int b, int c, int d, int seq) {
+    int key[4];
+    make_key(a, b, c, d, key);
+    r.k0 = (static_cast<uint64_t>(static_cast<uint32_t>(key[0])) << 32) | static_cast<uint32_t>(key[1]);
+    r.k1 = (static_cast<uint64_t>(static_cast<uint32_t>(key[2])) << 32) | static_cast<uint32_t>(key[3]);
+    r.seq = seq;
+  };
+  parallel_for(Nb, [&](long b) {
+    const int* q = &bv[4 * static_cast<size_t>(b)];
+    fill_key(rec[b], q[0], q[1], q[2], q[3], static_cast<int>(b - Nb));
+  });
+  parallel_for(Nc, [&](long i) {
+    long e = m.cf_ptr[i];
+    expand(i, [&](int n1, int n2, int n3, int n4) {
+      int* o = &hv[4 * static_cast<size_t>(e)];
+      o[0] = n1;
+      o[1] = n2;
+      o[2] = n3;
+      o[3] = n4;
+      hcell[e] = static_cast<int>(i);
+      fill_key(rec[Nb + e], n1, n2, n3, n4, static_cast<int>(e));
+      ++e;
+    });
+  });
+  clk.mark("half-faces");
+  parallel_sort(rec, [](const HalfFace& a, const HalfFace& b) {
+    return a.k0 != b.k0 ? a.k0 < b.k0 : (a.k1 != b.k1 ? a.k1 < b.k1 : a.seq < b.seq);
+  });
+  clk.mark("half-face sort");
+  // pair the sides: partner[e1] = e2 for interior faces created by e1; bface[e] = b for boundary
+  hvec<int> partner(H), face_of(H);
+  hvec<uint8_t> creator(H);
+  parallel_for(H, [&](long e) { creator[e] = 0; });
+  std::vector<int> bcell(Nb, -1);
+  {
+    const size_t M = rec.size();
+    size_t s0 = 0;
+    while (s0 < M) {
+      size_t t = s0 + 1;
+      while (t < M && rec[t].k0 == rec[s0].k0 && rec[t].k1 == rec[s0].k1) ++t;
+      const HalfFace& a = rec[s0];
+      if (t - s0 > 2) throw std::runtime_error("mesh: a face is shared by more than two elements");
+      if (t - s0 == 1) {
+        if (a.seq < 0)
+          throw std::runtime_error("mesh: boundary element " + std::to_string(a.seq + Nb) + " has no cell");
+        throw std::runtime_error("mesh: unmatched face of cell " + std::to_string(hcell[a.seq]) +
+                                 "; the mesh is not conforming");
+      }
+      const HalfFace& b = rec[s0 + 1];
+      if (a.seq < 0) {
+        if (b.seq < 0) throw std::runtime_error("mesh: duplicate boundary element");
+        face_of[b.seq] = a.seq + Nb;
+        bcell[a.seq + Nb] = hcell[b.seq];
+      } else {
+        creator[a.seq] = 1;
+        partner[a.seq] = b.seq;
+      }
+      s0 = t;
     }
   }
-  clk.mark("interior faces");
-  const int Nf = static_cast<int>(m.f_c1.size());
-  m.n_face = Nf;
-  for (int f = 0; f < Nf; ++f) {
-    if (m.f_c2[f] == kVoidBoundary || m.f_c2[f] >= 0) continue;
-    throw std::runtime_error("mesh: unmatched face " + std::to_string(f) + " (cell " + std::to_string(m.f_c1[f]) +
-                             "); the mesh is not conforming");
+  hvec<HalfFace>().swap(rec);
+  int Nf = Nb;
+  for (long e = 0; e < H; ++e)
+    if (creator[e]) face_of[e] = Nf++;
+  m.f_c1.resize(Nf);
+  m.f_c2.resize(Nf);
+  m.f_bnd.resize(Nf);
+  m.f_v.resize(4 * static_cast<size_t>(Nf));
+  for (int b = 0; b < Nb; ++b) {
+    m.f_c1[b] = bcell[b];
+    m.f_c2[b] = kVoidBoundary;
+    m.f_bnd[b] = b;
+    for (int k = 0; k < 4; ++k) m.f_v[4 * static_cast<size_t>(b) + k] = bv[4 * static_cast<size_t>(b) + k];
   }
-  for (int b = 0; b < Nb; ++b)
-    if (m.f_c1[b] < 0) throw std::runtime_error("mesh: boundary element " + std::to_string(b) + " has no cell");
-  clk.mark("face check");
+  parallel_for(H, [&](long e) {
+    if (!creator[e]) return;
+    const int f = face_of[e], e2 = partner[e];
+    face_of[e2] = f;
+    m.f_c1[f] = hcell[e];
+    m.f_c2[f] = hcell[e2];
+    m.f_bnd[f] = -10;
+    for (int k = 0; k < 4; ++k) m.f_v[4 * static_cast<size_t>(f) + k] = hv[4 * static_cast<size_t>(e) + k];
+  });
+  m.c_face.resize(static_cast<size_t>(Nc) * K);
+  m.cf_pat.resize(H);
+  parallel_for(Nc, [&](long i) {
+    for (int j = 0; j < K; ++j) {
+      const int f = j < m.c_nface[i] ? face_of[m.cf_ptr[i] + j] : -1;
+      m.c_face[static_cast<size_t>(i) * K + j] = f;
+      if (j < m.c_nface[i]) m.cf_pat[m.cf_ptr[i] + j] = f;
+    }
+  });
+  m.n_face = Nf;
+  clk.mark("face numbering");
   // ---- cell metrics
   const double* X = m.vert.data();
   auto P = [&](int v) { return X + 3 * static_cast<size_t>(v); };
-  m.c_vol.assign(Nc, 0.0);
-  m.c_x.assign(3 * static_cast<size_t>(Nc), 0.0);
-  for (int i = 0; i < Nc; ++i) {
+  m.c_vol.resize(Nc);
+  m.c_x.resize(3 * static_cast<size_t>(Nc));
+  parallel_for(Nc, [&](long i) {
     const int* h = &cv[8 * static_cast<size_t>(i)];
     double* xc = &m.c_x[3 * static_cast<size_t>(i)];
     for (int j = 0; j < 3; ++j)
       xc[j] = 1. / 8. * (P(h[0])[j] + P(h[1])[j] + P(h[2])[j] + P(h[3])[j] + P(h[4])[j] + P(h[5])[j] +
                          P(h[6])[j] + P(h[7])[j]);
-    if (m.c_poly[i]) continue;
+    if (m.c_poly[i]) {
+      // tets on the face triangles about the vertex mean
+      double xp[3] = {xc[0], xc[1], xc[2]};
+      double vs = 0., acc[3] = {0., 0., 0.};
+      for (int j = 0; j < m.c_nface[i]; ++j) {
+        const int* q = &m.f_v[4 * static_cast<size_t>(m.c_face[static_cast<size_t>(i) * K + j])];
+        const int tri[2][3] = {{q[0], q[1], q[3]}, {q[1], q[2], q[3]}};
+        for (const auto& t : tri) {
+          double v = tet_vol(xp, P(t[0]), P(t[1]), P(t[2]));
+          vs += v;
+          for (int k = 0; k < 3; ++k) acc[k] += v * 0.25 * (xp[k] + P(t[0])[k] + P(t[1])[k] + P(t[2])[k]);
+        }
+      }
+      m.c_vol[i] = vs;
+      for (int k = 0; k < 3; ++k) xc[k] = acc[k] / vs;
+      return;
+    }
     const double *x1 = P(h[0]), *x2 = P(h[1]), *x3 = P(h[2]), *x4 = P(h[3]), *x5 = P(h[4]), *x6 = P(h[5]),
                  *x7 = P(h[6]), *x8 = P(h[7]);
     double V1 = tet_vol(x1, x2, x4, x6), V2 = tet_vol(x1, x5, x4, x6), V3 = tet_vol(x4, x5, x8, x6);
@@ -366,142 +474,137 @@ Mesh ingest_mesh(const s355_mesh_desc& d) {
     double vs = V1 + V2 + V3 + V4 + V5 + V6;
     for (int j = 0; j < 3; ++j)
       xc[j] = 1. / vs * (V1 * c[0][j] + V2 * c[1][j] + V3 * c[2][j] + V4 * c[3][j] + V5 * c[4][j] + V6 * c[5][j]);
-  }
-  for (int i = 0; i < Nc; ++i) {
-    if (!m.c_poly[i]) continue;
-    double* xc = &m.c_x[3 * static_cast<size_t>(i)];
-    double xp[3] = {xc[0], xc[1], xc[2]};
-    double vs = 0., acc[3] = {0., 0., 0.};
-    for (int j = 0; j < m.c_nface[i]; ++j) {
-      const int* q = &m.f_v[4 * static_cast<size_t>(m.c_face[static_cast<size_t>(i) * K + j])];
-      const int tri[2][3] = {{q[0], q[1], q[3]}, {q[1], q[2], q[3]}};
-      for (const auto& t : tri) {
-        double v = tet_vol(xp, P(t[0]), P(t[1]), P(t[2]));
-        vs += v;
-        for (int k = 0; k < 3; ++k) acc[k] += v * 0.25 * (xp[k] + P(t[0])[k] + P(t[1])[k] + P(t[2])[k]);
+  });
+  m.c_cell.resize(static_cast<size_t>(Nc) * K);
+  parallel_for(Nc, [&](long i) {
+    for (int j = 0; j < K; ++j) {
+      int nc = -1;
+      if (j < m.c_nface[i]) {
+        int f = m.c_face[static_cast<size_t>(i) * K + j];
+        nc = m.f_c1[f] == i ? m.f_c2[f] : m.f_c1[f];
       }
-    }
-    m.c_vol[i] = vs;
-    for (int k = 0; k < 3; ++k) xc[k] = acc[k] / vs;
-  }
-  clk.mark("cell metrics");
-  for (int i = 0; i < Nc; ++i)
-    for (int j = 0; j < m.c_nface[i]; ++j) {
-      int f = m.c_face[static_cast<size_t>(i) * K + j];
-      int nc = m.f_c1[f] == i ? m.f_c2[f] : m.f_c1[f];
       m.c_cell[static_cast<size_t>(i) * K + j] = nc;
     }
-  clk.mark("neighbours");
+  });
+  clk.mark("cell metrics");
   // ---- face metrics
-  m.f_x.assign(3 * static_cast<size_t>(Nf), 0.0);
-  m.f_n.assign(3 * static_cast<size_t>(Nf), 0.0);
-  m.f_xpac.assign(3 * static_cast<size_t>(Nf), 0.0);
-  m.f_xnac.assign(3 * static_cast<size_t>(Nf), 0.0);
-  m.f_area.assign(Nf, 0.0);
-  m.f_lambda.assign(Nf, 0.0);
-  m.f_rlencos.assign(Nf, 0.0);
-  int warped = 0;
-  for (int f = 0; f < Nf; ++f) {
-    const int* q = &m.f_v[4 * static_cast<size_t>(f)];
-    const double *p1 = P(q[0]), *p2 = P(q[1]), *p3 = P(q[2]), *p4 = P(q[3]);
-    double a1 = tri_area(p1, p2, p4), a2 = tri_area(p2, p3, p4);
-    double* fx = &m.f_x[3 * static_cast<size_t>(f)];
-    double* fn = &m.f_n[3 * static_cast<size_t>(f)];
-    double xc1[3], xc2[3];
-    for (int j = 0; j < 3; ++j) {
-      xc1[j] = 1. / 3 * (p1[j] + p2[j] + p4[j]);
-      xc2[j] = 1. / 3 * (p2[j] + p3[j] + p4[j]);
-    }
-    for (int j = 0; j < 3; ++j) fx[j] = (a1 * xc1[j] + a2 * xc2[j]) / (a1 + a2);
-    double d1[3], d2[3], dx[3];
-    sub3(d1, p1, p3);
-    sub3(d2, p2, p4);
-    cross3(fn, d1, d2);
-    for (int j = 0; j < 3; ++j) fn[j] = 0.5 * fn[j];
-    const double* x1 = &m.c_x[3 * static_cast<size_t>(m.f_c1[f])];
-    sub3(dx, fx, x1);
-    if (dot3(fn, dx) < 0.)
-      for (int j = 0; j < 3; ++j) fn[j] = -fn[j];
-    double area = len3(fn);
-    m.f_area[f] = area;
-    if (std::fabs((a1 + a2 - area) / area) > 1.e-4) warped++;
-    int c2 = m.f_c2[f];
-    if (c2 >= 0) {
-      const double* x2 = &m.c_x[3 * static_cast<size_t>(c2)];
-      sub3(d1, fx, x1);
-      sub3(d2, fx, x2);
-      double r1 = len3(d1), r2 = len3(d2);
-      m.f_lambda[f] = r2 / (r1 + r2);
-      sub3(dx, x2, x1);
-      m.f_rlencos[f] = area / std::fabs(dot3(dx, fn) / area);
-      double ss;
+  m.f_x.resize(3 * static_cast<size_t>(Nf));
+  m.f_n.resize(3 * static_cast<size_t>(Nf));
+  m.f_xpac.resize(3 * static_cast<size_t>(Nf));
+  m.f_xnac.resize(3 * static_cast<size_t>(Nf));
+  m.f_area.resize(Nf);
+  m.f_lambda.resize(Nf);
+  m.f_rlencos.resize(Nf);
+  std::atomic<int> warped_sum{0};
+  parallel_chunks(Nf, [&](long f0, long f1) {
+    int warped = 0;
+    for (long f = f0; f < f1; ++f) {
+      const int* q = &m.f_v[4 * static_cast<size_t>(f)];
+      const double *p1 = P(q[0]), *p2 = P(q[1]), *p3 = P(q[2]), *p4 = P(q[3]);
+      double a1 = tri_area(p1, p2, p4), a2 = tri_area(p2, p3, p4);
+      double* fx = &m.f_x[3 * static_cast<size_t>(f)];
+      double* fn = &m.f_n[3 * static_cast<size_t>(f)];
+      double xc1[3], xc2[3];
+      for (int j = 0; j < 3; ++j) {
+        xc1[j] = 1. / 3 * (p1[j] + p2[j] + p4[j]);
+        xc2[j] = 1. / 3 * (p2[j] + p3[j] + p4[j]);
+      }
+      for (int j = 0; j < 3; ++j) fx[j] = (a1 * xc1[j] + a2 * xc2[j]) / (a1 + a2);
+      double d1[3], d2[3], dx[3];
+      sub3(d1, p1, p3);
+      sub3(d2, p2, p4);
+      cross3(fn, d1, d2);
+      for (int j = 0; j < 3; ++j) fn[j] = 0.5 * fn[j];
+      const double* x1 = &m.c_x[3 * static_cast<size_t>(m.f_c1[f])];
       sub3(dx, fx, x1);
-      ss = dot3(dx, fn) / area;
-      for (int j = 0; j < 3; ++j) dx[j] = ss * fn[j] / area;
-      sub3(&m.f_xpac[3 * static_cast<size_t>(f)], fx, dx);
-      sub3(dx, fx, x2);
-      ss = dot3(dx, fn) / area;
-      for (int j = 0; j < 3; ++j) dx[j] = ss * fn[j] / area;
-      sub3(&m.f_xnac[3 * static_cast<size_t>(f)], fx, dx);
-    } else {
-      m.f_lambda[f] = 1.;
-      sub3(dx, fx, x1);
-      m.f_rlencos[f] = area / (dot3(dx, fn) / area);
+      if (dot3(fn, dx) < 0.)
+        for (int j = 0; j < 3; ++j) fn[j] = -fn[j];
+      double area = len3(fn);
+      m.f_area[f] = area;
+      if (std::fabs((a1 + a2 - area) / area) > 1.e-4) warped++;
+      int c2 = m.f_c2[f];
+      if (c2 >= 0) {
+        const double* x2 = &m.c_x[3 * static_cast<size_t>(c2)];
+        sub3(d1, fx, x1);
+        sub3(d2, fx, x2);
+        double r1 = len3(d1), r2 = len3(d2);
+        m.f_lambda[f] = r2 / (r1 + r2);
+        sub3(dx, x2, x1);
+        m.f_rlencos[f] = area / std::fabs(dot3(dx, fn) / area);
+        double ss;
+        sub3(dx, fx, x1);
+        ss = dot3(dx, fn) / area;
+        for (int j = 0; j < 3; ++j) dx[j] = ss * fn[j] / area;
+        sub3(&m.f_xpac[3 * static_cast<size_t>(f)], fx, dx);
+        sub3(dx, fx, x2);
+        ss = dot3(dx, fn) / area;
+        for (int j = 0; j < 3; ++j) dx[j] = ss * fn[j] / area;
+        sub3(&m.f_xnac[3 * static_cast<size_t>(f)], fx, dx);
+      } else {
+        m.f_lambda[f] = 1.;
+        sub3(dx, fx, x1);
+        m.f_rlencos[f] = area / (dot3(dx, fn) / area);
+        for (int j = 0; j < 3; ++j) m.f_xpac[3 * static_cast<size_t>(f) + j] = m.f_xnac[3 * static_cast<size_t>(f) + j] = 0.0;
+      }
     }
-  }
+    warped_sum += warped;
+  });
+  m.n_warped = warped_sum.load();
   clk.mark("face metrics");
-  m.n_warped = warped;
-  m.b_dist.assign(Nb, 0.0);
+  m.b_dist.resize(Nb);
   for (int b = 0; b < Nb; ++b) {
     double dx[3];
     sub3(dx, &m.f_x[3 * static_cast<size_t>(b)], &m.c_x[3 * static_cast<size_t>(m.f_c1[b])]);
     m.b_dist[b] = dot3(dx, &m.f_n[3 * static_cast<size_t>(b)]) / m.f_area[b];
     if (m.b_dist[b] < 0.) throw std::runtime_error("mesh: wall distance is negative at boundary " + std::to_string(b));
   }
-  clk.mark("boundary distance");
-  // ---- CSR pattern
-  m.rowptr.assign(Nc + 1, 0);
-  m.diag.assign(Nc, -1);
-  m.slot.assign(static_cast<size_t>(Nc) * K, -1);
-  m.col.reserve(static_cast<size_t>(Nc) + 2 * static_cast<size_t>(Nf - Nb));
-  for (int i = 0; i < Nc; ++i) {
-    int cols[kMaxFace + 1];
+  // ---- CSR pattern: count, prefix, fill
+  m.rowptr.resize(Nc + 1);
+  m.rowptr[0] = 0;
+  m.diag.resize(Nc);
+  m.slot.resize(static_cast<size_t>(Nc) * K);
+  auto row_cols = [&](long i, int* cols) {
     int nc = 0;
-    cols[nc++] = i;
+    cols[nc++] = static_cast<int>(i);
     for (int j = 0; j < m.c_nface[i]; ++j) {
       int nb = m.c_cell[static_cast<size_t>(i) * K + j];
       if (nb >= 0) cols[nc++] = nb;
     }
     std::sort(cols, cols + nc);
-    nc = static_cast<int>(std::unique(cols, cols + nc) - cols);
-    int base = static_cast<int>(m.col.size());
+    return static_cast<int>(std::unique(cols, cols + nc) - cols);
+  };
+  parallel_for(Nc, [&](long i) {
+    int cols[kMaxFace + 1];
+    m.rowptr[i + 1] = row_cols(i, cols);
+  });
+  for (int i = 0; i < Nc; ++i) m.rowptr[i + 1] += m.rowptr[i];
+  m.col.resize(m.rowptr[Nc]);
+  parallel_for(Nc, [&](long i) {
+    int cols[kMaxFace + 1];
+    const int nc = row_cols(i, cols);
+    const int base = m.rowptr[i];
     for (int k = 0; k < nc; ++k) {
-      m.col.push_back(cols[k]);
+      m.col[base + k] = cols[k];
       if (cols[k] == i) m.diag[i] = base + k;
     }
-    for (int j = 0; j < m.c_nface[i]; ++j) {
+    for (int j = 0; j < K; ++j) {
       int nb = m.c_cell[static_cast<size_t>(i) * K + j];
-      if (nb < 0) continue;
-      m.slot[static_cast<size_t>(i) * K + j] = base + static_cast<int>(std::lower_bound(cols, cols + nc, nb) - cols);
+      m.slot[static_cast<size_t>(i) * K + j] =
+          nb < 0 ? -1 : base + static_cast<int>(std::lower_bound(cols, cols + nc, nb) - cols);
     }
-    m.rowptr[i + 1] = static_cast<int>(m.col.size());
-  }
+  });
   clk.mark("CSR");
-  // ---- compact cell->face lists
-  m.cf_ptr.assign(Nc + 1, 0);
-  for (int i = 0; i < Nc; ++i) m.cf_ptr[i + 1] = m.cf_ptr[i] + m.c_nface[i];
-  m.cf_pat.resize(m.cf_ptr[Nc]);
+  // ---- compact cell->face lists (cf_ptr / cf_pat were filled with the faces)
   m.cf_slot.resize(m.cf_ptr[Nc]);
   m.cf_sorted.resize(m.cf_ptr[Nc]);
-  for (int i = 0; i < Nc; ++i) {
-    int b = m.cf_ptr[i];
+  parallel_for(Nc, [&](long i) {
+    const int b = m.cf_ptr[i];
     for (int j = 0; j < m.c_nface[i]; ++j) {
-      m.cf_pat[b + j] = m.c_face[static_cast<size_t>(i) * K + j];
       m.cf_slot[b + j] = m.slot[static_cast<size_t>(i) * K + j];
       m.cf_sorted[b + j] = m.cf_pat[b + j];
     }
     std::sort(m.cf_sorted.begin() + b, m.cf_sorted.begin() + b + m.c_nface[i]);
-  }
+  });
   clk.mark("cell-face lists");
   return m;
 }

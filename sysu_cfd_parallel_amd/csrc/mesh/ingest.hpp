@@ -3,7 +3,9 @@
 // arrays so they upload to HBM unchanged.
 #pragma once
 #include <cstdint>
+#include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../../include/simple355.h"
@@ -11,36 +13,59 @@
 namespace s355 {
 
 constexpr int kMaxFace = S355_MAX_CELL_FACES;
+
+// Allocator whose resize() leaves elements uninitialised, so that multi-GB
+// mesh arrays are first touched by the parallel loops that fill them.
+template <class T>
+struct DefaultInit : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = DefaultInit<U>;
+  };
+  DefaultInit() = default;
+  template <class U>
+  DefaultInit(const DefaultInit<U>&) noexcept {}
+  template <class U>
+  void construct(U* p) noexcept {
+    ::new (static_cast<void*>(p)) U;
+  }
+  template <class U, class... A>
+  void construct(U* p, A&&... a) {
+    ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+  }
+};
+template <class T>
+using hvec = std::vector<T, DefaultInit<T>>;
 constexpr int kVoidBoundary = -10;
 
 struct Mesh {
   int n_vert = 0, n_cell = 0, n_face = 0, n_bnd = 0, n_elem = 0, n_warped = 0;
-  std::vector<int> perm;      // element order after the partition sort
-  std::vector<double> vert;   // 3 per vertex
+  hvec<int> perm;      // element order after the partition sort
+  hvec<double> vert;   // 3 per vertex
 
   // faces: boundary faces are 0..n_bnd-1 with face i <-> boundary record i
-  std::vector<int> f_c1, f_c2, f_bnd, f_v;  // f_v: 4 per face
-  std::vector<double> f_x, f_n, f_xpac, f_xnac;  // 3 per face
-  std::vector<double> f_area, f_lambda, f_rlencos;
+  hvec<int> f_c1, f_c2, f_bnd, f_v;  // f_v: 4 per face
+  hvec<double> f_x, f_n, f_xpac, f_xnac;  // 3 per face
+  hvec<double> f_area, f_lambda, f_rlencos;
 
   // cells: faces in the reference's pattern order (CellData::face)
-  std::vector<int> c_rid, c_nface, c_face, c_cell;  // c_face/c_cell: kMaxFace per cell, -1 padded
-  std::vector<uint8_t> c_poly;
-  std::vector<double> c_vol, c_x;  // c_x: 3 per cell
+  hvec<int> c_rid, c_nface, c_face, c_cell;  // c_face/c_cell: kMaxFace per cell, -1 padded
+  hvec<uint8_t> c_poly;
+  hvec<double> c_vol, c_x;  // c_x: 3 per cell
 
   // boundary records
-  std::vector<int> b_rid;
-  std::vector<double> b_dist;
+  hvec<int> b_rid;
+  hvec<double> b_dist;
 
   // CSR pattern (rows = cells, columns ascending like PETSc AIJ)
-  std::vector<int> rowptr, col, diag;
-  std::vector<int> slot;  // kMaxFace per cell: CSR index of the neighbour, -1 for boundary
+  hvec<int> rowptr, col, diag;
+  hvec<int> slot;  // kMaxFace per cell: CSR index of the neighbour, -1 for boundary
 
   // compact cell->face lists for device gathers
-  std::vector<int> cf_ptr;     // n_cell + 1
-  std::vector<int> cf_pat;     // faces in pattern order
-  std::vector<int> cf_slot;    // CSR slot per pattern entry (-1 boundary)
-  std::vector<int> cf_sorted;  // faces in ascending order
+  hvec<int> cf_ptr;     // n_cell + 1
+  hvec<int> cf_pat;     // faces in pattern order
+  hvec<int> cf_slot;    // CSR slot per pattern entry (-1 boundary)
+  hvec<int> cf_sorted;  // faces in ascending order
 
   std::vector<s355_region> regions;
 

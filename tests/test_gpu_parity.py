@@ -58,8 +58,9 @@ def test_build_velocity_matrix(name):
     d.call("build_velocity_matrix")
     o.call("build_velocity_matrix")
     d.call("synchronize")
-    # outlet rescale sums the boundary mass flow in a different order -> rounding
-    tol = 1e-13 if d.case.meta["kind"] == "channel" else 0.0
+    # the outlet rescale sums the boundary mass flow in a different order, so
+    # cases with an outlet agree to rounding instead of bitwise
+    tol = 1e-13 if d.case.meta["kind"] in ("channel", "step") else 0.0
     assert_close(d, o, ["BRO", "BU", "BV", "BW", "BPRE", "DUDX", "DVDX", "DWDX", "DPDX", "AU", "BU_RHS", "BV_RHS",
                         "BW_RHS", "APR", "SHEAR"], tol)
 
