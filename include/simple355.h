@@ -250,6 +250,10 @@ int s355_calculate_velocity(s355_ctx* ctx, s355_solve_info info[3]);
 int s355_calculate_pressure(s355_ctx* ctx, s355_solve_info* info);
 int s355_update_tur_kepsilon(s355_ctx* ctx, s355_solve_info info[2]);
 
+/* Solve records of the last SIMPLE phases run through the phase-level
+ * entries: info[0..5] = U, V, W, pressure correction, k, epsilon. */
+int s355_last_solve_info(s355_ctx* ctx, s355_solve_info info[6]);
+
 /* NSSolve steady loop body (navier.cpp:52-141) for n_steps steps.  history
  * receives 11 doubles per step: Residual[0..9] and ResMax.  *steps_done is
  * the number of completed steps (stops early when ResMax < residual_steady

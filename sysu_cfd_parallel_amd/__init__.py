@@ -15,16 +15,16 @@ from typing import Optional
 import numpy as np
 
 from . import _abi
-from ._abi import (FIELD_ID, FIELD_KIND, MESH_DBL_ID, MESH_INT_ID, FP64, KSP_CG, KSP_GMRES, MIXED, PC_ILU0,
-                   PC_JACOBI, PC_NONE, KspOpts, MeshDesc, Params, Region, Sizes, SolveInfo, field_len,
-                   mesh_dbl_len, mesh_int_len)
+from ._abi import (FIELD_ID, FIELD_KIND, MESH_DBL_ID, MESH_INT_ID, FP64, KSP_CG, KSP_GMRES, KSP_PRESSURE,
+                   KSP_SCALAR, KSP_VELOCITY, MIXED, PC_ILU0, PC_JACOBI, PC_NONE, KspOpts, MeshDesc, Params, Region,
+                   Sizes, SolveInfo, field_len, mesh_dbl_len, mesh_int_len)
 
 LIB_PATH = Path(__file__).resolve().parent / "_build" / "libsimple355.so"
 
 __all__ = [
     "LIB_PATH", "library", "CaseData", "cavity", "channel", "step", "default_params", "set_all_ksp", "HostMesh",
     "Solver", "S355Error", "NotConverged", "FP64", "MIXED", "KSP_GMRES", "KSP_CG", "PC_NONE", "PC_JACOBI",
-    "PC_ILU0",
+    "PC_ILU0", "KSP_VELOCITY", "KSP_PRESSURE", "KSP_SCALAR",
 ]
 
 
@@ -86,6 +86,7 @@ def library(path: Optional[Path] = None) -> C.CDLL:
         "calculate_pressure": (I, [V, C.POINTER(SolveInfo)]),
         "update_tur_kepsilon": (I, [V, C.POINTER(SolveInfo)]),
         "spmv_time": (I, [V, I, I, I, pD]),
+        "last_solve_info": (I, [V, C.POINTER(SolveInfo)]),
     }
     for name, (res, args) in more.items():
         fn = getattr(lib, f"s355_{name}")
@@ -321,6 +322,12 @@ class Solver:
         h = hist.reshape(-1, 11)[: done.value]
         _check(self.lib, rc, f"simple_iterate (after {done.value} steps)")
         return h, done.value
+
+    def last_solve_info(self) -> dict:
+        """Solve records of the last phases: U, V, W, pressure, k, epsilon."""
+        info = (SolveInfo * 6)()
+        _check(self.lib, self.lib.s355_last_solve_info(self.ctx, info), "last_solve_info")
+        return {n: info[k].as_dict() for k, n in enumerate(("U", "V", "W", "P", "TE", "ED"))}
 
     def stage_times(self, reset: bool = False) -> np.ndarray:
         out = np.zeros(6)

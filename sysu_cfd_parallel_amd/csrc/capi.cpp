@@ -489,6 +489,15 @@ int s355_update_tur_kepsilon(s355_ctx* ctx, s355_solve_info info[2]) {
   });
 }
 
+int s355_last_solve_info(s355_ctx* ctx, s355_solve_info info[6]) {
+  return guarded([&] {
+    dev(ctx);
+    if (!info) throw std::invalid_argument("null info");
+    for (int k = 0; k < 6; ++k) info[k] = ctx->ns->last[k];
+    return S355_OK;
+  });
+}
+
 int s355_simple_iterate(s355_ctx* ctx, int n_steps, double* history, int* steps_done) {
   int done = 0;
   int rc = guarded([&] {

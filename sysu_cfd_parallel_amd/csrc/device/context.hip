@@ -332,16 +332,64 @@ void fill_info(const KResult& r, s355_solve_info* info) {
   info->reason = r.reason;
   info->rnorm = r.rnorm;
   info->rnorm0 = r.rnorm0;
-  info->refinements = 0;
+  info->refinements = r.refinements;
   info->pad = 0;
 }
 }  // namespace
 
+void Context::ensure_ilu() {
+  if (have_levels_) return;
+  std::vector<int> lrows, urows;
+  ilu_levels(dm.Nc, mesh.rowptr.data(), mesh.col.data(), mesh.diag.data(), lrows, levels_.lptr, urows, levels_.uptr);
+  levels_.lrows = upload(allocs_, lrows, stream);
+  levels_.urows = upload(allocs_, urows, stream);
+  kwork_.lu = alloc<double>(dm.nnz);
+  kwork_.ilu = &levels_;
+  sync();  // host staging of the level lists
+  have_levels_ = true;
+}
+
+void Context::ensure_mixed() {
+  if (a32_) return;
+  const size_t Nc = dm.Nc;
+  a32_ = alloc<float>(dm.nnz);
+  kwork32_.r = alloc<float>(Nc);
+  kwork32_.lu = alloc<float>(dm.nnz);
+  kwork32_.n = dm.Nc;
+  kwork32_.mmax = max_restart_;
+  kwork32_.V = alloc<float>((static_cast<size_t>(max_restart_) + 1) * Nc);
+  kwork32_.pcinv = alloc<float>(Nc);
+  kwork32_.partial = partial_;
+  kwork32_.partial_len = partial_len_;
+  kwork32_.st = kwork_.st;
+  kwork32_.host = kwork_.host;
+  ir_.r64 = alloc<double>(Nc);
+  ir_.r32 = alloc<float>(Nc);
+  ir_.e32 = alloc<float>(Nc);
+  ir_.inv32 = alloc<float>(Nc);
+  ir_.partial = partial_;
+  ir_.partial_len = partial_len_;
+  ir_.scal = scalars_ + 20;
+  ir_.host = reinterpret_cast<double*>(static_cast<char*>(host_pinned_) + 3584);
+}
+
 int Context::run_solve(int which, const double* b, double* x, const s355_ksp_opts& o, bool zero_guess,
                        s355_solve_info* info) {
   const double* val = which == 0 ? df.Au : (which == 1 ? df.Ap : df.As);
-  CsrView<double> A{dm.Nc, dm.rowptr, dm.col, dm.diag, val};
-  KResult r = krylov_solve<double>(A, b, x, o, zero_guess, kwork_, stream);
+  KResult r;
+  if (o.pc == S355_PC_ILU0) ensure_ilu();
+  if (par.precision == S355_MIXED) {
+    // FP32 inner solves on the rounded matrix inside FP64 refinement
+    ensure_mixed();
+    kwork32_.ilu = kwork_.ilu;
+    convert_to_float(val, a32_, dm.nnz, stream);
+    CsrView<float> A{dm.Nc, dm.rowptr, dm.col, dm.diag, a32_, dm.nnz};
+    CsrView<double> A64{dm.Nc, dm.rowptr, dm.col, dm.diag, val, dm.nnz};
+    r = ir_solve(A64, A, b, x, o, zero_guess, par.ir_max, par.ir_inner_rtol, kwork32_, ir_, stream);
+  } else {
+    CsrView<double> A{dm.Nc, dm.rowptr, dm.col, dm.diag, val, dm.nnz};
+    r = krylov_solve<double>(A, b, x, o, zero_guess, kwork_, stream);
+  }
   fill_info(r, info);
   return r.reason < 0 ? S355_NOT_CONVERGED : S355_OK;
 }
@@ -473,7 +521,7 @@ double Context::time_spmv(int which, int precision, int repeats) {
   if (repeats < 1) repeats = 1;
   double* y = static_cast<double*>(kwork_.V);
   if (precision == 0) {
-    CsrView<double> A{dm.Nc, dm.rowptr, dm.col, dm.diag, val};
+    CsrView<double> A{dm.Nc, dm.rowptr, dm.col, dm.diag, val, dm.nnz};
     return bench_spmv<double>(A, df.U, y, kwork_, repeats, stream);
   }
   // FP32 copy of the matrix and of x in the Krylov workspace
@@ -485,7 +533,7 @@ double Context::time_spmv(int which, int precision, int repeats) {
     throw std::runtime_error("time_spmv: workspace too small for the FP32 copy");
   convert_to_float(val, vf, dm.nnz, stream);
   convert_to_float(df.U, xf, dm.Nc, stream);
-  CsrView<float> A{dm.Nc, dm.rowptr, dm.col, dm.diag, vf};
+  CsrView<float> A{dm.Nc, dm.rowptr, dm.col, dm.diag, vf, dm.nnz};
   return bench_spmv<float>(A, xf, yf, kwork_, repeats, stream);
 }
 

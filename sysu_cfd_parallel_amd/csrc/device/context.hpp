@@ -79,6 +79,8 @@ class Context {
   void free_all();
   int run_solve(int which_matrix, const double* b, double* x, const s355_ksp_opts& o, bool zero_guess,
                 s355_solve_info* info);
+  void ensure_mixed();
+  void ensure_ilu();
 
   std::vector<void*> allocs_;
   double* partial_ = nullptr;
@@ -88,6 +90,13 @@ class Context {
   double *u0_ = nullptr, *v0_ = nullptr, *w0_ = nullptr;
   KepsWork kw_{};
   KWork kwork_{};
+  // mixed precision: FP32 matrix copy, FP32 Krylov workspace, refinement buffers
+  float* a32_ = nullptr;
+  KWork kwork32_{};
+  IrWork ir_{};
+  // ILU(0): level schedule of the pattern and factor storage
+  IluLevels levels_{};
+  bool have_levels_ = false;
   void* host_pinned_ = nullptr;
   bool has_outlet_ = false;
   int max_restart_ = 0;

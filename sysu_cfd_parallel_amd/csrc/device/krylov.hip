@@ -9,6 +9,9 @@
 // Every kernel returns at once when the device state says the solve has
 // ended, so the host may launch iterations ahead and read the state back
 // only every few iterations.
+#include <algorithm>
+#include <cmath>
+
 #include "common.hpp"
 #include "krylov.hpp"
 
@@ -37,20 +40,143 @@ __global__ void __launch_bounds__(kBlock) k_jacobi_setup(CsrView<T> A, T* inv) {
   inv[i] = static_cast<T>(d != 0.0 ? 1.0 / d : 1.0);
 }
 
+// ---- ILU(0), natural ordering, one thread per row of a level
+// factor: IKJ elimination a_ik /= u_kk, a_ij -= l_ik u_kj on the pattern
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_ilu_factor(CsrView<T> A, T* __restrict__ lu, const int* __restrict__ rows,
+                                                       int nrows) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrows) return;
+  const int i = rows[t];
+  const int pd = A.diag[i], pe = A.rowptr[i + 1];
+  for (int p = A.rowptr[i]; p < pd; ++p) {
+    const int k = A.col[p];
+    lu[p] = lu[p] / lu[A.diag[k]];
+    const T lik = lu[p];
+    int q = A.diag[k] + 1, r = p + 1;
+    const int qe = A.rowptr[k + 1];
+    while (q < qe && r < pe) {
+      const int cq = A.col[q], cr = A.col[r];
+      if (cq == cr) {
+        lu[r] -= lik * lu[q];
+        ++q;
+        ++r;
+      } else if (cq < cr) {
+        ++q;
+      } else {
+        ++r;
+      }
+    }
+  }
+}
+
+// z = L^-1 r (unit lower triangle) on the rows of one level
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_ilu_lower(CsrView<T> A, const T* __restrict__ lu, const T* r, T* z,
+                                                      const int* __restrict__ rows, int nrows, const KState* st) {
+  if (st && st->reason) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrows) return;
+  const int i = rows[t];
+  T s = r[i];
+  for (int p = A.rowptr[i]; p < A.diag[i]; ++p) s -= lu[p] * z[A.col[p]];
+  z[i] = s;
+}
+
+// z = U^-1 z on the rows of one level (in place)
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_ilu_upper(CsrView<T> A, const T* __restrict__ lu, T* z,
+                                                      const int* __restrict__ rows, int nrows, const KState* st) {
+  if (st && st->reason) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrows) return;
+  const int i = rows[t];
+  T s = z[i];
+  const int pd = A.diag[i];
+  for (int p = pd + 1; p < A.rowptr[i + 1]; ++p) s -= lu[p] * z[A.col[p]];
+  z[i] = s / lu[pd];
+}
+
+// partial (z,z) and (z,r) for CG with a non-pointwise preconditioner
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_dot2(int n, const T* z, const T* r, double* partial, const KState* st) {
+  if (st && st->reason) return;
+  __shared__ double lds[kBlock / kWave];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double zz = 0.0, zr = 0.0;
+  if (i < n) {
+    const double zv = static_cast<double>(z[i]);
+    zz = zv * zv;
+    zr = zv * static_cast<double>(r[i]);
+  }
+  zz = block_sum(zz, lds);
+  zr = block_sum(zr, lds);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = zz;
+    partial[2 * blockIdx.x + 1] = zr;
+  }
+}
+
+template <class T>
+void ilu_factor(const CsrView<T>& A, int nnz, const IluLevels& lv, T* lu, hipStream_t s) {
+  S355_HIP(hipMemcpyAsync(lu, A.val, sizeof(T) * static_cast<size_t>(nnz), hipMemcpyDeviceToDevice, s));
+  for (int l = 0; l < lv.nl(); ++l) {
+    const int nr = lv.lptr[l + 1] - lv.lptr[l];
+    hipLaunchKernelGGL((k_ilu_factor<T>), dim3(grid_for(nr)), dim3(kBlock), 0, s, A, lu, lv.lrows + lv.lptr[l], nr);
+  }
+  S355_LAUNCH_CHECK();
+}
+
+// z = (LU)^-1 r ; r and z distinct
+template <class T>
+void ilu_apply(const CsrView<T>& A, const IluLevels& lv, const T* lu, const T* r, T* z, const KState* st,
+               hipStream_t s) {
+  for (int l = 0; l < lv.nl(); ++l) {
+    const int nr = lv.lptr[l + 1] - lv.lptr[l];
+    hipLaunchKernelGGL((k_ilu_lower<T>), dim3(grid_for(nr)), dim3(kBlock), 0, s, A, lu, r, z, lv.lrows + lv.lptr[l],
+                       nr, st);
+  }
+  for (int l = 0; l < lv.nu(); ++l) {
+    const int nr = lv.uptr[l + 1] - lv.uptr[l];
+    hipLaunchKernelGGL((k_ilu_upper<T>), dim3(grid_for(nr)), dim3(kBlock), 0, s, A, lu, z, lv.urows + lv.uptr[l], nr,
+                       st);
+  }
+  S355_LAUNCH_CHECK();
+}
+
 // ---- y = scale * M^-1 A x
+// kRowLanes consecutive lanes share a row: their loads of values and column
+// ids are contiguous across the wavefront; a shuffle tree in a fixed order
+// sums the lanes.
+constexpr int kRowLanes = 8;
+
+template <class T>
+__device__ __forceinline__ T row_dot(const CsrView<T>& A, const T* __restrict__ x, int row, int lane) {
+  T s = T(0);
+  if (row < A.n) {
+    const int p1 = A.rowptr[row + 1];
+    for (int p = A.rowptr[row] + lane; p < p1; p += kRowLanes) s += A.val[p] * x[A.col[p]];
+  }
+#pragma unroll
+  for (int off = kRowLanes / 2; off > 0; off >>= 1) s += __shfl_down(s, off, kRowLanes);
+  return s;
+}
+
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_spmv_pc(CsrView<T> A, const T* __restrict__ x, const T* __restrict__ inv,
                                                     T* __restrict__ y, const KState* st, int it) {
   if (st->reason) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= A.n) return;
-  T s = T(0);
-  const int p1 = A.rowptr[i + 1];
-  for (int p = A.rowptr[i]; p < p1; ++p) s += A.val[p] * x[A.col[p]];
-  double z = static_cast<double>(s);
-  if (inv) z *= static_cast<double>(inv[i]);
-  y[i] = static_cast<T>(z * st->scale[it]);
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = g / kRowLanes, lane = g % kRowLanes;
+  const T s = row_dot(A, x, row, lane);
+  if (lane == 0 && row < A.n) {
+    double z = static_cast<double>(s);
+    if (inv) z *= static_cast<double>(inv[row]);
+    y[row] = static_cast<T>(z * st->scale[it]);
+  }
 }
+
+inline int spmv_grid(int n) { return grid_for(static_cast<long>(n) * kRowLanes); }
 
 // ---- partial[b][k] = sum over block b of w_i V_k,i, k = 0..it
 template <class T, int R>
@@ -136,82 +262,106 @@ __global__ void __launch_bounds__(kBlock) k_maxpy_norm(T* __restrict__ V, size_t
   if (threadIdx.x == 0) partial[blockIdx.x] = nrm;
 }
 
-// ---- Hessenberg column, Givens rotations, convergence test (one block)
+// ---- Hessenberg column, Givens rotations, convergence test (one block).
+// The column and the previous rotations are staged in LDS, so the serial
+// rotation sweep of thread 0 runs out of LDS rather than global memory.
 __global__ void __launch_bounds__(kBlock) k_arnoldi_finalize(const double* partial, int nblk, int it, int m,
                                                              KState* st) {
   if (st->reason) return;
   __shared__ double lds[kBlock / kWave];
+  __shared__ double col[kMaxRestart + 2], cc[kMaxRestart + 1], ss[kMaxRestart + 1];
+  __shared__ double res_sh;
+  __shared__ int reason_sh;
   double s = 0.0;
   for (int b = threadIdx.x; b < nblk; b += kBlock) s += partial[b];
-  s = block_sum(s, lds);
-  if (threadIdx.x != 0) return;
+  for (int j = threadIdx.x; j <= it; j += kBlock) {
+    col[j] = st->hcol[j];
+    cc[j] = st->cc[j];
+    ss[j] = st->ss[j];
+  }
+  s = block_sum(s, lds);  // contains the barrier that publishes col/cc/ss
   const int ld = m + 1;
-  const double tt = sqrt(s);
-  for (int j = 0; j <= it; ++j) Hm(st, ld, j, it) = st->hcol[j];
-  Hm(st, ld, it + 1, it) = tt;
-  st->scale[it + 1] = tt != 0.0 ? 1.0 / tt : 1.0;
-  double hapbnd = fabs(tt / st->grs[it]);
-  if (hapbnd > 1.e-30) hapbnd = 1.e-30;
-  const bool hapend = tt < hapbnd;
-  for (int j = 1; j <= it; ++j) {
-    const double t0 = Hm(st, ld, j - 1, it);
-    Hm(st, ld, j - 1, it) = st->cc[j - 1] * t0 + st->ss[j - 1] * Hm(st, ld, j, it);
-    Hm(st, ld, j, it) = st->cc[j - 1] * Hm(st, ld, j, it) - st->ss[j - 1] * t0;
-  }
-  double res;
-  int reason = 0;
-  if (!hapend) {
-    const double hv = Hm(st, ld, it, it), hn = Hm(st, ld, it + 1, it);
-    const double tr = sqrt(hv * hv + hn * hn);
-    if (tr == 0.0) {
-      reason = kDivergedNull;
-      res = st->rnorm;
-    } else {
-      st->cc[it] = hv / tr;
-      st->ss[it] = hn / tr;
-      st->grs[it + 1] = -(st->ss[it] * st->grs[it]);
-      st->grs[it] = st->cc[it] * st->grs[it];
-      Hm(st, ld, it, it) = st->cc[it] * hv + st->ss[it] * hn;
-      res = fabs(st->grs[it + 1]);
+  if (threadIdx.x == 0) {
+    const double tt = sqrt(s);
+    col[it + 1] = tt;
+    st->scale[it + 1] = tt != 0.0 ? 1.0 / tt : 1.0;
+    const double g_it = st->grs[it];
+    double hapbnd = fabs(tt / g_it);
+    if (hapbnd > 1.e-30) hapbnd = 1.e-30;
+    const bool hapend = tt < hapbnd;
+    for (int j = 1; j <= it; ++j) {
+      const double t0 = col[j - 1];
+      col[j - 1] = cc[j - 1] * t0 + ss[j - 1] * col[j];
+      col[j] = cc[j - 1] * col[j] - ss[j - 1] * t0;
     }
-  } else {
-    res = 0.0;
+    double res;
+    int reason = 0;
+    if (!hapend) {
+      const double hv = col[it], hn = col[it + 1];
+      const double tr = sqrt(hv * hv + hn * hn);
+      if (tr == 0.0) {
+        reason = kDivergedNull;
+        res = st->rnorm;
+      } else {
+        const double c = hv / tr, sn = hn / tr;
+        st->cc[it] = c;
+        st->ss[it] = sn;
+        st->grs[it + 1] = -(sn * g_it);
+        st->grs[it] = c * g_it;
+        col[it] = c * hv + sn * hn;
+        res = fabs(st->grs[it + 1]);
+      }
+    } else {
+      res = 0.0;
+    }
+    st->it = it + 1;
+    st->its += 1;
+    st->rnorm = res;
+    if (!reason) {
+      reason = conv_check(st, res);
+      if (hapend && !reason) reason = kDivergedBreakdown;
+    }
+    st->reason = reason;
+    res_sh = res;
+    reason_sh = reason;
   }
-  st->it = it + 1;
-  st->its += 1;
-  st->rnorm = res;
-  if (!reason) {
-    reason = conv_check(st, res);
-    if (hapend && !reason) reason = kDivergedBreakdown;
-  }
-  st->reason = reason;
+  __syncthreads();
+  for (int j = threadIdx.x; j <= it + 1; j += kBlock) Hm(st, ld, j, it) = col[j];
 }
 
-// ---- back substitution; y_j = scale_j * nrs_j for the it completed vectors
-__global__ void k_build_y(KState* st, int m, int* nupd) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// ---- back substitution R y = g on the it completed columns, y_j scaled by
+// scale_j; one wavefront, column-oriented: lane i keeps t_i, and once y_r is
+// known every lane i < r subtracts H(i,r) y_r.
+__global__ void __launch_bounds__(kWave) k_build_y(KState* st, int m, int* nupd) {
+  const int lane = threadIdx.x;
   const int ld = m + 1;
   const int k = st->it - 1;
-  *nupd = 0;
-  if (k < 0) return;
-  double nrs[kMaxRestart + 1];
-  if (Hm(st, ld, k, k) == 0.0) {
-    st->reason = kDivergedBreakdown;
+  if (k < 0) {
+    if (lane == 0) *nupd = 0;
     return;
   }
-  nrs[k] = st->grs[k] / Hm(st, ld, k, k);
-  for (int ii = 1; ii <= k; ++ii) {
-    const int r = k - ii;
-    double tt = st->grs[r];
-    for (int j = r + 1; j <= k; ++j) tt = tt - Hm(st, ld, r, j) * nrs[j];
-    if (Hm(st, ld, r, r) == 0.0) {
-      st->reason = kDivergedBreakdown;
-      return;
+  double t = lane <= k ? st->grs[lane] : 0.0;
+  double y = 0.0;
+  bool broke = false;
+  for (int r = k; r >= 0; --r) {
+    const double hrr = Hm(st, ld, r, r);
+    if (hrr == 0.0) {
+      broke = true;
+      break;
     }
-    nrs[r] = tt / Hm(st, ld, r, r);
+    const double yr = __shfl(t, r, kWave) / hrr;
+    if (lane == r) y = yr;
+    if (lane < r) t -= Hm(st, ld, lane, r) * yr;
   }
-  for (int j = 0; j <= k; ++j) st->y[j] = nrs[j] * st->scale[j];
-  *nupd = k + 1;
+  if (broke) {
+    if (lane == 0) {
+      st->reason = kDivergedBreakdown;
+      *nupd = 0;
+    }
+    return;
+  }
+  if (lane <= k) st->y[lane] = y * st->scale[lane];
+  if (lane == 0) *nupd = k + 1;
 }
 
 template <class T, int R>
@@ -551,11 +701,17 @@ KResult gmres(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, boo
   const size_t ld = static_cast<size_t>(n);
   T* V = static_cast<T*>(w.V);
   T* inv = nullptr;
+  T* lu = nullptr;
+  T* tmp = static_cast<T*>(w.r);
   if (o.pc == S355_PC_JACOBI) {
     inv = static_cast<T*>(w.pcinv);
     hipLaunchKernelGGL((k_jacobi_setup<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, A, inv);
+  } else if (o.pc == S355_PC_ILU0) {
+    if (!w.ilu || !w.lu || !w.r) throw std::invalid_argument("gmres: ILU(0) workspace missing");
+    lu = static_cast<T*>(w.lu);
+    ilu_factor(A, A.nnz, *w.ilu, lu, s);
   } else if (o.pc != S355_PC_NONE) {
-    throw std::invalid_argument("gmres: preconditioner not available on the device path (use Jacobi or none)");
+    throw std::invalid_argument("gmres: unknown preconditioner");
   }
   init_state(w, o, s);
   const int g1 = grid_for(n);
@@ -566,7 +722,13 @@ KResult gmres(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, boo
     throw std::runtime_error("gmres: partial buffer too small");
   int* nupd = reinterpret_cast<int*>(&w.st->pad);
   if (!zero_guess) {
-    hipLaunchKernelGGL((k_pc_norm<T>), dim3(g1), dim3(kBlock), 0, s, n, b, inv, w.partial);
+    if (lu) {
+      ilu_apply(A, *w.ilu, lu, b, V, w.st, s);
+      hipLaunchKernelGGL((k_pc_norm<T>), dim3(g1), dim3(kBlock), 0, s, n, V, static_cast<const T*>(nullptr),
+                         w.partial);
+    } else {
+      hipLaunchKernelGGL((k_pc_norm<T>), dim3(g1), dim3(kBlock), 0, s, n, b, inv, w.partial);
+    }
     hipLaunchKernelGGL(k_snorm_final, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st);
   } else {
     fill_zero(x, sizeof(T) * static_cast<size_t>(n), s);
@@ -576,7 +738,15 @@ KResult gmres(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, boo
   const int check_every = n >= (1 << 20) ? 4 : 8;
   KResult res;
   for (;;) {
-    hipLaunchKernelGGL((k_residual_pc<T>), dim3(g1), dim3(kBlock), 0, s, A, b, x, inv, V, w.partial, gz ? 1 : 0);
+    if (lu) {
+      hipLaunchKernelGGL((k_residual_pc<T>), dim3(g1), dim3(kBlock), 0, s, A, b, x, static_cast<const T*>(nullptr),
+                         tmp, w.partial, gz ? 1 : 0);
+      ilu_apply(A, *w.ilu, lu, tmp, V, w.st, s);
+      hipLaunchKernelGGL((k_pc_norm<T>), dim3(g1), dim3(kBlock), 0, s, n, V, static_cast<const T*>(nullptr),
+                         w.partial);
+    } else {
+      hipLaunchKernelGGL((k_residual_pc<T>), dim3(g1), dim3(kBlock), 0, s, A, b, x, inv, V, w.partial, gz ? 1 : 0);
+    }
     hipLaunchKernelGGL(k_restart_finalize, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st, first ? 1 : 0,
                        zero_guess ? 1 : 0);
     S355_LAUNCH_CHECK();
@@ -585,8 +755,14 @@ KResult gmres(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, boo
     int it = 0;
     bool ended = false;
     while (it < m && its < o.maxit) {
-      hipLaunchKernelGGL((k_spmv_pc<T>), dim3(g1), dim3(kBlock), 0, s, A, V + it * ld, inv, V + (it + 1) * ld, w.st,
-                         it);
+      if (lu) {
+        hipLaunchKernelGGL((k_spmv_pc<T>), dim3(spmv_grid(n)), dim3(kBlock), 0, s, A, V + it * ld,
+                           static_cast<const T*>(nullptr), tmp, w.st, it);
+        ilu_apply(A, *w.ilu, lu, tmp, V + (it + 1) * ld, w.st, s);
+      } else {
+        hipLaunchKernelGGL((k_spmv_pc<T>), dim3(spmv_grid(n)), dim3(kBlock), 0, s, A, V + it * ld, inv,
+                           V + (it + 1) * ld, w.st, it);
+      }
       Dispatch<T>::mdot(R, V, ld, n, it, w.partial, pstride, w.st, s);
       hipLaunchKernelGGL(k_mdot_final, dim3(it + 1), dim3(kBlock), 0, s, w.partial, gR, pstride, w.st);
       Dispatch<T>::maxpy(R, V, ld, n, it, w.partial, w.st, s);
@@ -603,7 +779,7 @@ KResult gmres(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, boo
       }
     }
     (void)ended;
-    hipLaunchKernelGGL(k_build_y, dim3(1), dim3(1), 0, s, w.st, m, nupd);
+    hipLaunchKernelGGL(k_build_y, dim3(1), dim3(kWave), 0, s, w.st, m, nupd);
     Dispatch<T>::update(R, V, ld, n, x, w.st, nupd, s);
     S355_LAUNCH_CHECK();
     read_header(w, s);
@@ -629,20 +805,35 @@ KResult cg(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, bool z
   if (w.mmax < 3) throw std::invalid_argument("cg: workspace needs 4 vectors");
   T *r = V, *z = V + ld, *p = V + 2 * ld, *wv = V + 3 * ld;
   T* inv = nullptr;
+  T* lu = nullptr;
   if (o.pc == S355_PC_JACOBI) {
     inv = static_cast<T*>(w.pcinv);
     hipLaunchKernelGGL((k_jacobi_setup<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, A, inv);
+  } else if (o.pc == S355_PC_ILU0) {
+    if (!w.ilu || !w.lu) throw std::invalid_argument("cg: ILU(0) workspace missing");
+    lu = static_cast<T*>(w.lu);
+    ilu_factor(A, A.nnz, *w.ilu, lu, s);
   } else if (o.pc != S355_PC_NONE) {
-    throw std::invalid_argument("cg: preconditioner not available on the device path (use Jacobi or none)");
+    throw std::invalid_argument("cg: unknown preconditioner");
   }
   init_state(w, o, s);
   const int g1 = grid_for(n);
   if (2 * static_cast<size_t>(g1) > w.partial_len) throw std::runtime_error("cg: partial buffer too small");
   if (!zero_guess) {
-    hipLaunchKernelGGL((k_pc_norm<T>), dim3(g1), dim3(kBlock), 0, s, n, b, inv, w.partial);
+    if (lu) {
+      ilu_apply(A, *w.ilu, lu, b, z, w.st, s);
+      hipLaunchKernelGGL((k_pc_norm<T>), dim3(g1), dim3(kBlock), 0, s, n, z, static_cast<const T*>(nullptr),
+                         w.partial);
+    } else {
+      hipLaunchKernelGGL((k_pc_norm<T>), dim3(g1), dim3(kBlock), 0, s, n, b, inv, w.partial);
+    }
     hipLaunchKernelGGL(k_snorm_final, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st);
   }
   hipLaunchKernelGGL((k_cg_start<T>), dim3(g1), dim3(kBlock), 0, s, A, b, x, inv, r, z, w.partial, zero_guess ? 1 : 0);
+  if (lu) {
+    ilu_apply(A, *w.ilu, lu, r, z, w.st, s);
+    hipLaunchKernelGGL((k_dot2<T>), dim3(g1), dim3(kBlock), 0, s, n, z, r, w.partial, w.st);
+  }
   hipLaunchKernelGGL(k_cg_start_final, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st, zero_guess ? 1 : 0);
   S355_LAUNCH_CHECK();
   const int check_every = n >= (1 << 20) ? 4 : 8;
@@ -653,6 +844,10 @@ KResult cg(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, bool z
     hipLaunchKernelGGL((k_cg_spmv_dot<T>), dim3(g1), dim3(kBlock), 0, s, A, p, wv, w.partial, w.st, i);
     hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st, i);
     hipLaunchKernelGGL((k_cg_update<T>), dim3(g1), dim3(kBlock), 0, s, n, x, r, z, p, wv, inv, w.partial, w.st);
+    if (lu) {
+      ilu_apply(A, *w.ilu, lu, r, z, w.st, s);
+      hipLaunchKernelGGL((k_dot2<T>), dim3(g1), dim3(kBlock), 0, s, n, z, r, w.partial, w.st);
+    }
     hipLaunchKernelGGL(k_cg_final, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st);
     S355_LAUNCH_CHECK();
     if ((i + 1) % check_every == 0) {
@@ -686,6 +881,53 @@ __global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t* p, size_t n, uint
   if (i < n) p[i] = v;
 }
 
+// ---- iterative refinement pieces
+__global__ void __launch_bounds__(kBlock) k_inv_diag32(CsrView<float> A, float* inv, int jacobi) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  const double d = static_cast<double>(A.val[A.diag[i]]);
+  inv[i] = jacobi ? static_cast<float>(d != 0.0 ? 1.0 / d : 1.0) : 1.0f;
+}
+
+// r = b - A x (FP64 matrix and accumulation), r32 = (float) r,
+// partial ||M^-1 r||^2 ; with b_only the residual of x = 0
+__global__ void __launch_bounds__(kBlock) k_ir_residual(CsrView<double> A, const double* __restrict__ b,
+                                                        const double* __restrict__ x, const float* __restrict__ inv,
+                                                        double* __restrict__ r, float* __restrict__ r32,
+                                                        double* partial, int b_only) {
+  __shared__ double lds[kBlock / kWave];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double nrm = 0.0;
+  if (i < A.n) {
+    double rv = b[i];
+    if (!b_only) {
+      double s = 0.0;
+      const int p1 = A.rowptr[i + 1];
+      for (int p = A.rowptr[i]; p < p1; ++p) s += A.val[p] * x[A.col[p]];
+      rv -= s;
+    }
+    r[i] = rv;
+    r32[i] = static_cast<float>(rv);
+    const double z = rv * static_cast<double>(inv[i]);
+    nrm = z * z;
+  }
+  nrm = block_sum(nrm, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = nrm;
+}
+
+__global__ void __launch_bounds__(kBlock) k_sum_to(const double* partial, int nblk, double* out) {
+  __shared__ double lds[kBlock / kWave];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += kBlock) s += partial[b];
+  s = block_sum(s, lds);
+  if (threadIdx.x == 0) *out = s;
+}
+
+__global__ void __launch_bounds__(kBlock) k_add_correction(int n, double* x, const float* e) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += static_cast<double>(e[i]);
+}
+
 }  // namespace
 
 template <class T>
@@ -710,10 +952,79 @@ template KResult krylov_solve<float>(const CsrView<float>&, const float*, float*
 template void residual64<double>(const CsrView<double>&, const double*, const double*, double*, hipStream_t);
 template void residual64<float>(const CsrView<float>&, const double*, const double*, double*, hipStream_t);
 
+KResult ir_solve(const CsrView<double>& A64, const CsrView<float>& A, const double* b, double* x,
+                 const s355_ksp_opts& o, bool zero_guess, int ir_max, double inner_rtol, KWork& w32, IrWork& ir,
+                 hipStream_t s) {
+  const int n = A.n;
+  KResult res;
+  if (n <= 0) {
+    res.reason = kConvergedAtol;
+    return res;
+  }
+  const int g = grid_for(n);
+  if (static_cast<size_t>(g) > ir.partial_len) throw std::runtime_error("ir_solve: partial buffer too small");
+  hipLaunchKernelGGL(k_inv_diag32, dim3(g), dim3(kBlock), 0, s, A, ir.inv32, o.pc == S355_PC_JACOBI ? 1 : 0);
+  if (zero_guess) fill_zero(x, sizeof(double) * static_cast<size_t>(n), s);
+  // ||M^-1 b|| for the reference norm of a nonzero initial guess
+  hipLaunchKernelGGL(k_ir_residual, dim3(g), dim3(kBlock), 0, s, A64, b, x, ir.inv32, ir.r64, ir.r32, ir.partial, 1);
+  hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(kBlock), 0, s, ir.partial, g, ir.scal + 1);
+  S355_LAUNCH_CHECK();
+  s355_ksp_opts inner = o;
+  inner.rtol = inner_rtol;
+  inner.abstol = 0.0;
+  inner.dtol = 1e30;
+  int total = 0;
+  double rnorm0 = 0.0, ttol = 0.0;
+  for (int k = 0;; ++k) {
+    hipLaunchKernelGGL(k_ir_residual, dim3(g), dim3(kBlock), 0, s, A64, b, x, ir.inv32, ir.r64, ir.r32, ir.partial,
+                       zero_guess && k == 0 ? 1 : 0);
+    hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(kBlock), 0, s, ir.partial, g, ir.scal);
+    S355_LAUNCH_CHECK();
+    S355_HIP(hipMemcpyAsync(ir.host, ir.scal, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    S355_HIP(hipStreamSynchronize(s));
+    const double rn = std::sqrt(ir.host[0]);
+    if (k == 0) {
+      const double snorm = std::sqrt(ir.host[1]);
+      rnorm0 = zero_guess ? rn : (snorm != 0.0 ? snorm : rn);
+      ttol = std::max(o.rtol * rnorm0, o.abstol);
+    }
+    res.rnorm = rn;
+    res.rnorm0 = rnorm0;
+    res.refinements = k;
+    if (std::isnan(rn) || std::isinf(rn)) {
+      res.reason = kDivergedNanorinf;
+      break;
+    }
+    if (rn <= ttol) {
+      res.reason = rn < o.abstol ? kConvergedAtol : kConvergedRtol;
+      break;
+    }
+    if (rn >= o.dtol * rnorm0) {
+      res.reason = kDivergedDtol;
+      break;
+    }
+    if (total >= o.maxit || k >= ir_max) {
+      res.reason = kDivergedIts;
+      break;
+    }
+    inner.maxit = o.maxit - total;
+    KResult r = krylov_solve<float>(A, ir.r32, ir.e32, inner, true, w32, s);
+    total += r.iters;
+    if (r.iters == 0 && r.reason < 0) {
+      res.reason = r.reason;
+      break;
+    }
+    hipLaunchKernelGGL(k_add_correction, dim3(g), dim3(kBlock), 0, s, n, x, ir.e32);
+    S355_LAUNCH_CHECK();
+  }
+  res.iters = total;
+  return res;
+}
+
 template <class T>
 double bench_spmv(const CsrView<T>& A, const T* x, T* y, KWork& w, int repeats, hipStream_t s) {
   hipLaunchKernelGGL(k_bench_init, dim3(1), dim3(1), 0, s, w.st);
-  const int g = grid_for(A.n);
+  const int g = spmv_grid(A.n);
   hipLaunchKernelGGL((k_spmv_pc<T>), dim3(g), dim3(kBlock), 0, s, A, x, static_cast<const T*>(nullptr), y, w.st, 0);
   S355_LAUNCH_CHECK();
   hipEvent_t e0, e1;
@@ -746,5 +1057,34 @@ void fill_zero(void* p, size_t bytes, hipStream_t s) {
 
 template double bench_spmv<double>(const CsrView<double>&, const double*, double*, KWork&, int, hipStream_t);
 template double bench_spmv<float>(const CsrView<float>&, const float*, float*, KWork&, int, hipStream_t);
+
+void ilu_levels(int n, const int* rowptr, const int* col, const int* diag, std::vector<int>& lrows,
+                std::vector<int>& lptr, std::vector<int>& urows, std::vector<int>& uptr) {
+  std::vector<int> lev(n, 0);
+  int nl = 0;
+  for (int i = 0; i < n; ++i) {
+    int l = 0;
+    for (int p = rowptr[i]; p < diag[i]; ++p) l = std::max(l, lev[col[p]] + 1);
+    lev[i] = l;
+    nl = std::max(nl, l + 1);
+  }
+  auto group = [&](int nlev, std::vector<int>& rows, std::vector<int>& ptr) {
+    ptr.assign(nlev + 1, 0);
+    for (int i = 0; i < n; ++i) ptr[lev[i] + 1]++;
+    for (int l = 0; l < nlev; ++l) ptr[l + 1] += ptr[l];
+    rows.assign(n, 0);
+    std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+    for (int i = 0; i < n; ++i) rows[pos[lev[i]]++] = i;
+  };
+  group(nl, lrows, lptr);
+  int nu = 0;
+  for (int i = n - 1; i >= 0; --i) {
+    int l = 0;
+    for (int p = diag[i] + 1; p < rowptr[i + 1]; ++p) l = std::max(l, lev[col[p]] + 1);
+    lev[i] = l;
+    nu = std::max(nu, l + 1);
+  }
+  group(nu, urows, uptr);
+}
 
 }  // namespace s355

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <vector>
 
 #include "../../../include/simple355.h"
 
@@ -46,14 +47,32 @@ struct CsrView {
   const int* col = nullptr;
   const int* diag = nullptr;
   const T* val = nullptr;
+  int nnz = 0;
 };
 
-// Workspace owned by the caller; sized by krylov_workspace_bytes.
+// Level schedule of the CSR pattern for ILU(0) (natural ordering): rows of a
+// level depend only on rows of earlier levels.  Lower levels order the
+// forward sweep and the factorisation, upper levels the backward sweep.
+struct IluLevels {
+  std::vector<int> lptr, uptr;  // host: level boundaries into lrows / urows
+  const int* lrows = nullptr;   // device: rows grouped by lower level
+  const int* urows = nullptr;   // device: rows grouped by upper level
+  int nl() const { return static_cast<int>(lptr.size()) - 1; }
+  int nu() const { return static_cast<int>(uptr.size()) - 1; }
+};
+
+// Host computation of the level schedule.
+void ilu_levels(int n, const int* rowptr, const int* col, const int* diag, std::vector<int>& lrows,
+                std::vector<int>& lptr, std::vector<int>& urows, std::vector<int>& uptr);
+
+// Workspace owned by the caller.
 struct KWork {
   int n = 0, mmax = 0;
   void* V = nullptr;        // (mmax + 1) * n elements of T
-  void* r = nullptr;        // n elements of T (CG: r, z, p, w use V[0..3])
+  void* r = nullptr;        // n elements of T: preconditioner input (CG: r, z, p, w use V[0..3])
   void* pcinv = nullptr;    // n elements of T (Jacobi inverse diagonal)
+  void* lu = nullptr;       // nnz elements of T (ILU(0) factors), when ilu is set
+  const IluLevels* ilu = nullptr;
   double* partial = nullptr;
   size_t partial_len = 0;
   KState* st = nullptr;     // device
@@ -63,7 +82,31 @@ struct KWork {
 struct KResult {
   int iters = 0, reason = 0;
   double rnorm = 0, rnorm0 = 0;
+  int refinements = 0;
 };
+
+// Buffers of the iterative-refinement solve (mixed precision).
+struct IrWork {
+  double* r64 = nullptr;  // FP64 residual
+  float* r32 = nullptr;   // FP32 copy of the residual (inner right-hand side)
+  float* e32 = nullptr;   // FP32 correction
+  float* inv32 = nullptr; // FP32 Jacobi inverse diagonal of the outer test
+  double* partial = nullptr;
+  size_t partial_len = 0;
+  double* scal = nullptr; // device scalars: [0] ||M^-1 r||^2, [1] ||M^-1 b||^2
+  double* host = nullptr; // pinned, 2 doubles
+};
+
+// FP64 iterative refinement around FP32 inner Krylov solves: the residual
+// r = b - A64 x with the FP64 matrix and the convergence test (PETSc's, on
+// ||M^-1 r||) are in FP64, x is FP64, each correction comes from an FP32
+// GMRES/CG solve of A32 e = r (A32 = A64 rounded) to relative tolerance
+// inner_rtol.  Taking the outer residual with A64 keeps the exact null space
+// of the pressure-correction matrix (A 1 = 0), which rounding breaks.  The
+// iteration cap counts inner iterations.
+KResult ir_solve(const CsrView<double>& A64, const CsrView<float>& A32, const double* b, double* x,
+                 const s355_ksp_opts& o, bool zero_guess, int ir_max, double inner_rtol, KWork& w32, IrWork& ir,
+                 hipStream_t s);
 
 // Solve A x = b.  x holds the initial guess unless zero_guess.
 template <class T>

@@ -81,7 +81,7 @@ def test_flux_pressure_matrix_and_corrections(name):
     assert_close(d, o, ["RUFACE"], 0.0)
 
 
-@pytest.mark.parametrize("pc", [s.PC_JACOBI, s.PC_NONE])
+@pytest.mark.parametrize("pc", [s.PC_JACOBI, s.PC_NONE, s.PC_ILU0])
 def test_velocity_solve_matches_oracle(pc):
     case = s.channel(10, 6, 8, jitter=0.1, seed=3)
     p = params(restart=20, pc=pc)
@@ -98,10 +98,11 @@ def test_velocity_solve_matches_oracle(pc):
     assert_close(d, o, ["U", "V", "W"], 1e-9)
 
 
-@pytest.mark.parametrize("ksp", [s.KSP_GMRES, s.KSP_CG])
-def test_pressure_solve_matches_oracle(ksp):
+@pytest.mark.parametrize("ksp,pc", [(s.KSP_GMRES, s.PC_JACOBI), (s.KSP_CG, s.PC_JACOBI), (s.KSP_GMRES, s.PC_ILU0),
+                                     (s.KSP_CG, s.PC_ILU0)])
+def test_pressure_solve_matches_oracle(ksp, pc):
     case = s.cavity(8, jitter=0.1, seed=5)
-    p = params(restart=30, ksp=ksp)
+    p = params(restart=30, ksp=ksp, pc=pc)
     d, o = pair(case, p)
     same_state(d, o, ["U", "V", "W", "P", "R", "APR", "DPDX", "RUFACE"], seed=7)
     for k in ("bc_velocity", "cal_ru_face", "build_pressure_matrix"):
