@@ -1,12 +1,24 @@
 #!/usr/bin/env python3
 """SIMPLE hot-path benchmark on one MI355X.
 
-Workload (default): the mixed-precision channel case — 200x100x200 box,
-15 % seeded node jitter, 10 % of the hexes split into 6 tetrahedra (about
-6.0 M cells), laminar channel at Re = 1000, GMRES(50) solves.  One "step" is
-one SIMPLE iteration (momentum assembly + 3 solves, Rhie-Chow flux,
-pressure-correction assembly + solve, corrections) with every field,
-matrix and Krylov vector resident in HBM.
+Default workload: the mixed-precision channel case — 200x100x200 box, 15 %
+seeded node jitter, 10 % of the hexes split into 6 tetrahedra (6.0 M cells),
+laminar channel at Re = 1000, GMRES(50) + Jacobi on every solve.  It runs the
+FP64 path and the mixed path (FP32 inner Krylov solves inside FP64 iterative
+refinement) on the same case from rest; `value` is the mixed path's SIMPLE
+iterations/s and the line also carries the FP64 rate, the speed-up and the
+per-field relative L2 difference between the two runs.
+
+One "step" is one SIMPLE iteration (momentum assembly + 3 solves, Rhie-Chow
+flux, pressure-correction assembly + solve, corrections, k-epsilon update for
+turbulent workloads) with every field, matrix and Krylov vector resident in
+HBM.
+
+Other workloads (--workload): `cavity` (32^3 lid-driven cavity, Re 100,
+GMRES(30)+Jacobi, 200 iterations, FP64), `step` (backward-facing step,
+1.12 M jittered tetrahedra, k-epsilon, Re 5000, FP64 with the reference's
+default solvers), `large` (400x200x250 jittered hex channel, 20 M cells,
+k-epsilon, mixed GMRES(50)+ILU(0)).
 
 Prints one JSON line: SIMPLE iterations/s, the roofline of the SpMV kernel
 (algorithmic bytes per launch / HIP-event-timed launch duration vs 8 TB/s),
@@ -25,73 +37,267 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "SIMPLE iters/s @4M cells, FP64 vs mixed (<=1e-6 rel err); SpMV % HBM peak"
+STAGES = ["total", "build_velocity", "solve_velocity", "build_pressure", "solve_pressure", "other"]
 
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def workload(size: str):
+# --------------------------------------------------------------------------- workloads
+
+def _channel_case(dims, jitter, split, re, turbulent=False):
     import sysu_cfd_parallel_amd as s
 
-    if size == "full":
-        dims = (200, 100, 200)
-    elif size == "medium":
-        dims = (100, 50, 100)
-    else:
-        dims = (40, 20, 40)
-    case = s.channel(*dims, lx=2.0, ly=1.0, lz=2.0, jitter=0.15, split=0.10, seed=0, re=1000.0)
-    return case, dims
+    return s.channel(*dims, lx=2.0, ly=1.0, lz=2.0, jitter=jitter, split=split, seed=0, re=re,
+                     turbulent=turbulent)
 
 
-def solver_params(precision: int, restart: int = 50, maxit: int = 20000):
+def _gmres_all(p, restart, pc, maxit=20000, rtol=None):
     import sysu_cfd_parallel_amd as s
 
-    p = s.default_params()
-    # GMRES(m) + Jacobi on every solve (the unprefixed KSPSetFromOptions of the
-    # reference applies one KSP setting to all solves); in-code tolerances
-    # 1e-8 (velocity) and 1e-6 (pressure); the iteration cap raised as a
-    # command-line -ksp_max_it would.
+    # one KSP setting on every solve, as the reference's unprefixed
+    # KSPSetFromOptions applies command-line options to all of them; the
+    # iteration cap raised as a command-line -ksp_max_it would
     for k in range(3):
         o = p.ksp[k]
-        o.type, o.restart, o.pc, o.maxit = s.KSP_GMRES, restart, s.PC_JACOBI, maxit
-    p.residual_steady = 0.0
-    p.precision = precision
+        o.type, o.restart, o.pc, o.maxit = s.KSP_GMRES, restart, pc, maxit
+        if rtol is not None:
+            o.rtol = rtol
     return p
 
+
+class Channel:
+    name = "channel"
+    sizes = {"full": (200, 100, 200), "medium": (100, 50, 100), "small": (40, 20, 40)}
+    precisions = ["fp64", "mixed"]
+    steps, warmup = 5, 1
+    turbulent = False
+
+    def __init__(self, size):
+        self.dims = self.sizes[size]
+
+    def case(self, dims=None):
+        return _channel_case(dims or self.dims, 0.15, 0.10, 1000.0)
+
+    def params(self, prec):
+        import sysu_cfd_parallel_amd as s
+
+        p = _gmres_all(s.default_params(), 50, s.PC_JACOBI)
+        p.residual_steady = 0.0
+        p.precision = prec
+        return p
+
+    def describe(self, prec_name):
+        d = self.dims
+        return (f"laminar channel Re=1000, {d[0]}x{d[1]}x{d[2]} box, 15% jitter, 10% hexes split into 6 tets, "
+                f"GMRES(50)+Jacobi, " + ("FP32 inner solves in FP64 refinement" if prec_name == "mixed" else "FP64"))
+
+    def cpu_sample(self):
+        # 1/8 of the cells: the full mesh takes minutes per CPU step
+        dims = self.sizes["medium"] if self.dims == self.sizes["full"] else self.dims
+        return self.case(dims), "channel %dx%dx%d, same jitter/split/Re/solver settings" % dims
+
+
+class Cavity(Channel):
+    name = "cavity"
+    sizes = {"full": 32, "medium": 32, "small": 16}
+    precisions = ["fp64"]
+    steps, warmup = 200, 0
+
+    def case(self, dims=None):
+        import sysu_cfd_parallel_amd as s
+
+        return s.cavity(dims or self.dims, re=100.0)
+
+    def params(self, prec):
+        import sysu_cfd_parallel_amd as s
+
+        p = _gmres_all(s.default_params(), 30, s.PC_JACOBI, maxit=1000, rtol=1e-8)
+        p.ksp[s.KSP_SCALAR].maxit = 500
+        p.residual_steady = 0.0
+        p.precision = prec
+        return p
+
+    def describe(self, prec_name):
+        n = self.dims
+        return f"lid-driven cavity Re=100, {n}^3 uniform hexes, GMRES(30)+Jacobi rtol 1e-8, FP64"
+
+    def cpu_sample(self):
+        return self.case(), "the same cavity"
+
+
+class Step(Channel):
+    name = "step"
+    sizes = {"full": 9, "medium": 6, "small": 3}
+    precisions = ["fp64"]
+    steps, warmup = 5, 1
+    turbulent = True
+
+    def case(self, dims=None):
+        import sysu_cfd_parallel_amd as s
+
+        return s.step(dims or self.dims, jitter=0.10, tets=True, turbulent=True, re=5000.0)
+
+    def params(self, prec):
+        import sysu_cfd_parallel_amd as s
+
+        # the reference's solver defaults: GMRES(30)+ILU(0) for momentum and
+        # k/epsilon, CG+ILU(0) for the symmetric pressure correction
+        p = s.default_params()
+        p.turb_model = 1
+        p.residual_steady = 0.0
+        p.precision = prec
+        return p
+
+    def describe(self, prec_name):
+        n = self.dims
+        return (f"backward-facing step ER=2, {n} cells per step height, jittered (10%) tetrahedra, "
+                f"k-epsilon with wall functions, Re=5000, GMRES(30)/CG + ILU(0), FP64")
+
+    def cpu_sample(self):
+        return self.case(), "the same step case"
+
+
+class Large(Channel):
+    name = "large"
+    sizes = {"full": (400, 200, 250), "medium": (200, 100, 125), "small": (40, 20, 25)}
+    precisions = ["mixed"]
+    steps, warmup = 3, 1
+    turbulent = True
+
+    def case(self, dims=None):
+        return _channel_case(dims or self.dims, 0.10, 0.0, 13750.0, turbulent=True)
+
+    def params(self, prec):
+        import sysu_cfd_parallel_amd as s
+
+        p = _gmres_all(s.default_params(), 50, s.PC_ILU0)
+        p.turb_model = 1
+        p.residual_steady = 0.0
+        p.precision = prec
+        return p
+
+    def describe(self, prec_name):
+        d = self.dims
+        return (f"turbulent channel (k-epsilon, Re_b=13750 ~ Re_tau 395), {d[0]}x{d[1]}x{d[2]} hexes, 10% jitter, "
+                f"GMRES(50)+ILU(0), " + ("FP32 inner solves in FP64 refinement" if prec_name == "mixed" else "FP64"))
+
+    def cpu_sample(self):
+        dims = (100, 50, 62) if self.dims == self.sizes["full"] else self.dims
+        return self.case(dims), "turbulent channel %dx%dx%d, same jitter/Re/solver settings" % dims
+
+
+WORKLOADS = {w.name: w for w in (Channel, Cavity, Step, Large)}
+
+
+# --------------------------------------------------------------------------- measurement
 
 def spmv_bytes(nnz: int, nc: int, b_val: int, b_x: int) -> float:
     # algorithmic bytes of y = A x on CSR: values + column ids, row pointers, x and y
     return (b_val + 4) * nnz + 4 * (nc + 1) + 2 * b_x * nc
 
 
-def cpu_baseline(threads: int, sample_dims=(40, 20, 40), budget_s: float = 25.0):
-    """FP64 CPU oracle on a bounded sample of the same workload."""
-    import sysu_cfd_parallel_amd as s
+def cpu_baseline(wl, threads: int, budget_s: float = 20.0, max_steps: int = 3) -> dict:
+    """FP64 CPU oracle on a bounded sample of the same workload, started from
+    rest like the GPU run and stepped until the budget is spent.  Where the
+    sample is smaller than the GPU case, Krylov iteration counts are lower on
+    the sample, so its cells*iters/s overstates the CPU rate on the full mesh:
+    the scaled figure is an upper bound for the CPU."""
     from oracle.pyoracle import Oracle
 
-    case = s.channel(*sample_dims, lx=2.0, ly=1.0, lz=2.0, jitter=0.15, split=0.10, seed=0, re=1000.0)
-    p = solver_params(0)
-    o = Oracle(case, p, threads=threads)
+    case, what = wl.cpu_sample()
+    o = Oracle(case, wl.params(0), threads=threads)
     n = o.sizes.n_cell
-    t0 = time.perf_counter()
     steps = 0
-    while steps < 50:
-        _, done, rc = o.iterate(1)
+    t0 = time.perf_counter()
+    while steps < max_steps:
+        o.iterate(1)
         steps += 1
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
-    return {"iters_per_s": steps / dt, "cells": n, "steps": steps, "seconds": dt, "dims": list(sample_dims)}
+    o.close()
+    return {"iters_per_s": steps / dt, "cells": n, "steps": steps, "seconds": dt, "what": what}
+
+
+def run_path(wl, case, prec: int, steps: int, warmup: int, label: str) -> dict:
+    """W untimed + K timed SIMPLE steps of one precision path from rest."""
+    import numpy as np
+
+    import sysu_cfd_parallel_amd as s
+
+    t = time.perf_counter()
+    dev = s.Solver(case, wl.params(prec))
+    z = dev.sizes
+    log(f"[{label}] device context: {z.n_cell} cells, {z.n_face} faces, nnz {z.nnz} "
+        f"({time.perf_counter() - t:.1f}s)")
+    for k in range(warmup):
+        t = time.perf_counter()
+        h, _ = dev.iterate(1)
+        log(f"[{label}] warmup step {k + 1}/{warmup}: {time.perf_counter() - t:.2f}s, ResMax {h[-1, 10]:.3e}")
+    dev.call("synchronize")
+    dev.stage_times(reset=True)
+    names = ("U", "V", "W", "P", "TE", "ED") if wl.turbulent else ("U", "V", "W", "P")
+    rows, its = [], {n: 0 for n in names}
+    refinements = 0
+    quiet = steps > 20
+    t0 = time.perf_counter()
+    for k in range(steps):
+        t = time.perf_counter()
+        h, _ = dev.iterate(1)
+        rows.append(h[-1])
+        info = dev.last_solve_info()
+        for n in its:
+            its[n] += info[n]["iters"]
+        refinements += sum(info[n]["refinements"] for n in its)
+        if not quiet or (k + 1) % 25 == 0 or k + 1 == steps:
+            log(f"[{label}] step {k + 1}/{steps}: {time.perf_counter() - t:.3f}s, ResMax {h[-1, 10]:.3e}, "
+                f"krylov its U {info['U']['iters']} P {info['P']['iters']}")
+    dev.call("synchronize")
+    t1 = time.perf_counter()
+    stages = dev.stage_times()
+    out = {
+        "dev": dev, "sizes": z, "seconds": t1 - t0, "history": np.array(rows),
+        "fields": {n: dev.get(n) for n in names},
+        "krylov_its_per_step": {n: v / steps for n, v in its.items()},
+        "refinements_per_step": refinements / steps,
+        "stage_ms_per_step": {k: v / steps for k, v in zip(STAGES, stages.tolist())},
+    }
+    log(f"[{label}] {steps / (t1 - t0):.3f} SIMPLE it/s, {(t1 - t0) * 1e3 / steps:.1f} ms/step")
+    return out
+
+
+def l2_rel(a, b) -> float:
+    import numpy as np
+
+    return float(np.linalg.norm(a - b) / max(float(np.linalg.norm(b)), 1e-300))
+
+
+def spmv_roofline(dev, z, reps: int = 50) -> dict:
+    """Production SpMV kernel on the assembled pressure-correction matrix."""
+    out = {}
+    for name, prec, b in (("fp64", 0, 8), ("fp32", 1, 4)):
+        ms = dev.spmv_time(1, prec, reps)
+        nb = spmv_bytes(z.nnz, z.n_cell, b, b)
+        out[name] = {"kernel": "k_spmv_pc<%s>" % ("double" if b == 8 else "float"), "bytes_per_launch": nb,
+                     "ms_per_launch": ms, "achieved": nb / (ms * 1e-3) / 1e9}
+        out[name]["frac"] = out[name]["achieved"] / HBM_PEAK_GBS
+    log(f"SpMV fp64 {out['fp64']['ms_per_launch'] * 1e3:.1f} us -> {out['fp64']['achieved']:.0f} GB/s; "
+        f"fp32 {out['fp32']['ms_per_launch'] * 1e3:.1f} us -> {out['fp32']['achieved']:.0f} GB/s")
+    return out
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="channel")
     ap.add_argument("--size", choices=["full", "medium", "small"], default="full")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["default", "both", "fp64", "mixed"], default="default",
+                    help="both: FP64 then mixed on the same case, value is the mixed rate")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
@@ -100,96 +306,96 @@ def main() -> int:
         log("only --gpus 1 is supported: every case runs on a single MI355X")
         return 2
 
-    import numpy as np
-
     import sysu_cfd_parallel_amd as s
 
-    prec = s.MIXED if args.precision == "mixed" else s.FP64
-    t = time.perf_counter()
-    case, dims = workload(args.size)
-    log(f"case {dims} built in {time.perf_counter() - t:.1f}s ({case.n_cell_elems} volume elements)")
-    t = time.perf_counter()
-    dev = s.Solver(case, solver_params(prec))
-    z = dev.sizes
-    log(f"device context: {z.n_cell} cells, {z.n_face} faces, nnz {z.nnz} ({time.perf_counter() - t:.1f}s)")
+    wl = WORKLOADS[args.workload](args.size)
+    steps = args.steps if args.steps is not None else wl.steps
+    warmup = args.warmup if args.warmup is not None else wl.warmup
+    order = {"default": wl.precisions, "both": ["fp64", "mixed"], "fp64": ["fp64"], "mixed": ["mixed"]}[args.precision]
 
-    for k in range(args.warmup):
-        t = time.perf_counter()
-        h, _ = dev.iterate(1)
-        log(f"warmup step {k + 1}/{args.warmup}: {time.perf_counter() - t:.2f}s, ResMax {h[-1, 10]:.3e}")
-    dev.call("synchronize")
-    dev.stage_times(reset=True)
-    rows = []
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        t = time.perf_counter()
-        h, _ = dev.iterate(1)
-        rows.append(h[-1])
-        log(f"step {k + 1}/{args.steps}: {time.perf_counter() - t:.2f}s, ResMax {h[-1, 10]:.3e}")
-    dev.call("synchronize")
-    t1 = time.perf_counter()
-    hist = np.array(rows)
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    stages = dev.stage_times()
-    value = args.steps / (t1 - t0)
-    log(f"{value:.3f} SIMPLE it/s, {ms_per_step:.1f} ms/step, residual {hist[-1, 10]:.3e}")
+    t = time.perf_counter()
+    case = wl.case()
+    log(f"{wl.name} case built in {time.perf_counter() - t:.1f}s ({case.n_cell_elems} volume elements)")
 
-    # SpMV roofline: the production kernel on the assembled pressure matrix
-    reps = 50
-    ms_spmv = dev.spmv_time(1, 0, reps)
-    b = spmv_bytes(z.nnz, z.n_cell, 8, 8)
-    achieved = b / (ms_spmv * 1e-3) / 1e9
-    ms_spmv32 = dev.spmv_time(1, 1, reps)
-    b32 = spmv_bytes(z.nnz, z.n_cell, 4, 4)
-    achieved32 = b32 / (ms_spmv32 * 1e-3) / 1e9
-    log(f"SpMV fp64 {ms_spmv * 1e3:.1f} us -> {achieved:.0f} GB/s; fp32 {ms_spmv32 * 1e3:.1f} us -> {achieved32:.0f} GB/s")
+    runs = {}
+    for name in order:
+        runs[name] = run_path(wl, case, s.MIXED if name == "mixed" else s.FP64, steps, warmup, name)
+        if name != order[-1]:
+            runs[name]["dev"].close()
+    main_name = order[-1]
+    r = runs[main_name]
+    dev, z = r["dev"], r["sizes"]
+    value = steps / r["seconds"]
+
+    agreement = None
+    if "fp64" in runs and "mixed" in runs:
+        a, b = runs["mixed"], runs["fp64"]
+        agreement = {n: l2_rel(a["fields"][n], b["fields"][n]) for n in a["fields"]}
+        agreement["max"] = max(agreement.values())
+        agreement["residual_history_max_rel"] = float(
+            (abs(a["history"][:, :4] - b["history"][:, :4]) / abs(b["history"][:, :4]).clip(1e-300)).max())
+        log(f"mixed vs FP64 relative L2 field difference: {agreement}")
+
+    spmv = spmv_roofline(dev, z)
+    dom = spmv["fp32" if main_name == "mixed" else "fp64"]
+    copy_bytes = 1 << 30
+    ms_copy = dev.copy_time(copy_bytes, 20)
+    copy_gbs = 2 * copy_bytes / (ms_copy * 1e-3) / 1e9
+    log(f"plain device copy of 1 GiB: {ms_copy:.3f} ms -> {copy_gbs:.0f} GB/s read+write")
+    dev.close()
 
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"CPU oracle baseline on {threads} threads")
-        cb = cpu_baseline(threads)
+        cb = cpu_baseline(wl, threads)
         cpu = {
             "value": cb["iters_per_s"] * cb["cells"] / z.n_cell,
-            "unit": "SIMPLE iters/s (cells*iters/s of the sample scaled to the full mesh)",
+            "unit": "SIMPLE iterations/s (FP64 oracle; sample cells*iters/s divided by the GPU case's cells)",
             "cores": threads,
             "kind": "port",
-            "sample": f"channel {cb['dims'][0]}x{cb['dims'][1]}x{cb['dims'][2]} same jitter/split/Re, "
-                      f"{cb['cells']} cells, {cb['steps']} SIMPLE steps in {cb['seconds']:.1f}s "
-                      f"({cb['iters_per_s']:.3f} it/s on the sample)",
+            "sample": f"{cb['what']}, {cb['cells']} cells, first {cb['steps']} SIMPLE steps from rest in "
+                      f"{cb['seconds']:.1f}s ({cb['iters_per_s']:.4f} it/s on the sample)",
         }
 
     out = {
-        "metric": "SIMPLE iters/s @4M cells, FP64 vs mixed (<=1e-6 rel err); SpMV % HBM peak",
+        "metric": METRIC,
         "value": value,
         "unit": "SIMPLE iterations/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": r["seconds"] * 1e3 / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == s.FP64 else "f32/f64-mixed",
+        "dtype": "f32+f64" if main_name == "mixed" else "f64",
         "data": "synthetic (seeded jittered hex/tet mesh generated in-process)",
         "config": {
-            "workload": f"laminar channel Re=1000, {dims[0]}x{dims[1]}x{dims[2]} box, 15% jitter, 10% hexes "
-                        f"split into 6 tets, GMRES(50)+Jacobi",
+            "workload": wl.describe(main_name),
             "cells": z.n_cell, "faces": z.n_face, "nnz": z.nnz,
             "cells_iters_per_s": value * z.n_cell,
-            "precision": args.precision,
+            "precision": main_name,
         },
         "roofline": {
-            "bound": "hbm", "kernel": "k_spmv_pc<double> on the pressure matrix",
-            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None, "bytes_per_launch": b, "ms_per_launch": ms_spmv,
+            "bound": "hbm", "kernel": dom["kernel"] + " on the pressure-correction matrix",
+            "achieved": dom["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dom["frac"],
+            "traffic": None, "bytes_per_launch": dom["bytes_per_launch"], "ms_per_launch": dom["ms_per_launch"],
+            "measured_copy_gbs": copy_gbs, "frac_of_measured_copy": dom["achieved"] / copy_gbs,
         },
-        "spmv_fp32": {"achieved": achieved32, "frac": achieved32 / HBM_PEAK_GBS, "ms_per_launch": ms_spmv32},
-        "stage_ms_per_step": {k: v / args.steps for k, v in
-                              zip(["total", "build_velocity", "solve_velocity", "build_pressure", "solve_pressure",
-                                   "other"], stages.tolist())},
+        "spmv": spmv,
+        "paths": {
+            n: {"iters_per_s": steps / v["seconds"], "ms_per_step": v["seconds"] * 1e3 / steps,
+                "krylov_its_per_step": v["krylov_its_per_step"], "refinements_per_step": v["refinements_per_step"],
+                "stage_ms_per_step": v["stage_ms_per_step"],
+                "residual_history_last": v["history"][-1, [0, 1, 2, 3, 10]].tolist()}
+            for n, v in runs.items()
+        },
+        "mixed_vs_fp64": agreement,
         "cpu_baseline": cpu,
     }
+    if "fp64" in runs and "mixed" in runs:
+        out["mixed_speedup_vs_fp64"] = runs["fp64"]["seconds"] / runs["mixed"]["seconds"]
     print(json.dumps(out), flush=True)
     return 0
 

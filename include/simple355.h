@@ -265,6 +265,11 @@ int s355_simple_iterate(s355_ctx* ctx, int n_steps, double* history, int* steps_
  * device time over `repeats` launches (HIP events on the context stream). */
 int s355_spmv_time(s355_ctx* ctx, int which, int precision, int repeats, double* ms_per_call);
 
+/* Mean device time of a plain device-to-device copy of `bytes` (a multiple
+ * of 16) over `repeats` launches: the bandwidth reference for the roofline
+ * and for scaling hardware byte counters.  No reference counterpart. */
+int s355_copy_time(s355_ctx* ctx, size_t bytes, int repeats, double* ms_per_call);
+
 /* Host-side set-up only (no device): the integer and metric output of the
  * face builder, for inspection and for tests on machines without a GPU. */
 typedef struct s355_mesh s355_mesh;

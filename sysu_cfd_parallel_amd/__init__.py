@@ -87,6 +87,7 @@ def library(path: Optional[Path] = None) -> C.CDLL:
         "update_tur_kepsilon": (I, [V, C.POINTER(SolveInfo)]),
         "spmv_time": (I, [V, I, I, I, pD]),
         "last_solve_info": (I, [V, C.POINTER(SolveInfo)]),
+        "copy_time": (I, [V, C.c_size_t, I, pD]),
     }
     for name, (res, args) in more.items():
         fn = getattr(lib, f"s355_{name}")
@@ -337,6 +338,12 @@ class Solver:
     def spmv_time(self, which: int, precision: int, repeats: int) -> float:
         ms = C.c_double()
         _check(self.lib, self.lib.s355_spmv_time(self.ctx, which, precision, repeats, C.byref(ms)), "spmv_time")
+        return ms.value
+
+    def copy_time(self, nbytes: int, repeats: int) -> float:
+        """Mean device ms of a plain copy of nbytes (multiple of 16)."""
+        ms = C.c_double()
+        _check(self.lib, self.lib.s355_copy_time(self.ctx, nbytes, repeats, C.byref(ms)), "copy_time")
         return ms.value
 
     def close(self) -> None:

@@ -518,4 +518,12 @@ int s355_spmv_time(s355_ctx* ctx, int which, int precision, int repeats, double*
   });
 }
 
+int s355_copy_time(s355_ctx* ctx, size_t bytes, int repeats, double* ms_per_call) {
+  return guarded([&] {
+    if (!ms_per_call) throw std::invalid_argument("null output");
+    *ms_per_call = dev(ctx).time_copy(bytes, repeats);
+    return S355_OK;
+  });
+}
+
 }  // extern "C"

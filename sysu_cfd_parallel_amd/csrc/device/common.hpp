@@ -31,6 +31,18 @@ inline int grid_for(long n, int block = kBlock) {
   return static_cast<int>(g < 1 ? 1 : g);
 }
 
+// Workgroups of a launch are dispatched to the 8 XCDs round-robin (workgroup
+// b runs on XCD b % 8).  xcd_block(b, G) renumbers them so that each XCD
+// processes one contiguous range of logical blocks: for gather kernels its L2
+// then holds a contiguous slice of the rows' neighbourhoods instead of every
+// eighth block of the whole range.  A bijection on [0, G).
+constexpr int kXcd = 8;
+__device__ __forceinline__ int xcd_block(int b, int G) {
+  const int q = G / kXcd;
+  if (b >= q * kXcd) return b;
+  return (b % kXcd) * q + b / kXcd;
+}
+
 // Butterfly sum over the 64 lanes of a wavefront; every lane ends with the
 // same value, produced by the same fixed pairing order.
 template <class T>

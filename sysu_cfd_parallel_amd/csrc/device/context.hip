@@ -537,6 +537,31 @@ double Context::time_spmv(int which, int precision, int repeats) {
   return bench_spmv<float>(A, xf, yf, kwork_, repeats, stream);
 }
 
+double Context::time_copy(size_t bytes, int repeats) {
+  if (bytes == 0 || bytes % 16) throw std::invalid_argument("copy size must be a positive multiple of 16 bytes");
+  if (repeats < 1) repeats = 1;
+  void *a = nullptr, *b = nullptr;
+  S355_HIP(hipMalloc(&a, bytes));
+  if (hipMalloc(&b, bytes) != hipSuccess) {
+    hipFree(a);
+    throw HipError("time_copy: out of device memory");
+  }
+  double ms = 0.0;
+  try {
+    fill_zero(a, bytes, stream);
+    ms = bench_copy(b, a, bytes, repeats, stream);
+  } catch (...) {
+    hipStreamSynchronize(stream);
+    hipFree(a);
+    hipFree(b);
+    throw;
+  }
+  S355_HIP(hipStreamSynchronize(stream));
+  hipFree(a);
+  hipFree(b);
+  return ms;
+}
+
 hipEvent_t Context::take_event() {
   if (!pool_.empty()) {
     hipEvent_t e = pool_.back();

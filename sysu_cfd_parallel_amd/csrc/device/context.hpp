@@ -70,6 +70,7 @@ class Context {
 
   // benchmark helper
   double time_spmv(int which, int precision, int repeats);
+  double time_copy(size_t bytes, int repeats);
 
  private:
   template <class T>

@@ -11,6 +11,7 @@
 // only every few iterations.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "common.hpp"
 #include "krylov.hpp"
@@ -166,7 +167,7 @@ template <class T>
 __global__ void __launch_bounds__(kBlock) k_spmv_pc(CsrView<T> A, const T* __restrict__ x, const T* __restrict__ inv,
                                                     T* __restrict__ y, const KState* st, int it) {
   if (st->reason) return;
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int row = g / kRowLanes, lane = g % kRowLanes;
   const T s = row_dot(A, x, row, lane);
   if (lane == 0 && row < A.n) {
@@ -364,8 +365,8 @@ __global__ void __launch_bounds__(kWave) k_build_y(KState* st, int m, int* nupd)
   if (lane == 0) *nupd = k + 1;
 }
 
-template <class T, int R>
-__global__ void __launch_bounds__(kBlock) k_update_x(const T* __restrict__ V, size_t ld, int n, T* __restrict__ x,
+template <class T, class X, int R>
+__global__ void __launch_bounds__(kBlock) k_update_x(const T* __restrict__ V, size_t ld, int n, X* __restrict__ x,
                                                      const KState* st, const int* nupd) {
   __shared__ double y[kMaxRestart + 1];
   const int nk = *nupd;
@@ -379,15 +380,18 @@ __global__ void __launch_bounds__(kBlock) k_update_x(const T* __restrict__ V, si
     if (idx < static_cast<size_t>(n)) {
       double s = 0.0;
       for (int k = 0; k < nk; ++k) s += y[k] * static_cast<double>(V[static_cast<size_t>(k) * ld + idx]);
-      x[idx] = static_cast<T>(static_cast<double>(x[idx]) + s);
+      x[idx] = static_cast<X>(static_cast<double>(x[idx]) + s);
     }
   }
 }
 
-// ---- v0 = M^-1 (b - A x); partial ||v0||^2
-template <class T>
-__global__ void __launch_bounds__(kBlock) k_residual_pc(CsrView<T> A, const T* __restrict__ b,
-                                                        const T* __restrict__ x, const T* __restrict__ inv,
+// ---- v0 = M^-1 (b - A x); partial ||v0||^2.  The residual is formed in the
+// precision X of the solution and the matrix A (FP64 for the mixed solve,
+// whose Arnoldi cycle runs on an FP32 copy); v0 is stored in the basis type T
+// and the norm is taken before that rounding when T and X differ.
+template <class T, class X>
+__global__ void __launch_bounds__(kBlock) k_residual_pc(CsrView<X> A, const X* __restrict__ b,
+                                                        const X* __restrict__ x, const X* __restrict__ inv,
                                                         T* __restrict__ v0, double* partial, int zero_guess) {
   __shared__ double lds[kBlock / kWave];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -395,18 +399,33 @@ __global__ void __launch_bounds__(kBlock) k_residual_pc(CsrView<T> A, const T* _
   if (i < A.n) {
     double r = static_cast<double>(b[i]);
     if (!zero_guess) {
-      T s = T(0);
+      X s = X(0);
       const int p1 = A.rowptr[i + 1];
       for (int p = A.rowptr[i]; p < p1; ++p) s += A.val[p] * x[A.col[p]];
       r -= static_cast<double>(s);
     }
     if (inv) r *= static_cast<double>(inv[i]);
-    const T rv = static_cast<T>(r);
-    v0[i] = rv;
-    nrm = static_cast<double>(rv) * static_cast<double>(rv);
+    const X rx = static_cast<X>(r);
+    v0[i] = static_cast<T>(rx);
+    nrm = static_cast<double>(rx) * static_cast<double>(rx);
   }
   nrm = block_sum(nrm, lds);
   if (threadIdx.x == 0) partial[blockIdx.x] = nrm;
+}
+
+// ---- Jacobi inverses of the mixed solve: FP64 for the restart residual,
+// FP32 for the Arnoldi cycle, both from the FP64 diagonal
+__global__ void __launch_bounds__(kBlock) k_jacobi_mixed(CsrView<double> A, double* inv64, float* inv32) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  const double d = A.val[A.diag[i]];
+  const double v = d != 0.0 ? 1.0 / d : 1.0;
+  inv64[i] = v;
+  inv32[i] = static_cast<float>(v);
+}
+
+__global__ void k_clear_reason(KState* st) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) st->reason = 0;
 }
 
 // ---- partial ||M^-1 b||^2 (reference norm for a nonzero initial guess)
@@ -621,7 +640,12 @@ __global__ void __launch_bounds__(kBlock) k_residual64(CsrView<T> A, const doubl
   r[i] = b[i] - s;
 }
 
-int pick_r(int n) {
+// Rows per thread of the multi-dot, Gram-Schmidt and update kernels: enough
+// that each block's fixed reduction work is amortised over its loads, while
+// the grid keeps several blocks per CU.  FP32 halves the bytes per row, so it
+// takes twice the rows per thread on large systems.
+int pick_r(int n, size_t elem) {
+  if (elem == 4 && n >= 16 * kBlock * 1024) return 16;
   if (n >= 8 * kBlock * 1024) return 8;
   if (n >= 4 * kBlock * 1024) return 4;
   if (n >= 2 * kBlock * 1024) return 2;
@@ -638,10 +662,10 @@ void launch_maxpy(T* V, size_t ld, int n, int it, double* partial, const KState*
   const int g = grid_for(n, kBlock * R);
   hipLaunchKernelGGL((k_maxpy_norm<T, R>), dim3(g), dim3(kBlock), 0, s, V, ld, n, it, partial, st);
 }
-template <class T, int R>
-void launch_update(const T* V, size_t ld, int n, T* x, const KState* st, const int* nupd, hipStream_t s) {
+template <class T, class X, int R>
+void launch_update(const T* V, size_t ld, int n, X* x, const KState* st, const int* nupd, hipStream_t s) {
   const int g = grid_for(n, kBlock * R);
-  hipLaunchKernelGGL((k_update_x<T, R>), dim3(g), dim3(kBlock), 0, s, V, ld, n, x, st, nupd);
+  hipLaunchKernelGGL((k_update_x<T, X, R>), dim3(g), dim3(kBlock), 0, s, V, ld, n, x, st, nupd);
 }
 
 template <class T>
@@ -649,6 +673,7 @@ struct Dispatch {
   static void mdot(int R, const T* V, size_t ld, int n, int it, double* partial, int pstride, const KState* st,
                    hipStream_t s) {
     switch (R) {
+      case 16: launch_mdot<T, 16>(V, ld, n, it, partial, pstride, st, s); break;
       case 8: launch_mdot<T, 8>(V, ld, n, it, partial, pstride, st, s); break;
       case 4: launch_mdot<T, 4>(V, ld, n, it, partial, pstride, st, s); break;
       case 2: launch_mdot<T, 2>(V, ld, n, it, partial, pstride, st, s); break;
@@ -657,18 +682,21 @@ struct Dispatch {
   }
   static void maxpy(int R, T* V, size_t ld, int n, int it, double* partial, const KState* st, hipStream_t s) {
     switch (R) {
+      case 16: launch_maxpy<T, 16>(V, ld, n, it, partial, st, s); break;
       case 8: launch_maxpy<T, 8>(V, ld, n, it, partial, st, s); break;
       case 4: launch_maxpy<T, 4>(V, ld, n, it, partial, st, s); break;
       case 2: launch_maxpy<T, 2>(V, ld, n, it, partial, st, s); break;
       default: launch_maxpy<T, 1>(V, ld, n, it, partial, st, s); break;
     }
   }
-  static void update(int R, const T* V, size_t ld, int n, T* x, const KState* st, const int* nupd, hipStream_t s) {
+  template <class X>
+  static void update(int R, const T* V, size_t ld, int n, X* x, const KState* st, const int* nupd, hipStream_t s) {
     switch (R) {
-      case 8: launch_update<T, 8>(V, ld, n, x, st, nupd, s); break;
-      case 4: launch_update<T, 4>(V, ld, n, x, st, nupd, s); break;
-      case 2: launch_update<T, 2>(V, ld, n, x, st, nupd, s); break;
-      default: launch_update<T, 1>(V, ld, n, x, st, nupd, s); break;
+      case 16: launch_update<T, X, 16>(V, ld, n, x, st, nupd, s); break;
+      case 8: launch_update<T, X, 8>(V, ld, n, x, st, nupd, s); break;
+      case 4: launch_update<T, X, 4>(V, ld, n, x, st, nupd, s); break;
+      case 2: launch_update<T, X, 2>(V, ld, n, x, st, nupd, s); break;
+      default: launch_update<T, X, 1>(V, ld, n, x, st, nupd, s); break;
     }
   }
 };
@@ -692,20 +720,39 @@ void init_state(KWork& w, const s355_ksp_opts& o, hipStream_t s) {
   S355_HIP(hipStreamSynchronize(s));
 }
 
-template <class T>
-KResult gmres(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, bool zero_guess, KWork& w,
-              hipStream_t s) {
+// Restarted GMRES(m).  The Arnoldi cycle runs on A (values and basis of type
+// T); each restart forms the residual b - Ares x in the solution type X.  With
+// T = X and Ares = A this is the plain solve.  With T = float, X = double it
+// is the mixed solve: every cycle is one step of FP64 iterative refinement
+// (FP64 residual with the FP64 matrix, FP32 Krylov correction accumulated
+// into the FP64 x), so refinement costs no restarts beyond GMRES(m)'s own,
+// and convergence is accepted only on the FP64 residual: a cycle whose
+// Givens estimate passes the test ends early and the restart residual
+// decides.  inv_x (mixed Jacobi only) holds the FP64 inverse diagonal.
+template <class T, class X>
+KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, const s355_ksp_opts& o, bool zero_guess,
+              KWork& w, X* inv_x, hipStream_t s) {
+  constexpr bool mixed = !std::is_same<T, X>::value;
   const int n = A.n;
   const int m = o.restart < 1 ? 1 : (o.restart > kMaxRestart ? kMaxRestart : o.restart);
   if (m > w.mmax) throw std::invalid_argument("gmres: restart above workspace size");
+  if (Ares.n != n) throw std::invalid_argument("gmres: operator sizes differ");
   const size_t ld = static_cast<size_t>(n);
   T* V = static_cast<T*>(w.V);
   T* inv = nullptr;
+  const X* invr = nullptr;
   T* lu = nullptr;
   T* tmp = static_cast<T*>(w.r);
   if (o.pc == S355_PC_JACOBI) {
     inv = static_cast<T*>(w.pcinv);
-    hipLaunchKernelGGL((k_jacobi_setup<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, A, inv);
+    if constexpr (mixed) {
+      if (!inv_x) throw std::invalid_argument("gmres: mixed Jacobi needs an FP64 inverse buffer");
+      hipLaunchKernelGGL(k_jacobi_mixed, dim3(grid_for(n)), dim3(kBlock), 0, s, Ares, inv_x, inv);
+      invr = inv_x;
+    } else {
+      hipLaunchKernelGGL((k_jacobi_setup<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, A, inv);
+      invr = inv;
+    }
   } else if (o.pc == S355_PC_ILU0) {
     if (!w.ilu || !w.lu || !w.r) throw std::invalid_argument("gmres: ILU(0) workspace missing");
     lu = static_cast<T*>(w.lu);
@@ -715,7 +762,7 @@ KResult gmres(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, boo
   }
   init_state(w, o, s);
   const int g1 = grid_for(n);
-  const int R = pick_r(n);
+  const int R = pick_r(n, sizeof(T));
   const int gR = grid_for(n, kBlock * R);
   const int pstride = m + 1;
   if (static_cast<size_t>(gR) * pstride > w.partial_len || static_cast<size_t>(g1) > w.partial_len)
@@ -723,37 +770,53 @@ KResult gmres(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, boo
   int* nupd = reinterpret_cast<int*>(&w.st->pad);
   if (!zero_guess) {
     if (lu) {
-      ilu_apply(A, *w.ilu, lu, b, V, w.st, s);
+      const T* bt;
+      if constexpr (mixed) {
+        convert_to_float(b, tmp, ld, s);
+        bt = tmp;
+      } else {
+        bt = b;
+      }
+      ilu_apply(A, *w.ilu, lu, bt, V, w.st, s);
       hipLaunchKernelGGL((k_pc_norm<T>), dim3(g1), dim3(kBlock), 0, s, n, V, static_cast<const T*>(nullptr),
                          w.partial);
     } else {
-      hipLaunchKernelGGL((k_pc_norm<T>), dim3(g1), dim3(kBlock), 0, s, n, b, inv, w.partial);
+      hipLaunchKernelGGL((k_pc_norm<X>), dim3(g1), dim3(kBlock), 0, s, n, b, invr, w.partial);
     }
     hipLaunchKernelGGL(k_snorm_final, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st);
   } else {
-    fill_zero(x, sizeof(T) * static_cast<size_t>(n), s);
+    fill_zero(x, sizeof(X) * ld, s);
   }
   bool first = true, gz = zero_guess;
-  int its = 0;
+  int its = 0, cycles = 0;
   const int check_every = n >= (1 << 20) ? 4 : 8;
   KResult res;
   for (;;) {
     if (lu) {
-      hipLaunchKernelGGL((k_residual_pc<T>), dim3(g1), dim3(kBlock), 0, s, A, b, x, static_cast<const T*>(nullptr),
-                         tmp, w.partial, gz ? 1 : 0);
+      hipLaunchKernelGGL((k_residual_pc<T, X>), dim3(g1), dim3(kBlock), 0, s, Ares, b, x,
+                         static_cast<const X*>(nullptr), tmp, w.partial, gz ? 1 : 0);
       ilu_apply(A, *w.ilu, lu, tmp, V, w.st, s);
       hipLaunchKernelGGL((k_pc_norm<T>), dim3(g1), dim3(kBlock), 0, s, n, V, static_cast<const T*>(nullptr),
                          w.partial);
     } else {
-      hipLaunchKernelGGL((k_residual_pc<T>), dim3(g1), dim3(kBlock), 0, s, A, b, x, inv, V, w.partial, gz ? 1 : 0);
+      hipLaunchKernelGGL((k_residual_pc<T, X>), dim3(g1), dim3(kBlock), 0, s, Ares, b, x, invr, V, w.partial,
+                         gz ? 1 : 0);
     }
     hipLaunchKernelGGL(k_restart_finalize, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st, first ? 1 : 0,
                        zero_guess ? 1 : 0);
     S355_LAUNCH_CHECK();
+    if (mixed && !first) {
+      // the FP64 residual decides whether the solve has ended
+      read_header(w, s);
+      if (w.host->reason) break;
+      if (its >= o.maxit) {
+        res.reason = kDivergedIts;
+        break;
+      }
+    }
     first = false;
     gz = false;
     int it = 0;
-    bool ended = false;
     while (it < m && its < o.maxit) {
       if (lu) {
         hipLaunchKernelGGL((k_spmv_pc<T>), dim3(spmv_grid(n)), dim3(kBlock), 0, s, A, V + it * ld,
@@ -772,18 +835,22 @@ KResult gmres(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, boo
       ++its;
       if (it % check_every == 0 && it < m) {
         read_header(w, s);
-        if (w.host->reason) {
-          ended = true;
-          break;
-        }
+        if (w.host->reason) break;
       }
     }
-    (void)ended;
     hipLaunchKernelGGL(k_build_y, dim3(1), dim3(kWave), 0, s, w.st, m, nupd);
     Dispatch<T>::update(R, V, ld, n, x, w.st, nupd, s);
     S355_LAUNCH_CHECK();
     read_header(w, s);
     its = w.host->its;
+    ++cycles;
+    if constexpr (mixed) {
+      // a cycle that ended on the estimate, or on the cap, goes back to the
+      // FP64 residual; a breakdown or divergence ends the solve here
+      if (w.host->reason < 0 && w.host->reason != kDivergedBreakdown) break;
+      if (w.host->reason != 0) hipLaunchKernelGGL(k_clear_reason, dim3(1), dim3(1), 0, s, w.st);
+      continue;
+    }
     if (w.host->reason) break;
     if (its >= o.maxit) {
       res.reason = kDivergedIts;
@@ -794,6 +861,7 @@ KResult gmres(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, boo
   if (res.reason == 0) res.reason = w.host->reason;
   res.rnorm = w.host->rnorm;
   res.rnorm0 = w.host->rnorm0;
+  res.refinements = mixed ? cycles : 0;
   return res;
 }
 
@@ -936,7 +1004,7 @@ KResult krylov_solve(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts&
   if (A.n <= 0) return KResult{0, kConvergedAtol, 0.0, 0.0};
   if (o.maxit < 0 || !(o.rtol >= 0.0)) throw std::invalid_argument("krylov: bad tolerances");
   if (o.type == S355_KSP_CG) return cg(A, b, x, o, zero_guess, w, s);
-  return gmres(A, b, x, o, zero_guess, w, s);
+  return gmres<T, T>(A, A, b, x, o, zero_guess, w, nullptr, s);
 }
 
 template <class T>
@@ -961,6 +1029,8 @@ KResult ir_solve(const CsrView<double>& A64, const CsrView<float>& A, const doub
     res.reason = kConvergedAtol;
     return res;
   }
+  if (o.maxit < 0 || !(o.rtol >= 0.0)) throw std::invalid_argument("krylov: bad tolerances");
+  if (o.type != S355_KSP_CG) return gmres<float, double>(A, A64, b, x, o, zero_guess, w32, ir.r64, s);
   const int g = grid_for(n);
   if (static_cast<size_t>(g) > ir.partial_len) throw std::runtime_error("ir_solve: partial buffer too small");
   hipLaunchKernelGGL(k_inv_diag32, dim3(g), dim3(kBlock), 0, s, A, ir.inv32, o.pc == S355_PC_JACOBI ? 1 : 0);
@@ -1033,6 +1103,38 @@ double bench_spmv(const CsrView<T>& A, const T* x, T* y, KWork& w, int repeats, 
   S355_HIP(hipEventRecord(e0, s));
   for (int r = 0; r < repeats; ++r)
     hipLaunchKernelGGL((k_spmv_pc<T>), dim3(g), dim3(kBlock), 0, s, A, x, static_cast<const T*>(nullptr), y, w.st, 0);
+  S355_HIP(hipEventRecord(e1, s));
+  S355_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  S355_HIP(hipEventElapsedTime(&ms, e0, e1));
+  S355_HIP(hipEventDestroy(e0));
+  S355_HIP(hipEventDestroy(e1));
+  return static_cast<double>(ms) / repeats;
+}
+
+namespace {
+// plain 16-byte-per-lane copy: the reference point for bandwidth and for
+// scaling hardware byte counters
+__global__ void __launch_bounds__(kBlock) k_stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                        size_t n) {
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+}  // namespace
+
+double bench_copy(void* dst, const void* src, size_t bytes, int repeats, hipStream_t s) {
+  if (bytes % sizeof(uint4)) throw std::invalid_argument("bench_copy: size must be a multiple of 16 bytes");
+  const size_t n = bytes / sizeof(uint4);
+  const int g = static_cast<int>(std::min<size_t>((n + kBlock - 1) / kBlock, 8192));
+  const uint4* a = static_cast<const uint4*>(src);
+  uint4* b = static_cast<uint4*>(dst);
+  hipLaunchKernelGGL(k_stream_copy, dim3(g), dim3(kBlock), 0, s, a, b, n);
+  S355_LAUNCH_CHECK();
+  hipEvent_t e0, e1;
+  S355_HIP(hipEventCreate(&e0));
+  S355_HIP(hipEventCreate(&e1));
+  S355_HIP(hipEventRecord(e0, s));
+  for (int r = 0; r < repeats; ++r) hipLaunchKernelGGL(k_stream_copy, dim3(g), dim3(kBlock), 0, s, a, b, n);
   S355_HIP(hipEventRecord(e1, s));
   S355_HIP(hipEventSynchronize(e1));
   float ms = 0.f;

@@ -97,13 +97,16 @@ struct IrWork {
   double* host = nullptr; // pinned, 2 doubles
 };
 
-// FP64 iterative refinement around FP32 inner Krylov solves: the residual
-// r = b - A64 x with the FP64 matrix and the convergence test (PETSc's, on
-// ||M^-1 r||) are in FP64, x is FP64, each correction comes from an FP32
-// GMRES/CG solve of A32 e = r (A32 = A64 rounded) to relative tolerance
-// inner_rtol.  Taking the outer residual with A64 keeps the exact null space
-// of the pressure-correction matrix (A 1 = 0), which rounding breaks.  The
-// iteration cap counts inner iterations.
+// Mixed-precision solve of A64 x = b with FP64 x, b and convergence test
+// (PETSc's, on ||M^-1 r||), and FP32 Krylov work on A32 (A64 rounded).
+// GMRES(m): every restart cycle is one refinement step — the restart residual
+// r = b - A64 x is FP64, the Arnoldi cycle and its basis are FP32, the
+// Hessenberg least-squares problem is FP64 and the correction is accumulated
+// into the FP64 x; the FP64 residual alone decides convergence.  CG: FP64
+// refinement around FP32 CG solves of A32 e = r to relative tolerance
+// inner_rtol, at most ir_max of them.  Taking the residual with A64 keeps the
+// exact null space of the pressure-correction matrix (A 1 = 0), which
+// rounding breaks.  The iteration cap counts Krylov iterations.
 KResult ir_solve(const CsrView<double>& A64, const CsrView<float>& A32, const double* b, double* x,
                  const s355_ksp_opts& o, bool zero_guess, int ir_max, double inner_rtol, KWork& w32, IrWork& ir,
                  hipStream_t s);
@@ -122,6 +125,9 @@ void residual64(const CsrView<T>& A, const double* b, const double* x, double* r
 // `repeats` back-to-back launches, measured with HIP events on stream s.
 template <class T>
 double bench_spmv(const CsrView<T>& A, const T* x, T* y, KWork& w, int repeats, hipStream_t s);
+
+// Average device time (ms) of a plain copy of `bytes` (a multiple of 16).
+double bench_copy(void* dst, const void* src, size_t bytes, int repeats, hipStream_t s);
 
 void convert_to_float(const double* src, float* dst, size_t n, hipStream_t s);
 
