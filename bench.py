@@ -18,7 +18,8 @@ Other workloads (--workload): `cavity` (32^3 lid-driven cavity, Re 100,
 GMRES(30)+Jacobi, 200 iterations, FP64), `step` (backward-facing step,
 1.12 M jittered tetrahedra, k-epsilon, Re 5000, FP64 with the reference's
 default solvers), `large` (400x200x250 jittered hex channel, 20 M cells,
-k-epsilon, mixed GMRES(50)+ILU(0)).
+k-epsilon, mixed GMRES(50)+ILU(0)), `sweep` (64 lid-driven cavities of
+edge 16..64 and Re 100..1000 run as one batch, mixed GMRES(30)+Jacobi).
 
 Prints one JSON line: SIMPLE iterations/s, the roofline of the SpMV kernel
 (algorithmic bytes per launch / HIP-event-timed launch duration vs 8 TB/s),
@@ -189,7 +190,55 @@ class Large(Channel):
         return self.case(dims), "turbulent channel %dx%dx%d, same jitter/Re/solver settings" % dims
 
 
-WORKLOADS = {w.name: w for w in (Channel, Cavity, Step, Large)}
+class Sweep(Channel):
+    """64 independent lid-driven cavities: edge n ~ U{16..64}, Re ~
+    logU[100, 1000] (seeded), run as one batch."""
+    name = "sweep"
+    sizes = {"full": 64, "medium": 16, "small": 4}
+    precisions = ["mixed"]
+    steps, warmup = 5, 1
+    batch = True
+
+    def __init__(self, size):
+        import numpy as np
+
+        self.dims = self.sizes[size]
+        rng = np.random.default_rng(2024)
+        self.edges = [int(e) for e in rng.integers(16, 65, size=self.dims)]
+        self.res = [float(r) for r in np.exp(rng.uniform(np.log(100.0), np.log(1000.0), size=self.dims))]
+
+    def case(self, dims=None):
+        import sysu_cfd_parallel_amd as s
+
+        idx = range(self.dims) if dims is None else dims
+        return [s.cavity(self.edges[k], re=self.res[k]) for k in idx]
+
+    def params(self, prec):
+        import sysu_cfd_parallel_amd as s
+
+        p = _gmres_all(s.default_params(), 30, s.PC_JACOBI, maxit=5000)
+        p.residual_steady = 0.0
+        p.precision = prec
+        return p
+
+    def describe(self, prec_name):
+        cells = sum(e ** 3 for e in self.edges)
+        return (f"{self.dims} lid-driven cavities as one batch, edges U{{16..64}} (sum {cells} cells), "
+                f"Re logU[100,1000], seed 2024, GMRES(30)+Jacobi, "
+                + ("FP32 inner solves in FP64 refinement" if prec_name == "mixed" else "FP64"))
+
+    def cpu_sample(self):
+        # the cases of the batch up to about 300K cells, in batch order
+        idx, cells = [], 0
+        for k, e in enumerate(self.edges):
+            if cells + e ** 3 > 300_000 and idx:
+                break
+            idx.append(k)
+            cells += e ** 3
+        return self.case(idx), f"the first {len(idx)} cavities of the batch, run one after the other"
+
+
+WORKLOADS = {w.name: w for w in (Channel, Cavity, Step, Large, Sweep)}
 
 
 # --------------------------------------------------------------------------- measurement
@@ -208,17 +257,20 @@ def cpu_baseline(wl, threads: int, budget_s: float = 20.0, max_steps: int = 3) -
     from oracle.pyoracle import Oracle
 
     case, what = wl.cpu_sample()
-    o = Oracle(case, wl.params(0), threads=threads)
-    n = o.sizes.n_cell
+    cases = case if isinstance(case, list) else [case]
+    oracles = [Oracle(c, wl.params(0), threads=threads) for c in cases]
+    n = sum(o.sizes.n_cell for o in oracles)
     steps = 0
     t0 = time.perf_counter()
     while steps < max_steps:
-        o.iterate(1)
+        for o in oracles:
+            o.iterate(1)
         steps += 1
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
-    o.close()
+    for o in oracles:
+        o.close()
     return {"iters_per_s": steps / dt, "cells": n, "steps": steps, "seconds": dt, "what": what}
 
 
@@ -229,7 +281,7 @@ def run_path(wl, case, prec: int, steps: int, warmup: int, label: str) -> dict:
     import sysu_cfd_parallel_amd as s
 
     t = time.perf_counter()
-    dev = s.Solver(case, wl.params(prec))
+    dev = s.BatchSolver(case, wl.params(prec)) if isinstance(case, list) else s.Solver(case, wl.params(prec))
     z = dev.sizes
     log(f"[{label}] device context: {z.n_cell} cells, {z.n_face} faces, nnz {z.nnz} "
         f"({time.perf_counter() - t:.1f}s)")
@@ -315,7 +367,8 @@ def main() -> int:
 
     t = time.perf_counter()
     case = wl.case()
-    log(f"{wl.name} case built in {time.perf_counter() - t:.1f}s ({case.n_cell_elems} volume elements)")
+    n_elems = sum(c.n_cell_elems for c in case) if isinstance(case, list) else case.n_cell_elems
+    log(f"{wl.name} case built in {time.perf_counter() - t:.1f}s ({n_elems} volume elements)")
 
     runs = {}
     for name in order:

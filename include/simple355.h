@@ -250,6 +250,32 @@ int s355_calculate_velocity(s355_ctx* ctx, s355_solve_info info[3]);
 int s355_calculate_pressure(s355_ctx* ctx, s355_solve_info* info);
 int s355_update_tur_kepsilon(s355_ctx* ctx, s355_solve_info info[2]);
 
+/* ---- Batches of independent cases (parameter sweeps) ----------------------
+ * No reference counterpart: the reference runs one case per MPI job.  The
+ * cases are concatenated into one device layout (cells, rows and vertices
+ * case after case; the boundary faces of all cases first, then their interior
+ * faces, each case keeping its own order; region ids of case k shifted by
+ * k * (largest region id + 1)), and every phase entry above runs all of them
+ * with one launch per kernel.  Each case keeps its own Krylov solve, its
+ * own outlet mass-flow correction, pressure reference cell (the same index
+ * within every case) and residuals; s355_residuals returns the element-wise
+ * maximum over the cases.  Every system is solved with GMRES. */
+enum s355_batch_array {
+  S355_BATCH_CELL = 0,  /* n_case + 1 offsets of the cases' cells (and CSR rows) */
+  S355_BATCH_BND = 1,   /* boundary records / boundary faces */
+  S355_BATCH_IFACE = 2, /* interior faces, counted from n_bnd */
+  S355_BATCH_NNZ = 3,   /* CSR entries */
+  S355_BATCH_VERT = 4,
+  S355_BATCH_ELEM = 5
+};
+int s355_batch_create(int n_case, const s355_mesh_desc* meshes, const s355_params* params, s355_ctx** out);
+int s355_batch_size(const s355_ctx* ctx, int* n_case);
+int s355_batch_layout(const s355_ctx* ctx, int which, int* out, size_t n);
+/* res: 10 per case, in case order */
+int s355_batch_residuals(s355_ctx* ctx, double* res, int reset);
+/* per-case records of the last solve of a kind: 0..2 U, V, W, 3 p', 4 k, 5 eps */
+int s355_batch_solve_info(s355_ctx* ctx, int kind, s355_solve_info* info, int n_case);
+
 /* Solve records of the last SIMPLE phases run through the phase-level
  * entries: info[0..5] = U, V, W, pressure correction, k, epsilon. */
 int s355_last_solve_info(s355_ctx* ctx, s355_solve_info info[6]);
@@ -274,6 +300,8 @@ int s355_copy_time(s355_ctx* ctx, size_t bytes, int repeats, double* ms_per_call
  * face builder, for inspection and for tests on machines without a GPU. */
 typedef struct s355_mesh s355_mesh;
 int s355_mesh_build(const s355_mesh_desc* mesh, s355_mesh** out);
+/* the batch layout of s355_batch_create, host side */
+int s355_mesh_build_batch(int n_case, const s355_mesh_desc* meshes, s355_mesh** out);
 int s355_mesh_sizes(const s355_mesh* m, s355_sizes* out);
 int s355_mesh_get_int(const s355_mesh* m, int which, int* out, size_t n);
 int s355_mesh_get_dbl(const s355_mesh* m, int which, double* out, size_t n);

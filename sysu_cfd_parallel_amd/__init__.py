@@ -23,7 +23,7 @@ LIB_PATH = Path(__file__).resolve().parent / "_build" / "libsimple355.so"
 
 __all__ = [
     "LIB_PATH", "library", "CaseData", "cavity", "channel", "step", "default_params", "set_all_ksp", "HostMesh",
-    "Solver", "S355Error", "NotConverged", "FP64", "MIXED", "KSP_GMRES", "KSP_CG", "PC_NONE", "PC_JACOBI",
+    "Solver", "BatchSolver", "S355Error", "NotConverged", "FP64", "MIXED", "KSP_GMRES", "KSP_CG", "PC_NONE", "PC_JACOBI",
     "PC_ILU0", "KSP_VELOCITY", "KSP_PRESSURE", "KSP_SCALAR",
 ]
 
@@ -60,12 +60,18 @@ def library(path: Optional[Path] = None) -> C.CDLL:
         "case_copy": (I, [V, pD, pI, pI, pI, pI, C.POINTER(Region)]),
         "case_free": (None, [V]),
         "mesh_build": (I, [C.POINTER(MeshDesc), C.POINTER(V)]),
+        "mesh_build_batch": (I, [I, C.POINTER(MeshDesc), C.POINTER(V)]),
         "mesh_sizes": (I, [V, C.POINTER(Sizes)]),
         "mesh_get_int": (I, [V, I, pI, C.c_size_t]),
         "mesh_get_dbl": (I, [V, I, pD, C.c_size_t]),
         "mesh_free": (None, [V]),
         "ctx_create": (I, [C.POINTER(MeshDesc), C.POINTER(Params), C.POINTER(V)]),
         "ctx_destroy": (I, [V]),
+        "batch_create": (I, [I, C.POINTER(MeshDesc), C.POINTER(Params), C.POINTER(V)]),
+        "batch_size": (I, [V, pI]),
+        "batch_layout": (I, [V, I, pI, C.c_size_t]),
+        "batch_residuals": (I, [V, pD, I]),
+        "batch_solve_info": (I, [V, I, C.POINTER(SolveInfo), I]),
         "ctx_sizes": (I, [V, C.POINTER(Sizes)]),
         "ctx_set_params": (I, [V, C.POINTER(Params)]),
         "synchronize": (I, [V]),
@@ -211,14 +217,24 @@ def set_all_ksp(p: Params, ksp_type: int = KSP_GMRES, restart: int = 30, pc: int
     return p
 
 
-class HostMesh:
-    """Host-only mesh set-up (no GPU needed)."""
+def _descs(cases: list):
+    arr = (MeshDesc * len(cases))(*[c.desc() for c in cases])
+    return arr
 
-    def __init__(self, case: CaseData):
+
+class HostMesh:
+    """Host-only mesh set-up (no GPU needed); a list of cases gives the
+    batch layout of BatchSolver."""
+
+    def __init__(self, case):
         self.lib = library()
         self.h = C.c_void_p()
-        d = case.desc()
-        _check(self.lib, self.lib.s355_mesh_build(C.byref(d), C.byref(self.h)), "mesh_build")
+        if isinstance(case, (list, tuple)):
+            self._d = _descs(list(case))
+            _check(self.lib, self.lib.s355_mesh_build_batch(len(case), self._d, C.byref(self.h)), "mesh_build_batch")
+        else:
+            d = case.desc()
+            _check(self.lib, self.lib.s355_mesh_build(C.byref(d), C.byref(self.h)), "mesh_build")
         s = Sizes()
         _check(self.lib, self.lib.s355_mesh_sizes(self.h, C.byref(s)), "mesh_sizes")
         self.sizes = s
@@ -253,8 +269,14 @@ class Solver:
         self.case = case
         self.params = params
         self.ctx = C.c_void_p()
+        self._create(case, params)
+        self._finish(init)
+
+    def _create(self, case, params) -> None:
         d = case.desc()
         _check(self.lib, self.lib.s355_ctx_create(C.byref(d), C.byref(params), C.byref(self.ctx)), "ctx_create")
+
+    def _finish(self, init: bool) -> None:
         s = Sizes()
         _check(self.lib, self.lib.s355_ctx_sizes(self.ctx, C.byref(s)), "ctx_sizes")
         self.sizes = s
@@ -362,3 +384,59 @@ class Solver:
             self.close()
         except Exception:
             pass
+
+
+BATCH_ARRAYS = {"cell": 0, "bnd": 1, "iface": 2, "nnz": 3, "vert": 4, "elem": 5}
+SOLVE_KINDS = ("U", "V", "W", "P", "TE", "ED")
+
+
+class BatchSolver(Solver):
+    """Independent cases resident on the GPU as one batch: every SIMPLE phase
+    runs all of them with one launch per kernel, each case keeping its own
+    Krylov solves, residuals and outlet correction.  get/set act on the
+    concatenated arrays; case(k, name) slices one case's part."""
+
+    def __init__(self, cases: list, params: Params, init: bool = True):
+        self.lib = library()
+        self.cases = list(cases)
+        self.case = None
+        self.params = params
+        self.ctx = C.c_void_p()
+        self._d = _descs(self.cases)
+        _check(self.lib, self.lib.s355_batch_create(len(self.cases), self._d, C.byref(params), C.byref(self.ctx)),
+               "batch_create")
+        self._finish(init)
+        self.n_case = len(self.cases)
+        self.layout = {}
+        for name, w in BATCH_ARRAYS.items():
+            a = np.empty(self.n_case + 1, dtype=np.int32)
+            _check(self.lib, self.lib.s355_batch_layout(self.ctx, w, _ip(a), a.size), "batch_layout")
+            self.layout[name] = a
+
+    def case_slice(self, k: int, kind: str) -> slice:
+        if kind in ("c", "c3"):
+            a, b = self.layout["cell"][k], self.layout["cell"][k + 1]
+        elif kind in ("b", "b3"):
+            a, b = self.layout["bnd"][k], self.layout["bnd"][k + 1]
+        elif kind == "z":
+            a, b = self.layout["nnz"][k], self.layout["nnz"][k + 1]
+        else:
+            raise ValueError(f"no per-case slice for field kind {kind}")
+        w = 3 if kind in ("c3", "b3") else 1
+        return slice(w * int(a), w * int(b))
+
+    def case_field(self, k: int, name: str) -> np.ndarray:
+        return self.get(name)[self.case_slice(k, FIELD_KIND[name])]
+
+    def case_residuals(self, reset: bool = False) -> np.ndarray:
+        out = np.zeros(10 * self.n_case)
+        _check(self.lib, self.lib.s355_batch_residuals(self.ctx, _dp(out), int(reset)), "batch_residuals")
+        return out.reshape(self.n_case, 10)
+
+    def case_solve_info(self) -> dict:
+        res = {}
+        for kind, name in enumerate(SOLVE_KINDS):
+            info = (SolveInfo * self.n_case)()
+            _check(self.lib, self.lib.s355_batch_solve_info(self.ctx, kind, info, self.n_case), "batch_solve_info")
+            res[name] = [i.as_dict() for i in info]
+        return res

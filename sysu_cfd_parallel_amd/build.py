@@ -19,6 +19,7 @@ SOURCES = [
     "mesh/case_gen.cpp",
     "mesh/cases.cpp",
     "mesh/ingest.cpp",
+    "mesh/batch.cpp",
     "device/fv_kernels.hip",
     "device/keps_kernels.hip",
     "device/krylov.hip",

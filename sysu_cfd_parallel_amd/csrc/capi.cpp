@@ -266,6 +266,19 @@ int s355_mesh_build(const s355_mesh_desc* desc, s355_mesh** out) {
     return S355_OK;
   });
 }
+int s355_mesh_build_batch(int n_case, const s355_mesh_desc* descs, s355_mesh** out) {
+  return guarded([&] {
+    if (!descs || !out || n_case < 1) throw std::invalid_argument("null argument or no cases");
+    std::vector<s355::Mesh> parts;
+    parts.reserve(n_case);
+    for (int k = 0; k < n_case; ++k) parts.push_back(s355::ingest_mesh(descs[k]));
+    auto m = std::make_unique<s355_mesh>();
+    s355::BatchLayout lay;
+    m->m = s355::concat_meshes(parts, lay);
+    *out = m.release();
+    return S355_OK;
+  });
+}
 int s355_mesh_sizes(const s355_mesh* m, s355_sizes* out) {
   return guarded([&] {
     if (!m || !out) throw std::invalid_argument("null argument");
@@ -293,9 +306,82 @@ int s355_ctx_create(const s355_mesh_desc* mesh, const s355_params* params, s355_
     *out = nullptr;
     check_params(params);
     auto c = std::make_unique<s355_ctx>();
-    c->dev = std::make_unique<s355::Context>(*mesh, *params);
+    c->dev = std::make_unique<s355::Context>(mesh, 1, *params);
     c->ns = std::make_unique<s355host::NavierStokesSolver>(c.get(), *params);
     *out = c.release();
+    return S355_OK;
+  });
+}
+
+int s355_batch_create(int n_case, const s355_mesh_desc* meshes, const s355_params* params, s355_ctx** out) {
+  return guarded([&] {
+    if (!meshes || !out || n_case < 1) throw std::invalid_argument("null argument or no cases");
+    *out = nullptr;
+    check_params(params);
+    auto c = std::make_unique<s355_ctx>();
+    c->dev = std::make_unique<s355::Context>(meshes, n_case, *params);
+    c->ns = std::make_unique<s355host::NavierStokesSolver>(c.get(), *params);
+    *out = c.release();
+    return S355_OK;
+  });
+}
+
+int s355_batch_size(const s355_ctx* ctx, int* n_case) {
+  return guarded([&] {
+    if (!ctx || !n_case) throw std::invalid_argument("null argument");
+    *n_case = ctx->dev->ncase;
+    return S355_OK;
+  });
+}
+
+int s355_batch_layout(const s355_ctx* ctx, int which, int* out, size_t n) {
+  return guarded([&] {
+    if (!ctx || !out) throw std::invalid_argument("null argument");
+    const s355::BatchLayout& l = ctx->dev->lay;
+    const std::vector<int>* v = nullptr;
+    switch (which) {
+      case S355_BATCH_CELL: v = &l.cell; break;
+      case S355_BATCH_BND: v = &l.bnd; break;
+      case S355_BATCH_IFACE: v = &l.iface; break;
+      case S355_BATCH_NNZ: v = &l.nnz; break;
+      case S355_BATCH_VERT: v = &l.vert; break;
+      case S355_BATCH_ELEM: v = &l.elem; break;
+      default: throw std::invalid_argument("unknown batch layout array");
+    }
+    if (n != v->size()) throw std::invalid_argument("batch layout: length must be n_case + 1");
+    std::copy(v->begin(), v->end(), out);
+    return S355_OK;
+  });
+}
+
+int s355_batch_residuals(s355_ctx* ctx, double* res, int reset) {
+  return guarded([&] {
+    if (!res) throw std::invalid_argument("null res");
+    s355::Context& d = dev(ctx);
+    d.read_case_residuals(res);
+    d.check_flags();
+    if (reset) d.zero_residuals();
+    return S355_OK;
+  });
+}
+
+int s355_batch_solve_info(s355_ctx* ctx, int kind, s355_solve_info* info, int n_case) {
+  return guarded([&] {
+    s355::Context& d = dev(ctx);
+    if (!info || n_case != d.ncase) throw std::invalid_argument("batch solve info: n_case entries");
+    if (kind < 0 || kind > 5) throw std::invalid_argument("batch solve info: kind 0..5 (U, V, W, p', k, eps)");
+    const auto& v = d.case_info[kind];
+    for (int k = 0; k < n_case; ++k) {
+      s355_solve_info x{};
+      if (k < static_cast<int>(v.size())) {
+        x.iters = v[k].iters;
+        x.reason = v[k].reason;
+        x.rnorm = v[k].rnorm;
+        x.rnorm0 = v[k].rnorm0;
+        x.refinements = v[k].refinements;
+      }
+      info[k] = x;
+    }
     return S355_OK;
   });
 }

@@ -43,6 +43,53 @@ __device__ __forceinline__ int xcd_block(int b, int G) {
   return (b % kXcd) * q + b / kXcd;
 }
 
+// Rows [0, n) of one or several independent systems ("cases", stored one
+// after the other) split into tiles of at most `tile` rows that never
+// straddle two cases.  Block b of a tiled kernel works on tile b, so
+// per-tile partial sums reduce per case in a fixed order.  With one case the
+// tiles are the consecutive chunks of `tile` rows and no table is read.
+struct Tiles {
+  int n = 0, tile = 0, ntile = 0, ncase = 1;
+  const int* lo = nullptr;         // ntile + 1 tile starts (device), null for one case
+  const int* tcase = nullptr;      // ntile: case of each tile
+  const int* case_tile = nullptr;  // ncase + 1: first tile of each case
+  const int* case_row = nullptr;   // ncase + 1: first row of each case
+
+  __device__ __forceinline__ void range(int b, int& a, int& e, int& c) const {
+    if (!lo) {
+      a = b * tile;
+      e = min(a + tile, n);
+      c = 0;
+    } else {
+      a = lo[b];
+      e = lo[b + 1];
+      c = tcase[b];
+    }
+  }
+  __device__ __forceinline__ void tiles_of(int c, int& t0, int& t1) const {
+    if (!lo) {
+      t0 = 0;
+      t1 = ntile;
+    } else {
+      t0 = case_tile[c];
+      t1 = case_tile[c + 1];
+    }
+  }
+  // case of a row: binary search over the case starts
+  __device__ __forceinline__ int case_of(int row) const {
+    if (ncase == 1) return 0;
+    int l = 0, h = ncase;  // case_row[l] <= row < case_row[h]
+    while (h - l > 1) {
+      const int m = (l + h) >> 1;
+      if (case_row[m] <= row)
+        l = m;
+      else
+        h = m;
+    }
+    return l;
+  }
+};
+
 // Butterfly sum over the 64 lanes of a wavefront; every lane ends with the
 // same value, produced by the same fixed pairing order.
 template <class T>

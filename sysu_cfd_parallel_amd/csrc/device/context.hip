@@ -37,14 +37,36 @@ auto upload(std::vector<void*>& allocs, const Vec& v, hipStream_t s) {
 }
 }  // namespace
 
-Context::Context(const s355_mesh_desc& desc, const s355_params& params) : par(params) {
+Context::Context(const s355_mesh_desc* descs, int n_case, const s355_params& params) : par(params), ncase(n_case) {
   if (par.density_model != 0) throw std::invalid_argument("only DensityModel 0 (constant density) is supported");
   if (!par.steady) throw std::invalid_argument("only steady runs are supported");
   if (par.precision != S355_FP64 && par.precision != S355_MIXED) throw std::invalid_argument("bad precision mode");
+  if (!descs || ncase < 1) throw std::invalid_argument("no mesh");
+  if (ncase > 1)
+    for (int k = 0; k < 3; ++k)
+      if (par.ksp[k].type == S355_KSP_CG)
+        throw std::invalid_argument("a batch of cases solves every system with GMRES (KSP type CG requested)");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw HipError("no HIP device available");
-  mesh = ingest_mesh(desc);
-  if (par.cell_pressure_ref < 0 || par.cell_pressure_ref >= mesh.n_cell)
+  if (ncase == 1) {
+    mesh = ingest_mesh(descs[0]);
+    lay.cell = {0, mesh.n_cell};
+    lay.bnd = {0, mesh.n_bnd};
+    lay.iface = {0, mesh.n_face - mesh.n_bnd};
+    lay.nnz = {0, mesh.nnz()};
+    lay.vert = {0, mesh.n_vert};
+    lay.elem = {0, mesh.n_elem};
+    lay.cf = {0, mesh.cf_ptr[mesh.n_cell]};
+    lay.rid_stride = 0;
+  } else {
+    std::vector<Mesh> parts;
+    parts.reserve(ncase);
+    for (int k = 0; k < ncase; ++k) parts.push_back(ingest_mesh(descs[k]));
+    mesh = concat_meshes(parts, lay);
+  }
+  int min_cells = mesh.n_cell;
+  for (int k = 0; k < ncase; ++k) min_cells = std::min(min_cells, lay.cell[k + 1] - lay.cell[k]);
+  if (par.cell_pressure_ref < 0 || par.cell_pressure_ref >= min_cells)
     throw std::invalid_argument("cell_pressure_ref out of range");
   try {
     S355_HIP(hipStreamCreate(&stream));
@@ -59,6 +81,35 @@ Context::Context(const s355_mesh_desc& desc, const s355_params& params) : par(pa
 
 Context::~Context() { free_all(); }
 
+Tiles Context::make_tiles(const std::vector<int>& case_row, int tile) {
+  Tiles t;
+  t.n = case_row.back();
+  t.tile = tile;
+  t.ncase = ncase;
+  if (ncase == 1) {
+    t.ntile = std::max(1, (t.n + tile - 1) / tile);
+    return t;
+  }
+  std::vector<int> lo, tc, ct(ncase + 1);
+  for (int c = 0; c < ncase; ++c) {
+    ct[c] = static_cast<int>(lo.size());
+    for (int r = case_row[c]; r < case_row[c + 1]; r += tile) {
+      lo.push_back(r);
+      tc.push_back(c);
+    }
+  }
+  ct[ncase] = static_cast<int>(lo.size());
+  t.ntile = static_cast<int>(lo.size());
+  lo.push_back(t.n);
+  t.lo = upload(allocs_, lo, stream);
+  t.tcase = upload(allocs_, tc, stream);
+  t.case_tile = upload(allocs_, ct, stream);
+  t.case_row = upload(allocs_, case_row, stream);
+  S355_HIP(hipStreamSynchronize(stream));  // host staging vectors go out of scope
+  if (t.ntile < 1) throw std::invalid_argument("empty batch");
+  return t;
+}
+
 void Context::free_all() {
   if (stream) hipStreamSynchronize(stream);
   for (hipEvent_t e : all_events_) hipEventDestroy(e);
@@ -70,6 +121,8 @@ void Context::free_all() {
   allocs_.clear();
   if (host_pinned_) hipHostFree(host_pinned_);
   host_pinned_ = nullptr;
+  if (hdr_) hipHostFree(hdr_);
+  hdr_ = nullptr;
   if (stream) hipStreamDestroy(stream);
   stream = nullptr;
 }
@@ -118,6 +171,8 @@ void Context::upload_mesh() {
       wptr.push_back(static_cast<int>(wface.size()));
     }
   }
+  dm.ctile = make_tiles(lay.cell, kBlock);
+  dm.btile = make_tiles(lay.bnd, kBlock);
   dm.n_wcell = static_cast<int>(wcell.size());
   dm.wc_cell = upload(allocs_, wcell, stream);
   dm.wc_ptr = upload(allocs_, wptr, stream);
@@ -159,27 +214,36 @@ void Context::allocate_fields() {
   S355_HIP(hipMemsetAsync(df.shear, 0, std::max<size_t>(3 * Nb, 1) * sizeof(double), stream));
   S355_HIP(hipMemsetAsync(df.RUFace, 0, Nf * sizeof(double), stream));
   for (double* p : {df.Au, df.Ap, df.As}) S355_HIP(hipMemsetAsync(p, 0, nnz * sizeof(double), stream));
-  // scratch for reductions
+  // scratch for reductions: per-tile partials of the Krylov kernels (up to
+  // m + 1 columns) and of the cell / boundary reductions (up to 3 columns)
   max_restart_ = 4;
   for (int k = 0; k < 3; ++k) max_restart_ = std::max(max_restart_, std::min(par.ksp[k].restart, kMaxRestart));
-  const size_t g = static_cast<size_t>(grid_for(static_cast<long>(std::max<size_t>(Nc, Nf))));
-  partial_len_ = std::max<size_t>(g * (max_restart_ + 1), 3 * static_cast<size_t>(grid_for(std::max<size_t>(Nb, 1))));
+  for (int r = 0; r < 5; ++r) kwork_.tiles[r] = make_tiles(lay.cell, kBlock << r);
+  const size_t tiles1 = static_cast<size_t>(kwork_.tiles[0].ntile);
+  partial_len_ = std::max<size_t>({tiles1 * (max_restart_ + 1), 3 * static_cast<size_t>(dm.btile.ntile),
+                                   3 * static_cast<size_t>(dm.ctile.ntile)});
   partial_ = alloc<double>(partial_len_);
   scalars_ = alloc<double>(32);
+  res_ = alloc<double>(10 * static_cast<size_t>(ncase));
+  rate_ = alloc<double>(5 * static_cast<size_t>(ncase));
   flags_ = alloc<int>(8);
   S355_HIP(hipMemsetAsync(scalars_, 0, 32 * sizeof(double), stream));
+  S355_HIP(hipMemsetAsync(res_, 0, 10 * sizeof(double) * ncase, stream));
+  S355_HIP(hipMemsetAsync(rate_, 0, 5 * sizeof(double) * ncase, stream));
   S355_HIP(hipMemsetAsync(flags_, 0, 8 * sizeof(int), stream));
   // Krylov workspace
   kwork_.n = static_cast<int>(Nc);
+  kwork_.ncase = ncase;
   kwork_.mmax = max_restart_;
   kwork_.V = alloc<double>((static_cast<size_t>(max_restart_) + 1) * Nc);
   kwork_.r = alloc<double>(Nc);
   kwork_.pcinv = alloc<double>(Nc);
   kwork_.partial = partial_;
   kwork_.partial_len = partial_len_;
-  kwork_.st = alloc<KState>(1);
-  S355_HIP(hipHostMalloc(&host_pinned_, 4096, hipHostMallocDefault));
-  kwork_.host = static_cast<KHeader*>(host_pinned_);
+  kwork_.st = alloc<KState>(ncase);
+  S355_HIP(hipHostMalloc(&host_pinned_, 4096 + 10 * sizeof(double) * ncase, hipHostMallocDefault));
+  S355_HIP(hipHostMalloc(reinterpret_cast<void**>(&hdr_), sizeof(KHeader) * ncase, hipHostMallocDefault));
+  kwork_.host = hdr_;
 }
 
 void Context::set_params(const s355_params& p) {
@@ -187,7 +251,12 @@ void Context::set_params(const s355_params& p) {
   for (int k = 0; k < 3; ++k)
     if (std::min(p.ksp[k].restart, kMaxRestart) > max_restart_)
       throw std::invalid_argument("restart larger than the workspace built at context creation");
-  if (p.cell_pressure_ref < 0 || p.cell_pressure_ref >= dm.Nc) throw std::invalid_argument("cell_pressure_ref");
+  int min_cells = dm.Nc;
+  for (int k = 0; k < ncase; ++k) min_cells = std::min(min_cells, lay.cell[k + 1] - lay.cell[k]);
+  if (p.cell_pressure_ref < 0 || p.cell_pressure_ref >= min_cells) throw std::invalid_argument("cell_pressure_ref");
+  if (ncase > 1)
+    for (int k = 0; k < 3; ++k)
+      if (p.ksp[k].type == S355_KSP_CG) throw std::invalid_argument("a batch of cases solves every system with GMRES");
   par = p;
 }
 
@@ -258,8 +327,11 @@ void Context::init_flow_field() {
       mut[i] = R[i] * kCmuInit * k[i] * k[i] / (e[i] + kSmallInit);
     }
   };
-  if (const s355_region* r0 = find(0))
-    for (int i = 0; i < Nc; ++i) apply(i, *r0);
+  // region 0 (the default initial state of a case) first, then each cell's
+  // own region; in a batch the region ids of case k start at k * rid_stride
+  for (int k = 0; k < ncase; ++k)
+    if (const s355_region* r0 = find(k * lay.rid_stride))
+      for (int i = lay.cell[k]; i < lay.cell[k + 1]; ++i) apply(i, *r0);
   for (int i = 0; i < Nc; ++i) {
     const s355_region* r = find(mesh.c_rid[i]);
     if (!r) throw std::invalid_argument("cell region missing");
@@ -301,7 +373,7 @@ void Context::init_flow_field() {
   sync();  // host staging vectors go out of scope
 }
 
-void Context::bc_velocity() { launch_bc_velocity(dm, df, has_outlet_, partial_, scalars_ + 10, stream); }
+void Context::bc_velocity() { launch_bc_velocity(dm, df, has_outlet_, partial_, rate_, stream); }
 void Context::bc_pressure() { launch_bc_copy(dm, df.P, df.BPre, stream); }
 void Context::bc_deltap() { launch_bc_copy(dm, df.dp, df.BPre, stream); }
 
@@ -363,6 +435,8 @@ void Context::ensure_mixed() {
   kwork32_.partial_len = partial_len_;
   kwork32_.st = kwork_.st;
   kwork32_.host = kwork_.host;
+  kwork32_.ncase = kwork_.ncase;
+  for (int r = 0; r < 5; ++r) kwork32_.tiles[r] = kwork_.tiles[r];
   ir_.r64 = alloc<double>(Nc);
   ir_.r32 = alloc<float>(Nc);
   ir_.e32 = alloc<float>(Nc);
@@ -374,9 +448,12 @@ void Context::ensure_mixed() {
 }
 
 int Context::run_solve(int which, const double* b, double* x, const s355_ksp_opts& o, bool zero_guess,
-                       s355_solve_info* info) {
+                       s355_solve_info* info, int slot) {
   const double* val = which == 0 ? df.Au : (which == 1 ? df.Ap : df.As);
   KResult r;
+  kwork_.per_case = &case_info[slot];
+  kwork32_.per_case = &case_info[slot];
+  case_info[slot].assign(ncase, KResult{});
   if (o.pc == S355_PC_ILU0) ensure_ilu();
   if (par.precision == S355_MIXED) {
     // FP32 inner solves on the rounded matrix inside FP64 refinement
@@ -390,6 +467,7 @@ int Context::run_solve(int which, const double* b, double* x, const s355_ksp_opt
     CsrView<double> A{dm.Nc, dm.rowptr, dm.col, dm.diag, val, dm.nnz};
     r = krylov_solve<double>(A, b, x, o, zero_guess, kwork_, stream);
   }
+  if (ncase == 1) case_info[slot].assign(1, r);
   fill_info(r, info);
   return r.reason < 0 ? S355_NOT_CONVERGED : S355_OK;
 }
@@ -404,7 +482,7 @@ int Context::solve_velocity(double rtol, int maxit, s355_solve_info info[3]) {
   const double* b[3] = {df.bu, df.bv, df.bw};
   double* x[3] = {df.U, df.V, df.W};
   for (int k = 0; k < 3; ++k) {
-    rc = run_solve(0, b[k], x[k], o, false, &info[k]);
+    rc = run_solve(0, b[k], x[k], o, false, &info[k], k);
     if (rc != S355_OK) {
       for (int j = k + 1; j < 3; ++j) info[j] = s355_solve_info{0, 0, 0.0, 0.0, 0, 0};
       return rc;
@@ -420,7 +498,7 @@ int Context::solve_pressure(double rtol, int maxit, s355_solve_info* info) {
   s355_ksp_opts o = par.ksp[S355_KSP_PRESSURE];
   o.rtol = rtol;
   o.maxit = maxit;
-  return run_solve(1, df.bp, df.dp, o, true, info);
+  return run_solve(1, df.bp, df.dp, o, true, info, 3);
 }
 
 void Context::correct_fields() {
@@ -430,7 +508,7 @@ void Context::correct_fields() {
   double* out[1] = {df.dPdX};
   launch_gradient(dm, 1, phi, bphi, out, stream);
   launch_correct_fields(dm, df, par.cell_pressure_ref, par.urf[3], partial_, stream);
-  launch_reduce_partials(partial_, grid_for(dm.Nc), 1, scalars_ + 3, true, stream);
+  launch_reduce_tiles(partial_, dm.ctile, 1, res_ + 3, 10, true, stream);
 }
 
 void Context::correct_ru_face() { launch_correct_ruface(dm, df, par.urf[3], stream); }
@@ -444,7 +522,7 @@ void Context::save_velocity() {
 
 void Context::velocity_residual() {
   launch_residual3(dm, u0_, v0_, w0_, df.U, df.V, df.W, partial_, stream);
-  launch_reduce_partials(partial_, grid_for(dm.Nc), 3, scalars_, true, stream);
+  launch_reduce_tiles(partial_, dm.ctile, 3, res_, 10, true, stream);
 }
 
 void Context::kepsilon_sources() { launch_keps_sources(dm, df, kw_, stream); }
@@ -486,7 +564,7 @@ int Context::solve_scalar(int isca, double rtol, int maxit, s355_solve_info* inf
   s355_ksp_opts o = par.ksp[S355_KSP_SCALAR];
   o.rtol = rtol;
   o.maxit = maxit;
-  return run_solve(2, df.bs, isca == 2 ? df.TE : df.ED, o, false, info);
+  return run_solve(2, df.bs, isca == 2 ? df.TE : df.ED, o, false, info, isca == 2 ? 4 : 5);
 }
 
 void Context::kepsilon_post_k() {
@@ -499,13 +577,24 @@ void Context::kepsilon_post_eps() {
   launch_turb_viscosity(dm, df, stream);
 }
 
-void Context::zero_residuals() { fill_zero(scalars_, 10 * sizeof(double), stream); }
+void Context::zero_residuals() { fill_zero(res_, 10 * sizeof(double) * ncase, stream); }
+
+void Context::read_case_residuals(double* out) {
+  // staged after the 4 KiB of fixed pinned slots
+  double* h = reinterpret_cast<double*>(static_cast<char*>(host_pinned_) + 4096);
+  S355_HIP(hipMemcpyAsync(h, res_, 10 * sizeof(double) * ncase, hipMemcpyDeviceToHost, stream));
+  sync();
+  std::memcpy(out, h, 10 * sizeof(double) * ncase);
+}
 
 void Context::read_residuals(double out[10]) {
-  double* h = static_cast<double*>(host_pinned_) + 256;
-  S355_HIP(hipMemcpyAsync(h, scalars_, 10 * sizeof(double), hipMemcpyDeviceToHost, stream));
-  sync();
-  std::memcpy(out, h, 10 * sizeof(double));
+  std::vector<double> all(10 * static_cast<size_t>(ncase));
+  read_case_residuals(all.data());
+  for (int j = 0; j < 10; ++j) {
+    double v = all[j];
+    for (int k = 1; k < ncase; ++k) v = std::max(v, all[10 * k + j]);
+    out[j] = v;
+  }
 }
 
 void Context::check_flags() {

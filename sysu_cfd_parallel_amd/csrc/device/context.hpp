@@ -20,13 +20,17 @@ struct FieldRef {
 
 class Context {
  public:
-  Context(const s355_mesh_desc& desc, const s355_params& params);
+  // ncase > 1: independent cases concatenated into one batch (see
+  // concat_meshes); every operation runs all of them in one launch per kernel
+  Context(const s355_mesh_desc* descs, int ncase, const s355_params& params);
   ~Context();
   Context(const Context&) = delete;
   Context& operator=(const Context&) = delete;
 
-  Mesh mesh;  // host copy (reference numbering)
+  Mesh mesh;  // host copy (reference numbering; batch layout for ncase > 1)
   s355_params par;
+  int ncase = 1;
+  BatchLayout lay;  // case offsets (one case: the whole mesh)
   hipStream_t stream = nullptr;
   DMesh dm{};
   DFields df{};
@@ -59,7 +63,10 @@ class Context {
 
   // residual slots and error flags on the device
   void zero_residuals();
-  void read_residuals(double out[10]);
+  void read_residuals(double out[10]);          // element-wise maximum over the cases
+  void read_case_residuals(double* out);        // 10 per case
+  // per-case records of the last solve of each kind: U, V, W, p', k, epsilon
+  std::vector<KResult> case_info[6];
   void check_flags();  // throws on a flag raised by a kernel
   void sync();
 
@@ -79,14 +86,18 @@ class Context {
   void allocate_fields();
   void free_all();
   int run_solve(int which_matrix, const double* b, double* x, const s355_ksp_opts& o, bool zero_guess,
-                s355_solve_info* info);
+                s355_solve_info* info, int slot);
+  Tiles make_tiles(const std::vector<int>& case_row, int tile);
   void ensure_mixed();
   void ensure_ilu();
 
   std::vector<void*> allocs_;
   double* partial_ = nullptr;
   size_t partial_len_ = 0;
-  double* scalars_ = nullptr;  // [0..9] residuals, [10..14] outlet correction
+  double* scalars_ = nullptr;  // [20..21] refinement norms
+  double* res_ = nullptr;      // 10 residual slots per case
+  double* rate_ = nullptr;     // outlet correction per case: mode, rate, m_in, m_out, a_out
+  KHeader* hdr_ = nullptr;     // pinned, one Krylov header per case
   int* flags_ = nullptr;       // [0] bad boundary type, [1] inlet outflow in a scalar
   double *u0_ = nullptr, *v0_ = nullptr, *w0_ = nullptr;
   KepsWork kw_{};

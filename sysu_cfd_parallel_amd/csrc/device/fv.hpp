@@ -19,8 +19,10 @@ void launch_correct_fields(const DMesh& m, const DFields& F, int ref, double urf
 void launch_correct_ruface(const DMesh& m, const DFields& F, double urf, hipStream_t s);
 void launch_residual3(const DMesh& m, const double* u0, const double* v0, const double* w0, const double* u,
                       const double* v, const double* w, double* partial, hipStream_t s);
-void launch_reduce_partials(const double* partial, int nblk, int ncomp, double* out, bool accumulate,
-                            hipStream_t s);
+// per-case sums of per-tile partials (ncomp interleaved columns) into
+// out[case * stride + comp], accumulated or stored
+void launch_reduce_tiles(const double* partial, const Tiles& t, int ncomp, double* out, int stride, bool accumulate,
+                         hipStream_t s);
 
 // k-epsilon (keps_kernels.hip)
 struct KepsWork {

@@ -94,9 +94,11 @@ void launch_gradient(const DMesh& m, int nf, const double* const* phi, const dou
 // per-block partial sums of inlet mass flow, outlet mass flow, outlet area.
 __global__ void __launch_bounds__(kBlock) k_bc_velocity(DMesh m, DFields F, double* partial) {
   __shared__ double lds[3][kBlock / kWave];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int a, e, c;
+  m.btile.range(blockIdx.x, a, e, c);
+  const int i = a + threadIdx.x;
   double min = 0., mout = 0., aout = 0.;
-  if (i < m.Nb) {
+  if (i < e) {
     const int t = m.b_type[i];
     const int ic = m.f_c1[i];
     if (t == kWall) {
@@ -143,14 +145,18 @@ __global__ void __launch_bounds__(kBlock) k_bc_velocity(DMesh m, DFields F, doub
   }
 }
 
-// Reduce the partial sums in a fixed order and decide the outlet correction
-// (navier_bc.cpp:85-117).  out = {mode, rate, m_in, m_out, a_out}.
-__global__ void __launch_bounds__(kBlock) k_bc_outlet_rate(const double* partial, int nblk, double* out) {
+// Reduce the partial sums of one case (block) in a fixed order and decide its
+// outlet correction (navier_bc.cpp:85-117).  out = {mode, rate, m_in, m_out,
+// a_out} per case.
+__global__ void __launch_bounds__(kBlock) k_bc_outlet_rate(DMesh m, const double* partial, double* out_all) {
   __shared__ double lds[kBlock / kWave];
+  int t0, t1;
+  m.btile.tiles_of(blockIdx.x, t0, t1);
+  double* out = out_all + 5 * blockIdx.x;
   double sum[3];
   for (int c = 0; c < 3; ++c) {
     double s = 0.;
-    for (int b = threadIdx.x; b < nblk; b += kBlock) s += partial[3 * b + c];
+    for (int b = t0 + threadIdx.x; b < t1; b += kBlock) s += partial[3 * b + c];
     sum[c] = block_sum(s, lds);
   }
   if (threadIdx.x == 0) {
@@ -168,9 +174,12 @@ __global__ void __launch_bounds__(kBlock) k_bc_outlet_rate(const double* partial
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_bc_outlet_apply(DMesh m, DFields F, const double* rate) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m.Nb || m.b_type[i] != kOutlet) return;
+__global__ void __launch_bounds__(kBlock) k_bc_outlet_apply(DMesh m, DFields F, const double* rate_all) {
+  int a, e, c;
+  m.btile.range(blockIdx.x, a, e, c);
+  const int i = a + threadIdx.x;
+  if (i >= e || m.b_type[i] != kOutlet) return;
+  const double* rate = rate_all + 5 * c;
   const double mode = rate[0], r = rate[1];
   if (mode == 1.) {
     F.BU[i] *= r;
@@ -187,11 +196,11 @@ __global__ void __launch_bounds__(kBlock) k_bc_outlet_apply(DMesh m, DFields F, 
 void launch_bc_velocity(const DMesh& m, const DFields& F, bool has_outlet, double* partial, double* rate,
                         hipStream_t s) {
   if (m.Nb == 0) return;
-  const int g = grid_for(m.Nb);
+  const int g = m.btile.ntile;
   hipLaunchKernelGGL(k_bc_velocity, dim3(g), dim3(kBlock), 0, s, m, F, partial);
   S355_LAUNCH_CHECK();
   if (!has_outlet) return;  // the correction touches outlet faces only
-  hipLaunchKernelGGL(k_bc_outlet_rate, dim3(1), dim3(kBlock), 0, s, partial, g, rate);
+  hipLaunchKernelGGL(k_bc_outlet_rate, dim3(m.btile.ncase), dim3(kBlock), 0, s, m, partial, rate);
   hipLaunchKernelGGL(k_bc_outlet_apply, dim3(g), dim3(kBlock), 0, s, m, F, rate);
   S355_LAUNCH_CHECK();
 }
@@ -471,13 +480,16 @@ void launch_build_pressure(const DMesh& m, const DFields& F, hipStream_t s) {
 
 // ---------------------------------------------------------------- corrections
 // Field correction of CalculatePressure (navier_pressure.cpp:52-70) and the
-// per-block partial of sum |dp - dp_ref| vol.
+// per-tile partial of sum |dp - dp_ref| vol.  ref is the reference cell's
+// index within its case.
 __global__ void __launch_bounds__(kBlock) k_correct_fields(DMesh m, DFields F, int ref, double urf, double* partial) {
   __shared__ double lds[kBlock / kWave];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int a, e, c;
+  m.ctile.range(blockIdx.x, a, e, c);
+  const int i = a + threadIdx.x;
   double r = 0.;
-  if (i < m.Nc) {
-    const double dref = F.dp[ref];
+  if (i < e) {
+    const double dref = F.dp[m.ctile.lo ? m.ctile.case_row[c] + ref : ref];
     const double vol = m.c_vol[i];
     const double d = F.dp[i] - dref;
     r = fabs(d) * vol;
@@ -492,7 +504,7 @@ __global__ void __launch_bounds__(kBlock) k_correct_fields(DMesh m, DFields F, i
 }
 
 void launch_correct_fields(const DMesh& m, const DFields& F, int ref, double urf, double* partial, hipStream_t s) {
-  hipLaunchKernelGGL(k_correct_fields, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, F, ref, urf, partial);
+  hipLaunchKernelGGL(k_correct_fields, dim3(m.ctile.ntile), dim3(kBlock), 0, s, m, F, ref, urf, partial);
   S355_LAUNCH_CHECK();
 }
 
@@ -536,9 +548,11 @@ __global__ void __launch_bounds__(kBlock) k_residual3(DMesh m, const double* u0,
                                                       const double* u, const double* v, const double* w,
                                                       double* partial) {
   __shared__ double lds[3][kBlock / kWave];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int a, e, cs;
+  m.ctile.range(blockIdx.x, a, e, cs);
+  const int i = a + threadIdx.x;
   double r[3] = {0., 0., 0.};
-  if (i < m.Nc) {
+  if (i < e) {
     const double vol = m.c_vol[i];
     r[0] = fabs(u0[i] - u[i]) * vol;
     r[1] = fabs(v0[i] - v[i]) * vol;
@@ -560,24 +574,31 @@ __global__ void __launch_bounds__(kBlock) k_residual3(DMesh m, const double* u0,
 
 void launch_residual3(const DMesh& m, const double* u0, const double* v0, const double* w0, const double* u,
                       const double* v, const double* w, double* partial, hipStream_t s) {
-  hipLaunchKernelGGL(k_residual3, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, u0, v0, w0, u, v, w, partial);
+  hipLaunchKernelGGL(k_residual3, dim3(m.ctile.ntile), dim3(kBlock), 0, s, m, u0, v0, w0, u, v, w, partial);
   S355_LAUNCH_CHECK();
 }
 
-// Sum `ncomp` interleaved partial columns in a fixed order; out[c] (+)= sum.
-__global__ void __launch_bounds__(kBlock) k_reduce_partials(const double* partial, int nblk, int ncomp, double* out,
-                                                            int accumulate) {
+// Per case (blockIdx.y) and component (blockIdx.x): sum the case's tiles of
+// `ncomp` interleaved partial columns in a fixed order; out[case * stride +
+// comp] (+)= sum.
+__global__ void __launch_bounds__(kBlock) k_reduce_tiles(const double* partial, Tiles t, int ncomp, double* out,
+                                                         int stride, int accumulate) {
   __shared__ double lds[kBlock / kWave];
-  const int c = blockIdx.x;
+  const int comp = blockIdx.x, c = blockIdx.y;
+  int t0, t1;
+  t.tiles_of(c, t0, t1);
   double s = 0.;
-  for (int b = threadIdx.x; b < nblk; b += kBlock) s += partial[ncomp * b + c];
+  for (int b = t0 + threadIdx.x; b < t1; b += kBlock) s += partial[ncomp * b + comp];
   s = block_sum(s, lds);
-  if (threadIdx.x == 0) out[c] = accumulate ? out[c] + s : s;
+  if (threadIdx.x == 0) {
+    double* o = out + static_cast<size_t>(c) * stride + comp;
+    *o = accumulate ? *o + s : s;
+  }
 }
 
-void launch_reduce_partials(const double* partial, int nblk, int ncomp, double* out, bool accumulate,
-                            hipStream_t s) {
-  hipLaunchKernelGGL(k_reduce_partials, dim3(ncomp), dim3(kBlock), 0, s, partial, nblk, ncomp, out,
+void launch_reduce_tiles(const double* partial, const Tiles& t, int ncomp, double* out, int stride, bool accumulate,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_tiles, dim3(ncomp, t.ncase), dim3(kBlock), 0, s, partial, t, ncomp, out, stride,
                      accumulate ? 1 : 0);
   S355_LAUNCH_CHECK();
 }

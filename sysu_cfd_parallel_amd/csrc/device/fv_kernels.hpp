@@ -3,6 +3,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "common.hpp"
+
 namespace s355 {
 
 // Boundary type codes (BdRegion::type1)
@@ -34,6 +36,10 @@ struct DMesh {
   // boundary records
   const int* b_type = nullptr;
   const double* b_dist = nullptr;
+  // independent cases stored one after the other (one for a single case):
+  // kBlock-row tiles of the cells and of the boundary faces, for the per-case
+  // reductions (residuals, outlet mass flow, pressure reference)
+  Tiles ctile, btile;
   // cells touching a wall, with their wall faces in ascending order
   int n_wcell = 0;
   const int* wc_cell = nullptr;

@@ -164,34 +164,39 @@ __device__ __forceinline__ T row_dot(const CsrView<T>& A, const T* __restrict__ 
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_spmv_pc(CsrView<T> A, const T* __restrict__ x, const T* __restrict__ inv,
-                                                    T* __restrict__ y, const KState* st, int it) {
-  if (st->reason) return;
+__global__ void __launch_bounds__(kBlock) k_spmv_pc(CsrView<T> A, Tiles t, const T* __restrict__ x,
+                                                    const T* __restrict__ inv, T* __restrict__ y, const KState* st,
+                                                    int it) {
   const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int row = g / kRowLanes, lane = g % kRowLanes;
+  // the lanes of one row share its case, so they leave together
+  const KState* sc = st + (row < A.n ? t.case_of(row) : 0);
+  if (sc->reason) return;
   const T s = row_dot(A, x, row, lane);
   if (lane == 0 && row < A.n) {
     double z = static_cast<double>(s);
     if (inv) z *= static_cast<double>(inv[row]);
-    y[row] = static_cast<T>(z * st->scale[it]);
+    y[row] = static_cast<T>(z * sc->scale[it]);
   }
 }
 
 inline int spmv_grid(int n) { return grid_for(static_cast<long>(n) * kRowLanes); }
 
-// ---- partial[b][k] = sum over block b of w_i V_k,i, k = 0..it
+// ---- partial[b][k] = sum over tile b of w_i V_k,i, k = 0..it
 template <class T, int R>
-__global__ void __launch_bounds__(kBlock) k_mdot(const T* __restrict__ V, size_t ld, int n, int it,
+__global__ void __launch_bounds__(kBlock) k_mdot(const T* __restrict__ V, size_t ld, Tiles t, int it,
                                                  double* __restrict__ partial, int pstride, const KState* st) {
-  if (st->reason) return;
+  int a, e, c;
+  t.range(blockIdx.x, a, e, c);
+  if (st[c].reason) return;
   __shared__ double lds[kBlock / kWave][8];
   const T* w = V + static_cast<size_t>(it + 1) * ld;
-  const size_t base = static_cast<size_t>(blockIdx.x) * (kBlock * R) + threadIdx.x;
+  const size_t base = static_cast<size_t>(a) + threadIdx.x, end = static_cast<size_t>(e);
   double wr[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const size_t idx = base + static_cast<size_t>(r) * kBlock;
-    wr[r] = idx < static_cast<size_t>(n) ? static_cast<double>(w[idx]) : 0.0;
+    wr[r] = idx < end ? static_cast<double>(w[idx]) : 0.0;
   }
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
   for (int k0 = 0; k0 <= it; k0 += 8) {
@@ -205,7 +210,7 @@ __global__ void __launch_bounds__(kBlock) k_mdot(const T* __restrict__ V, size_t
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const size_t idx = base + static_cast<size_t>(r) * kBlock;
-          if (idx < static_cast<size_t>(n)) acc[kk] += wr[r] * static_cast<double>(vk[idx]);
+          if (idx < end) acc[kk] += wr[r] * static_cast<double>(vk[idx]);
         }
       }
     }
@@ -225,35 +230,42 @@ __global__ void __launch_bounds__(kBlock) k_mdot(const T* __restrict__ V, size_t
   }
 }
 
-// ---- hcol[k] = scale_k * sum_b partial[b][k]  (one block per k)
-__global__ void __launch_bounds__(kBlock) k_mdot_final(const double* partial, int nblk, int pstride, KState* st) {
-  if (st->reason) return;
+// ---- hcol[k] = scale_k * sum over the case's tiles of partial[b][k]
+// (block (k, case))
+__global__ void __launch_bounds__(kBlock) k_mdot_final(const double* partial, Tiles t, int pstride, KState* st) {
+  KState* s = st + blockIdx.y;
+  if (s->reason) return;
   __shared__ double lds[kBlock / kWave];
   const int k = blockIdx.x;
-  double s = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += kBlock) s += partial[static_cast<size_t>(b) * pstride + k];
-  s = block_sum(s, lds);
-  if (threadIdx.x == 0) st->hcol[k] = s * st->scale[k];
+  int t0, t1;
+  t.tiles_of(blockIdx.y, t0, t1);
+  double v = 0.0;
+  for (int b = t0 + threadIdx.x; b < t1; b += kBlock) v += partial[static_cast<size_t>(b) * pstride + k];
+  v = block_sum(v, lds);
+  if (threadIdx.x == 0) s->hcol[k] = v * s->scale[k];
 }
 
-// ---- w -= sum_k hcol_k scale_k V_k ; partial ||w||^2
+// ---- w -= sum_k hcol_k scale_k V_k ; partial ||w||^2 per tile
 template <class T, int R>
-__global__ void __launch_bounds__(kBlock) k_maxpy_norm(T* __restrict__ V, size_t ld, int n, int it,
+__global__ void __launch_bounds__(kBlock) k_maxpy_norm(T* __restrict__ V, size_t ld, Tiles t, int it,
                                                        double* __restrict__ partial, const KState* st) {
-  if (st->reason) return;
-  __shared__ double c[kMaxRestart + 1];
+  int a, e, c;
+  t.range(blockIdx.x, a, e, c);
+  const KState* sc = st + c;
+  if (sc->reason) return;
+  __shared__ double cf[kMaxRestart + 1];
   __shared__ double lds[kBlock / kWave];
-  for (int k = threadIdx.x; k <= it; k += kBlock) c[k] = st->hcol[k] * st->scale[k];
+  for (int k = threadIdx.x; k <= it; k += kBlock) cf[k] = sc->hcol[k] * sc->scale[k];
   __syncthreads();
   T* w = V + static_cast<size_t>(it + 1) * ld;
-  const size_t base = static_cast<size_t>(blockIdx.x) * (kBlock * R) + threadIdx.x;
+  const size_t base = static_cast<size_t>(a) + threadIdx.x, end = static_cast<size_t>(e);
   double nrm = 0.0;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const size_t idx = base + static_cast<size_t>(r) * kBlock;
-    if (idx < static_cast<size_t>(n)) {
+    if (idx < end) {
       double s = static_cast<double>(w[idx]);
-      for (int k = 0; k <= it; ++k) s -= c[k] * static_cast<double>(V[static_cast<size_t>(k) * ld + idx]);
+      for (int k = 0; k <= it; ++k) s -= cf[k] * static_cast<double>(V[static_cast<size_t>(k) * ld + idx]);
       const T sv = static_cast<T>(s);
       w[idx] = sv;
       nrm += static_cast<double>(sv) * static_cast<double>(sv);
@@ -263,37 +275,38 @@ __global__ void __launch_bounds__(kBlock) k_maxpy_norm(T* __restrict__ V, size_t
   if (threadIdx.x == 0) partial[blockIdx.x] = nrm;
 }
 
-// ---- Hessenberg column, Givens rotations, convergence test (one block).
-// The column and the previous rotations are staged in LDS, so the serial
-// rotation sweep of thread 0 runs out of LDS rather than global memory.
-__global__ void __launch_bounds__(kBlock) k_arnoldi_finalize(const double* partial, int nblk, int it, int m,
+// ---- Hessenberg column, Givens rotations, convergence test (one block per
+// case).  The column and the previous rotations are staged in LDS, so the
+// serial rotation sweep of thread 0 runs out of LDS rather than global memory.
+__global__ void __launch_bounds__(kBlock) k_arnoldi_finalize(const double* partial, Tiles t, int it, int m,
                                                              KState* st) {
-  if (st->reason) return;
+  KState* s_ = st + blockIdx.x;
+  if (s_->reason) return;
   __shared__ double lds[kBlock / kWave];
   __shared__ double col[kMaxRestart + 2], cc[kMaxRestart + 1], ss[kMaxRestart + 1];
-  __shared__ double res_sh;
-  __shared__ int reason_sh;
+  int t0, t1;
+  t.tiles_of(blockIdx.x, t0, t1);
   double s = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += kBlock) s += partial[b];
+  for (int b = t0 + threadIdx.x; b < t1; b += kBlock) s += partial[b];
   for (int j = threadIdx.x; j <= it; j += kBlock) {
-    col[j] = st->hcol[j];
-    cc[j] = st->cc[j];
-    ss[j] = st->ss[j];
+    col[j] = s_->hcol[j];
+    cc[j] = s_->cc[j];
+    ss[j] = s_->ss[j];
   }
   s = block_sum(s, lds);  // contains the barrier that publishes col/cc/ss
   const int ld = m + 1;
   if (threadIdx.x == 0) {
     const double tt = sqrt(s);
     col[it + 1] = tt;
-    st->scale[it + 1] = tt != 0.0 ? 1.0 / tt : 1.0;
-    const double g_it = st->grs[it];
+    s_->scale[it + 1] = tt != 0.0 ? 1.0 / tt : 1.0;
+    const double g_it = s_->grs[it];
     double hapbnd = fabs(tt / g_it);
     if (hapbnd > 1.e-30) hapbnd = 1.e-30;
     const bool hapend = tt < hapbnd;
     for (int j = 1; j <= it; ++j) {
-      const double t0 = col[j - 1];
-      col[j - 1] = cc[j - 1] * t0 + ss[j - 1] * col[j];
-      col[j] = cc[j - 1] * col[j] - ss[j - 1] * t0;
+      const double t0v = col[j - 1];
+      col[j - 1] = cc[j - 1] * t0v + ss[j - 1] * col[j];
+      col[j] = cc[j - 1] * col[j] - ss[j - 1] * t0v;
     }
     double res;
     int reason = 0;
@@ -302,82 +315,86 @@ __global__ void __launch_bounds__(kBlock) k_arnoldi_finalize(const double* parti
       const double tr = sqrt(hv * hv + hn * hn);
       if (tr == 0.0) {
         reason = kDivergedNull;
-        res = st->rnorm;
+        res = s_->rnorm;
       } else {
         const double c = hv / tr, sn = hn / tr;
-        st->cc[it] = c;
-        st->ss[it] = sn;
-        st->grs[it + 1] = -(sn * g_it);
-        st->grs[it] = c * g_it;
+        s_->cc[it] = c;
+        s_->ss[it] = sn;
+        s_->grs[it + 1] = -(sn * g_it);
+        s_->grs[it] = c * g_it;
         col[it] = c * hv + sn * hn;
-        res = fabs(st->grs[it + 1]);
+        res = fabs(s_->grs[it + 1]);
       }
     } else {
       res = 0.0;
     }
-    st->it = it + 1;
-    st->its += 1;
-    st->rnorm = res;
+    s_->it = it + 1;
+    s_->its += 1;
+    s_->rnorm = res;
     if (!reason) {
-      reason = conv_check(st, res);
+      reason = conv_check(s_, res);
       if (hapend && !reason) reason = kDivergedBreakdown;
     }
-    st->reason = reason;
-    res_sh = res;
-    reason_sh = reason;
+    s_->reason = reason;
   }
   __syncthreads();
-  for (int j = threadIdx.x; j <= it + 1; j += kBlock) Hm(st, ld, j, it) = col[j];
+  for (int j = threadIdx.x; j <= it + 1; j += kBlock) Hm(s_, ld, j, it) = col[j];
 }
 
-// ---- back substitution R y = g on the it completed columns, y_j scaled by
-// scale_j; one wavefront, column-oriented: lane i keeps t_i, and once y_r is
-// known every lane i < r subtracts H(i,r) y_r.
-__global__ void __launch_bounds__(kWave) k_build_y(KState* st, int m, int* nupd) {
+// ---- back substitution R y = g on the completed columns of each case, y_j
+// scaled by scale_j; one wavefront per case, column-oriented: lane i keeps
+// t_i, and once y_r is known every lane i < r subtracts H(i,r) y_r.  The
+// number of terms of the update goes to the state's pad word.
+__global__ void __launch_bounds__(kWave) k_build_y(KState* st, int m) {
+  KState* s = st + blockIdx.x;
+  int* nupd = reinterpret_cast<int*>(&s->pad);
   const int lane = threadIdx.x;
   const int ld = m + 1;
-  const int k = st->it - 1;
+  const int k = s->it - 1;
   if (k < 0) {
     if (lane == 0) *nupd = 0;
     return;
   }
-  double t = lane <= k ? st->grs[lane] : 0.0;
+  double t = lane <= k ? s->grs[lane] : 0.0;
   double y = 0.0;
   bool broke = false;
   for (int r = k; r >= 0; --r) {
-    const double hrr = Hm(st, ld, r, r);
+    const double hrr = Hm(s, ld, r, r);
     if (hrr == 0.0) {
       broke = true;
       break;
     }
     const double yr = __shfl(t, r, kWave) / hrr;
     if (lane == r) y = yr;
-    if (lane < r) t -= Hm(st, ld, lane, r) * yr;
+    if (lane < r) t -= Hm(s, ld, lane, r) * yr;
   }
   if (broke) {
     if (lane == 0) {
-      st->reason = kDivergedBreakdown;
+      s->reason = kDivergedBreakdown;
       *nupd = 0;
     }
     return;
   }
-  if (lane <= k) st->y[lane] = y * st->scale[lane];
+  if (lane <= k) s->y[lane] = y * s->scale[lane];
   if (lane == 0) *nupd = k + 1;
 }
 
 template <class T, class X, int R>
-__global__ void __launch_bounds__(kBlock) k_update_x(const T* __restrict__ V, size_t ld, int n, X* __restrict__ x,
-                                                     const KState* st, const int* nupd) {
-  __shared__ double y[kMaxRestart + 1];
-  const int nk = *nupd;
-  for (int k = threadIdx.x; k < nk; k += kBlock) y[k] = st->y[k];
-  __syncthreads();
+__global__ void __launch_bounds__(kBlock) k_update_x(const T* __restrict__ V, size_t ld, Tiles t, X* __restrict__ x,
+                                                     const KState* st) {
+  int a, e, c;
+  t.range(blockIdx.x, a, e, c);
+  const KState* sc = st + c;
+  const int nk = *reinterpret_cast<const int*>(&sc->pad);
   if (nk == 0) return;
-  const size_t base = static_cast<size_t>(blockIdx.x) * (kBlock * R) + threadIdx.x;
+  __shared__ double y[kMaxRestart + 1];
+  for (int k = threadIdx.x; k < nk; k += kBlock) y[k] = sc->y[k];
+  __syncthreads();
+  const size_t base = static_cast<size_t>(a) + threadIdx.x, end = static_cast<size_t>(e);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const size_t idx = base + static_cast<size_t>(r) * kBlock;
-    if (idx < static_cast<size_t>(n)) {
+    if (idx < end) {
       double s = 0.0;
       for (int k = 0; k < nk; ++k) s += y[k] * static_cast<double>(V[static_cast<size_t>(k) * ld + idx]);
       x[idx] = static_cast<X>(static_cast<double>(x[idx]) + s);
@@ -385,18 +402,22 @@ __global__ void __launch_bounds__(kBlock) k_update_x(const T* __restrict__ V, si
   }
 }
 
-// ---- v0 = M^-1 (b - A x); partial ||v0||^2.  The residual is formed in the
-// precision X of the solution and the matrix A (FP64 for the mixed solve,
-// whose Arnoldi cycle runs on an FP32 copy); v0 is stored in the basis type T
-// and the norm is taken before that rounding when T and X differ.
+// ---- v0 = M^-1 (b - A x); partial ||v0||^2 per tile.  The residual is
+// formed in the precision X of the solution and the matrix A (FP64 for the
+// mixed solve, whose Arnoldi cycle runs on an FP32 copy); v0 is stored in the
+// basis type T and the norm is taken before that rounding when T and X
+// differ.  Tiles of finished cases are skipped.
 template <class T, class X>
-__global__ void __launch_bounds__(kBlock) k_residual_pc(CsrView<X> A, const X* __restrict__ b,
+__global__ void __launch_bounds__(kBlock) k_residual_pc(CsrView<X> A, Tiles t, const X* __restrict__ b,
                                                         const X* __restrict__ x, const X* __restrict__ inv,
-                                                        T* __restrict__ v0, double* partial, int zero_guess) {
+                                                        T* __restrict__ v0, double* partial, int zero_guess,
+                                                        const KState* st) {
+  int a, e, c;
+  t.range(blockIdx.x, a, e, c);
+  if (st[c].done) return;
   __shared__ double lds[kBlock / kWave];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double nrm = 0.0;
-  if (i < A.n) {
+  for (int i = a + threadIdx.x; i < e; i += kBlock) {
     double r = static_cast<double>(b[i]);
     if (!zero_guess) {
       X s = X(0);
@@ -407,7 +428,7 @@ __global__ void __launch_bounds__(kBlock) k_residual_pc(CsrView<X> A, const X* _
     if (inv) r *= static_cast<double>(inv[i]);
     const X rx = static_cast<X>(r);
     v0[i] = static_cast<T>(rx);
-    nrm = static_cast<double>(rx) * static_cast<double>(rx);
+    nrm += static_cast<double>(rx) * static_cast<double>(rx);
   }
   nrm = block_sum(nrm, lds);
   if (threadIdx.x == 0) partial[blockIdx.x] = nrm;
@@ -424,57 +445,110 @@ __global__ void __launch_bounds__(kBlock) k_jacobi_mixed(CsrView<double> A, doub
   inv32[i] = static_cast<float>(v);
 }
 
-__global__ void k_clear_reason(KState* st) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) st->reason = 0;
-}
-
-// ---- partial ||M^-1 b||^2 (reference norm for a nonzero initial guess)
+// ---- partial ||M^-1 b||^2 per tile (reference norm for a nonzero guess)
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_pc_norm(int n, const T* __restrict__ b, const T* __restrict__ inv,
-                                                    double* partial) {
+__global__ void __launch_bounds__(kBlock) k_pc_norm(Tiles t, const T* __restrict__ b, const T* __restrict__ inv,
+                                                    double* partial, const KState* st) {
+  int a, e, c;
+  t.range(blockIdx.x, a, e, c);
+  if (st[c].done) return;
   __shared__ double lds[kBlock / kWave];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double nrm = 0.0;
-  if (i < n) {
+  for (int i = a + threadIdx.x; i < e; i += kBlock) {
     double r = static_cast<double>(b[i]);
     if (inv) r *= static_cast<double>(inv[i]);
     const T rv = static_cast<T>(r);
-    nrm = static_cast<double>(rv) * static_cast<double>(rv);
+    nrm += static_cast<double>(rv) * static_cast<double>(rv);
   }
   nrm = block_sum(nrm, lds);
   if (threadIdx.x == 0) partial[blockIdx.x] = nrm;
 }
 
-__global__ void __launch_bounds__(kBlock) k_snorm_final(const double* partial, int nblk, KState* st) {
+__global__ void __launch_bounds__(kBlock) k_snorm_final(const double* partial, Tiles t, KState* st) {
   __shared__ double lds[kBlock / kWave];
+  int t0, t1;
+  t.tiles_of(blockIdx.x, t0, t1);
   double s = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += kBlock) s += partial[b];
+  for (int b = t0 + threadIdx.x; b < t1; b += kBlock) s += partial[b];
   s = block_sum(s, lds);
-  if (threadIdx.x == 0) st->snorm = sqrt(s);
+  if (threadIdx.x == 0) st[blockIdx.x].snorm = sqrt(s);
 }
 
-// ---- start of a GMRES cycle
-__global__ void __launch_bounds__(kBlock) k_restart_finalize(const double* partial, int nblk, KState* st, int first,
-                                                             int guess_zero) {
+// ---- start of a GMRES cycle (one block per case).  The restart residual is
+// the true one, so a passed test here ends the case's solve; in the mixed
+// solve the iteration cap is applied here too, after the FP64 test.
+__global__ void __launch_bounds__(kBlock) k_restart_finalize(const double* partial, Tiles t, KState* st, int first,
+                                                             int guess_zero, int mixed) {
+  KState* s_ = st + blockIdx.x;
+  if (s_->done) {
+    if (threadIdx.x == 0) s_->it = 0;
+    return;
+  }
   __shared__ double lds[kBlock / kWave];
+  int t0, t1;
+  t.tiles_of(blockIdx.x, t0, t1);
   double s = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += kBlock) s += partial[b];
+  for (int b = t0 + threadIdx.x; b < t1; b += kBlock) s += partial[b];
   s = block_sum(s, lds);
   if (threadIdx.x != 0) return;
   const double res = sqrt(s);
-  st->grs[0] = res;
-  st->scale[0] = res != 0.0 ? 1.0 / res : 1.0;
-  st->it = 0;
-  st->rnorm = res;
+  s_->grs[0] = res;
+  s_->scale[0] = res != 0.0 ? 1.0 / res : 1.0;
+  s_->it = 0;
+  s_->cycles += 1;
+  s_->rnorm = res;
   if (first) {
-    st->rnorm0 = guess_zero ? res : (st->snorm != 0.0 ? st->snorm : res);
-    st->ttol = fmax(st->rtol * st->rnorm0, st->abstol);
+    s_->rnorm0 = guess_zero ? res : (s_->snorm != 0.0 ? s_->snorm : res);
+    s_->ttol = fmax(s_->rtol * s_->rnorm0, s_->abstol);
   }
-  if (res == 0.0) {
-    st->reason = kConvergedAtol;
-    return;
+  int reason = res == 0.0 ? kConvergedAtol : conv_check(s_, res);
+  if (mixed && !reason && s_->its >= s_->maxit) reason = kDivergedIts;
+  s_->reason = reason;
+  if (reason) s_->done = 1;
+}
+
+// ---- end of a GMRES cycle, after the update of x (one thread per case).
+// Plain solve: a set reason ends the case, then the iteration cap.  Mixed
+// solve: a passed estimate or a breakdown of the FP32 cycle is cleared and
+// goes back to the FP64 residual; a divergence ends the case.
+__global__ void k_cycle_end(KState* st, int ncase, int mixed) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncase; c += gridDim.x * blockDim.x) {
+    KState* s = st + c;
+    if (s->done) continue;
+    const int r = s->reason;
+    if (!mixed) {
+      if (r) {
+        s->done = 1;
+      } else if (s->its >= s->maxit) {
+        s->reason = kDivergedIts;
+        s->done = 1;
+      }
+    } else {
+      if (r < 0 && r != kDivergedBreakdown)
+        s->done = 1;
+      else
+        s->reason = 0;
+    }
   }
-  st->reason = conv_check(st, res);
+}
+
+// ---- solver state of every case at the start of a solve
+__global__ void k_init_states(KState* st, int ncase, int maxit, double rtol, double abstol, double dtol) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncase; c += gridDim.x * blockDim.x) {
+    KState* s = st + c;
+    s->it = 0;
+    s->its = 0;
+    s->reason = 0;
+    s->maxit = maxit;
+    s->done = 0;
+    s->cycles = 0;
+    s->rnorm = s->rnorm0 = s->ttol = s->snorm = 0.0;
+    s->rtol = rtol;
+    s->abstol = abstol;
+    s->dtol = dtol;
+    s->pad = 0.0;
+    s->beta = s->betaold = s->dpi = s->dpiold = s->alpha = 0.0;
+  }
 }
 
 // ---- CG pieces (left-preconditioned, preconditioned residual norm)
@@ -652,83 +726,121 @@ int pick_r(int n, size_t elem) {
   return 1;
 }
 
-template <class T, int R>
-void launch_mdot(const T* V, size_t ld, int n, int it, double* partial, int pstride, const KState* st, hipStream_t s) {
-  const int g = grid_for(n, kBlock * R);
-  hipLaunchKernelGGL((k_mdot<T, R>), dim3(g), dim3(kBlock), 0, s, V, ld, n, it, partial, pstride, st);
-}
-template <class T, int R>
-void launch_maxpy(T* V, size_t ld, int n, int it, double* partial, const KState* st, hipStream_t s) {
-  const int g = grid_for(n, kBlock * R);
-  hipLaunchKernelGGL((k_maxpy_norm<T, R>), dim3(g), dim3(kBlock), 0, s, V, ld, n, it, partial, st);
-}
-template <class T, class X, int R>
-void launch_update(const T* V, size_t ld, int n, X* x, const KState* st, const int* nupd, hipStream_t s) {
-  const int g = grid_for(n, kBlock * R);
-  hipLaunchKernelGGL((k_update_x<T, X, R>), dim3(g), dim3(kBlock), 0, s, V, ld, n, x, st, nupd);
-}
+int log2_r(int R) { return R == 16 ? 4 : R == 8 ? 3 : R == 4 ? 2 : R == 2 ? 1 : 0; }
 
 template <class T>
 struct Dispatch {
-  static void mdot(int R, const T* V, size_t ld, int n, int it, double* partial, int pstride, const KState* st,
-                   hipStream_t s) {
+  template <int R>
+  static void mdot_r(const T* V, size_t ld, const Tiles& t, int it, double* partial, int pstride, const KState* st,
+                     hipStream_t s) {
+    hipLaunchKernelGGL((k_mdot<T, R>), dim3(t.ntile), dim3(kBlock), 0, s, V, ld, t, it, partial, pstride, st);
+  }
+  template <int R>
+  static void maxpy_r(T* V, size_t ld, const Tiles& t, int it, double* partial, const KState* st, hipStream_t s) {
+    hipLaunchKernelGGL((k_maxpy_norm<T, R>), dim3(t.ntile), dim3(kBlock), 0, s, V, ld, t, it, partial, st);
+  }
+  template <int R, class X>
+  static void update_r(const T* V, size_t ld, const Tiles& t, X* x, const KState* st, hipStream_t s) {
+    hipLaunchKernelGGL((k_update_x<T, X, R>), dim3(t.ntile), dim3(kBlock), 0, s, V, ld, t, x, st);
+  }
+  static void mdot(int R, const T* V, size_t ld, const Tiles& t, int it, double* partial, int pstride,
+                   const KState* st, hipStream_t s) {
     switch (R) {
-      case 16: launch_mdot<T, 16>(V, ld, n, it, partial, pstride, st, s); break;
-      case 8: launch_mdot<T, 8>(V, ld, n, it, partial, pstride, st, s); break;
-      case 4: launch_mdot<T, 4>(V, ld, n, it, partial, pstride, st, s); break;
-      case 2: launch_mdot<T, 2>(V, ld, n, it, partial, pstride, st, s); break;
-      default: launch_mdot<T, 1>(V, ld, n, it, partial, pstride, st, s); break;
+      case 16: mdot_r<16>(V, ld, t, it, partial, pstride, st, s); break;
+      case 8: mdot_r<8>(V, ld, t, it, partial, pstride, st, s); break;
+      case 4: mdot_r<4>(V, ld, t, it, partial, pstride, st, s); break;
+      case 2: mdot_r<2>(V, ld, t, it, partial, pstride, st, s); break;
+      default: mdot_r<1>(V, ld, t, it, partial, pstride, st, s); break;
     }
   }
-  static void maxpy(int R, T* V, size_t ld, int n, int it, double* partial, const KState* st, hipStream_t s) {
+  static void maxpy(int R, T* V, size_t ld, const Tiles& t, int it, double* partial, const KState* st,
+                    hipStream_t s) {
     switch (R) {
-      case 16: launch_maxpy<T, 16>(V, ld, n, it, partial, st, s); break;
-      case 8: launch_maxpy<T, 8>(V, ld, n, it, partial, st, s); break;
-      case 4: launch_maxpy<T, 4>(V, ld, n, it, partial, st, s); break;
-      case 2: launch_maxpy<T, 2>(V, ld, n, it, partial, st, s); break;
-      default: launch_maxpy<T, 1>(V, ld, n, it, partial, st, s); break;
+      case 16: maxpy_r<16>(V, ld, t, it, partial, st, s); break;
+      case 8: maxpy_r<8>(V, ld, t, it, partial, st, s); break;
+      case 4: maxpy_r<4>(V, ld, t, it, partial, st, s); break;
+      case 2: maxpy_r<2>(V, ld, t, it, partial, st, s); break;
+      default: maxpy_r<1>(V, ld, t, it, partial, st, s); break;
     }
   }
   template <class X>
-  static void update(int R, const T* V, size_t ld, int n, X* x, const KState* st, const int* nupd, hipStream_t s) {
+  static void update(int R, const T* V, size_t ld, const Tiles& t, X* x, const KState* st, hipStream_t s) {
     switch (R) {
-      case 16: launch_update<T, X, 16>(V, ld, n, x, st, nupd, s); break;
-      case 8: launch_update<T, X, 8>(V, ld, n, x, st, nupd, s); break;
-      case 4: launch_update<T, X, 4>(V, ld, n, x, st, nupd, s); break;
-      case 2: launch_update<T, X, 2>(V, ld, n, x, st, nupd, s); break;
-      default: launch_update<T, X, 1>(V, ld, n, x, st, nupd, s); break;
+      case 16: update_r<16>(V, ld, t, x, st, s); break;
+      case 8: update_r<8>(V, ld, t, x, st, s); break;
+      case 4: update_r<4>(V, ld, t, x, st, s); break;
+      case 2: update_r<2>(V, ld, t, x, st, s); break;
+      default: update_r<1>(V, ld, t, x, st, s); break;
     }
   }
 };
 
-void read_header(KWork& w, hipStream_t s) {
-  S355_HIP(hipMemcpyAsync(w.host, w.st, sizeof(KHeader), hipMemcpyDeviceToHost, s));
+// headers of every case: one strided copy out of the state array
+void read_headers(KWork& w, hipStream_t s) {
+  S355_HIP(hipMemcpy2DAsync(w.host, sizeof(KHeader), w.st, sizeof(KState), sizeof(KHeader), w.ncase,
+                            hipMemcpyDeviceToHost, s));
   S355_HIP(hipStreamSynchronize(s));
 }
 
-void init_state(KWork& w, const s355_ksp_opts& o, hipStream_t s) {
-  KState h{};
-  h.reason = 0;
-  h.maxit = o.maxit;
-  h.rtol = o.rtol;
-  h.abstol = o.abstol;
-  h.dtol = o.dtol;
-  // header + scalars only: the arrays are written before being read
-  S355_HIP(hipMemcpyAsync(w.st, &h, offsetof(KState, H), hipMemcpyHostToDevice, s));
-  S355_HIP(hipMemcpyAsync(&w.st->beta, &h.beta, 5 * sizeof(double), hipMemcpyHostToDevice, s));
-  // the pinned staging copy must not be reused before the copy completes
-  S355_HIP(hipStreamSynchronize(s));
+bool all_done(const KWork& w) {
+  for (int c = 0; c < w.ncase; ++c)
+    if (!w.host[c].done) return false;
+  return true;
 }
 
-// Restarted GMRES(m).  The Arnoldi cycle runs on A (values and basis of type
-// T); each restart forms the residual b - Ares x in the solution type X.  With
-// T = X and Ares = A this is the plain solve.  With T = float, X = double it
-// is the mixed solve: every cycle is one step of FP64 iterative refinement
-// (FP64 residual with the FP64 matrix, FP32 Krylov correction accumulated
-// into the FP64 x), so refinement costs no restarts beyond GMRES(m)'s own,
-// and convergence is accepted only on the FP64 residual: a cycle whose
-// Givens estimate passes the test ends early and the restart residual
-// decides.  inv_x (mixed Jacobi only) holds the FP64 inverse diagonal.
+bool all_stopped(const KWork& w) {
+  for (int c = 0; c < w.ncase; ++c)
+    if (!w.host[c].done && !w.host[c].reason) return false;
+  return true;
+}
+
+void init_states(KWork& w, const s355_ksp_opts& o, hipStream_t s) {
+  hipLaunchKernelGGL(k_init_states, dim3(grid_for(w.ncase)), dim3(kBlock), 0, s, w.st, w.ncase, o.maxit, o.rtol,
+                     o.abstol, o.dtol);
+  S355_LAUNCH_CHECK();
+}
+
+// Per-case results from the headers; the aggregate is the worst case: the
+// first divergence if any, the largest iteration count, the largest
+// relative residual.
+KResult collect(KWork& w, bool mixed) {
+  KResult agg;
+  double worst = -1.0;
+  if (w.per_case) w.per_case->assign(w.ncase, KResult{});
+  for (int c = 0; c < w.ncase; ++c) {
+    const KHeader& h = w.host[c];
+    KResult r;
+    r.iters = h.its;
+    r.reason = h.reason;
+    r.rnorm = h.rnorm;
+    r.rnorm0 = h.rnorm0;
+    r.refinements = mixed ? h.cycles : 0;
+    if (w.per_case) (*w.per_case)[c] = r;
+    agg.iters = std::max(agg.iters, r.iters);
+    agg.refinements = std::max(agg.refinements, r.refinements);
+    if (r.reason < 0 && agg.reason >= 0) agg.reason = r.reason;
+    if (agg.reason >= 0 && (agg.reason == 0 || r.reason < agg.reason)) agg.reason = r.reason;
+    const double rel = r.rnorm0 > 0.0 ? r.rnorm / r.rnorm0 : r.rnorm;
+    if (rel > worst) {
+      worst = rel;
+      agg.rnorm = r.rnorm;
+      agg.rnorm0 = r.rnorm0;
+    }
+  }
+  return agg;
+}
+
+// Restarted GMRES(m) over the w.ncase systems stored one after the other.
+// The Arnoldi cycle runs on A (values and basis of type T); each restart
+// forms the residual b - Ares x in the solution type X.  With T = X and
+// Ares = A this is the plain solve.  With T = float, X = double it is the
+// mixed solve: every cycle is one step of FP64 iterative refinement (FP64
+// residual with the FP64 matrix, FP32 Krylov correction accumulated into the
+// FP64 x), so refinement costs no restarts beyond GMRES(m)'s own, and
+// convergence is accepted only on the FP64 residual: a cycle whose Givens
+// estimate passes the test ends early and the restart residual decides.
+// inv_x (mixed Jacobi only) holds the FP64 inverse diagonal.  Cases run their
+// cycles in lockstep; a case that has finished drops out of every kernel.
 template <class T, class X>
 KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, const s355_ksp_opts& o, bool zero_guess,
               KWork& w, X* inv_x, hipStream_t s) {
@@ -736,8 +848,14 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
   const int n = A.n;
   const int m = o.restart < 1 ? 1 : (o.restart > kMaxRestart ? kMaxRestart : o.restart);
   if (m > w.mmax) throw std::invalid_argument("gmres: restart above workspace size");
-  if (Ares.n != n) throw std::invalid_argument("gmres: operator sizes differ");
+  if (Ares.n != n || w.n != n) throw std::invalid_argument("gmres: operator sizes differ");
   const size_t ld = static_cast<size_t>(n);
+  const int R = pick_r(n, sizeof(T));
+  const Tiles& tl = w.tiles[log2_r(R)];
+  if (tl.tile != kBlock * R || tl.n != n) throw std::runtime_error("gmres: tile table missing");
+  const int nc = w.ncase;
+  // ILU sweeps skip their work only for a single system
+  const KState* ilu_st = nc == 1 ? w.st : nullptr;
   T* V = static_cast<T*>(w.V);
   T* inv = nullptr;
   const X* invr = nullptr;
@@ -760,14 +878,9 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
   } else if (o.pc != S355_PC_NONE) {
     throw std::invalid_argument("gmres: unknown preconditioner");
   }
-  init_state(w, o, s);
-  const int g1 = grid_for(n);
-  const int R = pick_r(n, sizeof(T));
-  const int gR = grid_for(n, kBlock * R);
+  init_states(w, o, s);
   const int pstride = m + 1;
-  if (static_cast<size_t>(gR) * pstride > w.partial_len || static_cast<size_t>(g1) > w.partial_len)
-    throw std::runtime_error("gmres: partial buffer too small");
-  int* nupd = reinterpret_cast<int*>(&w.st->pad);
+  if (static_cast<size_t>(tl.ntile) * pstride > w.partial_len) throw std::runtime_error("gmres: partial buffer too small");
   if (!zero_guess) {
     if (lu) {
       const T* bt;
@@ -777,92 +890,70 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
       } else {
         bt = b;
       }
-      ilu_apply(A, *w.ilu, lu, bt, V, w.st, s);
-      hipLaunchKernelGGL((k_pc_norm<T>), dim3(g1), dim3(kBlock), 0, s, n, V, static_cast<const T*>(nullptr),
-                         w.partial);
+      ilu_apply(A, *w.ilu, lu, bt, V, ilu_st, s);
+      hipLaunchKernelGGL((k_pc_norm<T>), dim3(tl.ntile), dim3(kBlock), 0, s, tl, V, static_cast<const T*>(nullptr),
+                         w.partial, w.st);
     } else {
-      hipLaunchKernelGGL((k_pc_norm<X>), dim3(g1), dim3(kBlock), 0, s, n, b, invr, w.partial);
+      hipLaunchKernelGGL((k_pc_norm<X>), dim3(tl.ntile), dim3(kBlock), 0, s, tl, b, invr, w.partial, w.st);
     }
-    hipLaunchKernelGGL(k_snorm_final, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st);
+    hipLaunchKernelGGL(k_snorm_final, dim3(nc), dim3(kBlock), 0, s, w.partial, tl, w.st);
   } else {
     fill_zero(x, sizeof(X) * ld, s);
   }
   bool first = true, gz = zero_guess;
-  int its = 0, cycles = 0;
+  int its = 0;
   const int check_every = n >= (1 << 20) ? 4 : 8;
-  KResult res;
   for (;;) {
     if (lu) {
-      hipLaunchKernelGGL((k_residual_pc<T, X>), dim3(g1), dim3(kBlock), 0, s, Ares, b, x,
-                         static_cast<const X*>(nullptr), tmp, w.partial, gz ? 1 : 0);
-      ilu_apply(A, *w.ilu, lu, tmp, V, w.st, s);
-      hipLaunchKernelGGL((k_pc_norm<T>), dim3(g1), dim3(kBlock), 0, s, n, V, static_cast<const T*>(nullptr),
-                         w.partial);
+      hipLaunchKernelGGL((k_residual_pc<T, X>), dim3(tl.ntile), dim3(kBlock), 0, s, Ares, tl, b, x,
+                         static_cast<const X*>(nullptr), tmp, w.partial, gz ? 1 : 0, w.st);
+      ilu_apply(A, *w.ilu, lu, tmp, V, ilu_st, s);
+      hipLaunchKernelGGL((k_pc_norm<T>), dim3(tl.ntile), dim3(kBlock), 0, s, tl, V, static_cast<const T*>(nullptr),
+                         w.partial, w.st);
     } else {
-      hipLaunchKernelGGL((k_residual_pc<T, X>), dim3(g1), dim3(kBlock), 0, s, Ares, b, x, invr, V, w.partial,
-                         gz ? 1 : 0);
+      hipLaunchKernelGGL((k_residual_pc<T, X>), dim3(tl.ntile), dim3(kBlock), 0, s, Ares, tl, b, x, invr, V,
+                         w.partial, gz ? 1 : 0, w.st);
     }
-    hipLaunchKernelGGL(k_restart_finalize, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st, first ? 1 : 0,
-                       zero_guess ? 1 : 0);
+    hipLaunchKernelGGL(k_restart_finalize, dim3(nc), dim3(kBlock), 0, s, w.partial, tl, w.st, first ? 1 : 0,
+                       zero_guess ? 1 : 0, mixed ? 1 : 0);
     S355_LAUNCH_CHECK();
     if (mixed && !first) {
       // the FP64 residual decides whether the solve has ended
-      read_header(w, s);
-      if (w.host->reason) break;
-      if (its >= o.maxit) {
-        res.reason = kDivergedIts;
-        break;
-      }
+      read_headers(w, s);
+      if (all_done(w)) break;
     }
     first = false;
     gz = false;
     int it = 0;
     while (it < m && its < o.maxit) {
       if (lu) {
-        hipLaunchKernelGGL((k_spmv_pc<T>), dim3(spmv_grid(n)), dim3(kBlock), 0, s, A, V + it * ld,
+        hipLaunchKernelGGL((k_spmv_pc<T>), dim3(spmv_grid(n)), dim3(kBlock), 0, s, A, tl, V + it * ld,
                            static_cast<const T*>(nullptr), tmp, w.st, it);
-        ilu_apply(A, *w.ilu, lu, tmp, V + (it + 1) * ld, w.st, s);
+        ilu_apply(A, *w.ilu, lu, tmp, V + (it + 1) * ld, ilu_st, s);
       } else {
-        hipLaunchKernelGGL((k_spmv_pc<T>), dim3(spmv_grid(n)), dim3(kBlock), 0, s, A, V + it * ld, inv,
+        hipLaunchKernelGGL((k_spmv_pc<T>), dim3(spmv_grid(n)), dim3(kBlock), 0, s, A, tl, V + it * ld, inv,
                            V + (it + 1) * ld, w.st, it);
       }
-      Dispatch<T>::mdot(R, V, ld, n, it, w.partial, pstride, w.st, s);
-      hipLaunchKernelGGL(k_mdot_final, dim3(it + 1), dim3(kBlock), 0, s, w.partial, gR, pstride, w.st);
-      Dispatch<T>::maxpy(R, V, ld, n, it, w.partial, w.st, s);
-      hipLaunchKernelGGL(k_arnoldi_finalize, dim3(1), dim3(kBlock), 0, s, w.partial, gR, it, m, w.st);
+      Dispatch<T>::mdot(R, V, ld, tl, it, w.partial, pstride, w.st, s);
+      hipLaunchKernelGGL(k_mdot_final, dim3(it + 1, nc), dim3(kBlock), 0, s, w.partial, tl, pstride, w.st);
+      Dispatch<T>::maxpy(R, V, ld, tl, it, w.partial, w.st, s);
+      hipLaunchKernelGGL(k_arnoldi_finalize, dim3(nc), dim3(kBlock), 0, s, w.partial, tl, it, m, w.st);
       S355_LAUNCH_CHECK();
       ++it;
       ++its;
       if (it % check_every == 0 && it < m) {
-        read_header(w, s);
-        if (w.host->reason) break;
+        read_headers(w, s);
+        if (all_stopped(w)) break;
       }
     }
-    hipLaunchKernelGGL(k_build_y, dim3(1), dim3(kWave), 0, s, w.st, m, nupd);
-    Dispatch<T>::update(R, V, ld, n, x, w.st, nupd, s);
+    hipLaunchKernelGGL(k_build_y, dim3(nc), dim3(kWave), 0, s, w.st, m);
+    Dispatch<T>::update(R, V, ld, tl, x, w.st, s);
+    hipLaunchKernelGGL(k_cycle_end, dim3(grid_for(nc)), dim3(kBlock), 0, s, w.st, nc, mixed ? 1 : 0);
     S355_LAUNCH_CHECK();
-    read_header(w, s);
-    its = w.host->its;
-    ++cycles;
-    if constexpr (mixed) {
-      // a cycle that ended on the estimate, or on the cap, goes back to the
-      // FP64 residual; a breakdown or divergence ends the solve here
-      if (w.host->reason < 0 && w.host->reason != kDivergedBreakdown) break;
-      if (w.host->reason != 0) hipLaunchKernelGGL(k_clear_reason, dim3(1), dim3(1), 0, s, w.st);
-      continue;
-    }
-    if (w.host->reason) break;
-    if (its >= o.maxit) {
-      res.reason = kDivergedIts;
-      break;
-    }
+    read_headers(w, s);
+    if (all_done(w)) break;
   }
-  res.iters = w.host->its;
-  if (res.reason == 0) res.reason = w.host->reason;
-  res.rnorm = w.host->rnorm;
-  res.rnorm0 = w.host->rnorm0;
-  res.refinements = mixed ? cycles : 0;
-  return res;
+  return collect(w, mixed);
 }
 
 template <class T>
@@ -871,6 +962,8 @@ KResult cg(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, bool z
   T* V = static_cast<T*>(w.V);
   const size_t ld = static_cast<size_t>(n);
   if (w.mmax < 3) throw std::invalid_argument("cg: workspace needs 4 vectors");
+  if (w.ncase != 1) throw std::invalid_argument("cg: batched systems are solved with GMRES");
+  const Tiles& t1 = w.tiles[0];
   T *r = V, *z = V + ld, *p = V + 2 * ld, *wv = V + 3 * ld;
   T* inv = nullptr;
   T* lu = nullptr;
@@ -884,18 +977,19 @@ KResult cg(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, bool z
   } else if (o.pc != S355_PC_NONE) {
     throw std::invalid_argument("cg: unknown preconditioner");
   }
-  init_state(w, o, s);
+  init_states(w, o, s);
   const int g1 = grid_for(n);
-  if (2 * static_cast<size_t>(g1) > w.partial_len) throw std::runtime_error("cg: partial buffer too small");
+  if (2 * static_cast<size_t>(g1) > w.partial_len || t1.ntile != g1)
+    throw std::runtime_error("cg: partial buffer too small");
   if (!zero_guess) {
     if (lu) {
       ilu_apply(A, *w.ilu, lu, b, z, w.st, s);
-      hipLaunchKernelGGL((k_pc_norm<T>), dim3(g1), dim3(kBlock), 0, s, n, z, static_cast<const T*>(nullptr),
-                         w.partial);
+      hipLaunchKernelGGL((k_pc_norm<T>), dim3(g1), dim3(kBlock), 0, s, t1, z, static_cast<const T*>(nullptr),
+                         w.partial, w.st);
     } else {
-      hipLaunchKernelGGL((k_pc_norm<T>), dim3(g1), dim3(kBlock), 0, s, n, b, inv, w.partial);
+      hipLaunchKernelGGL((k_pc_norm<T>), dim3(g1), dim3(kBlock), 0, s, t1, b, inv, w.partial, w.st);
     }
-    hipLaunchKernelGGL(k_snorm_final, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st);
+    hipLaunchKernelGGL(k_snorm_final, dim3(1), dim3(kBlock), 0, s, w.partial, t1, w.st);
   }
   hipLaunchKernelGGL((k_cg_start<T>), dim3(g1), dim3(kBlock), 0, s, A, b, x, inv, r, z, w.partial, zero_guess ? 1 : 0);
   if (lu) {
@@ -919,11 +1013,11 @@ KResult cg(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, bool z
     hipLaunchKernelGGL(k_cg_final, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st);
     S355_LAUNCH_CHECK();
     if ((i + 1) % check_every == 0) {
-      read_header(w, s);
+      read_headers(w, s);
       if (w.host->reason) break;
     }
   }
-  read_header(w, s);
+  read_headers(w, s);
   res.iters = w.host->its;
   res.reason = w.host->reason;
   if (res.reason == 0) res.reason = kDivergedIts;
@@ -932,10 +1026,10 @@ KResult cg(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, bool z
   return res;
 }
 
-__global__ void k_bench_init(KState* st) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    st->reason = 0;
-    st->scale[0] = 1.0;
+__global__ void k_bench_init(KState* st, int ncase) {
+  for (int c = threadIdx.x; c < ncase; c += blockDim.x) {
+    st[c].reason = 0;
+    st[c].scale[0] = 1.0;
   }
 }
 
@@ -1093,16 +1187,19 @@ KResult ir_solve(const CsrView<double>& A64, const CsrView<float>& A, const doub
 
 template <class T>
 double bench_spmv(const CsrView<T>& A, const T* x, T* y, KWork& w, int repeats, hipStream_t s) {
-  hipLaunchKernelGGL(k_bench_init, dim3(1), dim3(1), 0, s, w.st);
+  hipLaunchKernelGGL(k_bench_init, dim3(1), dim3(kBlock), 0, s, w.st, w.ncase);
   const int g = spmv_grid(A.n);
-  hipLaunchKernelGGL((k_spmv_pc<T>), dim3(g), dim3(kBlock), 0, s, A, x, static_cast<const T*>(nullptr), y, w.st, 0);
+  const Tiles& t = w.tiles[0];
+  hipLaunchKernelGGL((k_spmv_pc<T>), dim3(g), dim3(kBlock), 0, s, A, t, x, static_cast<const T*>(nullptr), y, w.st,
+                     0);
   S355_LAUNCH_CHECK();
   hipEvent_t e0, e1;
   S355_HIP(hipEventCreate(&e0));
   S355_HIP(hipEventCreate(&e1));
   S355_HIP(hipEventRecord(e0, s));
   for (int r = 0; r < repeats; ++r)
-    hipLaunchKernelGGL((k_spmv_pc<T>), dim3(g), dim3(kBlock), 0, s, A, x, static_cast<const T*>(nullptr), y, w.st, 0);
+    hipLaunchKernelGGL((k_spmv_pc<T>), dim3(g), dim3(kBlock), 0, s, A, t, x, static_cast<const T*>(nullptr), y,
+                       w.st, 0);
   S355_HIP(hipEventRecord(e1, s));
   S355_HIP(hipEventSynchronize(e1));
   float ms = 0.f;

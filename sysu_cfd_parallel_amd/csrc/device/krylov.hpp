@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../../include/simple355.h"
+#include "common.hpp"
 
 namespace s355 {
 
@@ -18,11 +19,14 @@ constexpr int kMaxRestart = 64;
 
 // Solver state kept on the device so that kernels can stop themselves once
 // the test has passed; the host reads the header every few iterations.
+// One state per case of a batched solve, stored contiguously.
 struct KState {
   int it;      // completed iterations in the current cycle
   int its;     // completed iterations in total
   int reason;  // 0 running, > 0 converged, < 0 diverged (KSPConvergedReason)
   int maxit;
+  int done;    // the case's solve has ended (reason final)
+  int cycles;  // restart cycles started
   double rnorm, rnorm0, ttol, snorm;
   double rtol, abstol, dtol;
   double pad;
@@ -36,9 +40,10 @@ struct KState {
 };
 
 struct KHeader {
-  int it, its, reason, maxit;
+  int it, its, reason, maxit, done, cycles;
   double rnorm, rnorm0, ttol, snorm;
 };
+static_assert(offsetof(KState, rnorm) == offsetof(KHeader, rnorm), "KHeader mirrors the head of KState");
 
 template <class T>
 struct CsrView {
@@ -65,9 +70,19 @@ struct IluLevels {
 void ilu_levels(int n, const int* rowptr, const int* col, const int* diag, std::vector<int>& lrows,
                 std::vector<int>& lptr, std::vector<int>& urows, std::vector<int>& uptr);
 
-// Workspace owned by the caller.
+struct KResult {
+  int iters = 0, reason = 0;
+  double rnorm = 0, rnorm0 = 0;
+  int refinements = 0;
+};
+
+// Workspace owned by the caller.  ncase independent systems of the same
+// type share one launch per kernel: their rows are consecutive, their states
+// are st[0..ncase), and tiles[k] splits the rows into tiles of kBlock * 2^k
+// rows that never straddle two cases.
 struct KWork {
-  int n = 0, mmax = 0;
+  int n = 0, mmax = 0, ncase = 1;
+  Tiles tiles[5];
   void* V = nullptr;        // (mmax + 1) * n elements of T
   void* r = nullptr;        // n elements of T: preconditioner input (CG: r, z, p, w use V[0..3])
   void* pcinv = nullptr;    // n elements of T (Jacobi inverse diagonal)
@@ -75,14 +90,9 @@ struct KWork {
   const IluLevels* ilu = nullptr;
   double* partial = nullptr;
   size_t partial_len = 0;
-  KState* st = nullptr;     // device
-  KHeader* host = nullptr;  // pinned
-};
-
-struct KResult {
-  int iters = 0, reason = 0;
-  double rnorm = 0, rnorm0 = 0;
-  int refinements = 0;
+  KState* st = nullptr;     // device, ncase states
+  KHeader* host = nullptr;  // pinned, ncase headers
+  std::vector<KResult>* per_case = nullptr;  // filled by a solve when set
 };
 
 // Buffers of the iterative-refinement solve (mixed precision).

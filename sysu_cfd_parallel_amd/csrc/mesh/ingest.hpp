@@ -76,4 +76,19 @@ struct Mesh {
 // Builds the mesh; throws std::runtime_error with a message on bad input.
 Mesh ingest_mesh(const s355_mesh_desc& desc);
 
+// Offsets of the cases of a batch mesh (ncase + 1 entries each): case k owns
+// cells [cell[k], cell[k+1]), boundary records [bnd[k], bnd[k+1]), interior
+// faces [n_bnd + iface[k], n_bnd + iface[k+1]), CSR entries, vertices,
+// elements and cell-face list entries likewise; its region ids are shifted
+// by k * rid_stride.
+struct BatchLayout {
+  std::vector<int> cell, bnd, iface, nnz, vert, elem, cf;
+  int rid_stride = 1;
+};
+
+// Concatenates independent cases into one mesh: cells, rows and vertices
+// case after case; the boundary faces of all cases first, then their
+// interior faces, each case keeping its own face order.
+Mesh concat_meshes(const std::vector<Mesh>& parts, BatchLayout& lay);
+
 }  // namespace s355
