@@ -93,6 +93,12 @@ typedef struct s355_params {
   int precision;        /* S355_FP64 or S355_MIXED */
   int ir_max;           /* mixed: cap on FP64 refinement steps per solve */
   double ir_inner_rtol; /* mixed: relative tolerance of each FP32 inner solve */
+  int autotune;         /* 1: on the first GMRES solve of each precision, time
+                           the SpMV lanes per row and the rows per thread of the
+                           Gram-Schmidt kernels and keep the fastest (cached per
+                           process and system shape); 0: fixed heuristics */
+  int graphs;           /* 1: capture each GMRES restart cycle as a hipGraph
+                           and replay it; 0: launch kernel by kernel */
 } s355_params;
 
 typedef struct s355_solve_info {
@@ -295,6 +301,12 @@ int s355_spmv_time(s355_ctx* ctx, int which, int precision, int repeats, double*
  * of 16) over `repeats` launches: the bandwidth reference for the roofline
  * and for scaling hardware byte counters.  No reference counterpart. */
 int s355_copy_time(s355_ctx* ctx, size_t bytes, int repeats, double* ms_per_call);
+
+/* Kernel shapes in use for the GMRES solves of each precision (index 0 FP64,
+ * 1 FP32): lanes[k] SpMV lanes per row, rows[k] rows per thread of the
+ * multi-dot / Gram-Schmidt / update kernels, tuned[k] 1 when they came from
+ * the autotuner.  No reference counterpart. */
+int s355_kernel_shapes(s355_ctx* ctx, int lanes[2], int rows[2], int tuned[2]);
 
 /* Host-side set-up only (no device): the integer and metric output of the
  * face builder, for inspection and for tests on machines without a GPU. */

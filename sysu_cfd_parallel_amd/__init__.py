@@ -94,6 +94,7 @@ def library(path: Optional[Path] = None) -> C.CDLL:
         "spmv_time": (I, [V, I, I, I, pD]),
         "last_solve_info": (I, [V, C.POINTER(SolveInfo)]),
         "copy_time": (I, [V, C.c_size_t, I, pD]),
+        "kernel_shapes": (I, [V, pI, pI, pI]),
     }
     for name, (res, args) in more.items():
         fn = getattr(lib, f"s355_{name}")
@@ -361,6 +362,12 @@ class Solver:
         ms = C.c_double()
         _check(self.lib, self.lib.s355_spmv_time(self.ctx, which, precision, repeats, C.byref(ms)), "spmv_time")
         return ms.value
+
+    def kernel_shapes(self) -> dict:
+        """SpMV lanes per row and Gram-Schmidt rows per thread, FP64 and FP32."""
+        lanes, rows, tuned = (C.c_int * 2)(), (C.c_int * 2)(), (C.c_int * 2)()
+        _check(self.lib, self.lib.s355_kernel_shapes(self.ctx, lanes, rows, tuned), "kernel_shapes")
+        return {p: {"lanes": lanes[k], "rows": rows[k], "tuned": bool(tuned[k])} for k, p in enumerate(("fp64", "fp32"))}
 
     def copy_time(self, nbytes: int, repeats: int) -> float:
         """Mean device ms of a plain copy of nbytes (multiple of 16)."""

@@ -70,6 +70,8 @@ class Params(C.Structure):
         ("precision", C.c_int),
         ("ir_max", C.c_int),
         ("ir_inner_rtol", C.c_double),
+        ("autotune", C.c_int),
+        ("graphs", C.c_int),
     ]
 
 

@@ -166,6 +166,8 @@ void s355_default_params(s355_params* p) {
   p->precision = S355_FP64;
   p->ir_max = 30;
   p->ir_inner_rtol = 1.e-3;
+  p->autotune = 0;
+  p->graphs = 0;
 }
 
 int s355_case_build(int kind, const int* ip, const double* dp, s355_case** out) {
@@ -600,6 +602,14 @@ int s355_spmv_time(s355_ctx* ctx, int which, int precision, int repeats, double*
     if (!ms_per_call || which < 0 || which > 2 || precision < 0 || precision > 1)
       throw std::invalid_argument("spmv_time arguments");
     *ms_per_call = dev(ctx).time_spmv(which, precision, repeats);
+    return S355_OK;
+  });
+}
+
+int s355_kernel_shapes(s355_ctx* ctx, int lanes[2], int rows[2], int tuned[2]) {
+  return guarded([&] {
+    if (!lanes || !rows || !tuned) throw std::invalid_argument("null output");
+    dev(ctx).kernel_shapes(lanes, rows, tuned);
     return S355_OK;
   });
 }

@@ -112,6 +112,7 @@ Tiles Context::make_tiles(const std::vector<int>& case_row, int tile) {
 
 void Context::free_all() {
   if (stream) hipStreamSynchronize(stream);
+  graphs_.reset();
   for (hipEvent_t e : all_events_) hipEventDestroy(e);
   all_events_.clear();
   pool_.clear();
@@ -451,6 +452,9 @@ int Context::run_solve(int which, const double* b, double* x, const s355_ksp_opt
                        s355_solve_info* info, int slot) {
   const double* val = which == 0 ? df.Au : (which == 1 ? df.Ap : df.As);
   KResult r;
+  if (par.graphs && !graphs_) graphs_ = std::make_unique<GraphCache>();
+  kwork_.graphs = par.graphs ? graphs_.get() : nullptr;
+  kwork32_.graphs = kwork_.graphs;
   kwork_.per_case = &case_info[slot];
   kwork32_.per_case = &case_info[slot];
   case_info[slot].assign(ncase, KResult{});
@@ -462,9 +466,13 @@ int Context::run_solve(int which, const double* b, double* x, const s355_ksp_opt
     convert_to_float(val, a32_, dm.nnz, stream);
     CsrView<float> A{dm.Nc, dm.rowptr, dm.col, dm.diag, a32_, dm.nnz};
     CsrView<double> A64{dm.Nc, dm.rowptr, dm.col, dm.diag, val, dm.nnz};
+    if (par.autotune && o.type == S355_KSP_GMRES && !kwork32_.tuned)
+      autotune<float>(A, kwork32_, std::min(o.restart, kMaxRestart), stream);
     r = ir_solve(A64, A, b, x, o, zero_guess, par.ir_max, par.ir_inner_rtol, kwork32_, ir_, stream);
   } else {
     CsrView<double> A{dm.Nc, dm.rowptr, dm.col, dm.diag, val, dm.nnz};
+    if (par.autotune && o.type == S355_KSP_GMRES && !kwork_.tuned)
+      autotune<double>(A, kwork_, std::min(o.restart, kMaxRestart), stream);
     r = krylov_solve<double>(A, b, x, o, zero_guess, kwork_, stream);
   }
   if (ncase == 1) case_info[slot].assign(1, r);
@@ -624,6 +632,16 @@ double Context::time_spmv(int which, int precision, int repeats) {
   convert_to_float(df.U, xf, dm.Nc, stream);
   CsrView<float> A{dm.Nc, dm.rowptr, dm.col, dm.diag, vf, dm.nnz};
   return bench_spmv<float>(A, xf, yf, kwork_, repeats, stream);
+}
+
+void Context::kernel_shapes(int lanes[2], int rows[2], int tuned[2]) {
+  const KWork* w[2] = {&kwork_, &kwork32_};
+  const size_t elem[2] = {sizeof(double), sizeof(float)};
+  for (int k = 0; k < 2; ++k) {
+    lanes[k] = w[k]->lanes;
+    rows[k] = w[k]->rows ? w[k]->rows : pick_rows(dm.Nc, elem[k]);
+    tuned[k] = w[k]->tuned ? 1 : 0;
+  }
 }
 
 double Context::time_copy(size_t bytes, int repeats) {

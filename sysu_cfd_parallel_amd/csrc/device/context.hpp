@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -78,6 +79,7 @@ class Context {
   // benchmark helper
   double time_spmv(int which, int precision, int repeats);
   double time_copy(size_t bytes, int repeats);
+  void kernel_shapes(int lanes[2], int rows[2], int tuned[2]);
 
  private:
   template <class T>
@@ -107,6 +109,7 @@ class Context {
   KWork kwork32_{};
   IrWork ir_{};
   // ILU(0): level schedule of the pattern and factor storage
+  std::unique_ptr<GraphCache> graphs_;  // captured GMRES cycles (params.graphs)
   IluLevels levels_{};
   bool have_levels_ = false;
   void* host_pinned_ = nullptr;

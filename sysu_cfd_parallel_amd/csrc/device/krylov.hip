@@ -11,6 +11,10 @@
 // only every few iterations.
 #include <algorithm>
 #include <cmath>
+#include <functional>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <type_traits>
 
 #include "common.hpp"
@@ -146,33 +150,31 @@ void ilu_apply(const CsrView<T>& A, const IluLevels& lv, const T* lu, const T* r
 }
 
 // ---- y = scale * M^-1 A x
-// kRowLanes consecutive lanes share a row: their loads of values and column
-// ids are contiguous across the wavefront; a shuffle tree in a fixed order
-// sums the lanes.
-constexpr int kRowLanes = 8;
-
-template <class T>
+// L consecutive lanes share a row (L = 4, 8 or 16, picked per matrix): their
+// loads of values and column ids are contiguous across the wavefront; a
+// shuffle tree in a fixed order sums the lanes.
+template <class T, int L>
 __device__ __forceinline__ T row_dot(const CsrView<T>& A, const T* __restrict__ x, int row, int lane) {
   T s = T(0);
   if (row < A.n) {
     const int p1 = A.rowptr[row + 1];
-    for (int p = A.rowptr[row] + lane; p < p1; p += kRowLanes) s += A.val[p] * x[A.col[p]];
+    for (int p = A.rowptr[row] + lane; p < p1; p += L) s += A.val[p] * x[A.col[p]];
   }
 #pragma unroll
-  for (int off = kRowLanes / 2; off > 0; off >>= 1) s += __shfl_down(s, off, kRowLanes);
+  for (int off = L / 2; off > 0; off >>= 1) s += __shfl_down(s, off, L);
   return s;
 }
 
-template <class T>
+template <class T, int L>
 __global__ void __launch_bounds__(kBlock) k_spmv_pc(CsrView<T> A, Tiles t, const T* __restrict__ x,
                                                     const T* __restrict__ inv, T* __restrict__ y, const KState* st,
                                                     int it) {
   const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  const int row = g / kRowLanes, lane = g % kRowLanes;
+  const int row = g / L, lane = g % L;
   // the lanes of one row share its case, so they leave together
   const KState* sc = st + (row < A.n ? t.case_of(row) : 0);
   if (sc->reason) return;
-  const T s = row_dot(A, x, row, lane);
+  const T s = row_dot<T, L>(A, x, row, lane);
   if (lane == 0 && row < A.n) {
     double z = static_cast<double>(s);
     if (inv) z *= static_cast<double>(inv[row]);
@@ -180,7 +182,16 @@ __global__ void __launch_bounds__(kBlock) k_spmv_pc(CsrView<T> A, Tiles t, const
   }
 }
 
-inline int spmv_grid(int n) { return grid_for(static_cast<long>(n) * kRowLanes); }
+template <class T>
+void launch_spmv(int L, const CsrView<T>& A, const Tiles& t, const T* x, const T* inv, T* y, const KState* st, int it,
+                 hipStream_t s) {
+  const int g = grid_for(static_cast<long>(A.n) * L);
+  switch (L) {
+    case 4: hipLaunchKernelGGL((k_spmv_pc<T, 4>), dim3(g), dim3(kBlock), 0, s, A, t, x, inv, y, st, it); break;
+    case 16: hipLaunchKernelGGL((k_spmv_pc<T, 16>), dim3(g), dim3(kBlock), 0, s, A, t, x, inv, y, st, it); break;
+    default: hipLaunchKernelGGL((k_spmv_pc<T, 8>), dim3(g), dim3(kBlock), 0, s, A, t, x, inv, y, st, it); break;
+  }
+}
 
 // ---- partial[b][k] = sum over tile b of w_i V_k,i, k = 0..it
 template <class T, int R>
@@ -335,6 +346,7 @@ __global__ void __launch_bounds__(kBlock) k_arnoldi_finalize(const double* parti
       reason = conv_check(s_, res);
       if (hapend && !reason) reason = kDivergedBreakdown;
     }
+    if (!reason && s_->its >= s_->maxit) reason = kDivergedIts;
     s_->reason = reason;
   }
   __syncthreads();
@@ -509,8 +521,9 @@ __global__ void __launch_bounds__(kBlock) k_restart_finalize(const double* parti
 
 // ---- end of a GMRES cycle, after the update of x (one thread per case).
 // Plain solve: a set reason ends the case, then the iteration cap.  Mixed
-// solve: a passed estimate or a breakdown of the FP32 cycle is cleared and
-// goes back to the FP64 residual; a divergence ends the case.
+// solve: a passed estimate, a breakdown of the FP32 cycle or the cap met
+// inside the cycle is cleared and goes back to the FP64 residual, which
+// applies the cap after its own test; a divergence ends the case.
 __global__ void k_cycle_end(KState* st, int ncase, int mixed) {
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncase; c += gridDim.x * blockDim.x) {
     KState* s = st + c;
@@ -524,7 +537,7 @@ __global__ void k_cycle_end(KState* st, int ncase, int mixed) {
         s->done = 1;
       }
     } else {
-      if (r < 0 && r != kDivergedBreakdown)
+      if (r < 0 && r != kDivergedBreakdown && r != kDivergedIts)
         s->done = 1;
       else
         s->reason = 0;
@@ -850,7 +863,8 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
   if (m > w.mmax) throw std::invalid_argument("gmres: restart above workspace size");
   if (Ares.n != n || w.n != n) throw std::invalid_argument("gmres: operator sizes differ");
   const size_t ld = static_cast<size_t>(n);
-  const int R = pick_r(n, sizeof(T));
+  const int R = w.rows ? w.rows : pick_r(n, sizeof(T));
+  const int L = w.lanes;
   const Tiles& tl = w.tiles[log2_r(R)];
   if (tl.tile != kBlock * R || tl.n != n) throw std::runtime_error("gmres: tile table missing");
   const int nc = w.ncase;
@@ -900,23 +914,70 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
   } else {
     fill_zero(x, sizeof(X) * ld, s);
   }
-  bool first = true, gz = zero_guess;
-  int its = 0;
-  const int check_every = n >= (1 << 20) ? 4 : 8;
-  for (;;) {
+  auto enqueue_restart = [&](bool first_c, bool gz_c) {
     if (lu) {
       hipLaunchKernelGGL((k_residual_pc<T, X>), dim3(tl.ntile), dim3(kBlock), 0, s, Ares, tl, b, x,
-                         static_cast<const X*>(nullptr), tmp, w.partial, gz ? 1 : 0, w.st);
+                         static_cast<const X*>(nullptr), tmp, w.partial, gz_c ? 1 : 0, w.st);
       ilu_apply(A, *w.ilu, lu, tmp, V, ilu_st, s);
       hipLaunchKernelGGL((k_pc_norm<T>), dim3(tl.ntile), dim3(kBlock), 0, s, tl, V, static_cast<const T*>(nullptr),
                          w.partial, w.st);
     } else {
       hipLaunchKernelGGL((k_residual_pc<T, X>), dim3(tl.ntile), dim3(kBlock), 0, s, Ares, tl, b, x, invr, V,
-                         w.partial, gz ? 1 : 0, w.st);
+                         w.partial, gz_c ? 1 : 0, w.st);
     }
-    hipLaunchKernelGGL(k_restart_finalize, dim3(nc), dim3(kBlock), 0, s, w.partial, tl, w.st, first ? 1 : 0,
+    hipLaunchKernelGGL(k_restart_finalize, dim3(nc), dim3(kBlock), 0, s, w.partial, tl, w.st, first_c ? 1 : 0,
                        zero_guess ? 1 : 0, mixed ? 1 : 0);
     S355_LAUNCH_CHECK();
+  };
+  auto enqueue_iteration = [&](int it) {
+    if (lu) {
+      launch_spmv<T>(L, A, tl, V + it * ld, static_cast<const T*>(nullptr), tmp, w.st, it, s);
+      ilu_apply(A, *w.ilu, lu, tmp, V + (it + 1) * ld, ilu_st, s);
+    } else {
+      launch_spmv<T>(L, A, tl, V + it * ld, inv, V + (it + 1) * ld, w.st, it, s);
+    }
+    Dispatch<T>::mdot(R, V, ld, tl, it, w.partial, pstride, w.st, s);
+    hipLaunchKernelGGL(k_mdot_final, dim3(it + 1, nc), dim3(kBlock), 0, s, w.partial, tl, pstride, w.st);
+    Dispatch<T>::maxpy(R, V, ld, tl, it, w.partial, w.st, s);
+    hipLaunchKernelGGL(k_arnoldi_finalize, dim3(nc), dim3(kBlock), 0, s, w.partial, tl, it, m, w.st);
+    S355_LAUNCH_CHECK();
+  };
+  auto enqueue_end = [&] {
+    hipLaunchKernelGGL(k_build_y, dim3(nc), dim3(kWave), 0, s, w.st, m);
+    Dispatch<T>::update(R, V, ld, tl, x, w.st, s);
+    hipLaunchKernelGGL(k_cycle_end, dim3(grid_for(nc)), dim3(kBlock), 0, s, w.st, nc, mixed ? 1 : 0);
+    S355_LAUNCH_CHECK();
+  };
+
+  if (w.graphs && !lu) {
+    // every restart cycle is one captured graph (the first cycle has its own:
+    // it sets the reference norm and may skip the product with a zero
+    // guess); a cycle always runs m iterations, the kernels of a finished
+    // case returning at once, and the iteration cap is applied on the device
+    for (bool first_c = true;; first_c = false) {
+      const std::vector<uintptr_t> key = {reinterpret_cast<uintptr_t>(A.val), reinterpret_cast<uintptr_t>(Ares.val),
+                                          reinterpret_cast<uintptr_t>(b), reinterpret_cast<uintptr_t>(x),
+                                          reinterpret_cast<uintptr_t>(inv), reinterpret_cast<uintptr_t>(invr),
+                                          static_cast<uintptr_t>(m), static_cast<uintptr_t>(R),
+                                          static_cast<uintptr_t>(L), sizeof(T), sizeof(X),
+                                          static_cast<uintptr_t>(first_c), static_cast<uintptr_t>(zero_guess)};
+      hipGraphExec_t g = w.graphs->get(key, s, [&] {
+        enqueue_restart(first_c, first_c && zero_guess);
+        for (int it = 0; it < m; ++it) enqueue_iteration(it);
+        enqueue_end();
+      });
+      S355_HIP(hipGraphLaunch(g, s));
+      read_headers(w, s);
+      if (all_done(w)) break;
+    }
+    return collect(w, mixed);
+  }
+
+  bool first = true, gz = zero_guess;
+  int its = 0;
+  const int check_every = n >= (1 << 20) ? 4 : 8;
+  for (;;) {
+    enqueue_restart(first, gz);
     if (mixed && !first) {
       // the FP64 residual decides whether the solve has ended
       read_headers(w, s);
@@ -926,19 +987,7 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
     gz = false;
     int it = 0;
     while (it < m && its < o.maxit) {
-      if (lu) {
-        hipLaunchKernelGGL((k_spmv_pc<T>), dim3(spmv_grid(n)), dim3(kBlock), 0, s, A, tl, V + it * ld,
-                           static_cast<const T*>(nullptr), tmp, w.st, it);
-        ilu_apply(A, *w.ilu, lu, tmp, V + (it + 1) * ld, ilu_st, s);
-      } else {
-        hipLaunchKernelGGL((k_spmv_pc<T>), dim3(spmv_grid(n)), dim3(kBlock), 0, s, A, tl, V + it * ld, inv,
-                           V + (it + 1) * ld, w.st, it);
-      }
-      Dispatch<T>::mdot(R, V, ld, tl, it, w.partial, pstride, w.st, s);
-      hipLaunchKernelGGL(k_mdot_final, dim3(it + 1, nc), dim3(kBlock), 0, s, w.partial, tl, pstride, w.st);
-      Dispatch<T>::maxpy(R, V, ld, tl, it, w.partial, w.st, s);
-      hipLaunchKernelGGL(k_arnoldi_finalize, dim3(nc), dim3(kBlock), 0, s, w.partial, tl, it, m, w.st);
-      S355_LAUNCH_CHECK();
+      enqueue_iteration(it);
       ++it;
       ++its;
       if (it % check_every == 0 && it < m) {
@@ -946,10 +995,7 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
         if (all_stopped(w)) break;
       }
     }
-    hipLaunchKernelGGL(k_build_y, dim3(nc), dim3(kWave), 0, s, w.st, m);
-    Dispatch<T>::update(R, V, ld, tl, x, w.st, s);
-    hipLaunchKernelGGL(k_cycle_end, dim3(grid_for(nc)), dim3(kBlock), 0, s, w.st, nc, mixed ? 1 : 0);
-    S355_LAUNCH_CHECK();
+    enqueue_end();
     read_headers(w, s);
     if (all_done(w)) break;
   }
@@ -1092,6 +1138,125 @@ __global__ void __launch_bounds__(kBlock) k_add_correction(int n, double* x, con
 
 }  // namespace
 
+GraphCache::~GraphCache() {
+  for (auto& kv : execs) hipGraphExecDestroy(kv.second);
+}
+
+hipGraphExec_t GraphCache::get(const std::vector<uintptr_t>& key, hipStream_t s, const std::function<void()>& record) {
+  auto it = execs.find(key);
+  if (it != execs.end()) return it->second;
+  hipGraph_t graph = nullptr;
+  S355_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    record();
+  } catch (...) {
+    hipStreamEndCapture(s, &graph);
+    if (graph) hipGraphDestroy(graph);
+    throw;
+  }
+  S355_HIP(hipStreamEndCapture(s, &graph));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  S355_HIP(e);
+  execs.emplace(key, exec);
+  return exec;
+}
+
+namespace {
+struct TuneKey {
+  int n, nnz, ncase, elem, m;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(n, nnz, ncase, elem, m) < std::tie(o.n, o.nnz, o.ncase, o.elem, o.m);
+  }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, std::pair<int, int>> g_tuned;  // (lanes, rows)
+
+template <class F>
+double time_ms(F&& launch, int reps, hipStream_t s) {
+  launch();
+  hipEvent_t e0, e1;
+  S355_HIP(hipEventCreate(&e0));
+  S355_HIP(hipEventCreate(&e1));
+  S355_HIP(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r) launch();
+  S355_HIP(hipEventRecord(e1, s));
+  S355_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  S355_HIP(hipEventElapsedTime(&ms, e0, e1));
+  S355_HIP(hipEventDestroy(e0));
+  S355_HIP(hipEventDestroy(e1));
+  return static_cast<double>(ms) / reps;
+}
+}  // namespace
+
+template <class T>
+void autotune(const CsrView<T>& A, KWork& w, int m, hipStream_t s) {
+  const TuneKey key{A.n, A.nnz, w.ncase, static_cast<int>(sizeof(T)), m};
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) {
+      w.lanes = it->second.first;
+      w.rows = it->second.second;
+      w.tuned = true;
+      return;
+    }
+  }
+  const int n = A.n;
+  if (n <= 0 || m < 1 || m > w.mmax) return;
+  const size_t ld = static_cast<size_t>(n);
+  T* V = static_cast<T*>(w.V);
+  // a zero basis: the timings do not depend on the values
+  fill_zero(V, sizeof(T) * ld * (static_cast<size_t>(m) + 1), s);
+  s355_ksp_opts o{};
+  o.maxit = 1 << 30;
+  o.rtol = 0.0;
+  o.dtol = 1e300;
+  init_states(w, o, s);
+  hipLaunchKernelGGL(k_bench_init, dim3(1), dim3(kBlock), 0, s, w.st, w.ncase);
+  S355_LAUNCH_CHECK();
+  const T* none = nullptr;
+  int best_l = 8;
+  double best = 1e300;
+  for (int L : {4, 8, 16}) {
+    const double ms = time_ms([&] { launch_spmv<T>(L, A, w.tiles[0], V, none, V + ld, w.st, 0, s); }, 5, s);
+    if (ms < best) {
+      best = ms;
+      best_l = L;
+    }
+  }
+  // one mid-cycle Arnoldi step of the Gram-Schmidt pair at every tile size
+  const int it = m / 2;
+  int best_r = pick_r(n, sizeof(T));
+  best = 1e300;
+  for (int R : {1, 2, 4, 8, 16}) {
+    const Tiles& t = w.tiles[log2_r(R)];
+    if (static_cast<size_t>(t.ntile) * (m + 1) > w.partial_len) continue;
+    const double ms = time_ms(
+        [&] {
+          Dispatch<T>::mdot(R, V, ld, t, it, w.partial, m + 1, w.st, s);
+          Dispatch<T>::maxpy(R, V, ld, t, it, w.partial, w.st, s);
+        },
+        3, s);
+    if (ms < best) {
+      best = ms;
+      best_r = R;
+    }
+  }
+  w.lanes = best_l;
+  w.rows = best_r;
+  w.tuned = true;
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  g_tuned[key] = {best_l, best_r};
+}
+
+int pick_rows(int n, size_t elem) { return pick_r(n, elem); }
+
+template void autotune<double>(const CsrView<double>&, KWork&, int, hipStream_t);
+template void autotune<float>(const CsrView<float>&, KWork&, int, hipStream_t);
+
 template <class T>
 KResult krylov_solve(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, bool zero_guess, KWork& w,
                      hipStream_t s) {
@@ -1188,18 +1353,16 @@ KResult ir_solve(const CsrView<double>& A64, const CsrView<float>& A, const doub
 template <class T>
 double bench_spmv(const CsrView<T>& A, const T* x, T* y, KWork& w, int repeats, hipStream_t s) {
   hipLaunchKernelGGL(k_bench_init, dim3(1), dim3(kBlock), 0, s, w.st, w.ncase);
-  const int g = spmv_grid(A.n);
   const Tiles& t = w.tiles[0];
-  hipLaunchKernelGGL((k_spmv_pc<T>), dim3(g), dim3(kBlock), 0, s, A, t, x, static_cast<const T*>(nullptr), y, w.st,
-                     0);
+  const T* none = nullptr;
+  launch_spmv<T>(w.lanes, A, t, x, none, y, w.st, 0, s);
   S355_LAUNCH_CHECK();
   hipEvent_t e0, e1;
   S355_HIP(hipEventCreate(&e0));
   S355_HIP(hipEventCreate(&e1));
   S355_HIP(hipEventRecord(e0, s));
   for (int r = 0; r < repeats; ++r)
-    hipLaunchKernelGGL((k_spmv_pc<T>), dim3(g), dim3(kBlock), 0, s, A, t, x, static_cast<const T*>(nullptr), y,
-                       w.st, 0);
+    launch_spmv<T>(w.lanes, A, t, x, none, y, w.st, 0, s);
   S355_HIP(hipEventRecord(e1, s));
   S355_HIP(hipEventSynchronize(e1));
   float ms = 0.f;

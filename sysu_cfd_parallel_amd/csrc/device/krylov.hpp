@@ -8,6 +8,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <cstdint>
+#include <functional>
+#include <map>
 #include <vector>
 
 #include "../../../include/simple355.h"
@@ -76,6 +79,14 @@ struct KResult {
   int refinements = 0;
 };
 
+// Instantiated hipGraphs of GMRES restart cycles, keyed by everything their
+// kernel arguments depend on; destroyed with the cache.
+struct GraphCache {
+  std::map<std::vector<uintptr_t>, hipGraphExec_t> execs;
+  ~GraphCache();
+  hipGraphExec_t get(const std::vector<uintptr_t>& key, hipStream_t s, const std::function<void()>& record);
+};
+
 // Workspace owned by the caller.  ncase independent systems of the same
 // type share one launch per kernel: their rows are consecutive, their states
 // are st[0..ncase), and tiles[k] splits the rows into tiles of kBlock * 2^k
@@ -90,6 +101,10 @@ struct KWork {
   const IluLevels* ilu = nullptr;
   double* partial = nullptr;
   size_t partial_len = 0;
+  int lanes = 8;            // SpMV lanes per row (4, 8, 16)
+  int rows = 0;             // rows per thread of the Gram-Schmidt kernels; 0: by size
+  bool tuned = false;       // lanes / rows chosen by autotune()
+  GraphCache* graphs = nullptr;  // set: GMRES cycles (Jacobi / no PC) replay captured graphs
   KState* st = nullptr;     // device, ncase states
   KHeader* host = nullptr;  // pinned, ncase headers
   std::vector<KResult>* per_case = nullptr;  // filled by a solve when set
@@ -120,6 +135,17 @@ struct IrWork {
 KResult ir_solve(const CsrView<double>& A64, const CsrView<float>& A32, const double* b, double* x,
                  const s355_ksp_opts& o, bool zero_guess, int ir_max, double inner_rtol, KWork& w32, IrWork& ir,
                  hipStream_t s);
+
+// Default rows per thread of the Gram-Schmidt kernels for n rows of elem bytes.
+int pick_rows(int n, size_t elem);
+
+// Times the SpMV at 4, 8 and 16 lanes per row and one mid-cycle Gram-Schmidt
+// step (multi-dot + update) at 1..16 rows per thread on A and the workspace,
+// and keeps the fastest in w.lanes / w.rows.  The choice is cached per
+// process and system shape, so every context of one process that solves
+// systems of that shape uses the same kernels (and reproduces its results).
+template <class T>
+void autotune(const CsrView<T>& A, KWork& w, int m, hipStream_t s);
 
 // Solve A x = b.  x holds the initial guess unless zero_guess.
 template <class T>

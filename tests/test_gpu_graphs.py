@@ -1,0 +1,96 @@
+"""Captured hipGraphs of GMRES restart cycles and the kernel-shape
+autotuner.  A graph replays exactly the kernels of the launch-by-launch path,
+so results agree bit for bit on every replay, with the program's own eager
+work (assembly, corrections, other solves) between the replays.  Tuned shapes
+change only the order of the reductions: results stay within rounding, and
+two contexts of one process share the tuned shapes and agree bit for bit."""
+import numpy as np
+import pytest
+
+import sysu_cfd_parallel_amd as s
+from helpers import params, rel_err
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("U", "V", "W", "P", "RUFACE")
+
+
+def run(case, p, steps, batch=False):
+    d = s.BatchSolver(case, p) if batch else s.Solver(case, p)
+    h, done = d.iterate(steps)
+    out = {n: d.get(n) for n in FIELDS}
+    info = d.last_solve_info()
+    shapes = d.kernel_shapes()
+    d.close()
+    return out, h, info, shapes
+
+
+def base(prec=s.FP64, restart=20):
+    p = params(restart=restart, rtol=1e-8, maxit=3000)
+    p.ksp[s.KSP_PRESSURE].rtol = 1e-6
+    p.residual_steady = 0.0
+    p.precision = prec
+    return p
+
+
+@pytest.mark.parametrize("prec", [s.FP64, s.MIXED])
+@pytest.mark.parametrize("name,make", [
+    ("cavity", lambda: s.cavity(8, re=400.0, jitter=0.1, seed=2)),
+    ("channel", lambda: s.channel(12, 6, 6, jitter=0.1, split=0.2, seed=1, re=500.0)),
+])
+def test_graph_replays_match_eager_launches(prec, name, make):
+    case = make()
+    p0 = base(prec)
+    p1 = base(prec)
+    p1.graphs = 1
+    a, ha, ia, _ = run(case, p0, 8)
+    b, hb, ib, _ = run(case, p1, 8)
+    for n in FIELDS:
+        assert np.array_equal(a[n], b[n]), n
+    assert np.array_equal(ha, hb)
+    for k in ("U", "P"):
+        assert ia[k]["iters"] == ib[k]["iters"] and ia[k]["reason"] == ib[k]["reason"]
+
+
+def test_graph_replays_in_a_batch():
+    cases = [s.cavity(6, re=100.0), s.cavity(9, re=700.0, jitter=0.1, seed=5), s.cavity(5, re=300.0, tets=True)]
+    p0 = base(s.MIXED)
+    p1 = base(s.MIXED)
+    p1.graphs = 1
+    a, ha, _, _ = run(cases, p0, 5, batch=True)
+    b, hb, _, _ = run(cases, p1, 5, batch=True)
+    for n in FIELDS:
+        assert np.array_equal(a[n], b[n]), n
+    assert np.array_equal(ha, hb)
+
+
+@pytest.mark.parametrize("graphs", [0, 1])
+def test_iteration_cap_inside_a_cycle(graphs):
+    case = s.cavity(10, re=400.0, jitter=0.1, seed=4)
+    p = base(s.FP64, restart=5)
+    p.graphs = graphs
+    d = s.Solver(case, p)
+    d.call("build_velocity_matrix")
+    d.call("cal_ru_face")
+    d.call("build_pressure_matrix")
+    info = d.solve_pressure(1e-14, 7)
+    assert info["reason"] == -3 and info["iters"] == 7
+
+
+def test_autotuned_shapes_are_shared_within_a_process():
+    case = s.channel(16, 8, 8, jitter=0.1, split=0.1, seed=6, re=800.0)
+    p = base(s.MIXED)
+    p.ksp[s.KSP_PRESSURE].rtol = 1e-10
+    p.autotune = 1
+    a, ha, _, sa = run(case, p, 4)
+    b, hb, _, sb = run(case, p, 4)
+    assert sa["fp32"]["tuned"] and sa == sb
+    assert sa["fp32"]["lanes"] in (4, 8, 16) and sa["fp32"]["rows"] in (1, 2, 4, 8, 16)
+    for n in FIELDS:
+        assert np.array_equal(a[n], b[n]), n
+    q = base(s.MIXED)
+    q.ksp[s.KSP_PRESSURE].rtol = 1e-10
+    c, _, _, sc = run(case, q, 4)
+    assert not sc["fp32"]["tuned"]
+    for n in ("U", "V", "W", "P"):
+        assert rel_err(a[n], c[n]) < 1e-7, n
