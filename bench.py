@@ -75,6 +75,7 @@ class Channel:
     precisions = ["fp64", "mixed"]
     steps, warmup = 5, 1
     turbulent = False
+    autotune, graphs = 1, 0
 
     def __init__(self, size):
         self.dims = self.sizes[size]
@@ -106,6 +107,7 @@ class Cavity(Channel):
     sizes = {"full": 32, "medium": 32, "small": 16}
     precisions = ["fp64"]
     steps, warmup = 200, 0
+    graphs = 1
 
     def case(self, dims=None):
         import sysu_cfd_parallel_amd as s
@@ -197,7 +199,7 @@ class Sweep(Channel):
     sizes = {"full": 64, "medium": 16, "small": 4}
     precisions = ["mixed"]
     steps, warmup = 5, 1
-    batch = True
+    graphs = 1
 
     def __init__(self, size):
         import numpy as np
@@ -281,7 +283,9 @@ def run_path(wl, case, prec: int, steps: int, warmup: int, label: str) -> dict:
     import sysu_cfd_parallel_amd as s
 
     t = time.perf_counter()
-    dev = s.BatchSolver(case, wl.params(prec)) if isinstance(case, list) else s.Solver(case, wl.params(prec))
+    p = wl.params(prec)
+    p.autotune, p.graphs = wl.autotune, wl.graphs
+    dev = s.BatchSolver(case, p) if isinstance(case, list) else s.Solver(case, p)
     z = dev.sizes
     log(f"[{label}] device context: {z.n_cell} cells, {z.n_face} faces, nnz {z.nnz} "
         f"({time.perf_counter() - t:.1f}s)")
@@ -310,8 +314,12 @@ def run_path(wl, case, prec: int, steps: int, warmup: int, label: str) -> dict:
     dev.call("synchronize")
     t1 = time.perf_counter()
     stages = dev.stage_times()
+    shapes = dev.kernel_shapes()
+    k = 1 if prec == s.MIXED else 0
+    tuning = {"shapes": shapes["fp32" if k else "fp64"], "report": dev.tune_report(k)}
+    log(f"[{label}] kernel shapes {tuning['shapes']}")
     out = {
-        "dev": dev, "sizes": z, "seconds": t1 - t0, "history": np.array(rows),
+        "dev": dev, "sizes": z, "seconds": t1 - t0, "history": np.array(rows), "tuning": tuning,
         "fields": {n: dev.get(n) for n in names},
         "krylov_its_per_step": {n: v / steps for n, v in its.items()},
         "refinements_per_step": refinements / steps,
@@ -425,7 +433,7 @@ def main() -> int:
         "dtype": "f32+f64" if main_name == "mixed" else "f64",
         "data": "synthetic (seeded jittered hex/tet mesh generated in-process)",
         "config": {
-            "workload": wl.describe(main_name),
+            "workload": wl.describe(main_name), "autotune": wl.autotune, "graphs": wl.graphs,
             "cells": z.n_cell, "faces": z.n_face, "nnz": z.nnz,
             "cells_iters_per_s": value * z.n_cell,
             "precision": main_name,
@@ -440,7 +448,7 @@ def main() -> int:
         "paths": {
             n: {"iters_per_s": steps / v["seconds"], "ms_per_step": v["seconds"] * 1e3 / steps,
                 "krylov_its_per_step": v["krylov_its_per_step"], "refinements_per_step": v["refinements_per_step"],
-                "stage_ms_per_step": v["stage_ms_per_step"],
+                "stage_ms_per_step": v["stage_ms_per_step"], "kernel_tuning": v["tuning"],
                 "residual_history_last": v["history"][-1, [0, 1, 2, 3, 10]].tolist()}
             for n, v in runs.items()
         },

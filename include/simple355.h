@@ -303,10 +303,17 @@ int s355_spmv_time(s355_ctx* ctx, int which, int precision, int repeats, double*
 int s355_copy_time(s355_ctx* ctx, size_t bytes, int repeats, double* ms_per_call);
 
 /* Kernel shapes in use for the GMRES solves of each precision (index 0 FP64,
- * 1 FP32): lanes[k] SpMV lanes per row, rows[k] rows per thread of the
- * multi-dot / Gram-Schmidt / update kernels, tuned[k] 1 when they came from
- * the autotuner.  No reference counterpart. */
-int s355_kernel_shapes(s355_ctx* ctx, int lanes[2], int rows[2], int tuned[2]);
+ * 1 FP32): lanes[k] SpMV lanes per row, nt[k] 1 when the SpMV loads the
+ * matrix non-temporally, rows[k] rows per thread of the multi-dot /
+ * Gram-Schmidt / update kernels, tuned[k] 1 when they came from the
+ * autotuner.  No reference counterpart. */
+int s355_kernel_shapes(s355_ctx* ctx, int lanes[2], int nt[2], int rows[2], int tuned[2]);
+
+/* The autotuner's timings for one precision (0 FP64, 1 FP32): spmv_ms[v] for
+ * v = 2 * log2(lanes / 4) + nt, gs_ms[r] for 2^r rows per thread (one
+ * multi-dot + Gram-Schmidt step at iteration *gs_it); -1 where not timed.
+ * No reference counterpart. */
+int s355_tune_report(s355_ctx* ctx, int precision, double spmv_ms[6], double gs_ms[5], int* gs_it);
 
 /* Host-side set-up only (no device): the integer and metric output of the
  * face builder, for inspection and for tests on machines without a GPU. */

@@ -94,7 +94,8 @@ def library(path: Optional[Path] = None) -> C.CDLL:
         "spmv_time": (I, [V, I, I, I, pD]),
         "last_solve_info": (I, [V, C.POINTER(SolveInfo)]),
         "copy_time": (I, [V, C.c_size_t, I, pD]),
-        "kernel_shapes": (I, [V, pI, pI, pI]),
+        "kernel_shapes": (I, [V, pI, pI, pI, pI]),
+        "tune_report": (I, [V, I, pD, pD, pI]),
     }
     for name, (res, args) in more.items():
         fn = getattr(lib, f"s355_{name}")
@@ -364,10 +365,19 @@ class Solver:
         return ms.value
 
     def kernel_shapes(self) -> dict:
-        """SpMV lanes per row and Gram-Schmidt rows per thread, FP64 and FP32."""
-        lanes, rows, tuned = (C.c_int * 2)(), (C.c_int * 2)(), (C.c_int * 2)()
-        _check(self.lib, self.lib.s355_kernel_shapes(self.ctx, lanes, rows, tuned), "kernel_shapes")
-        return {p: {"lanes": lanes[k], "rows": rows[k], "tuned": bool(tuned[k])} for k, p in enumerate(("fp64", "fp32"))}
+        """SpMV lanes per row / non-temporal loads and Gram-Schmidt rows per
+        thread, FP64 and FP32."""
+        lanes, nt, rows, tuned = (C.c_int * 2)(), (C.c_int * 2)(), (C.c_int * 2)(), (C.c_int * 2)()
+        _check(self.lib, self.lib.s355_kernel_shapes(self.ctx, lanes, nt, rows, tuned), "kernel_shapes")
+        return {p: {"lanes": lanes[k], "nt": nt[k], "rows": rows[k], "tuned": bool(tuned[k])}
+                for k, p in enumerate(("fp64", "fp32"))}
+
+    def tune_report(self, precision: int) -> dict:
+        """Autotuner timings (ms): SpMV per (lanes, nt), Gram-Schmidt step per rows."""
+        sp, gs, it = (C.c_double * 6)(), (C.c_double * 5)(), C.c_int()
+        _check(self.lib, self.lib.s355_tune_report(self.ctx, precision, sp, gs, C.byref(it)), "tune_report")
+        spmv = {f"L{4 << (v // 2)}{'_nt' if v % 2 else ''}": sp[v] for v in range(6) if sp[v] >= 0}
+        return {"spmv_ms": spmv, "gs_ms": {f"R{1 << r}": gs[r] for r in range(5) if gs[r] >= 0}, "gs_it": it.value}
 
     def copy_time(self, nbytes: int, repeats: int) -> float:
         """Mean device ms of a plain copy of nbytes (multiple of 16)."""

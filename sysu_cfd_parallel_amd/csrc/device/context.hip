@@ -631,14 +631,18 @@ double Context::time_spmv(int which, int precision, int repeats) {
   convert_to_float(val, vf, dm.nnz, stream);
   convert_to_float(df.U, xf, dm.Nc, stream);
   CsrView<float> A{dm.Nc, dm.rowptr, dm.col, dm.diag, vf, dm.nnz};
-  return bench_spmv<float>(A, xf, yf, kwork_, repeats, stream);
+  KWork w = kwork_;  // the FP32 solves' SpMV shape
+  w.lanes = kwork32_.lanes;
+  w.nt = kwork32_.nt;
+  return bench_spmv<float>(A, xf, yf, w, repeats, stream);
 }
 
-void Context::kernel_shapes(int lanes[2], int rows[2], int tuned[2]) {
+void Context::kernel_shapes(int lanes[2], int nt[2], int rows[2], int tuned[2]) {
   const KWork* w[2] = {&kwork_, &kwork32_};
   const size_t elem[2] = {sizeof(double), sizeof(float)};
   for (int k = 0; k < 2; ++k) {
     lanes[k] = w[k]->lanes;
+    nt[k] = w[k]->nt;
     rows[k] = w[k]->rows ? w[k]->rows : pick_rows(dm.Nc, elem[k]);
     tuned[k] = w[k]->tuned ? 1 : 0;
   }

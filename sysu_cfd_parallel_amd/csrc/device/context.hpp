@@ -79,7 +79,8 @@ class Context {
   // benchmark helper
   double time_spmv(int which, int precision, int repeats);
   double time_copy(size_t bytes, int repeats);
-  void kernel_shapes(int lanes[2], int rows[2], int tuned[2]);
+  void kernel_shapes(int lanes[2], int nt[2], int rows[2], int tuned[2]);
+  const TuneReport& tune_report(int precision) const { return precision ? kwork32_.report : kwork_.report; }
 
  private:
   template <class T>

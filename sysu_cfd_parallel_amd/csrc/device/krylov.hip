@@ -152,20 +152,30 @@ void ilu_apply(const CsrView<T>& A, const IluLevels& lv, const T* lu, const T* r
 // ---- y = scale * M^-1 A x
 // L consecutive lanes share a row (L = 4, 8 or 16, picked per matrix): their
 // loads of values and column ids are contiguous across the wavefront; a
-// shuffle tree in a fixed order sums the lanes.
-template <class T, int L>
+// shuffle tree in a fixed order sums the lanes.  With NT the matrix streams
+// are loaded non-temporally, so that the L2 keeps the gathered x instead.
+template <class V, bool NT>
+__device__ __forceinline__ V ld_stream(const V* p) {
+  if constexpr (NT)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
+
+template <class T, int L, bool NT>
 __device__ __forceinline__ T row_dot(const CsrView<T>& A, const T* __restrict__ x, int row, int lane) {
   T s = T(0);
   if (row < A.n) {
     const int p1 = A.rowptr[row + 1];
-    for (int p = A.rowptr[row] + lane; p < p1; p += L) s += A.val[p] * x[A.col[p]];
+    for (int p = A.rowptr[row] + lane; p < p1; p += L)
+      s += ld_stream<T, NT>(A.val + p) * x[ld_stream<int, NT>(A.col + p)];
   }
 #pragma unroll
   for (int off = L / 2; off > 0; off >>= 1) s += __shfl_down(s, off, L);
   return s;
 }
 
-template <class T, int L>
+template <class T, int L, bool NT>
 __global__ void __launch_bounds__(kBlock) k_spmv_pc(CsrView<T> A, Tiles t, const T* __restrict__ x,
                                                     const T* __restrict__ inv, T* __restrict__ y, const KState* st,
                                                     int it) {
@@ -174,7 +184,7 @@ __global__ void __launch_bounds__(kBlock) k_spmv_pc(CsrView<T> A, Tiles t, const
   // the lanes of one row share its case, so they leave together
   const KState* sc = st + (row < A.n ? t.case_of(row) : 0);
   if (sc->reason) return;
-  const T s = row_dot<T, L>(A, x, row, lane);
+  const T s = row_dot<T, L, NT>(A, x, row, lane);
   if (lane == 0 && row < A.n) {
     double z = static_cast<double>(s);
     if (inv) z *= static_cast<double>(inv[row]);
@@ -182,14 +192,28 @@ __global__ void __launch_bounds__(kBlock) k_spmv_pc(CsrView<T> A, Tiles t, const
   }
 }
 
-template <class T>
-void launch_spmv(int L, const CsrView<T>& A, const Tiles& t, const T* x, const T* inv, T* y, const KState* st, int it,
-                 hipStream_t s) {
+// SpMV variants: lanes per row 4, 8, 16, each with temporal or non-temporal
+// matrix loads; variant v = 2 * log2(L / 4) + nt
+constexpr int kSpmvVariants = 6;
+inline int spmv_variant(int lanes, int nt) { return 2 * (lanes == 4 ? 0 : lanes == 8 ? 1 : 2) + (nt ? 1 : 0); }
+
+template <class T, int L>
+void launch_spmv_l(bool nt, const CsrView<T>& A, const Tiles& t, const T* x, const T* inv, T* y, const KState* st,
+                   int it, hipStream_t s) {
   const int g = grid_for(static_cast<long>(A.n) * L);
+  if (nt)
+    hipLaunchKernelGGL((k_spmv_pc<T, L, true>), dim3(g), dim3(kBlock), 0, s, A, t, x, inv, y, st, it);
+  else
+    hipLaunchKernelGGL((k_spmv_pc<T, L, false>), dim3(g), dim3(kBlock), 0, s, A, t, x, inv, y, st, it);
+}
+
+template <class T>
+void launch_spmv(int L, int nt, const CsrView<T>& A, const Tiles& t, const T* x, const T* inv, T* y,
+                 const KState* st, int it, hipStream_t s) {
   switch (L) {
-    case 4: hipLaunchKernelGGL((k_spmv_pc<T, 4>), dim3(g), dim3(kBlock), 0, s, A, t, x, inv, y, st, it); break;
-    case 16: hipLaunchKernelGGL((k_spmv_pc<T, 16>), dim3(g), dim3(kBlock), 0, s, A, t, x, inv, y, st, it); break;
-    default: hipLaunchKernelGGL((k_spmv_pc<T, 8>), dim3(g), dim3(kBlock), 0, s, A, t, x, inv, y, st, it); break;
+    case 4: launch_spmv_l<T, 4>(nt != 0, A, t, x, inv, y, st, it, s); break;
+    case 16: launch_spmv_l<T, 16>(nt != 0, A, t, x, inv, y, st, it, s); break;
+    default: launch_spmv_l<T, 8>(nt != 0, A, t, x, inv, y, st, it, s); break;
   }
 }
 
@@ -931,10 +955,10 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
   };
   auto enqueue_iteration = [&](int it) {
     if (lu) {
-      launch_spmv<T>(L, A, tl, V + it * ld, static_cast<const T*>(nullptr), tmp, w.st, it, s);
+      launch_spmv<T>(L, w.nt, A, tl, V + it * ld, static_cast<const T*>(nullptr), tmp, w.st, it, s);
       ilu_apply(A, *w.ilu, lu, tmp, V + (it + 1) * ld, ilu_st, s);
     } else {
-      launch_spmv<T>(L, A, tl, V + it * ld, inv, V + (it + 1) * ld, w.st, it, s);
+      launch_spmv<T>(L, w.nt, A, tl, V + it * ld, inv, V + (it + 1) * ld, w.st, it, s);
     }
     Dispatch<T>::mdot(R, V, ld, tl, it, w.partial, pstride, w.st, s);
     hipLaunchKernelGGL(k_mdot_final, dim3(it + 1, nc), dim3(kBlock), 0, s, w.partial, tl, pstride, w.st);
@@ -959,7 +983,8 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
                                           reinterpret_cast<uintptr_t>(b), reinterpret_cast<uintptr_t>(x),
                                           reinterpret_cast<uintptr_t>(inv), reinterpret_cast<uintptr_t>(invr),
                                           static_cast<uintptr_t>(m), static_cast<uintptr_t>(R),
-                                          static_cast<uintptr_t>(L), sizeof(T), sizeof(X),
+                                          static_cast<uintptr_t>(L), static_cast<uintptr_t>(w.nt), sizeof(T),
+                                          sizeof(X),
                                           static_cast<uintptr_t>(first_c), static_cast<uintptr_t>(zero_guess)};
       hipGraphExec_t g = w.graphs->get(key, s, [&] {
         enqueue_restart(first_c, first_c && zero_guess);
@@ -1171,7 +1196,11 @@ struct TuneKey {
   }
 };
 std::mutex g_tune_mu;
-std::map<TuneKey, std::pair<int, int>> g_tuned;  // (lanes, rows)
+struct TuneResult {
+  int lanes, nt, rows;
+  TuneReport report;
+};
+std::map<TuneKey, TuneResult> g_tuned;
 
 template <class F>
 double time_ms(F&& launch, int reps, hipStream_t s) {
@@ -1198,8 +1227,10 @@ void autotune(const CsrView<T>& A, KWork& w, int m, hipStream_t s) {
     std::lock_guard<std::mutex> g(g_tune_mu);
     auto it = g_tuned.find(key);
     if (it != g_tuned.end()) {
-      w.lanes = it->second.first;
-      w.rows = it->second.second;
+      w.lanes = it->second.lanes;
+      w.nt = it->second.nt;
+      w.rows = it->second.rows;
+      w.report = it->second.report;
       w.tuned = true;
       return;
     }
@@ -1218,15 +1249,20 @@ void autotune(const CsrView<T>& A, KWork& w, int m, hipStream_t s) {
   hipLaunchKernelGGL(k_bench_init, dim3(1), dim3(kBlock), 0, s, w.st, w.ncase);
   S355_LAUNCH_CHECK();
   const T* none = nullptr;
-  int best_l = 8;
+  TuneReport rep;
+  int best_l = 8, best_nt = 0;
   double best = 1e300;
-  for (int L : {4, 8, 16}) {
-    const double ms = time_ms([&] { launch_spmv<T>(L, A, w.tiles[0], V, none, V + ld, w.st, 0, s); }, 5, s);
-    if (ms < best) {
-      best = ms;
-      best_l = L;
+  for (int L : {4, 8, 16})
+    for (int nt : {0, 1}) {
+      const double ms =
+          time_ms([&] { launch_spmv<T>(L, nt, A, w.tiles[0], V, none, V + ld, w.st, 0, s); }, 5, s);
+      rep.spmv_ms[spmv_variant(L, nt)] = ms;
+      if (ms < best) {
+        best = ms;
+        best_l = L;
+        best_nt = nt;
+      }
     }
-  }
   // one mid-cycle Arnoldi step of the Gram-Schmidt pair at every tile size
   const int it = m / 2;
   int best_r = pick_r(n, sizeof(T));
@@ -1240,16 +1276,20 @@ void autotune(const CsrView<T>& A, KWork& w, int m, hipStream_t s) {
           Dispatch<T>::maxpy(R, V, ld, t, it, w.partial, w.st, s);
         },
         3, s);
+    rep.gs_ms[log2_r(R)] = ms;
     if (ms < best) {
       best = ms;
       best_r = R;
     }
   }
+  rep.gs_it = it;
   w.lanes = best_l;
+  w.nt = best_nt;
   w.rows = best_r;
+  w.report = rep;
   w.tuned = true;
   std::lock_guard<std::mutex> g(g_tune_mu);
-  g_tuned[key] = {best_l, best_r};
+  g_tuned[key] = TuneResult{best_l, best_nt, best_r, rep};
 }
 
 int pick_rows(int n, size_t elem) { return pick_r(n, elem); }
@@ -1355,14 +1395,14 @@ double bench_spmv(const CsrView<T>& A, const T* x, T* y, KWork& w, int repeats, 
   hipLaunchKernelGGL(k_bench_init, dim3(1), dim3(kBlock), 0, s, w.st, w.ncase);
   const Tiles& t = w.tiles[0];
   const T* none = nullptr;
-  launch_spmv<T>(w.lanes, A, t, x, none, y, w.st, 0, s);
+  launch_spmv<T>(w.lanes, w.nt, A, t, x, none, y, w.st, 0, s);
   S355_LAUNCH_CHECK();
   hipEvent_t e0, e1;
   S355_HIP(hipEventCreate(&e0));
   S355_HIP(hipEventCreate(&e1));
   S355_HIP(hipEventRecord(e0, s));
   for (int r = 0; r < repeats; ++r)
-    launch_spmv<T>(w.lanes, A, t, x, none, y, w.st, 0, s);
+    launch_spmv<T>(w.lanes, w.nt, A, t, x, none, y, w.st, 0, s);
   S355_HIP(hipEventRecord(e1, s));
   S355_HIP(hipEventSynchronize(e1));
   float ms = 0.f;

@@ -87,6 +87,15 @@ struct GraphCache {
   hipGraphExec_t get(const std::vector<uintptr_t>& key, hipStream_t s, const std::function<void()>& record);
 };
 
+// Timings taken by autotune(): SpMV per variant (lanes 4, 8, 16 x temporal /
+// non-temporal matrix loads) and one Gram-Schmidt step (multi-dot + update
+// at iteration gs_it) per rows-per-thread 1, 2, 4, 8, 16; -1 = not timed.
+struct TuneReport {
+  double spmv_ms[6] = {-1, -1, -1, -1, -1, -1};
+  double gs_ms[5] = {-1, -1, -1, -1, -1};
+  int gs_it = 0;
+};
+
 // Workspace owned by the caller.  ncase independent systems of the same
 // type share one launch per kernel: their rows are consecutive, their states
 // are st[0..ncase), and tiles[k] splits the rows into tiles of kBlock * 2^k
@@ -102,8 +111,10 @@ struct KWork {
   double* partial = nullptr;
   size_t partial_len = 0;
   int lanes = 8;            // SpMV lanes per row (4, 8, 16)
+  int nt = 0;               // SpMV loads the matrix non-temporally
   int rows = 0;             // rows per thread of the Gram-Schmidt kernels; 0: by size
-  bool tuned = false;       // lanes / rows chosen by autotune()
+  bool tuned = false;       // lanes / nt / rows chosen by autotune()
+  TuneReport report;
   GraphCache* graphs = nullptr;  // set: GMRES cycles (Jacobi / no PC) replay captured graphs
   KState* st = nullptr;     // device, ncase states
   KHeader* host = nullptr;  // pinned, ncase headers
@@ -139,9 +150,10 @@ KResult ir_solve(const CsrView<double>& A64, const CsrView<float>& A32, const do
 // Default rows per thread of the Gram-Schmidt kernels for n rows of elem bytes.
 int pick_rows(int n, size_t elem);
 
-// Times the SpMV at 4, 8 and 16 lanes per row and one mid-cycle Gram-Schmidt
-// step (multi-dot + update) at 1..16 rows per thread on A and the workspace,
-// and keeps the fastest in w.lanes / w.rows.  The choice is cached per
+// Times the SpMV at 4, 8 and 16 lanes per row with temporal and non-temporal
+// matrix loads, and one mid-cycle Gram-Schmidt step (multi-dot + update) at
+// 1..16 rows per thread, on A and the workspace; keeps the fastest in
+// w.lanes / w.nt / w.rows and the timings in w.report.  The choice is cached per
 // process and system shape, so every context of one process that solves
 // systems of that shape uses the same kernels (and reproduces its results).
 template <class T>

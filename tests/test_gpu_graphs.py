@@ -86,6 +86,7 @@ def test_autotuned_shapes_are_shared_within_a_process():
     b, hb, _, sb = run(case, p, 4)
     assert sa["fp32"]["tuned"] and sa == sb
     assert sa["fp32"]["lanes"] in (4, 8, 16) and sa["fp32"]["rows"] in (1, 2, 4, 8, 16)
+    assert sa["fp32"]["nt"] in (0, 1)
     for n in FIELDS:
         assert np.array_equal(a[n], b[n]), n
     q = base(s.MIXED)
