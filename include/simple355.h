@@ -292,9 +292,11 @@ int s355_last_solve_info(s355_ctx* ctx, s355_solve_info info[6]);
  * or a solve fails). */
 int s355_simple_iterate(s355_ctx* ctx, int n_steps, double* history, int* steps_done);
 
-/* Times the production SpMV kernel y = A x on A in {0 Au, 1 Ap, 2 As} (as
- * last assembled): precision 0 FP64, 1 FP32 copy.  *ms_per_call is the mean
- * device time over `repeats` launches (HIP events on the context stream). */
+/* Times the production SpMV launch y = M^-1 A x (Jacobi scaling, the kernel
+ * and shape the GMRES solves use) on A in {0 Au, 1 Ap, 2 As} as last
+ * assembled: precision 0 FP64, 1 FP32 copy.  *ms_per_call is the mean device
+ * time of `repeats` launches, each timed alone with HIP events on the context
+ * stream after a 512 MiB overwrite has cleared the Infinity Cache. */
 int s355_spmv_time(s355_ctx* ctx, int which, int precision, int repeats, double* ms_per_call);
 
 /* Mean device time of a plain device-to-device copy of `bytes` (a multiple
@@ -309,11 +311,17 @@ int s355_copy_time(s355_ctx* ctx, size_t bytes, int repeats, double* ms_per_call
  * autotuner.  No reference counterpart. */
 int s355_kernel_shapes(s355_ctx* ctx, int lanes[2], int nt[2], int rows[2], int tuned[2]);
 
+/* Sets the kernel shapes of one precision's GMRES solves explicitly (the
+ * autotuner then leaves them alone): lanes 1, 2, 4, 8, 16 (CSR SpMV) or 0
+ * (SELL-64 SpMV), nt 0/1, rows 1, 2, 4, 8, 16 or 0 (by size).  No reference
+ * counterpart. */
+int s355_set_kernel_shapes(s355_ctx* ctx, int precision, int lanes, int nt, int rows);
+
 /* The autotuner's timings for one precision (0 FP64, 1 FP32): spmv_ms[v] for
- * v = 2 * log2(lanes / 4) + nt, gs_ms[r] for 2^r rows per thread (one
- * multi-dot + Gram-Schmidt step at iteration *gs_it); -1 where not timed.
- * No reference counterpart. */
-int s355_tune_report(s355_ctx* ctx, int precision, double spmv_ms[6], double gs_ms[5], int* gs_it);
+ * v = 2 k + nt, k = 0..4 for CSR at 1, 2, 4, 8, 16 lanes per row, k = 5 for
+ * SELL-64; gs_ms[r] for 2^r rows per thread (one multi-dot + Gram-Schmidt
+ * step at iteration *gs_it); -1 where not timed.  No reference counterpart. */
+int s355_tune_report(s355_ctx* ctx, int precision, double spmv_ms[12], double gs_ms[5], int* gs_it);
 
 /* Host-side set-up only (no device): the integer and metric output of the
  * face builder, for inspection and for tests on machines without a GPU. */

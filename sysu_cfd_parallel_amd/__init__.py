@@ -96,6 +96,7 @@ def library(path: Optional[Path] = None) -> C.CDLL:
         "copy_time": (I, [V, C.c_size_t, I, pD]),
         "kernel_shapes": (I, [V, pI, pI, pI, pI]),
         "tune_report": (I, [V, I, pD, pD, pI]),
+        "set_kernel_shapes": (I, [V, I, I, I, I]),
     }
     for name, (res, args) in more.items():
         fn = getattr(lib, f"s355_{name}")
@@ -374,10 +375,15 @@ class Solver:
 
     def tune_report(self, precision: int) -> dict:
         """Autotuner timings (ms): SpMV per (lanes, nt), Gram-Schmidt step per rows."""
-        sp, gs, it = (C.c_double * 6)(), (C.c_double * 5)(), C.c_int()
+        sp, gs, it = (C.c_double * 12)(), (C.c_double * 5)(), C.c_int()
         _check(self.lib, self.lib.s355_tune_report(self.ctx, precision, sp, gs, C.byref(it)), "tune_report")
-        spmv = {f"L{4 << (v // 2)}{'_nt' if v % 2 else ''}": sp[v] for v in range(6) if sp[v] >= 0}
+        names = ["L1", "L2", "L4", "L8", "L16", "SELL64"]
+        spmv = {f"{names[v // 2]}{'_nt' if v % 2 else ''}": sp[v] for v in range(12) if sp[v] >= 0}
         return {"spmv_ms": spmv, "gs_ms": {f"R{1 << r}": gs[r] for r in range(5) if gs[r] >= 0}, "gs_it": it.value}
+
+    def set_kernel_shapes(self, precision: int, lanes: int, nt: int = 0, rows: int = 0) -> None:
+        """Force the GMRES kernel shapes of one precision (lanes 0 = SELL-64)."""
+        _check(self.lib, self.lib.s355_set_kernel_shapes(self.ctx, precision, lanes, nt, rows), "set_kernel_shapes")
 
     def copy_time(self, nbytes: int, repeats: int) -> float:
         """Mean device ms of a plain copy of nbytes (multiple of 16)."""

@@ -614,11 +614,18 @@ int s355_kernel_shapes(s355_ctx* ctx, int lanes[2], int nt[2], int rows[2], int 
   });
 }
 
+int s355_set_kernel_shapes(s355_ctx* ctx, int precision, int lanes, int nt, int rows) {
+  return guarded([&] {
+    dev(ctx).set_kernel_shapes(precision, lanes, nt, rows);
+    return S355_OK;
+  });
+}
+
 int s355_tune_report(s355_ctx* ctx, int precision, double spmv_ms[6], double gs_ms[5], int* gs_it) {
   return guarded([&] {
     if (!spmv_ms || !gs_ms || !gs_it || precision < 0 || precision > 1) throw std::invalid_argument("tune_report");
     const s355::TuneReport& r = dev(ctx).tune_report(precision);
-    for (int k = 0; k < 6; ++k) spmv_ms[k] = r.spmv_ms[k];
+    for (int k = 0; k < 12; ++k) spmv_ms[k] = r.spmv_ms[k];
     for (int k = 0; k < 5; ++k) gs_ms[k] = r.gs_ms[k];
     *gs_it = r.gs_it;
     return S355_OK;

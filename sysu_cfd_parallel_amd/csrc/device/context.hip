@@ -466,13 +466,15 @@ int Context::run_solve(int which, const double* b, double* x, const s355_ksp_opt
     convert_to_float(val, a32_, dm.nnz, stream);
     CsrView<float> A{dm.Nc, dm.rowptr, dm.col, dm.diag, a32_, dm.nnz};
     CsrView<double> A64{dm.Nc, dm.rowptr, dm.col, dm.diag, val, dm.nnz};
-    if (par.autotune && o.type == S355_KSP_GMRES && !kwork32_.tuned)
-      autotune<float>(A, kwork32_, std::min(o.restart, kMaxRestart), stream);
+    const bool tune = par.autotune && o.type == S355_KSP_GMRES && !kwork32_.tuned;
+    if (o.type == S355_KSP_GMRES && (tune || kwork32_.lanes == 0)) prepare_sell(val, true);
+    if (tune) autotune<float>(A, kwork32_, std::min(o.restart, kMaxRestart), stream);
     r = ir_solve(A64, A, b, x, o, zero_guess, par.ir_max, par.ir_inner_rtol, kwork32_, ir_, stream);
   } else {
     CsrView<double> A{dm.Nc, dm.rowptr, dm.col, dm.diag, val, dm.nnz};
-    if (par.autotune && o.type == S355_KSP_GMRES && !kwork_.tuned)
-      autotune<double>(A, kwork_, std::min(o.restart, kMaxRestart), stream);
+    const bool tune = par.autotune && o.type == S355_KSP_GMRES && !kwork_.tuned;
+    if (o.type == S355_KSP_GMRES && (tune || kwork_.lanes == 0)) prepare_sell(val, false);
+    if (tune) autotune<double>(A, kwork_, std::min(o.restart, kMaxRestart), stream);
     r = krylov_solve<double>(A, b, x, o, zero_guess, kwork_, stream);
   }
   if (ncase == 1) case_info[slot].assign(1, r);
@@ -613,28 +615,87 @@ void Context::check_flags() {
   if (h[1]) throw std::runtime_error("reverse flow get out of inlet (scalar transport)");
 }
 
+void Context::ensure_sell() {
+  if (sell_src_) return;
+  std::vector<int> ptr, col, src;
+  sell_pattern(dm.Nc, mesh.rowptr.data(), mesh.col.data(), ptr, col, src);
+  sell_slots_ = src.size();
+  const int* dptr = upload(allocs_, ptr, stream);
+  const int* dcol = upload(allocs_, col, stream);
+  sell_src_ = upload(allocs_, src, stream);
+  sync();  // host staging vectors go out of scope
+  kwork_.sell_ptr = kwork32_.sell_ptr = dptr;
+  kwork_.sell_col = kwork32_.sell_col = dcol;
+}
+
+void Context::prepare_sell(const double* csr_val, bool fp32) {
+  ensure_sell();
+  if (fp32) {
+    if (!sell32_) sell32_ = alloc<float>(sell_slots_);
+    sell_fill<float, double>(sell_src_, sell_slots_, csr_val, sell32_, stream);
+    kwork32_.sell_val = sell32_;
+  } else {
+    if (!sell64_) sell64_ = alloc<double>(sell_slots_);
+    sell_fill<double, double>(sell_src_, sell_slots_, csr_val, sell64_, stream);
+    kwork_.sell_val = sell64_;
+  }
+}
+
 double Context::time_spmv(int which, int precision, int repeats) {
   const double* val = which == 0 ? df.Au : (which == 1 ? df.Ap : df.As);
   if (repeats < 1) repeats = 1;
-  double* y = static_cast<double*>(kwork_.V);
-  if (precision == 0) {
-    CsrView<double> A{dm.Nc, dm.rowptr, dm.col, dm.diag, val, dm.nnz};
-    return bench_spmv<double>(A, df.U, y, kwork_, repeats, stream);
+  const int m = std::min(par.ksp[S355_KSP_PRESSURE].restart, kMaxRestart);
+  // a scratch buffer larger than the 256 MB Infinity Cache
+  const size_t flush_bytes = size_t(512) << 20;
+  void* flush = nullptr;
+  S355_HIP(hipMalloc(&flush, flush_bytes));
+  double ms = 0.0;
+  try {
+    if (precision == 0) {
+      CsrView<double> A{dm.Nc, dm.rowptr, dm.col, dm.diag, val, dm.nnz};
+      prepare_sell(val, false);
+      if (par.autotune && !kwork_.tuned) autotune<double>(A, kwork_, m, stream);
+      double* inv = static_cast<double*>(kwork_.pcinv);
+      jacobi_inverse<double>(A, inv, stream);
+      double* y = static_cast<double*>(kwork_.V);
+      ms = bench_spmv<double>(A, df.U, y, inv, kwork_, repeats, flush, flush_bytes, stream);
+    } else {
+      ensure_mixed();
+      convert_to_float(val, a32_, dm.nnz, stream);
+      CsrView<float> A{dm.Nc, dm.rowptr, dm.col, dm.diag, a32_, dm.nnz};
+      prepare_sell(val, true);
+      if (par.autotune && !kwork32_.tuned) autotune<float>(A, kwork32_, m, stream);
+      float* inv = static_cast<float*>(kwork32_.pcinv);
+      jacobi_inverse<float>(A, inv, stream);
+      float* xf = static_cast<float*>(kwork32_.V);
+      float* yf = xf + dm.Nc;
+      convert_to_float(df.U, xf, dm.Nc, stream);
+      ms = bench_spmv<float>(A, xf, yf, inv, kwork32_, repeats, flush, flush_bytes, stream);
+    }
+  } catch (...) {
+    hipStreamSynchronize(stream);
+    hipFree(flush);
+    throw;
   }
-  // FP32 copy of the matrix and of x in the Krylov workspace
-  float* vf = reinterpret_cast<float*>(y + dm.Nc);
-  float* xf = vf + dm.nnz;
-  float* yf = xf + dm.Nc;
-  if ((static_cast<size_t>(max_restart_) + 1) * dm.Nc * sizeof(double) <
-      (static_cast<size_t>(dm.Nc) * 2 + dm.nnz) * sizeof(float) + dm.Nc * sizeof(double))
-    throw std::runtime_error("time_spmv: workspace too small for the FP32 copy");
-  convert_to_float(val, vf, dm.nnz, stream);
-  convert_to_float(df.U, xf, dm.Nc, stream);
-  CsrView<float> A{dm.Nc, dm.rowptr, dm.col, dm.diag, vf, dm.nnz};
-  KWork w = kwork_;  // the FP32 solves' SpMV shape
-  w.lanes = kwork32_.lanes;
-  w.nt = kwork32_.nt;
-  return bench_spmv<float>(A, xf, yf, w, repeats, stream);
+  sync();
+  hipFree(flush);
+  return ms;
+}
+
+void Context::set_kernel_shapes(int precision, int lanes, int nt, int rows) {
+  auto ok = [](int v, std::initializer_list<int> set) {
+    for (int x : set)
+      if (v == x) return true;
+    return false;
+  };
+  if (precision < 0 || precision > 1 || !ok(lanes, {0, 1, 2, 4, 8, 16}) || !ok(nt, {0, 1}) ||
+      !ok(rows, {0, 1, 2, 4, 8, 16}))
+    throw std::invalid_argument("kernel shapes: lanes 0 (SELL-64), 1, 2, 4, 8, 16; nt 0/1; rows 0, 1, 2, 4, 8, 16");
+  KWork& w = precision ? kwork32_ : kwork_;
+  w.lanes = lanes;
+  w.nt = nt;
+  w.rows = rows;
+  w.tuned = true;  // the autotuner leaves explicit shapes alone
 }
 
 void Context::kernel_shapes(int lanes[2], int nt[2], int rows[2], int tuned[2]) {

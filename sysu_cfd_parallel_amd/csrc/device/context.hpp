@@ -80,6 +80,7 @@ class Context {
   double time_spmv(int which, int precision, int repeats);
   double time_copy(size_t bytes, int repeats);
   void kernel_shapes(int lanes[2], int nt[2], int rows[2], int tuned[2]);
+  void set_kernel_shapes(int precision, int lanes, int nt, int rows);
   const TuneReport& tune_report(int precision) const { return precision ? kwork32_.report : kwork_.report; }
 
  private:
@@ -93,6 +94,8 @@ class Context {
   Tiles make_tiles(const std::vector<int>& case_row, int tile);
   void ensure_mixed();
   void ensure_ilu();
+  void ensure_sell();
+  void prepare_sell(const double* csr_val, bool fp32);
 
   std::vector<void*> allocs_;
   double* partial_ = nullptr;
@@ -111,6 +114,12 @@ class Context {
   IrWork ir_{};
   // ILU(0): level schedule of the pattern and factor storage
   std::unique_ptr<GraphCache> graphs_;  // captured GMRES cycles (params.graphs)
+  // SELL-64 copy of the matrix being solved: pattern and source map built once,
+  // values filled before each GMRES solve that uses it
+  const int* sell_src_ = nullptr;
+  size_t sell_slots_ = 0;
+  double* sell64_ = nullptr;
+  float* sell32_ = nullptr;
   IluLevels levels_{};
   bool have_levels_ = false;
   void* host_pinned_ = nullptr;

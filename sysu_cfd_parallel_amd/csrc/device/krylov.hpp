@@ -58,6 +58,29 @@ struct CsrView {
   int nnz = 0;
 };
 
+// Sliced ELL (SELL-64) copy of a CSR matrix for the SpMV: rows in slices of
+// 64, one row per lane; slice s holds width_s = its longest row's length
+// columns stored column-major (entry j of row 64 s + l at ptr[s] + 64 j + l),
+// shorter rows padded with zero values and their own column.  Every lane then
+// has all its row's loads independent and in flight, and a wave's loads are
+// contiguous.  Entries keep the CSR order within a row.
+template <class T>
+struct SellView {
+  int n = 0;
+  const int* ptr = nullptr;  // nslice + 1 offsets
+  const int* col = nullptr;
+  const T* val = nullptr;
+};
+
+// Host construction of the pattern: ptr, col and, for every SELL slot, the
+// CSR position it copies (-1 for padding).
+void sell_pattern(int n, const int* rowptr, const int* col, std::vector<int>& ptr, std::vector<int>& scol,
+                  std::vector<int>& src);
+
+// out[q] = csr_val[src[q]] (0 where src[q] < 0), converted to T.
+template <class T, class S>
+void sell_fill(const int* src, size_t nslots, const S* csr_val, T* out, hipStream_t s);
+
 // Level schedule of the CSR pattern for ILU(0) (natural ordering): rows of a
 // level depend only on rows of earlier levels.  Lower levels order the
 // forward sweep and the factorisation, upper levels the backward sweep.
@@ -87,11 +110,13 @@ struct GraphCache {
   hipGraphExec_t get(const std::vector<uintptr_t>& key, hipStream_t s, const std::function<void()>& record);
 };
 
-// Timings taken by autotune(): SpMV per variant (lanes 4, 8, 16 x temporal /
-// non-temporal matrix loads) and one Gram-Schmidt step (multi-dot + update
-// at iteration gs_it) per rows-per-thread 1, 2, 4, 8, 16; -1 = not timed.
+// Timings taken by autotune(): SpMV per variant v = 2 k + nt (k = 0..4: CSR
+// with 1, 2, 4, 8, 16 lanes per row, k = 5: SELL-64; nt: non-temporal matrix
+// loads) and one Gram-Schmidt step (multi-dot + update at iteration gs_it)
+// per rows-per-thread 1, 2, 4, 8, 16; -1 = not timed.
 struct TuneReport {
-  double spmv_ms[6] = {-1, -1, -1, -1, -1, -1};
+  // CSR lanes 1, 2, 4, 8, 16 then SELL-64, each temporal / non-temporal
+  double spmv_ms[12] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
   double gs_ms[5] = {-1, -1, -1, -1, -1};
   int gs_it = 0;
 };
@@ -110,8 +135,12 @@ struct KWork {
   const IluLevels* ilu = nullptr;
   double* partial = nullptr;
   size_t partial_len = 0;
-  int lanes = 8;            // SpMV lanes per row (4, 8, 16)
+  int lanes = 8;            // CSR SpMV lanes per row (1, 2, 4, 8, 16); 0 = SELL-64 SpMV
   int nt = 0;               // SpMV loads the matrix non-temporally
+  // SELL-64 copy of the matrix being solved (values of type T), when set
+  const int* sell_ptr = nullptr;
+  const int* sell_col = nullptr;
+  const void* sell_val = nullptr;
   int rows = 0;             // rows per thread of the Gram-Schmidt kernels; 0: by size
   bool tuned = false;       // lanes / nt / rows chosen by autotune()
   TuneReport report;
@@ -169,10 +198,17 @@ KResult krylov_solve(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts&
 template <class T>
 void residual64(const CsrView<T>& A, const double* b, const double* x, double* r, hipStream_t s);
 
-// Average device time (ms) of the production SpMV kernel y = A x over
-// `repeats` back-to-back launches, measured with HIP events on stream s.
+// Average device time (ms) of the production SpMV launch y = M^-1 A x (the
+// kernel and shape the GMRES solves use, inv = Jacobi inverse or null),
+// each of `repeats` launches timed alone with HIP events on stream s after
+// `flush_bytes` of `flush` have been overwritten (null: back to back).
 template <class T>
-double bench_spmv(const CsrView<T>& A, const T* x, T* y, KWork& w, int repeats, hipStream_t s);
+double bench_spmv(const CsrView<T>& A, const T* x, T* y, const T* inv, KWork& w, int repeats, void* flush,
+                  size_t flush_bytes, hipStream_t s);
+
+// inv = 1 / diag(A) (1 where the diagonal is zero)
+template <class T>
+void jacobi_inverse(const CsrView<T>& A, T* inv, hipStream_t s);
 
 // Average device time (ms) of a plain copy of `bytes` (a multiple of 16).
 double bench_copy(void* dst, const void* src, size_t bytes, int repeats, hipStream_t s);

@@ -70,9 +70,11 @@ def test_iteration_cap_inside_a_cycle(graphs):
     p = base(s.FP64, restart=5)
     p.graphs = graphs
     d = s.Solver(case, p)
+    d.iterate(1)  # a moving lid: a nonzero mass imbalance to solve for
     d.call("build_velocity_matrix")
     d.call("cal_ru_face")
     d.call("build_pressure_matrix")
+    assert np.abs(d.get("BP_RHS")).max() > 0
     info = d.solve_pressure(1e-14, 7)
     assert info["reason"] == -3 and info["iters"] == 7
 
@@ -95,3 +97,37 @@ def test_autotuned_shapes_are_shared_within_a_process():
     assert not sc["fp32"]["tuned"]
     for n in ("U", "V", "W", "P"):
         assert rel_err(a[n], c[n]) < 1e-7, n
+
+
+@pytest.mark.parametrize("prec", [s.FP64, s.MIXED])
+def test_sell_spmv_equals_single_lane_csr(prec):
+    # both sum each row's entries in CSR order, one row per lane
+    case = s.channel(12, 6, 6, jitter=0.15, split=0.2, seed=8, re=600.0)
+    k = 1 if prec == s.MIXED else 0
+    out = []
+    for lanes in (1, 0):
+        d = s.Solver(case, base(prec))
+        d.set_kernel_shapes(k, lanes)
+        h, _ = d.iterate(5)
+        assert d.kernel_shapes()["fp32" if k else "fp64"]["lanes"] == lanes
+        out.append(({n: d.get(n) for n in FIELDS}, h))
+        d.close()
+    for n in FIELDS:
+        assert np.array_equal(out[0][0][n], out[1][0][n]), n
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("lanes", [0, 2, 16])
+def test_every_spmv_shape_matches_the_oracle(lanes):
+    from oracle.pyoracle import Oracle
+
+    case = s.cavity(8, re=100.0, jitter=0.1, seed=1)
+    p = params(rtol=1e-12)
+    d = s.Solver(case, p)
+    d.set_kernel_shapes(0, lanes, nt=1 if lanes == 16 else 0, rows=2)
+    o = Oracle(case, p)
+    hd, _ = d.iterate(10)
+    ho, _, _ = o.iterate(10)
+    assert np.allclose(hd, ho, rtol=1e-7, atol=1e-14)
+    for n in ("U", "V", "W", "P"):
+        assert rel_err(d.get(n), o.get(n)) < 1e-8, n
