@@ -236,7 +236,11 @@ void Context::allocate_fields() {
   kwork_.n = static_cast<int>(Nc);
   kwork_.ncase = ncase;
   kwork_.mmax = max_restart_;
-  kwork_.V = alloc<double>((static_cast<size_t>(max_restart_) + 1) * Nc);
+  // basis columns padded to a multiple of 8 elements, so that every column
+  // starts 16-byte aligned for the vector loads of the Gram-Schmidt kernels
+  const size_t ldp = (Nc + 7) / 8 * 8;
+  kwork_.ld = ldp;
+  kwork_.V = alloc<double>((static_cast<size_t>(max_restart_) + 1) * ldp);
   kwork_.r = alloc<double>(Nc);
   kwork_.pcinv = alloc<double>(Nc);
   kwork_.partial = partial_;
@@ -430,7 +434,8 @@ void Context::ensure_mixed() {
   kwork32_.lu = alloc<float>(dm.nnz);
   kwork32_.n = dm.Nc;
   kwork32_.mmax = max_restart_;
-  kwork32_.V = alloc<float>((static_cast<size_t>(max_restart_) + 1) * Nc);
+  kwork32_.ld = kwork_.ld;
+  kwork32_.V = alloc<float>((static_cast<size_t>(max_restart_) + 1) * kwork32_.ld);
   kwork32_.pcinv = alloc<float>(Nc);
   kwork32_.partial = partial_;
   kwork32_.partial_len = partial_len_;

@@ -273,22 +273,82 @@ SellView<T> sell_of(const KWork& w, int n) {
   return v;
 }
 
+// ---- Gram-Schmidt kernels.  Thread t of a tile handles R rows in chunks of
+// VW consecutive rows (VW = 16 bytes of T, at most R): chunk q covers rows
+// a + (q * kBlock + t) * VW ... + VW - 1.  A block whose tile start and
+// column stride are multiples of VW loads each chunk with one 16-byte vector
+// load; any other block (a tile of a later case in a batch) loads the same
+// rows one by one, so the arithmetic and its order never depend on alignment.
+template <class T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  using type = float4;
+  static constexpr int n = 4;
+};
+template <>
+struct Vec16<double> {
+  using type = double2;
+  static constexpr int n = 2;
+};
+
+template <class T, int VW>
+__device__ __forceinline__ void load_chunk(const T* __restrict__ p, size_t i0, size_t end, bool vec, double* out) {
+  if constexpr (VW == Vec16<T>::n) {
+    if (vec && i0 + VW <= end) {
+      const typename Vec16<T>::type v = *reinterpret_cast<const typename Vec16<T>::type*>(p + i0);
+      const T* e = reinterpret_cast<const T*>(&v);
+#pragma unroll
+      for (int u = 0; u < VW; ++u) out[u] = static_cast<double>(e[u]);
+      return;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < VW; ++u) out[u] = i0 + u < end ? static_cast<double>(p[i0 + u]) : 0.0;
+}
+
+template <class T, int VW>
+__device__ __forceinline__ void store_chunk(T* __restrict__ p, size_t i0, size_t end, bool vec, const T* in) {
+  if constexpr (VW == Vec16<T>::n) {
+    if (vec && i0 + VW <= end) {
+      typename Vec16<T>::type v;
+      T* e = reinterpret_cast<T*>(&v);
+#pragma unroll
+      for (int u = 0; u < VW; ++u) e[u] = in[u];
+      *reinterpret_cast<typename Vec16<T>::type*>(p + i0) = v;
+      return;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < VW; ++u)
+    if (i0 + u < end) p[i0 + u] = in[u];
+}
+
+template <class T, int R>
+struct GsShape {
+  static constexpr int VW = R < Vec16<T>::n ? R : Vec16<T>::n;
+  static constexpr int NC = R / VW;
+  __device__ static size_t row0(int a, int q) {
+    return static_cast<size_t>(a) + (static_cast<size_t>(q) * kBlock + threadIdx.x) * VW;
+  }
+  __device__ static bool vec_ok(int a, size_t ld) { return (a % Vec16<T>::n) == 0 && (ld % Vec16<T>::n) == 0; }
+};
+
 // ---- partial[b][k] = sum over tile b of w_i V_k,i, k = 0..it
 template <class T, int R>
 __global__ void __launch_bounds__(kBlock) k_mdot(const T* __restrict__ V, size_t ld, Tiles t, int it,
                                                  double* __restrict__ partial, int pstride, const KState* st) {
+  using S = GsShape<T, R>;
   int a, e, c;
   t.range(blockIdx.x, a, e, c);
   if (st[c].reason) return;
   __shared__ double lds[kBlock / kWave][8];
+  const size_t end = static_cast<size_t>(e);
+  const bool vec = S::vec_ok(a, ld);
   const T* w = V + static_cast<size_t>(it + 1) * ld;
-  const size_t base = static_cast<size_t>(a) + threadIdx.x, end = static_cast<size_t>(e);
   double wr[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const size_t idx = base + static_cast<size_t>(r) * kBlock;
-    wr[r] = idx < end ? static_cast<double>(w[idx]) : 0.0;
-  }
+  for (int q = 0; q < S::NC; ++q) load_chunk<T, S::VW>(w, S::row0(a, q), end, vec, wr + q * S::VW);
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
   for (int k0 = 0; k0 <= it; k0 += 8) {
     double acc[8];
@@ -299,9 +359,11 @@ __global__ void __launch_bounds__(kBlock) k_mdot(const T* __restrict__ V, size_t
       if (k <= it) {
         const T* vk = V + static_cast<size_t>(k) * ld;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const size_t idx = base + static_cast<size_t>(r) * kBlock;
-          if (idx < end) acc[kk] += wr[r] * static_cast<double>(vk[idx]);
+        for (int q = 0; q < S::NC; ++q) {
+          double v[S::VW];
+          load_chunk<T, S::VW>(vk, S::row0(a, q), end, vec, v);
+#pragma unroll
+          for (int u = 0; u < S::VW; ++u) acc[kk] += wr[q * S::VW + u] * v[u];
         }
       }
     }
@@ -340,6 +402,7 @@ __global__ void __launch_bounds__(kBlock) k_mdot_final(const double* partial, Ti
 template <class T, int R>
 __global__ void __launch_bounds__(kBlock) k_maxpy_norm(T* __restrict__ V, size_t ld, Tiles t, int it,
                                                        double* __restrict__ partial, const KState* st) {
+  using S = GsShape<T, R>;
   int a, e, c;
   t.range(blockIdx.x, a, e, c);
   const KState* sc = st + c;
@@ -348,19 +411,28 @@ __global__ void __launch_bounds__(kBlock) k_maxpy_norm(T* __restrict__ V, size_t
   __shared__ double lds[kBlock / kWave];
   for (int k = threadIdx.x; k <= it; k += kBlock) cf[k] = sc->hcol[k] * sc->scale[k];
   __syncthreads();
+  const size_t end = static_cast<size_t>(e);
+  const bool vec = S::vec_ok(a, ld);
   T* w = V + static_cast<size_t>(it + 1) * ld;
-  const size_t base = static_cast<size_t>(a) + threadIdx.x, end = static_cast<size_t>(e);
   double nrm = 0.0;
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const size_t idx = base + static_cast<size_t>(r) * kBlock;
-    if (idx < end) {
-      double s = static_cast<double>(w[idx]);
-      for (int k = 0; k <= it; ++k) s -= cf[k] * static_cast<double>(V[static_cast<size_t>(k) * ld + idx]);
-      const T sv = static_cast<T>(s);
-      w[idx] = sv;
-      nrm += static_cast<double>(sv) * static_cast<double>(sv);
+  for (int q = 0; q < S::NC; ++q) {
+    const size_t i0 = S::row0(a, q);
+    double sacc[S::VW];
+    load_chunk<T, S::VW>(w, i0, end, vec, sacc);
+    for (int k = 0; k <= it; ++k) {
+      double v[S::VW];
+      load_chunk<T, S::VW>(V + static_cast<size_t>(k) * ld, i0, end, vec, v);
+#pragma unroll
+      for (int u = 0; u < S::VW; ++u) sacc[u] -= cf[k] * v[u];
     }
+    T out[S::VW];
+#pragma unroll
+    for (int u = 0; u < S::VW; ++u) {
+      out[u] = static_cast<T>(sacc[u]);
+      if (i0 + u < end) nrm += static_cast<double>(out[u]) * static_cast<double>(out[u]);
+    }
+    store_chunk<T, S::VW>(w, i0, end, vec, out);
   }
   nrm = block_sum(nrm, lds);
   if (threadIdx.x == 0) partial[blockIdx.x] = nrm;
@@ -474,6 +546,7 @@ __global__ void __launch_bounds__(kWave) k_build_y(KState* st, int m) {
 template <class T, class X, int R>
 __global__ void __launch_bounds__(kBlock) k_update_x(const T* __restrict__ V, size_t ld, Tiles t, X* __restrict__ x,
                                                      const KState* st) {
+  using S = GsShape<T, R>;
   int a, e, c;
   t.range(blockIdx.x, a, e, c);
   const KState* sc = st + c;
@@ -482,15 +555,23 @@ __global__ void __launch_bounds__(kBlock) k_update_x(const T* __restrict__ V, si
   __shared__ double y[kMaxRestart + 1];
   for (int k = threadIdx.x; k < nk; k += kBlock) y[k] = sc->y[k];
   __syncthreads();
-  const size_t base = static_cast<size_t>(a) + threadIdx.x, end = static_cast<size_t>(e);
+  const size_t end = static_cast<size_t>(e);
+  const bool vec = S::vec_ok(a, ld);
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const size_t idx = base + static_cast<size_t>(r) * kBlock;
-    if (idx < end) {
-      double s = 0.0;
-      for (int k = 0; k < nk; ++k) s += y[k] * static_cast<double>(V[static_cast<size_t>(k) * ld + idx]);
-      x[idx] = static_cast<X>(static_cast<double>(x[idx]) + s);
+  for (int q = 0; q < S::NC; ++q) {
+    const size_t i0 = S::row0(a, q);
+    double sacc[S::VW];
+#pragma unroll
+    for (int u = 0; u < S::VW; ++u) sacc[u] = 0.0;
+    for (int k = 0; k < nk; ++k) {
+      double v[S::VW];
+      load_chunk<T, S::VW>(V + static_cast<size_t>(k) * ld, i0, end, vec, v);
+#pragma unroll
+      for (int u = 0; u < S::VW; ++u) sacc[u] += y[k] * v[u];
     }
+#pragma unroll
+    for (int u = 0; u < S::VW; ++u)
+      if (i0 + u < end) x[i0 + u] = static_cast<X>(static_cast<double>(x[i0 + u]) + sacc[u]);
   }
 }
 
@@ -942,7 +1023,7 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
   const int m = o.restart < 1 ? 1 : (o.restart > kMaxRestart ? kMaxRestart : o.restart);
   if (m > w.mmax) throw std::invalid_argument("gmres: restart above workspace size");
   if (Ares.n != n || w.n != n) throw std::invalid_argument("gmres: operator sizes differ");
-  const size_t ld = static_cast<size_t>(n);
+  const size_t ld = w.ld ? w.ld : static_cast<size_t>(n);
   const int R = w.rows ? w.rows : pick_r(n, sizeof(T));
   const SellView<T> sell = sell_of<T>(w, n);
   const int L = (w.lanes == 0 && !sell.ptr) ? 8 : w.lanes;
@@ -1294,7 +1375,7 @@ void autotune(const CsrView<T>& A, KWork& w, int m, hipStream_t s) {
   }
   const int n = A.n;
   if (n <= 0 || m < 1 || m > w.mmax) return;
-  const size_t ld = static_cast<size_t>(n);
+  const size_t ld = w.ld ? w.ld : static_cast<size_t>(n);
   T* V = static_cast<T*>(w.V);
   // a zero basis: the timings do not depend on the values
   fill_zero(V, sizeof(T) * ld * (static_cast<size_t>(m) + 1), s);

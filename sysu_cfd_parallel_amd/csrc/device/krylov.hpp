@@ -127,8 +127,9 @@ struct TuneReport {
 // rows that never straddle two cases.
 struct KWork {
   int n = 0, mmax = 0, ncase = 1;
+  size_t ld = 0;            // column stride of V (n rounded up to 8; 0 = n)
   Tiles tiles[5];
-  void* V = nullptr;        // (mmax + 1) * n elements of T
+  void* V = nullptr;        // (mmax + 1) * ld elements of T
   void* r = nullptr;        // n elements of T: preconditioner input (CG: r, z, p, w use V[0..3])
   void* pcinv = nullptr;    // n elements of T (Jacobi inverse diagonal)
   void* lu = nullptr;       // nnz elements of T (ILU(0) factors), when ilu is set
