@@ -99,6 +99,10 @@ typedef struct s355_params {
                            process and system shape); 0: fixed heuristics */
   int graphs;           /* 1: capture each GMRES restart cycle as a hipGraph
                            and replay it; 0: launch kernel by kernel */
+  int solve_energy;     /* SolveEnergy: 1 = UpdateEnergy after the k-epsilon step */
+  int energy_pad;
+  double cp, prl, prte; /* heat capacity, laminar / turbulent Prandtl numbers
+                           (navier.cpp:489-492: 1006, 0.72, 0.9) */
 } s355_params;
 
 typedef struct s355_solve_info {
@@ -127,6 +131,8 @@ enum s355_field {
   S355_F_AU, S355_F_AP, S355_F_AS,
   /* right-hand sides, n = Ncel */
   S355_F_BU_RHS, S355_F_BV_RHS, S355_F_BW_RHS, S355_F_BP_RHS, S355_F_BS_RHS,
+  /* per boundary face, n = Nbnd: boundary temperature BTem */
+  S355_F_BTEM,
   S355_F_COUNT
 };
 
@@ -234,11 +240,16 @@ int s355_residuals(s355_ctx* ctx, double res[10], int reset);
 /* k-epsilon pieces of UpdateTurKEpsilon (navier_scalar.cpp:11-197). */
 int s355_kepsilon_sources(s355_ctx* ctx);                  /* 33-64: mu_k, mu_eps, P_k, S_k, Ap_k */
 int s355_bc_kepsilon(s355_ctx* ctx);                       /* SetBCKEpsilon navier_bc.cpp:220-345 */
-int s355_build_scalar_matrix(s355_ctx* ctx, int isca);     /* BuildScalarMatrix 407-592; 2 = k, 3 = eps */
+int s355_build_scalar_matrix(s355_ctx* ctx, int isca);     /* BuildScalarMatrix 407-592; 1 = T, 2 = k, 3 = eps */
 int s355_eps_wall_rows(s355_ctx* ctx);                     /* 129-154: wall-cell rows -> identity */
 int s355_solve_scalar(s355_ctx* ctx, int isca, double rtol, int maxit, s355_solve_info* info); /* MPIStructure.cpp:378-446 */
 int s355_kepsilon_post_k(s355_ctx* ctx);                   /* 105-121: clip k, eps sources */
 int s355_kepsilon_post_eps(s355_ctx* ctx);                 /* 165-178: clip eps, mu_t */
+
+/* Energy pieces of UpdateEnergy (navier_scalar.cpp:204-326, DensityModel 0,
+ * steady): BuildScalarMatrix(1) and solve_scalar(1) act on T / BTem. */
+int s355_energy_coefficients(s355_ctx* ctx);  /* 215-218: kcond = cp (mu/prl + mu_t/prte) / cp, ESource = ApE = 0 */
+int s355_bc_temperature(s355_ctx* ctx);       /* SetBCTemperature navier_bc.cpp:183-214 */
 
 /* Device-timed log stages, the counterpart of PetscLogStagePush/Pop around
  * the phases (navier.cpp:26-37): 1 BuildVelocity, 2 SolveVelocity,
@@ -255,6 +266,7 @@ int s355_log_stage_times(s355_ctx* ctx, double ms[6], int reset);
 int s355_calculate_velocity(s355_ctx* ctx, s355_solve_info info[3]);
 int s355_calculate_pressure(s355_ctx* ctx, s355_solve_info* info);
 int s355_update_tur_kepsilon(s355_ctx* ctx, s355_solve_info info[2]);
+int s355_update_energy(s355_ctx* ctx, s355_solve_info* info);  /* navier.cpp:84-86 */
 
 /* ---- Batches of independent cases (parameter sweeps) ----------------------
  * No reference counterpart: the reference runs one case per MPI job.  The
@@ -279,12 +291,12 @@ int s355_batch_size(const s355_ctx* ctx, int* n_case);
 int s355_batch_layout(const s355_ctx* ctx, int which, int* out, size_t n);
 /* res: 10 per case, in case order */
 int s355_batch_residuals(s355_ctx* ctx, double* res, int reset);
-/* per-case records of the last solve of a kind: 0..2 U, V, W, 3 p', 4 k, 5 eps */
+/* per-case records of the last solve of a kind: 0..2 U, V, W, 3 p', 4 k, 5 eps, 6 T */
 int s355_batch_solve_info(s355_ctx* ctx, int kind, s355_solve_info* info, int n_case);
 
 /* Solve records of the last SIMPLE phases run through the phase-level
- * entries: info[0..5] = U, V, W, pressure correction, k, epsilon. */
-int s355_last_solve_info(s355_ctx* ctx, s355_solve_info info[6]);
+ * entries: info[0..6] = U, V, W, pressure correction, k, epsilon, T. */
+int s355_last_solve_info(s355_ctx* ctx, s355_solve_info info[7]);
 
 /* NSSolve steady loop body (navier.cpp:52-141) for n_steps steps.  history
  * receives 11 doubles per step: Residual[0..9] and ResMax.  *steps_done is

@@ -248,6 +248,16 @@ int orc_update_tur_kepsilon(void* h, s355_solve_info info[2]) {
   });
 }
 
+int orc_bc_temperature(void* h) { return guard([&] { O(h)->SetBCTemperature(); return S355_OK; }); }
+
+int orc_update_energy(void* h, s355_solve_info* info) {
+  return guard([&] {
+    int rc = O(h)->UpdateEnergy();
+    if (info) fill_info(O(h)->last[6], info);
+    return rc;
+  });
+}
+
 int orc_simple_iterate(void* h, int n, double* history, int* done) {
   return guard([&] { return O(h)->SimpleIterate(n, history, done); });
 }

@@ -84,7 +84,7 @@ class Oracle {
   std::vector<double> RUFace;
   std::vector<double> Au, Ap, As, bu, bv, bw, bp, bs;
   double localRes[10] = {0};
-  SolveResult last[6];  // u, v, w, p, k, eps
+  SolveResult last[7];  // u, v, w, p, k, eps, T
 
   // --- phases (NavierStokesSolver members) ---
   void InitFlowField();
@@ -100,6 +100,8 @@ class Oracle {
   int CalculateVelocity();
   int CalculatePressure();
   int UpdateTurKEpsilon();
+  void SetBCTemperature();
+  int UpdateEnergy();
   int SolveVelocity(double rtol, int maxit);
   int SolvePressure(double rtol, int maxit);
   int SimpleIterate(int n_steps, double* history, int* done);

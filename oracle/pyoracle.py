@@ -52,6 +52,10 @@ def library() -> C.CDLL:
         lib.orc_calculate_pressure.argtypes = [V, C.POINTER(C.c_double), C.POINTER(SolveInfo)]
         lib.orc_update_tur_kepsilon.restype = I
         lib.orc_update_tur_kepsilon.argtypes = [V, C.POINTER(SolveInfo)]
+        lib.orc_update_energy.restype = I
+        lib.orc_update_energy.argtypes = [V, C.POINTER(SolveInfo)]
+        lib.orc_bc_temperature.restype = I
+        lib.orc_bc_temperature.argtypes = [V]
         _lib = lib
     return _lib
 
@@ -150,6 +154,11 @@ class Oracle:
         info = (SolveInfo * 2)()
         rc = self._check(self.lib.orc_update_tur_kepsilon(self.h, info), "update_tur_kepsilon")
         return rc, [i.as_dict() for i in info]
+
+    def update_energy(self):
+        info = SolveInfo()
+        rc = self._check(self.lib.orc_update_energy(self.h, C.byref(info)), "update_energy")
+        return rc, info.as_dict()
 
     def iterate(self, n_steps: int):
         hist = np.zeros(11 * max(n_steps, 1))

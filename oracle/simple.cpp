@@ -93,6 +93,7 @@ std::vector<double>* Oracle::field(int id) {
     case S355_F_BW_RHS: return &bw;
     case S355_F_BP_RHS: return &bp;
     case S355_F_BS_RHS: return &bs;
+    case S355_F_BTEM: return &BTem;
     default: return nullptr;
   }
 }
@@ -760,6 +761,37 @@ int Oracle::UpdateTurKEpsilon() {
   return S355_OK;
 }
 
+// SetBCTemperature (navier_bc.cpp:183-214): walls and inlets keep their
+// prescribed value, outlets and symmetry planes take the cell value.
+void Oracle::SetBCTemperature() {
+  for (int i = 0; i < Nbnd; ++i) {
+    const int ic = face[bnd[i].face].c1;
+    switch (type1_of(bnd[i].rid)) {
+      case 1:
+      case 2:
+        break;
+      case 3:
+      case 4:
+        BTem[i] = T[ic];
+        break;
+      default:
+        throw std::runtime_error("no such boundary type");
+    }
+  }
+}
+
+// UpdateEnergy (navier_scalar.cpp:204-326) for DensityModel 0, steady: the
+// conduction coefficient cp (mu/prl + mu_t/prte) / cp, no source, then the
+// generic scalar assembly and one GMRES solve into T.
+int Oracle::UpdateEnergy() {
+  std::vector<double> kcond(Ncel), esrc(Ncel, 0.), ape(Ncel, 0.);
+  for (int i = 0; i < Ncel; ++i) kcond[i] = par.cp * (VisLam[i] / par.prl + VisTur[i] / par.prte) / par.cp;
+  SetBCTemperature();
+  BuildScalarMatrix(T, BTem, kcond, esrc, ape);
+  last[6] = ksp_solve(csr, As, bs, T, par.ksp[S355_KSP_SCALAR], false);
+  return last[6].reason < 0 ? S355_NOT_CONVERGED : S355_OK;
+}
+
 int Oracle::SimpleIterate(int n_steps, double* history, int* done) {
   *done = 0;
   for (int step = 1; step <= n_steps; ++step) {
@@ -770,6 +802,10 @@ int Oracle::SimpleIterate(int n_steps, double* history, int* done) {
     if (rc != S355_OK) return rc;
     if (par.turb_model == 1) {
       rc = UpdateTurKEpsilon();
+      if (rc != S355_OK) return rc;
+    }
+    if (par.solve_energy) {
+      rc = UpdateEnergy();
       if (rc != S355_OK) return rc;
     }
     double resmax = 0.;

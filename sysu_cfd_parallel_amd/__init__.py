@@ -35,7 +35,12 @@ class S355Error(RuntimeError):
 
 
 class NotConverged(S355Error):
-    pass
+    """A Krylov solve ended without converging (the reference's ConvergeError);
+    `info` holds the solve record(s) when the call returns them."""
+
+    def __init__(self, code: int, msg: str, info=None):
+        super().__init__(code, msg)
+        self.info = info
 
 
 _lib: Optional[C.CDLL] = None
@@ -91,6 +96,9 @@ def library(path: Optional[Path] = None) -> C.CDLL:
         "calculate_velocity": (I, [V, C.POINTER(SolveInfo)]),
         "calculate_pressure": (I, [V, C.POINTER(SolveInfo)]),
         "update_tur_kepsilon": (I, [V, C.POINTER(SolveInfo)]),
+        "update_energy": (I, [V, C.POINTER(SolveInfo)]),
+        "energy_coefficients": (I, [V]),
+        "bc_temperature": (I, [V]),
         "spmv_time": (I, [V, I, I, I, pD]),
         "last_solve_info": (I, [V, C.POINTER(SolveInfo)]),
         "copy_time": (I, [V, C.c_size_t, I, pD]),
@@ -107,12 +115,12 @@ def library(path: Optional[Path] = None) -> C.CDLL:
     return lib
 
 
-def _check(lib: C.CDLL, rc: int, what: str) -> int:
+def _check(lib: C.CDLL, rc: int, what: str, info=None) -> int:
     if rc == 0:
         return rc
     msg = (lib.s355_last_error() or b"").decode(errors="replace")
     if rc == _abi.S355_NOT_CONVERGED:
-        raise NotConverged(rc, f"{what}: {msg}")
+        raise NotConverged(rc, f"{what}: {msg}", info)
     raise S355Error(rc, f"{what}: {msg}")
 
 
@@ -322,12 +330,13 @@ class Solver:
     def solve_velocity(self, rtol: float, maxit: int) -> list:
         info = (SolveInfo * 3)()
         rc = self.lib.s355_solve_velocity(self.ctx, rtol, maxit, info)
-        _check(self.lib, rc, "solve_velocity")
+        _check(self.lib, rc, "solve_velocity", [i.as_dict() for i in info])
         return [i.as_dict() for i in info]
 
     def solve_pressure(self, rtol: float, maxit: int) -> dict:
         info = SolveInfo()
-        _check(self.lib, self.lib.s355_solve_pressure(self.ctx, rtol, maxit, C.byref(info)), "solve_pressure")
+        rc = self.lib.s355_solve_pressure(self.ctx, rtol, maxit, C.byref(info))
+        _check(self.lib, rc, "solve_pressure", info.as_dict())
         return info.as_dict()
 
     def correct_fields(self) -> float:
@@ -349,11 +358,17 @@ class Solver:
         _check(self.lib, rc, f"simple_iterate (after {done.value} steps)")
         return h, done.value
 
+    def update_energy(self) -> dict:
+        """UpdateEnergy (navier_scalar.cpp:204-326): one energy solve into T."""
+        info = SolveInfo()
+        _check(self.lib, self.lib.s355_update_energy(self.ctx, C.byref(info)), "update_energy")
+        return info.as_dict()
+
     def last_solve_info(self) -> dict:
         """Solve records of the last phases: U, V, W, pressure, k, epsilon."""
-        info = (SolveInfo * 6)()
+        info = (SolveInfo * 7)()
         _check(self.lib, self.lib.s355_last_solve_info(self.ctx, info), "last_solve_info")
-        return {n: info[k].as_dict() for k, n in enumerate(("U", "V", "W", "P", "TE", "ED"))}
+        return {n: info[k].as_dict() for k, n in enumerate(("U", "V", "W", "P", "TE", "ED", "T"))}
 
     def stage_times(self, reset: bool = False) -> np.ndarray:
         out = np.zeros(6)
@@ -410,7 +425,7 @@ class Solver:
 
 
 BATCH_ARRAYS = {"cell": 0, "bnd": 1, "iface": 2, "nnz": 3, "vert": 4, "elem": 5}
-SOLVE_KINDS = ("U", "V", "W", "P", "TE", "ED")
+SOLVE_KINDS = ("U", "V", "W", "P", "TE", "ED", "T")
 
 
 class BatchSolver(Solver):

@@ -72,6 +72,11 @@ class Params(C.Structure):
         ("ir_inner_rtol", C.c_double),
         ("autotune", C.c_int),
         ("graphs", C.c_int),
+        ("solve_energy", C.c_int),
+        ("energy_pad", C.c_int),
+        ("cp", C.c_double),
+        ("prl", C.c_double),
+        ("prte", C.c_double),
     ]
 
 
@@ -104,7 +109,7 @@ FIELDS = [
     "DUDX", "DVDX", "DWDX", "DPDX", "DPHIDX",
     "BRO", "BU", "BV", "BW", "BPRE", "BTE", "BED",
     "SHEAR", "RUFACE", "AU", "AP", "AS",
-    "BU_RHS", "BV_RHS", "BW_RHS", "BP_RHS", "BS_RHS",
+    "BU_RHS", "BV_RHS", "BW_RHS", "BP_RHS", "BS_RHS", "BTEM",
 ]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 # which size each field has: c = cells, c3 = 3*cells, b = boundary, b3, f = faces, z = nnz
@@ -112,7 +117,7 @@ FIELD_KIND = {
     **{n: "c" for n in ("R", "U", "V", "W", "P", "T", "TE", "ED", "DELTAP", "VISLAM", "VISTUR", "APR",
                         "BU_RHS", "BV_RHS", "BW_RHS", "BP_RHS", "BS_RHS")},
     **{n: "c3" for n in ("DUDX", "DVDX", "DWDX", "DPDX", "DPHIDX")},
-    **{n: "b" for n in ("BRO", "BU", "BV", "BW", "BPRE", "BTE", "BED")},
+    **{n: "b" for n in ("BRO", "BU", "BV", "BW", "BPRE", "BTE", "BED", "BTEM")},
     "SHEAR": "b3", "RUFACE": "f", "AU": "z", "AP": "z", "AS": "z",
 }
 

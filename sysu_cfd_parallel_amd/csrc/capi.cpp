@@ -140,6 +140,15 @@ void check_params(const s355_params* p) {
     if (o.maxit < 1 || !(o.rtol >= 0.0) || !(o.dtol > 0.0)) throw std::invalid_argument("bad KSP tolerances");
   }
   if (!(p->urf[0] > 0.0) || !(p->urf[5] > 0.0)) throw std::invalid_argument("relaxation factors must be positive");
+  if (p->solve_energy && !(p->cp > 0.0 && p->prl > 0.0 && p->prte > 0.0))
+    throw std::invalid_argument("energy: cp, prl and prte must be positive");
+}
+
+// the message of a solve that ended without converging (its rc is not an
+// exception, so guarded() does not set one)
+void note_not_converged(const char* var, size_t len, const s355_solve_info& i) {
+  g_err = std::string(var, strnlen(var, len)) + " did not converge: reason " + std::to_string(i.reason) + " after " +
+          std::to_string(i.iters) + " iterations, residual " + std::to_string(i.rnorm);
 }
 
 }  // namespace
@@ -168,6 +177,10 @@ void s355_default_params(s355_params* p) {
   p->ir_inner_rtol = 1.e-3;
   p->autotune = 0;
   p->graphs = 0;
+  p->solve_energy = 0;  // navier.cpp:481
+  p->cp = 1006.;        // navier.cpp:489-492
+  p->prl = 0.72;
+  p->prte = 0.9;
 }
 
 int s355_case_build(int kind, const int* ip, const double* dp, s355_case** out) {
@@ -371,7 +384,7 @@ int s355_batch_solve_info(s355_ctx* ctx, int kind, s355_solve_info* info, int n_
   return guarded([&] {
     s355::Context& d = dev(ctx);
     if (!info || n_case != d.ncase) throw std::invalid_argument("batch solve info: n_case entries");
-    if (kind < 0 || kind > 5) throw std::invalid_argument("batch solve info: kind 0..5 (U, V, W, p', k, eps)");
+    if (kind < 0 || kind > 6) throw std::invalid_argument("batch solve info: kind 0..6 (U, V, W, p', k, eps, T)");
     const auto& v = d.case_info[kind];
     for (int k = 0; k < n_case; ++k) {
       s355_solve_info x{};
@@ -472,7 +485,10 @@ int s355_build_velocity_matrix(s355_ctx* ctx) {
 int s355_solve_velocity(s355_ctx* ctx, double rtol, int maxit, s355_solve_info info[3]) {
   return guarded([&] {
     if (!info) throw std::invalid_argument("null info");
-    return dev(ctx).solve_velocity(rtol, maxit, info);
+    const int rc = dev(ctx).solve_velocity(rtol, maxit, info);
+    for (int k = 0; k < 3 && rc == S355_NOT_CONVERGED; ++k)
+      if (info[k].reason < 0) note_not_converged("UVW" + k, 1, info[k]);
+    return rc;
   });
 }
 
@@ -484,7 +500,9 @@ int s355_build_pressure_matrix(s355_ctx* ctx) {
 int s355_solve_pressure(s355_ctx* ctx, double rtol, int maxit, s355_solve_info* info) {
   return guarded([&] {
     if (!info) throw std::invalid_argument("null info");
-    return dev(ctx).solve_pressure(rtol, maxit, info);
+    const int rc = dev(ctx).solve_pressure(rtol, maxit, info);
+    if (rc == S355_NOT_CONVERGED) note_not_converged("deltaP", 6, *info);
+    return rc;
   });
 }
 
@@ -530,7 +548,9 @@ int s355_eps_wall_rows(s355_ctx* ctx) { return guarded([&] { dev(ctx).eps_wall_r
 int s355_solve_scalar(s355_ctx* ctx, int isca, double rtol, int maxit, s355_solve_info* info) {
   return guarded([&] {
     if (!info) throw std::invalid_argument("null info");
-    return dev(ctx).solve_scalar(isca, rtol, maxit, info);
+    const int rc = dev(ctx).solve_scalar(isca, rtol, maxit, info);
+    if (rc == S355_NOT_CONVERGED) note_not_converged(isca == 1 ? "Tn" : (isca == 2 ? "TE" : "ED"), 2, *info);
+    return rc;
   });
 }
 int s355_kepsilon_post_k(s355_ctx* ctx) { return guarded([&] { dev(ctx).kepsilon_post_k(); return S355_OK; }); }
@@ -577,11 +597,34 @@ int s355_update_tur_kepsilon(s355_ctx* ctx, s355_solve_info info[2]) {
   });
 }
 
-int s355_last_solve_info(s355_ctx* ctx, s355_solve_info info[6]) {
+int s355_energy_coefficients(s355_ctx* ctx) {
+  return guarded([&] {
+    dev(ctx).energy_coefficients();
+    return S355_OK;
+  });
+}
+
+int s355_bc_temperature(s355_ctx* ctx) {
+  return guarded([&] {
+    dev(ctx).bc_temperature();
+    return S355_OK;
+  });
+}
+
+int s355_update_energy(s355_ctx* ctx, s355_solve_info* info) {
+  return guarded([&] {
+    dev(ctx);
+    ctx->ns->UpdateEnergy();
+    if (info) *info = ctx->ns->last[6];
+    return S355_OK;
+  });
+}
+
+int s355_last_solve_info(s355_ctx* ctx, s355_solve_info info[7]) {
   return guarded([&] {
     dev(ctx);
     if (!info) throw std::invalid_argument("null info");
-    for (int k = 0; k < 6; ++k) info[k] = ctx->ns->last[k];
+    for (int k = 0; k < 7; ++k) info[k] = ctx->ns->last[k];
     return S355_OK;
   });
 }

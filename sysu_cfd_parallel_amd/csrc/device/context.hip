@@ -187,7 +187,7 @@ void Context::allocate_fields() {
   for (double** p : cell1) *p = alloc<double>(Nc);
   double** cell3[] = {&df.dUdX, &df.dVdX, &df.dWdX, &df.dPdX, &df.dPhidX};
   for (double** p : cell3) *p = alloc<double>(3 * Nc);
-  double** bnd1[] = {&df.BRo, &df.BU, &df.BV, &df.BW, &df.BPre, &df.BTE, &df.BED};
+  double** bnd1[] = {&df.BRo, &df.BU, &df.BV, &df.BW, &df.BPre, &df.BTE, &df.BED, &df.BTem};
   for (double** p : bnd1) *p = alloc<double>(Nb);
   df.shear = alloc<double>(3 * Nb);
   df.RUFace = alloc<double>(Nf);
@@ -204,13 +204,15 @@ void Context::allocate_fields() {
   kw_.visted = alloc<double>(Nc);
   kw_.apte = alloc<double>(Nc);
   kw_.aped = alloc<double>(Nc);
+  kw_.kcond = alloc<double>(Nc);
+  kw_.zero = alloc<double>(Nc);
   // zero everything once so that get_field before any phase is defined
   double* all_cell[] = {df.R, df.U, df.V, df.W, df.P, df.T, df.TE, df.ED, df.dp, df.mu, df.mut, df.apr,
                         df.bu, df.bv, df.bw, df.bp, df.bs};
   for (double* p : all_cell) S355_HIP(hipMemsetAsync(p, 0, Nc * sizeof(double), stream));
   for (double* p : {df.dUdX, df.dVdX, df.dWdX, df.dPdX, df.dPhidX})
     S355_HIP(hipMemsetAsync(p, 0, 3 * Nc * sizeof(double), stream));
-  for (double* p : {df.BRo, df.BU, df.BV, df.BW, df.BPre, df.BTE, df.BED})
+  for (double* p : {df.BRo, df.BU, df.BV, df.BW, df.BPre, df.BTE, df.BED, df.BTem})
     S355_HIP(hipMemsetAsync(p, 0, std::max<size_t>(Nb, 1) * sizeof(double), stream));
   S355_HIP(hipMemsetAsync(df.shear, 0, std::max<size_t>(3 * Nb, 1) * sizeof(double), stream));
   S355_HIP(hipMemsetAsync(df.RUFace, 0, Nf * sizeof(double), stream));
@@ -302,6 +304,7 @@ FieldRef Context::field(int id) {
     case S355_F_BW_RHS: return {df.bw, Nc};
     case S355_F_BP_RHS: return {df.bp, Nc};
     case S355_F_BS_RHS: return {df.bs, Nc};
+    case S355_F_BTEM: return {df.BTem, Nb};
     default: throw std::invalid_argument("unknown field id " + std::to_string(id));
   }
 }
@@ -342,14 +345,17 @@ void Context::init_flow_field() {
     if (!r) throw std::invalid_argument("cell region missing");
     apply(i, *r);
   }
-  std::vector<double> BU(Nb, 0.0), BV(Nb, 0.0), BW(Nb, 0.0), BRo(Nb, 0.0), BTE(Nb, 0.0), BED(Nb, 0.0);
+  std::vector<double> BU(Nb, 0.0), BV(Nb, 0.0), BW(Nb, 0.0), BRo(Nb, 0.0), BTE(Nb, 0.0), BED(Nb, 0.0),
+      BTem(Nb, 0.0);
   for (int b = 0; b < Nb; ++b) {
     const s355_region* r = find(mesh.b_rid[b]);
+    if (r->type1 == kWall && r->type2 == 0) BTem[b] = r->fixed_value;  // isothermal wall
     if (r->type1 == kInlet) {
       BU[b] = r->init[0];
       BV[b] = r->init[1];
       BW[b] = r->init[2];
       BRo[b] = r->init[4];
+      BTem[b] = r->init[5];
       BTE[b] = r->init[6];
       BED[b] = r->init[7];
     }
@@ -374,6 +380,7 @@ void Context::init_flow_field() {
   put(df.BRo, BRo);
   put(df.BTE, BTE);
   put(df.BED, BED);
+  put(df.BTem, BTem);
   S355_HIP(hipMemsetAsync(df.RUFace, 0, dm.Nf * sizeof(double), stream));
   sync();  // host staging vectors go out of scope
 }
@@ -547,10 +554,20 @@ void Context::bc_kepsilon() {
   launch_keps_copy_bc(dm, df, stream);
 }
 
+void Context::bc_temperature() { launch_bc_temperature(dm, df, flags_, stream); }
+
+void Context::energy_coefficients() { launch_energy_coef(dm, df, kw_, par.cp, par.prl, par.prte, stream); }
+
 void Context::build_scalar_matrix(int isca) {
   double *phi, *bphi;
   const double *diff, *src, *app;
-  if (isca == 2) {
+  if (isca == 1) {
+    phi = df.T;
+    bphi = df.BTem;
+    diff = kw_.kcond;
+    src = kw_.zero;
+    app = kw_.zero;
+  } else if (isca == 2) {
     phi = df.TE;
     bphi = df.BTE;
     diff = kw_.viste;
@@ -563,7 +580,7 @@ void Context::build_scalar_matrix(int isca) {
     src = kw_.eds;
     app = kw_.aped;
   } else {
-    throw std::invalid_argument("build_scalar_matrix: isca 2 (k) or 3 (epsilon)");
+    throw std::invalid_argument("build_scalar_matrix: isca 1 (T), 2 (k) or 3 (epsilon)");
   }
   const double* p1[1] = {phi};
   const double* b1[1] = {bphi};
@@ -575,11 +592,12 @@ void Context::build_scalar_matrix(int isca) {
 void Context::eps_wall_rows() { launch_dirichlet_rows(dm, df, stream); }
 
 int Context::solve_scalar(int isca, double rtol, int maxit, s355_solve_info* info) {
-  if (isca != 2 && isca != 3) throw std::invalid_argument("solve_scalar: isca 2 or 3");
+  if (isca < 1 || isca > 3) throw std::invalid_argument("solve_scalar: isca 1 (T), 2 (k) or 3 (epsilon)");
   s355_ksp_opts o = par.ksp[S355_KSP_SCALAR];
   o.rtol = rtol;
   o.maxit = maxit;
-  return run_solve(2, df.bs, isca == 2 ? df.TE : df.ED, o, false, info, isca == 2 ? 4 : 5);
+  double* x = isca == 1 ? df.T : (isca == 2 ? df.TE : df.ED);
+  return run_solve(2, df.bs, x, o, false, info, isca == 1 ? 6 : (isca == 2 ? 4 : 5));
 }
 
 void Context::kepsilon_post_k() {

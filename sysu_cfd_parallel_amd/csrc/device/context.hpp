@@ -61,13 +61,15 @@ class Context {
   int solve_scalar(int isca, double rtol, int maxit, s355_solve_info* info);
   void kepsilon_post_k();
   void kepsilon_post_eps();
+  void bc_temperature();
+  void energy_coefficients();
 
   // residual slots and error flags on the device
   void zero_residuals();
   void read_residuals(double out[10]);          // element-wise maximum over the cases
   void read_case_residuals(double* out);        // 10 per case
-  // per-case records of the last solve of each kind: U, V, W, p', k, epsilon
-  std::vector<KResult> case_info[6];
+  // per-case records of the last solve of each kind: U, V, W, p', k, epsilon, T
+  std::vector<KResult> case_info[7];
   void check_flags();  // throws on a flag raised by a kernel
   void sync();
 

@@ -27,6 +27,7 @@ void launch_reduce_tiles(const double* partial, const Tiles& t, int ncomp, doubl
 // k-epsilon (keps_kernels.hip)
 struct KepsWork {
   double *prod, *tes, *eds, *viste, *visted, *apte, *aped;
+  double *kcond, *zero;  // energy: conduction coefficient, a zero source / diagonal
 };
 void launch_keps_sources(const DMesh& m, const DFields& F, const KepsWork& w, hipStream_t s);
 void launch_keps_wall(const DMesh& m, const DFields& F, const KepsWork& w, double cmu25, double cmu75,
@@ -38,5 +39,10 @@ void launch_clip(int n, double* v, double floor, hipStream_t s);
 void launch_eps_sources(const DMesh& m, const DFields& F, const KepsWork& w, hipStream_t s);
 void launch_dirichlet_rows(const DMesh& m, const DFields& F, hipStream_t s);
 void launch_turb_viscosity(const DMesh& m, const DFields& F, hipStream_t s);
+
+// energy (keps_kernels.hip)
+void launch_bc_temperature(const DMesh& m, const DFields& F, int* bad, hipStream_t s);
+void launch_energy_coef(const DMesh& m, const DFields& F, const KepsWork& w, double cp, double prl, double prte,
+                        hipStream_t s);
 
 }  // namespace s355

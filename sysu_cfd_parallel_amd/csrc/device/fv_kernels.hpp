@@ -50,7 +50,7 @@ struct DMesh {
 struct DFields {
   double *R, *U, *V, *W, *P, *T, *TE, *ED, *dp, *mu, *mut, *apr;
   double *dUdX, *dVdX, *dWdX, *dPdX, *dPhidX;
-  double *BRo, *BU, *BV, *BW, *BPre, *BTE, *BED, *shear;
+  double *BRo, *BU, *BV, *BW, *BPre, *BTE, *BED, *BTem, *shear;
   double* RUFace;
   double *Au, *Ap, *As, *bu, *bv, *bw, *bp, *bs;
 };

@@ -104,6 +104,41 @@ void launch_keps_copy_bc(const DMesh& m, const DFields& F, hipStream_t s) {
   S355_LAUNCH_CHECK();
 }
 
+// SetBCTemperature (navier_bc.cpp:183-214): walls and inlets keep their
+// prescribed value, outlets and symmetry planes take the cell value.
+__global__ void __launch_bounds__(kBlock) k_bc_temperature(DMesh m, DFields F, int* bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.Nb) return;
+  const int t = m.b_type[i];
+  if (t == kOutlet || t == kSym)
+    F.BTem[i] = F.T[m.f_c1[i]];
+  else if (t != kWall && t != kInlet)
+    bad[0] = 1;
+}
+
+void launch_bc_temperature(const DMesh& m, const DFields& F, int* bad, hipStream_t s) {
+  if (m.Nb == 0) return;
+  hipLaunchKernelGGL(k_bc_temperature, dim3(grid_for(m.Nb)), dim3(kBlock), 0, s, m, F, bad);
+  S355_LAUNCH_CHECK();
+}
+
+// UpdateEnergy coefficients (navier_scalar.cpp:213-218, DensityModel 0):
+// kcond = cp (mu/prl + mu_t/prte) / cp, no source, no extra diagonal.
+__global__ void __launch_bounds__(kBlock) k_energy_coef(int n, const double* mu, const double* mut, double cp,
+                                                        double prl, double prte, double* kcond, double* zero) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  kcond[i] = cp * (mu[i] / prl + mut[i] / prte) / cp;
+  zero[i] = 0.;
+}
+
+void launch_energy_coef(const DMesh& m, const DFields& F, const KepsWork& w, double cp, double prl, double prte,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_energy_coef, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m.Nc, F.mu, F.mut, cp, prl, prte,
+                     w.kcond, w.zero);
+  S355_LAUNCH_CHECK();
+}
+
 // BuildScalarMatrix cell loop (navier_scalar.cpp:421-589).  The gradient of
 // phi is in F.dPhidX.  bad[0] flags unknown boundary types, bad[1] an inlet
 // with outflow (the reference exits there, navier_scalar.cpp:476-479).

@@ -2,6 +2,7 @@
 //   CalculateVelocity   navier_velocity.cpp:10-60
 //   CalculatePressure   navier_pressure.cpp:9-97
 //   UpdateTurKEpsilon   navier_scalar.cpp:11-197
+//   UpdateEnergy        navier_scalar.cpp:204-326
 //   NSSolve             navier.cpp:52-141 (steady: one SIMPLE pass per step)
 // Log stages follow navier.cpp:26-37 (1 BuildVelocity, 2 SolveVelocity,
 // 3 BuildPressure, 4 SolvePressure, 5 other).  A failed solve throws
@@ -40,7 +41,7 @@ void DataPartition::solvePressureCorrection(double tol, int maxIter, s355_solve_
 
 void DataPartition::solveScarlar_GMRES(double tol, int maxIter, int isca, s355_solve_info* info) {
   int rc = s355_solve_scalar(ctx_, isca, tol, maxIter, info);
-  check_solve(rc, *info, isca == 2 ? "TE" : "ED");
+  check_solve(rc, *info, isca == 1 ? "Tn" : (isca == 2 ? "TE" : "ED"));
 }
 
 int NavierStokesSolver::Gradient(int phi, int bphi, int phigd) {
@@ -126,6 +127,24 @@ void NavierStokesSolver::UpdateTurKEpsilon() {
   check(s355_log_stage_pop(ctx_), "log");
 }
 
+void NavierStokesSolver::SetBCTemperature() { check(s355_bc_temperature(ctx_), "SetBCTemperature"); }
+
+// UpdateEnergy (navier_scalar.cpp:204-326) for DensityModel 0, steady runs.
+void NavierStokesSolver::UpdateEnergy() {
+  check(s355_log_stage_push(ctx_, 5), "log");
+  const s355_ksp_opts& o = par_.ksp[S355_KSP_SCALAR];
+  try {
+    check(s355_energy_coefficients(ctx_), "energy coefficients");
+    SetBCTemperature();
+    BuildScalarMatrix(1);
+    dataPartition.solveScarlar_GMRES(o.rtol, o.maxit, 1, &last[6]);
+  } catch (...) {
+    s355_log_stage_pop(ctx_);
+    throw;
+  }
+  check(s355_log_stage_pop(ctx_), "log");
+}
+
 int NavierStokesSolver::NSSolve(int max_step, double* history, int* steps_done) {
   *steps_done = 0;
   for (int step = 1; step <= max_step; ++step) {
@@ -133,6 +152,7 @@ int NavierStokesSolver::NSSolve(int max_step, double* history, int* steps_done) 
     CalculateVelocity();
     CalculatePressure();
     if (par_.turb_model == 1) UpdateTurKEpsilon();
+    if (par_.solve_energy) UpdateEnergy();
     check(s355_residuals(ctx_, Residual, 0), "residuals");
     double resmax = 0.0;  // vec_max starts at 0 (tools.cpp:45-51)
     for (double r : Residual) resmax = std::max(resmax, r);

@@ -31,7 +31,7 @@ class DataPartition {
   void solveVelocity_GMRES(double tol, int maxIter, s355_solve_info info[3]);
   // solvePressureCorrection (MPIStructure.cpp:449-531): zero initial guess.
   void solvePressureCorrection(double tol, int maxIter, s355_solve_info* info);
-  // solveScarlar_GMRES (MPIStructure.cpp:378-446): isca 2 = k, 3 = eps.
+  // solveScarlar_GMRES (MPIStructure.cpp:378-446): isca 1 = T, 2 = k, 3 = eps.
   void solveScarlar_GMRES(double tol, int maxIter, int isca, s355_solve_info* info);
 
  private:
@@ -48,6 +48,7 @@ class NavierStokesSolver {
   int CalculateVelocity();
   int CalculatePressure();
   void UpdateTurKEpsilon();
+  void UpdateEnergy();
 
   // members they are made of
   void BuildVelocityMatrix();
@@ -60,12 +61,13 @@ class NavierStokesSolver {
   void SetBCPressure();
   void SetBCDeltaP();
   void SetBCKEpsilon();
+  void SetBCTemperature();
 
   // NSSolve steady loop; history gets 11 doubles per step.
   int NSSolve(int max_step, double* history, int* steps_done);
 
   double Residual[10] = {0};
-  s355_solve_info last[6] = {};  // u, v, w, p, k, eps
+  s355_solve_info last[7] = {};  // u, v, w, p, k, eps, T
 
  private:
   void check(int rc, const char* what);

@@ -77,3 +77,38 @@ def rel_err(a, b):
     b = np.asarray(b)
     den = max(np.max(np.abs(b)), 1e-300)
     return float(np.max(np.abs(a - b)) / den)
+
+
+def conduction_column(n=6, t_cold=300.0, t_hot=400.0, t0=350.0, mu=0.01):
+    """n unit-width hexes stacked in z between an isothermal cold wall (z = 0)
+    and an isothermal hot wall (z = 1); symmetry planes on the four sides."""
+    h = 1.0 / n
+    xyz = []
+    for j in range(n + 1):
+        z = j * h
+        xyz += [[0, 0, z], [1, 0, z], [1, 1, z], [0, 1, z]]
+    elems = [(3, 1, [0, 1, 2, 3]), (3, 6, [4 * n, 4 * n + 1, 4 * n + 2, 4 * n + 3])]
+    for j in range(n):
+        a, b = 4 * j, 4 * (j + 1)
+        for q in ([a, a + 1, b + 1, b], [a + 1, a + 2, b + 2, b + 1], [a + 2, a + 3, b + 3, b + 2],
+                  [a + 3, a, b, b + 3]):
+            elems.append((3, 4, q))
+    for j in range(n):
+        a, b = 4 * j, 4 * (j + 1)
+        elems.append((5, 100, [a, a + 1, a + 2, a + 3, b, b + 1, b + 2, b + 3]))
+    cold, hot = region(1, 1), region(6, 1)
+    cold.fixed_value, hot.fixed_value = t_cold, t_hot
+    regs = [cold, hot, region(4, 4), region(100, 5, [0, 0, 0, 1, t0, mu, 0, 0])]
+    return manual_case(xyz, elems, regs)
+
+
+def heated(case, t_wall=350.0, t_in=300.0, t0=300.0):
+    """Isothermal walls at t_wall, inlet fluid at t_in, initial fluid at t0."""
+    for r in case.regions:
+        if r.type1 == 1:
+            r.type2, r.fixed_value = 0, t_wall
+        elif r.type1 == 2:
+            r.init[5] = t_in
+        elif r.type1 == 5:
+            r.init[4] = t0
+    return case

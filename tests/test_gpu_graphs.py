@@ -75,8 +75,11 @@ def test_iteration_cap_inside_a_cycle(graphs):
     d.call("cal_ru_face")
     d.call("build_pressure_matrix")
     assert np.abs(d.get("BP_RHS")).max() > 0
-    info = d.solve_pressure(1e-14, 7)
+    with pytest.raises(s.NotConverged) as err:
+        d.solve_pressure(1e-14, 7)
+    info = err.value.info
     assert info["reason"] == -3 and info["iters"] == 7
+    assert "did not converge" in str(err.value)
 
 
 def test_autotuned_shapes_are_shared_within_a_process():

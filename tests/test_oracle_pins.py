@@ -382,3 +382,59 @@ def test_cavity_converges_and_is_physical():
     assert hist[-1, 10] < 1e-6 <= hist[-2, 10]
     U = o.get("U")
     assert -0.26 < U.min() < -0.15
+
+
+def test_energy_conduction_column_is_linear():
+    # steady conduction between isothermal walls at 300 and 400 with
+    # symmetry sides: the discrete solution on a uniform column is the
+    # linear profile at the cell centres, and the relaxed updates
+    # (URF[5]) converge to it
+    from helpers import conduction_column
+
+    n = 6
+    case = conduction_column(n)
+    p = params(rtol=1e-13)
+    p.solve_energy = 1
+    o = Oracle(case, p)
+    z = o.dbls("CELL_X").reshape(-1, 3)[:, 2]
+    btem = o.get("BTEM")
+    assert sorted(set(np.round(btem, 9)) - {0.0}) == [300.0, 400.0]
+    for _ in range(60):
+        rc, info = o.update_energy()
+        assert rc == 0 and info["reason"] > 0
+    t = o.get("T")
+    assert np.max(np.abs(t - (300.0 + 100.0 * z))) < 1e-8
+    # symmetry planes carry the cell value, walls keep theirs
+    o.call("bc_temperature")
+    bt = o.get("BTEM")
+    c1 = o.ints("FACE_C1")[: o.sizes.n_bnd]
+    brid = o.ints("BND_RID")
+    assert np.array_equal(bt[brid == 4], t[c1[brid == 4]])
+    assert np.all(bt[brid == 1] == 300.0) and np.all(bt[brid == 6] == 400.0)
+
+
+def test_energy_conduction_coefficient_and_no_source():
+    # kcond = cp (mu / prl + mu_t / prte) / cp enters the diffusion
+    # coefficients; with a uniform T the assembled system is solved by that T
+    from helpers import conduction_column
+
+    case = conduction_column(4, t_cold=350.0, t_hot=350.0, t0=350.0)
+    p = params(rtol=1e-13)
+    p.solve_energy = 1
+    o = Oracle(case, p)
+    rc, info = o.update_energy()
+    assert rc == 0
+    assert np.max(np.abs(o.get("T") - 350.0)) < 1e-10
+    # off-diagonal of an interior face (unit area, centres h apart):
+    # -kcond * rlencos = -(mu / 0.72) / h
+    a = o.get("AS")
+    rp, col = o.ints("CSR_ROWPTR"), o.ints("CSR_COL")
+    h = 0.25
+    off = [a[k] for i in range(o.sizes.n_cell) for k in range(rp[i], rp[i + 1]) if col[k] != i]
+    assert len(off) == 6 and np.allclose(off, -(0.01 / 0.72) / h, rtol=1e-12)
+    p.prl = 0.5
+    o2 = Oracle(case, p)
+    o2.update_energy()
+    a2 = o2.get("AS")
+    off2 = [a2[k] for i in range(o2.sizes.n_cell) for k in range(rp[i], rp[i + 1]) if col[k] != i]
+    assert np.allclose(off2, -(0.01 / 0.5) / h, rtol=1e-12)
