@@ -323,6 +323,14 @@ int s355_copy_time(s355_ctx* ctx, size_t bytes, int repeats, double* ms_per_call
  * autotuner.  No reference counterpart. */
 int s355_kernel_shapes(s355_ctx* ctx, int lanes[2], int nt[2], int rows[2], int tuned[2]);
 
+/* Times the production SpMV launches inside the GMRES solves of one
+ * precision (0 FP64, 1 FP32): every 4th Arnoldi SpMV is bracketed by HIP
+ * events on the context stream, without added synchronisation.  mode 1
+ * starts (and resets), 0 reads, -1 reads and stops; *avg_ms is the mean
+ * launch time, *count the launches timed.  Solves replayed from captured
+ * graphs are not timed.  No reference counterpart. */
+int s355_spmv_profile(s355_ctx* ctx, int precision, int mode, double* avg_ms, int* count);
+
 /* Sets the kernel shapes of one precision's GMRES solves explicitly (the
  * autotuner then leaves them alone): lanes 1, 2, 4, 8, 16 (CSR SpMV) or 0
  * (SELL-64 SpMV), nt 0/1, rows 1, 2, 4, 8, 16 or 0 (by size).  No reference

@@ -105,6 +105,7 @@ def library(path: Optional[Path] = None) -> C.CDLL:
         "kernel_shapes": (I, [V, pI, pI, pI, pI]),
         "tune_report": (I, [V, I, pD, pD, pI]),
         "set_kernel_shapes": (I, [V, I, I, I, I]),
+        "spmv_profile": (I, [V, I, I, pD, pI]),
     }
     for name, (res, args) in more.items():
         fn = getattr(lib, f"s355_{name}")
@@ -395,6 +396,12 @@ class Solver:
         names = ["L1", "L2", "L4", "L8", "L16", "SELL64"]
         spmv = {f"{names[v // 2]}{'_nt' if v % 2 else ''}": sp[v] for v in range(12) if sp[v] >= 0}
         return {"spmv_ms": spmv, "gs_ms": {f"R{1 << r}": gs[r] for r in range(5) if gs[r] >= 0}, "gs_it": it.value}
+
+    def spmv_profile(self, precision: int, mode: int) -> tuple:
+        """In-solve SpMV timing: mode 1 start, 0 read, -1 stop; (mean ms, launches)."""
+        ms, n = C.c_double(), C.c_int()
+        _check(self.lib, self.lib.s355_spmv_profile(self.ctx, precision, mode, C.byref(ms), C.byref(n)), "spmv_profile")
+        return ms.value, n.value
 
     def set_kernel_shapes(self, precision: int, lanes: int, nt: int = 0, rows: int = 0) -> None:
         """Force the GMRES kernel shapes of one precision (lanes 0 = SELL-64)."""

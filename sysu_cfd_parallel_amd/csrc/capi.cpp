@@ -657,6 +657,14 @@ int s355_kernel_shapes(s355_ctx* ctx, int lanes[2], int nt[2], int rows[2], int 
   });
 }
 
+int s355_spmv_profile(s355_ctx* ctx, int precision, int mode, double* avg_ms, int* count) {
+  return guarded([&] {
+    if (!avg_ms || !count) throw std::invalid_argument("null output");
+    *avg_ms = dev(ctx).spmv_profile(precision, mode, count);
+    return S355_OK;
+  });
+}
+
 int s355_set_kernel_shapes(s355_ctx* ctx, int precision, int lanes, int nt, int rows) {
   return guarded([&] {
     dev(ctx).set_kernel_shapes(precision, lanes, nt, rows);

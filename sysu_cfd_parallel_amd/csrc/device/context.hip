@@ -113,6 +113,9 @@ Tiles Context::make_tiles(const std::vector<int>& case_row, int tile) {
 void Context::free_all() {
   if (stream) hipStreamSynchronize(stream);
   graphs_.reset();
+  kwork_.timer = kwork32_.timer = nullptr;
+  timers_[0].reset();
+  timers_[1].reset();
   for (hipEvent_t e : all_events_) hipEventDestroy(e);
   all_events_.clear();
   pool_.clear();
@@ -703,6 +706,27 @@ double Context::time_spmv(int which, int precision, int repeats) {
   sync();
   hipFree(flush);
   return ms;
+}
+
+double Context::spmv_profile(int precision, int mode, int* count) {
+  if (precision < 0 || precision > 1 || mode < -1 || mode > 1) throw std::invalid_argument("spmv_profile");
+  KWork& w = precision ? kwork32_ : kwork_;
+  std::unique_ptr<SpmvTimer>& t = timers_[precision];
+  if (mode == 1) {
+    t = std::make_unique<SpmvTimer>();
+    w.timer = t.get();
+  }
+  double avg = 0.0;
+  int n = 0;
+  if (t) {
+    sync();
+    t->harvest();
+    n = static_cast<int>(t->count);
+    avg = t->count ? t->total_ms / t->count : 0.0;
+  }
+  if (mode == -1) w.timer = nullptr;
+  if (count) *count = n;
+  return avg;
 }
 
 void Context::set_kernel_shapes(int precision, int lanes, int nt, int rows) {

@@ -83,6 +83,9 @@ class Context {
   double time_copy(size_t bytes, int repeats);
   void kernel_shapes(int lanes[2], int nt[2], int rows[2], int tuned[2]);
   void set_kernel_shapes(int precision, int lanes, int nt, int rows);
+  // mode 1: start timing the Arnoldi SpMV launches of one precision (reset),
+  // 0: read, -1: read and stop; returns the mean ms and the launches timed
+  double spmv_profile(int precision, int mode, int* count);
   const TuneReport& tune_report(int precision) const { return precision ? kwork32_.report : kwork_.report; }
 
  private:
@@ -116,6 +119,7 @@ class Context {
   IrWork ir_{};
   // ILU(0): level schedule of the pattern and factor storage
   std::unique_ptr<GraphCache> graphs_;  // captured GMRES cycles (params.graphs)
+  std::unique_ptr<SpmvTimer> timers_[2];  // in-solve SpMV timing, FP64 / FP32
   // SELL-64 copy of the matrix being solved: pattern and source map built once,
   // values filled before each GMRES solve that uses it
   const int* sell_src_ = nullptr;

@@ -1091,13 +1091,17 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
                        zero_guess ? 1 : 0, mixed ? 1 : 0);
     S355_LAUNCH_CHECK();
   };
-  auto enqueue_iteration = [&](int it) {
+  auto enqueue_iteration = [&](int it, bool timed) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const bool rec = timed && w.timer && (w.timer->seen++ % w.timer->every) == 0 && w.timer->take(e0, e1);
+    if (rec) S355_HIP(hipEventRecord(e0, s));
     if (lu) {
       launch_spmv<T>(L, w.nt, A, sell, tl, V + it * ld, static_cast<const T*>(nullptr), tmp, w.st, it, s);
-      ilu_apply(A, *w.ilu, lu, tmp, V + (it + 1) * ld, ilu_st, s);
     } else {
       launch_spmv<T>(L, w.nt, A, sell, tl, V + it * ld, inv, V + (it + 1) * ld, w.st, it, s);
     }
+    if (rec) S355_HIP(hipEventRecord(e1, s));
+    if (lu) ilu_apply(A, *w.ilu, lu, tmp, V + (it + 1) * ld, ilu_st, s);
     Dispatch<T>::mdot(R, V, ld, tl, it, w.partial, pstride, w.st, s);
     hipLaunchKernelGGL(k_mdot_final, dim3(it + 1, nc), dim3(kBlock), 0, s, w.partial, tl, pstride, w.st);
     Dispatch<T>::maxpy(R, V, ld, tl, it, w.partial, w.st, s);
@@ -1126,7 +1130,7 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
                                           static_cast<uintptr_t>(first_c), static_cast<uintptr_t>(zero_guess)};
       hipGraphExec_t g = w.graphs->get(key, s, [&] {
         enqueue_restart(first_c, first_c && zero_guess);
-        for (int it = 0; it < m; ++it) enqueue_iteration(it);
+        for (int it = 0; it < m; ++it) enqueue_iteration(it, false);
         enqueue_end();
       });
       S355_HIP(hipGraphLaunch(g, s));
@@ -1150,7 +1154,7 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
     gz = false;
     int it = 0;
     while (it < m && its < o.maxit) {
-      enqueue_iteration(it);
+      enqueue_iteration(it, true);
       ++it;
       ++its;
       if (it % check_every == 0 && it < m) {
@@ -1162,6 +1166,7 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
     read_headers(w, s);
     if (all_done(w)) break;
   }
+  if (w.timer) w.timer->harvest();  // the header reads have synchronised the stream
   return collect(w, mixed);
 }
 
@@ -1300,6 +1305,36 @@ __global__ void __launch_bounds__(kBlock) k_add_correction(int n, double* x, con
 }
 
 }  // namespace
+
+SpmvTimer::~SpmvTimer() {
+  for (hipEvent_t e : pool) hipEventDestroy(e);
+}
+
+bool SpmvTimer::take(hipEvent_t& a, hipEvent_t& b) {
+  if (used + 2 > pool.size()) {
+    if (pool.size() >= 8192) return false;  // bounded: the rest go untimed until harvest
+    for (int k = 0; k < 2; ++k) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return false;
+      pool.push_back(e);
+    }
+  }
+  a = pool[used];
+  b = pool[used + 1];
+  used += 2;
+  return true;
+}
+
+void SpmvTimer::harvest() {
+  for (size_t k = 0; k + 1 < used; k += 2) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, pool[k], pool[k + 1]) == hipSuccess) {
+      total_ms += ms;
+      ++count;
+    }
+  }
+  used = 0;
+}
 
 GraphCache::~GraphCache() {
   for (auto& kv : execs) hipGraphExecDestroy(kv.second);

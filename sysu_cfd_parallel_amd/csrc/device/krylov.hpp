@@ -121,6 +121,21 @@ struct TuneReport {
   int gs_it = 0;
 };
 
+// Device time of the production SpMV launches inside GMRES solves: every
+// `every`-th Arnoldi SpMV is bracketed by a pair of HIP events on the solve's
+// stream (no synchronisation is added); harvest() reads the pairs once the
+// stream has passed them.  Launch-by-launch solves only.
+struct SpmvTimer {
+  int every = 4;
+  long seen = 0, count = 0;
+  double total_ms = 0.0;
+  std::vector<hipEvent_t> pool;  // pairs
+  size_t used = 0;
+  ~SpmvTimer();
+  bool take(hipEvent_t& a, hipEvent_t& b);
+  void harvest();
+};
+
 // Workspace owned by the caller.  ncase independent systems of the same
 // type share one launch per kernel: their rows are consecutive, their states
 // are st[0..ncase), and tiles[k] splits the rows into tiles of kBlock * 2^k
@@ -146,6 +161,7 @@ struct KWork {
   bool tuned = false;       // lanes / nt / rows chosen by autotune()
   TuneReport report;
   GraphCache* graphs = nullptr;  // set: GMRES cycles (Jacobi / no PC) replay captured graphs
+  SpmvTimer* timer = nullptr;    // set: time the Arnoldi SpMV launches
   KState* st = nullptr;     // device, ncase states
   KHeader* host = nullptr;  // pinned, ncase headers
   std::vector<KResult>* per_case = nullptr;  // filled by a solve when set

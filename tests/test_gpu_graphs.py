@@ -134,3 +134,16 @@ def test_every_spmv_shape_matches_the_oracle(lanes):
     assert np.allclose(hd, ho, rtol=1e-7, atol=1e-14)
     for n in ("U", "V", "W", "P"):
         assert rel_err(d.get(n), o.get(n)) < 1e-8, n
+
+
+def test_in_solve_spmv_timing():
+    case = s.cavity(10, re=400.0)
+    d = s.Solver(case, base(s.FP64))
+    d.spmv_profile(0, 1)
+    d.iterate(2)
+    ms, n = d.spmv_profile(0, -1)
+    its = sum(v["iters"] for v in d.last_solve_info().values())
+    assert n > 0 and n <= its * 2 and 0.0 < ms < 50.0
+    # stopped: further solves add nothing
+    d.iterate(1)
+    assert d.spmv_profile(0, 0)[1] == 0
