@@ -146,4 +146,4 @@ def test_in_solve_spmv_timing():
     assert n > 0 and n <= its * 2 and 0.0 < ms < 50.0
     # stopped: further solves add nothing
     d.iterate(1)
-    assert d.spmv_profile(0, 0)[1] == 0
+    assert d.spmv_profile(0, 0)[1] == n
