@@ -334,7 +334,9 @@ struct GsShape {
   __device__ static bool vec_ok(int a, size_t ld) { return (a % Vec16<T>::n) == 0 && (ld % Vec16<T>::n) == 0; }
 };
 
-// ---- partial[b][k] = sum over tile b of w_i V_k,i, k = 0..it
+// ---- partial[k][b] = sum over tile b of w_i V_k,i, k = 0..it (k-major, so
+// that the per-k reduction over the tiles reads contiguously; the row stride
+// is the tile count, pstride only bounds the capacity)
 template <class T, int R>
 __global__ void __launch_bounds__(kBlock) k_mdot(const T* __restrict__ V, size_t ld, Tiles t, int it,
                                                  double* __restrict__ partial, int pstride, const KState* st) {
@@ -377,13 +379,13 @@ __global__ void __launch_bounds__(kBlock) k_mdot(const T* __restrict__ V, size_t
       double s = 0.0;
 #pragma unroll
       for (int q = 0; q < kBlock / kWave; ++q) s += lds[q][threadIdx.x];
-      partial[static_cast<size_t>(blockIdx.x) * pstride + k0 + threadIdx.x] = s;
+      partial[static_cast<size_t>(k0 + threadIdx.x) * t.ntile + blockIdx.x] = s;
     }
     __syncthreads();
   }
 }
 
-// ---- hcol[k] = scale_k * sum over the case's tiles of partial[b][k]
+// ---- hcol[k] = scale_k * sum over the case's tiles of partial[k][b]
 // (block (k, case))
 __global__ void __launch_bounds__(kBlock) k_mdot_final(const double* partial, Tiles t, int pstride, KState* st) {
   KState* s = st + blockIdx.y;
@@ -393,7 +395,8 @@ __global__ void __launch_bounds__(kBlock) k_mdot_final(const double* partial, Ti
   int t0, t1;
   t.tiles_of(blockIdx.y, t0, t1);
   double v = 0.0;
-  for (int b = t0 + threadIdx.x; b < t1; b += kBlock) v += partial[static_cast<size_t>(b) * pstride + k];
+  const double* pk = partial + static_cast<size_t>(k) * t.ntile;  // column k: contiguous over tiles
+  for (int b = t0 + threadIdx.x; b < t1; b += kBlock) v += pk[b];
   v = block_sum(v, lds);
   if (threadIdx.x == 0) s->hcol[k] = v * s->scale[k];
 }
