@@ -373,6 +373,8 @@ def main() -> int:
                     help="both: FP64 then mixed on the same case, value is the mixed rate")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--graphs", type=int, choices=[0, 1], default=None,
+                    help="replay GMRES cycles from captured graphs (default: per workload)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
@@ -382,6 +384,8 @@ def main() -> int:
     import sysu_cfd_parallel_amd as s
 
     wl = WORKLOADS[args.workload](args.size)
+    if args.graphs is not None:
+        wl.graphs = args.graphs
     steps = args.steps if args.steps is not None else wl.steps
     warmup = args.warmup if args.warmup is not None else wl.warmup
     order = {"default": wl.precisions, "both": ["fp64", "mixed"], "fp64": ["fp64"], "mixed": ["mixed"]}[args.precision]

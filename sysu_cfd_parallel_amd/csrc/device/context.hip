@@ -643,14 +643,16 @@ void Context::check_flags() {
 
 void Context::ensure_sell() {
   if (sell_src_) return;
-  std::vector<int> ptr, col, src;
-  sell_pattern(dm.Nc, mesh.rowptr.data(), mesh.col.data(), ptr, col, src);
+  std::vector<int> ptr, row, col, src;
+  sell_pattern(dm.Nc, mesh.rowptr.data(), mesh.col.data(), kSellSigma, ptr, row, col, src);
   sell_slots_ = src.size();
   const int* dptr = upload(allocs_, ptr, stream);
+  const int* drow = upload(allocs_, row, stream);
   const int* dcol = upload(allocs_, col, stream);
   sell_src_ = upload(allocs_, src, stream);
   sync();  // host staging vectors go out of scope
   kwork_.sell_ptr = kwork32_.sell_ptr = dptr;
+  kwork_.sell_row = kwork32_.sell_row = drow;
   kwork_.sell_col = kwork32_.sell_col = dcol;
 }
 

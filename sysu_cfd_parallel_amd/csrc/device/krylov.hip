@@ -198,11 +198,12 @@ template <class T, bool NT>
 __global__ void __launch_bounds__(kBlock) k_spmv_sell(SellView<T> A, Tiles t, const T* __restrict__ x,
                                                       const T* __restrict__ inv, T* __restrict__ y, const KState* st,
                                                       int it) {
-  const int row = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (row >= A.n) return;
+  const int pos = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (pos >= A.n) return;
+  const int row = A.row[pos];
   const KState* sc = st + t.case_of(row);
   if (sc->reason) return;
-  const int sl = row >> 6, lane = row & 63;
+  const int sl = pos >> 6, lane = pos & 63;
   const int q1 = A.ptr[sl + 1];
   T acc = T(0);
   for (int q = A.ptr[sl] + lane; q < q1; q += 64) acc += ld_stream<T, NT>(A.val + q) * x[ld_stream<int, NT>(A.col + q)];
@@ -267,6 +268,7 @@ SellView<T> sell_of(const KWork& w, int n) {
   if (w.sell_ptr) {
     v.n = n;
     v.ptr = w.sell_ptr;
+    v.row = w.sell_row;
     v.col = w.sell_col;
     v.val = static_cast<const T*>(w.sell_val);
   }
@@ -1641,13 +1643,24 @@ double bench_copy(void* dst, const void* src, size_t bytes, int repeats, hipStre
   return best;
 }
 
-void sell_pattern(int n, const int* rowptr, const int* col, std::vector<int>& ptr, std::vector<int>& scol,
-                  std::vector<int>& src) {
+void sell_pattern(int n, const int* rowptr, const int* col, int sigma, std::vector<int>& ptr, std::vector<int>& srow,
+                  std::vector<int>& scol, std::vector<int>& src) {
+  if (sigma < 64 || sigma % 64) throw std::invalid_argument("SELL sigma must be a positive multiple of 64");
+  // slot-row order: each window of sigma rows by decreasing length, stable
+  srow.resize(n);
+  for (int w0 = 0; w0 < n; w0 += sigma) {
+    const int w1 = std::min(n, w0 + sigma);
+    for (int r = w0; r < w1; ++r) srow[r] = r;
+    std::stable_sort(srow.begin() + w0, srow.begin() + w1, [&](int a, int b) {
+      return rowptr[a + 1] - rowptr[a] > rowptr[b + 1] - rowptr[b];
+    });
+  }
+  auto len = [&](int pos) { return pos < n ? rowptr[srow[pos] + 1] - rowptr[srow[pos]] : 0; };
   const int ns = (n + 63) / 64;
   ptr.assign(ns + 1, 0);
   for (int sl = 0; sl < ns; ++sl) {
     int w = 0;
-    for (int r = 64 * sl; r < std::min(n, 64 * sl + 64); ++r) w = std::max(w, rowptr[r + 1] - rowptr[r]);
+    for (int pos = 64 * sl; pos < std::min(n, 64 * sl + 64); ++pos) w = std::max(w, len(pos));
     const long next = static_cast<long>(ptr[sl]) + 64L * w;
     if (next > 0x7fffffffL) throw std::invalid_argument("SELL-64 copy above 32-bit indices");
     ptr[sl + 1] = static_cast<int>(next);
@@ -1657,15 +1670,16 @@ void sell_pattern(int n, const int* rowptr, const int* col, std::vector<int>& pt
   for (int sl = 0; sl < ns; ++sl) {
     const int w = (ptr[sl + 1] - ptr[sl]) / 64;
     for (int l = 0; l < 64; ++l) {
-      const int r = 64 * sl + l;
-      const int len = r < n ? rowptr[r + 1] - rowptr[r] : 0;
+      const int pos = 64 * sl + l;
+      const int r = pos < n ? srow[pos] : 0;
+      const int lr = len(pos);
       for (int j = 0; j < w; ++j) {
         const size_t q = static_cast<size_t>(ptr[sl]) + 64 * static_cast<size_t>(j) + l;
-        if (j < len) {
+        if (j < lr) {
           scol[q] = col[rowptr[r] + j];
           src[q] = rowptr[r] + j;
         } else {
-          scol[q] = r < n ? r : 0;  // a valid column; the value is zero
+          scol[q] = r;  // a valid column; the value is zero
         }
       }
     }

@@ -58,24 +58,31 @@ struct CsrView {
   int nnz = 0;
 };
 
-// Sliced ELL (SELL-64) copy of a CSR matrix for the SpMV: rows in slices of
-// 64, one row per lane; slice s holds width_s = its longest row's length
-// columns stored column-major (entry j of row 64 s + l at ptr[s] + 64 j + l),
-// shorter rows padded with zero values and their own column.  Every lane then
-// has all its row's loads independent and in flight, and a wave's loads are
-// contiguous.  Entries keep the CSR order within a row.
+// Sliced ELL (SELL-64-sigma) copy of a CSR matrix for the SpMV: within each
+// window of sigma consecutive rows the rows are ordered by decreasing length
+// (stable), then cut into slices of 64, one row per lane; slice s holds
+// width_s = its longest row's length columns stored column-major (entry j of
+// slot row 64 s + l at ptr[s] + 64 j + l), shorter rows padded with zero
+// values and their own column; row[64 s + l] is the matrix row it holds.
+// Every lane then has all its row's loads independent and in flight, a
+// wave's loads are contiguous, and sorting by length keeps the padding small
+// on meshes that mix tetrahedra, hexahedra and polyhedra.  Entries keep the
+// CSR order within a row.
 template <class T>
 struct SellView {
   int n = 0;
   const int* ptr = nullptr;  // nslice + 1 offsets
+  const int* row = nullptr;  // n: matrix row of each slot row
   const int* col = nullptr;
   const T* val = nullptr;
 };
 
-// Host construction of the pattern: ptr, col and, for every SELL slot, the
-// CSR position it copies (-1 for padding).
-void sell_pattern(int n, const int* rowptr, const int* col, std::vector<int>& ptr, std::vector<int>& scol,
-                  std::vector<int>& src);
+constexpr int kSellSigma = 2048;
+
+// Host construction of the pattern: ptr, row, col and, for every SELL slot,
+// the CSR position it copies (-1 for padding).
+void sell_pattern(int n, const int* rowptr, const int* col, int sigma, std::vector<int>& ptr, std::vector<int>& srow,
+                  std::vector<int>& scol, std::vector<int>& src);
 
 // out[q] = csr_val[src[q]] (0 where src[q] < 0), converted to T.
 template <class T, class S>
@@ -155,6 +162,7 @@ struct KWork {
   int nt = 0;               // SpMV loads the matrix non-temporally
   // SELL-64 copy of the matrix being solved (values of type T), when set
   const int* sell_ptr = nullptr;
+  const int* sell_row = nullptr;
   const int* sell_col = nullptr;
   const void* sell_val = nullptr;
   int rows = 0;             // rows per thread of the Gram-Schmidt kernels; 0: by size
