@@ -137,6 +137,7 @@ class Step(Channel):
     precisions = ["fp64"]
     steps, warmup = 5, 1
     turbulent = True
+    graphs = 1  # the ILU(0) level sweeps replay from captured graphs
 
     def case(self, dims=None):
         import sysu_cfd_parallel_amd as s
@@ -169,6 +170,7 @@ class Large(Channel):
     precisions = ["mixed"]
     steps, warmup = 3, 1
     turbulent = True
+    graphs = 1  # the ILU(0) level sweeps replay from captured graphs
 
     def case(self, dims=None):
         return _channel_case(dims or self.dims, 0.10, 0.0, 13750.0, turbulent=True)

@@ -132,10 +132,19 @@ void ilu_factor(const CsrView<T>& A, int nnz, const IluLevels& lv, T* lu, hipStr
   S355_LAUNCH_CHECK();
 }
 
-// z = (LU)^-1 r ; r and z distinct
+// z = (LU)^-1 r ; r and z distinct.  With a graph cache the level sweeps
+// (one launch per level) are captured once per (factor, r, z) and replayed.
 template <class T>
 void ilu_apply(const CsrView<T>& A, const IluLevels& lv, const T* lu, const T* r, T* z, const KState* st,
-               hipStream_t s) {
+               hipStream_t s, GraphCache* graphs = nullptr) {
+  if (graphs) {
+    const std::vector<uintptr_t> key = {0x11u, reinterpret_cast<uintptr_t>(lu), reinterpret_cast<uintptr_t>(r),
+                                        reinterpret_cast<uintptr_t>(z), reinterpret_cast<uintptr_t>(st),
+                                        reinterpret_cast<uintptr_t>(lv.lrows), sizeof(T)};
+    hipGraphExec_t g = graphs->get(key, s, [&] { ilu_apply(A, lv, lu, r, z, st, s, nullptr); });
+    S355_HIP(hipGraphLaunch(g, s));
+    return;
+  }
   for (int l = 0; l < lv.nl(); ++l) {
     const int nr = lv.lptr[l + 1] - lv.lptr[l];
     hipLaunchKernelGGL((k_ilu_lower<T>), dim3(grid_for(nr)), dim3(kBlock), 0, s, A, lu, r, z, lv.lrows + lv.lptr[l],
@@ -1085,7 +1094,7 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
     if (lu) {
       hipLaunchKernelGGL((k_residual_pc<T, X>), dim3(tl.ntile), dim3(kBlock), 0, s, Ares, tl, b, x,
                          static_cast<const X*>(nullptr), tmp, w.partial, gz_c ? 1 : 0, w.st);
-      ilu_apply(A, *w.ilu, lu, tmp, V, ilu_st, s);
+      ilu_apply(A, *w.ilu, lu, tmp, V, ilu_st, s, w.graphs);
       hipLaunchKernelGGL((k_pc_norm<T>), dim3(tl.ntile), dim3(kBlock), 0, s, tl, V, static_cast<const T*>(nullptr),
                          w.partial, w.st);
     } else {
@@ -1106,7 +1115,7 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
       launch_spmv<T>(L, w.nt, A, sell, tl, V + it * ld, inv, V + (it + 1) * ld, w.st, it, s);
     }
     if (rec) S355_HIP(hipEventRecord(e1, s));
-    if (lu) ilu_apply(A, *w.ilu, lu, tmp, V + (it + 1) * ld, ilu_st, s);
+    if (lu) ilu_apply(A, *w.ilu, lu, tmp, V + (it + 1) * ld, ilu_st, s, w.graphs);
     Dispatch<T>::mdot(R, V, ld, tl, it, w.partial, pstride, w.st, s);
     hipLaunchKernelGGL(k_mdot_final, dim3(it + 1, nc), dim3(kBlock), 0, s, w.partial, tl, pstride, w.st);
     Dispatch<T>::maxpy(R, V, ld, tl, it, w.partial, w.st, s);
@@ -1212,7 +1221,7 @@ KResult cg(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, bool z
   }
   hipLaunchKernelGGL((k_cg_start<T>), dim3(g1), dim3(kBlock), 0, s, A, b, x, inv, r, z, w.partial, zero_guess ? 1 : 0);
   if (lu) {
-    ilu_apply(A, *w.ilu, lu, r, z, w.st, s);
+    ilu_apply(A, *w.ilu, lu, r, z, w.st, s, w.graphs);
     hipLaunchKernelGGL((k_dot2<T>), dim3(g1), dim3(kBlock), 0, s, n, z, r, w.partial, w.st);
   }
   hipLaunchKernelGGL(k_cg_start_final, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st, zero_guess ? 1 : 0);
@@ -1226,7 +1235,7 @@ KResult cg(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, bool z
     hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st, i);
     hipLaunchKernelGGL((k_cg_update<T>), dim3(g1), dim3(kBlock), 0, s, n, x, r, z, p, wv, inv, w.partial, w.st);
     if (lu) {
-      ilu_apply(A, *w.ilu, lu, r, z, w.st, s);
+      ilu_apply(A, *w.ilu, lu, r, z, w.st, s, w.graphs);
       hipLaunchKernelGGL((k_dot2<T>), dim3(g1), dim3(kBlock), 0, s, n, z, r, w.partial, w.st);
     }
     hipLaunchKernelGGL(k_cg_final, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st);

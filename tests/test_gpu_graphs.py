@@ -147,3 +147,21 @@ def test_in_solve_spmv_timing():
     # stopped: further solves add nothing
     d.iterate(1)
     assert d.spmv_profile(0, 0)[1] == n
+
+
+@pytest.mark.parametrize("ksp", [s.KSP_GMRES, s.KSP_CG])
+def test_ilu_sweeps_replayed_from_graphs(ksp):
+    case = s.channel(10, 6, 6, jitter=0.1, split=0.2, seed=2, re=500.0)
+    out = []
+    for graphs in (0, 1):
+        p = params(restart=20, pc=s.PC_ILU0, rtol=1e-8, maxit=3000)
+        p.ksp[s.KSP_PRESSURE].type = ksp
+        p.ksp[s.KSP_PRESSURE].rtol = 1e-6
+        p.graphs = graphs
+        d = s.Solver(case, p)
+        h, _ = d.iterate(5)
+        out.append(({n: d.get(n) for n in FIELDS}, h))
+        d.close()
+    for n in FIELDS:
+        assert np.array_equal(out[0][0][n], out[1][0][n]), n
+    assert np.array_equal(out[0][1], out[1][1])
