@@ -36,19 +36,22 @@ def main() -> int:
     for k in range(3):
         p.ksp[k].maxit = 20000
     p.precision = s.MIXED
+    p.autotune = 1  # the production kernel shapes (SELL-64 SpMV on this mesh)
     copy_bytes = args.copy_mib << 20
     with s.Solver(case, p) as dev:
         z = dev.sizes
         dev.iterate(1)
         ms64 = dev.spmv_time(1, 0, args.repeats)
         ms32 = dev.spmv_time(1, 1, args.repeats)
+        shapes = dev.kernel_shapes()
         ms_copy = dev.copy_time(copy_bytes, args.repeats)
 
     def spmv_bytes(bv, bx):
-        return (bv + 4) * z.nnz + 4 * (z.n_cell + 1) + 2 * bx * z.n_cell
+        # CSR values + column ids, row pointers, x, y and the Jacobi diagonal
+        return (bv + 4) * z.nnz + 4 * (z.n_cell + 1) + 3 * bx * z.n_cell
 
     print(json.dumps({
-        "cells": z.n_cell, "nnz": z.nnz, "repeats": args.repeats,
+        "cells": z.n_cell, "nnz": z.nnz, "repeats": args.repeats, "shapes": shapes,
         "spmv_fp64": {"bytes_per_launch": spmv_bytes(8, 8), "ms": ms64},
         "spmv_fp32": {"bytes_per_launch": spmv_bytes(4, 4), "ms": ms32},
         "copy": {"bytes_read": copy_bytes, "bytes_written": copy_bytes, "ms": ms_copy,
