@@ -196,6 +196,16 @@ int s355_case_sizes(const s355_case* c, int* sizes4);
 int s355_case_copy(const s355_case* c, double* xyz, int* type, int* tag, int* vptr, int* vidx,
                    s355_region* regions);
 void s355_case_free(s355_case* c);
+/* Mesh files in the reference's formats: binary = 0 gmsh 2 ASCII
+ * (RootProcess::read, RootStructure.cpp:428-487), 1 the binary form of
+ * RootProcess::readBin (RootStructure.cpp:61-116, written by postCycas.py).
+ * The first tag of each element is its region id; unreferenced vertices are
+ * dropped and the rest renumbered in order (RootStructure.cpp:930-968).  A
+ * case read from a file has no region table: the caller supplies one (the
+ * reference's param.in). */
+int s355_case_read(const char* path, int binary, s355_case** out);
+/* Writes a mesh (its element stream) in either format. */
+int s355_case_write(const s355_mesh_desc* mesh, const char* path, int binary);
 
 /* --- context: set-up (ReadGridFile / CreateFaces / CellFaceInfo /
  *     CheckAndAllocate / initPetsc / InitFlowField) --------------------- */

@@ -22,7 +22,8 @@ from ._abi import (FIELD_ID, FIELD_KIND, MESH_DBL_ID, MESH_INT_ID, FP64, KSP_CG,
 LIB_PATH = Path(__file__).resolve().parent / "_build" / "libsimple355.so"
 
 __all__ = [
-    "LIB_PATH", "library", "CaseData", "cavity", "channel", "step", "default_params", "set_all_ksp", "HostMesh",
+    "LIB_PATH", "library", "CaseData", "cavity", "channel", "step", "read_mesh", "write_mesh", "default_params",
+    "set_all_ksp", "HostMesh",
     "Solver", "BatchSolver", "S355Error", "NotConverged", "FP64", "MIXED", "KSP_GMRES", "KSP_CG", "PC_NONE", "PC_JACOBI",
     "PC_ILU0", "KSP_VELOCITY", "KSP_PRESSURE", "KSP_SCALAR",
 ]
@@ -64,6 +65,8 @@ def library(path: Optional[Path] = None) -> C.CDLL:
         "case_sizes": (I, [V, pI]),
         "case_copy": (I, [V, pD, pI, pI, pI, pI, C.POINTER(Region)]),
         "case_free": (None, [V]),
+        "case_read": (I, [C.c_char_p, I, C.POINTER(V)]),
+        "case_write": (I, [C.POINTER(MeshDesc), C.c_char_p, I]),
         "mesh_build": (I, [C.POINTER(MeshDesc), C.POINTER(V)]),
         "mesh_build_batch": (I, [I, C.POINTER(MeshDesc), C.POINTER(V)]),
         "mesh_sizes": (I, [V, C.POINTER(Sizes)]),
@@ -168,6 +171,29 @@ def build_case(kind: int, ip: list, dp: list, meta: Optional[dict] = None) -> Ca
     dpa = np.asarray(dp, dtype=np.float64)
     h = C.c_void_p()
     _check(lib, lib.s355_case_build(kind, _ip(ipa), _dp(dpa), C.byref(h)), "case_build")
+    return _take_case(lib, h, meta)
+
+
+def read_mesh(path, regions: list, binary: bool = False) -> CaseData:
+    """A mesh file in the reference's formats (gmsh 2 ASCII, or the binary
+    form of RootProcess::readBin) with the region table `regions` (what the
+    reference reads from param.in)."""
+    lib = library()
+    h = C.c_void_p()
+    _check(lib, lib.s355_case_read(str(path).encode(), int(binary), C.byref(h)), "case_read")
+    case = _take_case(lib, h, {"kind": "file", "path": str(path)})
+    case.regions = list(regions)
+    return case
+
+
+def write_mesh(case: CaseData, path, binary: bool = False) -> None:
+    """Write a case's mesh in the reference's gmsh 2 ASCII or binary format."""
+    lib = library()
+    d = case.desc()
+    _check(lib, lib.s355_case_write(C.byref(d), str(path).encode(), int(binary)), "case_write")
+
+
+def _take_case(lib, h, meta) -> CaseData:
     try:
         sz = np.zeros(4, dtype=np.int32)
         _check(lib, lib.s355_case_sizes(h, _ip(sz)), "case_sizes")

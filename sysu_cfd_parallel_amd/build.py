@@ -20,6 +20,7 @@ SOURCES = [
     "mesh/cases.cpp",
     "mesh/ingest.cpp",
     "mesh/batch.cpp",
+    "mesh/gmsh.cpp",
     "device/fv_kernels.hip",
     "device/keps_kernels.hip",
     "device/krylov.hip",

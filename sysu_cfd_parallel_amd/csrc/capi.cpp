@@ -12,6 +12,7 @@
 #include "device/context.hpp"
 #include "host/navier.hpp"
 #include "mesh/cases.hpp"
+#include "mesh/gmsh.hpp"
 #include "mesh/ingest.hpp"
 
 struct s355_ctx {
@@ -271,6 +272,35 @@ int s355_case_copy(const s355_case* c, double* xyz, int* type, int* tag, int* vp
 }
 
 void s355_case_free(s355_case* c) { delete c; }
+
+int s355_case_read(const char* path, int binary, s355_case** out) {
+  return guarded([&] {
+    if (!path || !out) throw std::invalid_argument("null argument");
+    auto c = std::make_unique<s355_case>();
+    c->es = binary ? s355::read_gmsh_binary(path) : s355::read_gmsh_ascii(path);
+    *out = c.release();
+    return S355_OK;
+  });
+}
+
+int s355_case_write(const s355_mesh_desc* mesh, const char* path, int binary) {
+  return guarded([&] {
+    if (!mesh || !path) throw std::invalid_argument("null argument");
+    s355::ElementStream es;
+    es.xyz.assign(mesh->xyz, mesh->xyz + 3 * static_cast<size_t>(mesh->n_vert));
+    for (int e = 0; e < mesh->n_elem; ++e) {
+      const int t = mesh->elem_type[e];
+      if (s355::gmsh_nodes(t) != mesh->elem_vptr[e + 1] - mesh->elem_vptr[e])
+        throw std::invalid_argument("element " + std::to_string(e) + ": vertex count does not match its type");
+      es.add(t, mesh->elem_tag[e], mesh->elem_vidx + mesh->elem_vptr[e]);
+    }
+    if (binary)
+      s355::write_gmsh_binary(es, path);
+    else
+      s355::write_gmsh_ascii(es, path);
+    return S355_OK;
+  });
+}
 
 int s355_mesh_build(const s355_mesh_desc* desc, s355_mesh** out) {
   return guarded([&] {
