@@ -430,7 +430,23 @@ void Context::ensure_ilu() {
   ilu_levels(dm.Nc, mesh.rowptr.data(), mesh.col.data(), mesh.diag.data(), lrows, levels_.lptr, urows, levels_.uptr);
   levels_.lrows = upload(allocs_, lrows, stream);
   levels_.urows = upload(allocs_, urows, stream);
+  // level-ELL copies of the factors for the sweeps
+  for (int up = 0; up < 2; ++up) {
+    IluEll& e = up ? levels_.uell : levels_.lell;
+    std::vector<int> ecol, esrc, len, dsrc;
+    ilu_ell(dm.Nc, mesh.rowptr.data(), mesh.col.data(), mesh.diag.data(), up ? urows : lrows,
+            up ? levels_.uptr : levels_.lptr, up != 0, e.off, ecol, esrc, len, dsrc);
+    e.slots = esrc.size();
+    e.col = upload(allocs_, ecol, stream);
+    e.src = upload(allocs_, esrc, stream);
+    e.len = upload(allocs_, len, stream);
+    if (up) e.dsrc = upload(allocs_, dsrc, stream);
+    sync();  // host staging vectors go out of scope
+  }
   kwork_.lu = alloc<double>(dm.nnz);
+  kwork_.ilu_lval = alloc<double>(levels_.lell.slots);
+  kwork_.ilu_uval = alloc<double>(levels_.uell.slots);
+  kwork_.ilu_dval = alloc<double>(dm.Nc);
   kwork_.ilu = &levels_;
   sync();  // host staging of the level lists
   have_levels_ = true;
@@ -478,6 +494,11 @@ int Context::run_solve(int which, const double* b, double* x, const s355_ksp_opt
     // FP32 inner solves on the rounded matrix inside FP64 refinement
     ensure_mixed();
     kwork32_.ilu = kwork_.ilu;
+    if (kwork32_.ilu && !kwork32_.ilu_lval) {
+      kwork32_.ilu_lval = alloc<float>(levels_.lell.slots);
+      kwork32_.ilu_uval = alloc<float>(levels_.uell.slots);
+      kwork32_.ilu_dval = alloc<float>(dm.Nc);
+    }
     convert_to_float(val, a32_, dm.nnz, stream);
     CsrView<float> A{dm.Nc, dm.rowptr, dm.col, dm.diag, a32_, dm.nnz};
     CsrView<double> A64{dm.Nc, dm.rowptr, dm.col, dm.diag, val, dm.nnz};

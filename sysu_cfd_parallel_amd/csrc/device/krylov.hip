@@ -102,6 +102,52 @@ __global__ void __launch_bounds__(kBlock) k_ilu_upper(CsrView<T> A, const T* __r
   z[i] = s / lu[pd];
 }
 
+// the same sweeps on the level-ELL copies (one level per launch)
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_ilu_lower_ell(const int* __restrict__ rows, const int* __restrict__ len,
+                                                          const int* __restrict__ ecol, const T* __restrict__ eval,
+                                                          long base, int nrows, const T* r, T* z, const KState* st) {
+  if (st && st->reason) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nrows) return;
+  const int i = rows[k];
+  T s = r[i];
+  const int L = len[k];
+  for (int j = 0; j < L; ++j) {
+    const size_t q = static_cast<size_t>(base) + static_cast<size_t>(j) * nrows + k;
+    s -= eval[q] * z[ecol[q]];
+  }
+  z[i] = s;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_ilu_upper_ell(const int* __restrict__ rows, const int* __restrict__ len,
+                                                          const int* __restrict__ ecol, const T* __restrict__ eval,
+                                                          const T* __restrict__ dval, long base, int nrows, T* z,
+                                                          const KState* st) {
+  if (st && st->reason) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nrows) return;
+  const int i = rows[k];
+  T s = z[i];
+  const int L = len[k];
+  for (int j = 0; j < L; ++j) {
+    const size_t q = static_cast<size_t>(base) + static_cast<size_t>(j) * nrows + k;
+    s -= eval[q] * z[ecol[q]];
+  }
+  z[i] = s / dval[k];
+}
+
+// out[q] = lu[src[q]] (0 where src[q] < 0)
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_gather(const int* __restrict__ src, size_t n, const T* __restrict__ lu,
+                                                   T* __restrict__ out) {
+  const size_t q = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int p = src[q];
+  out[q] = p >= 0 ? lu[p] : T(0);
+}
+
 // partial (z,z) and (z,r) for CG with a non-pointwise preconditioner
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_dot2(int n, const T* z, const T* r, double* partial, const KState* st) {
@@ -122,12 +168,23 @@ __global__ void __launch_bounds__(kBlock) k_dot2(int n, const T* z, const T* r, 
   }
 }
 
+// ILU(0) factors of A into lu (CSR), then their level-ELL copies into the
+// workspace's value buffers when those are set
 template <class T>
-void ilu_factor(const CsrView<T>& A, int nnz, const IluLevels& lv, T* lu, hipStream_t s) {
+void ilu_factor(const CsrView<T>& A, int nnz, const IluLevels& lv, T* lu, hipStream_t s, const KWork* w = nullptr) {
   S355_HIP(hipMemcpyAsync(lu, A.val, sizeof(T) * static_cast<size_t>(nnz), hipMemcpyDeviceToDevice, s));
   for (int l = 0; l < lv.nl(); ++l) {
     const int nr = lv.lptr[l + 1] - lv.lptr[l];
     hipLaunchKernelGGL((k_ilu_factor<T>), dim3(grid_for(nr)), dim3(kBlock), 0, s, A, lu, lv.lrows + lv.lptr[l], nr);
+  }
+  if (w && w->ilu_lval && lv.lell.col) {
+    const size_t nrow = lv.uptr.back();
+    hipLaunchKernelGGL((k_gather<T>), dim3(grid_for(static_cast<long>(lv.lell.slots))), dim3(kBlock), 0, s,
+                       lv.lell.src, lv.lell.slots, lu, static_cast<T*>(w->ilu_lval));
+    hipLaunchKernelGGL((k_gather<T>), dim3(grid_for(static_cast<long>(lv.uell.slots))), dim3(kBlock), 0, s,
+                       lv.uell.src, lv.uell.slots, lu, static_cast<T*>(w->ilu_uval));
+    hipLaunchKernelGGL((k_gather<T>), dim3(grid_for(static_cast<long>(nrow))), dim3(kBlock), 0, s, lv.uell.dsrc, nrow,
+                       lu, static_cast<T*>(w->ilu_dval));
   }
   S355_LAUNCH_CHECK();
 }
@@ -136,13 +193,31 @@ void ilu_factor(const CsrView<T>& A, int nnz, const IluLevels& lv, T* lu, hipStr
 // (one launch per level) are captured once per (factor, r, z) and replayed.
 template <class T>
 void ilu_apply(const CsrView<T>& A, const IluLevels& lv, const T* lu, const T* r, T* z, const KState* st,
-               hipStream_t s, GraphCache* graphs = nullptr) {
+               hipStream_t s, GraphCache* graphs = nullptr, const KWork* w = nullptr) {
   if (graphs) {
     const std::vector<uintptr_t> key = {0x11u, reinterpret_cast<uintptr_t>(lu), reinterpret_cast<uintptr_t>(r),
                                         reinterpret_cast<uintptr_t>(z), reinterpret_cast<uintptr_t>(st),
-                                        reinterpret_cast<uintptr_t>(lv.lrows), sizeof(T)};
-    hipGraphExec_t g = graphs->get(key, s, [&] { ilu_apply(A, lv, lu, r, z, st, s, nullptr); });
+                                        reinterpret_cast<uintptr_t>(lv.lrows), sizeof(T),
+                                        reinterpret_cast<uintptr_t>(w ? w->ilu_lval : nullptr)};
+    hipGraphExec_t g = graphs->get(key, s, [&] { ilu_apply(A, lv, lu, r, z, st, s, nullptr, w); });
     S355_HIP(hipGraphLaunch(g, s));
+    return;
+  }
+  if (w && w->ilu_lval && lv.lell.col) {
+    const T* lval = static_cast<const T*>(w->ilu_lval);
+    const T* uval = static_cast<const T*>(w->ilu_uval);
+    const T* dval = static_cast<const T*>(w->ilu_dval);
+    for (int l = 0; l < lv.nl(); ++l) {
+      const int nr = lv.lptr[l + 1] - lv.lptr[l];
+      hipLaunchKernelGGL((k_ilu_lower_ell<T>), dim3(grid_for(nr)), dim3(kBlock), 0, s, lv.lrows + lv.lptr[l],
+                         lv.lell.len + lv.lptr[l], lv.lell.col, lval, lv.lell.off[l], nr, r, z, st);
+    }
+    for (int l = 0; l < lv.nu(); ++l) {
+      const int nr = lv.uptr[l + 1] - lv.uptr[l];
+      hipLaunchKernelGGL((k_ilu_upper_ell<T>), dim3(grid_for(nr)), dim3(kBlock), 0, s, lv.urows + lv.uptr[l],
+                         lv.uell.len + lv.uptr[l], lv.uell.col, uval, dval + lv.uptr[l], lv.uell.off[l], nr, z, st);
+    }
+    S355_LAUNCH_CHECK();
     return;
   }
   for (int l = 0; l < lv.nl(); ++l) {
@@ -1064,7 +1139,7 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
   } else if (o.pc == S355_PC_ILU0) {
     if (!w.ilu || !w.lu || !w.r) throw std::invalid_argument("gmres: ILU(0) workspace missing");
     lu = static_cast<T*>(w.lu);
-    ilu_factor(A, A.nnz, *w.ilu, lu, s);
+    ilu_factor(A, A.nnz, *w.ilu, lu, s, &w);
   } else if (o.pc != S355_PC_NONE) {
     throw std::invalid_argument("gmres: unknown preconditioner");
   }
@@ -1080,7 +1155,7 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
       } else {
         bt = b;
       }
-      ilu_apply(A, *w.ilu, lu, bt, V, ilu_st, s);
+      ilu_apply(A, *w.ilu, lu, bt, V, ilu_st, s, nullptr, &w);
       hipLaunchKernelGGL((k_pc_norm<T>), dim3(tl.ntile), dim3(kBlock), 0, s, tl, V, static_cast<const T*>(nullptr),
                          w.partial, w.st);
     } else {
@@ -1094,7 +1169,7 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
     if (lu) {
       hipLaunchKernelGGL((k_residual_pc<T, X>), dim3(tl.ntile), dim3(kBlock), 0, s, Ares, tl, b, x,
                          static_cast<const X*>(nullptr), tmp, w.partial, gz_c ? 1 : 0, w.st);
-      ilu_apply(A, *w.ilu, lu, tmp, V, ilu_st, s, w.graphs);
+      ilu_apply(A, *w.ilu, lu, tmp, V, ilu_st, s, w.graphs, &w);
       hipLaunchKernelGGL((k_pc_norm<T>), dim3(tl.ntile), dim3(kBlock), 0, s, tl, V, static_cast<const T*>(nullptr),
                          w.partial, w.st);
     } else {
@@ -1115,7 +1190,7 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
       launch_spmv<T>(L, w.nt, A, sell, tl, V + it * ld, inv, V + (it + 1) * ld, w.st, it, s);
     }
     if (rec) S355_HIP(hipEventRecord(e1, s));
-    if (lu) ilu_apply(A, *w.ilu, lu, tmp, V + (it + 1) * ld, ilu_st, s, w.graphs);
+    if (lu) ilu_apply(A, *w.ilu, lu, tmp, V + (it + 1) * ld, ilu_st, s, w.graphs, &w);
     Dispatch<T>::mdot(R, V, ld, tl, it, w.partial, pstride, w.st, s);
     hipLaunchKernelGGL(k_mdot_final, dim3(it + 1, nc), dim3(kBlock), 0, s, w.partial, tl, pstride, w.st);
     Dispatch<T>::maxpy(R, V, ld, tl, it, w.partial, w.st, s);
@@ -1201,7 +1276,7 @@ KResult cg(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, bool z
   } else if (o.pc == S355_PC_ILU0) {
     if (!w.ilu || !w.lu) throw std::invalid_argument("cg: ILU(0) workspace missing");
     lu = static_cast<T*>(w.lu);
-    ilu_factor(A, A.nnz, *w.ilu, lu, s);
+    ilu_factor(A, A.nnz, *w.ilu, lu, s, &w);
   } else if (o.pc != S355_PC_NONE) {
     throw std::invalid_argument("cg: unknown preconditioner");
   }
@@ -1211,7 +1286,7 @@ KResult cg(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, bool z
     throw std::runtime_error("cg: partial buffer too small");
   if (!zero_guess) {
     if (lu) {
-      ilu_apply(A, *w.ilu, lu, b, z, w.st, s);
+      ilu_apply(A, *w.ilu, lu, b, z, w.st, s, nullptr, &w);
       hipLaunchKernelGGL((k_pc_norm<T>), dim3(g1), dim3(kBlock), 0, s, t1, z, static_cast<const T*>(nullptr),
                          w.partial, w.st);
     } else {
@@ -1221,7 +1296,7 @@ KResult cg(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, bool z
   }
   hipLaunchKernelGGL((k_cg_start<T>), dim3(g1), dim3(kBlock), 0, s, A, b, x, inv, r, z, w.partial, zero_guess ? 1 : 0);
   if (lu) {
-    ilu_apply(A, *w.ilu, lu, r, z, w.st, s, w.graphs);
+    ilu_apply(A, *w.ilu, lu, r, z, w.st, s, w.graphs, &w);
     hipLaunchKernelGGL((k_dot2<T>), dim3(g1), dim3(kBlock), 0, s, n, z, r, w.partial, w.st);
   }
   hipLaunchKernelGGL(k_cg_start_final, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st, zero_guess ? 1 : 0);
@@ -1235,7 +1310,7 @@ KResult cg(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, bool z
     hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st, i);
     hipLaunchKernelGGL((k_cg_update<T>), dim3(g1), dim3(kBlock), 0, s, n, x, r, z, p, wv, inv, w.partial, w.st);
     if (lu) {
-      ilu_apply(A, *w.ilu, lu, r, z, w.st, s, w.graphs);
+      ilu_apply(A, *w.ilu, lu, r, z, w.st, s, w.graphs, &w);
       hipLaunchKernelGGL((k_dot2<T>), dim3(g1), dim3(kBlock), 0, s, n, z, r, w.partial, w.st);
     }
     hipLaunchKernelGGL(k_cg_final, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st);
@@ -1729,6 +1804,46 @@ void jacobi_inverse(const CsrView<T>& A, T* inv, hipStream_t s) {
 }
 template void jacobi_inverse<double>(const CsrView<double>&, double*, hipStream_t);
 template void jacobi_inverse<float>(const CsrView<float>&, float*, hipStream_t);
+
+void ilu_ell(int n, const int* rowptr, const int* col, const int* diag, const std::vector<int>& rows,
+             const std::vector<int>& ptr, bool upper, std::vector<long>& off, std::vector<int>& ecol,
+             std::vector<int>& esrc, std::vector<int>& len, std::vector<int>& dsrc) {
+  const int nlev = static_cast<int>(ptr.size()) - 1;
+  auto first = [&](int i) { return upper ? diag[i] + 1 : rowptr[i]; };
+  auto last = [&](int i) { return upper ? rowptr[i + 1] : diag[i]; };
+  off.assign(nlev + 1, 0);
+  len.assign(n, 0);
+  if (upper) dsrc.assign(n, 0);
+  for (int l = 0; l < nlev; ++l) {
+    int w = 0;
+    for (int k = ptr[l]; k < ptr[l + 1]; ++k) {
+      const int i = rows[k];
+      len[k] = last(i) - first(i);
+      w = std::max(w, len[k]);
+      if (upper) dsrc[k] = diag[i];
+    }
+    off[l + 1] = off[l] + static_cast<long>(w) * (ptr[l + 1] - ptr[l]);
+  }
+  if (off[nlev] > 0x7fffffffL) throw std::invalid_argument("ILU level-ELL copy above 32-bit indices");
+  ecol.assign(off[nlev], 0);
+  esrc.assign(off[nlev], -1);
+  for (int l = 0; l < nlev; ++l) {
+    const int nr = ptr[l + 1] - ptr[l];
+    for (int k = 0; k < nr; ++k) {
+      const int i = rows[ptr[l] + k];
+      const int w = static_cast<int>((off[l + 1] - off[l]) / std::max(nr, 1));
+      for (int j = 0; j < w; ++j) {
+        const size_t q = static_cast<size_t>(off[l]) + static_cast<size_t>(j) * nr + k;
+        if (j < len[ptr[l] + k]) {
+          ecol[q] = col[first(i) + j];
+          esrc[q] = first(i) + j;
+        } else {
+          ecol[q] = i;
+        }
+      }
+    }
+  }
+}
 
 void ilu_levels(int n, const int* rowptr, const int* col, const int* diag, std::vector<int>& lrows,
                 std::vector<int>& lptr, std::vector<int>& urows, std::vector<int>& uptr) {

@@ -91,10 +91,26 @@ void sell_fill(const int* src, size_t nslots, const S* csr_val, T* out, hipStrea
 // Level schedule of the CSR pattern for ILU(0) (natural ordering): rows of a
 // level depend only on rows of earlier levels.  Lower levels order the
 // forward sweep and the factorisation, upper levels the backward sweep.
+//
+// The sweeps read the factors from level-ELL copies: the strictly lower
+// (upper) entries of the rows of one level are stored column-major, entry j
+// of the level's k-th row at off[l] + j * nrows_l + k, so that a wave's loads
+// of values and column ids are contiguous; len gives each row's count (rows
+// keep their CSR entry order, so the sums equal those of a CSR row walk).
+struct IluEll {
+  std::vector<long> off;        // host: nlev + 1 slot offsets
+  const int* col = nullptr;     // device: column of each slot
+  const int* src = nullptr;     // device: CSR position of each slot (-1 padding)
+  const int* len = nullptr;     // device: entries of each row, by position in the level lists
+  const int* dsrc = nullptr;    // device (upper only): CSR position of each row's diagonal
+  size_t slots = 0;
+};
+
 struct IluLevels {
   std::vector<int> lptr, uptr;  // host: level boundaries into lrows / urows
   const int* lrows = nullptr;   // device: rows grouped by lower level
   const int* urows = nullptr;   // device: rows grouped by upper level
+  IluEll lell, uell;
   int nl() const { return static_cast<int>(lptr.size()) - 1; }
   int nu() const { return static_cast<int>(uptr.size()) - 1; }
 };
@@ -102,6 +118,12 @@ struct IluLevels {
 // Host computation of the level schedule.
 void ilu_levels(int n, const int* rowptr, const int* col, const int* diag, std::vector<int>& lrows,
                 std::vector<int>& lptr, std::vector<int>& urows, std::vector<int>& uptr);
+
+// Host construction of one level-ELL copy (upper: the strictly upper part and
+// the diagonal positions; else the strictly lower part).
+void ilu_ell(int n, const int* rowptr, const int* col, const int* diag, const std::vector<int>& rows,
+             const std::vector<int>& ptr, bool upper, std::vector<long>& off, std::vector<int>& ecol,
+             std::vector<int>& esrc, std::vector<int>& len, std::vector<int>& dsrc);
 
 struct KResult {
   int iters = 0, reason = 0;
@@ -155,6 +177,9 @@ struct KWork {
   void* r = nullptr;        // n elements of T: preconditioner input (CG: r, z, p, w use V[0..3])
   void* pcinv = nullptr;    // n elements of T (Jacobi inverse diagonal)
   void* lu = nullptr;       // nnz elements of T (ILU(0) factors), when ilu is set
+  void* ilu_lval = nullptr; // level-ELL factor values of T: lower slots, upper slots, diagonal per row
+  void* ilu_uval = nullptr;
+  void* ilu_dval = nullptr;
   const IluLevels* ilu = nullptr;
   double* partial = nullptr;
   size_t partial_len = 0;
