@@ -329,9 +329,10 @@ int s355_copy_time(s355_ctx* ctx, size_t bytes, int repeats, double* ms_per_call
 /* Kernel shapes in use for the GMRES solves of each precision (index 0 FP64,
  * 1 FP32): lanes[k] SpMV lanes per row, nt[k] 1 when the SpMV loads the
  * matrix non-temporally, rows[k] rows per thread of the multi-dot /
- * Gram-Schmidt / update kernels, tuned[k] 1 when they came from the
- * autotuner.  No reference counterpart. */
-int s355_kernel_shapes(s355_ctx* ctx, int lanes[2], int nt[2], int rows[2], int tuned[2]);
+ * Gram-Schmidt / update kernels, sigma[k] the row window the SELL-64 copy
+ * is sorted in (used when lanes[k] is 0), tuned[k] 1 when they came from
+ * the autotuner.  No reference counterpart. */
+int s355_kernel_shapes(s355_ctx* ctx, int lanes[2], int nt[2], int rows[2], int sigma[2], int tuned[2]);
 
 /* Times the production SpMV launches inside the GMRES solves of one
  * precision (0 FP64, 1 FP32): every 4th Arnoldi SpMV is bracketed by HIP
@@ -343,15 +344,17 @@ int s355_spmv_profile(s355_ctx* ctx, int precision, int mode, double* avg_ms, in
 
 /* Sets the kernel shapes of one precision's GMRES solves explicitly (the
  * autotuner then leaves them alone): lanes 1, 2, 4, 8, 16 (CSR SpMV) or 0
- * (SELL-64 SpMV), nt 0/1, rows 1, 2, 4, 8, 16 or 0 (by size).  No reference
- * counterpart. */
-int s355_set_kernel_shapes(s355_ctx* ctx, int precision, int lanes, int nt, int rows);
+ * (SELL-64 SpMV), nt 0/1, rows 1, 2, 4, 8, 16 or 0 (by size), sigma the
+ * window of rows the SELL-64 copy sorts by length: 64, 512, 4096 or 0 (4096).
+ * No reference counterpart. */
+int s355_set_kernel_shapes(s355_ctx* ctx, int precision, int lanes, int nt, int rows, int sigma);
 
 /* The autotuner's timings for one precision (0 FP64, 1 FP32): spmv_ms[v] for
- * v = 2 k + nt, k = 0..4 for CSR at 1, 2, 4, 8, 16 lanes per row, k = 5 for
- * SELL-64; gs_ms[r] for 2^r rows per thread (one multi-dot + Gram-Schmidt
- * step at iteration *gs_it); -1 where not timed.  No reference counterpart. */
-int s355_tune_report(s355_ctx* ctx, int precision, double spmv_ms[12], double gs_ms[5], int* gs_it);
+ * v = 2 k + nt, k = 0..4 for CSR at 1, 2, 4, 8, 16 lanes per row, k = 5, 6, 7
+ * for SELL-64 sorted in windows of 64, 512, 4096 rows; gs_ms[r] for 2^r rows
+ * per thread (one multi-dot + Gram-Schmidt step at iteration *gs_it); -1
+ * where not timed.  No reference counterpart. */
+int s355_tune_report(s355_ctx* ctx, int precision, double spmv_ms[16], double gs_ms[5], int* gs_it);
 
 /* Host-side set-up only (no device): the integer and metric output of the
  * face builder, for inspection and for tests on machines without a GPU. */

@@ -105,9 +105,9 @@ def library(path: Optional[Path] = None) -> C.CDLL:
         "spmv_time": (I, [V, I, I, I, pD]),
         "last_solve_info": (I, [V, C.POINTER(SolveInfo)]),
         "copy_time": (I, [V, C.c_size_t, I, pD]),
-        "kernel_shapes": (I, [V, pI, pI, pI, pI]),
+        "kernel_shapes": (I, [V, pI, pI, pI, pI, pI]),
         "tune_report": (I, [V, I, pD, pD, pI]),
-        "set_kernel_shapes": (I, [V, I, I, I, I]),
+        "set_kernel_shapes": (I, [V, I, I, I, I, I]),
         "spmv_profile": (I, [V, I, I, pD, pI]),
     }
     for name, (res, args) in more.items():
@@ -408,19 +408,19 @@ class Solver:
         return ms.value
 
     def kernel_shapes(self) -> dict:
-        """SpMV lanes per row / non-temporal loads and Gram-Schmidt rows per
-        thread, FP64 and FP32."""
-        lanes, nt, rows, tuned = (C.c_int * 2)(), (C.c_int * 2)(), (C.c_int * 2)(), (C.c_int * 2)()
-        _check(self.lib, self.lib.s355_kernel_shapes(self.ctx, lanes, nt, rows, tuned), "kernel_shapes")
-        return {p: {"lanes": lanes[k], "nt": nt[k], "rows": rows[k], "tuned": bool(tuned[k])}
+        """SpMV lanes per row (0 = SELL-64) / non-temporal loads / SELL
+        sorting window and Gram-Schmidt rows per thread, FP64 and FP32."""
+        lanes, nt, rows, sigma, tuned = ((C.c_int * 2)() for _ in range(5))
+        _check(self.lib, self.lib.s355_kernel_shapes(self.ctx, lanes, nt, rows, sigma, tuned), "kernel_shapes")
+        return {p: {"lanes": lanes[k], "nt": nt[k], "rows": rows[k], "sigma": sigma[k], "tuned": bool(tuned[k])}
                 for k, p in enumerate(("fp64", "fp32"))}
 
     def tune_report(self, precision: int) -> dict:
         """Autotuner timings (ms): SpMV per (lanes, nt), Gram-Schmidt step per rows."""
-        sp, gs, it = (C.c_double * 12)(), (C.c_double * 5)(), C.c_int()
+        sp, gs, it = (C.c_double * 16)(), (C.c_double * 5)(), C.c_int()
         _check(self.lib, self.lib.s355_tune_report(self.ctx, precision, sp, gs, C.byref(it)), "tune_report")
-        names = ["L1", "L2", "L4", "L8", "L16", "SELL64"]
-        spmv = {f"{names[v // 2]}{'_nt' if v % 2 else ''}": sp[v] for v in range(12) if sp[v] >= 0}
+        names = ["L1", "L2", "L4", "L8", "L16", "SELL64w64", "SELL64w512", "SELL64w4096"]
+        spmv = {f"{names[v // 2]}{'_nt' if v % 2 else ''}": sp[v] for v in range(16) if sp[v] >= 0}
         return {"spmv_ms": spmv, "gs_ms": {f"R{1 << r}": gs[r] for r in range(5) if gs[r] >= 0}, "gs_it": it.value}
 
     def spmv_profile(self, precision: int, mode: int) -> tuple:
@@ -429,9 +429,11 @@ class Solver:
         _check(self.lib, self.lib.s355_spmv_profile(self.ctx, precision, mode, C.byref(ms), C.byref(n)), "spmv_profile")
         return ms.value, n.value
 
-    def set_kernel_shapes(self, precision: int, lanes: int, nt: int = 0, rows: int = 0) -> None:
-        """Force the GMRES kernel shapes of one precision (lanes 0 = SELL-64)."""
-        _check(self.lib, self.lib.s355_set_kernel_shapes(self.ctx, precision, lanes, nt, rows), "set_kernel_shapes")
+    def set_kernel_shapes(self, precision: int, lanes: int, nt: int = 0, rows: int = 0, sigma: int = 0) -> None:
+        """Force the GMRES kernel shapes of one precision (lanes 0 = SELL-64
+        sorted in windows of sigma rows: 64, 512, 4096; 0 = 4096)."""
+        _check(self.lib, self.lib.s355_set_kernel_shapes(self.ctx, precision, lanes, nt, rows, sigma),
+               "set_kernel_shapes")
 
     def copy_time(self, nbytes: int, repeats: int) -> float:
         """Mean device ms of a plain copy of nbytes (multiple of 16)."""

@@ -679,10 +679,10 @@ int s355_spmv_time(s355_ctx* ctx, int which, int precision, int repeats, double*
   });
 }
 
-int s355_kernel_shapes(s355_ctx* ctx, int lanes[2], int nt[2], int rows[2], int tuned[2]) {
+int s355_kernel_shapes(s355_ctx* ctx, int lanes[2], int nt[2], int rows[2], int sigma[2], int tuned[2]) {
   return guarded([&] {
-    if (!lanes || !nt || !rows || !tuned) throw std::invalid_argument("null output");
-    dev(ctx).kernel_shapes(lanes, nt, rows, tuned);
+    if (!lanes || !nt || !rows || !sigma || !tuned) throw std::invalid_argument("null output");
+    dev(ctx).kernel_shapes(lanes, nt, rows, sigma, tuned);
     return S355_OK;
   });
 }
@@ -695,18 +695,18 @@ int s355_spmv_profile(s355_ctx* ctx, int precision, int mode, double* avg_ms, in
   });
 }
 
-int s355_set_kernel_shapes(s355_ctx* ctx, int precision, int lanes, int nt, int rows) {
+int s355_set_kernel_shapes(s355_ctx* ctx, int precision, int lanes, int nt, int rows, int sigma) {
   return guarded([&] {
-    dev(ctx).set_kernel_shapes(precision, lanes, nt, rows);
+    dev(ctx).set_kernel_shapes(precision, lanes, nt, rows, sigma);
     return S355_OK;
   });
 }
 
-int s355_tune_report(s355_ctx* ctx, int precision, double spmv_ms[6], double gs_ms[5], int* gs_it) {
+int s355_tune_report(s355_ctx* ctx, int precision, double spmv_ms[16], double gs_ms[5], int* gs_it) {
   return guarded([&] {
     if (!spmv_ms || !gs_ms || !gs_it || precision < 0 || precision > 1) throw std::invalid_argument("tune_report");
     const s355::TuneReport& r = dev(ctx).tune_report(precision);
-    for (int k = 0; k < 12; ++k) spmv_ms[k] = r.spmv_ms[k];
+    for (int k = 0; k < 16; ++k) spmv_ms[k] = r.spmv_ms[k];
     for (int k = 0; k < 5; ++k) gs_ms[k] = r.gs_ms[k];
     *gs_it = r.gs_it;
     return S355_OK;

@@ -503,13 +503,13 @@ int Context::run_solve(int which, const double* b, double* x, const s355_ksp_opt
     CsrView<float> A{dm.Nc, dm.rowptr, dm.col, dm.diag, a32_, dm.nnz};
     CsrView<double> A64{dm.Nc, dm.rowptr, dm.col, dm.diag, val, dm.nnz};
     const bool tune = par.autotune && o.type == S355_KSP_GMRES && !kwork32_.tuned;
-    if (o.type == S355_KSP_GMRES && (tune || kwork32_.lanes == 0)) prepare_sell(val, true);
+    if (o.type == S355_KSP_GMRES && (tune || kwork32_.lanes == 0)) prepare_sell(val, true, tune);
     if (tune) autotune<float>(A, kwork32_, std::min(o.restart, kMaxRestart), stream);
     r = ir_solve(A64, A, b, x, o, zero_guess, par.ir_max, par.ir_inner_rtol, kwork32_, ir_, stream);
   } else {
     CsrView<double> A{dm.Nc, dm.rowptr, dm.col, dm.diag, val, dm.nnz};
     const bool tune = par.autotune && o.type == S355_KSP_GMRES && !kwork_.tuned;
-    if (o.type == S355_KSP_GMRES && (tune || kwork_.lanes == 0)) prepare_sell(val, false);
+    if (o.type == S355_KSP_GMRES && (tune || kwork_.lanes == 0)) prepare_sell(val, false, tune);
     if (tune) autotune<double>(A, kwork_, std::min(o.restart, kMaxRestart), stream);
     r = krylov_solve<double>(A, b, x, o, zero_guess, kwork_, stream);
   }
@@ -663,30 +663,39 @@ void Context::check_flags() {
 }
 
 void Context::ensure_sell() {
-  if (sell_src_) return;
-  std::vector<int> ptr, row, col, src;
-  sell_pattern(dm.Nc, mesh.rowptr.data(), mesh.col.data(), kSellSigma, ptr, row, col, src);
-  sell_slots_ = src.size();
-  const int* dptr = upload(allocs_, ptr, stream);
-  const int* drow = upload(allocs_, row, stream);
-  const int* dcol = upload(allocs_, col, stream);
-  sell_src_ = upload(allocs_, src, stream);
-  sync();  // host staging vectors go out of scope
-  kwork_.sell_ptr = kwork32_.sell_ptr = dptr;
-  kwork_.sell_row = kwork32_.sell_row = drow;
-  kwork_.sell_col = kwork32_.sell_col = dcol;
+  if (have_sell_) return;
+  for (int v = 0; v < kSellVariants; ++v) {
+    std::vector<int> ptr, row, col, src;
+    sell_pattern(dm.Nc, mesh.rowptr.data(), mesh.col.data(), kSellSigma[v], ptr, row, col, src);
+    SellPat pt;
+    pt.slots = src.size();
+    pt.ptr = upload(allocs_, ptr, stream);
+    pt.row = upload(allocs_, row, stream);
+    pt.col = upload(allocs_, col, stream);
+    pt.src = upload(allocs_, src, stream);
+    sync();  // host staging vectors go out of scope
+    kwork_.sell[v] = kwork32_.sell[v] = pt;
+  }
+  have_sell_ = true;
 }
 
-void Context::prepare_sell(const double* csr_val, bool fp32) {
+// Values of the SELL copies from the CSR values: of the window in use, or of
+// every window (before the autotuner times them).
+void Context::prepare_sell(const double* csr_val, bool fp32, bool all) {
   ensure_sell();
-  if (fp32) {
-    if (!sell32_) sell32_ = alloc<float>(sell_slots_);
-    sell_fill<float, double>(sell_src_, sell_slots_, csr_val, sell32_, stream);
-    kwork32_.sell_val = sell32_;
-  } else {
-    if (!sell64_) sell64_ = alloc<double>(sell_slots_);
-    sell_fill<double, double>(sell_src_, sell_slots_, csr_val, sell64_, stream);
-    kwork_.sell_val = sell64_;
+  KWork& w = fp32 ? kwork32_ : kwork_;
+  for (int v = 0; v < kSellVariants; ++v) {
+    if (!all && v != w.sell_v) continue;
+    const SellPat& pt = w.sell[v];
+    if (fp32) {
+      if (!sell32_[v]) sell32_[v] = alloc<float>(pt.slots);
+      sell_fill<float, double>(pt.src, pt.slots, csr_val, sell32_[v], stream);
+      w.sell_vals[v] = sell32_[v];
+    } else {
+      if (!sell64_[v]) sell64_[v] = alloc<double>(pt.slots);
+      sell_fill<double, double>(pt.src, pt.slots, csr_val, sell64_[v], stream);
+      w.sell_vals[v] = sell64_[v];
+    }
   }
 }
 
@@ -702,7 +711,7 @@ double Context::time_spmv(int which, int precision, int repeats) {
   try {
     if (precision == 0) {
       CsrView<double> A{dm.Nc, dm.rowptr, dm.col, dm.diag, val, dm.nnz};
-      prepare_sell(val, false);
+      prepare_sell(val, false, par.autotune && !kwork_.tuned);
       if (par.autotune && !kwork_.tuned) autotune<double>(A, kwork_, m, stream);
       double* inv = static_cast<double*>(kwork_.pcinv);
       jacobi_inverse<double>(A, inv, stream);
@@ -712,7 +721,7 @@ double Context::time_spmv(int which, int precision, int repeats) {
       ensure_mixed();
       convert_to_float(val, a32_, dm.nnz, stream);
       CsrView<float> A{dm.Nc, dm.rowptr, dm.col, dm.diag, a32_, dm.nnz};
-      prepare_sell(val, true);
+      prepare_sell(val, true, par.autotune && !kwork32_.tuned);
       if (par.autotune && !kwork32_.tuned) autotune<float>(A, kwork32_, m, stream);
       float* inv = static_cast<float*>(kwork32_.pcinv);
       jacobi_inverse<float>(A, inv, stream);
@@ -752,29 +761,34 @@ double Context::spmv_profile(int precision, int mode, int* count) {
   return avg;
 }
 
-void Context::set_kernel_shapes(int precision, int lanes, int nt, int rows) {
+void Context::set_kernel_shapes(int precision, int lanes, int nt, int rows, int sigma) {
   auto ok = [](int v, std::initializer_list<int> set) {
     for (int x : set)
       if (v == x) return true;
     return false;
   };
   if (precision < 0 || precision > 1 || !ok(lanes, {0, 1, 2, 4, 8, 16}) || !ok(nt, {0, 1}) ||
-      !ok(rows, {0, 1, 2, 4, 8, 16}))
-    throw std::invalid_argument("kernel shapes: lanes 0 (SELL-64), 1, 2, 4, 8, 16; nt 0/1; rows 0, 1, 2, 4, 8, 16");
+      !ok(rows, {0, 1, 2, 4, 8, 16}) || !ok(sigma, {0, kSellSigma[0], kSellSigma[1], kSellSigma[2]}))
+    throw std::invalid_argument(
+        "kernel shapes: lanes 0 (SELL-64), 1, 2, 4, 8, 16; nt 0/1; rows 0, 1, 2, 4, 8, 16; sigma 0, 64, 512, 4096");
   KWork& w = precision ? kwork32_ : kwork_;
   w.lanes = lanes;
   w.nt = nt;
   w.rows = rows;
+  w.sell_v = kSellDefault;
+  for (int v = 0; v < kSellVariants; ++v)
+    if (sigma == kSellSigma[v]) w.sell_v = v;
   w.tuned = true;  // the autotuner leaves explicit shapes alone
 }
 
-void Context::kernel_shapes(int lanes[2], int nt[2], int rows[2], int tuned[2]) {
+void Context::kernel_shapes(int lanes[2], int nt[2], int rows[2], int sigma[2], int tuned[2]) {
   const KWork* w[2] = {&kwork_, &kwork32_};
   const size_t elem[2] = {sizeof(double), sizeof(float)};
   for (int k = 0; k < 2; ++k) {
     lanes[k] = w[k]->lanes;
     nt[k] = w[k]->nt;
     rows[k] = w[k]->rows ? w[k]->rows : pick_rows(dm.Nc, elem[k]);
+    sigma[k] = kSellSigma[w[k]->sell_v];
     tuned[k] = w[k]->tuned ? 1 : 0;
   }
 }

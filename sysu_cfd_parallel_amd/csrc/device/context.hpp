@@ -81,8 +81,8 @@ class Context {
   // benchmark helper
   double time_spmv(int which, int precision, int repeats);
   double time_copy(size_t bytes, int repeats);
-  void kernel_shapes(int lanes[2], int nt[2], int rows[2], int tuned[2]);
-  void set_kernel_shapes(int precision, int lanes, int nt, int rows);
+  void kernel_shapes(int lanes[2], int nt[2], int rows[2], int sigma[2], int tuned[2]);
+  void set_kernel_shapes(int precision, int lanes, int nt, int rows, int sigma);
   // mode 1: start timing the Arnoldi SpMV launches of one precision (reset),
   // 0: read, -1: read and stop; returns the mean ms and the launches timed
   double spmv_profile(int precision, int mode, int* count);
@@ -100,7 +100,7 @@ class Context {
   void ensure_mixed();
   void ensure_ilu();
   void ensure_sell();
-  void prepare_sell(const double* csr_val, bool fp32);
+  void prepare_sell(const double* csr_val, bool fp32, bool all);
 
   std::vector<void*> allocs_;
   double* partial_ = nullptr;
@@ -120,12 +120,12 @@ class Context {
   // ILU(0): level schedule of the pattern and factor storage
   std::unique_ptr<GraphCache> graphs_;  // captured GMRES cycles (params.graphs)
   std::unique_ptr<SpmvTimer> timers_[2];  // in-solve SpMV timing, FP64 / FP32
-  // SELL-64 copy of the matrix being solved: pattern and source map built once,
-  // values filled before each GMRES solve that uses it
-  const int* sell_src_ = nullptr;
-  size_t sell_slots_ = 0;
-  double* sell64_ = nullptr;
-  float* sell32_ = nullptr;
+  // SELL-64 copies of the matrix being solved, one per sorting window: patterns
+  // and source maps built once, values filled before each GMRES solve that
+  // uses them
+  bool have_sell_ = false;
+  double* sell64_[kSellVariants] = {nullptr, nullptr, nullptr};
+  float* sell32_[kSellVariants] = {nullptr, nullptr, nullptr};
   IluLevels levels_{};
   bool have_levels_ = false;
   void* host_pinned_ = nullptr;

@@ -306,10 +306,10 @@ __global__ void __launch_bounds__(kBlock) k_sell_fill(const int* __restrict__ sr
 }
 
 // SpMV variants: v = 2 k + nt, k = 0..4 CSR with 1, 2, 4, 8, 16 lanes per row,
-// k = 5 SELL-64 (lanes = 0)
-constexpr int kSpmvVariants = 12;
-inline int spmv_variant(int lanes, int nt) {
-  const int k = lanes == 0 ? 5 : lanes == 1 ? 0 : lanes == 2 ? 1 : lanes == 4 ? 2 : lanes == 8 ? 3 : 4;
+// k = 5 + s SELL-64 with window kSellSigma[s] (lanes = 0)
+constexpr int kSpmvVariants = 16;
+inline int spmv_variant(int lanes, int nt, int sell_v) {
+  const int k = lanes == 0 ? 5 + sell_v : lanes == 1 ? 0 : lanes == 2 ? 1 : lanes == 4 ? 2 : lanes == 8 ? 3 : 4;
   return 2 * k + (nt ? 1 : 0);
 }
 
@@ -347,14 +347,15 @@ void launch_spmv(int L, int nt, const CsrView<T>& A, const SellView<T>& sell, co
 }
 
 template <class T>
-SellView<T> sell_of(const KWork& w, int n) {
+SellView<T> sell_of(const KWork& w, int n, int variant = -1) {
+  const int sv = variant < 0 ? w.sell_v : variant;
   SellView<T> v;
-  if (w.sell_ptr) {
+  if (w.sell[sv].ptr && w.sell_vals[sv]) {
     v.n = n;
-    v.ptr = w.sell_ptr;
-    v.row = w.sell_row;
-    v.col = w.sell_col;
-    v.val = static_cast<const T*>(w.sell_val);
+    v.ptr = w.sell[sv].ptr;
+    v.row = w.sell[sv].row;
+    v.col = w.sell[sv].col;
+    v.val = static_cast<const T*>(w.sell_vals[sv]);
   }
   return v;
 }
@@ -1214,7 +1215,8 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
                                           reinterpret_cast<uintptr_t>(b), reinterpret_cast<uintptr_t>(x),
                                           reinterpret_cast<uintptr_t>(inv), reinterpret_cast<uintptr_t>(invr),
                                           static_cast<uintptr_t>(m), static_cast<uintptr_t>(R),
-                                          static_cast<uintptr_t>(L), static_cast<uintptr_t>(w.nt), sizeof(T),
+                                          static_cast<uintptr_t>(L), static_cast<uintptr_t>(w.nt),
+                                          static_cast<uintptr_t>(w.sell_v), sizeof(T),
                                           sizeof(X),
                                           static_cast<uintptr_t>(first_c), static_cast<uintptr_t>(zero_guess)};
       hipGraphExec_t g = w.graphs->get(key, s, [&] {
@@ -1459,7 +1461,7 @@ struct TuneKey {
 };
 std::mutex g_tune_mu;
 struct TuneResult {
-  int lanes, nt, rows;
+  int lanes, nt, rows, sell_v;
   TuneReport report;
 };
 std::map<TuneKey, TuneResult> g_tuned;
@@ -1492,6 +1494,7 @@ void autotune(const CsrView<T>& A, KWork& w, int m, hipStream_t s) {
       w.lanes = it->second.lanes;
       w.nt = it->second.nt;
       w.rows = it->second.rows;
+      w.sell_v = it->second.sell_v;
       w.report = it->second.report;
       w.tuned = true;
       return;
@@ -1514,17 +1517,27 @@ void autotune(const CsrView<T>& A, KWork& w, int m, hipStream_t s) {
   TuneReport rep;
   int best_l = 8, best_nt = 0;
   double best = 1e300;
-  const SellView<T> sell = sell_of<T>(w, n);
+  int best_sv = w.sell_v;
   for (int L : {1, 2, 4, 8, 16, 0})
-    for (int nt : {0, 1}) {
+    for (int sv = 0; sv < (L == 0 ? kSellVariants : 1); ++sv) {
+      const SellView<T> sell = sell_of<T>(w, n, sv);
       if (L == 0 && !sell.ptr) continue;
-      const double ms =
-          time_ms([&] { launch_spmv<T>(L, nt, A, sell, w.tiles[0], V, none, V + ld, w.st, 0, s); }, 5, s);
-      rep.spmv_ms[spmv_variant(L, nt)] = ms;
-      if (ms < best) {
-        best = ms;
-        best_l = L;
-        best_nt = nt;
+      if (L == 0) {
+        // values of this window's copy from the CSR matrix being tuned on
+        const SellPat& pt = w.sell[sv];
+        hipLaunchKernelGGL((k_gather<T>), dim3(grid_for(static_cast<long>(pt.slots))), dim3(kBlock), 0, s, pt.src,
+                           pt.slots, A.val, static_cast<T*>(w.sell_vals[sv]));
+      }
+      for (int nt : {0, 1}) {
+        const double ms =
+            time_ms([&] { launch_spmv<T>(L, nt, A, sell, w.tiles[0], V, none, V + ld, w.st, 0, s); }, 5, s);
+        rep.spmv_ms[spmv_variant(L, nt, sv)] = ms;
+        if (ms < best) {
+          best = ms;
+          best_l = L;
+          best_nt = nt;
+          best_sv = sv;
+        }
       }
     }
   // one mid-cycle Arnoldi step of the Gram-Schmidt pair at every tile size
@@ -1550,10 +1563,11 @@ void autotune(const CsrView<T>& A, KWork& w, int m, hipStream_t s) {
   w.lanes = best_l;
   w.nt = best_nt;
   w.rows = best_r;
+  w.sell_v = best_sv;
   w.report = rep;
   w.tuned = true;
   std::lock_guard<std::mutex> g(g_tune_mu);
-  g_tuned[key] = TuneResult{best_l, best_nt, best_r, rep};
+  g_tuned[key] = TuneResult{best_l, best_nt, best_r, best_sv, rep};
 }
 
 int pick_rows(int n, size_t elem) { return pick_r(n, elem); }

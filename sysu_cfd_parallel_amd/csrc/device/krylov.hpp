@@ -77,7 +77,21 @@ struct SellView {
   const T* val = nullptr;
 };
 
-constexpr int kSellSigma = 2048;
+// SELL windows the autotuner chooses from: a short window keeps the rows of
+// a slice (and so their gathers of x) close together, a long one removes the
+// padding of meshes mixing row lengths.
+constexpr int kSellVariants = 3;
+constexpr int kSellSigma[kSellVariants] = {64, 512, 4096};
+constexpr int kSellDefault = 2;
+
+// one SELL pattern on the device (values live with each precision's workspace)
+struct SellPat {
+  const int* ptr = nullptr;
+  const int* row = nullptr;
+  const int* col = nullptr;
+  const int* src = nullptr;
+  size_t slots = 0;
+};
 
 // Host construction of the pattern: ptr, row, col and, for every SELL slot,
 // the CSR position it copies (-1 for padding).
@@ -140,12 +154,13 @@ struct GraphCache {
 };
 
 // Timings taken by autotune(): SpMV per variant v = 2 k + nt (k = 0..4: CSR
-// with 1, 2, 4, 8, 16 lanes per row, k = 5: SELL-64; nt: non-temporal matrix
-// loads) and one Gram-Schmidt step (multi-dot + update at iteration gs_it)
+// with 1, 2, 4, 8, 16 lanes per row, k = 5..7: SELL-64 with window 64, 512,
+// 4096; nt: non-temporal matrix loads) and one Gram-Schmidt step (multi-dot + update at iteration gs_it)
 // per rows-per-thread 1, 2, 4, 8, 16; -1 = not timed.
 struct TuneReport {
-  // CSR lanes 1, 2, 4, 8, 16 then SELL-64, each temporal / non-temporal
-  double spmv_ms[12] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+  // CSR lanes 1, 2, 4, 8, 16 then SELL-64 with windows 64, 512, 4096, each
+  // temporal / non-temporal
+  double spmv_ms[16] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
   double gs_ms[5] = {-1, -1, -1, -1, -1};
   int gs_it = 0;
 };
@@ -185,11 +200,10 @@ struct KWork {
   size_t partial_len = 0;
   int lanes = 8;            // CSR SpMV lanes per row (1, 2, 4, 8, 16); 0 = SELL-64 SpMV
   int nt = 0;               // SpMV loads the matrix non-temporally
-  // SELL-64 copy of the matrix being solved (values of type T), when set
-  const int* sell_ptr = nullptr;
-  const int* sell_row = nullptr;
-  const int* sell_col = nullptr;
-  const void* sell_val = nullptr;
+  int sell_v = kSellDefault;  // SELL window in use (index into kSellSigma) when lanes == 0
+  // SELL-64 copies of the matrix being solved: patterns, and values of type T
+  SellPat sell[kSellVariants];
+  void* sell_vals[kSellVariants] = {nullptr, nullptr, nullptr};
   int rows = 0;             // rows per thread of the Gram-Schmidt kernels; 0: by size
   bool tuned = false;       // lanes / nt / rows chosen by autotune()
   TuneReport report;

@@ -104,30 +104,33 @@ def test_autotuned_shapes_are_shared_within_a_process():
 
 @pytest.mark.parametrize("prec", [s.FP64, s.MIXED])
 def test_sell_spmv_equals_single_lane_csr(prec):
-    # both sum each row's entries in CSR order, one row per lane
+    # all sum each row's entries in CSR order, one row per lane, whatever
+    # window the SELL copy sorts its rows in
     case = s.channel(12, 6, 6, jitter=0.15, split=0.2, seed=8, re=600.0)
     k = 1 if prec == s.MIXED else 0
     out = []
-    for lanes in (1, 0):
+    for lanes, sigma in ((1, 0), (0, 64), (0, 512), (0, 4096)):
         d = s.Solver(case, base(prec))
-        d.set_kernel_shapes(k, lanes)
+        d.set_kernel_shapes(k, lanes, sigma=sigma)
         h, _ = d.iterate(5)
-        assert d.kernel_shapes()["fp32" if k else "fp64"]["lanes"] == lanes
+        shape = d.kernel_shapes()["fp32" if k else "fp64"]
+        assert shape["lanes"] == lanes and (lanes or shape["sigma"] == sigma)
         out.append(({n: d.get(n) for n in FIELDS}, h))
         d.close()
-    for n in FIELDS:
-        assert np.array_equal(out[0][0][n], out[1][0][n]), n
-    assert np.array_equal(out[0][1], out[1][1])
+    for o in out[1:]:
+        for n in FIELDS:
+            assert np.array_equal(out[0][0][n], o[0][n]), n
+        assert np.array_equal(out[0][1], o[1])
 
 
-@pytest.mark.parametrize("lanes", [0, 2, 16])
-def test_every_spmv_shape_matches_the_oracle(lanes):
+@pytest.mark.parametrize("lanes,sigma", [(0, 64), (0, 4096), (2, 0), (16, 0)])
+def test_every_spmv_shape_matches_the_oracle(lanes, sigma):
     from oracle.pyoracle import Oracle
 
     case = s.cavity(8, re=100.0, jitter=0.1, seed=1)
     p = params(rtol=1e-12)
     d = s.Solver(case, p)
-    d.set_kernel_shapes(0, lanes, nt=1 if lanes == 16 else 0, rows=2)
+    d.set_kernel_shapes(0, lanes, nt=1 if lanes == 16 else 0, rows=2, sigma=sigma)
     o = Oracle(case, p)
     hd, _ = d.iterate(10)
     ho, _, _ = o.iterate(10)
