@@ -9,8 +9,8 @@
  *   S355_NOT_CONVERGED (1)    a Krylov solve hit its cap or diverged; the
  *                             s355_solve_info records say where (this is the
  *                             reference's ConvergeError, BasicType.h:62-68)
- *   < 0                       bad argument, bad mesh, HIP error;
- *                             s355_last_error() holds the message.
+ *   < 0                       bad argument, bad mesh, HIP error, file
+ *                             error; s355_last_error() holds the message.
  * Calls on one context are made from one host thread; contexts are
  * independent.  Field arrays are in the reference's cell / face / boundary
  * order (geometry.cpp:222-358), so host arrays copy in and out unpermuted.
@@ -32,6 +32,7 @@ extern "C" {
 #define S355_E_MESH (-3)
 #define S355_E_STATE (-4)
 #define S355_E_NOMEM (-5)
+#define S355_E_IO (-6)
 
 #define S355_MAX_CELL_FACES 12
 
@@ -152,6 +153,7 @@ enum s355_mesh_int {
   S355_M_CSR_ROWPTR,    /* Ncel + 1 */
   S355_M_CSR_COL,       /* nnz */
   S355_M_CSR_DIAG,      /* Ncel */
+  S355_M_CELL_VERT,     /* 8 * Ncel: vertices of each cell as a degenerate hex (CellData::vertices) */
   S355_M_INT_COUNT
 };
 
@@ -167,6 +169,7 @@ enum s355_mesh_dbl {
   S355_G_FACE_XPAC,    /* 3 * Nfac */
   S355_G_FACE_XNAC,    /* 3 * Nfac */
   S355_G_BND_DIST,     /* Nbnd */
+  S355_G_VERT,         /* 3 * Nvrt: vertex coordinates after compaction */
   S355_G_COUNT
 };
 
@@ -366,6 +369,58 @@ int s355_mesh_sizes(const s355_mesh* m, s355_sizes* out);
 int s355_mesh_get_int(const s355_mesh* m, int which, int* out, size_t n);
 int s355_mesh_get_dbl(const s355_mesh* m, int which, double* out, size_t n);
 void s355_mesh_free(s355_mesh* m);
+
+/* --- result and restart files (navierIO.cpp), one partition ------------ */
+enum s355_output_format {
+  S355_OUT_TECPLOT = 0, /* Output2Tecplot (navierIO.cpp:161-246): Tecplot ASCII, tec/resNNNN.dat */
+  S355_OUT_TEC_DATA,    /* Output2TecplotBinary (navierIO.cpp:132-159): tec/data */
+  S355_OUT_TEC_GEO,     /* OutputGridBinary (navierIO.cpp:29-56): tec/geo */
+  S355_OUT_GRID,        /* OutputGrid (navierIO.cpp:58-122): tec/grid.dat */
+  S355_OUT_BACKUP,      /* WriteBackupFile (navierIO.cpp:370-386): res.sav */
+  S355_OUT_COUNT
+};
+
+/* Cell arrays of one result (length Ncel each); vistur, te, ed are used
+ * only when turbulent (vistur may be null for a laminar result). */
+typedef struct s355_result {
+  const double *R, *U, *V, *W, *P, *T, *vislam, *vistur, *te, *ed;
+  int turbulent;
+  int pad;
+} s355_result;
+
+/* Host only: writes one file of a host mesh and host arrays (the same
+ * writers the context uses).  The grid formats ignore r (may be null). */
+int s355_mesh_write_result(const s355_mesh* m, int format, const char* path, const s355_result* r);
+/* Host only: ReadBackupFile (navierIO.cpp:389-403) into out, laid out as
+ * R U V W P T VisLam (VisTur TE ED when turbulent), n_cell values each. */
+int s355_backup_read(const char* path, int n_cell, int turbulent, double* out);
+
+/* Starts a device-to-host copy of the cell fields (R U V W P T VisLam
+ * VisTur TE ED) into pinned staging, ordered after the work already
+ * enqueued, and returns without waiting.  Waits first for a writer still
+ * reading the previous snapshot.  No reference counterpart (the reference
+ * writes from host-resident fields). */
+int s355_output_snapshot(s355_ctx* ctx);
+/* Writes one file in `format` from the latest snapshot (the grid formats
+ * need none).  async = 1: on a writer thread, so the caller can keep the
+ * device busy; errors come back from s355_output_wait. */
+int s355_output_write(s355_ctx* ctx, int format, const char* path, int async);
+/* Joins the writer thread; returns its error, if any. */
+int s355_output_wait(s355_ctx* ctx);
+/* ReadBackupFile (navierIO.cpp:389-403): sets R U V W P T VisLam (VisTur TE
+ * ED with k-epsilon) from a res.sav.  InitFlowField reads the backup in
+ * place of the region initial values (navier.cpp:591-592); call it after
+ * s355_init_flow_field, which still sets the boundary values. */
+int s355_read_backup(s355_ctx* ctx, const char* path);
+/* The output schedule of NSSolve (navier.cpp:113-122, 149): every noutput
+ * steps (step - 1 divisible by noutput) a backup dir/res.sav and either
+ * dir/tec/res0000.dat (binary = 0) or dir/tec/data (binary = 1), written
+ * from a snapshot on a writer thread while the next steps run; the grid
+ * file (dir/tec/grid.dat or dir/tec/geo) now; and a last Tecplot file
+ * after the loop.  noutput = 0 turns output off (the default).  The
+ * reference never advances its output counter (navier.cpp:750), so every
+ * ASCII result goes to res0000.dat. */
+int s355_output_schedule(s355_ctx* ctx, int noutput, int binary, const char* dir);
 
 #ifdef __cplusplus
 }

@@ -113,6 +113,7 @@ int orc_get_mesh_int(void* h, int which, int* out, size_t n) {
       case S355_M_CSR_ROWPTR: v = o->csr.rowptr; break;
       case S355_M_CSR_COL: v = o->csr.col; break;
       case S355_M_CSR_DIAG: v = o->csr.diag; break;
+      case S355_M_CELL_VERT: for (auto& c : o->cell) v.insert(v.end(), c.v, c.v + 8); break;
       default: throw std::runtime_error("bad mesh int id");
     }
     if (n != v.size()) throw std::runtime_error("size mismatch for mesh int " + std::to_string(which));
@@ -136,6 +137,7 @@ int orc_get_mesh_dbl(void* h, int which, double* out, size_t n) {
       case S355_G_FACE_XPAC: for (auto& f : o->face) v.insert(v.end(), f.xpac, f.xpac + 3); break;
       case S355_G_FACE_XNAC: for (auto& f : o->face) v.insert(v.end(), f.xnac, f.xnac + 3); break;
       case S355_G_BND_DIST: for (auto& b : o->bnd) v.push_back(b.distance); break;
+      case S355_G_VERT: v = o->vert; break;
       default: throw std::runtime_error("bad mesh dbl id");
     }
     if (n != v.size()) throw std::runtime_error("size mismatch for mesh dbl " + std::to_string(which));

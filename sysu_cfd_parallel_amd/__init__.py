@@ -109,6 +109,13 @@ def library(path: Optional[Path] = None) -> C.CDLL:
         "tune_report": (I, [V, I, pD, pD, pI]),
         "set_kernel_shapes": (I, [V, I, I, I, I, I]),
         "spmv_profile": (I, [V, I, I, pD, pI]),
+        "mesh_write_result": (I, [V, I, C.c_char_p, C.POINTER(_abi.Result)]),
+        "backup_read": (I, [C.c_char_p, I, I, pD]),
+        "output_snapshot": (I, [V]),
+        "output_write": (I, [V, I, C.c_char_p, I]),
+        "output_wait": (I, [V]),
+        "read_backup": (I, [V, C.c_char_p]),
+        "output_schedule": (I, [V, I, I, C.c_char_p]),
     }
     for name, (res, args) in more.items():
         fn = getattr(lib, f"s355_{name}")
@@ -184,6 +191,15 @@ def read_mesh(path, regions: list, binary: bool = False) -> CaseData:
     case = _take_case(lib, h, {"kind": "file", "path": str(path)})
     case.regions = list(regions)
     return case
+
+
+def read_backup_file(path, n_cell: int, turbulent: bool = False) -> dict:
+    """ReadBackupFile on the host: the cell arrays of a res.sav."""
+    lib = library()
+    names = _abi.RESULT_FIELDS[:10 if turbulent else 7]
+    out = np.empty(len(names) * n_cell)
+    _check(lib, lib.s355_backup_read(str(path).encode(), int(n_cell), int(turbulent), _dp(out)), "backup_read")
+    return {n: out[k * n_cell:(k + 1) * n_cell].copy() for k, n in enumerate(names)}
 
 
 def write_mesh(case: CaseData, path, binary: bool = False) -> None:
@@ -287,6 +303,23 @@ class HostMesh:
         _check(self.lib, self.lib.s355_mesh_get_dbl(self.h, MESH_DBL_ID[name], _dp(out), out.size), name)
         return out
 
+    def write_result(self, fmt: str, path, fields: Optional[dict] = None, turbulent: bool = False) -> None:
+        """One result file of this mesh from host arrays (fmt: tecplot,
+        tec_data, tec_geo, grid, backup; the grid formats need no fields)."""
+        keep, r = [], None
+        if fields is not None:
+            r = _abi.Result()
+            for n in _abi.RESULT_FIELDS:
+                if fields.get(n) is not None:
+                    a = np.ascontiguousarray(fields[n], dtype=np.float64)
+                    if a.size != self.sizes.n_cell:
+                        raise ValueError(f"{n}: expected {self.sizes.n_cell} values")
+                    keep.append(a)
+                    setattr(r, n.lower(), _dp(a))
+            r.turbulent = int(turbulent)
+        _check(self.lib, self.lib.s355_mesh_write_result(self.h, _abi.OUTPUT_FORMATS[fmt], str(path).encode(),
+                                                         C.byref(r) if r is not None else None), "mesh_write_result")
+
     def close(self) -> None:
         if self.h:
             self.lib.s355_mesh_free(self.h)
@@ -349,6 +382,29 @@ class Solver:
     def set_params(self, params: Params) -> None:
         _check(self.lib, self.lib.s355_ctx_set_params(self.ctx, C.byref(params)), "set_params")
         self.params = params
+
+    # --- result and restart files
+    def snapshot(self) -> None:
+        """Start copying the cell fields to pinned host staging (no wait)."""
+        self.call("output_snapshot")
+
+    def write_output(self, fmt: str, path, async_: bool = False) -> None:
+        """Write one file (tecplot, tec_data, tec_geo, grid, backup) from the
+        latest snapshot; async_: on a writer thread (see wait_output)."""
+        self.call("output_write", _abi.OUTPUT_FORMATS[fmt], str(path).encode(), int(async_))
+
+    def wait_output(self) -> None:
+        self.call("output_wait")
+
+    def read_backup(self, path) -> None:
+        """Restart from a res.sav (after init_flow_field, which the
+        constructor runs)."""
+        self.call("read_backup", str(path).encode())
+
+    def output_schedule(self, noutput: int, directory, binary: bool = False) -> None:
+        """NSSolve's output: every noutput steps res.sav and the Tecplot
+        result under `directory` (0 turns it off)."""
+        self.call("output_schedule", int(noutput), int(binary), str(directory).encode())
 
     # --- phases
     def gradient(self, field_name: str, bfield: str, out: str) -> None:

@@ -13,6 +13,7 @@ S355_E_HIP = -2
 S355_E_MESH = -3
 S355_E_STATE = -4
 S355_E_NOMEM = -5
+S355_E_IO = -6
 MAX_CELL_FACES = 12
 
 KSP_GMRES, KSP_CG = 0, 1
@@ -103,6 +104,17 @@ class Sizes(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+# result file formats (enum s355_output_format)
+OUTPUT_FORMATS = {"tecplot": 0, "tec_data": 1, "tec_geo": 2, "grid": 3, "backup": 4}
+# cell arrays of a result, in s355_result order
+RESULT_FIELDS = ("R", "U", "V", "W", "P", "T", "VISLAM", "VISTUR", "TE", "ED")
+
+
+class Result(C.Structure):
+    _fields_ = [(n.lower(), C.POINTER(C.c_double)) for n in RESULT_FIELDS] + [("turbulent", C.c_int),
+                                                                               ("pad", C.c_int)]
+
+
 # field ids (enum s355_field)
 FIELDS = [
     "R", "U", "V", "W", "P", "T", "TE", "ED", "DELTAP", "VISLAM", "VISTUR", "APR",
@@ -122,10 +134,10 @@ FIELD_KIND = {
 }
 
 MESH_INT = ["ELEM_PERM", "FACE_C1", "FACE_C2", "FACE_BND", "FACE_VERT", "CELL_NFACE", "CELL_FACE", "CELL_CELL",
-            "BND_FACE", "BND_RID", "CELL_RID", "CSR_ROWPTR", "CSR_COL", "CSR_DIAG"]
+            "BND_FACE", "BND_RID", "CELL_RID", "CSR_ROWPTR", "CSR_COL", "CSR_DIAG", "CELL_VERT"]
 MESH_INT_ID = {n: i for i, n in enumerate(MESH_INT)}
 MESH_DBL = ["CELL_VOL", "CELL_X", "FACE_X", "FACE_N", "FACE_AREA", "FACE_LAMBDA", "FACE_RLENCOS", "FACE_XPAC",
-            "FACE_XNAC", "BND_DIST"]
+            "FACE_XNAC", "BND_DIST", "VERT"]
 MESH_DBL_ID = {n: i for i, n in enumerate(MESH_DBL)}
 
 
@@ -142,7 +154,7 @@ def mesh_int_len(name: str, s: Sizes) -> int:
         "ELEM_PERM": s.n_elem, "FACE_C1": s.n_face, "FACE_C2": s.n_face, "FACE_BND": s.n_face,
         "FACE_VERT": 4 * s.n_face, "CELL_NFACE": s.n_cell, "CELL_FACE": K * s.n_cell, "CELL_CELL": K * s.n_cell,
         "BND_FACE": s.n_bnd, "BND_RID": s.n_bnd, "CELL_RID": s.n_cell, "CSR_ROWPTR": s.n_cell + 1,
-        "CSR_COL": s.nnz, "CSR_DIAG": s.n_cell,
+        "CSR_COL": s.nnz, "CSR_DIAG": s.n_cell, "CELL_VERT": 8 * s.n_cell,
     }[name]
 
 
@@ -150,7 +162,7 @@ def mesh_dbl_len(name: str, s: Sizes) -> int:
     return {
         "CELL_VOL": s.n_cell, "CELL_X": 3 * s.n_cell, "FACE_X": 3 * s.n_face, "FACE_N": 3 * s.n_face,
         "FACE_AREA": s.n_face, "FACE_LAMBDA": s.n_face, "FACE_RLENCOS": s.n_face, "FACE_XPAC": 3 * s.n_face,
-        "FACE_XNAC": 3 * s.n_face, "BND_DIST": s.n_bnd,
+        "FACE_XNAC": 3 * s.n_face, "BND_DIST": s.n_bnd, "VERT": 3 * s.n_vert,
     }[name]
 
 

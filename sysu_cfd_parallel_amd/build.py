@@ -26,6 +26,8 @@ SOURCES = [
     "device/krylov.hip",
     "device/context.hip",
     "host/navier.cpp",
+    "host/output.cpp",
+    "host/export.cpp",
     "capi.cpp",
 ]
 

@@ -6,11 +6,14 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/simple355.h"
 #include "device/common.hpp"
 #include "device/context.hpp"
+#include "host/export.hpp"
 #include "host/navier.hpp"
+#include "host/output.hpp"
 #include "mesh/cases.hpp"
 #include "mesh/gmsh.hpp"
 #include "mesh/ingest.hpp"
@@ -18,6 +21,7 @@
 struct s355_ctx {
   std::unique_ptr<s355::Context> dev;
   std::unique_ptr<s355host::NavierStokesSolver> ns;
+  std::unique_ptr<s355host::FieldExport> out;  // created on first use; released before dev
 };
 
 struct s355_case {
@@ -46,6 +50,9 @@ int guarded(F&& f) {
   } catch (const s355::HipError& e) {
     g_err = e.what();
     return S355_E_HIP;
+  } catch (const s355host::IoError& e) {
+    g_err = e.what();
+    return S355_E_IO;
   } catch (const std::invalid_argument& e) {
     g_err = e.what();
     return S355_E_ARG;
@@ -102,6 +109,7 @@ int mesh_int(const s355::Mesh& m, int which, int* out, size_t n) {
     case S355_M_CSR_ROWPTR: v = &m.rowptr; break;
     case S355_M_CSR_COL: v = &m.col; break;
     case S355_M_CSR_DIAG: v = &m.diag; break;
+    case S355_M_CELL_VERT: v = &m.c_vert; break;
     default: throw std::invalid_argument("unknown mesh int id");
   }
   if (!out || n != v->size()) throw std::invalid_argument("mesh int " + std::to_string(which) + ": expected length " +
@@ -123,6 +131,7 @@ int mesh_dbl(const s355::Mesh& m, int which, double* out, size_t n) {
     case S355_G_FACE_XPAC: v = &m.f_xpac; break;
     case S355_G_FACE_XNAC: v = &m.f_xnac; break;
     case S355_G_BND_DIST: v = &m.b_dist; break;
+    case S355_G_VERT: v = &m.vert; break;
     default: throw std::invalid_argument("unknown mesh dbl id");
   }
   if (!out || n != v->size()) throw std::invalid_argument("mesh dbl " + std::to_string(which) + ": expected length " +
@@ -344,6 +353,115 @@ int s355_mesh_get_dbl(const s355_mesh* m, int which, double* out, size_t n) {
   });
 }
 void s355_mesh_free(s355_mesh* m) { delete m; }
+
+namespace {
+s355host::OutFormat out_format(int format) {
+  if (format < 0 || format >= S355_OUT_COUNT) throw std::invalid_argument("unknown output format " + std::to_string(format));
+  return static_cast<s355host::OutFormat>(format);
+}
+
+s355host::FieldExport& exporter(s355_ctx* c) {
+  s355::Context& d = dev(c);
+  if (!c->out) c->out = std::make_unique<s355host::FieldExport>(d);
+  return *c->out;
+}
+}  // namespace
+
+int s355_mesh_write_result(const s355_mesh* m, int format, const char* path, const s355_result* r) {
+  return guarded([&] {
+    if (!m || !path) throw std::invalid_argument("null argument");
+    const s355host::OutFormat f = out_format(format);
+    const bool grid = f == s355host::OutFormat::GridBinary || f == s355host::OutFormat::GridAscii;
+    if (!grid && !r) throw std::invalid_argument("null result");
+    s355host::ResultView v;
+    v.n_vert = m->m.n_vert;
+    v.n_cell = m->m.n_cell;
+    v.vert = m->m.vert.data();
+    v.cell_vert = m->m.c_vert.data();
+    if (r) {
+      v.R = r->R;
+      v.U = r->U;
+      v.V = r->V;
+      v.W = r->W;
+      v.P = r->P;
+      v.T = r->T;
+      v.vislam = r->vislam;
+      v.vistur = r->vistur;
+      v.te = r->te;
+      v.ed = r->ed;
+      v.turbulent = r->turbulent != 0;
+    }
+    s355host::write_result(v, f, path);
+    return S355_OK;
+  });
+}
+
+int s355_backup_read(const char* path, int n_cell, int turbulent, double* out) {
+  return guarded([&] {
+    if (!path || !out || n_cell < 1) throw std::invalid_argument("backup_read arguments");
+    const size_t n = n_cell;
+    s355host::BackupArrays a{out, out + n, out + 2 * n, out + 3 * n, out + 4 * n, out + 5 * n, out + 6 * n,
+                             turbulent ? out + 7 * n : nullptr, turbulent ? out + 8 * n : nullptr,
+                             turbulent ? out + 9 * n : nullptr};
+    s355host::read_backup(path, n_cell, turbulent != 0, a);
+    return S355_OK;
+  });
+}
+
+int s355_output_snapshot(s355_ctx* ctx) {
+  return guarded([&] {
+    exporter(ctx).snapshot();
+    return S355_OK;
+  });
+}
+
+int s355_output_write(s355_ctx* ctx, int format, const char* path, int async) {
+  return guarded([&] {
+    if (!path) throw std::invalid_argument("null path");
+    exporter(ctx).write({{out_format(format), path}}, async != 0);
+    return S355_OK;
+  });
+}
+
+int s355_output_wait(s355_ctx* ctx) {
+  return guarded([&] {
+    dev(ctx);
+    if (ctx->out) ctx->out->wait();
+    return S355_OK;
+  });
+}
+
+int s355_read_backup(s355_ctx* ctx, const char* path) {
+  return guarded([&] {
+    s355::Context& d = dev(ctx);
+    if (!path) throw std::invalid_argument("null path");
+    if (ctx->out) ctx->out->wait();
+    const bool turb = d.par.turb_model == 1;
+    const size_t n = d.dm.Nc;
+    std::vector<double> h((turb ? 10 : 7) * n);
+    s355host::BackupArrays a{h.data(), h.data() + n, h.data() + 2 * n, h.data() + 3 * n, h.data() + 4 * n,
+                             h.data() + 5 * n, h.data() + 6 * n, turb ? h.data() + 7 * n : nullptr,
+                             turb ? h.data() + 8 * n : nullptr, turb ? h.data() + 9 * n : nullptr};
+    s355host::read_backup(path, static_cast<int>(n), turb, a);
+    const int ids[10] = {S355_F_R, S355_F_U, S355_F_V, S355_F_W, S355_F_P, S355_F_T, S355_F_VISLAM, S355_F_VISTUR,
+                         S355_F_TE, S355_F_ED};
+    for (int k = 0; k < (turb ? 10 : 7); ++k)
+      S355_HIP(hipMemcpyAsync(d.field(ids[k]).p, h.data() + k * n, n * sizeof(double), hipMemcpyHostToDevice,
+                              d.stream));
+    d.sync();
+    return S355_OK;
+  });
+}
+
+int s355_output_schedule(s355_ctx* ctx, int noutput, int binary, const char* dir) {
+  return guarded([&] {
+    dev(ctx);
+    if (noutput < 0) throw std::invalid_argument("noutput < 0");
+    if (noutput > 0 && !dir) throw std::invalid_argument("null output directory");
+    ctx->ns->SetOutput(noutput, binary != 0, noutput > 0 ? dir : "");
+    return S355_OK;
+  });
+}
 
 int s355_ctx_create(const s355_mesh_desc* mesh, const s355_params* params, s355_ctx** out) {
   return guarded([&] {

@@ -11,6 +11,8 @@
 #include "navier.hpp"
 
 #include <algorithm>
+#include <cstdio>
+#include <filesystem>
 
 namespace s355host {
 
@@ -145,6 +147,71 @@ void NavierStokesSolver::UpdateEnergy() {
   check(s355_log_stage_pop(ctx_), "log");
 }
 
+void NavierStokesSolver::SetOutput(int noutput, bool binary, const std::string& dir) {
+  check(s355_output_wait(ctx_), "output");
+  noutput_ = noutput;
+  output_binary_ = binary;
+  outdir_ = dir;
+  if (noutput_ <= 0) return;
+  std::error_code ec;
+  std::filesystem::create_directories(std::filesystem::path(outdir_) / "tec", ec);
+  if (ec) throw DeviceError{S355_E_IO, "cannot create " + outdir_ + "/tec: " + ec.message()};
+  // the grid file goes out once, at set-up (geometry.cpp:208-213)
+  if (output_binary_)
+    OutputGridBinary();
+  else
+    OutputGrid();
+}
+
+void NavierStokesSolver::OutputGrid() {
+  check(s355_output_write(ctx_, S355_OUT_GRID, (outdir_ + "/tec/grid.dat").c_str(), 0), "OutputGrid");
+}
+
+void NavierStokesSolver::OutputGridBinary() {
+  check(s355_output_write(ctx_, S355_OUT_TEC_GEO, (outdir_ + "/tec/geo").c_str(), 0), "OutputGridBinary");
+}
+
+std::string NavierStokesSolver::result_path(int format) const {
+  char name[32];
+  switch (format) {
+    case S355_OUT_TECPLOT: std::snprintf(name, sizeof name, "/tec/res%04d.dat", outputCounter); break;
+    case S355_OUT_TEC_DATA: std::snprintf(name, sizeof name, "/tec/data"); break;
+    default: std::snprintf(name, sizeof name, "/res.sav"); break;
+  }
+  return outdir_ + name;
+}
+
+void NavierStokesSolver::Output2Tecplot(bool async) {
+  check(s355_output_snapshot(ctx_), "snapshot");
+  check(s355_output_write(ctx_, S355_OUT_TECPLOT, result_path(S355_OUT_TECPLOT).c_str(), async), "Output2Tecplot");
+}
+
+void NavierStokesSolver::Output2TecplotBinary(bool async) {
+  check(s355_output_snapshot(ctx_), "snapshot");
+  check(s355_output_write(ctx_, S355_OUT_TEC_DATA, result_path(S355_OUT_TEC_DATA).c_str(), async),
+        "Output2TecplotBinary");
+}
+
+void NavierStokesSolver::WriteBackupFile(bool async) {
+  check(s355_output_snapshot(ctx_), "snapshot");
+  check(s355_output_write(ctx_, S355_OUT_BACKUP, result_path(S355_OUT_BACKUP).c_str(), async), "WriteBackupFile");
+}
+
+void NavierStokesSolver::ReadBackupFile(const std::string& path) {
+  check(s355_read_backup(ctx_, path.c_str()), "ReadBackupFile");
+}
+
+// navier.cpp:113-122: backup, then the Tecplot result, every noutput steps.
+// Both come from one snapshot; the writer thread formats them while the
+// next steps run on the device.
+void NavierStokesSolver::output_step(int step) {
+  if (noutput_ <= 0 || (step - 1) % noutput_ != 0) return;
+  check(s355_output_snapshot(ctx_), "snapshot");
+  const int result = output_binary_ ? S355_OUT_TEC_DATA : S355_OUT_TECPLOT;
+  check(s355_output_write(ctx_, S355_OUT_BACKUP, result_path(S355_OUT_BACKUP).c_str(), 1), "WriteBackupFile");
+  check(s355_output_write(ctx_, result, result_path(result).c_str(), 1), "Output2Tecplot");
+}
+
 int NavierStokesSolver::NSSolve(int max_step, double* history, int* steps_done) {
   *steps_done = 0;
   for (int step = 1; step <= max_step; ++step) {
@@ -161,7 +228,12 @@ int NavierStokesSolver::NSSolve(int max_step, double* history, int* steps_done) 
       history[11 * (step - 1) + 10] = resmax;
     }
     *steps_done = step;
+    output_step(step);
     if (resmax < par_.residual_steady) break;
+  }
+  if (noutput_ > 0) {
+    Output2Tecplot(false);  // the last result (navier.cpp:149), written before returning
+    check(s355_output_wait(ctx_), "output");
   }
   return 0;
 }

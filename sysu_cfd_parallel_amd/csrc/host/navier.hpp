@@ -66,13 +66,28 @@ class NavierStokesSolver {
   // NSSolve steady loop; history gets 11 doubles per step.
   int NSSolve(int max_step, double* history, int* steps_done);
 
+  // result and restart files (navierIO.cpp), under the output directory
+  void SetOutput(int noutput, bool binary, const std::string& dir);
+  void Output2Tecplot(bool async);
+  void Output2TecplotBinary(bool async);
+  void OutputGrid();
+  void OutputGridBinary();
+  void WriteBackupFile(bool async);
+  void ReadBackupFile(const std::string& path);
+
   double Residual[10] = {0};
   s355_solve_info last[7] = {};  // u, v, w, p, k, eps, T
 
  private:
   void check(int rc, const char* what);
+  void output_step(int step);
+  std::string result_path(int format) const;
   s355_ctx* ctx_;
   s355_params par_;
+  int noutput_ = 0;  // 0: no output (the reference's default is 1, navier.cpp:515)
+  bool output_binary_ = false;
+  std::string outdir_;
+  int outputCounter = 0;  // never advanced by the reference (navier.cpp:750)
 
  public:
   DataPartition dataPartition;

@@ -1,0 +1,48 @@
+// Result files in the reference's formats: Tecplot ASCII and the binary
+// tec/geo + tec/data pair read by postCycas.py, the ASCII grid file, and the
+// res.sav backup that restarts a run.  Host code only: the device side hands
+// over host copies of the cell fields.
+#pragma once
+#include <string>
+
+#include <stdexcept>
+
+namespace s355host {
+
+// A file that cannot be opened, read or written, or a malformed one.
+struct IoError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+// Host views of one result: the vertex pool, each cell's 8 vertices
+// (0-based, degenerate-hex form) and the cell fields.  te / ed / vistur may
+// be null for a laminar run (turbulent == false); vistur null reads as 0.
+struct ResultView {
+  int n_vert = 0, n_cell = 0;
+  const double* vert = nullptr;  // 3 per vertex
+  const int* cell_vert = nullptr;
+  const double *R = nullptr, *U = nullptr, *V = nullptr, *W = nullptr, *P = nullptr, *T = nullptr;
+  const double *vislam = nullptr, *vistur = nullptr, *te = nullptr, *ed = nullptr;
+  bool turbulent = false;
+};
+
+enum class OutFormat {
+  TecplotAscii = 0,  // Output2Tecplot: tec/resNNNN.dat
+  TecplotData = 1,   // Output2TecplotBinary: tec/data
+  GridBinary = 2,    // OutputGridBinary: tec/geo
+  GridAscii = 3,     // OutputGrid: tec/grid.dat
+  Backup = 4,        // WriteBackupFile: res.sav
+};
+
+// Writes one file; throws std::runtime_error on a bad view or an I/O error.
+void write_result(const ResultView& r, OutFormat f, const std::string& path);
+
+// Cell arrays a backup file restores (ReadBackupFile): each of length n_cell;
+// vistur / te / ed are read only when turbulent.
+struct BackupArrays {
+  double *R = nullptr, *U = nullptr, *V = nullptr, *W = nullptr, *P = nullptr, *T = nullptr;
+  double *vislam = nullptr, *vistur = nullptr, *te = nullptr, *ed = nullptr;
+};
+void read_backup(const std::string& path, int n_cell, bool turbulent, const BackupArrays& out);
+
+}  // namespace s355host

@@ -191,7 +191,7 @@ Mesh ingest_mesh(const s355_mesh_desc& d) {
       for (int k = 0; k < 3; ++k) m.vert[3 * static_cast<size_t>(local[v]) + k] = d.xyz[3 * static_cast<size_t>(v) + k];
   clk.mark("vertex compaction");
   // ---- cells and boundary records
-  std::vector<int> cv;     // 8 per cell
+  hvec<int>& cv = m.c_vert;  // 8 per cell
   std::vector<int> bv;     // 4 per boundary record
   for (int e : m.perm) {
     int t = d.elem_type[e];

@@ -112,3 +112,25 @@ def heated(case, t_wall=350.0, t_in=300.0, t0=300.0):
         elif r.type1 == 5:
             r.init[4] = t0
     return case
+
+
+def parse_tecplot(path) -> dict:
+    """A Tecplot ASCII result of Output2Tecplot (navierIO.cpp:161-246) read
+    back: the variable names, the vertex coordinates, the cell-centred
+    arrays and the 8 vertex ids (1-based) of each cell."""
+    text = open(path).read()
+    head, zone, body = text.split("\n", 2)
+    names = [t.strip().strip('"') for t in head.split("=", 1)[1].replace('" "', '","').split(",")]
+    names = [n for n in names if n]
+    kv = dict(part.split("=", 1) for part in zone[len("ZONE "):].split(", ")[:2])
+    nv, ne = int(kv["N"]), int(kv["E"])
+    tok = body.split()
+    xyz = np.array(tok[:3 * nv], dtype=float).reshape(3, nv).T
+    off = 3 * nv
+    out = {"names": names, "xyz": xyz}
+    for name in names[3:]:
+        out[name] = np.array(tok[off:off + ne], dtype=float)
+        off += ne
+    out["conn"] = np.array(tok[off:off + 8 * ne], dtype=np.int64).reshape(ne, 8)
+    assert off + 8 * ne == len(tok)
+    return out

@@ -1,0 +1,236 @@
+"""Result and restart files in the reference's layouts (navierIO.cpp), host
+side: the Tecplot ASCII file of Output2Tecplot, the tec/geo + tec/data pair
+of OutputGridBinary / Output2TecplotBinary (read back the way postCycas.py
+readBuf reads them), the grid.dat of OutputGrid, and the res.sav of
+WriteBackupFile / ReadBackupFile.  The cells' 8 vertex ids follow the
+degenerate-hex mapping of ReadGridFile (geometry.cpp:160-195)."""
+import struct
+
+import numpy as np
+import pytest
+
+import sysu_cfd_parallel_amd as s
+from helpers import manual_case, region, tet_cases
+
+REGS = [region(1, 1), region(100, 5, [0, 0, 0, 1, 300, 0.01, 0, 0])]
+
+
+def prism():
+    x = [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [0, 1, 1]]
+    return manual_case(x, [(2, 1, [0, 2, 1]), (2, 1, [3, 4, 5]), (3, 1, [0, 1, 4, 3]), (3, 1, [1, 2, 5, 4]),
+                           (3, 1, [2, 0, 3, 5]), (6, 100, [0, 1, 2, 3, 4, 5])], REGS)
+
+
+def pyramid():
+    x = [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0.5, 0.5, 1]]
+    return manual_case(x, [(3, 1, [0, 3, 2, 1]), (2, 1, [0, 1, 4]), (2, 1, [1, 2, 4]), (2, 1, [2, 3, 4]),
+                           (2, 1, [3, 0, 4]), (7, 100, [0, 1, 2, 3, 4])], REGS)
+
+
+def hexa():
+    x = [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]]
+    q = [[0, 3, 2, 1], [4, 5, 6, 7], [0, 1, 5, 4], [1, 2, 6, 5], [2, 3, 7, 6], [3, 0, 4, 7]]
+    return manual_case(x, [(3, 1, f) for f in q] + [(5, 100, list(range(8)))], REGS)
+
+
+@pytest.mark.parametrize("make,expect", [
+    (lambda: tet_cases()[0], [0, 1, 2, 2, 3, 3, 3, 3]),
+    (prism, [0, 1, 2, 2, 3, 4, 5, 5]),
+    (pyramid, [0, 1, 2, 3, 4, 4, 4, 4]),
+    (hexa, list(range(8))),
+])
+def test_cell_vertices_are_the_degenerate_hex(make, expect):
+    m = s.HostMesh(make())
+    assert m.sizes.n_cell == 1
+    assert list(m.ints("CELL_VERT")) == expect
+
+
+def tet_fields():
+    return {"P": [1.25], "U": [3.0], "V": [4.0], "W": [0.0], "R": [1.0], "T": [300.0], "VISLAM": [0.01]}
+
+
+TET_TECPLOT = (
+    'variables= "x", "y", "z", "p", "u", "v", "w", "ro", "T","Ma","mu" \n'
+    "ZONE N=4, E=1, VARLOCATION=([1-3]=NODAL,[4-11]=CELLCENTERED) DATAPACKING=BLOCK, ZONETYPE=FEBRICK\n"
+    "            0.00000             1.00000             0.00000             0.00000 \n"
+    "            0.00000             0.00000             1.00000             0.00000 \n"
+    "            0.00000             0.00000             0.00000             1.00000 \n"
+    "            1.25000 \n"      # p
+    "            3.00000 \n"      # u
+    "            4.00000 \n"      # v
+    "            0.00000 \n"      # w
+    "            1.00000 \n"      # ro
+    "          300.00000 \n"      # T
+    "            5.00000 \n"      # Ma: |u| for constant density
+    "            0.01000 \n"      # mu = VisLam + VisTur
+    "1 2 3 3 4 4 4 4\n "
+)
+
+
+def test_tecplot_ascii_known_answer(tmp_path):
+    m = s.HostMesh(tet_cases()[0])
+    m.write_result("tecplot", tmp_path / "res0000.dat", tet_fields())
+    assert (tmp_path / "res0000.dat").read_text() == TET_TECPLOT
+
+
+def test_tecplot_ascii_lines_of_five_and_turbulence_columns(tmp_path):
+    case = s.cavity(2)  # 27 vertices, 8 cells
+    m = s.HostMesh(case)
+    n = m.sizes.n_cell
+    rng = np.random.default_rng(3)
+    f = {k: rng.normal(size=n) * 10.0 ** rng.integers(-3, 4, size=n) for k in s._abi.RESULT_FIELDS}
+    m.write_result("tecplot", tmp_path / "t.dat", f, turbulent=True)
+    text = (tmp_path / "t.dat").read_text()
+    head, zone, rest = text.split("\n", 2)
+    assert head == 'variables= "x", "y", "z", "p", "u", "v", "w", "ro", "T","Ma","mu" "te", "ed"'
+    assert zone == (f"ZONE N=27, E={n}, VARLOCATION=([1-3]=NODAL,[4-13]=CELLCENTERED) DATAPACKING=BLOCK, "
+                    "ZONETYPE=FEBRICK")
+    lines = rest.split("\n")
+    # 27 vertex values: 5 full lines and one of 2; 8 cell values: one of 5, one of 3
+    xs = lines[:6]
+    assert [len(l) for l in xs] == [100] * 5 + [40]
+    vert = m.dbls("VERT").reshape(-1, 3)
+    assert np.array_equal(np.array(" ".join(xs).split(), dtype=float), np.round(vert[:, 0], 5))
+    blocks = lines[18:]
+    names = ["P", "U", "V", "W", "R", "T", "Ma", "mu", "TE", "ED"]
+    vals = {}
+    for k, nm in enumerate(names):
+        two = blocks[2 * k: 2 * k + 2]
+        assert [len(l) for l in two] == [100, 60]
+        vals[nm] = np.array(" ".join(two).split(), dtype=float)
+    for nm in ("P", "U", "V", "W", "R", "T", "TE", "ED"):
+        assert np.allclose(vals[nm], f[nm], atol=6e-6, rtol=0), nm
+    assert np.allclose(vals["Ma"], np.sqrt(f["U"] ** 2 + f["V"] ** 2 + f["W"] ** 2), atol=6e-6, rtol=0)
+    assert np.allclose(vals["mu"], f["VISLAM"] + f["VISTUR"], atol=6e-6, rtol=0)
+    conn = blocks[20:]
+    cv = m.ints("CELL_VERT").reshape(-1, 8) + 1
+    assert conn[0] == " ".join(map(str, cv[0]))
+    for i in range(1, n):
+        assert conn[i] == " " + " ".join(map(str, cv[i]))
+    assert conn[n] == " "  # the file ends with the space after the last newline
+
+
+def test_block_of_five_gets_a_blank_line(tmp_path):
+    # 5 values: the fifth ends its line and the array adds one more newline
+    case = s.channel(5, 1, 1)
+    m = s.HostMesh(case)
+    assert m.sizes.n_cell == 5
+    f = {k: np.arange(5.0) for k in ("P", "U", "V", "W", "R", "T", "VISLAM")}
+    m.write_result("tecplot", tmp_path / "t.dat", f)
+    text = (tmp_path / "t.dat").read_text()
+    p = "".join("%19.5f " % v for v in range(5))
+    assert "\n" + p + "\n\n" + p + "\n\n" in text
+
+
+def read_postcycas(geo, data):
+    """tec/geo and tec/data the way postCycas.py readBuf reads them
+    (postCycas.py:37-86), for one rank."""
+    g = geo.read_bytes()
+    rank, nproc, nc, nv = struct.unpack_from("4i", g, 0)
+    off = 16
+    xyz = np.frombuffer(g, dtype="<f8", count=3 * nv, offset=off).reshape(nv, 3)
+    off += 24 * nv
+    conn = np.frombuffer(g, dtype="<i4", count=8 * nc, offset=off).reshape(nc, 8)
+    assert off + 32 * nc == len(g)
+    d = data.read_bytes()
+    drank, nvar = struct.unpack_from("2i", d, 0)
+    vals = np.frombuffer(d, dtype="<f8", offset=8).reshape(nvar, nc)
+    return (rank, nproc, nc, nv, drank, nvar), xyz, conn, vals
+
+
+@pytest.mark.parametrize("turbulent", [False, True])
+def test_binary_geo_and_data(tmp_path, turbulent):
+    case = s.step(2, upstream=2, downstream=3, span=2, tets=True)
+    m = s.HostMesh(case)
+    n = m.sizes.n_cell
+    rng = np.random.default_rng(5)
+    f = {k: rng.normal(size=n) for k in s._abi.RESULT_FIELDS}
+    m.write_result("tec_geo", tmp_path / "geo")
+    m.write_result("tec_data", tmp_path / "data", f, turbulent=turbulent)
+    head, xyz, conn, vals = read_postcycas(tmp_path / "geo", tmp_path / "data")
+    assert head == (0, 1, n, m.sizes.n_vert, 0, 8 if turbulent else 6)
+    assert np.array_equal(xyz.ravel(), m.dbls("VERT"))
+    assert np.array_equal(conn.ravel(), m.ints("CELL_VERT") + 1)
+    order = ["P", "U", "V", "W", "R", "T"] + (["TE", "ED"] if turbulent else [])
+    for k, nm in enumerate(order):
+        assert np.array_equal(vals[k], f[nm]), nm
+
+
+def test_grid_ascii(tmp_path):
+    m = s.HostMesh(hexa())
+    m.write_result("grid", tmp_path / "grid.dat")
+    assert (tmp_path / "grid.dat").read_text() == (
+        'title = "cycas partition grid"\nvariables="x","y","z","p"\n'
+        "zone n=8, e=1, f=fepoint, et=BRICK\n"
+        "0.000000 0.000000 0.000000 0.000000\n1.000000 0.000000 0.000000 0.000000\n"
+        "1.000000 1.000000 0.000000 0.000000\n0.000000 1.000000 0.000000 0.000000\n"
+        "0.000000 0.000000 1.000000 0.000000\n1.000000 0.000000 1.000000 0.000000\n"
+        "1.000000 1.000000 1.000000 0.000000\n0.000000 1.000000 1.000000 0.000000\n"
+        "       1        2        3        4        5        6        7        8\n")
+
+
+def test_backup_known_answer_and_read_back(tmp_path):
+    m = s.HostMesh(tet_cases()[1])  # two tets
+    f = {"R": [1.0, 1.5], "U": [0.1, -2.0], "V": [1e-7, 123456789.0], "W": [0.0, -0.0], "P": [1.25, 2.0 / 3.0],
+         "T": [300.0, 301.5], "VISLAM": [0.01, 1e-5], "VISTUR": [0.5, 0.25], "TE": [1e-3, 2e-3], "ED": [4.0, 5.0]}
+    m.write_result("backup", tmp_path / "res.sav", f)
+    assert (tmp_path / "res.sav").read_text() == (
+        "1 0.1 1e-07 0 1.25 300 1 0.01\n"
+        "1.5 -2 1.23457e+08 -0 0.666667 301.5 1.5 1e-05\n")
+    m.write_result("backup", tmp_path / "turb.sav", f, turbulent=True)
+    assert (tmp_path / "turb.sav").read_text() == (
+        "1 0.1 1e-07 0 1.25 300 1 0.01\n0.5 0.001 4\n"
+        "1.5 -2 1.23457e+08 -0 0.666667 301.5 1.5 1e-05\n0.25 0.002 5\n")
+    back = s.read_backup_file(tmp_path / "turb.sav", 2, turbulent=True)
+    for k in s._abi.RESULT_FIELDS:
+        assert np.array_equal(back[k], [float("%g" % v) for v in f[k]]), k
+
+
+def test_backup_keeps_the_second_density(tmp_path):
+    # ReadBackupFile reads Rn twice per line; the later value stays
+    (tmp_path / "res.sav").write_text("1 2 3 4 5 6 7 8\n")
+    back = s.read_backup_file(tmp_path / "res.sav", 1)
+    assert back["R"][0] == 7.0 and back["VISLAM"][0] == 8.0 and back["T"][0] == 6.0
+
+
+def test_truncated_backup_is_an_io_error(tmp_path):
+    (tmp_path / "res.sav").write_text("1 2 3 4 5 6 7 8\n1 2 3\n")
+    with pytest.raises(s.S355Error) as err:
+        s.read_backup_file(tmp_path / "res.sav", 2)
+    assert err.value.code == s._abi.S355_E_IO
+    with pytest.raises(s.S355Error) as err:
+        s.read_backup_file(tmp_path / "missing.sav", 2)
+    assert err.value.code == s._abi.S355_E_IO
+
+
+def test_unwritable_path_and_missing_fields(tmp_path):
+    m = s.HostMesh(hexa())
+    with pytest.raises(s.S355Error) as err:
+        m.write_result("tec_geo", tmp_path / "no_such_dir" / "geo")
+    assert err.value.code == s._abi.S355_E_IO
+    with pytest.raises(s.S355Error):
+        m.write_result("tecplot", tmp_path / "t.dat", {"P": [0.0]})
+
+
+def test_batch_cell_vertices_are_offset(tmp_path):
+    cases = [s.cavity(2), tet_cases()[1], s.cavity(3, tets=True)]
+    parts = [s.HostMesh(c) for c in cases]
+    batch = s.HostMesh(cases)
+    v0 = np.concatenate([[0], np.cumsum([p.sizes.n_vert for p in parts])])
+    got = batch.ints("CELL_VERT")
+    want = np.concatenate([p.ints("CELL_VERT") + v0[k] for k, p in enumerate(parts)])
+    assert np.array_equal(got, want)
+    assert np.array_equal(batch.dbls("VERT"), np.concatenate([p.dbls("VERT") for p in parts]))
+
+
+def test_tecplot_reader_of_the_tests(tmp_path):
+    # the parser the device tests use, on a file of known content
+    from helpers import parse_tecplot
+
+    m = s.HostMesh(tet_cases()[0])
+    m.write_result("tecplot", tmp_path / "r.dat", tet_fields())
+    t = parse_tecplot(tmp_path / "r.dat")
+    assert t["names"] == ["x", "y", "z", "p", "u", "v", "w", "ro", "T", "Ma", "mu"]
+    assert t["p"][0] == 1.25 and t["Ma"][0] == 5.0 and t["T"][0] == 300.0
+    assert np.array_equal(t["xyz"].ravel(), m.dbls("VERT"))
+    assert list(t["conn"][0]) == [1, 2, 3, 3, 4, 4, 4, 4]
