@@ -85,7 +85,8 @@ enum { S355_KSP_VELOCITY = 0, S355_KSP_PRESSURE = 1, S355_KSP_SCALAR = 2 };
 typedef struct s355_params {
   int turb_model;        /* TurModel: 0 laminar, 1 k-epsilon */
   int density_model;     /* DensityModel: only 0 (constant density) */
-  int steady;            /* IfSteady: only 1 */
+  int steady;            /* IfSteady: 1 steady (one SIMPLE pass per step),
+                            0 unsteady (time steps of dt, see time_scheme) */
   int cell_pressure_ref; /* 0-based cell of the pressure reference */
   double urf[8];         /* URF[0..7] (navier.cpp:500-507) */
   double gravity[3];
@@ -104,6 +105,14 @@ typedef struct s355_params {
   int energy_pad;
   double cp, prl, prte; /* heat capacity, laminar / turbulent Prandtl numbers
                            (navier.cpp:489-492: 1006, 0.72, 0.9) */
+  int limiter;          /* gradient limiter (readparamfile.cpp:149-162): 0 none,
+                           1 Barth, 2 MLP, 3 WENO (navier_gradient.cpp:51-62) */
+  int time_scheme;      /* TimeScheme (unsteady): 1 Euler, 2 BDF2 */
+  double dt;            /* time step (readparamfile.cpp:86-95) */
+  double total_time;    /* the time loop ends once cur_time >= total_time */
+  int max_outer_step;   /* MaxOuterStep: outer SIMPLE passes iter = 1 ..
+                           MaxOuterStep - 1 per time step (navier.cpp:60) */
+  int transient_pad;
 } s355_params;
 
 typedef struct s355_solve_info {
@@ -134,6 +143,10 @@ enum s355_field {
   S355_F_BU_RHS, S355_F_BV_RHS, S355_F_BW_RHS, S355_F_BP_RHS, S355_F_BS_RHS,
   /* per boundary face, n = Nbnd: boundary temperature BTem */
   S355_F_BTEM,
+  /* per cell, unsteady runs after s355_save_transient_old_data: the
+   * previous time level (Unp ...) and the one before (Unp2 ...) */
+  S355_F_UNP, S355_F_VNP, S355_F_WNP, S355_F_TNP, S355_F_TEP, S355_F_EDP,
+  S355_F_UNP2, S355_F_VNP2, S355_F_WNP2, S355_F_TNP2, S355_F_TEP2, S355_F_EDP2,
   S355_F_COUNT
 };
 
@@ -259,10 +272,23 @@ int s355_solve_scalar(s355_ctx* ctx, int isca, double rtol, int maxit, s355_solv
 int s355_kepsilon_post_k(s355_ctx* ctx);                   /* 105-121: clip k, eps sources */
 int s355_kepsilon_post_eps(s355_ctx* ctx);                 /* 165-178: clip eps, mu_t */
 
-/* Energy pieces of UpdateEnergy (navier_scalar.cpp:204-326, DensityModel 0,
- * steady): BuildScalarMatrix(1) and solve_scalar(1) act on T / BTem. */
+/* Energy pieces of UpdateEnergy (navier_scalar.cpp:204-326, DensityModel 0):
+ * BuildScalarMatrix(1) and solve_scalar(1) act on T / BTem. */
 int s355_energy_coefficients(s355_ctx* ctx);  /* 215-218: kcond = cp (mu/prl + mu_t/prte) / cp, ESource = ApE = 0 */
 int s355_bc_temperature(s355_ctx* ctx);       /* SetBCTemperature navier_bc.cpp:183-214 */
+
+/* Unsteady runs (params.steady = 0).  The momentum time term is part of
+ * s355_build_velocity_matrix (navier_velocity.cpp:107-123); with
+ * params.limiter the limiter runs inside every gradient
+ * (navier_gradient.cpp:51-62). */
+int s355_save_transient_old_data(s355_ctx* ctx); /* SaveTransientOldData navier.cpp:700-742 */
+/* time terms of the scalar systems: isca 2 adds them to the k and epsilon
+ * sources / diagonals after s355_bc_kepsilon (navier_scalar.cpp:71-90),
+ * isca 1 gives the energy system its source / diagonal (290-304); a no-op
+ * for steady runs */
+int s355_unsteady_sources(s355_ctx* ctx, int isca);
+/* physical time and time steps taken by NSSolve (s355_simple_iterate) */
+int s355_time_state(s355_ctx* ctx, double* cur_time, int* time_steps);
 
 /* Device-timed log stages, the counterpart of PetscLogStagePush/Pop around
  * the phases (navier.cpp:26-37): 1 BuildVelocity, 2 SolveVelocity,

@@ -264,4 +264,14 @@ int orc_simple_iterate(void* h, int n, double* history, int* done) {
   return guard([&] { return O(h)->SimpleIterate(n, history, done); });
 }
 
+int orc_save_transient_old_data(void* h) { return guard([&] { O(h)->SaveTransientOldData(); return S355_OK; }); }
+
+int orc_time_state(void* h, double* cur_time, int* time_steps) {
+  return guard([&] {
+    *cur_time = O(h)->cur_time;
+    *time_steps = O(h)->time_step;
+    return S355_OK;
+  });
+}
+
 }  // extern "C"

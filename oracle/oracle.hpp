@@ -83,6 +83,10 @@ class Oracle {
   std::vector<double> BRo, BU, BV, BW, BPre, BTE, BED, BTem, shear;
   std::vector<double> RUFace;
   std::vector<double> Au, Ap, As, bu, bv, bw, bp, bs;
+  // old time levels of an unsteady run (navier.h:104-130)
+  std::vector<double> Unp, Vnp, Wnp, Tnp, TEp, EDp, Unp2, Vnp2, Wnp2, Tnp2, TEp2, EDp2;
+  double cur_time = 0;
+  int time_step = 0;
   double localRes[10] = {0};
   SolveResult last[7];  // u, v, w, p, k, eps, T
 
@@ -105,6 +109,13 @@ class Oracle {
   int SolveVelocity(double rtol, int maxit);
   int SolvePressure(double rtol, int maxit);
   int SimpleIterate(int n_steps, double* history, int* done);
+  void SaveTransientOldData();
+  void Limiter_Barth(const std::vector<double>& UC, std::vector<double>& g);
+  void Limiter_MLP(const std::vector<double>& UC, std::vector<double>& g);
+  void Limiter_WENO(std::vector<double>& g);
+  // the transient blocks of UpdateTurKEpsilon / UpdateEnergy on their arrays
+  void TransientScalar(const std::vector<double>& old, const std::vector<double>& old2, std::vector<double>& ap,
+                       std::vector<double>& src);
 
   void BuildScalarMatrix(std::vector<double>& phi, std::vector<double>& bphi, const std::vector<double>& diff,
                          const std::vector<double>& source, const std::vector<double>& app_in);
@@ -122,6 +133,8 @@ class Oracle {
   void build_csr();
   void allocate();
   int type1_of(int rid) const;
+  int SimplePass(double* resmax);
+  void EnsureOld();
 };
 
 // Krylov solvers on a CSR matrix (krylov.cpp)

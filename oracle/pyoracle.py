@@ -160,6 +160,11 @@ class Oracle:
         rc = self._check(self.lib.orc_update_energy(self.h, C.byref(info)), "update_energy")
         return rc, info.as_dict()
 
+    def time_state(self) -> tuple:
+        t, n = C.c_double(), C.c_int()
+        self._check(self.lib.orc_time_state(self.h, C.byref(t), C.byref(n)), "time_state")
+        return t.value, n.value
+
     def iterate(self, n_steps: int):
         hist = np.zeros(11 * max(n_steps, 1))
         done = C.c_int(0)

@@ -10,16 +10,17 @@
 //   InitFlowField        navier.cpp:588-697
 //   SetBCVelocity        navier_bc.cpp:8-119     SetBCPressure 121-155
 //   SetBCDeltaP          navier_bc.cpp:157-181   SetBCKEpsilon 220-345
-//   Gradient             navier_gradient.cpp:12-74 (limiter 0)
-//   BuildVelocityMatrix  navier_velocity.cpp:63-393 (steady)
+//   Gradient             navier_gradient.cpp:12-74, limiters 126-345
+//   BuildVelocityMatrix  navier_velocity.cpp:63-393
 //   CalRUFace2           navier_velocity.cpp:425-476
 //   CalculateVelocity    navier_velocity.cpp:10-60
 //   CalculatePressure    navier_pressure.cpp:9-97
 //   BuildPressureMatrix  navier_pressure.cpp:99-232 (DensityModel 0)
 //   CorrectRUFace2       navier_pressure.cpp:235-278
-//   UpdateTurKEpsilon    navier_scalar.cpp:11-197 (steady)
+//   UpdateTurKEpsilon    navier_scalar.cpp:11-197
+//   SaveTransientOldData navier.cpp:700-742
 //   BuildScalarMatrix    navier_scalar.cpp:407-592
-//   NSSolve loop         navier.cpp:52-141 (steady: one pass per step)
+//   NSSolve loop         navier.cpp:52-141
 // Pins: closed-form assembly at iteration 1 of a uniform cavity, M-matrix
 // sign structure, A_p * 1 = 0 and symmetry to rounding, linear-field
 // gradients, zero Rhie-Chow correction for uniform u and linear p, the
@@ -38,7 +39,6 @@ const double kappa = 0.419, Cmu = 0.09, Ceps1 = 1.44, Ceps2 = 1.92, Sigma_k = 1.
 
 Oracle::Oracle(const s355_mesh_desc& mesh, const s355_params& params) : par(params) {
   if (par.density_model != 0) throw std::runtime_error("only DensityModel 0 is on the hot path");
-  if (!par.steady) throw std::runtime_error("only steady runs are on the hot path");
   ingest(mesh);
   create_faces();
   cell_face_info();
@@ -55,6 +55,7 @@ void Oracle::allocate() {
   shear.assign(3 * static_cast<size_t>(Nbnd), 0.0);
   RUFace.assign(Nfac, 0.0);
   for (auto* v : {&Au, &Ap, &As}) v->assign(csr.nnz(), 0.0);
+  if (!par.steady) EnsureOld();
 }
 
 std::vector<double>* Oracle::field(int id) {
@@ -94,6 +95,18 @@ std::vector<double>* Oracle::field(int id) {
     case S355_F_BP_RHS: return &bp;
     case S355_F_BS_RHS: return &bs;
     case S355_F_BTEM: return &BTem;
+    case S355_F_UNP: return &Unp;
+    case S355_F_VNP: return &Vnp;
+    case S355_F_WNP: return &Wnp;
+    case S355_F_TNP: return &Tnp;
+    case S355_F_TEP: return &TEp;
+    case S355_F_EDP: return &EDp;
+    case S355_F_UNP2: return &Unp2;
+    case S355_F_VNP2: return &Vnp2;
+    case S355_F_WNP2: return &Wnp2;
+    case S355_F_TNP2: return &Tnp2;
+    case S355_F_TEP2: return &TEp2;
+    case S355_F_EDP2: return &EDp2;
     default: return nullptr;
   }
 }
@@ -234,9 +247,117 @@ void Oracle::Gradient(const std::vector<double>& phi, const std::vector<double>&
     }
     for (int d = 0; d < 3; ++d) g[3 * static_cast<size_t>(i) + d] = a[d] / cell[i].vol;
   }
+  if (par.limiter == 1)
+    Limiter_Barth(phi, g);
+  else if (par.limiter == 2)
+    Limiter_MLP(phi, g);
+  else if (par.limiter == 3)
+    Limiter_WENO(g);
+  else if (par.limiter != 0)
+    throw std::runtime_error("no such limiter choice");
+}
+
+namespace {
+// CYCASMAX / CYCASMIN (tools.h:11-12)
+inline double cmax(double x, double y) { return x > y ? x : y; }
+inline double cmin(double x, double y) { return x < y ? x : y; }
+// Barth_fun (navier_gradient.cpp:126-136)
+inline double barth_fun(double us, double umin, double umax) {
+  if (us > 1.0e-10) return cmin(1., umax / us);
+  if (us < -1.0e-10) return cmin(1., umin / us);
+  return 1.;
+}
+}  // namespace
+
+// Limiter_MLP (navier_gradient.cpp:159-224); one partition, so no virtual
+// cells enter the vertex extrema
+void Oracle::Limiter_MLP(const std::vector<double>& UC, std::vector<double>& g) {
+  std::vector<double> ficell(Ncel), umaxv(Nvrt, -1.e8), uminv(Nvrt, 1.e8);
+  for (int i = 0; i < Ncel; ++i)
+    for (int j = 0; j < 8; ++j) {
+      const int iv = cell[i].v[j];
+      umaxv[iv] = cmax(umaxv[iv], UC[i]);
+      uminv[iv] = cmin(uminv[iv], UC[i]);
+    }
+  for (int i = 0; i < Ncel; ++i) {
+    double fi = 10.;
+    const double* gi = &g[3 * static_cast<size_t>(i)];
+    for (int j = 0; j < 8; ++j) {
+      const int iv = cell[i].v[j];
+      const double umax = umaxv[iv] - UC[i];
+      const double umin = uminv[iv] - UC[i];
+      const double dx = vert[3 * static_cast<size_t>(iv)] - cell[i].x[0];
+      const double dy = vert[3 * static_cast<size_t>(iv) + 1] - cell[i].x[1];
+      const double dz = vert[3 * static_cast<size_t>(iv) + 2] - cell[i].x[2];
+      const double us = gi[0] * dx + gi[1] * dy + gi[2] * dz;
+      fi = cmin(fi, barth_fun(us, umin, umax));
+    }
+    ficell[i] = fi;
+  }
+  for (int i = 0; i < Ncel; ++i)
+    for (int d = 0; d < 3; ++d) g[3 * static_cast<size_t>(i) + d] = ficell[i] * g[3 * static_cast<size_t>(i) + d];
+}
+
+// Limiter_Barth (navier_gradient.cpp:226-265)
+void Oracle::Limiter_Barth(const std::vector<double>& UC, std::vector<double>& g) {
+  std::vector<double> ficell(Ncel);
+  for (int i = 0; i < Ncel; ++i) {
+    double fi = 10.;
+    double umax = UC[i], umin = UC[i];
+    for (int j = 0; j < cell[i].nface; ++j) {
+      const int in = cell[i].cell[j];
+      if (in < 0) continue;
+      umax = cmax(umax, UC[in]);
+      umin = cmin(umin, UC[in]);
+    }
+    umax -= UC[i];
+    umin -= UC[i];
+    const double* gi = &g[3 * static_cast<size_t>(i)];
+    for (int j = 0; j < 8; ++j) {
+      const int iv = cell[i].v[j];
+      const double dx = vert[3 * static_cast<size_t>(iv)] - cell[i].x[0];
+      const double dy = vert[3 * static_cast<size_t>(iv) + 1] - cell[i].x[1];
+      const double dz = vert[3 * static_cast<size_t>(iv) + 2] - cell[i].x[2];
+      const double us = gi[0] * dx + gi[1] * dy + gi[2] * dz;
+      fi = cmin(fi, barth_fun(us, umin, umax));
+    }
+    ficell[i] = fi;
+  }
+  for (int i = 0; i < Ncel; ++i)
+    for (int d = 0; d < 3; ++d) g[3 * static_cast<size_t>(i) + d] = ficell[i] * g[3 * static_cast<size_t>(i) + d];
+}
+
+// Limiter_WENO (navier_gradient.cpp:267-343)
+void Oracle::Limiter_WENO(std::vector<double>& g) {
+  std::vector<double> tmp(3 * static_cast<size_t>(Ncel), 0.);
+  for (int i = 0; i < Ncel; ++i) {
+    double ss = 0.;
+    double* t = &tmp[3 * static_cast<size_t>(i)];
+    for (int j = 0; j < cell[i].nface; ++j) {
+      const int ic = cell[i].cell[j];
+      if (ic < 0) continue;
+      const double* gc = &g[3 * static_cast<size_t>(ic)];
+      const double wei = 1. / (gc[0] * gc[0] + gc[1] * gc[1] + gc[2] * gc[2] + 1.e-16);
+      ss += wei;
+      for (int d = 0; d < 3; ++d) t[d] += wei * gc[d];
+    }
+    const double* gi = &g[3 * static_cast<size_t>(i)];
+    const double wei = 1. / (gi[0] * gi[0] + gi[1] * gi[1] + gi[2] * gi[2] + 1.e-16);
+    ss += wei;
+    for (int d = 0; d < 3; ++d) t[d] += wei * gi[d];
+    for (int d = 0; d < 3; ++d) t[d] /= ss;
+  }
+  g = tmp;
+}
+
+// old levels of an unsteady run read as zero before the first save
+void Oracle::EnsureOld() {
+  for (auto* v : {&Unp, &Vnp, &Wnp, &Tnp, &TEp, &EDp, &Unp2, &Vnp2, &Wnp2, &Tnp2, &TEp2, &EDp2})
+    if (v->size() != static_cast<size_t>(Ncel)) v->assign(Ncel, 0.);
 }
 
 void Oracle::BuildVelocityMatrix() {
+  if (!par.steady) EnsureOld();
   SetBCVelocity();
   SetBCPressure();
   Gradient(U, BU, dUdX);
@@ -253,6 +374,21 @@ void Oracle::BuildVelocityMatrix() {
     int nj = 0;
     for (int j = 0; j < kMaxFace; ++j) apn[j] = 0.;
     const Cell& C = cell[i];
+    // unsteady terms (navier_velocity.cpp:107-123)
+    if (!par.steady) {
+      const double coef = R[i] / par.dt * C.vol;
+      if (par.time_scheme == 1) {
+        app += coef;
+        su += coef * Unp[i];
+        sv += coef * Vnp[i];
+        sw += coef * Wnp[i];
+      } else if (par.time_scheme == 2) {
+        app += 1.5 * coef;
+        su += coef * (2 * Unp[i] - 0.5 * Unp2[i]);
+        sv += coef * (2 * Vnp[i] - 0.5 * Vnp2[i]);
+        sw += coef * (2 * Wnp[i] - 0.5 * Wnp2[i]);
+      }
+    }
     for (int j = 0; j < C.nface; ++j) {
       int iface = C.face[j];
       const Face& f = face[iface];
@@ -726,6 +862,11 @@ int Oracle::UpdateTurKEpsilon() {
     apte[i] = dis / TE[i] * vol;
   }
   SetBCKEpsilon(tes, eds, apte, aped, prod);
+  if (!par.steady) {  // navier_scalar.cpp:71-90
+    EnsureOld();
+    TransientScalar(TEp, TEp2, apte, tes);
+    TransientScalar(EDp, EDp2, aped, eds);
+  }
   BuildScalarMatrix(TE, BTE, viste, tes, apte);
   s355_ksp_opts o = par.ksp[S355_KSP_SCALAR];
   last[4] = ksp_solve(csr, As, bs, TE, o, false);
@@ -780,42 +921,110 @@ void Oracle::SetBCTemperature() {
   }
 }
 
-// UpdateEnergy (navier_scalar.cpp:204-326) for DensityModel 0, steady: the
-// conduction coefficient cp (mu/prl + mu_t/prte) / cp, no source, then the
-// generic scalar assembly and one GMRES solve into T.
+// UpdateEnergy (navier_scalar.cpp:204-326) for DensityModel 0: the
+// conduction coefficient cp (mu/prl + mu_t/prte) / cp, no source but the
+// time term of an unsteady run, then the generic scalar assembly and one
+// GMRES solve into T.
 int Oracle::UpdateEnergy() {
   std::vector<double> kcond(Ncel), esrc(Ncel, 0.), ape(Ncel, 0.);
   for (int i = 0; i < Ncel; ++i) kcond[i] = par.cp * (VisLam[i] / par.prl + VisTur[i] / par.prte) / par.cp;
   SetBCTemperature();
+  if (!par.steady) {  // navier_scalar.cpp:290-304
+    EnsureOld();
+    TransientScalar(Tnp, Tnp2, ape, esrc);
+  }
   BuildScalarMatrix(T, BTem, kcond, esrc, ape);
   last[6] = ksp_solve(csr, As, bs, T, par.ksp[S355_KSP_SCALAR], false);
   return last[6].reason < 0 ? S355_NOT_CONVERGED : S355_OK;
 }
 
+// transient source / diagonal of a scalar system: Euler or BDF2 on the
+// old levels (navier_scalar.cpp:71-90, 290-304)
+void Oracle::TransientScalar(const std::vector<double>& old, const std::vector<double>& old2, std::vector<double>& ap,
+                             std::vector<double>& src) {
+  for (int i = 0; i < Ncel; ++i) {
+    const double coef = R[i] / par.dt * cell[i].vol;
+    if (par.time_scheme == 1) {
+      ap[i] += coef;
+      src[i] += coef * old[i];
+    } else if (par.time_scheme == 2) {
+      ap[i] += 1.5 * coef;
+      src[i] += coef * (2 * old[i] - 0.5 * old2[i]);
+    }
+  }
+}
+
+// SaveTransientOldData (navier.cpp:700-742): BDF2 sets both old levels to
+// the current fields (the reference's chained assignment)
+void Oracle::SaveTransientOldData() {
+  if (par.time_scheme != 1 && par.time_scheme != 2)
+    throw std::runtime_error("No unsteady time advance scheme");
+  EnsureOld();
+  auto save = [&](const std::vector<double>& cur, std::vector<double>& np, std::vector<double>& np2) {
+    np = cur;
+    if (par.time_scheme == 2) np2 = cur;
+  };
+  save(U, Unp, Unp2);
+  save(V, Vnp, Vnp2);
+  save(W, Wnp, Wnp2);
+  save(T, Tnp, Tnp2);
+  if (par.turb_model == 1) {
+    save(TE, TEp, TEp2);
+    save(ED, EDp, EDp2);
+  }
+}
+
+int Oracle::SimplePass(double* resmax) {
+  std::fill(localRes, localRes + 10, 0.0);
+  int rc = CalculateVelocity();
+  if (rc != S355_OK) return rc;
+  rc = CalculatePressure();
+  if (rc != S355_OK) return rc;
+  if (par.turb_model == 1) {
+    rc = UpdateTurKEpsilon();
+    if (rc != S355_OK) return rc;
+  }
+  if (par.solve_energy) {
+    rc = UpdateEnergy();
+    if (rc != S355_OK) return rc;
+  }
+  *resmax = 0.;
+  for (int k = 0; k < 10; ++k) *resmax = std::max(*resmax, localRes[k]);
+  return S355_OK;
+}
+
+// NSSolve (navier.cpp:52-141): steady, one pass per step; unsteady, per
+// time step SaveTransientOldData, cur_time += dt and outer passes iter = 1 ..
+// MaxOuterStep - 1, leaving the loop once the residual has dropped by more
+// than 100; MaxStep = (int)(total_time / dt), and the run ends once
+// cur_time >= total_time.  Calls continue the run.
 int Oracle::SimpleIterate(int n_steps, double* history, int* done) {
   *done = 0;
+  const int max_time_steps = par.steady ? 0 : static_cast<int>(par.total_time / par.dt);
   for (int step = 1; step <= n_steps; ++step) {
-    std::fill(localRes, localRes + 10, 0.0);
-    int rc = CalculateVelocity();
-    if (rc != S355_OK) return rc;
-    rc = CalculatePressure();
-    if (rc != S355_OK) return rc;
-    if (par.turb_model == 1) {
-      rc = UpdateTurKEpsilon();
-      if (rc != S355_OK) return rc;
-    }
-    if (par.solve_energy) {
-      rc = UpdateEnergy();
-      if (rc != S355_OK) return rc;
-    }
     double resmax = 0.;
-    for (int k = 0; k < 10; ++k) resmax = std::max(resmax, localRes[k]);
+    if (par.steady) {
+      int rc = SimplePass(&resmax);
+      if (rc != S355_OK) return rc;
+    } else {
+      if (time_step >= max_time_steps) break;
+      SaveTransientOldData();
+      cur_time += par.dt;
+      ++time_step;
+      double resmax0 = 0.;
+      for (int iter = 1; iter < par.max_outer_step; ++iter) {
+        int rc = SimplePass(&resmax);
+        if (rc != S355_OK) return rc;
+        if (iter == 1) resmax0 = resmax;
+        if (resmax0 / (resmax + 1.e-16) > 100.) break;
+      }
+    }
     if (history) {
       for (int k = 0; k < 10; ++k) history[11 * (step - 1) + k] = localRes[k];
       history[11 * (step - 1) + 10] = resmax;
     }
     *done = step;
-    if (resmax < par.residual_steady) break;
+    if (par.steady ? resmax < par.residual_steady : cur_time >= par.total_time) break;
   }
   return S355_OK;
 }

@@ -116,6 +116,9 @@ def library(path: Optional[Path] = None) -> C.CDLL:
         "output_wait": (I, [V]),
         "read_backup": (I, [V, C.c_char_p]),
         "output_schedule": (I, [V, I, I, C.c_char_p]),
+        "save_transient_old_data": (I, [V]),
+        "unsteady_sources": (I, [V, I]),
+        "time_state": (I, [V, pD, pI]),
     }
     for name, (res, args) in more.items():
         fn = getattr(lib, f"s355_{name}")
@@ -382,6 +385,12 @@ class Solver:
     def set_params(self, params: Params) -> None:
         _check(self.lib, self.lib.s355_ctx_set_params(self.ctx, C.byref(params)), "set_params")
         self.params = params
+
+    def time_state(self) -> tuple:
+        """(cur_time, time steps taken) of an unsteady run."""
+        t, n = C.c_double(), C.c_int()
+        _check(self.lib, self.lib.s355_time_state(self.ctx, C.byref(t), C.byref(n)), "time_state")
+        return t.value, n.value
 
     # --- result and restart files
     def snapshot(self) -> None:

@@ -78,6 +78,12 @@ class Params(C.Structure):
         ("cp", C.c_double),
         ("prl", C.c_double),
         ("prte", C.c_double),
+        ("limiter", C.c_int),
+        ("time_scheme", C.c_int),
+        ("dt", C.c_double),
+        ("total_time", C.c_double),
+        ("max_outer_step", C.c_int),
+        ("transient_pad", C.c_int),
     ]
 
 
@@ -122,6 +128,7 @@ FIELDS = [
     "BRO", "BU", "BV", "BW", "BPRE", "BTE", "BED",
     "SHEAR", "RUFACE", "AU", "AP", "AS",
     "BU_RHS", "BV_RHS", "BW_RHS", "BP_RHS", "BS_RHS", "BTEM",
+    "UNP", "VNP", "WNP", "TNP", "TEP", "EDP", "UNP2", "VNP2", "WNP2", "TNP2", "TEP2", "EDP2",
 ]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 # which size each field has: c = cells, c3 = 3*cells, b = boundary, b3, f = faces, z = nnz
@@ -131,6 +138,7 @@ FIELD_KIND = {
     **{n: "c3" for n in ("DUDX", "DVDX", "DWDX", "DPDX", "DPHIDX")},
     **{n: "b" for n in ("BRO", "BU", "BV", "BW", "BPRE", "BTE", "BED", "BTEM")},
     "SHEAR": "b3", "RUFACE": "f", "AU": "z", "AP": "z", "AS": "z",
+    **{n: "c" for n in ("UNP", "VNP", "WNP", "TNP", "TEP", "EDP", "UNP2", "VNP2", "WNP2", "TNP2", "TEP2", "EDP2")},
 }
 
 MESH_INT = ["ELEM_PERM", "FACE_C1", "FACE_C2", "FACE_BND", "FACE_VERT", "CELL_NFACE", "CELL_FACE", "CELL_CELL",
@@ -196,6 +204,8 @@ def declare(lib: C.CDLL, prefix: str, ctx_type=C.c_void_p) -> None:
         "correct_fields": (I, [V, pD]),
         "correct_ru_face": (I, [V]),
         "simple_iterate": (I, [V, I, pD, pI]),
+        "save_transient_old_data": (I, [V]),
+        "time_state": (I, [V, pD, pI]),
     }
     for name, (res, args) in spec.items():
         fn = getattr(lib, f"{P}_{name}")

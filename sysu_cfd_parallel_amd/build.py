@@ -23,6 +23,7 @@ SOURCES = [
     "mesh/gmsh.cpp",
     "device/fv_kernels.hip",
     "device/keps_kernels.hip",
+    "device/transient_kernels.hip",
     "device/krylov.hip",
     "device/context.hip",
     "host/navier.cpp",

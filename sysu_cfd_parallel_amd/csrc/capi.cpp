@@ -152,6 +152,13 @@ void check_params(const s355_params* p) {
   if (!(p->urf[0] > 0.0) || !(p->urf[5] > 0.0)) throw std::invalid_argument("relaxation factors must be positive");
   if (p->solve_energy && !(p->cp > 0.0 && p->prl > 0.0 && p->prte > 0.0))
     throw std::invalid_argument("energy: cp, prl and prte must be positive");
+  if (p->limiter < 0 || p->limiter > 3) throw std::invalid_argument("limiter: 0 none, 1 Barth, 2 MLP, 3 WENO");
+  if (p->steady != 0 && p->steady != 1) throw std::invalid_argument("steady: 0 or 1");
+  if (!p->steady) {
+    if (p->time_scheme != 1 && p->time_scheme != 2) throw std::invalid_argument("time_scheme: 1 Euler, 2 BDF2");
+    if (!(p->dt > 0.0)) throw std::invalid_argument("dt must be positive");
+    if (p->max_outer_step < 2) throw std::invalid_argument("max_outer_step must be at least 2");
+  }
 }
 
 // the message of a solve that ended without converging (its rc is not an
@@ -191,6 +198,11 @@ void s355_default_params(s355_params* p) {
   p->cp = 1006.;        // navier.cpp:489-492
   p->prl = 0.72;
   p->prte = 0.9;
+  p->limiter = 0;  // navier.cpp:509
+  p->time_scheme = 1;  // navier.cpp:513-523
+  p->dt = 0.1;
+  p->total_time = 0.;
+  p->max_outer_step = 50;  // navier.cpp:476
 }
 
 int s355_case_build(int kind, const int* ip, const double* dp, s355_case** out) {
@@ -748,6 +760,30 @@ int s355_update_tur_kepsilon(s355_ctx* ctx, s355_solve_info info[2]) {
 int s355_energy_coefficients(s355_ctx* ctx) {
   return guarded([&] {
     dev(ctx).energy_coefficients();
+    return S355_OK;
+  });
+}
+
+int s355_save_transient_old_data(s355_ctx* ctx) {
+  return guarded([&] {
+    dev(ctx).save_old_data();
+    return S355_OK;
+  });
+}
+
+int s355_unsteady_sources(s355_ctx* ctx, int isca) {
+  return guarded([&] {
+    dev(ctx).unsteady_sources(isca);
+    return S355_OK;
+  });
+}
+
+int s355_time_state(s355_ctx* ctx, double* cur_time, int* time_steps) {
+  return guarded([&] {
+    dev(ctx);
+    if (!cur_time || !time_steps) throw std::invalid_argument("null output");
+    *cur_time = ctx->ns->cur_time;
+    *time_steps = ctx->ns->time_steps();
     return S355_OK;
   });
 }

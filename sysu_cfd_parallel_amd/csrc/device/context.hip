@@ -39,7 +39,6 @@ auto upload(std::vector<void*>& allocs, const Vec& v, hipStream_t s) {
 
 Context::Context(const s355_mesh_desc* descs, int n_case, const s355_params& params) : par(params), ncase(n_case) {
   if (par.density_model != 0) throw std::invalid_argument("only DensityModel 0 (constant density) is supported");
-  if (!par.steady) throw std::invalid_argument("only steady runs are supported");
   if (par.precision != S355_FP64 && par.precision != S355_MIXED) throw std::invalid_argument("bad precision mode");
   if (!descs || ncase < 1) throw std::invalid_argument("no mesh");
   if (ncase > 1)
@@ -257,7 +256,7 @@ void Context::allocate_fields() {
 }
 
 void Context::set_params(const s355_params& p) {
-  if (p.density_model != 0 || !p.steady) throw std::invalid_argument("unsupported model switch");
+  if (p.density_model != 0) throw std::invalid_argument("unsupported model switch");
   for (int k = 0; k < 3; ++k)
     if (std::min(p.ksp[k].restart, kMaxRestart) > max_restart_)
       throw std::invalid_argument("restart larger than the workspace built at context creation");
@@ -308,6 +307,14 @@ FieldRef Context::field(int id) {
     case S355_F_BP_RHS: return {df.bp, Nc};
     case S355_F_BS_RHS: return {df.bs, Nc};
     case S355_F_BTEM: return {df.BTem, Nb};
+    case S355_F_UNP: case S355_F_VNP: case S355_F_WNP: case S355_F_TNP: case S355_F_TEP: case S355_F_EDP:
+    case S355_F_UNP2: case S355_F_VNP2: case S355_F_WNP2: case S355_F_TNP2: case S355_F_TEP2: case S355_F_EDP2: {
+      double* old[12] = {df.Unp, df.Vnp, df.Wnp, df.Tnp, df.TEp, df.EDp,
+                         df.Unp2, df.Vnp2, df.Wnp2, df.Tnp2, df.TEp2, df.EDp2};
+      double* p = old[id - S355_F_UNP];
+      if (!p) throw std::invalid_argument("old time levels exist once an unsteady run has saved them");
+      return {p, Nc};
+    }
     default: throw std::invalid_argument("unknown field id " + std::to_string(id));
   }
 }
@@ -392,6 +399,16 @@ void Context::bc_velocity() { launch_bc_velocity(dm, df, has_outlet_, partial_, 
 void Context::bc_pressure() { launch_bc_copy(dm, df.P, df.BPre, stream); }
 void Context::bc_deltap() { launch_bc_copy(dm, df.dp, df.BPre, stream); }
 
+// Green-Gauss gradients of nf fields, then the limiter on each (Gradient,
+// navier_gradient.cpp:12-74)
+void Context::grad(int nf, const double* const* phi, const double* const* bphi, double* const* out) {
+  launch_gradient(dm, nf, phi, bphi, out, stream);
+  if (par.limiter == 0) return;
+  ensure_vertices();
+  for (int k = 0; k < nf; ++k)
+    launch_limiter(dm, par.limiter, phi[k], out[k], lim_vmax_, lim_vmin_, lim_tmp_, stream);
+}
+
 void Context::gradient(int field_id, int bfield_id, int out_id) {
   FieldRef f = field(field_id), b = field(bfield_id), g = field(out_id);
   if (f.n != static_cast<size_t>(dm.Nc) || b.n != static_cast<size_t>(dm.Nb) || g.n != 3 * static_cast<size_t>(dm.Nc))
@@ -399,7 +416,7 @@ void Context::gradient(int field_id, int bfield_id, int out_id) {
   const double* phi[1] = {f.p};
   const double* bphi[1] = {b.p};
   double* out[1] = {g.p};
-  launch_gradient(dm, 1, phi, bphi, out, stream);
+  grad(1, phi, bphi, out);
 }
 
 void Context::build_velocity_matrix() {
@@ -408,8 +425,68 @@ void Context::build_velocity_matrix() {
   const double* phi[4] = {df.U, df.V, df.W, df.P};
   const double* bphi[4] = {df.BU, df.BV, df.BW, df.BPre};
   double* out[4] = {df.dUdX, df.dVdX, df.dWdX, df.dPdX};
-  launch_gradient(dm, 4, phi, bphi, out, stream);
-  launch_build_momentum(dm, df, par.urf[0], par.gravity, flags_, stream);
+  grad(4, phi, bphi, out);
+  const int scheme = par.steady ? 0 : par.time_scheme;
+  if (scheme) ensure_transient();
+  launch_build_momentum(dm, df, par.urf[0], par.gravity, flags_, scheme, par.dt, stream);
+}
+
+// vertex arrays of the limiters: each cell's 8 vertices, the coordinates, and
+// the cells around each vertex (each cell once, ascending)
+void Context::ensure_vertices() {
+  if (dm.c_vert) return;
+  const int nv = mesh.n_vert, nc = mesh.n_cell;
+  std::vector<int> ptr(nv + 1, 0);
+  auto distinct = [&](int i, int j) {
+    const int* h = &mesh.c_vert[8 * static_cast<size_t>(i)];
+    for (int q = 0; q < j; ++q)
+      if (h[q] == h[j]) return false;
+    return true;
+  };
+  for (int i = 0; i < nc; ++i)
+    for (int j = 0; j < 8; ++j)
+      if (distinct(i, j)) ++ptr[mesh.c_vert[8 * static_cast<size_t>(i) + j] + 1];
+  for (int v = 0; v < nv; ++v) ptr[v + 1] += ptr[v];
+  std::vector<int> v2c(ptr[nv]), fill(ptr.begin(), ptr.end() - 1);
+  for (int i = 0; i < nc; ++i)
+    for (int j = 0; j < 8; ++j)
+      if (distinct(i, j)) v2c[fill[mesh.c_vert[8 * static_cast<size_t>(i) + j]]++] = i;
+  std::vector<int> cv(mesh.c_vert.begin(), mesh.c_vert.end());
+  std::vector<double> xyz(mesh.vert.begin(), mesh.vert.end());
+  dm.Nv = nv;
+  dm.v2c_ptr = upload(allocs_, ptr, stream);
+  dm.v2c = upload(allocs_, v2c, stream);
+  dm.c_vert = upload(allocs_, cv, stream);
+  dm.vert = upload(allocs_, xyz, stream);
+  lim_vmax_ = alloc<double>(std::max(nv, 1));
+  lim_vmin_ = alloc<double>(std::max(nv, 1));
+  lim_tmp_ = alloc<double>(3 * static_cast<size_t>(dm.Nc));
+  sync();  // host staging vectors go out of scope
+}
+
+// old time levels and the energy time-term arrays, zeroed once
+void Context::ensure_transient() {
+  if (df.Unp) return;
+  const size_t Nc = dm.Nc;
+  double** old[] = {&df.Unp, &df.Vnp, &df.Wnp, &df.Tnp, &df.TEp, &df.EDp,
+                    &df.Unp2, &df.Vnp2, &df.Wnp2, &df.Tnp2, &df.TEp2, &df.EDp2, &kw_.esrc, &kw_.ape};
+  for (double** p : old) {
+    *p = alloc<double>(Nc);
+    fill_zero(*p, Nc * sizeof(double), stream);
+  }
+}
+
+void Context::save_old_data() {
+  if (par.steady) throw std::logic_error("SaveTransientOldData: the run is steady");
+  ensure_transient();
+  launch_save_old(dm, df, par.time_scheme, par.turb_model == 1, stream);
+}
+
+void Context::unsteady_sources(int isca) {
+  if (isca != 1 && isca != 2) throw std::invalid_argument("unsteady_sources: isca 1 (T) or 2 (k and epsilon)");
+  if (par.steady) return;
+  ensure_transient();
+  launch_unsteady_scalar(dm, df, kw_, isca, par.time_scheme, par.dt, stream);
 }
 
 namespace {
@@ -552,7 +629,7 @@ void Context::correct_fields() {
   const double* phi[1] = {df.dp};
   const double* bphi[1] = {df.BPre};
   double* out[1] = {df.dPdX};
-  launch_gradient(dm, 1, phi, bphi, out, stream);
+  grad(1, phi, bphi, out);
   launch_correct_fields(dm, df, par.cell_pressure_ref, par.urf[3], partial_, stream);
   launch_reduce_tiles(partial_, dm.ctile, 1, res_ + 3, 10, true, stream);
 }
@@ -589,8 +666,10 @@ void Context::build_scalar_matrix(int isca) {
     phi = df.T;
     bphi = df.BTem;
     diff = kw_.kcond;
-    src = kw_.zero;
-    app = kw_.zero;
+    // unsteady: the time term written by unsteady_sources(1)
+    if (!par.steady) ensure_transient();
+    src = par.steady ? kw_.zero : kw_.esrc;
+    app = par.steady ? kw_.zero : kw_.ape;
   } else if (isca == 2) {
     phi = df.TE;
     bphi = df.BTE;
@@ -609,7 +688,7 @@ void Context::build_scalar_matrix(int isca) {
   const double* p1[1] = {phi};
   const double* b1[1] = {bphi};
   double* g1[1] = {df.dPhidX};
-  launch_gradient(dm, 1, p1, b1, g1, stream);
+  grad(1, p1, b1, g1);
   launch_build_scalar(dm, df, phi, bphi, diff, src, app, par.urf[5], flags_, stream);
 }
 

@@ -63,6 +63,10 @@ class Context {
   void kepsilon_post_eps();
   void bc_temperature();
   void energy_coefficients();
+  // unsteady runs: SaveTransientOldData, and the time terms of the scalar
+  // systems (isca 2: k and epsilon, after bc_kepsilon; isca 1: energy)
+  void save_old_data();
+  void unsteady_sources(int isca);
 
   // residual slots and error flags on the device
   void zero_residuals();
@@ -101,6 +105,9 @@ class Context {
   void ensure_ilu();
   void ensure_sell();
   void prepare_sell(const double* csr_val, bool fp32, bool all);
+  void grad(int nf, const double* const* phi, const double* const* bphi, double* const* out);
+  void ensure_vertices();
+  void ensure_transient();
 
   std::vector<void*> allocs_;
   double* partial_ = nullptr;
@@ -111,6 +118,7 @@ class Context {
   KHeader* hdr_ = nullptr;     // pinned, one Krylov header per case
   int* flags_ = nullptr;       // [0] bad boundary type, [1] inlet outflow in a scalar
   double *u0_ = nullptr, *v0_ = nullptr, *w0_ = nullptr;
+  double *lim_vmax_ = nullptr, *lim_vmin_ = nullptr, *lim_tmp_ = nullptr;  // limiter scratch
   KepsWork kw_{};
   KWork kwork_{};
   // mixed precision: FP32 matrix copy, FP32 Krylov workspace, refinement buffers

@@ -11,8 +11,9 @@ void launch_gradient(const DMesh& m, int nf, const double* const* phi, const dou
 void launch_bc_velocity(const DMesh& m, const DFields& F, bool has_outlet, double* partial, double* rate,
                         hipStream_t s);
 void launch_bc_copy(const DMesh& m, const double* cellv, double* bval, hipStream_t s);
-void launch_build_momentum(const DMesh& m, const DFields& F, double urf, const double* grav, int* bad,
-                           hipStream_t s);
+// scheme: 0 steady, 1 Euler, 2 BDF2 (time step dt, old levels in F)
+void launch_build_momentum(const DMesh& m, const DFields& F, double urf, const double* grav, int* bad, int scheme,
+                           double dt, hipStream_t s);
 void launch_ruface(const DMesh& m, const DFields& F, hipStream_t s);
 void launch_build_pressure(const DMesh& m, const DFields& F, hipStream_t s);
 void launch_correct_fields(const DMesh& m, const DFields& F, int ref, double urf, double* partial, hipStream_t s);
@@ -28,6 +29,7 @@ void launch_reduce_tiles(const double* partial, const Tiles& t, int ncomp, doubl
 struct KepsWork {
   double *prod, *tes, *eds, *viste, *visted, *apte, *aped;
   double *kcond, *zero;  // energy: conduction coefficient, a zero source / diagonal
+  double *esrc, *ape;    // energy, unsteady: source and diagonal of the time term
 };
 void launch_keps_sources(const DMesh& m, const DFields& F, const KepsWork& w, hipStream_t s);
 void launch_keps_wall(const DMesh& m, const DFields& F, const KepsWork& w, double cmu25, double cmu75,
@@ -44,5 +46,18 @@ void launch_turb_viscosity(const DMesh& m, const DFields& F, hipStream_t s);
 void launch_bc_temperature(const DMesh& m, const DFields& F, int* bad, hipStream_t s);
 void launch_energy_coef(const DMesh& m, const DFields& F, const KepsWork& w, double cp, double prl, double prte,
                         hipStream_t s);
+
+// gradient limiters and time terms (transient_kernels.hip)
+// Limiter_Barth / Limiter_MLP / Limiter_WENO (navier_gradient.cpp:126-345)
+// applied to the gradient g of phi in place; vmax / vmin (Nv each) and tmp
+// (3 Nc) are scratch.
+void launch_limiter(const DMesh& m, int limiter, const double* phi, double* g, double* vmax, double* vmin, double* tmp,
+                    hipStream_t s);
+// SaveTransientOldData: the old levels from the current fields
+void launch_save_old(const DMesh& m, const DFields& F, int scheme, bool turb, hipStream_t s);
+// transient source / diagonal blocks: isca 2 k-epsilon (navier_scalar.cpp:
+// 71-90), isca 1 energy (290-304, written, not added)
+void launch_unsteady_scalar(const DMesh& m, const DFields& F, const KepsWork& w, int isca, int scheme, double dt,
+                            hipStream_t s);
 
 }  // namespace s355

@@ -221,14 +221,30 @@ void launch_bc_copy(const DMesh& m, const double* cellv, double* bval, hipStream
 struct MomArgs {
   double urf;
   double g0, g1, g2;
-  int* bad;  // set when a boundary type is unknown
+  int* bad;    // set when a boundary type is unknown
+  int scheme;  // 0 steady, 1 Euler, 2 BDF2
+  double dt;
 };
 
-// BuildVelocityMatrix cell loop (navier_velocity.cpp:100-383), steady form.
+// BuildVelocityMatrix cell loop (navier_velocity.cpp:100-383); the time
+// term (107-123) comes first, as in the reference.
 __global__ void __launch_bounds__(kBlock) k_build_momentum(DMesh m, DFields F, MomArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.Nc) return;
   double app = 0., su = 0., sv = 0., sw = 0.;
+  if (a.scheme == 1) {
+    const double coef = F.R[i] / a.dt * m.c_vol[i];
+    app += coef;
+    su += coef * F.Unp[i];
+    sv += coef * F.Vnp[i];
+    sw += coef * F.Wnp[i];
+  } else if (a.scheme == 2) {
+    const double coef = F.R[i] / a.dt * m.c_vol[i];
+    app += 1.5 * coef;
+    su += coef * (2 * F.Unp[i] - 0.5 * F.Unp2[i]);
+    sv += coef * (2 * F.Vnp[i] - 0.5 * F.Vnp2[i]);
+    sw += coef * (2 * F.Wnp[i] - 0.5 * F.Wnp2[i]);
+  }
   const double xi0 = m.c_x[3 * i], xi1 = m.c_x[3 * i + 1], xi2 = m.c_x[3 * i + 2];
   const double Ui = F.U[i], Vi = F.V[i], Wi = F.W[i];
   const double mui = F.mu[i] + F.mut[i];
@@ -384,9 +400,9 @@ __global__ void __launch_bounds__(kBlock) k_build_momentum(DMesh m, DFields F, M
   F.bw[i] = sw + (1. - a.urf) * app * Wi + (-gp[2] + Ri * a.g2) * vol;
 }
 
-void launch_build_momentum(const DMesh& m, const DFields& F, double urf, const double* grav, int* bad,
-                           hipStream_t s) {
-  MomArgs a{urf, grav[0], grav[1], grav[2], bad};
+void launch_build_momentum(const DMesh& m, const DFields& F, double urf, const double* grav, int* bad, int scheme,
+                           double dt, hipStream_t s) {
+  MomArgs a{urf, grav[0], grav[1], grav[2], bad, scheme, dt};
   hipLaunchKernelGGL(k_build_momentum, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, F, a);
   S355_LAUNCH_CHECK();
 }

@@ -45,6 +45,14 @@ struct DMesh {
   const int* wc_cell = nullptr;
   const int* wc_ptr = nullptr;
   const int* wc_face = nullptr;
+  // vertices, for the gradient limiters (uploaded when a limiter is on):
+  // each cell's 8 vertices (degenerate hex), coordinates, and the cells of
+  // each vertex (ascending, one entry per cell)
+  int Nv = 0;
+  const int* c_vert = nullptr;
+  const double* vert = nullptr;
+  const int* v2c_ptr = nullptr;
+  const int* v2c = nullptr;
 };
 
 struct DFields {
@@ -53,6 +61,10 @@ struct DFields {
   double *BRo, *BU, *BV, *BW, *BPre, *BTE, *BED, *BTem, *shear;
   double* RUFace;
   double *Au, *Ap, *As, *bu, *bv, *bw, *bp, *bs;
+  // old time levels (unsteady runs; null otherwise): *np the previous step,
+  // *np2 the one before (SaveTransientOldData, navier.cpp:700-742)
+  double *Unp, *Vnp, *Wnp, *Tnp, *TEp, *EDp;
+  double *Unp2, *Vnp2, *Wnp2, *Tnp2, *TEp2, *EDp2;
 };
 
 }  // namespace s355

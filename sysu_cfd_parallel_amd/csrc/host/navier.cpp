@@ -115,6 +115,8 @@ void NavierStokesSolver::UpdateTurKEpsilon() {
   try {
     check(s355_kepsilon_sources(ctx_), "k-eps sources");
     SetBCKEpsilon();
+    // time terms after the boundary treatment (navier_scalar.cpp:71-90)
+    if (!par_.steady) check(s355_unsteady_sources(ctx_, 2), "k-eps time terms");
     BuildScalarMatrix(2);
     dataPartition.solveScarlar_GMRES(o.rtol, o.maxit, 2, &last[4]);
     check(s355_kepsilon_post_k(ctx_), "k clip / eps sources");
@@ -131,13 +133,14 @@ void NavierStokesSolver::UpdateTurKEpsilon() {
 
 void NavierStokesSolver::SetBCTemperature() { check(s355_bc_temperature(ctx_), "SetBCTemperature"); }
 
-// UpdateEnergy (navier_scalar.cpp:204-326) for DensityModel 0, steady runs.
+// UpdateEnergy (navier_scalar.cpp:204-326) for DensityModel 0.
 void NavierStokesSolver::UpdateEnergy() {
   check(s355_log_stage_push(ctx_, 5), "log");
   const s355_ksp_opts& o = par_.ksp[S355_KSP_SCALAR];
   try {
     check(s355_energy_coefficients(ctx_), "energy coefficients");
     SetBCTemperature();
+    if (!par_.steady) check(s355_unsteady_sources(ctx_, 1), "energy time term");  // 290-304
     BuildScalarMatrix(1);
     dataPartition.solveScarlar_GMRES(o.rtol, o.maxit, 1, &last[6]);
   } catch (...) {
@@ -212,24 +215,54 @@ void NavierStokesSolver::output_step(int step) {
   check(s355_output_write(ctx_, result, result_path(result).c_str(), 1), "Output2Tecplot");
 }
 
+void NavierStokesSolver::SaveTransientOldData() { check(s355_save_transient_old_data(ctx_), "SaveTransientOldData"); }
+
+// One SIMPLE pass (navier.cpp:62-96); returns ResMax and leaves the ten
+// residuals in Residual.
+double NavierStokesSolver::simple_pass() {
+  check(s355_residuals(ctx_, Residual, 1), "residual reset");
+  CalculateVelocity();
+  CalculatePressure();
+  if (par_.turb_model == 1) UpdateTurKEpsilon();
+  if (par_.solve_energy) UpdateEnergy();
+  check(s355_residuals(ctx_, Residual, 0), "residuals");
+  double resmax = 0.0;  // vec_max starts at 0 (tools.cpp:45-51)
+  for (double r : Residual) resmax = std::max(resmax, r);
+  return resmax;
+}
+
+// NSSolve (navier.cpp:52-141).  Steady: one SIMPLE pass per step until
+// ResMax < ResidualSteady.  Unsteady: per time step SaveTransientOldData,
+// cur_time += dt, then outer passes iter = 1 .. MaxOuterStep - 1 until the
+// residual has dropped by more than 100 from the first pass; the run ends
+// after MaxStep = (int)(total_time / dt) steps or once cur_time >= total_time.
+// Calls continue the run: max_step bounds the steps of this call.
 int NavierStokesSolver::NSSolve(int max_step, double* history, int* steps_done) {
   *steps_done = 0;
+  const int max_time_steps = par_.steady ? 0 : static_cast<int>(par_.total_time / par_.dt);
   for (int step = 1; step <= max_step; ++step) {
-    check(s355_residuals(ctx_, Residual, 1), "residual reset");
-    CalculateVelocity();
-    CalculatePressure();
-    if (par_.turb_model == 1) UpdateTurKEpsilon();
-    if (par_.solve_energy) UpdateEnergy();
-    check(s355_residuals(ctx_, Residual, 0), "residuals");
-    double resmax = 0.0;  // vec_max starts at 0 (tools.cpp:45-51)
-    for (double r : Residual) resmax = std::max(resmax, r);
+    double resmax = 0.0;
+    if (par_.steady) {
+      resmax = simple_pass();
+    } else {
+      if (time_step_ >= max_time_steps) break;
+      SaveTransientOldData();
+      cur_time += par_.dt;
+      ++time_step_;
+      double resmax0 = 0.0;
+      for (int iter = 1; iter < par_.max_outer_step; ++iter) {
+        resmax = simple_pass();
+        if (iter == 1) resmax0 = resmax;
+        if (resmax0 / (resmax + 1.e-16) > 100.) break;
+      }
+    }
     if (history) {
       std::copy(Residual, Residual + 10, history + 11 * (step - 1));
       history[11 * (step - 1) + 10] = resmax;
     }
     *steps_done = step;
-    output_step(step);
-    if (resmax < par_.residual_steady) break;
+    output_step(par_.steady ? step : time_step_);
+    if (par_.steady ? resmax < par_.residual_steady : cur_time >= par_.total_time) break;
   }
   if (noutput_ > 0) {
     Output2Tecplot(false);  // the last result (navier.cpp:149), written before returning

@@ -63,8 +63,12 @@ class NavierStokesSolver {
   void SetBCKEpsilon();
   void SetBCTemperature();
 
-  // NSSolve steady loop; history gets 11 doubles per step.
+  // NSSolve loop; history gets 11 doubles per step (time step when
+  // unsteady: the residuals of its last outer pass).
   int NSSolve(int max_step, double* history, int* steps_done);
+  void SaveTransientOldData();
+  double cur_time = 0.0;  // physical time of an unsteady run
+  int time_steps() const { return time_step_; }
 
   // result and restart files (navierIO.cpp), under the output directory
   void SetOutput(int noutput, bool binary, const std::string& dir);
@@ -81,6 +85,8 @@ class NavierStokesSolver {
  private:
   void check(int rc, const char* what);
   void output_step(int step);
+  double simple_pass();
+  int time_step_ = 0;  // time steps taken (unsteady)
   std::string result_path(int format) const;
   s355_ctx* ctx_;
   s355_params par_;
