@@ -278,6 +278,7 @@ __global__ void __launch_bounds__(kBlock) k_spmv_pc(CsrView<T> A, Tiles t, const
 
 // ---- the same product on the SELL-64 copy: one row per lane, the row's
 // entries summed in CSR order
+constexpr int kSellBatch = 8;
 template <class T, bool NT>
 __global__ void __launch_bounds__(kBlock) k_spmv_sell(SellView<T> A, Tiles t, const T* __restrict__ x,
                                                       const T* __restrict__ inv, T* __restrict__ y, const KState* st,
@@ -290,7 +291,25 @@ __global__ void __launch_bounds__(kBlock) k_spmv_sell(SellView<T> A, Tiles t, co
   const int sl = pos >> 6, lane = pos & 63;
   const int q1 = A.ptr[sl + 1];
   T acc = T(0);
-  for (int q = A.ptr[sl] + lane; q < q1; q += 64) acc += ld_stream<T, NT>(A.val + q) * x[ld_stream<int, NT>(A.col + q)];
+  // kSellBatch slots at a time: every column and value load of the batch is
+  // issued, then every gather of x, then the products are added in slot
+  // order (the wave's slice has one width, so the guards are wave-uniform)
+  for (int q0 = A.ptr[sl] + lane; q0 < q1; q0 += 64 * kSellBatch) {
+    T v[kSellBatch], xv[kSellBatch];
+    int c[kSellBatch];
+#pragma unroll
+    for (int u = 0; u < kSellBatch; ++u)
+      if (q0 + 64 * u < q1) {
+        v[u] = ld_stream<T, NT>(A.val + q0 + 64 * u);
+        c[u] = ld_stream<int, NT>(A.col + q0 + 64 * u);
+      }
+#pragma unroll
+    for (int u = 0; u < kSellBatch; ++u)
+      if (q0 + 64 * u < q1) xv[u] = x[c[u]];
+#pragma unroll
+    for (int u = 0; u < kSellBatch; ++u)
+      if (q0 + 64 * u < q1) acc += v[u] * xv[u];
+  }
   double z = static_cast<double>(acc);
   if (inv) z *= static_cast<double>(inv[row]);
   y[row] = static_cast<T>(z * sc->scale[it]);
