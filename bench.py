@@ -38,6 +38,12 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the production SpMV on the channel workload's
+# pressure-correction matrix (5,995,990 cells), from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE passes scaled by a 1 GiB copy (profiles/hbm_traffic_channel_6M.md)
+MEASURED_TRAFFIC = {
+    ("k_spmv_sell<float, true, true>", 5995990): 437.8e6,
+}
 METRIC = "SIMPLE iters/s @4M cells, FP64 vs mixed (<=1e-6 rel err); SpMV % HBM peak"
 STAGES = ["total", "build_velocity", "solve_velocity", "build_pressure", "solve_pressure", "other"]
 
@@ -247,16 +253,17 @@ WORKLOADS = {w.name: w for w in (Channel, Cavity, Step, Large, Sweep)}
 
 # --------------------------------------------------------------------------- measurement
 
-def spmv_bytes(nnz: int, nc: int, b_val: int, b_x: int) -> float:
+def spmv_bytes(nnz: int, nc: int, b_val: int, b_x: int, jacobi: bool = True) -> float:
     # algorithmic bytes of the production y = M^-1 A x: CSR values + column
-    # ids, row pointers, x, y and the Jacobi inverse diagonal
-    return (b_val + 4) * nnz + 4 * (nc + 1) + 3 * b_x * nc
+    # ids, row pointers, x, y and (Jacobi) the inverse diagonal
+    return (b_val + 4) * nnz + 4 * (nc + 1) + (3 if jacobi else 2) * b_x * nc
 
 
-def spmv_kernel_name(shape: dict, elem: str) -> str:
+def spmv_kernel_name(shape: dict, elem: str, jacobi: bool = True) -> str:
+    tf = lambda v: "true" if v else "false"  # noqa: E731
     if shape["lanes"] == 0:
-        return f"k_spmv_sell<{elem}, {'true' if shape['nt'] else 'false'}>"
-    return f"k_spmv_pc<{elem}, {shape['lanes']}, {'true' if shape['nt'] else 'false'}>"
+        return f"k_spmv_sell<{elem}, {tf(shape['nt'])}, {tf(jacobi)}>"
+    return f"k_spmv_pc<{elem}, {shape['lanes']}, {tf(shape['nt'])}>"
 
 
 def cpu_baseline(wl, threads: int, budget_s: float = 20.0, max_steps: int = 3) -> dict:
@@ -420,7 +427,13 @@ def main() -> int:
     dom = dict(spmv["fp32" if main_name == "mixed" else "fp64"])
     ins = r["spmv_in_solve"]
     if ins["launches_timed"] > 0:
-        # the roofline of the launches inside the timed solves themselves
+        # the roofline of the launches inside the timed solves themselves;
+        # with ILU(0) the Arnoldi SpMV carries no Jacobi scaling
+        jacobi = dev.params.ksp[s.KSP_PRESSURE].pc == s.PC_JACOBI
+        b = 4 if main_name == "mixed" else 8
+        dom["kernel"] = spmv_kernel_name(dev.kernel_shapes()[main_name if main_name == "fp64" else "fp32"],
+                                         "float" if b == 4 else "double", jacobi)
+        dom["bytes_per_launch"] = spmv_bytes(z.nnz, z.n_cell, b, b, jacobi)
         dom["ms_per_launch"] = ins["ms_per_launch"]
         dom["achieved"] = dom["bytes_per_launch"] / (ins["ms_per_launch"] * 1e-3) / 1e9
         dom["frac"] = dom["achieved"] / HBM_PEAK_GBS
@@ -469,7 +482,10 @@ def main() -> int:
         "roofline": {
             "bound": "hbm", "kernel": dom["kernel"] + " on the pressure-correction matrix",
             "achieved": dom["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dom["frac"],
-            "traffic": None, "bytes_per_launch": dom["bytes_per_launch"], "ms_per_launch": dom["ms_per_launch"],
+            "traffic": MEASURED_TRAFFIC.get((dom["kernel"], z.n_cell)),
+            "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/hbm_traffic_channel_6M.md"
+            if (dom["kernel"], z.n_cell) in MEASURED_TRAFFIC else None,
+            "bytes_per_launch": dom["bytes_per_launch"], "ms_per_launch": dom["ms_per_launch"],
             "timing": dom["timing"],
             "measured_copy_gbs": copy_gbs, "frac_of_measured_copy": dom["achieved"] / copy_gbs,
         },

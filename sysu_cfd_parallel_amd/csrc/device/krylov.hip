@@ -279,22 +279,27 @@ __global__ void __launch_bounds__(kBlock) k_spmv_pc(CsrView<T> A, Tiles t, const
 // ---- the same product on the SELL-64 copy: one row per lane, the row's
 // entries summed in CSR order
 constexpr int kSellBatch = 8;
-template <class T, bool NT>
+template <class T, bool NT, bool PC>
 __global__ void __launch_bounds__(kBlock) k_spmv_sell(SellView<T> A, Tiles t, const T* __restrict__ x,
                                                       const T* __restrict__ inv, T* __restrict__ y, const KState* st,
                                                       int it) {
   const int pos = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (pos >= A.n) return;
+  // the slice bounds, the row and (one case) the state load together
+  const int sl = pos >> 6, lane = pos & 63;
+  const int qa = A.ptr[sl], q1 = A.ptr[sl + 1];
   const int row = A.row[pos];
   const KState* sc = st + t.case_of(row);
   if (sc->reason) return;
-  const int sl = pos >> 6, lane = pos & 63;
-  const int q1 = A.ptr[sl + 1];
+  // loaded ahead of the slots, used after them (PC: Jacobi, inv is set)
+  const double scl = sc->scale[it];
+  T invr = T(1);
+  if constexpr (PC) invr = inv[row];
   T acc = T(0);
   // kSellBatch slots at a time: every column and value load of the batch is
   // issued, then every gather of x, then the products are added in slot
   // order (the wave's slice has one width, so the guards are wave-uniform)
-  for (int q0 = A.ptr[sl] + lane; q0 < q1; q0 += 64 * kSellBatch) {
+  for (int q0 = qa + lane; q0 < q1; q0 += 64 * kSellBatch) {
     T v[kSellBatch], xv[kSellBatch];
     int c[kSellBatch];
 #pragma unroll
@@ -311,8 +316,8 @@ __global__ void __launch_bounds__(kBlock) k_spmv_sell(SellView<T> A, Tiles t, co
       if (q0 + 64 * u < q1) acc += v[u] * xv[u];
   }
   double z = static_cast<double>(acc);
-  if (inv) z *= static_cast<double>(inv[row]);
-  y[row] = static_cast<T>(z * sc->scale[it]);
+  if constexpr (PC) z *= static_cast<double>(invr);
+  y[row] = static_cast<T>(z * scl);
 }
 
 template <class T, class S>
@@ -351,10 +356,14 @@ void launch_spmv(int L, int nt, const CsrView<T>& A, const SellView<T>& sell, co
     case 0: {
       if (!sell.ptr) throw std::logic_error("SELL-64 SpMV without a SELL copy");
       const int g = grid_for(A.n);
-      if (nt)
-        hipLaunchKernelGGL((k_spmv_sell<T, true>), dim3(g), dim3(kBlock), 0, s, sell, t, x, inv, y, st, it);
+      if (nt && inv)
+        hipLaunchKernelGGL((k_spmv_sell<T, true, true>), dim3(g), dim3(kBlock), 0, s, sell, t, x, inv, y, st, it);
+      else if (nt)
+        hipLaunchKernelGGL((k_spmv_sell<T, true, false>), dim3(g), dim3(kBlock), 0, s, sell, t, x, inv, y, st, it);
+      else if (inv)
+        hipLaunchKernelGGL((k_spmv_sell<T, false, true>), dim3(g), dim3(kBlock), 0, s, sell, t, x, inv, y, st, it);
       else
-        hipLaunchKernelGGL((k_spmv_sell<T, false>), dim3(g), dim3(kBlock), 0, s, sell, t, x, inv, y, st, it);
+        hipLaunchKernelGGL((k_spmv_sell<T, false, false>), dim3(g), dim3(kBlock), 0, s, sell, t, x, inv, y, st, it);
       break;
     }
     case 1: launch_spmv_l<T, 1>(nt != 0, A, t, x, inv, y, st, it, s); break;
