@@ -102,22 +102,44 @@ __global__ void __launch_bounds__(kBlock) k_ilu_upper(CsrView<T> A, const T* __r
   z[i] = s / lu[pd];
 }
 
+// s - sum_j val_j z[col_j] over the L entries of level-ELL row k, subtracted
+// in entry order; each batch of entries loads its column ids and values
+// before any of its gathers of z, so one row has a batch of loads in flight
+constexpr int kEllBatch = 8;
+template <class T>
+__device__ __forceinline__ T ell_row_sub(T s, int L, const int* __restrict__ ecol, const T* __restrict__ eval,
+                                         long base, int nrows, int k, const T* z) {
+  for (int j0 = 0; j0 < L; j0 += kEllBatch) {
+    T v[kEllBatch], zv[kEllBatch];
+    int c[kEllBatch];
+#pragma unroll
+    for (int u = 0; u < kEllBatch; ++u)
+      if (j0 + u < L) {
+        const size_t q = static_cast<size_t>(base) + static_cast<size_t>(j0 + u) * nrows + k;
+        v[u] = eval[q];
+        c[u] = ecol[q];
+      }
+#pragma unroll
+    for (int u = 0; u < kEllBatch; ++u)
+      if (j0 + u < L) zv[u] = z[c[u]];
+#pragma unroll
+    for (int u = 0; u < kEllBatch; ++u)
+      if (j0 + u < L) s -= v[u] * zv[u];
+  }
+  return s;
+}
+
 // the same sweeps on the level-ELL copies (one level per launch)
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_ilu_lower_ell(const int* __restrict__ rows, const int* __restrict__ len,
                                                           const int* __restrict__ ecol, const T* __restrict__ eval,
                                                           long base, int nrows, const T* r, T* z, const KState* st) {
-  if (st && st->reason) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nrows) return;
   const int i = rows[k];
-  T s = r[i];
   const int L = len[k];
-  for (int j = 0; j < L; ++j) {
-    const size_t q = static_cast<size_t>(base) + static_cast<size_t>(j) * nrows + k;
-    s -= eval[q] * z[ecol[q]];
-  }
-  z[i] = s;
+  if (st && st->reason) return;
+  z[i] = ell_row_sub(r[i], L, ecol, eval, base, nrows, k, z);
 }
 
 template <class T>
@@ -125,17 +147,13 @@ __global__ void __launch_bounds__(kBlock) k_ilu_upper_ell(const int* __restrict_
                                                           const int* __restrict__ ecol, const T* __restrict__ eval,
                                                           const T* __restrict__ dval, long base, int nrows, T* z,
                                                           const KState* st) {
-  if (st && st->reason) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nrows) return;
   const int i = rows[k];
-  T s = z[i];
   const int L = len[k];
-  for (int j = 0; j < L; ++j) {
-    const size_t q = static_cast<size_t>(base) + static_cast<size_t>(j) * nrows + k;
-    s -= eval[q] * z[ecol[q]];
-  }
-  z[i] = s / dval[k];
+  if (st && st->reason) return;
+  const T d = dval[k];
+  z[i] = ell_row_sub(z[i], L, ecol, eval, base, nrows, k, z) / d;
 }
 
 // out[q] = lu[src[q]] (0 where src[q] < 0)
