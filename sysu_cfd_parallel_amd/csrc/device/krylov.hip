@@ -102,16 +102,44 @@ __global__ void __launch_bounds__(kBlock) k_ilu_upper(CsrView<T> A, const T* __r
   z[i] = s / lu[pd];
 }
 
-// s - sum_j val_j z[col_j] over the L entries of level-ELL row k, subtracted
-// in entry order; each batch of entries loads its column ids and values
-// before any of its gathers of z, so one row has a batch of loads in flight
+// One level of the lower (z = L^-1 r) or upper (z = U^-1 z) sweep on the
+// level-ELL copy: row k of the level has L entries among the level's `width`
+// slots (padding: value 0, column = the row itself, so every slot's gather
+// is in bounds).  All width slots' values and column ids load with the row id
+// and before the state check; then the row's input and the gathers of z;
+// the entries are subtracted in entry order, the padding never.
 constexpr int kEllBatch = 8;
-template <class T>
-__device__ __forceinline__ T ell_row_sub(T s, int L, const int* __restrict__ ecol, const T* __restrict__ eval,
-                                         long base, int nrows, int k, const T* z) {
-  for (int j0 = 0; j0 < L; j0 += kEllBatch) {
-    T v[kEllBatch], zv[kEllBatch];
-    int c[kEllBatch];
+template <class T, bool Upper>
+__global__ void __launch_bounds__(kBlock) k_ilu_sweep_ell(const int* __restrict__ rows, const int* __restrict__ len,
+                                                          const int* __restrict__ ecol, const T* __restrict__ eval,
+                                                          const T* __restrict__ dval, long base, int nrows, int width,
+                                                          const T* r, T* z, const KState* st) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nrows) return;
+  const int i = rows[k];
+  const int L = len[k];
+  const int w0 = width < kEllBatch ? width : kEllBatch;
+  T v[kEllBatch], zv[kEllBatch];
+  int c[kEllBatch];
+#pragma unroll
+  for (int u = 0; u < kEllBatch; ++u)
+    if (u < w0) {
+      const size_t q = static_cast<size_t>(base) + static_cast<size_t>(u) * nrows + k;
+      v[u] = eval[q];
+      c[u] = ecol[q];
+    }
+  T d = T(1);
+  if constexpr (Upper) d = dval[k];
+  if (st && st->reason) return;
+  T s = Upper ? z[i] : r[i];
+#pragma unroll
+  for (int u = 0; u < kEllBatch; ++u)
+    if (u < w0) zv[u] = z[c[u]];
+#pragma unroll
+  for (int u = 0; u < kEllBatch; ++u)
+    if (u < L) s -= v[u] * zv[u];
+  // rows longer than one batch (polyhedral cells): the rest, batch by batch
+  for (int j0 = kEllBatch; j0 < L; j0 += kEllBatch) {
 #pragma unroll
     for (int u = 0; u < kEllBatch; ++u)
       if (j0 + u < L) {
@@ -126,34 +154,10 @@ __device__ __forceinline__ T ell_row_sub(T s, int L, const int* __restrict__ eco
     for (int u = 0; u < kEllBatch; ++u)
       if (j0 + u < L) s -= v[u] * zv[u];
   }
-  return s;
-}
-
-// the same sweeps on the level-ELL copies (one level per launch)
-template <class T>
-__global__ void __launch_bounds__(kBlock) k_ilu_lower_ell(const int* __restrict__ rows, const int* __restrict__ len,
-                                                          const int* __restrict__ ecol, const T* __restrict__ eval,
-                                                          long base, int nrows, const T* r, T* z, const KState* st) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nrows) return;
-  const int i = rows[k];
-  const int L = len[k];
-  if (st && st->reason) return;
-  z[i] = ell_row_sub(r[i], L, ecol, eval, base, nrows, k, z);
-}
-
-template <class T>
-__global__ void __launch_bounds__(kBlock) k_ilu_upper_ell(const int* __restrict__ rows, const int* __restrict__ len,
-                                                          const int* __restrict__ ecol, const T* __restrict__ eval,
-                                                          const T* __restrict__ dval, long base, int nrows, T* z,
-                                                          const KState* st) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nrows) return;
-  const int i = rows[k];
-  const int L = len[k];
-  if (st && st->reason) return;
-  const T d = dval[k];
-  z[i] = ell_row_sub(z[i], L, ecol, eval, base, nrows, k, z) / d;
+  if constexpr (Upper)
+    z[i] = s / d;
+  else
+    z[i] = s;
 }
 
 // out[q] = lu[src[q]] (0 where src[q] < 0)
@@ -225,15 +229,20 @@ void ilu_apply(const CsrView<T>& A, const IluLevels& lv, const T* lu, const T* r
     const T* lval = static_cast<const T*>(w->ilu_lval);
     const T* uval = static_cast<const T*>(w->ilu_uval);
     const T* dval = static_cast<const T*>(w->ilu_dval);
+    auto width = [](const IluEll& e, int l, int nr) { return static_cast<int>((e.off[l + 1] - e.off[l]) / nr); };
     for (int l = 0; l < lv.nl(); ++l) {
       const int nr = lv.lptr[l + 1] - lv.lptr[l];
-      hipLaunchKernelGGL((k_ilu_lower_ell<T>), dim3(grid_for(nr)), dim3(kBlock), 0, s, lv.lrows + lv.lptr[l],
-                         lv.lell.len + lv.lptr[l], lv.lell.col, lval, lv.lell.off[l], nr, r, z, st);
+      if (nr == 0) continue;
+      hipLaunchKernelGGL((k_ilu_sweep_ell<T, false>), dim3(grid_for(nr)), dim3(kBlock), 0, s, lv.lrows + lv.lptr[l],
+                         lv.lell.len + lv.lptr[l], lv.lell.col, lval, nullptr, lv.lell.off[l], nr,
+                         width(lv.lell, l, nr), r, z, st);
     }
     for (int l = 0; l < lv.nu(); ++l) {
       const int nr = lv.uptr[l + 1] - lv.uptr[l];
-      hipLaunchKernelGGL((k_ilu_upper_ell<T>), dim3(grid_for(nr)), dim3(kBlock), 0, s, lv.urows + lv.uptr[l],
-                         lv.uell.len + lv.uptr[l], lv.uell.col, uval, dval + lv.uptr[l], lv.uell.off[l], nr, z, st);
+      if (nr == 0) continue;
+      hipLaunchKernelGGL((k_ilu_sweep_ell<T, true>), dim3(grid_for(nr)), dim3(kBlock), 0, s, lv.urows + lv.uptr[l],
+                         lv.uell.len + lv.uptr[l], lv.uell.col, uval, dval + lv.uptr[l], lv.uell.off[l], nr,
+                         width(lv.uell, l, nr), z, z, st);
     }
     S355_LAUNCH_CHECK();
     return;
