@@ -1,0 +1,113 @@
+"""Size-independent properties at the full size of the channel workload
+(200 x 100 x 200 box, 15 % jitter, 10 % of the hexes split into 6 tets:
+5,995,990 cells, too large for the oracle): the mesh and CSR invariants, the
+pressure-correction matrix's zero row sums and symmetry to rounding
+(navier_pressure.cpp:154-175), the Jacobi-preconditioned stopping test checked
+against an explicit FP64 residual on the host, continuity after the face-flux
+correction (sum of corrected fluxes = the solve's residual), bitwise reruns,
+and the mixed-precision solve meeting the same FP64 test."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import sysu_cfd_parallel_amd as s
+from helpers import params
+
+pytestmark = pytest.mark.gpu
+
+CELLS = 5_995_990
+
+
+def channel_params(prec=s.FP64):
+    p = params(restart=50, rtol=1e-8, maxit=20000)
+    p.ksp[s.KSP_PRESSURE].rtol = 1e-6
+    p.residual_steady = 0.0
+    p.precision = prec
+    return p
+
+
+@pytest.fixture(scope="module")
+def full():
+    case = s.channel(200, 100, 200, lx=2.0, ly=1.0, lz=2.0, jitter=0.15, split=0.10, seed=0, re=1000.0)
+    d = s.Solver(case, channel_params())
+    print(f"[full] channel context: {d.sizes.n_cell} cells", flush=True)
+    d.iterate(1)  # one SIMPLE step from rest, so the fields are not trivial
+    yield case, d
+    d.close()
+
+
+def csr(d, name):
+    z = d.sizes
+    return sp.csr_matrix((d.get(name), d.ints("CSR_COL"), d.ints("CSR_ROWPTR")), shape=(z.n_cell, z.n_cell))
+
+
+def test_mesh_and_csr_invariants(full):
+    _, d = full
+    z = d.sizes
+    assert z.n_cell == CELLS
+    assert int(d.ints("CELL_NFACE").sum()) == 2 * z.n_interior + z.n_bnd
+    rp, col, dg = d.ints("CSR_ROWPTR"), d.ints("CSR_COL"), d.ints("CSR_DIAG")
+    assert rp[-1] == z.nnz == z.n_cell + 2 * z.n_interior
+    rows = np.repeat(np.arange(z.n_cell), np.diff(rp))
+    # columns ascending inside every row (PETSc AIJ), the diagonal among them
+    inner = np.diff(col) > 0
+    assert np.all(inner | (np.diff(rows) > 0))
+    assert np.array_equal(col[dg], np.arange(z.n_cell))
+
+
+def test_pressure_system_continuity_and_stopping_test(full):
+    _, d = full
+    d.call("build_velocity_matrix")
+    d.solve_velocity(1e-8, 20000)
+    d.call("cal_ru_face")
+    d.call("build_pressure_matrix")
+    A, b = csr(d, "AP"), d.get("BP_RHS")
+    amax = np.abs(A.data).max()
+    # zero row sums and symmetry to rounding, negative diagonal
+    assert np.abs(A @ np.ones(A.shape[0])).max() <= 1e-13 * amax
+    assert np.abs((A - A.T).data).max() <= 1e-12 * amax
+    assert np.all(A.diagonal() < 0)
+    info = d.solve_pressure(1e-6, 20000)
+    assert info["reason"] > 0
+    dp = d.get("DELTAP")
+    # left Jacobi preconditioning: ||D^-1 (b - A dp)|| <= rtol ||D^-1 b||
+    dinv = 1.0 / A.diagonal()
+    res = np.linalg.norm(dinv * (b - A @ dp))
+    assert res <= 1e-6 * np.linalg.norm(dinv * b) * 1.01, (res, info)
+    assert res == pytest.approx(info["rnorm"], rel=1e-2)
+    # continuity: the corrected face fluxes sum, per cell, to the residual
+    d.correct_fields()
+    d.call("bc_velocity")
+    d.call("correct_ru_face")
+    ru = d.get("RUFACE")
+    z = d.sizes
+    c1, c2 = d.ints("FACE_C1"), d.ints("FACE_C2")
+    div = np.bincount(c1, weights=ru, minlength=z.n_cell)
+    div -= np.bincount(c2[z.n_bnd:], weights=ru[z.n_bnd:], minlength=z.n_cell)
+    r = b - A @ dp
+    assert np.abs(div - r).max() <= 1e-9 * np.abs(b).max()
+
+
+def test_bitwise_rerun_and_mixed_solve(full):
+    case, d = full
+    d.call("build_velocity_matrix")
+    d.solve_velocity(1e-8, 20000)
+    d.call("cal_ru_face")
+    d.call("build_pressure_matrix")
+    A, b = csr(d, "AP"), d.get("BP_RHS")
+    first = []
+    for _ in range(2):
+        d.set("AP", A.data)
+        d.set("BP_RHS", b)
+        d.solve_pressure(1e-6, 20000)
+        first.append(d.get("DELTAP"))
+    assert np.array_equal(first[0], first[1])
+    with s.Solver(case, channel_params(s.MIXED)) as m:
+        m.set("AP", A.data)
+        m.set("BP_RHS", b)
+        info = m.solve_pressure(1e-6, 20000)
+        dpm = m.get("DELTAP")
+    assert info["reason"] > 0 and info["refinements"] > 0
+    dinv = 1.0 / A.diagonal()
+    res = np.linalg.norm(dinv * (b - A @ dpm))
+    assert res <= 1e-6 * np.linalg.norm(dinv * b) * 1.01, (res, info)
