@@ -349,6 +349,41 @@ def run_path(wl, case, prec: int, steps: int, warmup: int, label: str) -> dict:
     return out
 
 
+# the state one SIMPLE step starts from (matrices and gradients are rebuilt)
+STATE = ("R", "U", "V", "W", "P", "T", "TE", "ED", "VISLAM", "VISTUR", "APR", "DELTAP", "BRO", "BU", "BV", "BW",
+         "BPRE", "BTE", "BED", "BTEM", "SHEAR", "RUFACE")
+
+
+def one_step_agreement(dev, names) -> dict:
+    """The mixed path's own error, without the drift of two trajectories:
+    one SIMPLE step from the same state in mixed precision, in FP64, and in
+    FP64 with every solve converged 10^4 times further (the step's converged
+    result), on the same context; relative L2 differences of each field."""
+    import sysu_cfd_parallel_amd as s
+
+    state = {n: dev.get(n) for n in STATE}
+    p = dev.params
+    rtol = [p.ksp[k].rtol for k in range(3)]
+    out = {}
+    for key, prec, tight in (("mixed", s.MIXED, 1.0), ("fp64", s.FP64, 1.0), ("converged", s.FP64, 1e-4)):
+        for n, v in state.items():
+            dev.set(n, v)
+        p.precision = prec
+        for k in range(3):
+            p.ksp[k].rtol = rtol[k] * tight
+        dev.set_params(p)
+        dev.iterate(1)
+        out[key] = {n: dev.get(n) for n in names}
+    p.precision = s.MIXED
+    for k in range(3):
+        p.ksp[k].rtol = rtol[k]
+    dev.set_params(p)
+    res = {f"mixed_vs_fp64.{n}": l2_rel(out["mixed"][n], out["fp64"][n]) for n in names}
+    res.update({f"mixed_vs_converged.{n}": l2_rel(out["mixed"][n], out["converged"][n]) for n in names})
+    res.update({f"fp64_vs_converged.{n}": l2_rel(out["fp64"][n], out["converged"][n]) for n in names})
+    return res
+
+
 def l2_rel(a, b) -> float:
     import numpy as np
 
@@ -422,6 +457,12 @@ def main() -> int:
         agreement["residual_history_max_rel"] = float(
             (abs(a["history"][:, :4] - b["history"][:, :4]) / abs(b["history"][:, :4]).clip(1e-300)).max())
         log(f"mixed vs FP64 relative L2 field difference: {agreement}")
+
+    if agreement is not None:
+        t = time.perf_counter()
+        one = one_step_agreement(dev, list(runs["mixed"]["fields"]))
+        agreement["one_step_from_the_same_state"] = one
+        log(f"mixed vs FP64, one step from the same state: {one} ({time.perf_counter() - t:.1f}s)")
 
     spmv = spmv_roofline(dev, z)
     dom = dict(spmv["fp32" if main_name == "mixed" else "fp64"])
