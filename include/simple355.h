@@ -447,6 +447,16 @@ int s355_read_backup(s355_ctx* ctx, const char* path);
  * reference never advances its output counter (navier.cpp:750), so every
  * ASCII result goes to res0000.dat. */
 int s355_output_schedule(s355_ctx* ctx, int noutput, int binary, const char* dir);
+/* OutputMoniter (navierIO.cpp:405-468): per wall region the summed wall
+ * force (Bnd.shear) and per fluid region of type2 0 the summed W^2 vol, region
+ * id 0 left out, ascending region id; 6 doubles per record: rid, type1,
+ * fx, fy, fz, scalar. */
+int s355_monitor_records(s355_ctx* ctx, double* out, int max_records, int* n_records);
+/* monitorResult.mon: header = 1 starts the file with the header of
+ * readyToSolve (navier.cpp:552-585); header = 0 appends the line of the
+ * current records at time cur_time.  NSSolve writes it every step while an
+ * output schedule is set. */
+int s355_monitor_write(s355_ctx* ctx, const char* path, double cur_time, int header);
 
 #ifdef __cplusplus
 }

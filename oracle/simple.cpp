@@ -1004,6 +1004,7 @@ int Oracle::SimpleIterate(int n_steps, double* history, int* done) {
   for (int step = 1; step <= n_steps; ++step) {
     double resmax = 0.;
     if (par.steady) {
+      cur_time += par.dt;  // navier.cpp:59
       int rc = SimplePass(&resmax);
       if (rc != S355_OK) return rc;
     } else {

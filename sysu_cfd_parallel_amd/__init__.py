@@ -119,6 +119,8 @@ def library(path: Optional[Path] = None) -> C.CDLL:
         "save_transient_old_data": (I, [V]),
         "unsteady_sources": (I, [V, I]),
         "time_state": (I, [V, pD, pI]),
+        "monitor_records": (I, [V, pD, I, pI]),
+        "monitor_write": (I, [V, C.c_char_p, C.c_double, I]),
     }
     for name, (res, args) in more.items():
         fn = getattr(lib, f"s355_{name}")
@@ -409,6 +411,14 @@ class Solver:
         """Restart from a res.sav (after init_flow_field, which the
         constructor runs)."""
         self.call("read_backup", str(path).encode())
+
+    def monitor_records(self) -> list:
+        """OutputMoniter's records: wall force per wall region, W^2 vol per
+        fluid region (region id 0 left out), ascending region id."""
+        out, n = np.empty(6 * 256), C.c_int()
+        _check(self.lib, self.lib.s355_monitor_records(self.ctx, _dp(out), 256, C.byref(n)), "monitor_records")
+        r = out[: 6 * n.value].reshape(-1, 6)
+        return [{"rid": int(x[0]), "type1": int(x[1]), "force": x[2:5].copy(), "scalar": float(x[5])} for x in r]
 
     def output_schedule(self, noutput: int, directory, binary: bool = False) -> None:
         """NSSolve's output: every noutput steps res.sav and the Tecplot

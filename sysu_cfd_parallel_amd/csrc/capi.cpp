@@ -2,7 +2,10 @@
 // catches every exception and turns it into a return code plus a message
 // for s355_last_error(); nothing blocks or aborts the process (the
 // reference's ErrorHandler waits on getchar() then aborts, BasicType.h:23-55).
+#include <algorithm>
+#include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -199,8 +202,8 @@ void s355_default_params(s355_params* p) {
   p->prl = 0.72;
   p->prte = 0.9;
   p->limiter = 0;  // navier.cpp:509
-  p->time_scheme = 1;  // navier.cpp:513-523
-  p->dt = 0.1;
+  p->time_scheme = 1;  // navier.cpp:512
+  p->dt = 1.0;  // navier.cpp:511
   p->total_time = 0.;
   p->max_outer_step = 50;  // navier.cpp:476
 }
@@ -372,6 +375,41 @@ s355host::OutFormat out_format(int format) {
   return static_cast<s355host::OutFormat>(format);
 }
 
+// OutputMoniter's records (navierIO.cpp:405-468), summed on the host in the
+// reference's order: boundary records, then cells, ascending
+std::vector<s355host::MonitorRecord> monitor_records(s355::Context& d) {
+  std::vector<s355_region> regs(d.mesh.regions.begin(), d.mesh.regions.end());
+  std::sort(regs.begin(), regs.end(), [](const s355_region& a, const s355_region& b) { return a.rid < b.rid; });
+  std::vector<s355host::MonitorRecord> rec;
+  std::map<int, size_t> slot;
+  std::map<int, const s355_region*> by_rid;
+  for (const s355_region& r : regs) {
+    by_rid[r.rid] = &r;
+    if ((r.type1 != 1 && r.type1 != 5) || r.rid == 0) continue;
+    slot[r.rid] = rec.size();
+    s355host::MonitorRecord m;
+    m.rid = r.rid;
+    m.type1 = r.type1;
+    rec.push_back(m);
+  }
+  const size_t nb = d.dm.Nb, nc = d.dm.Nc;
+  std::vector<double> shear(3 * nb), w(nc);
+  if (nb) S355_HIP(hipMemcpyAsync(shear.data(), d.df.shear, 3 * nb * sizeof(double), hipMemcpyDeviceToHost, d.stream));
+  S355_HIP(hipMemcpyAsync(w.data(), d.df.W, nc * sizeof(double), hipMemcpyDeviceToHost, d.stream));
+  d.sync();
+  for (size_t i = 0; i < nb; ++i) {
+    auto it = slot.find(d.mesh.b_rid[i]);
+    if (it == slot.end() || rec[it->second].type1 != 1) continue;
+    for (int k = 0; k < 3; ++k) rec[it->second].force[k] += shear[3 * i + k];
+  }
+  for (size_t i = 0; i < nc; ++i) {
+    auto it = slot.find(d.mesh.c_rid[i]);
+    if (it == slot.end() || rec[it->second].type1 != 5 || by_rid[d.mesh.c_rid[i]]->type2 != 0) continue;
+    rec[it->second].scalar += w[i] * w[i] * d.mesh.c_vol[i];
+  }
+  return rec;
+}
+
 s355host::FieldExport& exporter(s355_ctx* c) {
   s355::Context& d = dev(c);
   if (!c->out) c->out = std::make_unique<s355host::FieldExport>(d);
@@ -461,6 +499,38 @@ int s355_read_backup(s355_ctx* ctx, const char* path) {
       S355_HIP(hipMemcpyAsync(d.field(ids[k]).p, h.data() + k * n, n * sizeof(double), hipMemcpyHostToDevice,
                               d.stream));
     d.sync();
+    return S355_OK;
+  });
+}
+
+int s355_monitor_records(s355_ctx* ctx, double* out, int max_records, int* n_records) {
+  return guarded([&] {
+    if (!out || !n_records) throw std::invalid_argument("null output");
+    const auto rec = monitor_records(dev(ctx));
+    if (static_cast<int>(rec.size()) > max_records) throw std::invalid_argument("monitor: more regions than max_records");
+    for (size_t k = 0; k < rec.size(); ++k) {
+      double* o = out + 6 * k;
+      o[0] = rec[k].rid;
+      o[1] = rec[k].type1;
+      o[2] = rec[k].force[0];
+      o[3] = rec[k].force[1];
+      o[4] = rec[k].force[2];
+      o[5] = rec[k].scalar;
+    }
+    *n_records = static_cast<int>(rec.size());
+    return S355_OK;
+  });
+}
+
+int s355_monitor_write(s355_ctx* ctx, const char* path, double cur_time, int header) {
+  return guarded([&] {
+    if (!path) throw std::invalid_argument("null path");
+    const auto rec = monitor_records(dev(ctx));
+    const std::string text = header ? s355host::monitor_header(rec) : s355host::monitor_line(cur_time, rec);
+    FILE* f = std::fopen(path, header ? "wb" : "ab");
+    if (!f) throw s355host::IoError(std::string("cannot open ") + path);
+    const bool ok = std::fwrite(text.data(), 1, text.size(), f) == text.size();
+    if (std::fclose(f) != 0 || !ok) throw s355host::IoError(std::string("error writing ") + path);
     return S355_OK;
   });
 }

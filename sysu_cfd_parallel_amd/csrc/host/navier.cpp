@@ -164,6 +164,12 @@ void NavierStokesSolver::SetOutput(int noutput, bool binary, const std::string& 
     OutputGridBinary();
   else
     OutputGrid();
+  // the monitor file's header (readyToSolve, navier.cpp:552-585)
+  check(s355_monitor_write(ctx_, (outdir_ + "/monitorResult.mon").c_str(), cur_time, 1), "monitor file");
+}
+
+void NavierStokesSolver::OutputMoniter() {
+  check(s355_monitor_write(ctx_, (outdir_ + "/monitorResult.mon").c_str(), cur_time, 0), "OutputMoniter");
 }
 
 void NavierStokesSolver::OutputGrid() {
@@ -243,6 +249,7 @@ int NavierStokesSolver::NSSolve(int max_step, double* history, int* steps_done) 
   for (int step = 1; step <= max_step; ++step) {
     double resmax = 0.0;
     if (par_.steady) {
+      cur_time += par_.dt;  // advanced in steady runs too (navier.cpp:59)
       resmax = simple_pass();
     } else {
       if (time_step_ >= max_time_steps) break;
@@ -262,6 +269,7 @@ int NavierStokesSolver::NSSolve(int max_step, double* history, int* steps_done) 
     }
     *steps_done = step;
     output_step(par_.steady ? step : time_step_);
+    if (noutput_ > 0) OutputMoniter();  // every step (navier.cpp:125)
     if (par_.steady ? resmax < par_.residual_steady : cur_time >= par_.total_time) break;
   }
   if (noutput_ > 0) {

@@ -77,6 +77,7 @@ class NavierStokesSolver {
   void OutputGrid();
   void OutputGridBinary();
   void WriteBackupFile(bool async);
+  void OutputMoniter();
   void ReadBackupFile(const std::string& path);
 
   double Residual[10] = {0};

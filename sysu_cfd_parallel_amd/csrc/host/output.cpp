@@ -242,4 +242,28 @@ void read_backup(const std::string& path, int n_cell, bool turbulent, const Back
   }
 }
 
+std::string monitor_header(const std::vector<MonitorRecord>& regions) {
+  char buf[256];
+  std::snprintf(buf, sizeof buf, "%20s || ", "time");
+  std::string out = buf;
+  for (const MonitorRecord& r : regions) {
+    const bool wall = r.type1 == 1;
+    std::snprintf(buf, sizeof buf, "%20s: %20d: %20s | %20s || ", wall ? "wall" : "fluid", r.rid,
+                  wall ? "drag/lift force" : "Null", wall ? "Null" : "w energy");
+    out += buf;
+  }
+  return out + "\n";
+}
+
+std::string monitor_line(double cur_time, const std::vector<MonitorRecord>& records) {
+  char buf[256];
+  std::snprintf(buf, sizeof buf, "%20e || ", cur_time);
+  std::string out = buf;
+  for (const MonitorRecord& r : records) {
+    std::snprintf(buf, sizeof buf, "%20e, %20e, %20e | %20e || ", r.force[0], r.force[1], r.force[2], r.scalar);
+    out += buf;
+  }
+  return out + "\n";
+}
+
 }  // namespace s355host

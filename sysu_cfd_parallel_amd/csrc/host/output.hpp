@@ -3,9 +3,9 @@
 // res.sav backup that restarts a run.  Host code only: the device side hands
 // over host copies of the cell fields.
 #pragma once
-#include <string>
-
 #include <stdexcept>
+#include <string>
+#include <vector>
 
 namespace s355host {
 
@@ -44,5 +44,19 @@ struct BackupArrays {
   double *vislam = nullptr, *vistur = nullptr, *te = nullptr, *ed = nullptr;
 };
 void read_backup(const std::string& path, int n_cell, bool turbulent, const BackupArrays& out);
+
+// The monitor file (monitorResult.mon).  One record per wall region (the
+// wall force: the summed Bnd.shear) and per fluid region (the summed W^2 vol
+// of a type2 = 0 region), region id 0 left out, in ascending region id
+// (NavierStokesSolver::readyToSolve navier.cpp:552-585 writes the header,
+// OutputMoniter navierIO.cpp:405-468 a line per step).
+struct MonitorRecord {
+  int rid = 0;
+  int type1 = 0;  // 1 wall, 5 fluid
+  double force[3] = {0, 0, 0};
+  double scalar = 0;
+};
+std::string monitor_header(const std::vector<MonitorRecord>& regions);
+std::string monitor_line(double cur_time, const std::vector<MonitorRecord>& records);
 
 }  // namespace s355host

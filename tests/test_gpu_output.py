@@ -118,3 +118,37 @@ def test_file_errors_leave_the_context_usable(tmp_path):
         assert err.value.code == s._abi.S355_E_IO
         d.iterate(1)
         d.wait_output()  # the error was reported once
+
+
+def test_monitor_records_and_file(tmp_path):
+    # OutputMoniter: wall force = summed wall shear, fluid record = sum W^2 vol
+    case = s.cavity(6, re=100.0, jitter=0.1, seed=4)
+    p = params()
+    p.dt = 0.5
+    with s.Solver(case, p) as d:
+        d.output_schedule(5, tmp_path)
+        d.iterate(3)
+        rec = d.monitor_records()
+        shear = d.get("SHEAR").reshape(-1, 3)
+        brid = d.ints("BND_RID")
+        w, vol, crid = d.get("W"), d.dbls("CELL_VOL"), d.ints("CELL_RID")
+        t, _ = d.time_state()
+    assert t == 1.5  # steady steps advance cur_time by dt too
+    types = {r.rid: (r.type1, r.type2) for r in case.regions}
+    want = [r for r in sorted(types) if types[r][0] in (1, 5) and r != 0]
+    assert [r["rid"] for r in rec] == want
+    for r in rec:
+        if r["type1"] == 1:
+            assert np.allclose(r["force"], shear[brid == r["rid"]].sum(0), rtol=1e-12, atol=1e-15)
+            assert r["scalar"] == 0.0
+        else:
+            assert np.allclose(r["scalar"], np.sum(w[crid == r["rid"]] ** 2 * vol[crid == r["rid"]]), rtol=1e-12)
+            assert np.all(r["force"] == 0.0)
+    lines = (tmp_path / "monitorResult.mon").read_text().split("\n")
+    assert lines[0].startswith("%20s || " % "time")
+    assert "drag/lift force" in lines[0] and "w energy" in lines[0]
+    assert len(lines) == 1 + 3 + 1  # header, one line per step, the final newline
+    assert lines[3].startswith("%20e || " % 1.5)
+    vals = [float(v) for v in lines[3].replace("||", ",").replace("|", ",").split(",")[1:] if v.strip()]
+    flat = [x for r in rec for x in (*r["force"], r["scalar"])]
+    assert np.allclose(vals, flat, rtol=1e-6, atol=0)
