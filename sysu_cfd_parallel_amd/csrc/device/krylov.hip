@@ -478,10 +478,14 @@ struct GsShape {
 
 // ---- partial[k][b] = sum over tile b of w_i V_k,i, k = 0..it (k-major, so
 // that the per-k reduction over the tiles reads contiguously; the row stride
-// is the tile count, pstride only bounds the capacity)
+// is the tile count, pstride only bounds the capacity).  The k run in chunks
+// of 8; block (b, g) takes chunks g * cpb .. (g + 1) * cpb - 1 of tile b, so
+// a small system spreads its chunks over more blocks (each chunk's sums are
+// the same whichever block computes them).
 template <class T, int R>
 __global__ void __launch_bounds__(kBlock) k_mdot(const T* __restrict__ V, size_t ld, Tiles t, int it,
-                                                 double* __restrict__ partial, int pstride, const KState* st) {
+                                                 double* __restrict__ partial, int pstride, const KState* st,
+                                                 int cpb) {
   using S = GsShape<T, R>;
   int a, e, c;
   t.range(blockIdx.x, a, e, c);
@@ -494,7 +498,8 @@ __global__ void __launch_bounds__(kBlock) k_mdot(const T* __restrict__ V, size_t
 #pragma unroll
   for (int q = 0; q < S::NC; ++q) load_chunk<T, S::VW>(w, S::row0(a, q), end, vec, wr + q * S::VW);
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  for (int k0 = 0; k0 <= it; k0 += 8) {
+  const int kend = 8 * cpb * (static_cast<int>(blockIdx.y) + 1);
+  for (int k0 = 8 * cpb * static_cast<int>(blockIdx.y); k0 <= it && k0 < kend; k0 += 8) {
     double acc[8];
 #pragma unroll
     for (int kk = 0; kk < 8; ++kk) {
@@ -1045,6 +1050,9 @@ int pick_r(int n, size_t elem) {
   return 1;
 }
 
+// blocks a multi-dot launch aims at before splitting its dot products
+constexpr int kMdotBlocks = 1024;
+
 int log2_r(int R) { return R == 16 ? 4 : R == 8 ? 3 : R == 4 ? 2 : R == 2 ? 1 : 0; }
 
 template <class T>
@@ -1052,7 +1060,13 @@ struct Dispatch {
   template <int R>
   static void mdot_r(const T* V, size_t ld, const Tiles& t, int it, double* partial, int pstride, const KState* st,
                      hipStream_t s) {
-    hipLaunchKernelGGL((k_mdot<T, R>), dim3(t.ntile), dim3(kBlock), 0, s, V, ld, t, it, partial, pstride, st);
+    // below kMdotBlocks tiles, the chunks of 8 dot products go to separate blocks
+    const int chunks = (it + 8) / 8;
+    const int groups = std::max(1, std::min(chunks, (kMdotBlocks + t.ntile - 1) / t.ntile));
+    const int cpb = (chunks + groups - 1) / groups;
+    const int gy = (chunks + cpb - 1) / cpb;
+    hipLaunchKernelGGL((k_mdot<T, R>), dim3(t.ntile, gy), dim3(kBlock), 0, s, V, ld, t, it, partial, pstride, st,
+                       cpb);
   }
   template <int R>
   static void maxpy_r(T* V, size_t ld, const Tiles& t, int it, double* partial, const KState* st, hipStream_t s) {
