@@ -273,14 +273,37 @@ __device__ __forceinline__ V ld_stream(const V* p) {
     return *p;
 }
 
+// sum over p = p0, p0 + step, ... < p1 of val[p] * x[col[p]] (X: the
+// accumulator type), in that order; each batch of 8 entries loads its values
+// and column ids before any of its gathers of x
+template <class X, class V, bool NT>
+__device__ __forceinline__ X csr_dot(const V* __restrict__ val, const int* __restrict__ col, int p0, int p1, int step,
+                                     const X* __restrict__ x, X s) {
+  constexpr int B = 8;
+  for (int pb = p0; pb < p1; pb += B * step) {
+    V v[B];
+    X xv[B];
+    int c[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      if (pb + u * step < p1) {
+        v[u] = ld_stream<V, NT>(val + pb + u * step);
+        c[u] = ld_stream<int, NT>(col + pb + u * step);
+      }
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      if (pb + u * step < p1) xv[u] = x[c[u]];
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      if (pb + u * step < p1) s += static_cast<X>(v[u]) * xv[u];
+  }
+  return s;
+}
+
 template <class T, int L, bool NT>
 __device__ __forceinline__ T row_dot(const CsrView<T>& A, const T* __restrict__ x, int row, int lane) {
   T s = T(0);
-  if (row < A.n) {
-    const int p1 = A.rowptr[row + 1];
-    for (int p = A.rowptr[row] + lane; p < p1; p += L)
-      s += ld_stream<T, NT>(A.val + p) * x[ld_stream<int, NT>(A.col + p)];
-  }
+  if (row < A.n) s = csr_dot<T, T, NT>(A.val, A.col, A.rowptr[row] + lane, A.rowptr[row + 1], L, x, s);
 #pragma unroll
   for (int off = L / 2; off > 0; off >>= 1) s += __shfl_down(s, off, L);
   return s;
@@ -743,9 +766,7 @@ __global__ void __launch_bounds__(kBlock) k_residual_pc(CsrView<X> A, Tiles t, c
   for (int i = a + threadIdx.x; i < e; i += kBlock) {
     double r = static_cast<double>(b[i]);
     if (!zero_guess) {
-      X s = X(0);
-      const int p1 = A.rowptr[i + 1];
-      for (int p = A.rowptr[i]; p < p1; ++p) s += A.val[p] * x[A.col[p]];
+      const X s = csr_dot<X, X, false>(A.val, A.col, A.rowptr[i], A.rowptr[i + 1], 1, x, X(0));
       r -= static_cast<double>(s);
     }
     if (inv) r *= static_cast<double>(inv[i]);
@@ -1033,8 +1054,7 @@ template <class T>
 __global__ void __launch_bounds__(kBlock) k_residual64(CsrView<T> A, const double* b, const double* x, double* r) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
-  double s = 0.0;
-  for (int p = A.rowptr[i]; p < A.rowptr[i + 1]; ++p) s += static_cast<double>(A.val[p]) * x[A.col[p]];
+  const double s = csr_dot<double, T, false>(A.val, A.col, A.rowptr[i], A.rowptr[i + 1], 1, x, 0.0);
   r[i] = b[i] - s;
 }
 
