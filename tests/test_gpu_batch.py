@@ -74,3 +74,24 @@ def test_batch_rejects_cg():
     p.ksp[s.KSP_PRESSURE].type = s.KSP_CG
     with pytest.raises(s.S355Error):
         s.BatchSolver(sweep_cases()[:2], p)
+
+
+@pytest.mark.parametrize("scheme,limiter", [(1, 1), (2, 3)])
+def test_batch_unsteady_with_limiter_reproduces_single_runs(scheme, limiter):
+    # one outer pass per time step (MaxOuterStep 2), so that the batch's
+    # shared outer-loop exit cannot differ from a case's own
+    cases = sweep_cases()
+    p = params(restart=30, rtol=1e-8, maxit=2000)
+    p.ksp[s.KSP_PRESSURE].rtol = 1e-6
+    p.steady, p.time_scheme, p.dt, p.total_time, p.max_outer_step = 0, scheme, 0.05, 10.0, 2
+    p.limiter = limiter
+    steps = 3
+    alone = run_alone(cases, p, steps)
+    with s.BatchSolver(cases, p) as b:
+        _, done = b.iterate(steps)
+        assert done == steps
+        assert b.time_state() == (0.05 + 0.05 + 0.05, 3)
+        for k, (fields, _, dk) in enumerate(alone):
+            assert dk == steps
+            for n in ("U", "V", "W", "P"):
+                assert np.array_equal(b.case_field(k, n), fields[n]), (k, n)
