@@ -403,6 +403,7 @@ enum s355_output_format {
   S355_OUT_TEC_GEO,     /* OutputGridBinary (navierIO.cpp:29-56): tec/geo */
   S355_OUT_GRID,        /* OutputGrid (navierIO.cpp:58-122): tec/grid.dat */
   S355_OUT_BACKUP,      /* WriteBackupFile (navierIO.cpp:370-386): res.sav */
+  S355_OUT_VTK,         /* Output2VTK (navierIO.cpp:287-365): res.vtk */
   S355_OUT_COUNT
 };
 

@@ -111,7 +111,7 @@ class Sizes(C.Structure):
 
 
 # result file formats (enum s355_output_format)
-OUTPUT_FORMATS = {"tecplot": 0, "tec_data": 1, "tec_geo": 2, "grid": 3, "backup": 4}
+OUTPUT_FORMATS = {"tecplot": 0, "tec_data": 1, "tec_geo": 2, "grid": 3, "backup": 4, "vtk": 5}
 # cell arrays of a result, in s355_result order
 RESULT_FIELDS = ("R", "U", "V", "W", "P", "T", "VISLAM", "VISTUR", "TE", "ED")
 

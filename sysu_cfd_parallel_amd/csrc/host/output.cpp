@@ -13,7 +13,10 @@
 //    fourth per-vertex value naming the neighbour partition (0 on one rank);
 //  * res.sav (WriteBackupFile / ReadBackupFile, navierIO.cpp:370-403): per
 //    cell "R U V W P T R VisLam" and, with k-epsilon, "VisTur TE ED" lines in
-//    the stream's default format (6 significant digits, as %g).
+//    the stream's default format (6 significant digits, as %g);
+//  * res.vtk (Output2VTK, navierIO.cpp:287-365): legacy ASCII unstructured
+//    grid of 8-node cells (type 12, 0-based ids), the velocity vector and the
+//    scalars P ro T mach mu (te ed) Elemtype as floats, stream format.
 // One process writes the whole file (rank 0 of 1 partition).  Values of
 // 1e13 and above overflow the 19-character field: the reference's buffer
 // arithmetic then loses characters, here they are written in full.
@@ -103,6 +106,16 @@ void check_view(const ResultView& r, bool fields) {
     throw IoError("result: k-epsilon output needs VisTur, TE and ED");
 }
 
+// "8 v0 ... v7 \n" per cell with the 0-based ids (Output2VTK's CELLS block)
+void put_connectivity_raw(File& f, const ResultView& r) {
+  put_parallel(f, r.n_cell, 1, [&](long i, std::string& s, char* buf) {
+    const int* v = r.cell_vert + 8 * i;
+    const int k = std::snprintf(buf, kBuf, "8 %d %d %d %d %d %d %d %d \n", v[0], v[1], v[2], v[3], v[4], v[5], v[6],
+                                v[7]);
+    s.append(buf, k);
+  });
+}
+
 void put_connectivity(File& f, const ResultView& r, const char* fmt) {
   put_parallel(f, r.n_cell, 1, [&](long i, std::string& s, char* buf) {
     const int* v = r.cell_vert + 8 * i;
@@ -181,6 +194,59 @@ void grid_ascii(const ResultView& r, const std::string& path) {
   f.close();
 }
 
+// Output2VTK; "%g" is the stream's default format, and the scalars are
+// written as floats (outputVTKScalar, navierIO.cpp:18-25)
+void vtk(const ResultView& r, const std::string& path) {
+  File f(path, "wb");
+  char head[256];
+  int k = std::snprintf(head, sizeof head,
+                        "# vtk DataFile Version 2.0\nNavier-Stokes solver result\nASCII\nDATASET UNSTRUCTURED_GRID\n"
+                        "POINTS %d FLOAT \n",
+                        r.n_vert);
+  f.put(head, k);
+  put_parallel(f, r.n_vert, 1, [&](long i, std::string& s, char* buf) {
+    const int m = std::snprintf(buf, kBuf, "%g %g %g \n", r.vert[3 * i], r.vert[3 * i + 1], r.vert[3 * i + 2]);
+    s.append(buf, m);
+  });
+  k = std::snprintf(head, sizeof head, "\nCELLS %d %ld\n", r.n_cell, 9L * r.n_cell);
+  f.put(head, k);
+  put_connectivity_raw(f, r);
+  k = std::snprintf(head, sizeof head, "CELL_TYPES %d\n", r.n_cell);
+  f.put(head, k);
+  put_parallel(f, r.n_cell, 8, [&](long i, std::string& s, char*) {
+    s.append("12 ");
+    if ((i + 1) % 8 == 0) s.push_back('\n');
+  });
+  k = std::snprintf(head, sizeof head, "\nCELL_DATA %d\nVECTORS velocity FLOAT\n", r.n_cell);
+  f.put(head, k);
+  put_parallel(f, r.n_cell, 1, [&](long i, std::string& s, char* buf) {
+    const int m = std::snprintf(buf, kBuf, "%g %g %g\n", r.U[i], r.V[i], r.W[i]);
+    s.append(buf, m);
+  });
+  f.put("\n", 1);
+  auto scalar = [&](const char* name, auto value) {
+    char h[128];
+    const int m = std::snprintf(h, sizeof h, "SCALARS %s FLOAT\nLOOKUP_TABLE default\n", name);
+    f.put(h, m);
+    put_parallel(f, r.n_cell, 1, [&](long i, std::string& s, char* buf) {
+      const int q = std::snprintf(buf, kBuf, "%g\n", static_cast<double>(static_cast<float>(value(i))));
+      s.append(buf, q);
+    });
+    f.put("\n", 1);
+  };
+  scalar("P", [&](long i) { return r.P[i]; });
+  scalar("ro", [&](long i) { return r.R[i]; });
+  scalar("T", [&](long i) { return r.T[i]; });
+  scalar("mach", [&](long i) { return std::sqrt(r.U[i] * r.U[i] + r.V[i] * r.V[i] + r.W[i] * r.W[i]); });
+  scalar("mu", [&](long i) { return r.vislam[i] + (r.vistur ? r.vistur[i] : 0.0); });
+  if (r.turbulent) {
+    scalar("te", [&](long i) { return r.te[i]; });
+    scalar("ed", [&](long i) { return r.ed[i]; });
+  }
+  scalar("Elemtype", [](long) { return 2.0; });
+  f.close();
+}
+
 void backup(const ResultView& r, const std::string& path) {
   File f(path, "wb");
   put_parallel(f, r.n_cell, 1, [&](long i, std::string& s, char* buf) {
@@ -206,6 +272,7 @@ void write_result(const ResultView& r, OutFormat fmt, const std::string& path) {
     case OutFormat::GridBinary: grid_binary(r, path); break;
     case OutFormat::GridAscii: grid_ascii(r, path); break;
     case OutFormat::Backup: backup(r, path); break;
+    case OutFormat::Vtk: vtk(r, path); break;
     default: throw IoError("unknown output format");
   }
 }

@@ -32,6 +32,7 @@ enum class OutFormat {
   GridBinary = 2,    // OutputGridBinary: tec/geo
   GridAscii = 3,     // OutputGrid: tec/grid.dat
   Backup = 4,        // WriteBackupFile: res.sav
+  Vtk = 5,           // Output2VTK: res.vtk (legacy ASCII unstructured grid)
 };
 
 // Writes one file; throws std::runtime_error on a bad view or an I/O error.

@@ -234,3 +234,33 @@ def test_tecplot_reader_of_the_tests(tmp_path):
     assert t["p"][0] == 1.25 and t["Ma"][0] == 5.0 and t["T"][0] == 300.0
     assert np.array_equal(t["xyz"].ravel(), m.dbls("VERT"))
     assert list(t["conn"][0]) == [1, 2, 3, 3, 4, 4, 4, 4]
+
+
+def test_vtk_known_answer(tmp_path):
+    # Output2VTK (navierIO.cpp:287-365): 0-based ids, stream-formatted values,
+    # scalars as floats, a blank line after every block
+    m = s.HostMesh(tet_cases()[0])
+    f = tet_fields()
+    f["P"] = [1.0 / 3.0]
+    m.write_result("vtk", tmp_path / "res.vtk", f)
+    assert (tmp_path / "res.vtk").read_text() == (
+        "# vtk DataFile Version 2.0\nNavier-Stokes solver result\nASCII\nDATASET UNSTRUCTURED_GRID\n"
+        "POINTS 4 FLOAT \n0 0 0 \n1 0 0 \n0 1 0 \n0 0 1 \n\n"
+        "CELLS 1 9\n8 0 1 2 2 3 3 3 3 \n"
+        "CELL_TYPES 1\n12 \n"
+        "CELL_DATA 1\nVECTORS velocity FLOAT\n3 4 0\n\n"
+        "SCALARS P FLOAT\nLOOKUP_TABLE default\n0.333333\n\n"
+        "SCALARS ro FLOAT\nLOOKUP_TABLE default\n1\n\n"
+        "SCALARS T FLOAT\nLOOKUP_TABLE default\n300\n\n"
+        "SCALARS mach FLOAT\nLOOKUP_TABLE default\n5\n\n"
+        "SCALARS mu FLOAT\nLOOKUP_TABLE default\n0.01\n\n"
+        "SCALARS Elemtype FLOAT\nLOOKUP_TABLE default\n2\n\n")
+
+
+def test_vtk_cell_types_eight_to_a_line(tmp_path):
+    m = s.HostMesh(s.cavity(2))  # 8 cells: one full line, then the block's blank line
+    n = m.sizes.n_cell
+    f = {k: np.zeros(n) for k in ("P", "U", "V", "W", "R", "T", "VISLAM")}
+    m.write_result("vtk", tmp_path / "res.vtk", f)
+    text = (tmp_path / "res.vtk").read_text()
+    assert "CELL_TYPES 8\n" + "12 " * 8 + "\n\nCELL_DATA 8\n" in text
