@@ -507,12 +507,20 @@ void Context::ensure_ilu() {
   ilu_levels(dm.Nc, mesh.rowptr.data(), mesh.col.data(), mesh.diag.data(), lrows, levels_.lptr, urows, levels_.uptr);
   levels_.lrows = upload(allocs_, lrows, stream);
   levels_.urows = upload(allocs_, urows, stream);
+  // positions of each row in the two level orders (the sweeps' z spaces)
+  std::vector<int> lpos(dm.Nc), upos(dm.Nc), u2l(dm.Nc);
+  for (int p = 0; p < dm.Nc; ++p) {
+    lpos[lrows[p]] = p;
+    upos[urows[p]] = p;
+  }
+  for (int p = 0; p < dm.Nc; ++p) u2l[p] = lpos[urows[p]];
+  levels_.u2l = upload(allocs_, u2l, stream);
   // level-ELL copies of the factors for the sweeps
   for (int up = 0; up < 2; ++up) {
     IluEll& e = up ? levels_.uell : levels_.lell;
     std::vector<int> ecol, esrc, len, dsrc;
     ilu_ell(dm.Nc, mesh.rowptr.data(), mesh.col.data(), mesh.diag.data(), up ? urows : lrows,
-            up ? levels_.uptr : levels_.lptr, up != 0, e.off, ecol, esrc, len, dsrc);
+            up ? levels_.uptr : levels_.lptr, up ? upos : lpos, up != 0, e.off, ecol, esrc, len, dsrc);
     e.slots = esrc.size();
     e.col = upload(allocs_, ecol, stream);
     e.src = upload(allocs_, esrc, stream);
@@ -524,6 +532,8 @@ void Context::ensure_ilu() {
   kwork_.ilu_lval = alloc<double>(levels_.lell.slots);
   kwork_.ilu_uval = alloc<double>(levels_.uell.slots);
   kwork_.ilu_dval = alloc<double>(dm.Nc);
+  kwork_.ilu_zl = alloc<double>(dm.Nc);
+  kwork_.ilu_zu = alloc<double>(dm.Nc);
   kwork_.ilu = &levels_;
   sync();  // host staging of the level lists
   have_levels_ = true;
@@ -575,6 +585,8 @@ int Context::run_solve(int which, const double* b, double* x, const s355_ksp_opt
       kwork32_.ilu_lval = alloc<float>(levels_.lell.slots);
       kwork32_.ilu_uval = alloc<float>(levels_.uell.slots);
       kwork32_.ilu_dval = alloc<float>(dm.Nc);
+      kwork32_.ilu_zl = alloc<float>(dm.Nc);
+      kwork32_.ilu_zu = alloc<float>(dm.Nc);
     }
     convert_to_float(val, a32_, dm.nnz, stream);
     CsrView<float> A{dm.Nc, dm.rowptr, dm.col, dm.diag, a32_, dm.nnz};

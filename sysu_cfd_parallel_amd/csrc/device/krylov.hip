@@ -75,48 +75,28 @@ __global__ void __launch_bounds__(kBlock) k_ilu_factor(CsrView<T> A, T* __restri
   }
 }
 
-// z = L^-1 r (unit lower triangle) on the rows of one level
-template <class T>
-__global__ void __launch_bounds__(kBlock) k_ilu_lower(CsrView<T> A, const T* __restrict__ lu, const T* r, T* z,
-                                                      const int* __restrict__ rows, int nrows, const KState* st) {
-  if (st && st->reason) return;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nrows) return;
-  const int i = rows[t];
-  T s = r[i];
-  for (int p = A.rowptr[i]; p < A.diag[i]; ++p) s -= lu[p] * z[A.col[p]];
-  z[i] = s;
-}
-
-// z = U^-1 z on the rows of one level (in place)
-template <class T>
-__global__ void __launch_bounds__(kBlock) k_ilu_upper(CsrView<T> A, const T* __restrict__ lu, T* z,
-                                                      const int* __restrict__ rows, int nrows, const KState* st) {
-  if (st && st->reason) return;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nrows) return;
-  const int i = rows[t];
-  T s = z[i];
-  const int pd = A.diag[i];
-  for (int p = pd + 1; p < A.rowptr[i + 1]; ++p) s -= lu[p] * z[A.col[p]];
-  z[i] = s / lu[pd];
-}
-
-// One level of the lower (z = L^-1 r) or upper (z = U^-1 z) sweep on the
-// level-ELL copy: row k of the level has L entries among the level's `width`
-// slots (padding: value 0, column = the row itself, so every slot's gather
-// is in bounds).  All width slots' values and column ids load with the row id
-// and before the state check; then the row's input and the gathers of z;
-// the entries are subtracted in entry order, the padding never.
+// One level of the lower (zl = L^-1 r) or upper (zu = U^-1 zl) sweep on the
+// level-ELL copy, in level space: the level's k-th row sits at position
+// p0 + k of the sweep's order, where its value is stored and where the
+// column ids of later levels point.  Row k has L entries among the level's
+// `width` slots (padding: value 0, column = the row's own position, so every
+// slot's gather is in bounds).  All width slots' values and column ids load
+// before the state check; then the row's input and the gathers of z; the
+// entries are subtracted in entry order, the padding never.
+//   lower: input r[rows[k]] (natural order), output zs[p0 + k];
+//   upper: input zl[src[k]] (src: the row's lower position), output zs[p0 + k]
+//          and out[rows[k]] (natural order).
 constexpr int kEllBatch = 8;
 template <class T, bool Upper>
-__global__ void __launch_bounds__(kBlock) k_ilu_sweep_ell(const int* __restrict__ rows, const int* __restrict__ len,
-                                                          const int* __restrict__ ecol, const T* __restrict__ eval,
-                                                          const T* __restrict__ dval, long base, int nrows, int width,
-                                                          const T* r, T* z, const KState* st) {
+__global__ void __launch_bounds__(kBlock) k_ilu_sweep_ell(const int* __restrict__ rows, const int* __restrict__ src,
+                                                          const int* __restrict__ len, const int* __restrict__ ecol,
+                                                          const T* __restrict__ eval, const T* __restrict__ dval,
+                                                          long base, int p0, int nrows, int width,
+                                                          const T* __restrict__ in, T* zs, T* __restrict__ out,
+                                                          const KState* st) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nrows) return;
-  const int i = rows[k];
+  const int i = Upper ? src[k] : rows[k];
   const int L = len[k];
   const int w0 = width < kEllBatch ? width : kEllBatch;
   T v[kEllBatch], zv[kEllBatch];
@@ -129,12 +109,16 @@ __global__ void __launch_bounds__(kBlock) k_ilu_sweep_ell(const int* __restrict_
       c[u] = ecol[q];
     }
   T d = T(1);
-  if constexpr (Upper) d = dval[k];
+  int o = 0;
+  if constexpr (Upper) {
+    d = dval[k];
+    o = rows[k];
+  }
   if (st && st->reason) return;
-  T s = Upper ? z[i] : r[i];
+  T s = in[i];
 #pragma unroll
   for (int u = 0; u < kEllBatch; ++u)
-    if (u < w0) zv[u] = z[c[u]];
+    if (u < w0) zv[u] = zs[c[u]];
 #pragma unroll
   for (int u = 0; u < kEllBatch; ++u)
     if (u < L) s -= v[u] * zv[u];
@@ -149,15 +133,18 @@ __global__ void __launch_bounds__(kBlock) k_ilu_sweep_ell(const int* __restrict_
       }
 #pragma unroll
     for (int u = 0; u < kEllBatch; ++u)
-      if (j0 + u < L) zv[u] = z[c[u]];
+      if (j0 + u < L) zv[u] = zs[c[u]];
 #pragma unroll
     for (int u = 0; u < kEllBatch; ++u)
       if (j0 + u < L) s -= v[u] * zv[u];
   }
-  if constexpr (Upper)
-    z[i] = s / d;
-  else
-    z[i] = s;
+  if constexpr (Upper) {
+    const T zi = s / d;
+    zs[p0 + k] = zi;
+    out[o] = zi;
+  } else {
+    zs[p0 + k] = s;
+  }
 }
 
 // out[q] = lu[src[q]] (0 where src[q] < 0)
@@ -225,37 +212,27 @@ void ilu_apply(const CsrView<T>& A, const IluLevels& lv, const T* lu, const T* r
     S355_HIP(hipGraphLaunch(g, s));
     return;
   }
-  if (w && w->ilu_lval && lv.lell.col) {
-    const T* lval = static_cast<const T*>(w->ilu_lval);
-    const T* uval = static_cast<const T*>(w->ilu_uval);
-    const T* dval = static_cast<const T*>(w->ilu_dval);
-    auto width = [](const IluEll& e, int l, int nr) { return static_cast<int>((e.off[l + 1] - e.off[l]) / nr); };
-    for (int l = 0; l < lv.nl(); ++l) {
-      const int nr = lv.lptr[l + 1] - lv.lptr[l];
-      if (nr == 0) continue;
-      hipLaunchKernelGGL((k_ilu_sweep_ell<T, false>), dim3(grid_for(nr)), dim3(kBlock), 0, s, lv.lrows + lv.lptr[l],
-                         lv.lell.len + lv.lptr[l], lv.lell.col, lval, nullptr, lv.lell.off[l], nr,
-                         width(lv.lell, l, nr), r, z, st);
-    }
-    for (int l = 0; l < lv.nu(); ++l) {
-      const int nr = lv.uptr[l + 1] - lv.uptr[l];
-      if (nr == 0) continue;
-      hipLaunchKernelGGL((k_ilu_sweep_ell<T, true>), dim3(grid_for(nr)), dim3(kBlock), 0, s, lv.urows + lv.uptr[l],
-                         lv.uell.len + lv.uptr[l], lv.uell.col, uval, dval + lv.uptr[l], lv.uell.off[l], nr,
-                         width(lv.uell, l, nr), z, z, st);
-    }
-    S355_LAUNCH_CHECK();
-    return;
-  }
+  if (!w || !w->ilu_lval || !w->ilu_zl || !w->ilu_zu || !lv.lell.col || !lv.u2l)
+    throw std::invalid_argument("ILU(0) application without its level-ELL workspace");
+  const T* lval = static_cast<const T*>(w->ilu_lval);
+  const T* uval = static_cast<const T*>(w->ilu_uval);
+  const T* dval = static_cast<const T*>(w->ilu_dval);
+  T* zl = static_cast<T*>(w->ilu_zl);
+  T* zu = static_cast<T*>(w->ilu_zu);
+  auto width = [](const IluEll& e, int l, int nr) { return static_cast<int>((e.off[l + 1] - e.off[l]) / nr); };
   for (int l = 0; l < lv.nl(); ++l) {
-    const int nr = lv.lptr[l + 1] - lv.lptr[l];
-    hipLaunchKernelGGL((k_ilu_lower<T>), dim3(grid_for(nr)), dim3(kBlock), 0, s, A, lu, r, z, lv.lrows + lv.lptr[l],
-                       nr, st);
+    const int p0 = lv.lptr[l], nr = lv.lptr[l + 1] - p0;
+    if (nr == 0) continue;
+    hipLaunchKernelGGL((k_ilu_sweep_ell<T, false>), dim3(grid_for(nr)), dim3(kBlock), 0, s, lv.lrows + p0, nullptr,
+                       lv.lell.len + p0, lv.lell.col, lval, nullptr, lv.lell.off[l], p0, nr, width(lv.lell, l, nr), r,
+                       zl, nullptr, st);
   }
   for (int l = 0; l < lv.nu(); ++l) {
-    const int nr = lv.uptr[l + 1] - lv.uptr[l];
-    hipLaunchKernelGGL((k_ilu_upper<T>), dim3(grid_for(nr)), dim3(kBlock), 0, s, A, lu, z, lv.urows + lv.uptr[l], nr,
-                       st);
+    const int p0 = lv.uptr[l], nr = lv.uptr[l + 1] - p0;
+    if (nr == 0) continue;
+    hipLaunchKernelGGL((k_ilu_sweep_ell<T, true>), dim3(grid_for(nr)), dim3(kBlock), 0, s, lv.urows + p0,
+                       lv.u2l + p0, lv.uell.len + p0, lv.uell.col, uval, dval + p0, lv.uell.off[l], p0, nr,
+                       width(lv.uell, l, nr), zl, zu, z, st);
   }
   S355_LAUNCH_CHECK();
 }
@@ -1909,8 +1886,8 @@ template void jacobi_inverse<double>(const CsrView<double>&, double*, hipStream_
 template void jacobi_inverse<float>(const CsrView<float>&, float*, hipStream_t);
 
 void ilu_ell(int n, const int* rowptr, const int* col, const int* diag, const std::vector<int>& rows,
-             const std::vector<int>& ptr, bool upper, std::vector<long>& off, std::vector<int>& ecol,
-             std::vector<int>& esrc, std::vector<int>& len, std::vector<int>& dsrc) {
+             const std::vector<int>& ptr, const std::vector<int>& pos, bool upper, std::vector<long>& off,
+             std::vector<int>& ecol, std::vector<int>& esrc, std::vector<int>& len, std::vector<int>& dsrc) {
   const int nlev = static_cast<int>(ptr.size()) - 1;
   auto first = [&](int i) { return upper ? diag[i] + 1 : rowptr[i]; };
   auto last = [&](int i) { return upper ? rowptr[i + 1] : diag[i]; };
@@ -1938,10 +1915,10 @@ void ilu_ell(int n, const int* rowptr, const int* col, const int* diag, const st
       for (int j = 0; j < w; ++j) {
         const size_t q = static_cast<size_t>(off[l]) + static_cast<size_t>(j) * nr + k;
         if (j < len[ptr[l] + k]) {
-          ecol[q] = col[first(i) + j];
+          ecol[q] = pos[col[first(i) + j]];
           esrc[q] = first(i) + j;
         } else {
-          ecol[q] = i;
+          ecol[q] = pos[i];
         }
       }
     }

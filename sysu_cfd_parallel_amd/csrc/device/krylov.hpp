@@ -111,9 +111,14 @@ void sell_fill(const int* src, size_t nslots, const S* csr_val, T* out, hipStrea
 // of the level's k-th row at off[l] + j * nrows_l + k, so that a wave's loads
 // of values and column ids are contiguous; len gives each row's count (rows
 // keep their CSR entry order, so the sums equal those of a CSR row walk).
+//
+// Each sweep keeps its z in level space: the row at position p of the
+// sweep's level lists stores its value at p, and the ELL column ids are
+// such positions.  A level's stores are then one contiguous run, and its
+// gathers land in the runs of earlier levels instead of all over the vector.
 struct IluEll {
   std::vector<long> off;        // host: nlev + 1 slot offsets
-  const int* col = nullptr;     // device: column of each slot
+  const int* col = nullptr;     // device: level-space position of each slot's column
   const int* src = nullptr;     // device: CSR position of each slot (-1 padding)
   const int* len = nullptr;     // device: entries of each row, by position in the level lists
   const int* dsrc = nullptr;    // device (upper only): CSR position of each row's diagonal
@@ -124,6 +129,7 @@ struct IluLevels {
   std::vector<int> lptr, uptr;  // host: level boundaries into lrows / urows
   const int* lrows = nullptr;   // device: rows grouped by lower level
   const int* urows = nullptr;   // device: rows grouped by upper level
+  const int* u2l = nullptr;     // device: lower-level position of the row at each upper position
   IluEll lell, uell;
   int nl() const { return static_cast<int>(lptr.size()) - 1; }
   int nu() const { return static_cast<int>(uptr.size()) - 1; }
@@ -134,10 +140,11 @@ void ilu_levels(int n, const int* rowptr, const int* col, const int* diag, std::
                 std::vector<int>& lptr, std::vector<int>& urows, std::vector<int>& uptr);
 
 // Host construction of one level-ELL copy (upper: the strictly upper part and
-// the diagonal positions; else the strictly lower part).
+// the diagonal positions; else the strictly lower part).  pos[i] is the
+// position of row i in rows, the space the column ids are given in.
 void ilu_ell(int n, const int* rowptr, const int* col, const int* diag, const std::vector<int>& rows,
-             const std::vector<int>& ptr, bool upper, std::vector<long>& off, std::vector<int>& ecol,
-             std::vector<int>& esrc, std::vector<int>& len, std::vector<int>& dsrc);
+             const std::vector<int>& ptr, const std::vector<int>& pos, bool upper, std::vector<long>& off,
+             std::vector<int>& ecol, std::vector<int>& esrc, std::vector<int>& len, std::vector<int>& dsrc);
 
 struct KResult {
   int iters = 0, reason = 0;
@@ -195,6 +202,8 @@ struct KWork {
   void* ilu_lval = nullptr; // level-ELL factor values of T: lower slots, upper slots, diagonal per row
   void* ilu_uval = nullptr;
   void* ilu_dval = nullptr;
+  void* ilu_zl = nullptr;   // n elements of T each: the lower and upper sweeps' z in level space
+  void* ilu_zu = nullptr;
   const IluLevels* ilu = nullptr;
   double* partial = nullptr;
   size_t partial_len = 0;

@@ -90,7 +90,8 @@ def test_autotuned_shapes_are_shared_within_a_process():
     a, ha, _, sa = run(case, p, 4)
     b, hb, _, sb = run(case, p, 4)
     assert sa["fp32"]["tuned"] and sa == sb
-    assert sa["fp32"]["lanes"] in (4, 8, 16) and sa["fp32"]["rows"] in (1, 2, 4, 8, 16)
+    # any shape the tuner offers may win on a case this small (0 = SELL-64)
+    assert sa["fp32"]["lanes"] in (0, 1, 2, 4, 8, 16) and sa["fp32"]["rows"] in (1, 2, 4, 8, 16)
     assert sa["fp32"]["nt"] in (0, 1)
     for n in FIELDS:
         assert np.array_equal(a[n], b[n]), n
