@@ -508,13 +508,15 @@ void Context::ensure_ilu() {
   levels_.lrows = upload(allocs_, lrows, stream);
   levels_.urows = upload(allocs_, urows, stream);
   // positions of each row in the two level orders (the sweeps' z spaces)
-  std::vector<int> lpos(dm.Nc), upos(dm.Nc), u2l(dm.Nc);
+  std::vector<int> lpos(dm.Nc), upos(dm.Nc), l2u(dm.Nc);
   for (int p = 0; p < dm.Nc; ++p) {
     lpos[lrows[p]] = p;
     upos[urows[p]] = p;
   }
-  for (int p = 0; p < dm.Nc; ++p) u2l[p] = lpos[urows[p]];
-  levels_.u2l = upload(allocs_, u2l, stream);
+  for (int p = 0; p < dm.Nc; ++p) l2u[p] = upos[lrows[p]];
+  levels_.lpos = upload(allocs_, lpos, stream);
+  levels_.l2u = upload(allocs_, l2u, stream);
+  levels_.upos = upload(allocs_, upos, stream);
   // level-ELL copies of the factors for the sweeps
   for (int up = 0; up < 2; ++up) {
     IluEll& e = up ? levels_.uell : levels_.lell;

@@ -75,28 +75,22 @@ __global__ void __launch_bounds__(kBlock) k_ilu_factor(CsrView<T> A, T* __restri
   }
 }
 
-// One level of the lower (zl = L^-1 r) or upper (zu = U^-1 zl) sweep on the
-// level-ELL copy, in level space: the level's k-th row sits at position
-// p0 + k of the sweep's order, where its value is stored and where the
-// column ids of later levels point.  Row k has L entries among the level's
-// `width` slots (padding: value 0, column = the row's own position, so every
-// slot's gather is in bounds).  All width slots' values and column ids load
-// before the state check; then the row's input and the gathers of z; the
-// entries are subtracted in entry order, the padding never.
-//   lower: input r[rows[k]] (natural order), output zs[p0 + k];
-//   upper: input zl[src[k]] (src: the row's lower position), output zs[p0 + k]
-//          and out[rows[k]] (natural order).
+// One level of the lower (z = L^-1 z) or upper (z = U^-1 z) sweep on the
+// level-ELL copy, in place in level space: the level's k-th row sits at
+// position p0 + k of the sweep's order, where its input is found, its result
+// is stored and the column ids of later levels point.  Row k has L entries
+// among the level's `width` slots (padding: value 0, column = the row's own
+// position, so every slot's gather is in bounds).  All width slots' values
+// and column ids load before the state check; then the row's input and the
+// gathers of z; the entries are subtracted in entry order, the padding never.
 constexpr int kEllBatch = 8;
 template <class T, bool Upper>
-__global__ void __launch_bounds__(kBlock) k_ilu_sweep_ell(const int* __restrict__ rows, const int* __restrict__ src,
-                                                          const int* __restrict__ len, const int* __restrict__ ecol,
+__global__ void __launch_bounds__(kBlock) k_ilu_sweep_ell(const int* __restrict__ len, const int* __restrict__ ecol,
                                                           const T* __restrict__ eval, const T* __restrict__ dval,
-                                                          long base, int p0, int nrows, int width,
-                                                          const T* __restrict__ in, T* zs, T* __restrict__ out,
+                                                          long base, int p0, int nrows, int width, T* zs,
                                                           const KState* st) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nrows) return;
-  const int i = Upper ? src[k] : rows[k];
   const int L = len[k];
   const int w0 = width < kEllBatch ? width : kEllBatch;
   T v[kEllBatch], zv[kEllBatch];
@@ -109,13 +103,9 @@ __global__ void __launch_bounds__(kBlock) k_ilu_sweep_ell(const int* __restrict_
       c[u] = ecol[q];
     }
   T d = T(1);
-  int o = 0;
-  if constexpr (Upper) {
-    d = dval[k];
-    o = rows[k];
-  }
+  if constexpr (Upper) d = dval[k];
   if (st && st->reason) return;
-  T s = in[i];
+  T s = zs[p0 + k];
 #pragma unroll
   for (int u = 0; u < kEllBatch; ++u)
     if (u < w0) zv[u] = zs[c[u]];
@@ -138,13 +128,40 @@ __global__ void __launch_bounds__(kBlock) k_ilu_sweep_ell(const int* __restrict_
     for (int u = 0; u < kEllBatch; ++u)
       if (j0 + u < L) s -= v[u] * zv[u];
   }
-  if constexpr (Upper) {
-    const T zi = s / d;
-    zs[p0 + k] = zi;
-    out[o] = zi;
-  } else {
+  if constexpr (Upper)
+    zs[p0 + k] = s / d;
+  else
     zs[p0 + k] = s;
-  }
+}
+
+// The moves between the natural order and the sweeps' level orders (Tag
+// only names the move in kernel traces).  Inside a lower level rows keep
+// their natural order, inside an upper level their lower-level order, so the
+// rows of a run of consecutive source positions that fall in one level of
+// the destination order are consecutive there: a workgroup reading (or
+// writing) a contiguous range writes (reads) one short contiguous run per
+// level, and consecutive workgroups on one XCD extend the same runs in its L2.
+// scatter: dst[pos[q]] = src[q]
+template <class T, int Tag>
+__global__ void __launch_bounds__(kBlock) k_scatter(int n, const int* __restrict__ pos, const T* __restrict__ src,
+                                                    T* __restrict__ dst, const KState* st) {
+  const int q = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int p = pos[q];
+  const T v = src[q];
+  if (st && st->reason) return;
+  dst[p] = v;
+}
+
+// gather: dst[q] = src[idx[q]]
+template <class T, int Tag>
+__global__ void __launch_bounds__(kBlock) k_gather_at(int n, const int* __restrict__ idx, const T* __restrict__ src,
+                                                      T* __restrict__ dst, const KState* st) {
+  const int q = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int p = idx[q];
+  if (st && st->reason) return;
+  dst[q] = src[p];
 }
 
 // out[q] = lu[src[q]] (0 where src[q] < 0)
@@ -212,7 +229,7 @@ void ilu_apply(const CsrView<T>& A, const IluLevels& lv, const T* lu, const T* r
     S355_HIP(hipGraphLaunch(g, s));
     return;
   }
-  if (!w || !w->ilu_lval || !w->ilu_zl || !w->ilu_zu || !lv.lell.col || !lv.u2l)
+  if (!w || !w->ilu_lval || !w->ilu_zl || !w->ilu_zu || !lv.lell.col || !lv.lpos || !lv.l2u || !lv.upos)
     throw std::invalid_argument("ILU(0) application without its level-ELL workspace");
   const T* lval = static_cast<const T*>(w->ilu_lval);
   const T* uval = static_cast<const T*>(w->ilu_uval);
@@ -220,20 +237,25 @@ void ilu_apply(const CsrView<T>& A, const IluLevels& lv, const T* lu, const T* r
   T* zl = static_cast<T*>(w->ilu_zl);
   T* zu = static_cast<T*>(w->ilu_zu);
   auto width = [](const IluEll& e, int l, int nr) { return static_cast<int>((e.off[l + 1] - e.off[l]) / nr); };
+  const int n = lv.lptr.back();
+  const int gn = grid_for(n);
+  // r into the lower order, the lower sweep, its result into the upper
+  // order, the upper sweep, z back into the natural order
+  hipLaunchKernelGGL((k_scatter<T, 0>), dim3(gn), dim3(kBlock), 0, s, n, lv.lpos, r, zl, st);
   for (int l = 0; l < lv.nl(); ++l) {
     const int p0 = lv.lptr[l], nr = lv.lptr[l + 1] - p0;
     if (nr == 0) continue;
-    hipLaunchKernelGGL((k_ilu_sweep_ell<T, false>), dim3(grid_for(nr)), dim3(kBlock), 0, s, lv.lrows + p0, nullptr,
-                       lv.lell.len + p0, lv.lell.col, lval, nullptr, lv.lell.off[l], p0, nr, width(lv.lell, l, nr), r,
-                       zl, nullptr, st);
+    hipLaunchKernelGGL((k_ilu_sweep_ell<T, false>), dim3(grid_for(nr)), dim3(kBlock), 0, s, lv.lell.len + p0,
+                       lv.lell.col, lval, nullptr, lv.lell.off[l], p0, nr, width(lv.lell, l, nr), zl, st);
   }
+  hipLaunchKernelGGL((k_scatter<T, 1>), dim3(gn), dim3(kBlock), 0, s, n, lv.l2u, zl, zu, st);
   for (int l = 0; l < lv.nu(); ++l) {
     const int p0 = lv.uptr[l], nr = lv.uptr[l + 1] - p0;
     if (nr == 0) continue;
-    hipLaunchKernelGGL((k_ilu_sweep_ell<T, true>), dim3(grid_for(nr)), dim3(kBlock), 0, s, lv.urows + p0,
-                       lv.u2l + p0, lv.uell.len + p0, lv.uell.col, uval, dval + p0, lv.uell.off[l], p0, nr,
-                       width(lv.uell, l, nr), zl, zu, z, st);
+    hipLaunchKernelGGL((k_ilu_sweep_ell<T, true>), dim3(grid_for(nr)), dim3(kBlock), 0, s, lv.uell.len + p0,
+                       lv.uell.col, uval, dval + p0, lv.uell.off[l], p0, nr, width(lv.uell, l, nr), zu, st);
   }
+  hipLaunchKernelGGL((k_gather_at<T, 2>), dim3(gn), dim3(kBlock), 0, s, n, lv.upos, zu, z, st);
   S355_LAUNCH_CHECK();
 }
 
@@ -1935,15 +1957,19 @@ void ilu_levels(int n, const int* rowptr, const int* col, const int* diag, std::
     lev[i] = l;
     nl = std::max(nl, l + 1);
   }
-  auto group = [&](int nlev, std::vector<int>& rows, std::vector<int>& ptr) {
+  // rows by level; inside a level in the order they come in `order`
+  auto group = [&](int nlev, const int* order, std::vector<int>& rows, std::vector<int>& ptr) {
     ptr.assign(nlev + 1, 0);
     for (int i = 0; i < n; ++i) ptr[lev[i] + 1]++;
     for (int l = 0; l < nlev; ++l) ptr[l + 1] += ptr[l];
     rows.assign(n, 0);
     std::vector<int> pos(ptr.begin(), ptr.end() - 1);
-    for (int i = 0; i < n; ++i) rows[pos[lev[i]]++] = i;
+    for (int k = 0; k < n; ++k) {
+      const int i = order ? order[k] : k;
+      rows[pos[lev[i]]++] = i;
+    }
   };
-  group(nl, lrows, lptr);
+  group(nl, nullptr, lrows, lptr);  // natural order inside a lower level
   int nu = 0;
   for (int i = n - 1; i >= 0; --i) {
     int l = 0;
@@ -1951,7 +1977,10 @@ void ilu_levels(int n, const int* rowptr, const int* col, const int* diag, std::
     lev[i] = l;
     nu = std::max(nu, l + 1);
   }
-  group(nu, urows, uptr);
+  // lower-level order inside an upper level: the rows of a run of one lower
+  // level that share an upper level stay consecutive in it, which keeps the
+  // move of the lower sweep's result into the upper order local
+  group(nu, lrows.data(), urows, uptr);
 }
 
 }  // namespace s355

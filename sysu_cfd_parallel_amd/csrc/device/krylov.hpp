@@ -112,10 +112,13 @@ void sell_fill(const int* src, size_t nslots, const S* csr_val, T* out, hipStrea
 // of values and column ids are contiguous; len gives each row's count (rows
 // keep their CSR entry order, so the sums equal those of a CSR row walk).
 //
-// Each sweep keeps its z in level space: the row at position p of the
-// sweep's level lists stores its value at p, and the ELL column ids are
-// such positions.  A level's stores are then one contiguous run, and its
-// gathers land in the runs of earlier levels instead of all over the vector.
+// Each sweep runs in place in level space: the row at position p of the
+// sweep's level lists finds its input and stores its value at p, and the ELL
+// column ids are such positions.  A level's loads and stores are then one
+// contiguous run, and its gathers land in the runs of earlier levels instead
+// of all over the vector.  One gather pass over the whole vector moves r
+// into the lower order, one moves the lower result into the upper order and
+// one moves z back into the natural order.
 struct IluEll {
   std::vector<long> off;        // host: nlev + 1 slot offsets
   const int* col = nullptr;     // device: level-space position of each slot's column
@@ -129,7 +132,9 @@ struct IluLevels {
   std::vector<int> lptr, uptr;  // host: level boundaries into lrows / urows
   const int* lrows = nullptr;   // device: rows grouped by lower level
   const int* urows = nullptr;   // device: rows grouped by upper level
-  const int* u2l = nullptr;     // device: lower-level position of the row at each upper position
+  const int* lpos = nullptr;    // device: lower-level position of each row
+  const int* l2u = nullptr;     // device: upper-level position of the row at each lower position
+  const int* upos = nullptr;    // device: upper-level position of each row
   IluEll lell, uell;
   int nl() const { return static_cast<int>(lptr.size()) - 1; }
   int nu() const { return static_cast<int>(uptr.size()) - 1; }
