@@ -554,6 +554,43 @@ __global__ void __launch_bounds__(kBlock) k_mdot(const T* __restrict__ V, size_t
   }
 }
 
+// Sum of p[b] over b = b0 + threadIdx.x, + kBlock, ... below b1, added in
+// that order; the loads of 8 consecutive terms are issued before any is
+// added (indices clamped into the range, the clamped terms never added).
+__device__ __forceinline__ double strided_sum(const double* __restrict__ p, int b0, int b1) {
+  double v = 0.0;
+  for (int b = b0 + static_cast<int>(threadIdx.x); b < b1; b += 8 * kBlock) {
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x[u] = p[min(b + u * kBlock, b1 - 1)];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (b + u * kBlock < b1) v += x[u];
+  }
+  return v;
+}
+
+// The same for interleaved pairs: s0 = sum of p[2b], s1 = sum of p[2b + 1].
+__device__ __forceinline__ void strided_sum2(const double* __restrict__ p, int n, double& s0, double& s1) {
+  s0 = 0.0;
+  s1 = 0.0;
+  for (int b = static_cast<int>(threadIdx.x); b < n; b += 8 * kBlock) {
+    double x0[8], x1[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int q = min(b + u * kBlock, n - 1);
+      x0[u] = p[2 * q];
+      x1[u] = p[2 * q + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (b + u * kBlock < n) {
+        s0 += x0[u];
+        s1 += x1[u];
+      }
+  }
+}
+
 // ---- hcol[k] = scale_k * sum over the case's tiles of partial[k][b]
 // (block (k, case))
 __global__ void __launch_bounds__(kBlock) k_mdot_final(const double* partial, Tiles t, int pstride, KState* st) {
@@ -565,7 +602,7 @@ __global__ void __launch_bounds__(kBlock) k_mdot_final(const double* partial, Ti
   t.tiles_of(blockIdx.y, t0, t1);
   double v = 0.0;
   const double* pk = partial + static_cast<size_t>(k) * t.ntile;  // column k: contiguous over tiles
-  for (int b = t0 + threadIdx.x; b < t1; b += kBlock) v += pk[b];
+  v = strided_sum(pk, t0, t1);
   v = block_sum(v, lds);
   if (threadIdx.x == 0) s->hcol[k] = v * s->scale[k];
 }
@@ -622,7 +659,7 @@ __global__ void __launch_bounds__(kBlock) k_arnoldi_finalize(const double* parti
   int t0, t1;
   t.tiles_of(blockIdx.x, t0, t1);
   double s = 0.0;
-  for (int b = t0 + threadIdx.x; b < t1; b += kBlock) s += partial[b];
+  s = strided_sum(partial, t0, t1);
   for (int j = threadIdx.x; j <= it; j += kBlock) {
     col[j] = s_->hcol[j];
     cc[j] = s_->cc[j];
@@ -812,7 +849,7 @@ __global__ void __launch_bounds__(kBlock) k_snorm_final(const double* partial, T
   int t0, t1;
   t.tiles_of(blockIdx.x, t0, t1);
   double s = 0.0;
-  for (int b = t0 + threadIdx.x; b < t1; b += kBlock) s += partial[b];
+  s = strided_sum(partial, t0, t1);
   s = block_sum(s, lds);
   if (threadIdx.x == 0) st[blockIdx.x].snorm = sqrt(s);
 }
@@ -831,7 +868,7 @@ __global__ void __launch_bounds__(kBlock) k_restart_finalize(const double* parti
   int t0, t1;
   t.tiles_of(blockIdx.x, t0, t1);
   double s = 0.0;
-  for (int b = t0 + threadIdx.x; b < t1; b += kBlock) s += partial[b];
+  s = strided_sum(partial, t0, t1);
   s = block_sum(s, lds);
   if (threadIdx.x != 0) return;
   const double res = sqrt(s);
@@ -929,10 +966,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_start(CsrView<T> A, const T* b, T
 __global__ void __launch_bounds__(kBlock) k_cg_start_final(const double* partial, int nblk, KState* st, int guess_zero) {
   __shared__ double lds[kBlock / kWave];
   double zz = 0.0, zr = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += kBlock) {
-    zz += partial[2 * b];
-    zr += partial[2 * b + 1];
-  }
+  strided_sum2(partial, nblk, zz, zr);
   zz = block_sum(zz, lds);
   zr = block_sum(zr, lds);
   if (threadIdx.x != 0) return;
@@ -982,7 +1016,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_alpha(const double* partial, int 
   if (st->reason) return;
   __shared__ double lds[kBlock / kWave];
   double s = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += kBlock) s += partial[b];
+  s = strided_sum(partial, 0, nblk);
   s = block_sum(s, lds);
   if (threadIdx.x != 0) return;
   const double beta = st->beta;
@@ -1032,10 +1066,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_final(const double* partial, int 
   if (st->reason) return;
   __shared__ double lds[kBlock / kWave];
   double zz = 0.0, zr = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += kBlock) {
-    zz += partial[2 * b];
-    zr += partial[2 * b + 1];
-  }
+  strided_sum2(partial, nblk, zz, zr);
   zz = block_sum(zz, lds);
   zr = block_sum(zr, lds);
   if (threadIdx.x != 0) return;
@@ -1473,7 +1504,7 @@ __global__ void __launch_bounds__(kBlock) k_ir_residual(CsrView<double> A, const
 __global__ void __launch_bounds__(kBlock) k_sum_to(const double* partial, int nblk, double* out) {
   __shared__ double lds[kBlock / kWave];
   double s = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += kBlock) s += partial[b];
+  s = strided_sum(partial, 0, nblk);
   s = block_sum(s, lds);
   if (threadIdx.x == 0) *out = s;
 }
