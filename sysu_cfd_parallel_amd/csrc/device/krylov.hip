@@ -512,7 +512,8 @@ __global__ void __launch_bounds__(kBlock) k_mdot(const T* __restrict__ V, size_t
   int a, e, c;
   t.range(blockIdx.x, a, e, c);
   if (st[c].reason) return;
-  __shared__ double lds[kBlock / kWave][8];
+  // each wave's sum of every k of the block, reduced over the waves once
+  __shared__ double lds[kBlock / kWave][8 * ((kMaxRestart + 8) / 8)];
   const size_t end = static_cast<size_t>(e);
   const bool vec = S::vec_ok(a, ld);
   const T* w = V + static_cast<size_t>(it + 1) * ld;
@@ -520,8 +521,9 @@ __global__ void __launch_bounds__(kBlock) k_mdot(const T* __restrict__ V, size_t
 #pragma unroll
   for (int q = 0; q < S::NC; ++q) load_chunk<T, S::VW>(w, S::row0(a, q), end, vec, wr + q * S::VW);
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  const int kend = 8 * cpb * (static_cast<int>(blockIdx.y) + 1);
-  for (int k0 = 8 * cpb * static_cast<int>(blockIdx.y); k0 <= it && k0 < kend; k0 += 8) {
+  const int kbeg = 8 * cpb * static_cast<int>(blockIdx.y);
+  const int kend = min(it + 1, 8 * cpb * (static_cast<int>(blockIdx.y) + 1));
+  for (int k0 = kbeg; k0 < kend; k0 += 8) {
     double acc[8];
 #pragma unroll
     for (int kk = 0; kk < 8; ++kk) {
@@ -538,19 +540,15 @@ __global__ void __launch_bounds__(kBlock) k_mdot(const T* __restrict__ V, size_t
         }
       }
     }
+    const double sk = wave_sum8(acc);
+    if ((lane & 7) == 0) lds[wid][k0 - kbeg + (lane >> 3)] = sk;
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < kend - kbeg; j += kBlock) {
+    double s = 0.0;
 #pragma unroll
-    for (int kk = 0; kk < 8; ++kk) {
-      acc[kk] = wave_sum(acc[kk]);
-      if (lane == 0) lds[wid][kk] = acc[kk];
-    }
-    __syncthreads();
-    if (threadIdx.x < 8 && k0 + static_cast<int>(threadIdx.x) <= it) {
-      double s = 0.0;
-#pragma unroll
-      for (int q = 0; q < kBlock / kWave; ++q) s += lds[q][threadIdx.x];
-      partial[static_cast<size_t>(k0 + threadIdx.x) * t.ntile + blockIdx.x] = s;
-    }
-    __syncthreads();
+    for (int q = 0; q < kBlock / kWave; ++q) s += lds[q][j];
+    partial[static_cast<size_t>(kbeg + j) * t.ntile + blockIdx.x] = s;
   }
 }
 
