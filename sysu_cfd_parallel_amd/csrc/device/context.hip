@@ -766,6 +766,24 @@ void Context::ensure_sell() {
     pt.row = upload(allocs_, row, stream);
     pt.col = upload(allocs_, col, stream);
     pt.src = upload(allocs_, src, stream);
+    std::vector<int> scase;
+    if (ncase > 1) {
+      // the case of each slice whose rows all belong to one case, else -1
+      const std::vector<int>& cr = lay.cell;
+      auto case_of = [&](int r) {
+        return static_cast<int>(std::upper_bound(cr.begin(), cr.end(), r) - cr.begin()) - 1;
+      };
+      const int nslice = static_cast<int>(ptr.size()) - 1;
+      scase.assign(nslice, -1);
+      for (int sl = 0; sl < nslice; ++sl) {
+        const int p0 = sl * 64, p1 = std::min(dm.Nc, p0 + 64);
+        const int c0 = case_of(row[p0]);
+        bool one = true;
+        for (int p = p0 + 1; p < p1 && one; ++p) one = case_of(row[p]) == c0;
+        if (one) scase[sl] = c0;
+      }
+      pt.scase = upload(allocs_, scase, stream);
+    }
     sync();  // host staging vectors go out of scope
     kwork_.sell[v] = kwork32_.sell[v] = pt;
   }

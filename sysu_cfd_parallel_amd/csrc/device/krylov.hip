@@ -334,23 +334,45 @@ __global__ void __launch_bounds__(kBlock) k_spmv_sell(SellView<T> A, Tiles t, co
                                                       int it) {
   const int pos = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (pos >= A.n) return;
-  // the slice bounds, the row and (one case) the state load together
-  const int sl = pos >> 6, lane = pos & 63;
+  // a wave is one slice (64 consecutive positions), so the slice index is
+  // wave-uniform and its bounds load through the scalar cache
+  const int sl = __builtin_amdgcn_readfirstlane(pos >> 6), lane = pos & 63;
   const int qa = A.ptr[sl], q1 = A.ptr[sl + 1];
   const int row = A.row[pos];
-  const KState* sc = st + t.case_of(row);
+  // kSellBatch slots at a time: every column and value load of the batch is
+  // issued, then every gather of x, then the products are added in slot
+  // order (the wave's slice has one width, so the guards are wave-uniform).
+  // The first batch loads before the state check, in parallel with the row
+  // and the state, which it does not depend on; its loads are unconditional
+  // (a slot past the slice width reloads the lane's first slot and is never
+  // added), so the compiler issues all of them before waiting on any.
+  T v[kSellBatch], xv[kSellBatch];
+  int c[kSellBatch];
+  int q0 = qa + lane;
+#pragma unroll
+  for (int u = 0; u < kSellBatch; ++u) {
+    const int q = q0 + 64 * u < q1 ? q0 + 64 * u : q0;
+    v[u] = ld_stream<T, NT>(A.val + q);
+    c[u] = ld_stream<int, NT>(A.col + q);
+  }
+  // the case: per slice when the slice lies in one case (batches), else by row
+  int cs = 0;
+  if (A.scase) cs = A.scase[sl];
+  if (cs < 0 || !A.scase) cs = t.case_of(row);
+  const KState* sc = st + cs;
   if (sc->reason) return;
-  // loaded ahead of the slots, used after them (PC: Jacobi, inv is set)
+  // used after the slots (PC: Jacobi, inv is set)
   const double scl = sc->scale[it];
   T invr = T(1);
   if constexpr (PC) invr = inv[row];
   T acc = T(0);
-  // kSellBatch slots at a time: every column and value load of the batch is
-  // issued, then every gather of x, then the products are added in slot
-  // order (the wave's slice has one width, so the guards are wave-uniform)
-  for (int q0 = qa + lane; q0 < q1; q0 += 64 * kSellBatch) {
-    T v[kSellBatch], xv[kSellBatch];
-    int c[kSellBatch];
+#pragma unroll
+  for (int u = 0; u < kSellBatch; ++u) xv[u] = x[c[u]];
+#pragma unroll
+  for (int u = 0; u < kSellBatch; ++u)
+    if (q0 + 64 * u < q1) acc += v[u] * xv[u];
+  // rows longer than one batch: the rest, batch by batch
+  for (q0 += 64 * kSellBatch; q0 < q1; q0 += 64 * kSellBatch) {
 #pragma unroll
     for (int u = 0; u < kSellBatch; ++u)
       if (q0 + 64 * u < q1) {
@@ -433,6 +455,7 @@ SellView<T> sell_of(const KWork& w, int n, int variant = -1) {
     v.row = w.sell[sv].row;
     v.col = w.sell[sv].col;
     v.val = static_cast<const T*>(w.sell_vals[sv]);
+    v.scase = w.sell[sv].scase;
   }
   return v;
 }

@@ -71,10 +71,11 @@ struct CsrView {
 template <class T>
 struct SellView {
   int n = 0;
-  const int* ptr = nullptr;  // nslice + 1 offsets
-  const int* row = nullptr;  // n: matrix row of each slot row
+  const int* ptr = nullptr;    // nslice + 1 offsets
+  const int* row = nullptr;    // n: matrix row of each slot row
   const int* col = nullptr;
   const T* val = nullptr;
+  const int* scase = nullptr;  // batches: case of each slice, -1 when its rows span two cases
 };
 
 // SELL windows the autotuner chooses from: a short window keeps the rows of
@@ -90,6 +91,7 @@ struct SellPat {
   const int* row = nullptr;
   const int* col = nullptr;
   const int* src = nullptr;
+  const int* scase = nullptr;  // batches only (see SellView)
   size_t slots = 0;
 };
 
