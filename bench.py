@@ -42,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # pressure-correction matrix (5,995,990 cells), from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes scaled by a 1 GiB copy (profiles/hbm_traffic_channel_6M.md)
 MEASURED_TRAFFIC = {
-    ("k_spmv_sell<float, true, true>", 5995990): 437.8e6,
+    ("k_spmv_sell<float, true, true>", 5995990): 447.1e6,
 }
 METRIC = "SIMPLE iters/s @4M cells, FP64 vs mixed (<=1e-6 rel err); SpMV % HBM peak"
 STAGES = ["total", "build_velocity", "solve_velocity", "build_pressure", "solve_pressure", "other"]

@@ -381,8 +381,10 @@ int s355_set_kernel_shapes(s355_ctx* ctx, int precision, int lanes, int nt, int 
 /* The autotuner's timings for one precision (0 FP64, 1 FP32): spmv_ms[v] for
  * v = 2 k + nt, k = 0..4 for CSR at 1, 2, 4, 8, 16 lanes per row, k = 5, 6, 7
  * for SELL-64 sorted in windows of 64, 512, 4096 rows; gs_ms[r] for 2^r rows
- * per thread (one multi-dot + Gram-Schmidt step at iteration *gs_it); -1
- * where not timed.  No reference counterpart. */
+ * per thread (multi-dot, its final reduction and the Gram-Schmidt update,
+ * averaged over the Arnoldi steps a quarter, half and three quarters into the
+ * restart cycle; *gs_it is the middle one); -1 where not timed.  No reference
+ * counterpart. */
 int s355_tune_report(s355_ctx* ctx, int precision, double spmv_ms[16], double gs_ms[5], int* gs_it);
 
 /* Host-side set-up only (no device): the integer and metric output of the

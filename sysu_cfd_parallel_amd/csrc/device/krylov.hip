@@ -1680,7 +1680,9 @@ void autotune(const CsrView<T>& A, KWork& w, int m, hipStream_t s) {
         }
       }
     }
-  // one mid-cycle Arnoldi step of the Gram-Schmidt pair at every tile size
+  // the Gram-Schmidt launches of three Arnoldi steps across the cycle (a
+  // quarter, half and three quarters of the restart length) at every tile
+  // size; the tile count also sets the work of the final reduction
   const int it = m / 2;
   int best_r = pick_r(n, sizeof(T));
   best = 1e300;
@@ -1689,10 +1691,13 @@ void autotune(const CsrView<T>& A, KWork& w, int m, hipStream_t s) {
     if (static_cast<size_t>(t.ntile) * (m + 1) > w.partial_len) continue;
     const double ms = time_ms(
         [&] {
-          Dispatch<T>::mdot(R, V, ld, t, it, w.partial, m + 1, w.st, s);
-          Dispatch<T>::maxpy(R, V, ld, t, it, w.partial, w.st, s);
+          for (int j : {m / 4, m / 2, (3 * m) / 4}) {
+            Dispatch<T>::mdot(R, V, ld, t, j, w.partial, m + 1, w.st, s);
+            hipLaunchKernelGGL(k_mdot_final, dim3(j + 1, w.ncase), dim3(kBlock), 0, s, w.partial, t, m + 1, w.st);
+            Dispatch<T>::maxpy(R, V, ld, t, j, w.partial, w.st, s);
+          }
         },
-        3, s);
+        5, s) / 3.0;
     rep.gs_ms[log2_r(R)] = ms;
     if (ms < best) {
       best = ms;
