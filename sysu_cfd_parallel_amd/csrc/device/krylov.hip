@@ -328,38 +328,36 @@ __global__ void __launch_bounds__(kBlock) k_spmv_pc(CsrView<T> A, Tiles t, const
 // ---- the same product on the SELL-64 copy: one row per lane, the row's
 // entries summed in CSR order
 constexpr int kSellBatch = 8;
-template <class T, bool NT, bool PC>
-__global__ void __launch_bounds__(kBlock) k_spmv_sell(SellView<T> A, Tiles t, const T* __restrict__ x,
-                                                      const T* __restrict__ inv, T* __restrict__ y, const KState* st,
-                                                      int it) {
-  const int pos = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (pos >= A.n) return;
-  // a wave is one slice (64 consecutive positions), so the slice index is
-  // wave-uniform and its bounds load through the scalar cache
-  const int sl = __builtin_amdgcn_readfirstlane(pos >> 6), lane = pos & 63;
-  const int qa = A.ptr[sl], q1 = A.ptr[sl + 1];
+
+// One row of a slice of width W (1..kSellBatch, a compile-time constant, so
+// the W slot loads are unconditional and the compiler issues all of them
+// before waiting on any), or of width > kSellBatch for W = 0 (batches of
+// kSellBatch slots, the guards wave-uniform).  Every column and value load of
+// a batch is issued, then every gather of x, then the products are added in
+// slot order.  The first batch loads before the state check, in parallel
+// with the row id and the state, which it does not depend on.
+template <class T, bool NT, bool PC, int W>
+__device__ __forceinline__ void sell_row(const SellView<T>& A, const Tiles& t, const T* __restrict__ x,
+                                         const T* __restrict__ inv, T* __restrict__ y, const KState* st, int it,
+                                         int sl, int pos, int qa, int q1) {
+  constexpr int B = W > 0 ? W : kSellBatch;
   const int row = A.row[pos];
-  // kSellBatch slots at a time: every column and value load of the batch is
-  // issued, then every gather of x, then the products are added in slot
-  // order (the wave's slice has one width, so the guards are wave-uniform).
-  // The first batch loads before the state check, in parallel with the row
-  // and the state, which it does not depend on; its loads are unconditional
-  // (a slot past the slice width reloads the lane's first slot and is never
-  // added), so the compiler issues all of them before waiting on any.
-  T v[kSellBatch], xv[kSellBatch];
-  int c[kSellBatch];
-  int q0 = qa + lane;
+  int q0 = qa + (pos & 63);
+  T v[B], xv[B];
+  int c[B];
 #pragma unroll
-  for (int u = 0; u < kSellBatch; ++u) {
-    const int q = q0 + 64 * u < q1 ? q0 + 64 * u : q0;
-    v[u] = ld_stream<T, NT>(A.val + q);
-    c[u] = ld_stream<int, NT>(A.col + q);
+  for (int u = 0; u < B; ++u) {
+    v[u] = ld_stream<T, NT>(A.val + q0 + 64 * u);
+    c[u] = ld_stream<int, NT>(A.col + q0 + 64 * u);
   }
-  // the case: per slice when the slice lies in one case (batches), else by row
-  int cs = 0;
-  if (A.scase) cs = A.scase[sl];
-  if (cs < 0 || !A.scase) cs = t.case_of(row);
-  const KState* sc = st + cs;
+  // the case: one case without a slice table; in batches per slice when the
+  // slice lies in one case, else by row
+  const KState* sc = st;
+  if (A.scase) {
+    int cs = A.scase[sl];
+    if (cs < 0) cs = t.case_of(row);
+    sc = st + cs;
+  }
   if (sc->reason) return;
   // used after the slots (PC: Jacobi, inv is set)
   const double scl = sc->scale[it];
@@ -367,28 +365,52 @@ __global__ void __launch_bounds__(kBlock) k_spmv_sell(SellView<T> A, Tiles t, co
   if constexpr (PC) invr = inv[row];
   T acc = T(0);
 #pragma unroll
-  for (int u = 0; u < kSellBatch; ++u) xv[u] = x[c[u]];
+  for (int u = 0; u < B; ++u) xv[u] = x[c[u]];
 #pragma unroll
-  for (int u = 0; u < kSellBatch; ++u)
-    if (q0 + 64 * u < q1) acc += v[u] * xv[u];
-  // rows longer than one batch: the rest, batch by batch
-  for (q0 += 64 * kSellBatch; q0 < q1; q0 += 64 * kSellBatch) {
+  for (int u = 0; u < B; ++u) acc += v[u] * xv[u];
+  if constexpr (W == 0) {
+    // rows longer than one batch: the rest, batch by batch
+    for (q0 += 64 * kSellBatch; q0 < q1; q0 += 64 * kSellBatch) {
 #pragma unroll
-    for (int u = 0; u < kSellBatch; ++u)
-      if (q0 + 64 * u < q1) {
-        v[u] = ld_stream<T, NT>(A.val + q0 + 64 * u);
-        c[u] = ld_stream<int, NT>(A.col + q0 + 64 * u);
-      }
+      for (int u = 0; u < kSellBatch; ++u)
+        if (q0 + 64 * u < q1) {
+          v[u] = ld_stream<T, NT>(A.val + q0 + 64 * u);
+          c[u] = ld_stream<int, NT>(A.col + q0 + 64 * u);
+        }
 #pragma unroll
-    for (int u = 0; u < kSellBatch; ++u)
-      if (q0 + 64 * u < q1) xv[u] = x[c[u]];
+      for (int u = 0; u < kSellBatch; ++u)
+        if (q0 + 64 * u < q1) xv[u] = x[c[u]];
 #pragma unroll
-    for (int u = 0; u < kSellBatch; ++u)
-      if (q0 + 64 * u < q1) acc += v[u] * xv[u];
+      for (int u = 0; u < kSellBatch; ++u)
+        if (q0 + 64 * u < q1) acc += v[u] * xv[u];
+    }
   }
   double z = static_cast<double>(acc);
   if constexpr (PC) z *= static_cast<double>(invr);
   y[row] = static_cast<T>(z * scl);
+}
+
+template <class T, bool NT, bool PC>
+__global__ void __launch_bounds__(kBlock) k_spmv_sell(SellView<T> A, Tiles t, const T* __restrict__ x,
+                                                      const T* __restrict__ inv, T* __restrict__ y, const KState* st,
+                                                      int it) {
+  const int pos = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (pos >= A.n) return;
+  // a wave is one slice (64 consecutive positions), so the slice index and
+  // width are wave-uniform and the slice bounds load through the scalar cache
+  const int sl = __builtin_amdgcn_readfirstlane(pos >> 6);
+  const int qa = A.ptr[sl], q1 = A.ptr[sl + 1];
+  switch ((q1 - qa) >> 6) {
+    case 1: sell_row<T, NT, PC, 1>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
+    case 2: sell_row<T, NT, PC, 2>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
+    case 3: sell_row<T, NT, PC, 3>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
+    case 4: sell_row<T, NT, PC, 4>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
+    case 5: sell_row<T, NT, PC, 5>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
+    case 6: sell_row<T, NT, PC, 6>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
+    case 7: sell_row<T, NT, PC, 7>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
+    case 8: sell_row<T, NT, PC, 8>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
+    default: sell_row<T, NT, PC, 0>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
+  }
 }
 
 template <class T, class S>
