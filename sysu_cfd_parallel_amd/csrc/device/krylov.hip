@@ -987,8 +987,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_start(CsrView<T> A, const T* b, T
     if (zero_guess) {
       x[i] = T(0);
     } else {
-      T s = T(0);
-      for (int p = A.rowptr[i]; p < A.rowptr[i + 1]; ++p) s += A.val[p] * x[A.col[p]];
+      const T s = csr_dot<T, T, false>(A.val, A.col, A.rowptr[i], A.rowptr[i + 1], 1, x, T(0));
       rv -= static_cast<double>(s);
     }
     const T rt = static_cast<T>(rv);
@@ -1046,8 +1045,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_spmv_dot(CsrView<T> A, const T* p
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double d = 0.0;
   if (i < A.n) {
-    T s = T(0);
-    for (int q = A.rowptr[i]; q < A.rowptr[i + 1]; ++q) s += A.val[q] * p[A.col[q]];
+    // the row's slots loaded in batches ahead of their gathers, summed in order
+    const T s = csr_dot<T, T, false>(A.val, A.col, A.rowptr[i], A.rowptr[i + 1], 1, p, T(0));
     w[i] = s;
     d = static_cast<double>(p[i]) * static_cast<double>(s);
   }
@@ -1530,9 +1529,7 @@ __global__ void __launch_bounds__(kBlock) k_ir_residual(CsrView<double> A, const
   if (i < A.n) {
     double rv = b[i];
     if (!b_only) {
-      double s = 0.0;
-      const int p1 = A.rowptr[i + 1];
-      for (int p = A.rowptr[i]; p < p1; ++p) s += A.val[p] * x[A.col[p]];
+      const double s = csr_dot<double, double, false>(A.val, A.col, A.rowptr[i], A.rowptr[i + 1], 1, x, 0.0);
       rv -= s;
     }
     r[i] = rv;
