@@ -240,19 +240,25 @@ void ilu_apply(const CsrView<T>& A, const IluLevels& lv, const T* lu, const T* r
   const int n = lv.lptr.back();
   const int gn = grid_for(n);
   // r into the lower order, the lower sweep, its result into the upper
-  // order, the upper sweep, z back into the natural order
+  // order, the upper sweep, z back into the natural order.  A level below
+  // 64 K rows launches one-wave workgroups, so its rows spread over all CUs
+  // (each row is a chain of dependent loads, not a share of bandwidth);
+  // larger levels keep 256-thread workgroups
+  auto sweep_block = [](int nr) { return nr < (kBlock << 8) ? kWave : kBlock; };
   hipLaunchKernelGGL((k_scatter<T, 0>), dim3(gn), dim3(kBlock), 0, s, n, lv.lpos, r, zl, st);
   for (int l = 0; l < lv.nl(); ++l) {
     const int p0 = lv.lptr[l], nr = lv.lptr[l + 1] - p0;
     if (nr == 0) continue;
-    hipLaunchKernelGGL((k_ilu_sweep_ell<T, false>), dim3(grid_for(nr)), dim3(kBlock), 0, s, lv.lell.len + p0,
+    const int b = sweep_block(nr);
+    hipLaunchKernelGGL((k_ilu_sweep_ell<T, false>), dim3(grid_for(nr, b)), dim3(b), 0, s, lv.lell.len + p0,
                        lv.lell.col, lval, nullptr, lv.lell.off[l], p0, nr, width(lv.lell, l, nr), zl, st);
   }
   hipLaunchKernelGGL((k_scatter<T, 1>), dim3(gn), dim3(kBlock), 0, s, n, lv.l2u, zl, zu, st);
   for (int l = 0; l < lv.nu(); ++l) {
     const int p0 = lv.uptr[l], nr = lv.uptr[l + 1] - p0;
     if (nr == 0) continue;
-    hipLaunchKernelGGL((k_ilu_sweep_ell<T, true>), dim3(grid_for(nr)), dim3(kBlock), 0, s, lv.uell.len + p0,
+    const int b = sweep_block(nr);
+    hipLaunchKernelGGL((k_ilu_sweep_ell<T, true>), dim3(grid_for(nr, b)), dim3(b), 0, s, lv.uell.len + p0,
                        lv.uell.col, uval, dval + p0, lv.uell.off[l], p0, nr, width(lv.uell, l, nr), zu, st);
   }
   hipLaunchKernelGGL((k_gather_at<T, 2>), dim3(gn), dim3(kBlock), 0, s, n, lv.upos, zu, z, st);
