@@ -547,7 +547,20 @@ struct GsShape {
     return static_cast<size_t>(a) + (static_cast<size_t>(q) * kBlock + threadIdx.x) * VW;
   }
   __device__ static bool vec_ok(int a, size_t ld) { return (a % Vec16<T>::n) == 0 && (ld % Vec16<T>::n) == 0; }
+  // every chunk of this thread is whole and loads as one 16-byte vector
+  __device__ static bool full(int a, size_t end, size_t ld) {
+    return VW == Vec16<T>::n && vec_ok(a, ld) && row0(a, NC - 1) + VW <= end;
+  }
 };
+
+template <class T>
+__device__ __forceinline__ typename Vec16<T>::type load16(const T* p) {
+  return *reinterpret_cast<const typename Vec16<T>::type*>(p);
+}
+template <class T>
+__device__ __forceinline__ double lane_of(const typename Vec16<T>::type& v, int u) {
+  return static_cast<double>(reinterpret_cast<const T*>(&v)[u]);
+}
 
 // ---- partial[k][b] = sum over tile b of w_i V_k,i, k = 0..it (k-major, so
 // that the per-k reduction over the tiles reads contiguously; the row stride
@@ -574,20 +587,39 @@ __global__ void __launch_bounds__(kBlock) k_mdot(const T* __restrict__ V, size_t
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
   const int kbeg = 8 * cpb * static_cast<int>(blockIdx.y);
   const int kend = min(it + 1, 8 * cpb * (static_cast<int>(blockIdx.y) + 1));
+  const bool full = S::full(a, end, ld);
   for (int k0 = kbeg; k0 < kend; k0 += 8) {
     double acc[8];
 #pragma unroll
-    for (int kk = 0; kk < 8; ++kk) {
-      acc[kk] = 0.0;
-      const int k = k0 + kk;
-      if (k <= it) {
-        const T* vk = V + static_cast<size_t>(k) * ld;
+    for (int kk = 0; kk < 8; ++kk) acc[kk] = 0.0;
+    if (full && k0 + 8 <= kend) {
+      // whole 16-byte chunks of 8 existing vectors: every load issued before
+      // any use; the products are added in the element path's order
+      typename Vec16<T>::type raw[8][S::NC];
 #pragma unroll
-        for (int q = 0; q < S::NC; ++q) {
-          double v[S::VW];
-          load_chunk<T, S::VW>(vk, S::row0(a, q), end, vec, v);
+      for (int kk = 0; kk < 8; ++kk)
 #pragma unroll
-          for (int u = 0; u < S::VW; ++u) acc[kk] += wr[q * S::VW + u] * v[u];
+        for (int q = 0; q < S::NC; ++q)
+          raw[kk][q] = load16<T>(V + static_cast<size_t>(k0 + kk) * ld + S::row0(a, q));
+#pragma unroll
+      for (int kk = 0; kk < 8; ++kk)
+#pragma unroll
+        for (int q = 0; q < S::NC; ++q)
+#pragma unroll
+          for (int u = 0; u < S::VW; ++u) acc[kk] += wr[q * S::VW + u] * lane_of<T>(raw[kk][q], u);
+    } else {
+#pragma unroll
+      for (int kk = 0; kk < 8; ++kk) {
+        const int k = k0 + kk;
+        if (k <= it) {
+          const T* vk = V + static_cast<size_t>(k) * ld;
+#pragma unroll
+          for (int q = 0; q < S::NC; ++q) {
+            double v[S::VW];
+            load_chunk<T, S::VW>(vk, S::row0(a, q), end, vec, v);
+#pragma unroll
+            for (int u = 0; u < S::VW; ++u) acc[kk] += wr[q * S::VW + u] * v[u];
+          }
         }
       }
     }
