@@ -515,6 +515,17 @@ void Context::ensure_ilu() {
   }
   for (int p = 0; p < dm.Nc; ++p) l2u[p] = upos[lrows[p]];
   levels_.lpos = upload(allocs_, lpos, stream);
+  {
+    std::vector<int> sptr, sp, sd, su, sr, sq;
+    ilu_steps(dm.Nc, mesh.rowptr.data(), mesh.col.data(), mesh.diag.data(), lrows, sptr, sp, sd, su, sr, sq);
+    levels_.steps.ptr = upload(allocs_, sptr, stream);
+    levels_.steps.p = upload(allocs_, sp, stream);
+    levels_.steps.d = upload(allocs_, sd, stream);
+    levels_.steps.u = upload(allocs_, su, stream);
+    levels_.steps.r = upload(allocs_, sr, stream);
+    levels_.steps.q = upload(allocs_, sq, stream);
+    sync();  // host staging vectors go out of scope
+  }
   levels_.l2u = upload(allocs_, l2u, stream);
   levels_.upos = upload(allocs_, upos, stream);
   // level-ELL copies of the factors for the sweeps

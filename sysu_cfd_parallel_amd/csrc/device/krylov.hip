@@ -45,33 +45,24 @@ __global__ void __launch_bounds__(kBlock) k_jacobi_setup(CsrView<T> A, T* inv) {
   inv[i] = static_cast<T>(d != 0.0 ? 1.0 / d : 1.0);
 }
 
-// ---- ILU(0), natural ordering, one thread per row of a level
-// factor: IKJ elimination a_ik /= u_kk, a_ij -= l_ik u_kj on the pattern
+// ---- ILU(0), natural ordering, one thread per row of a level.  IKJ
+// elimination a_ik /= u_kk, a_ij -= l_ik u_kj on the pattern, from the
+// precomputed steps of each row (its lower entries p in order, the pivot
+// position of each, and the (target, source) slot pairs where row i's pattern
+// after p meets the upper part of row k = col[p], in column order): the
+// arithmetic and its order are those of merging the two sorted rows.
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_ilu_factor(CsrView<T> A, T* __restrict__ lu, const int* __restrict__ rows,
-                                                       int nrows) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nrows) return;
-  const int i = rows[t];
-  const int pd = A.diag[i], pe = A.rowptr[i + 1];
-  for (int p = A.rowptr[i]; p < pd; ++p) {
-    const int k = A.col[p];
-    lu[p] = lu[p] / lu[A.diag[k]];
+__global__ void __launch_bounds__(kBlock) k_ilu_factor(T* __restrict__ lu, IluSteps f, int t0, int nrows) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nrows) return;
+  const int t = t0 + k;
+  const int s0 = f.ptr[t], s1 = f.ptr[t + 1];
+  for (int s = s0; s < s1; ++s) {
+    const int p = f.p[s];
+    lu[p] = lu[p] / lu[f.d[s]];
     const T lik = lu[p];
-    int q = A.diag[k] + 1, r = p + 1;
-    const int qe = A.rowptr[k + 1];
-    while (q < qe && r < pe) {
-      const int cq = A.col[q], cr = A.col[r];
-      if (cq == cr) {
-        lu[r] -= lik * lu[q];
-        ++q;
-        ++r;
-      } else if (cq < cr) {
-        ++q;
-      } else {
-        ++r;
-      }
-    }
+    const int u1 = f.u[s + 1];
+    for (int u = f.u[s]; u < u1; ++u) lu[f.r[u]] -= lik * lu[f.q[u]];
   }
 }
 
@@ -199,9 +190,12 @@ __global__ void __launch_bounds__(kBlock) k_dot2(int n, const T* z, const T* r, 
 template <class T>
 void ilu_factor(const CsrView<T>& A, int nnz, const IluLevels& lv, T* lu, hipStream_t s, const KWork* w = nullptr) {
   S355_HIP(hipMemcpyAsync(lu, A.val, sizeof(T) * static_cast<size_t>(nnz), hipMemcpyDeviceToDevice, s));
+  if (!lv.steps.ptr) throw std::invalid_argument("ILU(0) factorisation without its elimination steps");
   for (int l = 0; l < lv.nl(); ++l) {
     const int nr = lv.lptr[l + 1] - lv.lptr[l];
-    hipLaunchKernelGGL((k_ilu_factor<T>), dim3(grid_for(nr)), dim3(kBlock), 0, s, A, lu, lv.lrows + lv.lptr[l], nr);
+    if (nr == 0) continue;
+    const int b = nr < (kBlock << 8) ? kWave : kBlock;
+    hipLaunchKernelGGL((k_ilu_factor<T>), dim3(grid_for(nr, b)), dim3(b), 0, s, lu, lv.steps, lv.lptr[l], nr);
   }
   if (w && w->ilu_lval && lv.lell.col) {
     const size_t nrow = lv.uptr.back();
@@ -2058,6 +2052,45 @@ void ilu_ell(int n, const int* rowptr, const int* col, const int* diag, const st
         }
       }
     }
+  }
+}
+
+void ilu_steps(int n, const int* rowptr, const int* col, const int* diag, const std::vector<int>& rows,
+               std::vector<int>& ptr, std::vector<int>& p, std::vector<int>& d, std::vector<int>& u,
+               std::vector<int>& r, std::vector<int>& q) {
+  ptr.assign(1, 0);
+  p.clear();
+  d.clear();
+  u.assign(1, 0);
+  r.clear();
+  q.clear();
+  for (int t = 0; t < n; ++t) {
+    const int i = rows[t];
+    const int pd = diag[i], pe = rowptr[i + 1];
+    for (int pp = rowptr[i]; pp < pd; ++pp) {
+      const int k = col[pp];
+      p.push_back(pp);
+      d.push_back(diag[k]);
+      // the merge of row i after pp with the upper part of row k
+      int a = diag[k] + 1, b = pp + 1;
+      const int ae = rowptr[k + 1];
+      while (a < ae && b < pe) {
+        if (col[a] == col[b]) {
+          r.push_back(b);
+          q.push_back(a);
+          ++a;
+          ++b;
+        } else if (col[a] < col[b]) {
+          ++a;
+        } else {
+          ++b;
+        }
+      }
+      if (r.size() > 0x7fffffffUL) throw std::invalid_argument("ILU(0) elimination steps above 32-bit indices");
+      u.push_back(static_cast<int>(r.size()));
+    }
+    if (p.size() > 0x7fffffffUL) throw std::invalid_argument("ILU(0) elimination steps above 32-bit indices");
+    ptr.push_back(static_cast<int>(p.size()));
   }
 }
 

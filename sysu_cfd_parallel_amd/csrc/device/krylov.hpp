@@ -130,6 +130,19 @@ struct IluEll {
   size_t slots = 0;
 };
 
+// The elimination steps of ILU(0) per row, rows in lower-level order: row
+// position t has steps ptr[t] .. ptr[t+1]-1; step s divides the lower entry
+// p[s] by the pivot d[s] (the diagonal of row col[p]) and then applies the
+// slot pairs u[s] .. u[s+1]-1: lu[r] -= l * lu[q].
+struct IluSteps {
+  const int* ptr = nullptr;
+  const int* p = nullptr;
+  const int* d = nullptr;
+  const int* u = nullptr;  // nsteps + 1
+  const int* r = nullptr;
+  const int* q = nullptr;
+};
+
 struct IluLevels {
   std::vector<int> lptr, uptr;  // host: level boundaries into lrows / urows
   const int* lrows = nullptr;   // device: rows grouped by lower level
@@ -138,9 +151,15 @@ struct IluLevels {
   const int* l2u = nullptr;     // device: upper-level position of the row at each lower position
   const int* upos = nullptr;    // device: upper-level position of each row
   IluEll lell, uell;
+  IluSteps steps;
   int nl() const { return static_cast<int>(lptr.size()) - 1; }
   int nu() const { return static_cast<int>(uptr.size()) - 1; }
 };
+
+// Host construction of the elimination steps (rows in the order of rows).
+void ilu_steps(int n, const int* rowptr, const int* col, const int* diag, const std::vector<int>& rows,
+               std::vector<int>& ptr, std::vector<int>& p, std::vector<int>& d, std::vector<int>& u,
+               std::vector<int>& r, std::vector<int>& q);
 
 // Host computation of the level schedule.
 void ilu_levels(int n, const int* rowptr, const int* col, const int* diag, std::vector<int>& lrows,
