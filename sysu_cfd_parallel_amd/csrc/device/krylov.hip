@@ -45,6 +45,12 @@ __global__ void __launch_bounds__(kBlock) k_jacobi_setup(CsrView<T> A, T* inv) {
   inv[i] = static_cast<T>(d != 0.0 ? 1.0 / d : 1.0);
 }
 
+// Workgroup size of a level launch of the ILU(0) factorisation and sweeps: a
+// level below 64 K rows launches one-wave workgroups, so its rows spread over
+// all CUs (each row is a chain of dependent loads, not a share of bandwidth);
+// larger levels keep 256-thread workgroups.
+inline int level_block(int nrows) { return nrows < (kBlock << 8) ? kWave : kBlock; }
+
 // ---- ILU(0), natural ordering, one thread per row of a level.  IKJ
 // elimination a_ik /= u_kk, a_ij -= l_ik u_kj on the pattern, from the
 // precomputed steps of each row (its lower entries p in order, the pivot
@@ -194,7 +200,7 @@ void ilu_factor(const CsrView<T>& A, int nnz, const IluLevels& lv, T* lu, hipStr
   for (int l = 0; l < lv.nl(); ++l) {
     const int nr = lv.lptr[l + 1] - lv.lptr[l];
     if (nr == 0) continue;
-    const int b = nr < (kBlock << 8) ? kWave : kBlock;
+    const int b = level_block(nr);
     hipLaunchKernelGGL((k_ilu_factor<T>), dim3(grid_for(nr, b)), dim3(b), 0, s, lu, lv.steps, lv.lptr[l], nr);
   }
   if (w && w->ilu_lval && lv.lell.col) {
@@ -234,16 +240,12 @@ void ilu_apply(const CsrView<T>& A, const IluLevels& lv, const T* lu, const T* r
   const int n = lv.lptr.back();
   const int gn = grid_for(n);
   // r into the lower order, the lower sweep, its result into the upper
-  // order, the upper sweep, z back into the natural order.  A level below
-  // 64 K rows launches one-wave workgroups, so its rows spread over all CUs
-  // (each row is a chain of dependent loads, not a share of bandwidth);
-  // larger levels keep 256-thread workgroups
-  auto sweep_block = [](int nr) { return nr < (kBlock << 8) ? kWave : kBlock; };
+  // order, the upper sweep, z back into the natural order
   hipLaunchKernelGGL((k_scatter<T, 0>), dim3(gn), dim3(kBlock), 0, s, n, lv.lpos, r, zl, st);
   for (int l = 0; l < lv.nl(); ++l) {
     const int p0 = lv.lptr[l], nr = lv.lptr[l + 1] - p0;
     if (nr == 0) continue;
-    const int b = sweep_block(nr);
+    const int b = level_block(nr);
     hipLaunchKernelGGL((k_ilu_sweep_ell<T, false>), dim3(grid_for(nr, b)), dim3(b), 0, s, lv.lell.len + p0,
                        lv.lell.col, lval, nullptr, lv.lell.off[l], p0, nr, width(lv.lell, l, nr), zl, st);
   }
@@ -251,7 +253,7 @@ void ilu_apply(const CsrView<T>& A, const IluLevels& lv, const T* lu, const T* r
   for (int l = 0; l < lv.nu(); ++l) {
     const int p0 = lv.uptr[l], nr = lv.uptr[l + 1] - p0;
     if (nr == 0) continue;
-    const int b = sweep_block(nr);
+    const int b = level_block(nr);
     hipLaunchKernelGGL((k_ilu_sweep_ell<T, true>), dim3(grid_for(nr, b)), dim3(b), 0, s, lv.uell.len + p0,
                        lv.uell.col, uval, dval + p0, lv.uell.off[l], p0, nr, width(lv.uell, l, nr), zu, st);
   }
