@@ -150,15 +150,39 @@ __global__ void __launch_bounds__(kBlock) k_scatter(int n, const int* __restrict
   dst[p] = v;
 }
 
-// gather: dst[q] = src[idx[q]]
-template <class T, int Tag>
+// gather: dst[q] = src[idx[q]].  With Dot (CG with ILU(0): one case, blocks
+// of kBlock threads) also the partial sums (dst, dst) and (dst, r) over the
+// block's kBlock rows b kBlock .. into partial[2b], partial[2b + 1] (block
+// sums in the fixed wave order), which CG's reductions then finish.
+template <class T, int Tag, bool Dot = false>
 __global__ void __launch_bounds__(kBlock) k_gather_at(int n, const int* __restrict__ idx, const T* __restrict__ src,
-                                                      T* __restrict__ dst, const KState* st) {
-  const int q = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  const int p = idx[q];
-  if (st && st->reason) return;
-  dst[q] = src[p];
+                                                      T* __restrict__ dst, const KState* st,
+                                                      const T* __restrict__ r = nullptr, double* partial = nullptr) {
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int q = b * blockDim.x + threadIdx.x;
+  if constexpr (!Dot) {
+    if (q >= n) return;
+    const int p = idx[q];
+    if (st && st->reason) return;
+    dst[q] = src[p];
+  } else {
+    __shared__ double lds[kBlock / kWave];
+    if (st && st->reason) return;  // one case: the whole grid leaves together
+    double zz = 0.0, zr = 0.0;
+    if (q < n) {
+      const T v = src[idx[q]];
+      dst[q] = v;
+      const double zv = static_cast<double>(v);
+      zz = zv * zv;
+      zr = zv * static_cast<double>(r[q]);
+    }
+    zz = block_sum(zz, lds);
+    zr = block_sum(zr, lds);
+    if (threadIdx.x == 0) {
+      partial[2 * b] = zz;
+      partial[2 * b + 1] = zr;
+    }
+  }
 }
 
 // out[q] = lu[src[q]] (0 where src[q] < 0)
@@ -169,26 +193,6 @@ __global__ void __launch_bounds__(kBlock) k_gather(const int* __restrict__ src, 
   if (q >= n) return;
   const int p = src[q];
   out[q] = p >= 0 ? lu[p] : T(0);
-}
-
-// partial (z,z) and (z,r) for CG with a non-pointwise preconditioner
-template <class T>
-__global__ void __launch_bounds__(kBlock) k_dot2(int n, const T* z, const T* r, double* partial, const KState* st) {
-  if (st && st->reason) return;
-  __shared__ double lds[kBlock / kWave];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double zz = 0.0, zr = 0.0;
-  if (i < n) {
-    const double zv = static_cast<double>(z[i]);
-    zz = zv * zv;
-    zr = zv * static_cast<double>(r[i]);
-  }
-  zz = block_sum(zz, lds);
-  zr = block_sum(zr, lds);
-  if (threadIdx.x == 0) {
-    partial[2 * blockIdx.x] = zz;
-    partial[2 * blockIdx.x + 1] = zr;
-  }
 }
 
 // ILU(0) factors of A into lu (CSR), then their level-ELL copies into the
@@ -217,15 +221,18 @@ void ilu_factor(const CsrView<T>& A, int nnz, const IluLevels& lv, T* lu, hipStr
 
 // z = (LU)^-1 r ; r and z distinct.  With a graph cache the level sweeps
 // (one launch per level) are captured once per (factor, r, z) and replayed.
+// With dot (CG) the last move also leaves the partial sums of (z, z) and
+// (z, r) per kBlock rows there (see k_gather_at).
 template <class T>
 void ilu_apply(const CsrView<T>& A, const IluLevels& lv, const T* lu, const T* r, T* z, const KState* st,
-               hipStream_t s, GraphCache* graphs = nullptr, const KWork* w = nullptr) {
+               hipStream_t s, GraphCache* graphs = nullptr, const KWork* w = nullptr, double* dot = nullptr) {
   if (graphs) {
     const std::vector<uintptr_t> key = {0x11u, reinterpret_cast<uintptr_t>(lu), reinterpret_cast<uintptr_t>(r),
                                         reinterpret_cast<uintptr_t>(z), reinterpret_cast<uintptr_t>(st),
                                         reinterpret_cast<uintptr_t>(lv.lrows), sizeof(T),
-                                        reinterpret_cast<uintptr_t>(w ? w->ilu_lval : nullptr)};
-    hipGraphExec_t g = graphs->get(key, s, [&] { ilu_apply(A, lv, lu, r, z, st, s, nullptr, w); });
+                                        reinterpret_cast<uintptr_t>(w ? w->ilu_lval : nullptr),
+                                        reinterpret_cast<uintptr_t>(dot)};
+    hipGraphExec_t g = graphs->get(key, s, [&] { ilu_apply(A, lv, lu, r, z, st, s, nullptr, w, dot); });
     S355_HIP(hipGraphLaunch(g, s));
     return;
   }
@@ -257,7 +264,10 @@ void ilu_apply(const CsrView<T>& A, const IluLevels& lv, const T* lu, const T* r
     hipLaunchKernelGGL((k_ilu_sweep_ell<T, true>), dim3(grid_for(nr, b)), dim3(b), 0, s, lv.uell.len + p0,
                        lv.uell.col, uval, dval + p0, lv.uell.off[l], p0, nr, width(lv.uell, l, nr), zu, st);
   }
-  hipLaunchKernelGGL((k_gather_at<T, 2>), dim3(gn), dim3(kBlock), 0, s, n, lv.upos, zu, z, st);
+  if (dot)
+    hipLaunchKernelGGL((k_gather_at<T, 2, true>), dim3(gn), dim3(kBlock), 0, s, n, lv.upos, zu, z, st, r, dot);
+  else
+    hipLaunchKernelGGL((k_gather_at<T, 2>), dim3(gn), dim3(kBlock), 0, s, n, lv.upos, zu, z, st, nullptr, nullptr);
   S355_LAUNCH_CHECK();
 }
 
@@ -1492,10 +1502,7 @@ KResult cg(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, bool z
     hipLaunchKernelGGL(k_snorm_final, dim3(1), dim3(kBlock), 0, s, w.partial, t1, w.st);
   }
   hipLaunchKernelGGL((k_cg_start<T>), dim3(g1), dim3(kBlock), 0, s, A, b, x, inv, r, z, w.partial, zero_guess ? 1 : 0);
-  if (lu) {
-    ilu_apply(A, *w.ilu, lu, r, z, w.st, s, w.graphs, &w);
-    hipLaunchKernelGGL((k_dot2<T>), dim3(g1), dim3(kBlock), 0, s, n, z, r, w.partial, w.st);
-  }
+  if (lu) ilu_apply(A, *w.ilu, lu, r, z, w.st, s, w.graphs, &w, w.partial);  // with the (z, z), (z, r) partials
   hipLaunchKernelGGL(k_cg_start_final, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st, zero_guess ? 1 : 0);
   S355_LAUNCH_CHECK();
   const int check_every = n >= (1 << 20) ? 4 : 8;
@@ -1506,10 +1513,7 @@ KResult cg(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, bool z
     hipLaunchKernelGGL((k_cg_spmv_dot<T>), dim3(g1), dim3(kBlock), 0, s, A, p, wv, w.partial, w.st, i);
     hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st, i);
     hipLaunchKernelGGL((k_cg_update<T>), dim3(g1), dim3(kBlock), 0, s, n, x, r, z, p, wv, inv, w.partial, w.st);
-    if (lu) {
-      ilu_apply(A, *w.ilu, lu, r, z, w.st, s, w.graphs, &w);
-      hipLaunchKernelGGL((k_dot2<T>), dim3(g1), dim3(kBlock), 0, s, n, z, r, w.partial, w.st);
-    }
+    if (lu) ilu_apply(A, *w.ilu, lu, r, z, w.st, s, w.graphs, &w, w.partial);
     hipLaunchKernelGGL(k_cg_final, dim3(1), dim3(kBlock), 0, s, w.partial, g1, w.st);
     S355_LAUNCH_CHECK();
     if ((i + 1) % check_every == 0) {
