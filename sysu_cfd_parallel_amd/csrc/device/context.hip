@@ -900,6 +900,7 @@ void Context::set_kernel_shapes(int precision, int lanes, int nt, int rows, int 
   w.sell_v = kSellDefault;
   for (int v = 0; v < kSellVariants; ++v)
     if (sigma == kSellSigma[v]) w.sell_v = v;
+  w.sell2 = -1;    // FP32 SELL: two slices per wave above 2^20 rows
   w.tuned = true;  // the autotuner leaves explicit shapes alone
 }
 

@@ -402,16 +402,11 @@ __device__ __forceinline__ void sell_row(const SellView<T>& A, const Tiles& t, c
   y[row] = static_cast<T>(z * scl);
 }
 
+// one row of slice sl by the slice's width (wave-uniform)
 template <class T, bool NT, bool PC>
-__global__ void __launch_bounds__(kBlock) k_spmv_sell(SellView<T> A, Tiles t, const T* __restrict__ x,
-                                                      const T* __restrict__ inv, T* __restrict__ y, const KState* st,
-                                                      int it) {
-  const int pos = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (pos >= A.n) return;
-  // a wave is one slice (64 consecutive positions), so the slice index and
-  // width are wave-uniform and the slice bounds load through the scalar cache
-  const int sl = __builtin_amdgcn_readfirstlane(pos >> 6);
-  const int qa = A.ptr[sl], q1 = A.ptr[sl + 1];
+__device__ __forceinline__ void sell_row_any(const SellView<T>& A, const Tiles& t, const T* __restrict__ x,
+                                             const T* __restrict__ inv, T* __restrict__ y, const KState* st, int it,
+                                             int sl, int pos, int qa, int q1) {
   switch ((q1 - qa) >> 6) {
     case 1: sell_row<T, NT, PC, 1>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
     case 2: sell_row<T, NT, PC, 2>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
@@ -422,6 +417,108 @@ __global__ void __launch_bounds__(kBlock) k_spmv_sell(SellView<T> A, Tiles t, co
     case 7: sell_row<T, NT, PC, 7>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
     case 8: sell_row<T, NT, PC, 8>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
     default: sell_row<T, NT, PC, 0>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
+  }
+}
+
+template <class T, bool NT, bool PC>
+__global__ void __launch_bounds__(kBlock) k_spmv_sell(SellView<T> A, Tiles t, const T* __restrict__ x,
+                                                      const T* __restrict__ inv, T* __restrict__ y, const KState* st,
+                                                      int it) {
+  const int pos = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (pos >= A.n) return;
+  // a wave is one slice (64 consecutive positions), so the slice index and
+  // width are wave-uniform and the slice bounds load through the scalar cache
+  const int sl = __builtin_amdgcn_readfirstlane(pos >> 6);
+  sell_row_any<T, NT, PC>(A, t, x, inv, y, st, it, sl, pos, A.ptr[sl], A.ptr[sl + 1]);
+}
+
+// Two adjacent slices per wave, one row of each per lane (one case, FP32 on
+// large systems): twice the loads in flight per wave at the same occupancy.
+// Both rows run the one-slice arithmetic: W is the wider slice's width, the
+// narrower slice's slots past its width reload the lane's first slot and are
+// never added (adjacent slices of a sorted window rarely differ in width).
+template <class T, bool NT, bool PC, int W>
+__device__ __forceinline__ void sell_rows2(const SellView<T>& A, const T* __restrict__ x, const T* __restrict__ inv,
+                                           T* __restrict__ y, const KState* st, int it, int sl, int lane, int qa0,
+                                           int qa1, int q11) {
+  const int w0 = (qa1 - qa0) >> 6, w1 = (q11 - qa1) >> 6;
+  const int pos0 = sl * 64 + lane, pos1 = pos0 + 64;
+  const bool ok1 = pos1 < A.n;
+  const int row0 = A.row[pos0];
+  const int row1 = A.row[ok1 ? pos1 : pos0];
+  T v0[W], v1[W], x0[W], x1[W];
+  int c0[W], c1[W];
+#pragma unroll
+  for (int u = 0; u < W; ++u) {
+    const int qa = qa0 + lane + (u < w0 ? 64 * u : 0);
+    const int qb = qa1 + lane + (u < w1 ? 64 * u : 0);
+    v0[u] = ld_stream<T, NT>(A.val + qa);
+    c0[u] = ld_stream<int, NT>(A.col + qa);
+    v1[u] = ld_stream<T, NT>(A.val + qb);
+    c1[u] = ld_stream<int, NT>(A.col + qb);
+  }
+  if (st->reason) return;
+  const double scl = st->scale[it];
+  T i0 = T(1), i1 = T(1);
+  if constexpr (PC) {
+    i0 = inv[row0];
+    i1 = inv[row1];
+  }
+#pragma unroll
+  for (int u = 0; u < W; ++u) {
+    x0[u] = x[c0[u]];
+    x1[u] = x[c1[u]];
+  }
+  T a0 = T(0), a1 = T(0);
+#pragma unroll
+  for (int u = 0; u < W; ++u) {
+    if (u < w0) a0 += v0[u] * x0[u];
+    if (u < w1) a1 += v1[u] * x1[u];
+  }
+  double z0 = static_cast<double>(a0), z1 = static_cast<double>(a1);
+  if constexpr (PC) {
+    z0 *= static_cast<double>(i0);
+    z1 *= static_cast<double>(i1);
+  }
+  y[row0] = static_cast<T>(z0 * scl);
+  if (ok1) y[row1] = static_cast<T>(z1 * scl);
+}
+
+template <class T, bool NT, bool PC>
+__global__ void __launch_bounds__(kBlock) k_spmv_sell2(SellView<T> A, Tiles t, const T* __restrict__ x,
+                                                       const T* __restrict__ inv, T* __restrict__ y, const KState* st,
+                                                       int it) {
+  // wave w takes slices 2w and 2w + 1
+  const int wv = xcd_block(blockIdx.x, gridDim.x) * (kBlock / kWave) + static_cast<int>(threadIdx.x) / kWave;
+  const int sl = __builtin_amdgcn_readfirstlane(2 * wv), lane = threadIdx.x & (kWave - 1);
+  const int nsl = (A.n + 63) >> 6;
+  if (sl >= nsl) return;
+  if (sl + 1 >= nsl) {
+    // an odd last slice: the one-slice body
+    const int pos = sl * 64 + lane;
+    if (pos < A.n) sell_row_any<T, NT, PC>(A, t, x, inv, y, st, it, sl, pos, A.ptr[sl], A.ptr[sl + 1]);
+    return;
+  }
+  const int qa0 = A.ptr[sl], qa1 = A.ptr[sl + 1], q11 = A.ptr[sl + 2];
+  const int wmax = max((qa1 - qa0) >> 6, (q11 - qa1) >> 6);
+  switch (wmax) {
+    case 1: sell_rows2<T, NT, PC, 1>(A, x, inv, y, st, it, sl, lane, qa0, qa1, q11); break;
+    case 2: sell_rows2<T, NT, PC, 2>(A, x, inv, y, st, it, sl, lane, qa0, qa1, q11); break;
+    case 3: sell_rows2<T, NT, PC, 3>(A, x, inv, y, st, it, sl, lane, qa0, qa1, q11); break;
+    case 4: sell_rows2<T, NT, PC, 4>(A, x, inv, y, st, it, sl, lane, qa0, qa1, q11); break;
+    case 5: sell_rows2<T, NT, PC, 5>(A, x, inv, y, st, it, sl, lane, qa0, qa1, q11); break;
+    case 6: sell_rows2<T, NT, PC, 6>(A, x, inv, y, st, it, sl, lane, qa0, qa1, q11); break;
+    case 7: sell_rows2<T, NT, PC, 7>(A, x, inv, y, st, it, sl, lane, qa0, qa1, q11); break;
+    case 8: sell_rows2<T, NT, PC, 8>(A, x, inv, y, st, it, sl, lane, qa0, qa1, q11); break;
+    default: {
+      // a slice wider than one batch: each slice with the one-slice body
+      // for its own width
+      for (int k = 0; k < 2; ++k) {
+        const int s2 = sl + k, pos = s2 * 64 + lane;
+        if (pos < A.n) sell_row_any<T, NT, PC>(A, t, x, inv, y, st, it, s2, pos, A.ptr[s2], A.ptr[s2 + 1]);
+      }
+      break;
+    }
   }
 }
 
@@ -460,6 +557,25 @@ void launch_spmv(int L, int nt, const CsrView<T>& A, const SellView<T>& sell, co
   switch (L) {
     case 0: {
       if (!sell.ptr) throw std::logic_error("SELL-64 SpMV without a SELL copy");
+      if constexpr (std::is_same<T, float>::value) {
+        if (sell.two) {
+          // two slices per wave: kBlock / kWave waves per block
+          const int nsl = (A.n + 63) / 64;
+          const int g2 = grid_for((nsl + 1) / 2, kBlock / kWave);
+          if (nt && inv)
+            hipLaunchKernelGGL((k_spmv_sell2<T, true, true>), dim3(g2), dim3(kBlock), 0, s, sell, t, x, inv, y, st, it);
+          else if (nt)
+            hipLaunchKernelGGL((k_spmv_sell2<T, true, false>), dim3(g2), dim3(kBlock), 0, s, sell, t, x, inv, y, st,
+                               it);
+          else if (inv)
+            hipLaunchKernelGGL((k_spmv_sell2<T, false, true>), dim3(g2), dim3(kBlock), 0, s, sell, t, x, inv, y, st,
+                               it);
+          else
+            hipLaunchKernelGGL((k_spmv_sell2<T, false, false>), dim3(g2), dim3(kBlock), 0, s, sell, t, x, inv, y, st,
+                               it);
+          break;
+        }
+      }
       const int g = grid_for(A.n);
       if (nt && inv)
         hipLaunchKernelGGL((k_spmv_sell<T, true, true>), dim3(g), dim3(kBlock), 0, s, sell, t, x, inv, y, st, it);
@@ -490,6 +606,7 @@ SellView<T> sell_of(const KWork& w, int n, int variant = -1) {
     v.col = w.sell[sv].col;
     v.val = static_cast<const T*>(w.sell_vals[sv]);
     v.scase = w.sell[sv].scase;
+    v.two = std::is_same<T, float>::value && !v.scase && (w.sell2 > 0 || (w.sell2 < 0 && n >= (1 << 20))) ? 1 : 0;
   }
   return v;
 }
@@ -1658,7 +1775,7 @@ struct TuneKey {
 };
 std::mutex g_tune_mu;
 struct TuneResult {
-  int lanes, nt, rows, sell_v;
+  int lanes, nt, rows, sell_v, sell2;
   TuneReport report;
 };
 std::map<TuneKey, TuneResult> g_tuned;
@@ -1692,6 +1809,7 @@ void autotune(const CsrView<T>& A, KWork& w, int m, hipStream_t s) {
       w.nt = it->second.nt;
       w.rows = it->second.rows;
       w.sell_v = it->second.sell_v;
+      w.sell2 = it->second.sell2;
       w.report = it->second.report;
       w.tuned = true;
       return;
@@ -1737,6 +1855,23 @@ void autotune(const CsrView<T>& A, KWork& w, int m, hipStream_t s) {
         }
       }
     }
+  // FP32 SELL on one case: the best window with one and with two slices per
+  // wave (twice the loads in flight per wave; it pays where the product is
+  // bound by its loads' latency, not where it already streams at the copy rate)
+  if constexpr (std::is_same<T, float>::value) {
+    if (best_l == 0 && w.ncase == 1 && n >= (1 << 18)) {
+      SellView<T> sell = sell_of<T>(w, n, best_sv);
+      if (sell.ptr && !sell.scase) {
+        double ms2[2];
+        for (int two = 0; two < 2; ++two) {
+          sell.two = two;
+          ms2[two] = time_ms([&] { launch_spmv<T>(0, best_nt, A, sell, w.tiles[0], V, none, V + ld, w.st, 0, s); },
+                             5, s);
+        }
+        w.sell2 = ms2[1] < ms2[0] ? 1 : 0;
+      }
+    }
+  }
   // the Gram-Schmidt launches of three Arnoldi steps across the cycle (a
   // quarter, half and three quarters of the restart length) at every tile
   // size; the tile count also sets the work of the final reduction
@@ -1769,7 +1904,7 @@ void autotune(const CsrView<T>& A, KWork& w, int m, hipStream_t s) {
   w.report = rep;
   w.tuned = true;
   std::lock_guard<std::mutex> g(g_tune_mu);
-  g_tuned[key] = TuneResult{best_l, best_nt, best_r, best_sv, rep};
+  g_tuned[key] = TuneResult{best_l, best_nt, best_r, best_sv, w.sell2, rep};
 }
 
 int pick_rows(int n, size_t elem) { return pick_r(n, elem); }

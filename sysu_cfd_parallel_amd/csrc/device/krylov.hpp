@@ -76,6 +76,7 @@ struct SellView {
   const int* col = nullptr;
   const T* val = nullptr;
   const int* scase = nullptr;  // batches: case of each slice, -1 when its rows span two cases
+  int two = 0;                 // FP32, one case: two slices per wave
 };
 
 // SELL windows the autotuner chooses from: a short window keeps the rows of
@@ -236,6 +237,7 @@ struct KWork {
   int lanes = 8;            // CSR SpMV lanes per row (1, 2, 4, 8, 16); 0 = SELL-64 SpMV
   int nt = 0;               // SpMV loads the matrix non-temporally
   int sell_v = kSellDefault;  // SELL window in use (index into kSellSigma) when lanes == 0
+  int sell2 = -1;           // FP32 SELL with two slices per wave: 1 / 0 as tuned, -1 above 2^20 rows
   // SELL-64 copies of the matrix being solved: patterns, and values of type T
   SellPat sell[kSellVariants];
   void* sell_vals[kSellVariants] = {nullptr, nullptr, nullptr};

@@ -124,6 +124,26 @@ def test_sell_spmv_equals_single_lane_csr(prec):
         assert np.array_equal(out[0][1], o[1])
 
 
+def test_two_slice_sell_equals_single_lane_csr():
+    # above 2^20 rows the FP32 SELL product takes two slices per wave (one row
+    # of each per lane); every row still sums its entries in CSR order
+    case = s.channel(112, 56, 112, jitter=0.15, split=0.12, seed=8, re=600.0)
+    p = base(s.MIXED, restart=50)
+    for k in range(3):
+        p.ksp[k].maxit = 20000
+    out = []
+    for lanes, sigma in ((1, 0), (0, 512)):
+        d = s.Solver(case, p)
+        assert d.sizes.n_cell >= 1 << 20
+        d.set_kernel_shapes(1, lanes, sigma=sigma)
+        h, _ = d.iterate(2)
+        out.append(({n: d.get(n) for n in FIELDS}, h))
+        d.close()
+    for n in FIELDS:
+        assert np.array_equal(out[0][0][n], out[1][0][n]), n
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 @pytest.mark.parametrize("lanes,sigma", [(0, 64), (0, 4096), (2, 0), (16, 0)])
 def test_every_spmv_shape_matches_the_oracle(lanes, sigma):
     from oracle.pyoracle import Oracle
