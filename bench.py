@@ -43,6 +43,7 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE passes scaled by a 1 GiB copy (profiles/hbm_traffic_channel_6M.md)
 MEASURED_TRAFFIC = {
     ("k_spmv_sell<float, true, true>", 5995990): 447.1e6,
+    ("k_spmv_sell2<float, true, true>", 5995990): 441.6e6,
 }
 METRIC = "SIMPLE iters/s @4M cells, FP64 vs mixed (<=1e-6 rel err); SpMV % HBM peak"
 STAGES = ["total", "build_velocity", "solve_velocity", "build_pressure", "solve_pressure", "other"]
@@ -262,7 +263,8 @@ def spmv_bytes(nnz: int, nc: int, b_val: int, b_x: int, jacobi: bool = True) -> 
 def spmv_kernel_name(shape: dict, elem: str, jacobi: bool = True) -> str:
     tf = lambda v: "true" if v else "false"  # noqa: E731
     if shape["lanes"] == 0:
-        return f"k_spmv_sell<{elem}, {tf(shape['nt'])}, {tf(jacobi)}>"
+        two = "2" if shape.get("slices") == 2 else ""  # two slices per wave
+        return f"k_spmv_sell{two}<{elem}, {tf(shape['nt'])}, {tf(jacobi)}>"
     return f"k_spmv_pc<{elem}, {shape['lanes']}, {tf(shape['nt'])}>"
 
 

@@ -360,8 +360,10 @@ int s355_copy_time(s355_ctx* ctx, size_t bytes, int repeats, double* ms_per_call
  * matrix non-temporally, rows[k] rows per thread of the multi-dot /
  * Gram-Schmidt / update kernels, sigma[k] the row window the SELL-64 copy
  * is sorted in (used when lanes[k] is 0), tuned[k] 1 when they came from
- * the autotuner.  No reference counterpart. */
-int s355_kernel_shapes(s355_ctx* ctx, int lanes[2], int nt[2], int rows[2], int sigma[2], int tuned[2]);
+ * the autotuner, slices[k] the SELL-64 slices each wave takes (1, or 2 for
+ * the FP32 product on one case; 0 with CSR).  No reference counterpart. */
+int s355_kernel_shapes(s355_ctx* ctx, int lanes[2], int nt[2], int rows[2], int sigma[2], int tuned[2],
+                       int slices[2]);
 
 /* Times the production SpMV launches inside the GMRES solves of one
  * precision (0 FP64, 1 FP32): every 4th Arnoldi SpMV is bracketed by HIP

@@ -485,9 +485,10 @@ class Solver:
     def kernel_shapes(self) -> dict:
         """SpMV lanes per row (0 = SELL-64) / non-temporal loads / SELL
         sorting window and Gram-Schmidt rows per thread, FP64 and FP32."""
-        lanes, nt, rows, sigma, tuned = ((C.c_int * 2)() for _ in range(5))
-        _check(self.lib, self.lib.s355_kernel_shapes(self.ctx, lanes, nt, rows, sigma, tuned), "kernel_shapes")
-        return {p: {"lanes": lanes[k], "nt": nt[k], "rows": rows[k], "sigma": sigma[k], "tuned": bool(tuned[k])}
+        lanes, nt, rows, sigma, tuned, slices = ((C.c_int * 2)() for _ in range(6))
+        _check(self.lib, self.lib.s355_kernel_shapes(self.ctx, lanes, nt, rows, sigma, tuned, slices), "kernel_shapes")
+        return {p: {"lanes": lanes[k], "nt": nt[k], "rows": rows[k], "sigma": sigma[k], "tuned": bool(tuned[k]),
+                    "slices": slices[k]}
                 for k, p in enumerate(("fp64", "fp32"))}
 
     def tune_report(self, precision: int) -> dict:

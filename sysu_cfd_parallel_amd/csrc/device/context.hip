@@ -904,7 +904,7 @@ void Context::set_kernel_shapes(int precision, int lanes, int nt, int rows, int 
   w.tuned = true;  // the autotuner leaves explicit shapes alone
 }
 
-void Context::kernel_shapes(int lanes[2], int nt[2], int rows[2], int sigma[2], int tuned[2]) {
+void Context::kernel_shapes(int lanes[2], int nt[2], int rows[2], int sigma[2], int tuned[2], int slices[2]) {
   const KWork* w[2] = {&kwork_, &kwork32_};
   const size_t elem[2] = {sizeof(double), sizeof(float)};
   for (int k = 0; k < 2; ++k) {
@@ -913,6 +913,9 @@ void Context::kernel_shapes(int lanes[2], int nt[2], int rows[2], int sigma[2], 
     rows[k] = w[k]->rows ? w[k]->rows : pick_rows(dm.Nc, elem[k]);
     sigma[k] = kSellSigma[w[k]->sell_v];
     tuned[k] = w[k]->tuned ? 1 : 0;
+    // the rule of the SELL launch: FP32 on one case, as tuned or above 2^20 rows
+    const bool two = k == 1 && ncase == 1 && (w[k]->sell2 > 0 || (w[k]->sell2 < 0 && dm.Nc >= (1 << 20)));
+    slices[k] = lanes[k] != 0 ? 0 : (two ? 2 : 1);
   }
 }
 

@@ -85,7 +85,7 @@ class Context {
   // benchmark helper
   double time_spmv(int which, int precision, int repeats);
   double time_copy(size_t bytes, int repeats);
-  void kernel_shapes(int lanes[2], int nt[2], int rows[2], int sigma[2], int tuned[2]);
+  void kernel_shapes(int lanes[2], int nt[2], int rows[2], int sigma[2], int tuned[2], int slices[2]);
   void set_kernel_shapes(int precision, int lanes, int nt, int rows, int sigma);
   // mode 1: start timing the Arnoldi SpMV launches of one precision (reset),
   // 0: read, -1: read and stop; returns the mean ms and the launches timed
