@@ -147,6 +147,11 @@ enum s355_field {
    * previous time level (Unp ...) and the one before (Unp2 ...) */
   S355_F_UNP, S355_F_VNP, S355_F_WNP, S355_F_TNP, S355_F_TEP, S355_F_EDP,
   S355_F_UNP2, S355_F_VNP2, S355_F_WNP2, S355_F_TNP2, S355_F_TEP2, S355_F_EDP2,
+  /* per cell, n = Ncel: the work arrays of UpdateTurKEpsilon
+   * (navier_scalar.cpp:20-31): Prod, TESource, EDSource, ApTE, ApED, VisTE,
+   * VisED, as the k-epsilon pieces below leave them */
+  S355_F_KE_PROD, S355_F_KE_TESRC, S355_F_KE_EDSRC, S355_F_KE_APTE, S355_F_KE_APED, S355_F_KE_VISTE,
+  S355_F_KE_VISED,
   S355_F_COUNT
 };
 
@@ -375,7 +380,9 @@ int s355_spmv_profile(s355_ctx* ctx, int precision, int mode, double* avg_ms, in
 
 /* Sets the kernel shapes of one precision's GMRES solves explicitly (the
  * autotuner then leaves them alone): lanes 1, 2, 4, 8, 16 (CSR SpMV) or 0
- * (SELL-64 SpMV), nt 0/1, rows 1, 2, 4, 8, 16 or 0 (by size), sigma the
+ * (SELL-64 SpMV; FP32 on one case takes two slices per wave above 2^20 rows),
+ * -1 (SELL-64, one slice per wave) or -2 (SELL-64, two slices per wave; FP32
+ * on one case only), nt 0/1, rows 1, 2, 4, 8, 16 or 0 (by size), sigma the
  * window of rows the SELL-64 copy sorts by length: 64, 512, 4096 or 0 (4096).
  * No reference counterpart. */
 int s355_set_kernel_shapes(s355_ctx* ctx, int precision, int lanes, int nt, int rows, int sigma);

@@ -250,6 +250,42 @@ int orc_update_tur_kepsilon(void* h, s355_solve_info info[2]) {
   });
 }
 
+// z = M^-1 r for a caller's CSR matrix (diag: the diagonal's position in
+// each row) with the Jacobi or ILU(0) preconditioner of the Krylov solves
+int orc_pc_apply(int n, const int* rowptr, const int* col, const int* diag, const double* val, int pc,
+                 const double* r, double* z) {
+  return guard([&] {
+    if (n < 0 || !rowptr || !col || !diag || !val || !r || !z) throw std::invalid_argument("pc_apply: bad arguments");
+    orc::Csr A;
+    A.rowptr.assign(rowptr, rowptr + n + 1);
+    A.col.assign(col, col + rowptr[n]);
+    A.diag.assign(diag, diag + n);
+    std::vector<double> v(val, val + rowptr[n]), rr(r, r + n), zz;
+    orc::pc_apply(A, v, pc, rr, zz);
+    std::memcpy(z, zz.data(), sizeof(double) * static_cast<size_t>(n));
+    return S355_OK;
+  });
+}
+
+// the pieces of UpdateTurKEpsilon, one entry each as in the device ABI
+int orc_kepsilon_sources(void* h) { return guard([&] { O(h)->KEpsilonSources(); return S355_OK; }); }
+int orc_bc_kepsilon(void* h) { return guard([&] { O(h)->KEpsilonBC(); return S355_OK; }); }
+int orc_kepsilon_time_terms(void* h) { return guard([&] { O(h)->KEpsilonTimeTerms(); return S355_OK; }); }
+int orc_build_scalar_matrix(void* h, int isca) {
+  return guard([&] { O(h)->BuildKEpsilonMatrix(isca); return S355_OK; });
+}
+int orc_eps_wall_rows(void* h) { return guard([&] { O(h)->EpsWallRows(); return S355_OK; }); }
+int orc_kepsilon_post_k(void* h) { return guard([&] { O(h)->KEpsilonPostK(); return S355_OK; }); }
+int orc_kepsilon_post_eps(void* h) { return guard([&] { O(h)->KEpsilonPostEps(); return S355_OK; }); }
+int orc_solve_scalar(void* h, int isca, double rtol, int maxit, s355_solve_info* info) {
+  return guard([&] {
+    if (isca < 1 || isca > 3) throw std::invalid_argument("solve_scalar: isca 1, 2 or 3");
+    int rc = O(h)->SolveScalar(isca, rtol, maxit);
+    fill_info(O(h)->last[isca == 1 ? 6 : isca + 2], info);
+    return rc;
+  });
+}
+
 int orc_bc_temperature(void* h) { return guard([&] { O(h)->SetBCTemperature(); return S355_OK; }); }
 
 int orc_update_energy(void* h, s355_solve_info* info) {

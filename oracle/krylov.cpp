@@ -364,4 +364,12 @@ SolveResult ksp_solve(const Csr& A, const std::vector<double>& val, const std::v
   return gmres(A, val, b, x, o, zero_guess);
 }
 
+void pc_apply(const Csr& A, const std::vector<double>& val, int pc, const std::vector<double>& r,
+              std::vector<double>& z) {
+  Pc m;
+  m.setup(A, val, pc);
+  z.assign(r.size(), 0.0);
+  m.apply(r, z);
+}
+
 }  // namespace orc

@@ -121,6 +121,17 @@ class Oracle {
                          const std::vector<double>& source, const std::vector<double>& app_in);
   void SetBCKEpsilon(std::vector<double>& tes, std::vector<double>& eds, std::vector<double>& apte,
                      std::vector<double>& aped, std::vector<double>& prod);
+  // UpdateTurKEpsilon in its pieces, on the member work arrays below
+  void KEpsilonSources();           // navier_scalar.cpp:33-64
+  void KEpsilonBC();                // SetBCKEpsilon on the work arrays (line 67)
+  void KEpsilonTimeTerms();         // 71-90
+  void BuildKEpsilonMatrix(int isca);  // 2: k (93), 3: epsilon (125)
+  void EpsWallRows();               // 127-154
+  void KEpsilonPostK();             // 105-121: clip k, epsilon sources
+  void KEpsilonPostEps();           // 165-178: clip epsilon, mu_t
+  int SolveScalar(int isca, double rtol, int maxit);  // solveScarlar_GMRES into T / TE / ED
+  // the work arrays of UpdateTurKEpsilon (navier_scalar.cpp:20-31)
+  std::vector<double> ke_prod, ke_tes, ke_eds, ke_viste, ke_visted, ke_apte, ke_aped;
 
   std::vector<double>* field(int id);
 
@@ -140,6 +151,10 @@ class Oracle {
 // Krylov solvers on a CSR matrix (krylov.cpp)
 SolveResult ksp_solve(const Csr& A, const std::vector<double>& val, const std::vector<double>& b,
                       std::vector<double>& x, const s355_ksp_opts& o, bool zero_guess);
+
+// z = M^-1 r with M the Jacobi or ILU(0) preconditioner of (A, val)
+void pc_apply(const Csr& A, const std::vector<double>& val, int pc, const std::vector<double>& r,
+              std::vector<double>& z);
 
 // deterministic reductions: fixed 1024-element chunks summed in order
 double chunked_dot(const double* a, const double* b, long n);

@@ -54,10 +54,36 @@ def library() -> C.CDLL:
         lib.orc_update_tur_kepsilon.argtypes = [V, C.POINTER(SolveInfo)]
         lib.orc_update_energy.restype = I
         lib.orc_update_energy.argtypes = [V, C.POINTER(SolveInfo)]
+        lib.orc_pc_apply.restype = I
+        lib.orc_pc_apply.argtypes = [I, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_double), I, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        lib.orc_kepsilon_time_terms.restype = I
+        lib.orc_kepsilon_time_terms.argtypes = [V]
         lib.orc_bc_temperature.restype = I
         lib.orc_bc_temperature.argtypes = [V]
         _lib = lib
     return _lib
+
+
+def pc_apply(rowptr, col, val, pc: int, r) -> np.ndarray:
+    """z = M^-1 r for a CSR matrix (rows with sorted columns and a diagonal
+    entry) with the oracle's Jacobi or natural-order ILU(0) preconditioner
+    (PETSc's PCJACOBI / PCILU with zero fill)."""
+    lib = library()
+    rp = np.ascontiguousarray(rowptr, dtype=np.int32)
+    cl = np.ascontiguousarray(col, dtype=np.int32)
+    n = rp.size - 1
+    rows = np.repeat(np.arange(n, dtype=np.int32), np.diff(rp))
+    dg = np.flatnonzero(cl == rows).astype(np.int32)
+    if dg.size != n:
+        raise ValueError("every row needs its diagonal entry")
+    v = np.ascontiguousarray(val, dtype=np.float64)
+    rr = np.ascontiguousarray(r, dtype=np.float64)
+    z = np.empty(n)
+    rc = lib.orc_pc_apply(n, _ip(rp), _ip(cl), _ip(dg), _dp(v), int(pc), _dp(rr), _dp(z))
+    if rc != 0:
+        raise OracleError(rc, lib.orc_last_error().decode())
+    return z
 
 
 class OracleError(RuntimeError):
@@ -131,6 +157,11 @@ class Oracle:
     def solve_pressure(self, rtol: float, maxit: int) -> dict:
         info = SolveInfo()
         self._check(self.lib.orc_solve_pressure(self.h, rtol, maxit, C.byref(info)), "solve_pressure")
+        return info.as_dict()
+
+    def solve_scalar(self, isca: int, rtol: float, maxit: int) -> dict:
+        info = SolveInfo()
+        self._check(self.lib.orc_solve_scalar(self.h, isca, rtol, maxit, C.byref(info)), "solve_scalar")
         return info.as_dict()
 
     def correct_fields(self) -> float:

@@ -55,6 +55,7 @@ void Oracle::allocate() {
   shear.assign(3 * static_cast<size_t>(Nbnd), 0.0);
   RUFace.assign(Nfac, 0.0);
   for (auto* v : {&Au, &Ap, &As}) v->assign(csr.nnz(), 0.0);
+  for (auto* v : {&ke_prod, &ke_tes, &ke_eds, &ke_viste, &ke_visted, &ke_apte, &ke_aped}) v->assign(Ncel, 0.0);
   if (!par.steady) EnsureOld();
 }
 
@@ -107,6 +108,13 @@ std::vector<double>* Oracle::field(int id) {
     case S355_F_TNP2: return &Tnp2;
     case S355_F_TEP2: return &TEp2;
     case S355_F_EDP2: return &EDp2;
+    case S355_F_KE_PROD: return &ke_prod;
+    case S355_F_KE_TESRC: return &ke_tes;
+    case S355_F_KE_EDSRC: return &ke_eds;
+    case S355_F_KE_APTE: return &ke_apte;
+    case S355_F_KE_APED: return &ke_aped;
+    case S355_F_KE_VISTE: return &ke_viste;
+    case S355_F_KE_VISED: return &ke_visted;
     default: return nullptr;
   }
 }
@@ -843,42 +851,44 @@ void Oracle::BuildScalarMatrix(std::vector<double>& phi, std::vector<double>& bp
   if (inlet_outflow) throw std::runtime_error("reverse flow get out of inlet");
 }
 
-int Oracle::UpdateTurKEpsilon() {
-  std::vector<double> prod(Ncel, 0.), tes(Ncel, 0.), eds(Ncel, 0.), viste(Ncel), visted(Ncel), apte(Ncel, 0.),
-      aped(Ncel, 0.);
+void Oracle::KEpsilonSources() {
+  for (auto* v : {&ke_tes, &ke_eds, &ke_apte, &ke_aped}) v->assign(Ncel, 0.);
   for (int i = 0; i < Ncel; ++i) {
     double vol = cell[i].vol;
-    viste[i] = VisLam[i] + VisTur[i] / kev::Sigma_k;
-    visted[i] = VisLam[i] + VisTur[i] / kev::Sigma_e;
+    ke_viste[i] = VisLam[i] + VisTur[i] / kev::Sigma_k;
+    ke_visted[i] = VisLam[i] + VisTur[i] / kev::Sigma_e;
     const double* du = &dUdX[3 * static_cast<size_t>(i)];
     const double* dv = &dVdX[3 * static_cast<size_t>(i)];
     const double* dw = &dWdX[3 * static_cast<size_t>(i)];
     double s1 = (du[0] + du[0]) * du[0] + (du[1] + dv[0]) * du[1] + (du[2] + dw[0]) * du[2];
     double s2 = (dv[0] + du[1]) * dv[0] + (dv[1] + dv[1]) * dv[1] + (dv[2] + dw[1]) * dv[2];
     double s3 = (dw[0] + du[2]) * dw[0] + (dw[1] + dv[2]) * dw[1] + (dw[2] + dw[2]) * dw[2];
-    prod[i] = VisTur[i] * (s1 + s2 + s3);
+    ke_prod[i] = VisTur[i] * (s1 + s2 + s3);
     double dis = R[i] * ED[i];
-    tes[i] = prod[i] * vol;
-    apte[i] = dis / TE[i] * vol;
+    ke_tes[i] = ke_prod[i] * vol;
+    ke_apte[i] = dis / TE[i] * vol;
   }
-  SetBCKEpsilon(tes, eds, apte, aped, prod);
-  if (!par.steady) {  // navier_scalar.cpp:71-90
-    EnsureOld();
-    TransientScalar(TEp, TEp2, apte, tes);
-    TransientScalar(EDp, EDp2, aped, eds);
-  }
-  BuildScalarMatrix(TE, BTE, viste, tes, apte);
-  s355_ksp_opts o = par.ksp[S355_KSP_SCALAR];
-  last[4] = ksp_solve(csr, As, bs, TE, o, false);
-  if (last[4].reason < 0) return S355_NOT_CONVERGED;
-  for (int i = 0; i < Ncel; ++i)
-    if (TE[i] < 1.e-9) TE[i] = 1.e-9;
-  for (int i = 0; i < Ncel; ++i) {
-    double fact = ED[i] / (TE[i] + kSmall) * cell[i].vol;
-    eds[i] = kev::Ceps1 * fact * prod[i];
-    aped[i] += kev::Ceps2 * fact * R[i];
-  }
-  BuildScalarMatrix(ED, BED, visted, eds, aped);
+}
+
+void Oracle::KEpsilonBC() { SetBCKEpsilon(ke_tes, ke_eds, ke_apte, ke_aped, ke_prod); }
+
+void Oracle::KEpsilonTimeTerms() {
+  if (par.steady) return;
+  EnsureOld();
+  TransientScalar(TEp, TEp2, ke_apte, ke_tes);
+  TransientScalar(EDp, EDp2, ke_aped, ke_eds);
+}
+
+void Oracle::BuildKEpsilonMatrix(int isca) {
+  if (isca == 2)
+    BuildScalarMatrix(TE, BTE, ke_viste, ke_tes, ke_apte);
+  else if (isca == 3)
+    BuildScalarMatrix(ED, BED, ke_visted, ke_eds, ke_aped);
+  else
+    throw std::invalid_argument("BuildKEpsilonMatrix: isca 2 (k) or 3 (epsilon)");
+}
+
+void Oracle::EpsWallRows() {
   for (int i = 0; i < Nbnd; ++i) {
     if (type1_of(bnd[i].rid) != 1) continue;
     int ic = face[bnd[i].face].c1;
@@ -889,8 +899,19 @@ int Oracle::UpdateTurKEpsilon() {
     }
     bs[ic] = ED[ic];
   }
-  last[5] = ksp_solve(csr, As, bs, ED, o, false);
-  if (last[5].reason < 0) return S355_NOT_CONVERGED;
+}
+
+void Oracle::KEpsilonPostK() {
+  for (int i = 0; i < Ncel; ++i)
+    if (TE[i] < 1.e-9) TE[i] = 1.e-9;
+  for (int i = 0; i < Ncel; ++i) {
+    double fact = ED[i] / (TE[i] + kSmall) * cell[i].vol;
+    ke_eds[i] = kev::Ceps1 * fact * ke_prod[i];
+    ke_aped[i] += kev::Ceps2 * fact * R[i];
+  }
+}
+
+void Oracle::KEpsilonPostEps() {
   for (int i = 0; i < Ncel; ++i)
     if (ED[i] < 1.e-12) ED[i] = 1.e-12;
   for (int i = 0; i < Ncel; ++i) {
@@ -899,6 +920,30 @@ int Oracle::UpdateTurKEpsilon() {
     else
       VisTur[i] = 0.;
   }
+}
+
+int Oracle::SolveScalar(int isca, double rtol, int maxit) {
+  s355_ksp_opts o = par.ksp[S355_KSP_SCALAR];
+  o.rtol = rtol;
+  o.maxit = maxit;
+  std::vector<double>& x = isca == 1 ? T : (isca == 2 ? TE : ED);
+  const int slot = isca == 1 ? 6 : (isca == 2 ? 4 : 5);
+  last[slot] = ksp_solve(csr, As, bs, x, o, false);
+  return last[slot].reason < 0 ? S355_NOT_CONVERGED : S355_OK;
+}
+
+int Oracle::UpdateTurKEpsilon() {
+  const s355_ksp_opts& o = par.ksp[S355_KSP_SCALAR];
+  KEpsilonSources();
+  KEpsilonBC();
+  KEpsilonTimeTerms();
+  BuildKEpsilonMatrix(2);
+  if (SolveScalar(2, o.rtol, o.maxit) != S355_OK) return S355_NOT_CONVERGED;
+  KEpsilonPostK();
+  BuildKEpsilonMatrix(3);
+  EpsWallRows();
+  if (SolveScalar(3, o.rtol, o.maxit) != S355_OK) return S355_NOT_CONVERGED;
+  KEpsilonPostEps();
   return S355_OK;
 }
 

@@ -441,6 +441,14 @@ class Solver:
         _check(self.lib, rc, "solve_pressure", info.as_dict())
         return info.as_dict()
 
+    def solve_scalar(self, isca: int, rtol: float, maxit: int) -> dict:
+        """solveScarlar_GMRES on the assembled scalar system into T (1), k (2)
+        or epsilon (3)."""
+        info = SolveInfo()
+        rc = self.lib.s355_solve_scalar(self.ctx, isca, rtol, maxit, C.byref(info))
+        _check(self.lib, rc, "solve_scalar", info.as_dict())
+        return info.as_dict()
+
     def correct_fields(self) -> float:
         r = C.c_double()
         self.call("correct_fields", C.byref(r))

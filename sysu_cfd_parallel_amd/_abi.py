@@ -129,6 +129,7 @@ FIELDS = [
     "SHEAR", "RUFACE", "AU", "AP", "AS",
     "BU_RHS", "BV_RHS", "BW_RHS", "BP_RHS", "BS_RHS", "BTEM",
     "UNP", "VNP", "WNP", "TNP", "TEP", "EDP", "UNP2", "VNP2", "WNP2", "TNP2", "TEP2", "EDP2",
+    "KE_PROD", "KE_TESRC", "KE_EDSRC", "KE_APTE", "KE_APED", "KE_VISTE", "KE_VISED",
 ]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 # which size each field has: c = cells, c3 = 3*cells, b = boundary, b3, f = faces, z = nnz
@@ -139,6 +140,7 @@ FIELD_KIND = {
     **{n: "b" for n in ("BRO", "BU", "BV", "BW", "BPRE", "BTE", "BED", "BTEM")},
     "SHEAR": "b3", "RUFACE": "f", "AU": "z", "AP": "z", "AS": "z",
     **{n: "c" for n in ("UNP", "VNP", "WNP", "TNP", "TEP", "EDP", "UNP2", "VNP2", "WNP2", "TNP2", "TEP2", "EDP2")},
+    **{n: "c" for n in ("KE_PROD", "KE_TESRC", "KE_EDSRC", "KE_APTE", "KE_APED", "KE_VISTE", "KE_VISED")},
 }
 
 MESH_INT = ["ELEM_PERM", "FACE_C1", "FACE_C2", "FACE_BND", "FACE_VERT", "CELL_NFACE", "CELL_FACE", "CELL_CELL",
@@ -206,6 +208,13 @@ def declare(lib: C.CDLL, prefix: str, ctx_type=C.c_void_p) -> None:
         "simple_iterate": (I, [V, I, pD, pI]),
         "save_transient_old_data": (I, [V]),
         "time_state": (I, [V, pD, pI]),
+        "kepsilon_sources": (I, [V]),
+        "bc_kepsilon": (I, [V]),
+        "build_scalar_matrix": (I, [V, I]),
+        "eps_wall_rows": (I, [V]),
+        "solve_scalar": (I, [V, I, D, I, pInfo]),
+        "kepsilon_post_k": (I, [V]),
+        "kepsilon_post_eps": (I, [V]),
     }
     for name, (res, args) in spec.items():
         fn = getattr(lib, f"{P}_{name}")

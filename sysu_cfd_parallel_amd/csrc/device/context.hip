@@ -315,6 +315,13 @@ FieldRef Context::field(int id) {
       if (!p) throw std::invalid_argument("old time levels exist once an unsteady run has saved them");
       return {p, Nc};
     }
+    case S355_F_KE_PROD: return {kw_.prod, Nc};
+    case S355_F_KE_TESRC: return {kw_.tes, Nc};
+    case S355_F_KE_EDSRC: return {kw_.eds, Nc};
+    case S355_F_KE_APTE: return {kw_.apte, Nc};
+    case S355_F_KE_APED: return {kw_.aped, Nc};
+    case S355_F_KE_VISTE: return {kw_.viste, Nc};
+    case S355_F_KE_VISED: return {kw_.visted, Nc};
     default: throw std::invalid_argument("unknown field id " + std::to_string(id));
   }
 }
@@ -889,18 +896,21 @@ void Context::set_kernel_shapes(int precision, int lanes, int nt, int rows, int 
       if (v == x) return true;
     return false;
   };
-  if (precision < 0 || precision > 1 || !ok(lanes, {0, 1, 2, 4, 8, 16}) || !ok(nt, {0, 1}) ||
-      !ok(rows, {0, 1, 2, 4, 8, 16}) || !ok(sigma, {0, kSellSigma[0], kSellSigma[1], kSellSigma[2]}))
+  if (precision < 0 || precision > 1 || !ok(lanes, {-2, -1, 0, 1, 2, 4, 8, 16}) || !ok(nt, {0, 1}) ||
+      !ok(rows, {0, 1, 2, 4, 8, 16}) || !ok(sigma, {0, kSellSigma[0], kSellSigma[1], kSellSigma[2]}) ||
+      (lanes == -2 && (precision != 1 || ncase != 1)))
     throw std::invalid_argument(
-        "kernel shapes: lanes 0 (SELL-64), 1, 2, 4, 8, 16; nt 0/1; rows 0, 1, 2, 4, 8, 16; sigma 0, 64, 512, 4096");
+        "kernel shapes: lanes 0 (SELL-64), -1 (SELL-64, one slice per wave), -2 (SELL-64, two slices per wave: FP32, "
+        "one case), 1, 2, 4, 8, 16; nt 0/1; rows 0, 1, 2, 4, 8, 16; sigma 0, 64, 512, 4096");
   KWork& w = precision ? kwork32_ : kwork_;
-  w.lanes = lanes;
+  w.lanes = lanes < 0 ? 0 : lanes;
   w.nt = nt;
   w.rows = rows;
   w.sell_v = kSellDefault;
   for (int v = 0; v < kSellVariants; ++v)
     if (sigma == kSellSigma[v]) w.sell_v = v;
-  w.sell2 = -1;    // FP32 SELL: two slices per wave above 2^20 rows
+  // FP32 SELL: two slices per wave above 2^20 rows, or as forced
+  w.sell2 = lanes == -2 ? 1 : (lanes == -1 ? 0 : -1);
   w.tuned = true;  // the autotuner leaves explicit shapes alone
 }
 

@@ -41,6 +41,21 @@ void launch_keps_sources(const DMesh& m, const DFields& F, const KepsWork& w, hi
   S355_LAUNCH_CHECK();
 }
 
+// k^1.5 rounded once: k sqrt(k) in double-double (the square root's error
+// and the product's recovered with fma), then the sum rounded, which gives
+// the correctly rounded power.  The host libm's pow(k, 1.5) (glibc, within
+// 0.52 ulp) returns the same value for all but ~0.1 % of arguments, where it
+// is 1 ulp away.
+__device__ __forceinline__ double pow_1p5(double k) {
+  if (!(k > 0.0)) return pow(k, 1.5);  // 0, negative (NaN) or NaN: as pow
+  const double s = sqrt(k);
+  const double r = fma(-s, s, k);   // k - s^2, exact
+  const double sl = r / (2.0 * s);  // sqrt(k) ~ s + sl
+  const double p = k * s;
+  const double e = fma(k, s, -p);   // k s - p, exact
+  return p + (e + k * sl);
+}
+
 // Wall functions (navier_bc.cpp:248-286), one thread per wall-adjacent cell,
 // its wall faces taken in ascending boundary order as the reference's loop does.
 __global__ void __launch_bounds__(kBlock) k_keps_wall(DMesh m, DFields F, KepsWork w, double cmu25, double cmu75) {
@@ -70,7 +85,7 @@ __global__ void __launch_bounds__(kBlock) k_keps_wall(DMesh m, DFields F, KepsWo
       utau = cmu25 * sqrt(k);
     const double prod = tauw * utau / (kev::kappa * d);
     w.prod[ic] = prod;
-    const double eps = cmu75 * pow(k, 1.5) / (kev::kappa * d);
+    const double eps = cmu75 * pow_1p5(k) / (kev::kappa * d);
     w.tes[ic] = prod * vol;
     w.apte[ic] -= R * F.ED[ic] / k * vol;
     w.apte[ic] += R * eps / k * vol;

@@ -1538,8 +1538,8 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
                                           reinterpret_cast<uintptr_t>(inv), reinterpret_cast<uintptr_t>(invr),
                                           static_cast<uintptr_t>(m), static_cast<uintptr_t>(R),
                                           static_cast<uintptr_t>(L), static_cast<uintptr_t>(w.nt),
-                                          static_cast<uintptr_t>(w.sell_v), sizeof(T),
-                                          sizeof(X),
+                                          static_cast<uintptr_t>(w.sell_v),
+                                          static_cast<uintptr_t>(w.sell2 + 1), sizeof(T), sizeof(X),
                                           static_cast<uintptr_t>(first_c), static_cast<uintptr_t>(zero_guess)};
       hipGraphExec_t g = w.graphs->get(key, s, [&] {
         enqueue_restart(first_c, first_c && zero_guess);
@@ -1583,8 +1583,11 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
   return collect(w, mixed);
 }
 
+// With factored (ILU(0) only) the factors already in w.lu are those of A and
+// are used as they are.
 template <class T>
-KResult cg(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, bool zero_guess, KWork& w, hipStream_t s) {
+KResult cg(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, bool zero_guess, KWork& w, hipStream_t s,
+           bool factored = false) {
   const int n = A.n;
   T* V = static_cast<T*>(w.V);
   const size_t ld = static_cast<size_t>(n);
@@ -1600,7 +1603,7 @@ KResult cg(const CsrView<T>& A, const T* b, T* x, const s355_ksp_opts& o, bool z
   } else if (o.pc == S355_PC_ILU0) {
     if (!w.ilu || !w.lu) throw std::invalid_argument("cg: ILU(0) workspace missing");
     lu = static_cast<T*>(w.lu);
-    ilu_factor(A, A.nnz, *w.ilu, lu, s, &w);
+    if (!factored) ilu_factor(A, A.nnz, *w.ilu, lu, s, &w);
   } else if (o.pc != S355_PC_NONE) {
     throw std::invalid_argument("cg: unknown preconditioner");
   }
@@ -1702,6 +1705,19 @@ __global__ void __launch_bounds__(kBlock) k_sum_to(const double* partial, int nb
   s = strided_sum(partial, 0, nblk);
   s = block_sum(s, lds);
   if (threadIdx.x == 0) *out = s;
+}
+
+// partial ||z||^2 per block (FP64 sums of the FP32 values)
+__global__ void __launch_bounds__(kBlock) k_sq_partial(int n, const float* __restrict__ z, double* partial) {
+  __shared__ double lds[kBlock / kWave];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (i < n) {
+    const double zi = static_cast<double>(z[i]);
+    v = zi * zi;
+  }
+  v = block_sum(v, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v;
 }
 
 __global__ void __launch_bounds__(kBlock) k_add_correction(int n, double* x, const float* e) {
@@ -1948,10 +1964,26 @@ KResult ir_solve(const CsrView<double>& A64, const CsrView<float>& A, const doub
   const int g = grid_for(n);
   if (static_cast<size_t>(g) > ir.partial_len) throw std::runtime_error("ir_solve: partial buffer too small");
   hipLaunchKernelGGL(k_inv_diag32, dim3(g), dim3(kBlock), 0, s, A, ir.inv32, o.pc == S355_PC_JACOBI ? 1 : 0);
+  // ILU(0): the outer test takes ||(LU)^-1 r|| as PETSc's CG does, with the
+  // FP32 factors of A32 (factored once here; the inner solves reuse them)
+  float* lu = nullptr;
+  if (o.pc == S355_PC_ILU0) {
+    if (!w32.ilu || !w32.lu) throw std::invalid_argument("ir_solve: ILU(0) workspace missing");
+    lu = static_cast<float*>(w32.lu);
+    ilu_factor(A, A.nnz, *w32.ilu, lu, s, &w32);
+  }
+  // partial ||M^-1 r||^2 of the residual just formed (Jacobi: inside k_ir_residual)
+  auto pc_norm = [&](double* out) {
+    if (lu) {
+      ilu_apply(A, *w32.ilu, lu, ir.r32, ir.e32, static_cast<const KState*>(nullptr), s, nullptr, &w32);
+      hipLaunchKernelGGL(k_sq_partial, dim3(g), dim3(kBlock), 0, s, n, ir.e32, ir.partial);
+    }
+    hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(kBlock), 0, s, ir.partial, g, out);
+  };
   if (zero_guess) fill_zero(x, sizeof(double) * static_cast<size_t>(n), s);
   // ||M^-1 b|| for the reference norm of a nonzero initial guess
   hipLaunchKernelGGL(k_ir_residual, dim3(g), dim3(kBlock), 0, s, A64, b, x, ir.inv32, ir.r64, ir.r32, ir.partial, 1);
-  hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(kBlock), 0, s, ir.partial, g, ir.scal + 1);
+  pc_norm(ir.scal + 1);
   S355_LAUNCH_CHECK();
   s355_ksp_opts inner = o;
   inner.rtol = inner_rtol;
@@ -1962,7 +1994,7 @@ KResult ir_solve(const CsrView<double>& A64, const CsrView<float>& A, const doub
   for (int k = 0;; ++k) {
     hipLaunchKernelGGL(k_ir_residual, dim3(g), dim3(kBlock), 0, s, A64, b, x, ir.inv32, ir.r64, ir.r32, ir.partial,
                        zero_guess && k == 0 ? 1 : 0);
-    hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(kBlock), 0, s, ir.partial, g, ir.scal);
+    pc_norm(ir.scal);
     S355_LAUNCH_CHECK();
     S355_HIP(hipMemcpyAsync(ir.host, ir.scal, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     S355_HIP(hipStreamSynchronize(s));
@@ -1992,7 +2024,7 @@ KResult ir_solve(const CsrView<double>& A64, const CsrView<float>& A, const doub
       break;
     }
     inner.maxit = o.maxit - total;
-    KResult r = krylov_solve<float>(A, ir.r32, ir.e32, inner, true, w32, s);
+    KResult r = cg<float>(A, ir.r32, ir.e32, inner, true, w32, s, lu != nullptr);
     total += r.iters;
     if (r.iters == 0 && r.reason < 0) {
       res.reason = r.reason;

@@ -270,7 +270,10 @@ struct IrWork {
 // Hessenberg least-squares problem is FP64 and the correction is accumulated
 // into the FP64 x; the FP64 residual alone decides convergence.  CG: FP64
 // refinement around FP32 CG solves of A32 e = r to relative tolerance
-// inner_rtol, at most ir_max of them.  Taking the residual with A64 keeps the
+// inner_rtol, at most ir_max of them; the outer test's norm is ||M^-1 r|| with
+// M the FP32 Jacobi diagonal or the FP32 ILU(0) factors of A32 (factored once
+// per solve and shared with the inner solves), as PETSc's CG tests the
+// preconditioned norm.  Taking the residual with A64 keeps the
 // exact null space of the pressure-correction matrix (A 1 = 0), which
 // rounding breaks.  The iteration cap counts Krylov iterations.
 KResult ir_solve(const CsrView<double>& A64, const CsrView<float>& A32, const double* b, double* x,

@@ -144,6 +144,36 @@ def test_two_slice_sell_equals_single_lane_csr():
     assert np.array_equal(out[0][1], out[1][1])
 
 
+@pytest.mark.parametrize("sigma", [64, 512, 4096])
+def test_forced_slices_per_wave_equal_single_lane_csr(sigma):
+    # 1040 rows = 17 slices of 64 (an odd count: the two-slice kernel's last
+    # wave has one slice), rows of up to 12 entries (polyhedral cells next to
+    # split hexes: slices wider than one batch of 8 slots)
+    case = s.channel(11, 7, 5, jitter=0.15, split=0.35, seed=3, re=600.0)
+    out = []
+    for lanes in (1, -1, -2):
+        d = s.Solver(case, base(s.MIXED))
+        assert d.sizes.n_cell == 1040
+        rp = d.ints("CSR_ROWPTR")
+        assert np.diff(rp).max() > 8
+        d.set_kernel_shapes(1, lanes, sigma=0 if lanes == 1 else sigma)
+        shape = d.kernel_shapes()["fp32"]
+        assert shape["slices"] == {1: 0, -1: 1, -2: 2}[lanes]
+        h, _ = d.iterate(5)
+        out.append(({n: d.get(n) for n in FIELDS}, h))
+        d.close()
+    for o in out[1:]:
+        for n in FIELDS:
+            assert np.array_equal(out[0][0][n], o[0][n]), n
+        assert np.array_equal(out[0][1], o[1])
+
+
+def test_two_slices_per_wave_only_for_one_fp32_case():
+    d = s.Solver(s.cavity(4), base(s.MIXED))
+    with pytest.raises(s.S355Error):
+        d.set_kernel_shapes(0, -2)
+
+
 @pytest.mark.parametrize("lanes,sigma", [(0, 64), (0, 4096), (2, 0), (16, 0)])
 def test_every_spmv_shape_matches_the_oracle(lanes, sigma):
     from oracle.pyoracle import Oracle
