@@ -83,6 +83,12 @@ class Channel:
     steps, warmup = 5, 1
     turbulent = False
     autotune, graphs = 1, 0
+    # CPU baseline on the GPU case itself from the GPU's post-warm-up state:
+    # "steps" times whole SIMPLE steps, "projected" times every phase but the
+    # pressure solve in full and the pressure solve per Krylov iteration
+    # (a step from this state takes minutes on the host cores); None: a
+    # smaller sample from rest (cpu_sample)
+    cpu_mode = "projected"
 
     def __init__(self, size):
         self.dims = self.sizes[size]
@@ -101,7 +107,9 @@ class Channel:
     def describe(self, prec_name):
         d = self.dims
         return (f"laminar channel Re=1000, {d[0]}x{d[1]}x{d[2]} box, 15% jitter, 10% hexes split into 6 tets, "
-                f"GMRES(50)+Jacobi, " + ("FP32 inner solves in FP64 refinement" if prec_name == "mixed" else "FP64"))
+                f"GMRES(50)+Jacobi, rtol 1e-8 momentum / 1e-6 pressure, iteration caps raised from the reference's "
+                f"1000 to 20000 (the pressure solve takes ~3000), "
+                + ("FP32 inner solves in FP64 refinement" if prec_name == "mixed" else "FP64"))
 
     def cpu_sample(self):
         # 1/8 of the cells: the full mesh takes minutes per CPU step
@@ -115,6 +123,7 @@ class Cavity(Channel):
     precisions = ["fp64"]
     steps, warmup = 200, 0
     graphs = 1
+    cpu_mode = "steps"
 
     def case(self, dims=None):
         import sysu_cfd_parallel_amd as s
@@ -145,6 +154,7 @@ class Step(Channel):
     steps, warmup = 5, 1
     turbulent = True
     graphs = 1  # the ILU(0) level sweeps replay from captured graphs
+    cpu_mode = "steps"
 
     def case(self, dims=None):
         import sysu_cfd_parallel_amd as s
@@ -160,12 +170,17 @@ class Step(Channel):
         p.turb_model = 1
         p.residual_steady = 0.0
         p.precision = prec
+        # the pressure correction relaxed by 0.3 (a param.in relaxation line):
+        # SIMPLE's default 0.5 is past the outer iteration's stability limit
+        # on these tetrahedra, and the run would diverge
+        p.urf[3] = 0.3
         return p
 
     def describe(self, prec_name):
         n = self.dims
-        return (f"backward-facing step ER=2, {n} cells per step height, jittered (10%) tetrahedra, "
-                f"k-epsilon with wall functions, Re=5000, GMRES(30)/CG + ILU(0), FP64")
+        return (f"backward-facing step ER=2, {n} cells per step height, 10%-jittered hex lattice with every hex "
+                f"split into 6 tetrahedra (Kuhn split, not a Delaunay fill), k-epsilon with wall functions, Re=5000, "
+                f"relaxation 0.6 momentum / 0.3 pressure, GMRES(30)/CG + ILU(0), FP64")
 
     def cpu_sample(self):
         return self.case(), "the same step case"
@@ -178,6 +193,7 @@ class Large(Channel):
     steps, warmup = 3, 1
     turbulent = True
     graphs = 1  # the ILU(0) level sweeps replay from captured graphs
+    cpu_mode = None  # the oracle's 20 M-cell context needs ~35 GB and minutes to set up
 
     def case(self, dims=None):
         return _channel_case(dims or self.dims, 0.10, 0.0, 13750.0, turbulent=True)
@@ -209,6 +225,7 @@ class Sweep(Channel):
     precisions = ["mixed"]
     steps, warmup = 5, 1
     graphs = 1
+    cpu_mode = None
 
     def __init__(self, size):
         import numpy as np
@@ -268,6 +285,88 @@ def spmv_kernel_name(shape: dict, elem: str, jacobi: bool = True) -> str:
     return f"k_spmv_pc<{elem}, {shape['lanes']}, {tf(shape['nt'])}>"
 
 
+def host_cores() -> dict:
+    """The host cores this process may use (its affinity, bounded by
+    OMP_NUM_THREADS where that is set) and the CPU model."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(avail, int(omp)) if omp.isdigit() and int(omp) > 0 else avail
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": avail, "threads": threads, "model": model}
+
+
+def _timed(fn):
+    t = time.perf_counter()
+    out = fn()
+    return time.perf_counter() - t, out
+
+
+def cpu_baseline_same_case(wl, case, state: dict, gpu_its: dict, threads: int, budget_s: float = 25.0,
+                           p_cap: int = 200) -> dict:
+    """FP64 CPU oracle on the GPU's case itself, started from the state the
+    GPU's timed steps start from (after its warm-up).  cpu_mode "steps": whole
+    SIMPLE steps until the budget is spent.  "projected": one step's phases
+    timed in full (momentum assembly and its 3 solves, pressure-correction
+    assembly, corrections, k-epsilon), the pressure solve over its first
+    p_cap iterations, the step's time = the phases + the GPU FP64 run's mean
+    pressure iterations per step x the CPU time per iteration (the same
+    GMRES(m) on the same matrix).  Both on `threads` host threads; the
+    single-core figure scales the step by the 1-thread / all-thread time of
+    a pressure Krylov iteration on the same matrix."""
+    from oracle.pyoracle import Oracle
+
+    t_set, o = _timed(lambda: Oracle(case, wl.params(0), threads=threads))
+    for n, v in state.items():
+        o.set(n, v)
+    out = {"setup_s": t_set, "cells": o.sizes.n_cell}
+    if wl.cpu_mode == "steps":
+        steps, t0 = 0, time.perf_counter()
+        while True:
+            _, done, rc = o.iterate(1)
+            steps += 1
+            if rc != 0 or time.perf_counter() - t0 > budget_s or steps >= 200:
+                break
+        dt = time.perf_counter() - t0
+        out.update({"step_s": dt / steps, "steps": steps, "how": f"{steps} whole SIMPLE steps in {dt:.1f}s"})
+        p_its = None
+    else:
+        rtol_p = o.params.ksp[1].rtol
+        t_v, (rc, _, vinfo) = _timed(o.calculate_velocity)
+        t_bp, _ = _timed(lambda: [o.call(k) for k in ("bc_velocity", "cal_ru_face", "build_pressure_matrix")])
+        t_sp, pinfo = _timed(lambda: o.solve_pressure(rtol_p, p_cap))
+        k = max(pinfo["iters"], 1)
+        t_it = t_sp / k
+        t_c, _ = _timed(lambda: (o.correct_fields(), o.call("bc_velocity"), o.call("correct_ru_face")))
+        t_ke = _timed(o.update_tur_kepsilon)[0] if wl.turbulent else 0.0
+        p_its = gpu_its.get("P", 0.0)
+        step = t_v + t_bp + p_its * t_it + t_c + t_ke
+        out.update({"step_s": step, "steps": 1, "pressure_s_per_iteration": t_it,
+                    "how": (f"momentum phase {t_v:.2f}s ({[i['iters'] for i in vinfo]} its), pressure assembly "
+                            f"{t_bp:.2f}s, pressure solve {t_it * 1e3:.2f} ms/iteration over its first {k} "
+                            f"iterations x {p_its:.0f} iterations per step (the GPU FP64 run's mean), corrections "
+                            f"{t_c:.2f}s" + (f", k-epsilon {t_ke:.2f}s" if wl.turbulent else ""))})
+    # single core: a pressure Krylov iteration on 1 thread against all threads
+    o.lib.orc_set_threads(1)
+    o.call("build_pressure_matrix")
+    t1, i1 = _timed(lambda: o.solve_pressure(o.params.ksp[1].rtol, 8))
+    o.lib.orc_set_threads(threads)
+    tn, iN = _timed(lambda: o.solve_pressure(o.params.ksp[1].rtol, 8))
+    r = (t1 / max(i1["iters"], 1)) / max(tn / max(iN["iters"], 1), 1e-12)
+    out["single_core_factor"] = r
+    o.close()
+    return out
+
+
 def cpu_baseline(wl, threads: int, budget_s: float = 20.0, max_steps: int = 3) -> dict:
     """FP64 CPU oracle on a bounded sample of the same workload, started from
     rest like the GPU run and stepped until the budget is spent.  Where the
@@ -294,8 +393,9 @@ def cpu_baseline(wl, threads: int, budget_s: float = 20.0, max_steps: int = 3) -
     return {"iters_per_s": steps / dt, "cells": n, "steps": steps, "seconds": dt, "what": what}
 
 
-def run_path(wl, case, prec: int, steps: int, warmup: int, label: str) -> dict:
-    """W untimed + K timed SIMPLE steps of one precision path from rest."""
+def run_path(wl, case, prec: int, steps: int, warmup: int, label: str, snapshot: bool = False) -> dict:
+    """W untimed + K timed SIMPLE steps of one precision path from rest;
+    with snapshot the state the timed steps start from is kept."""
     import numpy as np
 
     import sysu_cfd_parallel_amd as s
@@ -312,6 +412,7 @@ def run_path(wl, case, prec: int, steps: int, warmup: int, label: str) -> dict:
         h, _ = dev.iterate(1)
         log(f"[{label}] warmup step {k + 1}/{warmup}: {time.perf_counter() - t:.2f}s, ResMax {h[-1, 10]:.3e}")
     dev.call("synchronize")
+    state = {n: dev.get(n) for n in STATE} if snapshot else None
     dev.stage_times(reset=True)
     kprec = 1 if prec == s.MIXED else 0
     dev.spmv_profile(kprec, 1)  # HIP events around the Arnoldi SpMV launches of the timed steps
@@ -340,7 +441,7 @@ def run_path(wl, case, prec: int, steps: int, warmup: int, label: str) -> dict:
     tuning = {"shapes": shapes["fp32" if k else "fp64"], "report": dev.tune_report(k)}
     log(f"[{label}] kernel shapes {tuning['shapes']}")
     out = {
-        "dev": dev, "sizes": z, "seconds": t1 - t0, "history": np.array(rows), "tuning": tuning,
+        "dev": dev, "sizes": z, "seconds": t1 - t0, "history": np.array(rows), "tuning": tuning, "state": state,
         "spmv_in_solve": {"ms_per_launch": spmv_ms, "launches_timed": spmv_n},
         "fields": {n: dev.get(n) for n in names},
         "krylov_its_per_step": {n: v / steps for n, v in its.items()},
@@ -358,16 +459,18 @@ STATE = ("R", "U", "V", "W", "P", "T", "TE", "ED", "VISLAM", "VISTUR", "APR", "D
 
 def one_step_agreement(dev, names) -> dict:
     """The mixed path's own error, without the drift of two trajectories:
-    one SIMPLE step from the same state in mixed precision, in FP64, and in
-    FP64 with every solve converged 10^4 times further (the step's converged
-    result), on the same context; relative L2 differences of each field."""
+    one SIMPLE step from the same state in mixed precision and in FP64, at the
+    workload's solve tolerances and with every solve converged 10^4 times
+    further (each path's converged step), on the same context; relative L2
+    differences of each field."""
     import sysu_cfd_parallel_amd as s
 
     state = {n: dev.get(n) for n in STATE}
     p = dev.params
     rtol = [p.ksp[k].rtol for k in range(3)]
     out = {}
-    for key, prec, tight in (("mixed", s.MIXED, 1.0), ("fp64", s.FP64, 1.0), ("converged", s.FP64, 1e-4)):
+    for key, prec, tight in (("mixed", s.MIXED, 1.0), ("fp64", s.FP64, 1.0), ("converged", s.FP64, 1e-4),
+                             ("mixed_converged", s.MIXED, 1e-4)):
         for n, v in state.items():
             dev.set(n, v)
         p.precision = prec
@@ -383,6 +486,8 @@ def one_step_agreement(dev, names) -> dict:
     res = {f"mixed_vs_fp64.{n}": l2_rel(out["mixed"][n], out["fp64"][n]) for n in names}
     res.update({f"mixed_vs_converged.{n}": l2_rel(out["mixed"][n], out["converged"][n]) for n in names})
     res.update({f"fp64_vs_converged.{n}": l2_rel(out["fp64"][n], out["converged"][n]) for n in names})
+    res.update({f"converged_mixed_vs_fp64.{n}": l2_rel(out["mixed_converged"][n], out["converged"][n])
+                for n in names})
     return res
 
 
@@ -442,8 +547,11 @@ def main() -> int:
     log(f"{wl.name} case built in {time.perf_counter() - t:.1f}s ({n_elems} volume elements)")
 
     runs = {}
-    for name in order:
-        runs[name] = run_path(wl, case, s.MIXED if name == "mixed" else s.FP64, steps, warmup, name)
+    want_cpu = not args.no_cpu_baseline and wl.cpu_mode is not None and not isinstance(case, list)
+    for k, name in enumerate(order):
+        # the CPU baseline starts from the first path's post-warm-up state
+        runs[name] = run_path(wl, case, s.MIXED if name == "mixed" else s.FP64, steps, warmup, name,
+                              snapshot=want_cpu and k == 0)
         if name != order[-1]:
             runs[name]["dev"].close()
     main_name = order[-1]
@@ -460,11 +568,24 @@ def main() -> int:
             (abs(a["history"][:, :4] - b["history"][:, :4]) / abs(b["history"][:, :4]).clip(1e-300)).max())
         log(f"mixed vs FP64 relative L2 field difference: {agreement}")
 
+    accuracy = None
     if agreement is not None:
         t = time.perf_counter()
         one = one_step_agreement(dev, list(runs["mixed"]["fields"]))
         agreement["one_step_from_the_same_state"] = one
         log(f"mixed vs FP64, one step from the same state: {one} ({time.perf_counter() - t:.1f}s)")
+        conv = max(v for k, v in one.items() if k.startswith("converged_mixed_vs_fp64."))
+        accuracy = {
+            "target_rel_err": 1e-6,
+            "converged_step_max": conv, "pass": conv <= 1e-6,
+            "converged_step": "one SIMPLE step from the same state, mixed vs FP64, every solve converged 1e4 "
+                              "times below the workload's tolerance: the error of the mixed arithmetic itself",
+            "trajectory_max": agreement["max"], "trajectory_pass": agreement["max"] <= 1e-6,
+            "trajectory": f"after the {warmup} + {steps} steps from rest at the workload's tolerances, where each "
+                          "path stops every pressure solve at its own 1e-6 and the two drift apart by that",
+        }
+        log(f"accuracy vs the 1e-6 target: converged step {conv:.2e} ({'pass' if accuracy['pass'] else 'FAIL'}), "
+            f"trajectory {agreement['max']:.2e} ({'pass' if accuracy['trajectory_pass'] else 'fail'})")
 
     spmv = spmv_roofline(dev, z)
     dom = dict(spmv["fp32" if main_name == "mixed" else "fp64"])
@@ -491,17 +612,36 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"CPU oracle baseline on {threads} threads")
-        cb = cpu_baseline(wl, threads)
-        cpu = {
-            "value": cb["iters_per_s"] * cb["cells"] / z.n_cell,
-            "unit": "SIMPLE iterations/s (FP64 oracle; sample cells*iters/s divided by the GPU case's cells)",
-            "cores": threads,
-            "kind": "port",
-            "sample": f"{cb['what']}, {cb['cells']} cells, first {cb['steps']} SIMPLE steps from rest in "
-                      f"{cb['seconds']:.1f}s ({cb['iters_per_s']:.4f} it/s on the sample)",
-        }
+        host = host_cores()
+        threads = args.cpu_threads or host["threads"]
+        log(f"CPU oracle baseline on {threads} threads ({host['nproc']} cores seen, {host['affinity']} in this "
+            f"process's affinity, {host['model']})")
+        if want_cpu:
+            first = runs[order[0]]
+            its = runs["fp64"]["krylov_its_per_step"] if "fp64" in runs else first["krylov_its_per_step"]
+            cb = cpu_baseline_same_case(wl, case, first["state"], its, threads)
+            rate = 1.0 / cb["step_s"]
+            cpu = {
+                "value": rate, "unit": "SIMPLE iterations/s (FP64 oracle)", "cores": threads, "kind": "port",
+                "sample": f"the same {cb['cells']}-cell case from the GPU run's state after its {warmup} warm-up "
+                          f"steps: {cb['how']}",
+                "single_core": {"value": rate / cb["single_core_factor"],
+                                "how": f"the step scaled by the 1-thread / {threads}-thread time of a "
+                                       f"pressure-correction Krylov iteration ({cb['single_core_factor']:.2f})"},
+                "host": host, "setup_s": cb["setup_s"],
+            }
+        else:
+            cb = cpu_baseline(wl, threads)
+            cpu = {
+                "value": cb["iters_per_s"] * cb["cells"] / z.n_cell,
+                "unit": "SIMPLE iterations/s (FP64 oracle; sample cells*iters/s divided by the GPU case's cells)",
+                "cores": threads,
+                "kind": "port",
+                "sample": f"{cb['what']}, {cb['cells']} cells, first {cb['steps']} SIMPLE steps from rest in "
+                          f"{cb['seconds']:.1f}s ({cb['iters_per_s']:.4f} it/s on the sample)",
+                "host": host,
+            }
+        log(f"CPU baseline {cpu['value']:.4g} it/s: {cpu['sample']}")
 
     out = {
         "metric": METRIC,
@@ -542,6 +682,7 @@ def main() -> int:
             for n, v in runs.items()
         },
         "mixed_vs_fp64": agreement,
+        "accuracy": accuracy,
         "cpu_baseline": cpu,
     }
     if "fp64" in runs and "mixed" in runs:

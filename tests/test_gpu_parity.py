@@ -160,6 +160,21 @@ def test_simple_loop_step_kepsilon_fp64():
     assert_close(d, o, ["U", "V", "W", "P", "TE", "ED", "VISTUR"], 1e-6)
 
 
+def test_step_tets_kepsilon_converges_on_device_and_oracle():
+    # the backward-facing-step workload's form (jittered Kuhn tetrahedra,
+    # k-epsilon, Re 5000) with the pressure correction relaxed by 0.3: the
+    # outer residual falls on both paths, which agree step by step
+    case = s.step(2, upstream=2, downstream=6, span=2, turbulent=True, re=5000)
+    p = params(rtol=1e-12, turb=1, maxit=5000)
+    p.residual_steady = 0.0
+    p.urf[3] = 0.3
+    d, o, hd, ho = _loop_pair(case, p, 30)
+    for h in (hd, ho):
+        assert h[-1, 10] < 0.05 * h[:, 10].max()
+    assert np.allclose(hd, ho, rtol=1e-6, atol=1e-14)
+    assert_close(d, o, ["U", "V", "W", "P", "TE", "ED", "VISTUR"], 1e-6)
+
+
 def test_reference_tolerances_agree_to_solve_accuracy():
     # the reference's own settings: GMRES(30)+Jacobi, rtol 1e-8 everywhere (config 1 form)
     d, o, hd, ho = _loop_pair(s.cavity(10, re=100), params(rtol=1e-8), 20)
