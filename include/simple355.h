@@ -249,6 +249,15 @@ int s355_bc_deltap(s355_ctx* ctx);                         /* SetBCDeltaP    nav
 /* Gradient (navier_gradient.cpp:12-74): cell field `field` with boundary
  * values `bfield` into 3-vector field `out`. */
 int s355_gradient(s355_ctx* ctx, int field, int bfield, int out);
+/* s355_gradient for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc); no
+ * counterpart in the reference.  The limiters' scratch (MLP: 2 doubles per
+ * vertex, WENO: 3 per cell) lives in a fixed 24 MiB array per GPU
+ * (3 * 2^20 doubles), held until the process ends; larger meshes are
+ * rejected with S355_E_ARG.  The limiter's vertex arrays are uploaded when
+ * the context is created or its parameters set.  Calls that overlap on one
+ * GPU share the array and race. */
+int s355_gradient_noalloc(s355_ctx* ctx, int field, int bfield, int out);
 int s355_build_velocity_matrix(s355_ctx* ctx);             /* navier_velocity.cpp:63-393 */
 int s355_solve_velocity(s355_ctx* ctx, double rtol, int maxit, s355_solve_info info[3]); /* MPIStructure.cpp:234-375 */
 int s355_cal_ru_face(s355_ctx* ctx);                       /* CalRUFace2 navier_velocity.cpp:425-476 */

@@ -86,6 +86,7 @@ def library(path: Optional[Path] = None) -> C.CDLL:
         "save_velocity": (I, [V]),
         "velocity_residual": (I, [V]),
         "residuals": (I, [V, pD, I]),
+        "gradient_noalloc": (I, [V, I, I, I]),
         "kepsilon_sources": (I, [V]),
         "bc_kepsilon": (I, [V]),
         "build_scalar_matrix": (I, [V, I]),

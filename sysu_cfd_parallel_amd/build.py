@@ -25,6 +25,7 @@ SOURCES = [
     "device/keps_kernels.hip",
     "device/transient_kernels.hip",
     "device/krylov.hip",
+    "device/scratch.hip",
     "device/context.hip",
     "host/navier.cpp",
     "host/output.cpp",

@@ -708,6 +708,10 @@ int s355_gradient(s355_ctx* ctx, int field, int bfield, int out) {
   return guarded([&] { dev(ctx).gradient(field, bfield, out); return S355_OK; });
 }
 
+int s355_gradient_noalloc(s355_ctx* ctx, int field, int bfield, int out) {
+  return guarded([&] { dev(ctx).gradient_noalloc(field, bfield, out); return S355_OK; });
+}
+
 int s355_build_velocity_matrix(s355_ctx* ctx) {
   return guarded([&] { dev(ctx).build_velocity_matrix(); return S355_OK; });
 }

@@ -1,6 +1,7 @@
 // Device context: owns the HBM copies of one case and runs the device-level
 // operations of the SIMPLE hot path on one HIP stream.
 #include "context.hpp"
+#include "scratch.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -69,8 +70,11 @@ Context::Context(const s355_mesh_desc* descs, int n_case, const s355_params& par
     throw std::invalid_argument("cell_pressure_ref out of range");
   try {
     S355_HIP(hipStreamCreate(&stream));
+    scratch_ = scratch_for_current_device();
     upload_mesh();
     allocate_fields();
+    // the limiters' vertex arrays, so that a limited gradient allocates nothing
+    if (par.limiter != 0) ensure_vertices();
     S355_HIP(hipStreamSynchronize(stream));
   } catch (...) {
     free_all();
@@ -267,6 +271,7 @@ void Context::set_params(const s355_params& p) {
     for (int k = 0; k < 3; ++k)
       if (p.ksp[k].type == S355_KSP_CG) throw std::invalid_argument("a batch of cases solves every system with GMRES");
   par = p;
+  if (par.limiter != 0) ensure_vertices();
 }
 
 FieldRef Context::field(int id) {
@@ -424,6 +429,34 @@ void Context::gradient(int field_id, int bfield_id, int out_id) {
   const double* bphi[1] = {b.p};
   double* out[1] = {g.p};
   grad(1, phi, bphi, out);
+}
+
+// The same gradient with the limiters' scratch (MLP: the extrema at every
+// vertex, 2 Nvert doubles; WENO: the limited copy, 3 Ncel doubles) in the
+// fixed scratch array: no allocation.  Both are written in full before they
+// are read, so the array needs no clearing; calls that overlap on one GPU
+// share the array and race.
+void Context::gradient_noalloc(int field_id, int bfield_id, int out_id) {
+  FieldRef f = field(field_id), b = field(bfield_id), g = field(out_id);
+  if (f.n != static_cast<size_t>(dm.Nc) || b.n != static_cast<size_t>(dm.Nb) || g.n != 3 * static_cast<size_t>(dm.Nc))
+    throw std::invalid_argument("gradient: expects a cell field, a boundary field and a cell 3-vector field");
+  size_t need = 0;
+  if (par.limiter == 2) need = 2 * static_cast<size_t>(dm.Nv);
+  if (par.limiter == 3) need = 3 * static_cast<size_t>(dm.Nc);
+  if (need > kScratchDoubles)
+    throw std::invalid_argument("gradient_noalloc: the limiter needs " + std::to_string(need) +
+                                " doubles of scratch, above the fixed " + std::to_string(kScratchDoubles) +
+                                " (MLP: 2 x vertices, WENO: 3 x cells)");
+  if (par.limiter != 0 && !dm.c_vert) throw std::logic_error("gradient_noalloc: limiter vertex arrays missing");
+  if (!scratch_) throw std::logic_error("gradient_noalloc: no scratch address");
+  const double* phi[1] = {f.p};
+  const double* bphi[1] = {b.p};
+  double* out[1] = {g.p};
+  launch_gradient(dm, 1, phi, bphi, out, stream);
+  if (par.limiter == 0) return;
+  double* vmax = scratch_;
+  double* vmin = scratch_ + dm.Nv;
+  launch_limiter(dm, par.limiter, f.p, g.p, vmax, vmin, scratch_, stream);
 }
 
 void Context::build_velocity_matrix() {

@@ -45,6 +45,7 @@ class Context {
   void bc_pressure();
   void bc_deltap();
   void gradient(int field, int bfield, int out);
+  void gradient_noalloc(int field, int bfield, int out);  // limiter scratch in the fixed array
   void build_velocity_matrix();
   int solve_velocity(double rtol, int maxit, s355_solve_info info[3]);
   void cal_ru_face();
@@ -125,7 +126,7 @@ class Context {
   float* a32_ = nullptr;
   KWork kwork32_{};
   IrWork ir_{};
-  // ILU(0): level schedule of the pattern and factor storage
+  double* scratch_ = nullptr;  // this device's fixed scratch array (scratch.hpp)
   std::unique_ptr<GraphCache> graphs_;  // captured GMRES cycles (params.graphs)
   std::unique_ptr<SpmvTimer> timers_[2];  // in-solve SpMV timing, FP64 / FP32
   // SELL-64 copies of the matrix being solved, one per sorting window: patterns
