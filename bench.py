@@ -41,10 +41,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the production SpMV on the channel workload's
 # pressure-correction matrix (5,995,990 cells), from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes scaled by a 1 GiB copy (profiles/hbm_traffic_channel_6M.md)
-MEASURED_TRAFFIC = {
-    ("k_spmv_sell<float, true, true>", 5995990): 447.1e6,
-    ("k_spmv_sell2<float, true, true>", 5995990): 441.6e6,
-}
+MEASURED_TRAFFIC = {}
 METRIC = "SIMPLE iters/s @4M cells, FP64 vs mixed (<=1e-6 rel err); SpMV % HBM peak"
 STAGES = ["total", "build_velocity", "solve_velocity", "build_pressure", "solve_pressure", "other"]
 
@@ -102,6 +99,9 @@ class Channel:
         p = _gmres_all(s.default_params(), 50, s.PC_JACOBI)
         p.residual_steady = 0.0
         p.precision = prec
+        # the FP32 cycle on the rows in the order the generator wrote the
+        # cells (lattice order), with 16-bit SELL column offsets
+        p.inner_order = 2
         return p
 
     def describe(self, prec_name):
@@ -109,7 +109,8 @@ class Channel:
         return (f"laminar channel Re=1000, {d[0]}x{d[1]}x{d[2]} box, 15% jitter, 10% hexes split into 6 tets, "
                 f"GMRES(50)+Jacobi, rtol 1e-8 momentum / 1e-6 pressure, iteration caps raised from the reference's "
                 f"1000 to 20000 (the pressure solve takes ~3000), "
-                + ("FP32 inner solves in FP64 refinement" if prec_name == "mixed" else "FP64"))
+                + ("FP32 inner solves in FP64 refinement, the FP32 cycle on the rows in the mesh's input cell order"
+                   if prec_name == "mixed" else "FP64"))
 
     def cpu_sample(self):
         # 1/8 of the cells: the full mesh takes minutes per CPU step
@@ -281,7 +282,8 @@ def spmv_kernel_name(shape: dict, elem: str, jacobi: bool = True) -> str:
     tf = lambda v: "true" if v else "false"  # noqa: E731
     if shape["lanes"] == 0:
         two = "2" if shape.get("slices") == 2 else ""  # two slices per wave
-        return f"k_spmv_sell{two}<{elem}, {tf(shape['nt'])}, {tf(jacobi)}>"
+        c16 = tf(shape.get("order", 0))  # 16-bit column offsets in the inner order
+        return f"k_spmv_sell{two}<{elem}, {tf(shape['nt'])}, {tf(jacobi)}, {c16}>"
     return f"k_spmv_pc<{elem}, {shape['lanes']}, {tf(shape['nt'])}>"
 
 
@@ -503,9 +505,11 @@ def spmv_roofline(dev, z, reps: int = 50) -> dict:
     out = {}
     for name, prec, b in (("fp64", 0, 8), ("fp32", 1, 4)):
         ms = dev.spmv_time(1, prec, reps)
-        nb = spmv_bytes(z.nnz, z.n_cell, b, b)
         shape = dev.kernel_shapes()[name]
-        out[name] = {"kernel": spmv_kernel_name(shape, "double" if b == 8 else "float"), "bytes_per_launch": nb,
+        jacobi = not shape.get("order")  # the inner order holds the Jacobi scaling in the matrix
+        nb = spmv_bytes(z.nnz, z.n_cell, b, b, jacobi)
+        out[name] = {"kernel": spmv_kernel_name(shape, "double" if b == 8 else "float", jacobi),
+                     "bytes_per_launch": nb,
                      "ms_per_launch": ms, "achieved": nb / (ms * 1e-3) / 1e9, "timing": "cold, one launch at a time"}
         out[name]["frac"] = out[name]["achieved"] / HBM_PEAK_GBS
     log(f"SpMV fp64 {out['fp64']['ms_per_launch'] * 1e3:.1f} us -> {out['fp64']['achieved']:.0f} GB/s; "
@@ -592,11 +596,12 @@ def main() -> int:
     ins = r["spmv_in_solve"]
     if ins["launches_timed"] > 0:
         # the roofline of the launches inside the timed solves themselves;
-        # with ILU(0) the Arnoldi SpMV carries no Jacobi scaling
-        jacobi = dev.params.ksp[s.KSP_PRESSURE].pc == s.PC_JACOBI
+        # with ILU(0) the Arnoldi SpMV carries no Jacobi scaling, nor in the
+        # inner order, whose FP32 matrix holds it
+        shape = dev.kernel_shapes()[main_name if main_name == "fp64" else "fp32"]
+        jacobi = dev.params.ksp[s.KSP_PRESSURE].pc == s.PC_JACOBI and not shape.get("order")
         b = 4 if main_name == "mixed" else 8
-        dom["kernel"] = spmv_kernel_name(dev.kernel_shapes()[main_name if main_name == "fp64" else "fp32"],
-                                         "float" if b == 4 else "double", jacobi)
+        dom["kernel"] = spmv_kernel_name(shape, "float" if b == 4 else "double", jacobi)
         dom["bytes_per_launch"] = spmv_bytes(z.nnz, z.n_cell, b, b, jacobi)
         dom["ms_per_launch"] = ins["ms_per_launch"]
         dom["achieved"] = dom["bytes_per_launch"] / (ins["ms_per_launch"] * 1e-3) / 1e9

@@ -102,7 +102,14 @@ typedef struct s355_params {
   int graphs;           /* 1: capture each GMRES restart cycle as a hipGraph
                            and replay it; 0: launch kernel by kernel */
   int solve_energy;     /* SolveEnergy: 1 = UpdateEnergy after the k-epsilon step */
-  int energy_pad;
+  int inner_order;      /* mixed GMRES on one case without ILU(0): the FP32
+                           Arnoldi cycle runs on the rows in another order,
+                           its SELL-64 copies holding 16-bit column offsets
+                           (when every offset fits): 1 = reverse
+                           Cuthill-McKee, 2 = the cells' order in the input
+                           element stream; 0 = the natural order.  The
+                           order is built on the first solve that uses it;
+                           switching to the other order later is refused */
   double cp, prl, prte; /* heat capacity, laminar / turbulent Prandtl numbers
                            (navier.cpp:489-492: 1006, 0.72, 0.9) */
   int limiter;          /* gradient limiter (readparamfile.cpp:149-162): 0 none,
@@ -375,9 +382,11 @@ int s355_copy_time(s355_ctx* ctx, size_t bytes, int repeats, double* ms_per_call
  * Gram-Schmidt / update kernels, sigma[k] the row window the SELL-64 copy
  * is sorted in (used when lanes[k] is 0), tuned[k] 1 when they came from
  * the autotuner, slices[k] the SELL-64 slices each wave takes (1, or 2 for
- * the FP32 product on one case; 0 with CSR).  No reference counterpart. */
+ * the FP32 product on one case; 0 with CSR), order[k] 1 when the mixed
+ * solves' FP32 cycle runs in the inner order (params.inner_order set and its
+ * 16-bit offsets fitting; always 0 for FP64).  No reference counterpart. */
 int s355_kernel_shapes(s355_ctx* ctx, int lanes[2], int nt[2], int rows[2], int sigma[2], int tuned[2],
-                       int slices[2]);
+                       int slices[2], int order[2]);
 
 /* Times the production SpMV launches inside the GMRES solves of one
  * precision (0 FP64, 1 FP32): every 4th Arnoldi SpMV is bracketed by HIP

@@ -106,7 +106,7 @@ def library(path: Optional[Path] = None) -> C.CDLL:
         "spmv_time": (I, [V, I, I, I, pD]),
         "last_solve_info": (I, [V, C.POINTER(SolveInfo)]),
         "copy_time": (I, [V, C.c_size_t, I, pD]),
-        "kernel_shapes": (I, [V, pI, pI, pI, pI, pI]),
+        "kernel_shapes": (I, [V, pI, pI, pI, pI, pI, pI, pI]),
         "tune_report": (I, [V, I, pD, pD, pI]),
         "set_kernel_shapes": (I, [V, I, I, I, I, I]),
         "spmv_profile": (I, [V, I, I, pD, pI]),
@@ -494,10 +494,11 @@ class Solver:
     def kernel_shapes(self) -> dict:
         """SpMV lanes per row (0 = SELL-64) / non-temporal loads / SELL
         sorting window and Gram-Schmidt rows per thread, FP64 and FP32."""
-        lanes, nt, rows, sigma, tuned, slices = ((C.c_int * 2)() for _ in range(6))
-        _check(self.lib, self.lib.s355_kernel_shapes(self.ctx, lanes, nt, rows, sigma, tuned, slices), "kernel_shapes")
+        lanes, nt, rows, sigma, tuned, slices, order = ((C.c_int * 2)() for _ in range(7))
+        _check(self.lib, self.lib.s355_kernel_shapes(self.ctx, lanes, nt, rows, sigma, tuned, slices, order),
+               "kernel_shapes")
         return {p: {"lanes": lanes[k], "nt": nt[k], "rows": rows[k], "sigma": sigma[k], "tuned": bool(tuned[k]),
-                    "slices": slices[k]}
+                    "slices": slices[k], "order": order[k]}
                 for k, p in enumerate(("fp64", "fp32"))}
 
     def tune_report(self, precision: int) -> dict:

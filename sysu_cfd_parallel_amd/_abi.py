@@ -74,7 +74,7 @@ class Params(C.Structure):
         ("autotune", C.c_int),
         ("graphs", C.c_int),
         ("solve_energy", C.c_int),
-        ("energy_pad", C.c_int),
+        ("inner_order", C.c_int),
         ("cp", C.c_double),
         ("prl", C.c_double),
         ("prte", C.c_double),

@@ -21,6 +21,7 @@ SOURCES = [
     "mesh/ingest.cpp",
     "mesh/batch.cpp",
     "mesh/gmsh.cpp",
+    "mesh/reorder.cpp",
     "device/fv_kernels.hip",
     "device/keps_kernels.hip",
     "device/transient_kernels.hip",

@@ -908,10 +908,10 @@ int s355_spmv_time(s355_ctx* ctx, int which, int precision, int repeats, double*
 }
 
 int s355_kernel_shapes(s355_ctx* ctx, int lanes[2], int nt[2], int rows[2], int sigma[2], int tuned[2],
-                       int slices[2]) {
+                       int slices[2], int order[2]) {
   return guarded([&] {
-    if (!lanes || !nt || !rows || !sigma || !tuned || !slices) throw std::invalid_argument("null output");
-    dev(ctx).kernel_shapes(lanes, nt, rows, sigma, tuned, slices);
+    if (!lanes || !nt || !rows || !sigma || !tuned || !slices || !order) throw std::invalid_argument("null output");
+    dev(ctx).kernel_shapes(lanes, nt, rows, sigma, tuned, slices, order);
     return S355_OK;
   });
 }

@@ -2,6 +2,7 @@
 // operations of the SIMPLE hot path on one HIP stream.
 #include "context.hpp"
 #include "scratch.hpp"
+#include "../mesh/reorder.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -41,6 +42,7 @@ auto upload(std::vector<void*>& allocs, const Vec& v, hipStream_t s) {
 Context::Context(const s355_mesh_desc* descs, int n_case, const s355_params& params) : par(params), ncase(n_case) {
   if (par.density_model != 0) throw std::invalid_argument("only DensityModel 0 (constant density) is supported");
   if (par.precision != S355_FP64 && par.precision != S355_MIXED) throw std::invalid_argument("bad precision mode");
+  if (par.inner_order < 0 || par.inner_order > 2) throw std::invalid_argument("inner_order: 0, 1 (RCM) or 2 (input)");
   if (!descs || ncase < 1) throw std::invalid_argument("no mesh");
   if (ncase > 1)
     for (int k = 0; k < 3; ++k)
@@ -270,6 +272,9 @@ void Context::set_params(const s355_params& p) {
   if (ncase > 1)
     for (int k = 0; k < 3; ++k)
       if (p.ksp[k].type == S355_KSP_CG) throw std::invalid_argument("a batch of cases solves every system with GMRES");
+  if (p.inner_order < 0 || p.inner_order > 2) throw std::invalid_argument("inner_order: 0, 1 (RCM) or 2 (input)");
+  if (have_order_ && p.inner_order != 0 && p.inner_order != order_kind_)
+    throw std::invalid_argument("inner_order: the other inner order was built already");
   par = p;
   if (par.limiter != 0) ensure_vertices();
 }
@@ -645,9 +650,18 @@ int Context::run_solve(int which, const double* b, double* x, const s355_ksp_opt
     CsrView<float> A{dm.Nc, dm.rowptr, dm.col, dm.diag, a32_, dm.nnz};
     CsrView<double> A64{dm.Nc, dm.rowptr, dm.col, dm.diag, val, dm.nnz};
     const bool tune = par.autotune && o.type == S355_KSP_GMRES && !kwork32_.tuned;
-    if (o.type == S355_KSP_GMRES && (tune || kwork32_.lanes == 0)) prepare_sell(val, true, tune);
-    if (tune) autotune<float>(A, kwork32_, std::min(o.restart, kMaxRestart), stream);
-    r = ir_solve(A64, A, b, x, o, zero_guess, par.ir_max, par.ir_inner_rtol, kwork32_, ir_, stream);
+    if (use_order(o)) {
+      // the Arnoldi cycle on the rows in the inner order, tuned there
+      fill_order(val, o.pc == S355_PC_JACOBI);
+      CsrView<float> Ao{dm.Nc, ord_rowptr_, ord_col_, ord_diag_, a32o_, dm.nnz};
+      if (tune) tune_ordered(Ao, std::min(o.restart, kMaxRestart));
+      KWork w = ordered_work(false);
+      r = ir_solve(A64, Ao, b, x, o, zero_guess, par.ir_max, par.ir_inner_rtol, w, ir_, stream);
+    } else {
+      if (o.type == S355_KSP_GMRES && (tune || kwork32_.lanes == 0)) prepare_sell(val, true, tune);
+      if (tune) autotune<float>(A, kwork32_, std::min(o.restart, kMaxRestart), stream);
+      r = ir_solve(A64, A, b, x, o, zero_guess, par.ir_max, par.ir_inner_rtol, kwork32_, ir_, stream);
+    }
   } else {
     CsrView<double> A{dm.Nc, dm.rowptr, dm.col, dm.diag, val, dm.nnz};
     const bool tune = par.autotune && o.type == S355_KSP_GMRES && !kwork_.tuned;
@@ -841,6 +855,136 @@ void Context::ensure_sell() {
   have_sell_ = true;
 }
 
+bool Context::use_order(const s355_ksp_opts& o) {
+  if (!par.inner_order || par.precision != S355_MIXED || ncase != 1 || o.type != S355_KSP_GMRES ||
+      o.pc == S355_PC_ILU0)
+    return false;
+  ensure_order();
+  return order_ok_;
+}
+
+// The rows in the inner order -- 1: reverse Cuthill-McKee order of the
+// pattern; 2: the order of the cells' elements in the input stream, before
+// the partition sort (mesh generators write cells in a spatially coherent
+// order, which the sort of equal keys scatters) -- then within windows of
+// kSellSigma[1] positions by decreasing row length (stable), as a CSR by
+// position (columns as positions, each row's entries by ascending column)
+// and its SELL-64 copies with the columns as 16-bit offsets from the row,
+// built once.  The windows are the SELL copies' own, so their slot rows are
+// the positions themselves (no row map) except for the widest window.
+// Ascending columns make the j-th entries of neighbouring rows neighbours
+// too, so a wave's gathers of x for one slot fall in few cache lines; each
+// row's sum runs in that order (the inner cycle is FP32 work inside FP64
+// refinement, checked against the FP64 residual).
+void Context::ensure_order() {
+  if (have_order_) return;
+  have_order_ = true;
+  order_kind_ = par.inner_order;
+  const int n = dm.Nc;
+  std::vector<int> row;
+  if (par.inner_order == 2) {
+    row.resize(n);
+    for (int p = 0; p < n; ++p) row[p] = p;
+    std::sort(row.begin(), row.end(), [&](int a, int b) { return mesh.c_elem[a] < mesh.c_elem[b]; });
+  } else {
+    row = rcm_order(n, mesh.rowptr.data(), mesh.col.data());
+  }
+  auto len = [&](int r) { return mesh.rowptr[r + 1] - mesh.rowptr[r]; };
+  const int window = kSellSigma[1];
+  for (int w0 = 0; w0 < n; w0 += window)
+    std::stable_sort(row.begin() + w0, row.begin() + std::min(n, w0 + window),
+                     [&](int a, int b) { return len(a) > len(b); });
+  std::vector<int> pos(n);
+  for (int p = 0; p < n; ++p) pos[row[p]] = p;
+  const size_t nnz = mesh.col.size();
+  std::vector<int> rp(n + 1, 0), col(nnz), dg(n, -1), src(nnz);
+  size_t k = 0;
+  std::vector<std::pair<int, int>> ent;
+  for (int p = 0; p < n; ++p) {
+    const int r = row[p];
+    ent.clear();
+    for (int q = mesh.rowptr[r]; q < mesh.rowptr[r + 1]; ++q) ent.emplace_back(pos[mesh.col[q]], q);
+    std::sort(ent.begin(), ent.end());
+    for (const auto& e : ent) {
+      col[k] = e.first;
+      src[k] = e.second;
+      if (e.first == p) dg[p] = static_cast<int>(k);
+      ++k;
+    }
+    rp[p + 1] = static_cast<int>(k);
+  }
+  std::vector<int> ptr[kSellVariants], srow[kSellVariants], ssrc[kSellVariants];
+  std::vector<int16_t> off[kSellVariants];
+  bool ident[kSellVariants];
+  for (int v = 0; v < kSellVariants; ++v) {
+    std::vector<int> scol;
+    sell_pattern(n, rp.data(), col.data(), kSellSigma[v], ptr[v], srow[v], scol, ssrc[v]);
+    if (!sell_offsets16(n, ptr[v], srow[v], scol, off[v])) return;  // stays natural
+    ident[v] = true;
+    for (int p = 0; p < n && ident[v]; ++p) ident[v] = srow[v][p] == p;
+  }
+  ord_pos_ = upload(allocs_, pos, stream);
+  ord_row_ = upload(allocs_, row, stream);
+  ord_rowptr_ = upload(allocs_, rp, stream);
+  ord_col_ = upload(allocs_, col, stream);
+  ord_diag_ = upload(allocs_, dg, stream);
+  ord_src_ = upload(allocs_, src, stream);
+  a32o_ = alloc<float>(nnz);
+  for (int v = 0; v < kSellVariants; ++v) {
+    SellPat pt;
+    pt.slots = ssrc[v].size();
+    pt.ptr = upload(allocs_, ptr[v], stream);
+    pt.row = ident[v] ? nullptr : upload(allocs_, srow[v], stream);
+    pt.col16 = upload(allocs_, off[v], stream);
+    pt.src = upload(allocs_, ssrc[v], stream);  // positions in the ordered CSR
+    osell_[v] = pt;
+  }
+  sync();  // host staging vectors go out of scope
+  order_ok_ = true;
+}
+
+// The FP32 values of the ordered CSR from the FP64 matrix, with the Jacobi
+// scaling folded in when that is the preconditioner
+void Context::fill_order(const double* csr_val, bool jacobi) {
+  fill_ordered(dm.Nc, ord_rowptr_, ord_src_, ord_row_, dm.diag, csr_val, jacobi, a32o_, stream);
+  order_jacobi_ = jacobi;
+}
+
+// The FP32 workspace of a mixed GMRES solve in the inner order (after
+// fill_order): the shapes of the FP32 workspace, the ordered SELL copies (of
+// the window in use, or all of them) filled from the ordered CSR, and the
+// position maps.
+KWork Context::ordered_work(bool all) {
+  KWork w = kwork32_;
+  for (int v = 0; v < kSellVariants; ++v) {
+    w.sell[v] = osell_[v];
+    w.sell_vals[v] = nullptr;
+    if (all || (w.lanes == 0 && v == w.sell_v)) {
+      if (!osell32_[v]) osell32_[v] = alloc<float>(osell_[v].slots);
+      sell_fill<float, float>(osell_[v].src, osell_[v].slots, a32o_, osell32_[v], stream);
+      w.sell_vals[v] = osell32_[v];
+    }
+  }
+  w.ord_pos = ord_pos_;
+  w.ord_row = ord_row_;
+  w.pc_folded = order_jacobi_;
+  return w;
+}
+
+// The autotuner on the ordered system; the shapes it picks become the FP32
+// workspace's.
+void Context::tune_ordered(const CsrView<float>& Ao, int m) {
+  KWork w = ordered_work(true);
+  autotune<float>(Ao, w, m, stream);
+  kwork32_.lanes = w.lanes;
+  kwork32_.nt = w.nt;
+  kwork32_.rows = w.rows;
+  kwork32_.sell_v = w.sell_v;
+  kwork32_.sell2 = w.sell2;
+  kwork32_.report = w.report;
+  kwork32_.tuned = true;
+}
+
 // Values of the SELL copies from the CSR values: of the window in use, or of
 // every window (before the autotuner times them).
 void Context::prepare_sell(const double* csr_val, bool fp32, bool all) {
@@ -883,14 +1027,29 @@ double Context::time_spmv(int which, int precision, int repeats) {
       ensure_mixed();
       convert_to_float(val, a32_, dm.nnz, stream);
       CsrView<float> A{dm.Nc, dm.rowptr, dm.col, dm.diag, a32_, dm.nnz};
-      prepare_sell(val, true, par.autotune && !kwork32_.tuned);
-      if (par.autotune && !kwork32_.tuned) autotune<float>(A, kwork32_, m, stream);
       float* inv = static_cast<float*>(kwork32_.pcinv);
-      jacobi_inverse<float>(A, inv, stream);
       float* xf = static_cast<float*>(kwork32_.V);
       float* yf = xf + dm.Nc;
-      convert_to_float(df.U, xf, dm.Nc, stream);
-      ms = bench_spmv<float>(A, xf, yf, inv, kwork32_, repeats, flush, flush_bytes, stream);
+      s355_ksp_opts o = par.ksp[S355_KSP_PRESSURE];
+      o.type = S355_KSP_GMRES;
+      const bool tune = par.autotune && !kwork32_.tuned;
+      if (use_order(o)) {
+        // the product the pressure solve runs: the inner order's, its
+        // Jacobi scaling in the matrix
+        fill_order(val, o.pc == S355_PC_JACOBI);
+        CsrView<float> Ao{dm.Nc, ord_rowptr_, ord_col_, ord_diag_, a32o_, dm.nnz};
+        if (tune) tune_ordered(Ao, m);
+        KWork w = ordered_work(false);
+        convert_to_float(df.U, xf, dm.Nc, stream);
+        jacobi_inverse<float>(Ao, inv, stream);
+        ms = bench_spmv<float>(Ao, xf, yf, w.pc_folded ? nullptr : inv, w, repeats, flush, flush_bytes, stream);
+      } else {
+        prepare_sell(val, true, tune);
+        if (tune) autotune<float>(A, kwork32_, m, stream);
+        convert_to_float(df.U, xf, dm.Nc, stream);
+        jacobi_inverse<float>(A, inv, stream);
+        ms = bench_spmv<float>(A, xf, yf, inv, kwork32_, repeats, flush, flush_bytes, stream);
+      }
     }
   } catch (...) {
     hipStreamSynchronize(stream);
@@ -947,7 +1106,12 @@ void Context::set_kernel_shapes(int precision, int lanes, int nt, int rows, int 
   w.tuned = true;  // the autotuner leaves explicit shapes alone
 }
 
-void Context::kernel_shapes(int lanes[2], int nt[2], int rows[2], int sigma[2], int tuned[2], int slices[2]) {
+void Context::kernel_shapes(int lanes[2], int nt[2], int rows[2], int sigma[2], int tuned[2], int slices[2],
+                            int order[2]) {
+  order[0] = 0;
+  s355_ksp_opts o = par.ksp[S355_KSP_PRESSURE];
+  o.type = S355_KSP_GMRES;
+  order[1] = use_order(o) ? 1 : 0;
   const KWork* w[2] = {&kwork_, &kwork32_};
   const size_t elem[2] = {sizeof(double), sizeof(float)};
   for (int k = 0; k < 2; ++k) {

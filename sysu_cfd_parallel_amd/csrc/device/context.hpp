@@ -86,7 +86,7 @@ class Context {
   // benchmark helper
   double time_spmv(int which, int precision, int repeats);
   double time_copy(size_t bytes, int repeats);
-  void kernel_shapes(int lanes[2], int nt[2], int rows[2], int sigma[2], int tuned[2], int slices[2]);
+  void kernel_shapes(int lanes[2], int nt[2], int rows[2], int sigma[2], int tuned[2], int slices[2], int order[2]);
   void set_kernel_shapes(int precision, int lanes, int nt, int rows, int sigma);
   // mode 1: start timing the Arnoldi SpMV launches of one precision (reset),
   // 0: read, -1: read and stop; returns the mean ms and the launches timed
@@ -109,6 +109,13 @@ class Context {
   void grad(int nf, const double* const* phi, const double* const* bphi, double* const* out);
   void ensure_vertices();
   void ensure_transient();
+  // the inner order of mixed GMRES (params.inner_order): false when it does
+  // not apply or a 16-bit offset does not fit
+  bool use_order(const s355_ksp_opts& o);
+  void ensure_order();
+  void fill_order(const double* csr_val, bool jacobi);
+  KWork ordered_work(bool all);
+  void tune_ordered(const CsrView<float>& Ao, int m);
 
   std::vector<void*> allocs_;
   double* partial_ = nullptr;
@@ -126,6 +133,16 @@ class Context {
   float* a32_ = nullptr;
   KWork kwork32_{};
   IrWork ir_{};
+  // inner order of mixed GMRES: rows in reverse Cuthill-McKee order
+  bool have_order_ = false, order_ok_ = false;
+  int order_kind_ = 0;  // params.inner_order the order was built for
+  bool order_jacobi_ = false;  // a32o_ holds M^-1 A (Jacobi folded in)
+  const int *ord_pos_ = nullptr, *ord_row_ = nullptr;  // natural row -> position, position -> natural row
+  const int *ord_rowptr_ = nullptr, *ord_col_ = nullptr, *ord_diag_ = nullptr;  // the CSR by position
+  const int* ord_src_ = nullptr;  // natural CSR position of each entry of the ordered CSR
+  float* a32o_ = nullptr;         // FP32 values of the ordered CSR
+  SellPat osell_[kSellVariants];  // SELL-64 copies of the ordered CSR, 16-bit column offsets
+  float* osell32_[kSellVariants] = {nullptr, nullptr, nullptr};
   double* scratch_ = nullptr;  // this device's fixed scratch array (scratch.hpp)
   std::unique_ptr<GraphCache> graphs_;  // captured GMRES cycles (params.graphs)
   std::unique_ptr<SpmvTimer> timers_[2];  // in-solve SpMV timing, FP64 / FP32

@@ -348,19 +348,29 @@ constexpr int kSellBatch = 8;
 // a batch is issued, then every gather of x, then the products are added in
 // slot order.  The first batch loads before the state check, in parallel
 // with the row id and the state, which it does not depend on.
-template <class T, bool NT, bool PC, int W>
+// With C16 the columns are 16-bit offsets from the row (SellView::col16):
+// half the column bytes, added to the row id once both have loaded.
+template <bool NT, bool C16, class V>
+__device__ __forceinline__ int sell_col(const V& A, size_t q, int row) {
+  if constexpr (C16)
+    return row + static_cast<int>(ld_stream<int16_t, NT>(A.col16 + q));
+  else
+    return ld_stream<int, NT>(A.col + q);
+}
+
+template <class T, bool NT, bool PC, int W, bool C16>
 __device__ __forceinline__ void sell_row(const SellView<T>& A, const Tiles& t, const T* __restrict__ x,
                                          const T* __restrict__ inv, T* __restrict__ y, const KState* st, int it,
                                          int sl, int pos, int qa, int q1) {
   constexpr int B = W > 0 ? W : kSellBatch;
-  const int row = A.row[pos];
+  const int row = A.row ? A.row[pos] : pos;
   int q0 = qa + (pos & 63);
   T v[B], xv[B];
   int c[B];
 #pragma unroll
   for (int u = 0; u < B; ++u) {
     v[u] = ld_stream<T, NT>(A.val + q0 + 64 * u);
-    c[u] = ld_stream<int, NT>(A.col + q0 + 64 * u);
+    c[u] = sell_col<NT, C16>(A, q0 + 64 * u, row);
   }
   // the case: one case without a slice table; in batches per slice when the
   // slice lies in one case, else by row
@@ -387,7 +397,7 @@ __device__ __forceinline__ void sell_row(const SellView<T>& A, const Tiles& t, c
       for (int u = 0; u < kSellBatch; ++u)
         if (q0 + 64 * u < q1) {
           v[u] = ld_stream<T, NT>(A.val + q0 + 64 * u);
-          c[u] = ld_stream<int, NT>(A.col + q0 + 64 * u);
+          c[u] = sell_col<NT, C16>(A, q0 + 64 * u, row);
         }
 #pragma unroll
       for (int u = 0; u < kSellBatch; ++u)
@@ -403,24 +413,24 @@ __device__ __forceinline__ void sell_row(const SellView<T>& A, const Tiles& t, c
 }
 
 // one row of slice sl by the slice's width (wave-uniform)
-template <class T, bool NT, bool PC>
+template <class T, bool NT, bool PC, bool C16>
 __device__ __forceinline__ void sell_row_any(const SellView<T>& A, const Tiles& t, const T* __restrict__ x,
                                              const T* __restrict__ inv, T* __restrict__ y, const KState* st, int it,
                                              int sl, int pos, int qa, int q1) {
   switch ((q1 - qa) >> 6) {
-    case 1: sell_row<T, NT, PC, 1>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
-    case 2: sell_row<T, NT, PC, 2>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
-    case 3: sell_row<T, NT, PC, 3>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
-    case 4: sell_row<T, NT, PC, 4>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
-    case 5: sell_row<T, NT, PC, 5>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
-    case 6: sell_row<T, NT, PC, 6>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
-    case 7: sell_row<T, NT, PC, 7>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
-    case 8: sell_row<T, NT, PC, 8>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
-    default: sell_row<T, NT, PC, 0>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
+    case 1: sell_row<T, NT, PC, 1, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
+    case 2: sell_row<T, NT, PC, 2, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
+    case 3: sell_row<T, NT, PC, 3, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
+    case 4: sell_row<T, NT, PC, 4, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
+    case 5: sell_row<T, NT, PC, 5, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
+    case 6: sell_row<T, NT, PC, 6, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
+    case 7: sell_row<T, NT, PC, 7, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
+    case 8: sell_row<T, NT, PC, 8, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
+    default: sell_row<T, NT, PC, 0, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
   }
 }
 
-template <class T, bool NT, bool PC>
+template <class T, bool NT, bool PC, bool C16 = false>
 __global__ void __launch_bounds__(kBlock) k_spmv_sell(SellView<T> A, Tiles t, const T* __restrict__ x,
                                                       const T* __restrict__ inv, T* __restrict__ y, const KState* st,
                                                       int it) {
@@ -429,7 +439,7 @@ __global__ void __launch_bounds__(kBlock) k_spmv_sell(SellView<T> A, Tiles t, co
   // a wave is one slice (64 consecutive positions), so the slice index and
   // width are wave-uniform and the slice bounds load through the scalar cache
   const int sl = __builtin_amdgcn_readfirstlane(pos >> 6);
-  sell_row_any<T, NT, PC>(A, t, x, inv, y, st, it, sl, pos, A.ptr[sl], A.ptr[sl + 1]);
+  sell_row_any<T, NT, PC, C16>(A, t, x, inv, y, st, it, sl, pos, A.ptr[sl], A.ptr[sl + 1]);
 }
 
 // Two adjacent slices per wave, one row of each per lane (one case, FP32 on
@@ -437,15 +447,15 @@ __global__ void __launch_bounds__(kBlock) k_spmv_sell(SellView<T> A, Tiles t, co
 // Both rows run the one-slice arithmetic: W is the wider slice's width, the
 // narrower slice's slots past its width reload the lane's first slot and are
 // never added (adjacent slices of a sorted window rarely differ in width).
-template <class T, bool NT, bool PC, int W>
+template <class T, bool NT, bool PC, int W, bool C16>
 __device__ __forceinline__ void sell_rows2(const SellView<T>& A, const T* __restrict__ x, const T* __restrict__ inv,
                                            T* __restrict__ y, const KState* st, int it, int sl, int lane, int qa0,
                                            int qa1, int q11) {
   const int w0 = (qa1 - qa0) >> 6, w1 = (q11 - qa1) >> 6;
   const int pos0 = sl * 64 + lane, pos1 = pos0 + 64;
   const bool ok1 = pos1 < A.n;
-  const int row0 = A.row[pos0];
-  const int row1 = A.row[ok1 ? pos1 : pos0];
+  const int row0 = A.row ? A.row[pos0] : pos0;
+  const int row1 = ok1 ? (A.row ? A.row[pos1] : pos1) : row0;
   T v0[W], v1[W], x0[W], x1[W];
   int c0[W], c1[W];
 #pragma unroll
@@ -453,9 +463,9 @@ __device__ __forceinline__ void sell_rows2(const SellView<T>& A, const T* __rest
     const int qa = qa0 + lane + (u < w0 ? 64 * u : 0);
     const int qb = qa1 + lane + (u < w1 ? 64 * u : 0);
     v0[u] = ld_stream<T, NT>(A.val + qa);
-    c0[u] = ld_stream<int, NT>(A.col + qa);
+    c0[u] = sell_col<NT, C16>(A, qa, row0);
     v1[u] = ld_stream<T, NT>(A.val + qb);
-    c1[u] = ld_stream<int, NT>(A.col + qb);
+    c1[u] = sell_col<NT, C16>(A, qb, row1);
   }
   if (st->reason) return;
   const double scl = st->scale[it];
@@ -484,7 +494,7 @@ __device__ __forceinline__ void sell_rows2(const SellView<T>& A, const T* __rest
   if (ok1) y[row1] = static_cast<T>(z1 * scl);
 }
 
-template <class T, bool NT, bool PC>
+template <class T, bool NT, bool PC, bool C16 = false>
 __global__ void __launch_bounds__(kBlock) k_spmv_sell2(SellView<T> A, Tiles t, const T* __restrict__ x,
                                                        const T* __restrict__ inv, T* __restrict__ y, const KState* st,
                                                        int it) {
@@ -496,26 +506,26 @@ __global__ void __launch_bounds__(kBlock) k_spmv_sell2(SellView<T> A, Tiles t, c
   if (sl + 1 >= nsl) {
     // an odd last slice: the one-slice body
     const int pos = sl * 64 + lane;
-    if (pos < A.n) sell_row_any<T, NT, PC>(A, t, x, inv, y, st, it, sl, pos, A.ptr[sl], A.ptr[sl + 1]);
+    if (pos < A.n) sell_row_any<T, NT, PC, C16>(A, t, x, inv, y, st, it, sl, pos, A.ptr[sl], A.ptr[sl + 1]);
     return;
   }
   const int qa0 = A.ptr[sl], qa1 = A.ptr[sl + 1], q11 = A.ptr[sl + 2];
   const int wmax = max((qa1 - qa0) >> 6, (q11 - qa1) >> 6);
   switch (wmax) {
-    case 1: sell_rows2<T, NT, PC, 1>(A, x, inv, y, st, it, sl, lane, qa0, qa1, q11); break;
-    case 2: sell_rows2<T, NT, PC, 2>(A, x, inv, y, st, it, sl, lane, qa0, qa1, q11); break;
-    case 3: sell_rows2<T, NT, PC, 3>(A, x, inv, y, st, it, sl, lane, qa0, qa1, q11); break;
-    case 4: sell_rows2<T, NT, PC, 4>(A, x, inv, y, st, it, sl, lane, qa0, qa1, q11); break;
-    case 5: sell_rows2<T, NT, PC, 5>(A, x, inv, y, st, it, sl, lane, qa0, qa1, q11); break;
-    case 6: sell_rows2<T, NT, PC, 6>(A, x, inv, y, st, it, sl, lane, qa0, qa1, q11); break;
-    case 7: sell_rows2<T, NT, PC, 7>(A, x, inv, y, st, it, sl, lane, qa0, qa1, q11); break;
-    case 8: sell_rows2<T, NT, PC, 8>(A, x, inv, y, st, it, sl, lane, qa0, qa1, q11); break;
+    case 1: sell_rows2<T, NT, PC, 1, C16>(A, x, inv, y, st, it, sl, lane, qa0, qa1, q11); break;
+    case 2: sell_rows2<T, NT, PC, 2, C16>(A, x, inv, y, st, it, sl, lane, qa0, qa1, q11); break;
+    case 3: sell_rows2<T, NT, PC, 3, C16>(A, x, inv, y, st, it, sl, lane, qa0, qa1, q11); break;
+    case 4: sell_rows2<T, NT, PC, 4, C16>(A, x, inv, y, st, it, sl, lane, qa0, qa1, q11); break;
+    case 5: sell_rows2<T, NT, PC, 5, C16>(A, x, inv, y, st, it, sl, lane, qa0, qa1, q11); break;
+    case 6: sell_rows2<T, NT, PC, 6, C16>(A, x, inv, y, st, it, sl, lane, qa0, qa1, q11); break;
+    case 7: sell_rows2<T, NT, PC, 7, C16>(A, x, inv, y, st, it, sl, lane, qa0, qa1, q11); break;
+    case 8: sell_rows2<T, NT, PC, 8, C16>(A, x, inv, y, st, it, sl, lane, qa0, qa1, q11); break;
     default: {
       // a slice wider than one batch: each slice with the one-slice body
       // for its own width
       for (int k = 0; k < 2; ++k) {
         const int s2 = sl + k, pos = s2 * 64 + lane;
-        if (pos < A.n) sell_row_any<T, NT, PC>(A, t, x, inv, y, st, it, s2, pos, A.ptr[s2], A.ptr[s2 + 1]);
+        if (pos < A.n) sell_row_any<T, NT, PC, C16>(A, t, x, inv, y, st, it, s2, pos, A.ptr[s2], A.ptr[s2 + 1]);
       }
       break;
     }
@@ -549,6 +559,38 @@ void launch_spmv_l(bool nt, const CsrView<T>& A, const Tiles& t, const T* x, con
     hipLaunchKernelGGL((k_spmv_pc<T, L, false>), dim3(g), dim3(kBlock), 0, s, A, t, x, inv, y, st, it);
 }
 
+// the SELL-64 product: two slices per wave when the view says so (FP32, one
+// case), else one
+template <class T, bool NT, bool PC, bool C16>
+void launch_sell(const SellView<T>& sell, const Tiles& t, const T* x, const T* inv, T* y, const KState* st, int it,
+                 hipStream_t s) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (sell.two) {
+      // two slices per wave: kBlock / kWave waves per block
+      const int nsl = (sell.n + 63) / 64;
+      const int g2 = grid_for((nsl + 1) / 2, kBlock / kWave);
+      hipLaunchKernelGGL((k_spmv_sell2<T, NT, PC, C16>), dim3(g2), dim3(kBlock), 0, s, sell, t, x, inv, y, st, it);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_spmv_sell<T, NT, PC, C16>), dim3(grid_for(sell.n)), dim3(kBlock), 0, s, sell, t, x, inv, y,
+                     st, it);
+}
+
+template <class T, bool NT, bool PC>
+void launch_sell_c(const SellView<T>& sell, const Tiles& t, const T* x, const T* inv, T* y, const KState* st, int it,
+                   hipStream_t s) {
+  if (sell.col16) {
+    // 16-bit column offsets exist for the FP32 inner order only
+    if constexpr (std::is_same<T, float>::value)
+      launch_sell<T, NT, PC, true>(sell, t, x, inv, y, st, it, s);
+    else
+      throw std::logic_error("16-bit SELL columns in an FP64 product");
+  } else {
+    launch_sell<T, NT, PC, false>(sell, t, x, inv, y, st, it, s);
+  }
+}
+
 // y = scale M^-1 A x with the CSR kernel at L lanes per row, or with the
 // SELL-64 copy for L = 0 (sell must then be set)
 template <class T>
@@ -557,34 +599,15 @@ void launch_spmv(int L, int nt, const CsrView<T>& A, const SellView<T>& sell, co
   switch (L) {
     case 0: {
       if (!sell.ptr) throw std::logic_error("SELL-64 SpMV without a SELL copy");
-      if constexpr (std::is_same<T, float>::value) {
-        if (sell.two) {
-          // two slices per wave: kBlock / kWave waves per block
-          const int nsl = (A.n + 63) / 64;
-          const int g2 = grid_for((nsl + 1) / 2, kBlock / kWave);
-          if (nt && inv)
-            hipLaunchKernelGGL((k_spmv_sell2<T, true, true>), dim3(g2), dim3(kBlock), 0, s, sell, t, x, inv, y, st, it);
-          else if (nt)
-            hipLaunchKernelGGL((k_spmv_sell2<T, true, false>), dim3(g2), dim3(kBlock), 0, s, sell, t, x, inv, y, st,
-                               it);
-          else if (inv)
-            hipLaunchKernelGGL((k_spmv_sell2<T, false, true>), dim3(g2), dim3(kBlock), 0, s, sell, t, x, inv, y, st,
-                               it);
-          else
-            hipLaunchKernelGGL((k_spmv_sell2<T, false, false>), dim3(g2), dim3(kBlock), 0, s, sell, t, x, inv, y, st,
-                               it);
-          break;
-        }
-      }
-      const int g = grid_for(A.n);
+      if (sell.n != A.n) throw std::logic_error("SELL-64 copy of another size");
       if (nt && inv)
-        hipLaunchKernelGGL((k_spmv_sell<T, true, true>), dim3(g), dim3(kBlock), 0, s, sell, t, x, inv, y, st, it);
+        launch_sell_c<T, true, true>(sell, t, x, inv, y, st, it, s);
       else if (nt)
-        hipLaunchKernelGGL((k_spmv_sell<T, true, false>), dim3(g), dim3(kBlock), 0, s, sell, t, x, inv, y, st, it);
+        launch_sell_c<T, true, false>(sell, t, x, inv, y, st, it, s);
       else if (inv)
-        hipLaunchKernelGGL((k_spmv_sell<T, false, true>), dim3(g), dim3(kBlock), 0, s, sell, t, x, inv, y, st, it);
+        launch_sell_c<T, false, true>(sell, t, x, inv, y, st, it, s);
       else
-        hipLaunchKernelGGL((k_spmv_sell<T, false, false>), dim3(g), dim3(kBlock), 0, s, sell, t, x, inv, y, st, it);
+        launch_sell_c<T, false, false>(sell, t, x, inv, y, st, it, s);
       break;
     }
     case 1: launch_spmv_l<T, 1>(nt != 0, A, t, x, inv, y, st, it, s); break;
@@ -604,6 +627,7 @@ SellView<T> sell_of(const KWork& w, int n, int variant = -1) {
     v.ptr = w.sell[sv].ptr;
     v.row = w.sell[sv].row;
     v.col = w.sell[sv].col;
+    v.col16 = w.sell[sv].col16;
     v.val = static_cast<const T*>(w.sell_vals[sv]);
     v.scase = w.sell[sv].scase;
     v.two = std::is_same<T, float>::value && !v.scase && (w.sell2 > 0 || (w.sell2 < 0 && n >= (1 << 20))) ? 1 : 0;
@@ -956,9 +980,10 @@ __global__ void __launch_bounds__(kWave) k_build_y(KState* st, int m) {
   if (lane == 0) *nupd = k + 1;
 }
 
+// x += sum_k y_k V_k; with xrow (inner order) position i updates x[xrow[i]]
 template <class T, class X, int R>
 __global__ void __launch_bounds__(kBlock) k_update_x(const T* __restrict__ V, size_t ld, Tiles t, X* __restrict__ x,
-                                                     const KState* st) {
+                                                     const KState* st, const int* __restrict__ xrow) {
   using S = GsShape<T, R>;
   int a, e, c;
   t.range(blockIdx.x, a, e, c);
@@ -984,7 +1009,10 @@ __global__ void __launch_bounds__(kBlock) k_update_x(const T* __restrict__ V, si
     }
 #pragma unroll
     for (int u = 0; u < S::VW; ++u)
-      if (i0 + u < end) x[i0 + u] = static_cast<X>(static_cast<double>(x[i0 + u]) + sacc[u]);
+      if (i0 + u < end) {
+        const size_t r = xrow ? static_cast<size_t>(xrow[i0 + u]) : i0 + u;
+        x[r] = static_cast<X>(static_cast<double>(x[r]) + sacc[u]);
+      }
   }
 }
 
@@ -993,11 +1021,12 @@ __global__ void __launch_bounds__(kBlock) k_update_x(const T* __restrict__ V, si
 // mixed solve, whose Arnoldi cycle runs on an FP32 copy); v0 is stored in the
 // basis type T and the norm is taken before that rounding when T and X
 // differ.  Tiles of finished cases are skipped.
+// With vpos (inner order) row i's value goes to v0[vpos[i]].
 template <class T, class X>
 __global__ void __launch_bounds__(kBlock) k_residual_pc(CsrView<X> A, Tiles t, const X* __restrict__ b,
                                                         const X* __restrict__ x, const X* __restrict__ inv,
                                                         T* __restrict__ v0, double* partial, int zero_guess,
-                                                        const KState* st) {
+                                                        const KState* st, const int* __restrict__ vpos) {
   int a, e, c;
   t.range(blockIdx.x, a, e, c);
   if (st[c].done) return;
@@ -1011,7 +1040,7 @@ __global__ void __launch_bounds__(kBlock) k_residual_pc(CsrView<X> A, Tiles t, c
     }
     if (inv) r *= static_cast<double>(inv[i]);
     const X rx = static_cast<X>(r);
-    v0[i] = static_cast<T>(rx);
+    v0[vpos ? vpos[i] : i] = static_cast<T>(rx);
     nrm += static_cast<double>(rx) * static_cast<double>(rx);
   }
   nrm = block_sum(nrm, lds);
@@ -1020,13 +1049,15 @@ __global__ void __launch_bounds__(kBlock) k_residual_pc(CsrView<X> A, Tiles t, c
 
 // ---- Jacobi inverses of the mixed solve: FP64 for the restart residual,
 // FP32 for the Arnoldi cycle, both from the FP64 diagonal
-__global__ void __launch_bounds__(kBlock) k_jacobi_mixed(CsrView<double> A, double* inv64, float* inv32) {
+// (inv32 by position in the inner order when pos32 is set)
+__global__ void __launch_bounds__(kBlock) k_jacobi_mixed(CsrView<double> A, double* inv64, float* inv32,
+                                                         const int* __restrict__ pos32) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
   const double d = A.val[A.diag[i]];
   const double v = d != 0.0 ? 1.0 / d : 1.0;
   inv64[i] = v;
-  inv32[i] = static_cast<float>(v);
+  inv32[pos32 ? pos32[i] : i] = static_cast<float>(v);
 }
 
 // ---- partial ||M^-1 b||^2 per tile (reference norm for a nonzero guess)
@@ -1326,8 +1357,9 @@ struct Dispatch {
     hipLaunchKernelGGL((k_maxpy_norm<T, R>), dim3(t.ntile), dim3(kBlock), 0, s, V, ld, t, it, partial, st);
   }
   template <int R, class X>
-  static void update_r(const T* V, size_t ld, const Tiles& t, X* x, const KState* st, hipStream_t s) {
-    hipLaunchKernelGGL((k_update_x<T, X, R>), dim3(t.ntile), dim3(kBlock), 0, s, V, ld, t, x, st);
+  static void update_r(const T* V, size_t ld, const Tiles& t, X* x, const KState* st, const int* xrow,
+                       hipStream_t s) {
+    hipLaunchKernelGGL((k_update_x<T, X, R>), dim3(t.ntile), dim3(kBlock), 0, s, V, ld, t, x, st, xrow);
   }
   static void mdot(int R, const T* V, size_t ld, const Tiles& t, int it, double* partial, int pstride,
                    const KState* st, hipStream_t s) {
@@ -1350,13 +1382,14 @@ struct Dispatch {
     }
   }
   template <class X>
-  static void update(int R, const T* V, size_t ld, const Tiles& t, X* x, const KState* st, hipStream_t s) {
+  static void update(int R, const T* V, size_t ld, const Tiles& t, X* x, const KState* st, const int* xrow,
+                     hipStream_t s) {
     switch (R) {
-      case 16: update_r<16>(V, ld, t, x, st, s); break;
-      case 8: update_r<8>(V, ld, t, x, st, s); break;
-      case 4: update_r<4>(V, ld, t, x, st, s); break;
-      case 2: update_r<2>(V, ld, t, x, st, s); break;
-      default: update_r<1>(V, ld, t, x, st, s); break;
+      case 16: update_r<16>(V, ld, t, x, st, xrow, s); break;
+      case 8: update_r<8>(V, ld, t, x, st, xrow, s); break;
+      case 4: update_r<4>(V, ld, t, x, st, xrow, s); break;
+      case 2: update_r<2>(V, ld, t, x, st, xrow, s); break;
+      default: update_r<1>(V, ld, t, x, st, xrow, s); break;
     }
   }
 };
@@ -1442,6 +1475,12 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
   const Tiles& tl = w.tiles[log2_r(R)];
   if (tl.tile != kBlock * R || tl.n != n) throw std::runtime_error("gmres: tile table missing");
   const int nc = w.ncase;
+  // the inner order (mixed, one case, no ILU): the FP32 basis, the matrix A
+  // and its SELL copies are by position; the restart residual goes to its
+  // positions and the correction back to the natural rows
+  const bool ordered = w.ord_pos != nullptr;
+  if (ordered && (!mixed || nc != 1 || o.pc == S355_PC_ILU0 || !w.ord_row))
+    throw std::invalid_argument("gmres: the inner order serves mixed solves of one case without ILU(0)");
   // ILU sweeps skip their work only for a single system
   const KState* ilu_st = nc == 1 ? w.st : nullptr;
   T* V = static_cast<T*>(w.V);
@@ -1453,8 +1492,9 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
     inv = static_cast<T*>(w.pcinv);
     if constexpr (mixed) {
       if (!inv_x) throw std::invalid_argument("gmres: mixed Jacobi needs an FP64 inverse buffer");
-      hipLaunchKernelGGL(k_jacobi_mixed, dim3(grid_for(n)), dim3(kBlock), 0, s, Ares, inv_x, inv);
+      hipLaunchKernelGGL(k_jacobi_mixed, dim3(grid_for(n)), dim3(kBlock), 0, s, Ares, inv_x, inv, w.ord_pos);
       invr = inv_x;
+      if (w.pc_folded) inv = nullptr;  // A is M^-1 A already
     } else {
       hipLaunchKernelGGL((k_jacobi_setup<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, A, inv);
       invr = inv;
@@ -1491,13 +1531,14 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
   auto enqueue_restart = [&](bool first_c, bool gz_c) {
     if (lu) {
       hipLaunchKernelGGL((k_residual_pc<T, X>), dim3(tl.ntile), dim3(kBlock), 0, s, Ares, tl, b, x,
-                         static_cast<const X*>(nullptr), tmp, w.partial, gz_c ? 1 : 0, w.st);
+                         static_cast<const X*>(nullptr), tmp, w.partial, gz_c ? 1 : 0, w.st,
+                         static_cast<const int*>(nullptr));
       ilu_apply(A, *w.ilu, lu, tmp, V, ilu_st, s, w.graphs, &w);
       hipLaunchKernelGGL((k_pc_norm<T>), dim3(tl.ntile), dim3(kBlock), 0, s, tl, V, static_cast<const T*>(nullptr),
                          w.partial, w.st);
     } else {
       hipLaunchKernelGGL((k_residual_pc<T, X>), dim3(tl.ntile), dim3(kBlock), 0, s, Ares, tl, b, x, invr, V,
-                         w.partial, gz_c ? 1 : 0, w.st);
+                         w.partial, gz_c ? 1 : 0, w.st, w.ord_pos);
     }
     hipLaunchKernelGGL(k_restart_finalize, dim3(nc), dim3(kBlock), 0, s, w.partial, tl, w.st, first_c ? 1 : 0,
                        zero_guess ? 1 : 0, mixed ? 1 : 0);
@@ -1522,7 +1563,7 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
   };
   auto enqueue_end = [&] {
     hipLaunchKernelGGL(k_build_y, dim3(nc), dim3(kWave), 0, s, w.st, m);
-    Dispatch<T>::update(R, V, ld, tl, x, w.st, s);
+    Dispatch<T>::update(R, V, ld, tl, x, w.st, w.ord_row, s);
     hipLaunchKernelGGL(k_cycle_end, dim3(grid_for(nc)), dim3(kBlock), 0, s, w.st, nc, mixed ? 1 : 0);
     S355_LAUNCH_CHECK();
   };
@@ -1540,6 +1581,8 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
                                           static_cast<uintptr_t>(L), static_cast<uintptr_t>(w.nt),
                                           static_cast<uintptr_t>(w.sell_v),
                                           static_cast<uintptr_t>(w.sell2 + 1), sizeof(T), sizeof(X),
+                                          reinterpret_cast<uintptr_t>(w.ord_row),
+                                          reinterpret_cast<uintptr_t>(sell.col16),
                                           static_cast<uintptr_t>(first_c), static_cast<uintptr_t>(zero_guess)};
       hipGraphExec_t g = w.graphs->get(key, s, [&] {
         enqueue_restart(first_c, first_c && zero_guess);
@@ -1784,9 +1827,9 @@ hipGraphExec_t GraphCache::get(const std::vector<uintptr_t>& key, hipStream_t s,
 
 namespace {
 struct TuneKey {
-  int n, nnz, ncase, elem, m;
+  int n, nnz, ncase, elem, m, ordered;
   bool operator<(const TuneKey& o) const {
-    return std::tie(n, nnz, ncase, elem, m) < std::tie(o.n, o.nnz, o.ncase, o.elem, o.m);
+    return std::tie(n, nnz, ncase, elem, m, ordered) < std::tie(o.n, o.nnz, o.ncase, o.elem, o.m, o.ordered);
   }
 };
 std::mutex g_tune_mu;
@@ -1816,7 +1859,7 @@ double time_ms(F&& launch, int reps, hipStream_t s) {
 
 template <class T>
 void autotune(const CsrView<T>& A, KWork& w, int m, hipStream_t s) {
-  const TuneKey key{A.n, A.nnz, w.ncase, static_cast<int>(sizeof(T)), m};
+  const TuneKey key{A.n, A.nnz, w.ncase, static_cast<int>(sizeof(T)), m, w.ord_row ? 1 : 0};
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
     auto it = g_tuned.find(key);
@@ -2153,6 +2196,53 @@ void sell_pattern(int n, const int* rowptr, const int* col, int sigma, std::vect
   }
 }
 
+namespace {
+// out[k] = (float) (val[src[k]] / d) over the entries of position p's row,
+// d the diagonal of its natural row (1 without Jacobi, 1 for a zero diagonal)
+__global__ void __launch_bounds__(kBlock) k_fill_ordered(int n, const int* __restrict__ rowptr,
+                                                         const int* __restrict__ src, const int* __restrict__ row_nat,
+                                                         const int* __restrict__ diag_nat,
+                                                         const double* __restrict__ val, int jacobi,
+                                                         float* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  double inv = 1.0;
+  if (jacobi) {
+    const double d = val[diag_nat[row_nat[p]]];
+    inv = d != 0.0 ? 1.0 / d : 1.0;
+  }
+  for (int k = rowptr[p]; k < rowptr[p + 1]; ++k) out[k] = static_cast<float>(val[src[k]] * inv);
+}
+}  // namespace
+
+void fill_ordered(int n, const int* rowptr, const int* src, const int* row_nat, const int* diag_nat,
+                  const double* val, bool jacobi, float* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_ordered, dim3(grid_for(n)), dim3(kBlock), 0, s, n, rowptr, src, row_nat, diag_nat, val,
+                     jacobi ? 1 : 0, out);
+  S355_LAUNCH_CHECK();
+}
+
+bool sell_offsets16(int n, const std::vector<int>& ptr, const std::vector<int>& srow, const std::vector<int>& scol,
+                    std::vector<int16_t>& off) {
+  off.assign(scol.size(), 0);
+  const int ns = static_cast<int>(ptr.size()) - 1;
+  for (int sl = 0; sl < ns; ++sl) {
+    const int w = (ptr[sl + 1] - ptr[sl]) / 64;
+    for (int l = 0; l < 64; ++l) {
+      const int pos = 64 * sl + l;
+      if (pos >= n) continue;  // past the last row: offset 0 from the row the kernel substitutes
+      const long r = srow[pos];
+      for (int j = 0; j < w; ++j) {
+        const size_t q = static_cast<size_t>(ptr[sl]) + 64 * static_cast<size_t>(j) + l;
+        const long d = static_cast<long>(scol[q]) - r;
+        if (d < INT16_MIN || d > INT16_MAX) return false;
+        off[q] = static_cast<int16_t>(d);
+      }
+    }
+  }
+  return true;
+}
+
 template <class T, class S>
 void sell_fill(const int* src, size_t nslots, const S* csr_val, T* out, hipStream_t s) {
   hipLaunchKernelGGL((k_sell_fill<T, S>), dim3(grid_for(static_cast<long>(nslots))), dim3(kBlock), 0, s, src, nslots,
@@ -2161,6 +2251,7 @@ void sell_fill(const int* src, size_t nslots, const S* csr_val, T* out, hipStrea
 }
 template void sell_fill<double, double>(const int*, size_t, const double*, double*, hipStream_t);
 template void sell_fill<float, double>(const int*, size_t, const double*, float*, hipStream_t);
+template void sell_fill<float, float>(const int*, size_t, const float*, float*, hipStream_t);
 
 void convert_to_float(const double* src, float* dst, size_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_to_float, dim3(grid_for(static_cast<long>(n))), dim3(kBlock), 0, s, src, dst, n);

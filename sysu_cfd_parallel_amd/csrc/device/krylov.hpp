@@ -72,8 +72,9 @@ template <class T>
 struct SellView {
   int n = 0;
   const int* ptr = nullptr;    // nslice + 1 offsets
-  const int* row = nullptr;    // n: matrix row of each slot row
+  const int* row = nullptr;    // n: matrix row of each slot row (null: the slot row itself)
   const int* col = nullptr;
+  const int16_t* col16 = nullptr;  // set: column = row + col16 (16-bit offsets, no col)
   const T* val = nullptr;
   const int* scase = nullptr;  // batches: case of each slice, -1 when its rows span two cases
   int two = 0;                 // FP32, one case: two slices per wave
@@ -91,6 +92,7 @@ struct SellPat {
   const int* ptr = nullptr;
   const int* row = nullptr;
   const int* col = nullptr;
+  const int16_t* col16 = nullptr;  // the columns as offsets from the slot's row (see sell_offsets16)
   const int* src = nullptr;
   const int* scase = nullptr;  // batches only (see SellView)
   size_t slots = 0;
@@ -100,6 +102,18 @@ struct SellPat {
 // the CSR position it copies (-1 for padding).
 void sell_pattern(int n, const int* rowptr, const int* col, int sigma, std::vector<int>& ptr, std::vector<int>& srow,
                   std::vector<int>& scol, std::vector<int>& src);
+
+// The columns of a SELL pattern as 16-bit offsets from each slot's row
+// (padding and the slots past the last row: 0).  Returns false when an
+// offset does not fit in 16 bits.
+bool sell_offsets16(int n, const std::vector<int>& ptr, const std::vector<int>& srow, const std::vector<int>& scol,
+                    std::vector<int16_t>& off);
+
+// FP32 values of a CSR by position: entry k of position p's row is
+// val[src[k]], divided by the diagonal of the natural row row_nat[p] (its
+// Jacobi scaling, 1 for a zero diagonal) when jacobi is set.
+void fill_ordered(int n, const int* rowptr, const int* src, const int* row_nat, const int* diag_nat,
+                  const double* val, bool jacobi, float* out, hipStream_t s);
 
 // out[q] = csr_val[src[q]] (0 where src[q] < 0), converted to T.
 template <class T, class S>
@@ -245,6 +259,13 @@ struct KWork {
   bool tuned = false;       // lanes / nt / rows chosen by autotune()
   TuneReport report;
   GraphCache* graphs = nullptr;  // set: GMRES cycles (Jacobi / no PC) replay captured graphs
+  // mixed GMRES on one case: the FP32 Arnoldi cycle runs on the rows in
+  // another order (the matrix and its SELL copies given in that order);
+  // ord_pos[i] is the position of natural row i, ord_row[p] the natural row
+  // at position p.  The FP64 residual, solution and test stay natural.
+  const int* ord_pos = nullptr;
+  const int* ord_row = nullptr;
+  bool pc_folded = false;  // the FP32 matrix holds M^-1 A (Jacobi): the SpMV scales nothing
   SpmvTimer* timer = nullptr;    // set: time the Arnoldi SpMV launches
   KState* st = nullptr;     // device, ncase states
   KHeader* host = nullptr;  // pinned, ncase headers

@@ -152,6 +152,7 @@ Mesh concat_meshes(const std::vector<Mesh>& parts, BatchLayout& lay) {
     append(m.vert, p.vert);
     for (int r : p.c_rid) m.c_rid.push_back(r + R0);
     for (int v : p.c_vert) m.c_vert.push_back(v + lay.vert[k]);
+    for (int e : p.c_elem) m.c_elem.push_back(e + lay.elem[k]);
     append(m.c_nface, p.c_nface);
     append(m.c_poly, p.c_poly);
     append(m.c_vol, p.c_vol);

@@ -215,6 +215,7 @@ Mesh ingest_mesh(const s355_mesh_desc& d) {
         default: for (int k = 0; k < 4; ++k) h[k] = v[k]; h[4] = h[5] = h[6] = h[7] = v[4]; break;
       }
       m.c_rid.push_back(d.elem_tag[e]);
+      m.c_elem.push_back(e);
       cv.insert(cv.end(), h, h + 8);
     }
   }

@@ -51,6 +51,7 @@ struct Mesh {
   // cells: faces in the reference's pattern order (CellData::face)
   hvec<int> c_rid, c_nface, c_face, c_cell;  // c_face/c_cell: kMaxFace per cell, -1 padded
   hvec<int> c_vert;  // 8 per cell: the element's vertices as a degenerate hex (CellData::vertices)
+  hvec<int> c_elem;  // the element each cell comes from (its index in the input stream)
   hvec<uint8_t> c_poly;
   hvec<double> c_vol, c_x;  // c_x: 3 per cell
 

@@ -12,10 +12,12 @@ ROOT = Path(__file__).resolve().parent.parent
 
 
 def test_roofline_kernel_names():
-    sell = {"lanes": 0, "nt": 1, "slices": 1}
-    assert bench.spmv_kernel_name(sell, "float") == "k_spmv_sell<float, true, true>"
-    assert bench.spmv_kernel_name(dict(sell, slices=2), "float") == "k_spmv_sell2<float, true, true>"
-    assert bench.spmv_kernel_name(dict(sell, nt=0), "double", jacobi=False) == "k_spmv_sell<double, false, false>"
+    sell = {"lanes": 0, "nt": 1, "slices": 1, "order": 0}
+    assert bench.spmv_kernel_name(sell, "float") == "k_spmv_sell<float, true, true, false>"
+    assert bench.spmv_kernel_name(dict(sell, slices=2), "float") == "k_spmv_sell2<float, true, true, false>"
+    assert bench.spmv_kernel_name(dict(sell, slices=2, order=1), "float") == "k_spmv_sell2<float, true, true, true>"
+    assert (bench.spmv_kernel_name(dict(sell, nt=0), "double", jacobi=False)
+            == "k_spmv_sell<double, false, false, false>")
     assert bench.spmv_kernel_name({"lanes": 4, "nt": 0, "slices": 0}, "double") == "k_spmv_pc<double, 4, false>"
 
 

@@ -168,6 +168,81 @@ def test_forced_slices_per_wave_equal_single_lane_csr(sigma):
         assert np.array_equal(out[0][1], o[1])
 
 
+def ordered(prec=s.MIXED, restart=20, graphs=0, kind=1):
+    p = base(prec, restart)
+    p.inner_order = kind
+    p.graphs = graphs
+    return p
+
+
+@pytest.mark.parametrize("graphs,kind", [(0, 1), (1, 1), (0, 2)])
+def test_inner_order_every_spmv_shape_bitwise(graphs, kind):
+    # the FP32 cycle on the rows in reverse Cuthill-McKee order: CSR and the
+    # SELL copies with 16-bit column offsets (one and two slices per wave,
+    # every window) sum each row in the same entry order at the same
+    # positions, so every shape gives the same bits
+    case = s.channel(11, 7, 5, jitter=0.15, split=0.35, seed=3, re=600.0)
+    out = []
+    for lanes, sigma in ((1, 0), (0, 64), (0, 512), (0, 4096), (-1, 512), (-2, 512), (-2, 64)):
+        d = s.Solver(case, ordered(graphs=graphs, kind=kind))
+        d.set_kernel_shapes(1, lanes, sigma=sigma)
+        h, _ = d.iterate(5)
+        assert d.kernel_shapes()["fp32"]["order"] == 1
+        out.append(({n: d.get(n) for n in FIELDS}, h))
+        d.close()
+    for o in out[1:]:
+        for n in FIELDS:
+            assert np.array_equal(out[0][0][n], o[0][n]), n
+        assert np.array_equal(out[0][1], o[1])
+
+
+def test_inner_order_agrees_with_natural_order_and_oracle():
+    from oracle.pyoracle import Oracle
+
+    case = s.channel(12, 8, 10, jitter=0.15, split=0.15, seed=4, re=400.0)
+    res = {}
+    for order in (0, 1, 2):
+        p = params(restart=30, rtol=1e-12, maxit=5000)
+        p.precision, p.inner_order, p.residual_steady = s.MIXED, order, 0.0
+        d = s.Solver(case, p)
+        d.set_kernel_shapes(1, 0, sigma=512)
+        h, _ = d.iterate(6)
+        assert d.kernel_shapes()["fp32"]["order"] == (order > 0)
+        res[order] = ({n: d.get(n) for n in FIELDS}, h)
+        d.close()
+    p = params(restart=30, rtol=1e-12, maxit=5000)
+    p.residual_steady = 0.0
+    o = Oracle(case, p)
+    ho, _, _ = o.iterate(6)
+    for k in (1, 2):
+        for n in ("U", "V", "W", "P"):
+            assert rel_err(res[k][0][n], res[0][0][n]) < 1e-8, (k, n)
+            assert rel_err(res[k][0][n], o.get(n)) < 1e-8, (k, n)
+        assert np.allclose(res[k][1], ho, rtol=1e-6, atol=1e-14)
+
+
+def test_inner_order_kind_fixed_once_built():
+    case = s.channel(8, 5, 6, jitter=0.1, split=0.1, seed=2)
+    d = s.Solver(case, ordered(kind=2))
+    d.iterate(1)
+    p = ordered(kind=1)
+    with pytest.raises(s.S355Error):
+        d.set_params(p)
+    d.set_params(ordered(kind=0))  # off is always allowed
+    d.close()
+
+
+def test_inner_order_is_for_mixed_gmres_only():
+    case = s.channel(8, 5, 6, jitter=0.1, split=0.1, seed=2)
+    for prec, pc in ((s.FP64, s.PC_JACOBI), (s.MIXED, s.PC_ILU0)):
+        p = params(restart=20, pc=pc)
+        p.precision, p.inner_order = prec, 1
+        d = s.Solver(case, p)
+        d.iterate(2)
+        assert d.kernel_shapes()["fp32"]["order"] == 0
+        d.close()
+
+
 def test_two_slices_per_wave_only_for_one_fp32_case():
     d = s.Solver(s.cavity(4), base(s.MIXED))
     with pytest.raises(s.S355Error):
