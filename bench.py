@@ -395,9 +395,12 @@ def cpu_baseline(wl, threads: int, budget_s: float = 20.0, max_steps: int = 3) -
     return {"iters_per_s": steps / dt, "cells": n, "steps": steps, "seconds": dt, "what": what}
 
 
-def run_path(wl, case, prec: int, steps: int, warmup: int, label: str, snapshot: bool = False) -> dict:
+def run_path(wl, case, prec: int, steps: int, warmup: int, label: str, snapshot: bool = False,
+             shapes: dict | None = None) -> dict:
     """W untimed + K timed SIMPLE steps of one precision path from rest;
-    with snapshot the state the timed steps start from is kept."""
+    with snapshot the state the timed steps start from is kept; shapes
+    (precision index -> lanes, nt, rows, sigma) fixes the GMRES kernel shapes
+    instead of the autotuner."""
     import numpy as np
 
     import sysu_cfd_parallel_amd as s
@@ -405,7 +408,11 @@ def run_path(wl, case, prec: int, steps: int, warmup: int, label: str, snapshot:
     t = time.perf_counter()
     p = wl.params(prec)
     p.autotune, p.graphs = wl.autotune, wl.graphs
+    if shapes:
+        p.autotune = 0
     dev = s.BatchSolver(case, p) if isinstance(case, list) else s.Solver(case, p)
+    for k, (lanes, nt, rows, sigma) in (shapes or {}).items():
+        dev.set_kernel_shapes(k, lanes, nt=nt, rows=rows, sigma=sigma)
     z = dev.sizes
     log(f"[{label}] device context: {z.n_cell} cells, {z.n_face} faces, nnz {z.nnz} "
         f"({time.perf_counter() - t:.1f}s)")
@@ -528,6 +535,10 @@ def main() -> int:
                     help="both: FP64 then mixed on the same case, value is the mixed rate")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--shapes", default=None,
+                    help="fixed GMRES kernel shapes instead of the autotuner, per precision: "
+                         "'fp64=lanes,nt,rows,sigma;fp32=lanes,nt,rows,sigma' (lanes -2/-1/0: SELL-64 with two/one "
+                         "slices per wave/by size), e.g. for repeatable counter passes")
     ap.add_argument("--graphs", type=int, choices=[0, 1], default=None,
                     help="replay GMRES cycles from captured graphs (default: per workload)")
     args = ap.parse_args()
@@ -550,12 +561,18 @@ def main() -> int:
     n_elems = sum(c.n_cell_elems for c in case) if isinstance(case, list) else case.n_cell_elems
     log(f"{wl.name} case built in {time.perf_counter() - t:.1f}s ({n_elems} volume elements)")
 
+    shapes = None
+    if args.shapes:
+        shapes = {}
+        for part in args.shapes.split(";"):
+            key, vals = part.split("=")
+            shapes[{"fp64": 0, "fp32": 1}[key.strip()]] = tuple(int(v) for v in vals.split(","))
     runs = {}
     want_cpu = not args.no_cpu_baseline and wl.cpu_mode is not None and not isinstance(case, list)
     for k, name in enumerate(order):
         # the CPU baseline starts from the first path's post-warm-up state
         runs[name] = run_path(wl, case, s.MIXED if name == "mixed" else s.FP64, steps, warmup, name,
-                              snapshot=want_cpu and k == 0)
+                              snapshot=want_cpu and k == 0, shapes=shapes)
         if name != order[-1]:
             runs[name]["dev"].close()
     main_name = order[-1]
@@ -662,7 +679,7 @@ def main() -> int:
         "dtype": "f32+f64" if main_name == "mixed" else "f64",
         "data": "synthetic (seeded jittered hex/tet mesh generated in-process)",
         "config": {
-            "workload": wl.describe(main_name), "autotune": wl.autotune, "graphs": wl.graphs,
+            "workload": wl.describe(main_name), "autotune": 0 if shapes else wl.autotune, "graphs": wl.graphs,
             "cells": z.n_cell, "faces": z.n_face, "nnz": z.nnz,
             "cells_iters_per_s": value * z.n_cell,
             "precision": main_name,

@@ -43,6 +43,15 @@ __device__ __forceinline__ int xcd_block(int b, int G) {
   return (b % kXcd) * q + b / kXcd;
 }
 
+// Item of thread t in a loop over n items processed in the order `order`
+// (identity when null); blocks renumbered with xcd_block so that each XCD
+// takes a contiguous range of the order.  -1 past the end.
+__device__ __forceinline__ int ordered_item(const int* __restrict__ order, int n) {
+  const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= n) return -1;
+  return order ? order[t] : t;
+}
+
 // Rows [0, n) of one or several independent systems ("cases", stored one
 // after the other) split into tiles of at most `tile` rows that never
 // straddle two cases.  Block b of a tiled kernel works on tile b, so

@@ -180,12 +180,29 @@ void Context::upload_mesh() {
       wptr.push_back(static_cast<int>(wface.size()));
     }
   }
+  // processing orders: cells by input element, boundary faces in place,
+  // interior faces by the place of their first cell (then of their second)
+  {
+    std::vector<int> cord(m.n_cell), cpos(m.n_cell), ford(m.n_face);
+    for (int c = 0; c < m.n_cell; ++c) cord[c] = c;
+    std::sort(cord.begin(), cord.end(), [&](int a, int b) { return m.c_elem[a] < m.c_elem[b]; });
+    for (int t = 0; t < m.n_cell; ++t) cpos[cord[t]] = t;
+    for (int f = 0; f < m.n_face; ++f) ford[f] = f;
+    std::sort(ford.begin() + m.n_bnd, ford.end(), [&](int a, int b) {
+      const int a1 = cpos[m.f_c1[a]], b1 = cpos[m.f_c1[b]];
+      return a1 < b1 || (a1 == b1 && cpos[m.f_c2[a]] < cpos[m.f_c2[b]]);
+    });
+    dm.c_order = upload(allocs_, cord, stream);
+    dm.f_order = upload(allocs_, ford, stream);
+    S355_HIP(hipStreamSynchronize(stream));  // host staging vectors go out of scope
+  }
   dm.ctile = make_tiles(lay.cell, kBlock);
   dm.btile = make_tiles(lay.bnd, kBlock);
   dm.n_wcell = static_cast<int>(wcell.size());
   dm.wc_cell = upload(allocs_, wcell, stream);
   dm.wc_ptr = upload(allocs_, wptr, stream);
   dm.wc_face = upload(allocs_, wface, stream);
+  S355_HIP(hipStreamSynchronize(stream));  // host staging vectors go out of scope
 }
 
 void Context::allocate_fields() {

@@ -26,8 +26,8 @@ struct GradArgs {
 // face geometry is read once for all of them.
 template <int NF>
 __global__ void __launch_bounds__(kBlock) k_gradient(DMesh m, GradArgs<NF> a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m.Nc) return;
+  const int i = ordered_item(m.c_order, m.Nc);
+  if (i < 0) return;
   double acc[NF][3];
 #pragma unroll
   for (int k = 0; k < NF; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.;
@@ -229,8 +229,8 @@ struct MomArgs {
 // BuildVelocityMatrix cell loop (navier_velocity.cpp:100-383); the time
 // term (107-123) comes first, as in the reference.
 __global__ void __launch_bounds__(kBlock) k_build_momentum(DMesh m, DFields F, MomArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m.Nc) return;
+  const int i = ordered_item(m.c_order, m.Nc);
+  if (i < 0) return;
   double app = 0., su = 0., sv = 0., sw = 0.;
   if (a.scheme == 1) {
     const double coef = F.R[i] / a.dt * m.c_vol[i];
@@ -410,8 +410,8 @@ void launch_build_momentum(const DMesh& m, const DFields& F, double urf, const d
 // ---------------------------------------------------------------- Rhie-Chow face flux
 // CalRUFace2 (navier_velocity.cpp:425-476).
 __global__ void __launch_bounds__(kBlock) k_ruface(DMesh m, DFields F) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= m.Nf) return;
+  const int f = ordered_item(m.f_order, m.Nf);
+  if (f < 0) return;
   const double n0 = m.f_n[3 * f], n1 = m.f_n[3 * f + 1], n2 = m.f_n[3 * f + 2];
   if (f < m.Nb) {
     const double ro = F.BRo[f];
@@ -461,8 +461,8 @@ void launch_ruface(const DMesh& m, const DFields& F, hipStream_t s) {
 // ---------------------------------------------------------------- pressure-correction assembly
 // BuildPressureMatrix, DensityModel 0 (navier_pressure.cpp:106-213).
 __global__ void __launch_bounds__(kBlock) k_build_pressure(DMesh m, DFields F) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m.Nc) return;
+  const int i = ordered_item(m.c_order, m.Nc);
+  if (i < 0) return;
   double valcen = 0., bpv = 0.;
   const double rai = F.R[i] * F.apr[i];
   const double voli = m.c_vol[i];
@@ -526,8 +526,8 @@ void launch_correct_fields(const DMesh& m, const DFields& F, int ref, double urf
 
 // CorrectRUFace2 (navier_pressure.cpp:243-278).
 __global__ void __launch_bounds__(kBlock) k_correct_ruface(DMesh m, DFields F, double urf) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= m.Nf) return;
+  const int f = ordered_item(m.f_order, m.Nf);
+  if (f < 0) return;
   const double n0 = m.f_n[3 * f], n1 = m.f_n[3 * f + 1], n2 = m.f_n[3 * f + 2];
   if (f < m.Nb) {
     const double ro = F.BRo[f];

@@ -40,6 +40,13 @@ struct DMesh {
   // kBlock-row tiles of the cells and of the boundary faces, for the per-case
   // reductions (residuals, outlet mass flow, pressure reference)
   Tiles ctile, btile;
+  // processing orders of the cell and face loops: thread t of a cell kernel
+  // takes cell c_order[t], of a face kernel face f_order[t] -- the cells in
+  // the order of the input elements (spatially coherent, where the partition
+  // sort's numbering is not), the interior faces by their first cell's
+  // place in it.  Results do not depend on them (one writer per output).
+  const int* c_order = nullptr;
+  const int* f_order = nullptr;
   // cells touching a wall, with their wall faces in ascending order
   int n_wcell = 0;
   const int* wc_cell = nullptr;

@@ -12,8 +12,8 @@ constexpr double kSmall = 1.e-16;
 
 // Diffusivities, production and the k sources (navier_scalar.cpp:33-64).
 __global__ void __launch_bounds__(kBlock) k_keps_sources(DMesh m, DFields F, KepsWork w) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m.Nc) return;
+  const int i = ordered_item(m.c_order, m.Nc);
+  if (i < 0) return;
   const double vol = m.c_vol[i];
   const double mu = F.mu[i], mut = F.mut[i];
   w.viste[i] = mu + mut / kev::Sigma_k;
@@ -160,8 +160,8 @@ void launch_energy_coef(const DMesh& m, const DFields& F, const KepsWork& w, dou
 __global__ void __launch_bounds__(kBlock) k_build_scalar(DMesh m, DFields F, const double* phi, const double* bphi,
                                                          const double* diff, const double* source,
                                                          const double* app_in, double urf, int* bad) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m.Nc) return;
+  const int i = ordered_item(m.c_order, m.Nc);
+  if (i < 0) return;
   double app = app_in[i];
   double sphi = source[i];
   const double xi0 = m.c_x[3 * i], xi1 = m.c_x[3 * i + 1], xi2 = m.c_x[3 * i + 2];
