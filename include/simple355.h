@@ -119,7 +119,18 @@ typedef struct s355_params {
   double total_time;    /* the time loop ends once cur_time >= total_time */
   int max_outer_step;   /* MaxOuterStep: outer SIMPLE passes iter = 1 ..
                            MaxOuterStep - 1 per time step (navier.cpp:60) */
-  int transient_pad;
+  int grad_scheme;       /* cell gradients: 0 Green-Gauss (Gradient,
+                           navier_gradient.cpp:12-48), 1 weighted least squares
+                           over the face neighbours (exact for linear fields
+                           on any mesh; no counterpart in the reference);
+                           the limiters apply to either */
+  int assembly_fp32;     /* mixed precision: 1 = the momentum, pressure-
+                           correction and scalar coefficients are computed and
+                           accumulated in FP32 (the pressure-correction
+                           diagonal and mass imbalance in FP64, so A 1 = 0
+                           still holds); the FP64 refinement then solves that
+                           system.  0 = FP64 assembly */
+  int assembly_pad;
 } s355_params;
 
 typedef struct s355_solve_info {

@@ -96,6 +96,11 @@ class Oracle {
   void SetBCPressure();
   void SetBCDeltaP();
   void Gradient(const std::vector<double>& phi, const std::vector<double>& bphi, std::vector<double>& g);
+  // weighted least-squares gradient (grad_scheme 1) and its per-cell inverse
+  // normal matrices (6 values per cell, built once)
+  void GradientLS(const std::vector<double>& phi, const std::vector<double>& bphi, std::vector<double>& g);
+  void LsSetup();
+  std::vector<double> ls_inv;
   void BuildVelocityMatrix();
   void CalRUFace2();
   void BuildPressureMatrix();

@@ -235,7 +235,85 @@ void Oracle::SetBCDeltaP() {
   for (int i = 0; i < Nbnd; ++i) BPre[i] = deltaP[face[bnd[i].face].c1];
 }
 
+// The offset from cell i to the j-th neighbour of its pattern-order face
+// list: the other cell's centre, or the face centre on a boundary face; w =
+// 1 / |d|^2.
+namespace {
+inline void ls_offset(const Oracle& o, int i, int f, double d[3], double& w) {
+  const Face& F = o.face[f];
+  const double* xn = F.c2 >= 0 ? o.cell[F.c1 == i ? F.c2 : F.c1].x : F.x;
+  const double* xi = o.cell[i].x;
+  d[0] = xn[0] - xi[0];
+  d[1] = xn[1] - xi[1];
+  d[2] = xn[2] - xi[2];
+  w = 1. / (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+}
+}  // namespace
+
+// M = sum_f w d d^T over the cell's faces, inverted by cofactors: the six
+// entries (00, 01, 02, 11, 12, 22) of M^-1 per cell.
+void Oracle::LsSetup() {
+  if (!ls_inv.empty()) return;
+  ls_inv.assign(6 * static_cast<size_t>(Ncel), 0.);
+  for (int i = 0; i < Ncel; ++i) {
+    double a = 0., b = 0., c = 0., d = 0., e = 0., f = 0.;
+    for (int j = 0; j < cell[i].nface; ++j) {
+      double r[3], w;
+      ls_offset(*this, i, cell[i].face[j], r, w);
+      const double w0 = w * r[0], w1 = w * r[1], w2 = w * r[2];
+      a += w0 * r[0];
+      b += w0 * r[1];
+      c += w0 * r[2];
+      d += w1 * r[1];
+      e += w1 * r[2];
+      f += w2 * r[2];
+    }
+    const double c00 = d * f - e * e, c01 = c * e - b * f, c02 = b * e - c * d;
+    const double c11 = a * f - c * c, c12 = b * c - a * e, c22 = a * d - b * b;
+    const double det = a * c00 + b * c01 + c * c02;
+    double* o = &ls_inv[6 * static_cast<size_t>(i)];
+    o[0] = c00 / det;
+    o[1] = c01 / det;
+    o[2] = c02 / det;
+    o[3] = c11 / det;
+    o[4] = c12 / det;
+    o[5] = c22 / det;
+  }
+}
+
+// Weighted least squares: g minimises sum_f w_f (phi_i + g.d_f - phi_f)^2
+// over the cell's faces in pattern order, phi_f the neighbour's value or the
+// boundary value; g = M^-1 sum_f w_f d_f (phi_f - phi_i).  Exact for fields
+// linear in space on any mesh with exact boundary values.
+void Oracle::GradientLS(const std::vector<double>& phi, const std::vector<double>& bphi, std::vector<double>& g) {
+  LsSetup();
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < Ncel; ++i) {
+    double r0 = 0., r1 = 0., r2 = 0.;
+    for (int j = 0; j < cell[i].nface; ++j) {
+      const int fi = cell[i].face[j];
+      const Face& F = face[fi];
+      double d[3], w;
+      ls_offset(*this, i, fi, d, w);
+      const double pn = F.c2 >= 0 ? phi[F.c1 == i ? F.c2 : F.c1] : bphi[F.bnd];
+      const double t = w * (pn - phi[i]);
+      r0 += t * d[0];
+      r1 += t * d[1];
+      r2 += t * d[2];
+    }
+    const double* m = &ls_inv[6 * static_cast<size_t>(i)];
+    g[3 * static_cast<size_t>(i)] = m[0] * r0 + m[1] * r1 + m[2] * r2;
+    g[3 * static_cast<size_t>(i) + 1] = m[1] * r0 + m[3] * r1 + m[4] * r2;
+    g[3 * static_cast<size_t>(i) + 2] = m[2] * r0 + m[4] * r1 + m[5] * r2;
+  }
+}
+
 void Oracle::Gradient(const std::vector<double>& phi, const std::vector<double>& bphi, std::vector<double>& g) {
+  if (par.grad_scheme == 1) {
+    GradientLS(phi, bphi, g);
+  } else if (par.grad_scheme != 0) {
+    throw std::runtime_error("no such gradient scheme");
+  } else {
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < Ncel; ++i) {
     double a[3] = {0., 0., 0.};
@@ -254,6 +332,7 @@ void Oracle::Gradient(const std::vector<double>& phi, const std::vector<double>&
       }
     }
     for (int d = 0; d < 3; ++d) g[3 * static_cast<size_t>(i) + d] = a[d] / cell[i].vol;
+  }
   }
   if (par.limiter == 1)
     Limiter_Barth(phi, g);

@@ -83,7 +83,9 @@ class Params(C.Structure):
         ("dt", C.c_double),
         ("total_time", C.c_double),
         ("max_outer_step", C.c_int),
-        ("transient_pad", C.c_int),
+        ("grad_scheme", C.c_int),
+        ("assembly_fp32", C.c_int),
+        ("assembly_pad", C.c_int),
     ]
 
 

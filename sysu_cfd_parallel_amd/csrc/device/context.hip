@@ -43,6 +43,8 @@ Context::Context(const s355_mesh_desc* descs, int n_case, const s355_params& par
   if (par.density_model != 0) throw std::invalid_argument("only DensityModel 0 (constant density) is supported");
   if (par.precision != S355_FP64 && par.precision != S355_MIXED) throw std::invalid_argument("bad precision mode");
   if (par.inner_order < 0 || par.inner_order > 2) throw std::invalid_argument("inner_order: 0, 1 (RCM) or 2 (input)");
+  if (par.grad_scheme < 0 || par.grad_scheme > 1) throw std::invalid_argument("grad_scheme: 0 Green-Gauss, 1 least squares");
+  if (par.assembly_fp32 != 0 && par.assembly_fp32 != 1) throw std::invalid_argument("assembly_fp32: 0 or 1");
   if (!descs || ncase < 1) throw std::invalid_argument("no mesh");
   if (ncase > 1)
     for (int k = 0; k < 3; ++k)
@@ -75,8 +77,10 @@ Context::Context(const s355_mesh_desc* descs, int n_case, const s355_params& par
     scratch_ = scratch_for_current_device();
     upload_mesh();
     allocate_fields();
-    // the limiters' vertex arrays, so that a limited gradient allocates nothing
+    // the limiters' vertex arrays and the least-squares matrices, so that a
+    // gradient allocates nothing
     if (par.limiter != 0) ensure_vertices();
+    if (par.grad_scheme == 1) ensure_ls();
     S355_HIP(hipStreamSynchronize(stream));
   } catch (...) {
     free_all();
@@ -292,8 +296,11 @@ void Context::set_params(const s355_params& p) {
   if (p.inner_order < 0 || p.inner_order > 2) throw std::invalid_argument("inner_order: 0, 1 (RCM) or 2 (input)");
   if (have_order_ && p.inner_order != 0 && p.inner_order != order_kind_)
     throw std::invalid_argument("inner_order: the other inner order was built already");
+  if (p.grad_scheme < 0 || p.grad_scheme > 1) throw std::invalid_argument("grad_scheme: 0 Green-Gauss, 1 least squares");
+  if (p.assembly_fp32 != 0 && p.assembly_fp32 != 1) throw std::invalid_argument("assembly_fp32: 0 or 1");
   par = p;
   if (par.limiter != 0) ensure_vertices();
+  if (par.grad_scheme == 1) ensure_ls();
 }
 
 FieldRef Context::field(int id) {
@@ -436,7 +443,12 @@ void Context::bc_deltap() { launch_bc_copy(dm, df.dp, df.BPre, stream); }
 // Green-Gauss gradients of nf fields, then the limiter on each (Gradient,
 // navier_gradient.cpp:12-74)
 void Context::grad(int nf, const double* const* phi, const double* const* bphi, double* const* out) {
-  launch_gradient(dm, nf, phi, bphi, out, stream);
+  if (par.grad_scheme == 1) {
+    ensure_ls();
+    launch_gradient_ls(dm, nf, phi, bphi, out, ls_inv_, stream);
+  } else {
+    launch_gradient(dm, nf, phi, bphi, out, stream);
+  }
   if (par.limiter == 0) return;
   ensure_vertices();
   for (int k = 0; k < nf; ++k)
@@ -474,7 +486,12 @@ void Context::gradient_noalloc(int field_id, int bfield_id, int out_id) {
   const double* phi[1] = {f.p};
   const double* bphi[1] = {b.p};
   double* out[1] = {g.p};
-  launch_gradient(dm, 1, phi, bphi, out, stream);
+  if (par.grad_scheme == 1) {
+    if (!ls_inv_) throw std::logic_error("gradient_noalloc: least-squares matrices missing");
+    launch_gradient_ls(dm, 1, phi, bphi, out, ls_inv_, stream);
+  } else {
+    launch_gradient(dm, 1, phi, bphi, out, stream);
+  }
   if (par.limiter == 0) return;
   double* vmax = scratch_;
   double* vmin = scratch_ + dm.Nv;
@@ -490,7 +507,14 @@ void Context::build_velocity_matrix() {
   grad(4, phi, bphi, out);
   const int scheme = par.steady ? 0 : par.time_scheme;
   if (scheme) ensure_transient();
-  launch_build_momentum(dm, df, par.urf[0], par.gravity, flags_, scheme, par.dt, stream);
+  launch_build_momentum(dm, df, par.urf[0], par.gravity, flags_, scheme, par.dt, fp32_assembly(), stream);
+}
+
+// the inverse least-squares normal matrices, built once
+void Context::ensure_ls() {
+  if (ls_inv_) return;
+  ls_inv_ = alloc<double>(6 * static_cast<size_t>(dm.Nc));
+  launch_ls_setup(dm, ls_inv_, stream);
 }
 
 // vertex arrays of the limiters: each cell's 8 vertices, the coordinates, and
@@ -711,7 +735,7 @@ int Context::solve_velocity(double rtol, int maxit, s355_solve_info info[3]) {
 }
 
 void Context::cal_ru_face() { launch_ruface(dm, df, stream); }
-void Context::build_pressure_matrix() { launch_build_pressure(dm, df, stream); }
+void Context::build_pressure_matrix() { launch_build_pressure(dm, df, fp32_assembly(), stream); }
 
 int Context::solve_pressure(double rtol, int maxit, s355_solve_info* info) {
   s355_ksp_opts o = par.ksp[S355_KSP_PRESSURE];
@@ -785,7 +809,7 @@ void Context::build_scalar_matrix(int isca) {
   const double* b1[1] = {bphi};
   double* g1[1] = {df.dPhidX};
   grad(1, p1, b1, g1);
-  launch_build_scalar(dm, df, phi, bphi, diff, src, app, par.urf[5], flags_, stream);
+  launch_build_scalar(dm, df, phi, bphi, diff, src, app, par.urf[5], flags_, fp32_assembly(), stream);
 }
 
 void Context::eps_wall_rows() { launch_dirichlet_rows(dm, df, stream); }

@@ -108,10 +108,13 @@ class Context {
   void prepare_sell(const double* csr_val, bool fp32, bool all);
   void grad(int nf, const double* const* phi, const double* const* bphi, double* const* out);
   void ensure_vertices();
+  void ensure_ls();
   void ensure_transient();
   // the inner order of mixed GMRES (params.inner_order): false when it does
   // not apply or a 16-bit offset does not fit
   bool use_order(const s355_ksp_opts& o);
+  // the mixed path's FP32 assembly option
+  bool fp32_assembly() const { return par.precision == S355_MIXED && par.assembly_fp32 == 1; }
   void ensure_order();
   void fill_order(const double* csr_val, bool jacobi);
   KWork ordered_work(bool all);
@@ -127,6 +130,7 @@ class Context {
   int* flags_ = nullptr;       // [0] bad boundary type, [1] inlet outflow in a scalar
   double *u0_ = nullptr, *v0_ = nullptr, *w0_ = nullptr;
   double *lim_vmax_ = nullptr, *lim_vmin_ = nullptr, *lim_tmp_ = nullptr;  // limiter scratch
+  double* ls_inv_ = nullptr;  // least-squares gradients: inverse normal matrix per cell (6 doubles)
   KepsWork kw_{};
   KWork kwork_{};
   // mixed precision: FP32 matrix copy, FP32 Krylov workspace, refinement buffers

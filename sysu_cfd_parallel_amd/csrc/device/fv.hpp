@@ -8,14 +8,22 @@ namespace s355 {
 
 void launch_gradient(const DMesh& m, int nf, const double* const* phi, const double* const* bphi, double* const* g,
                      hipStream_t s);
+// weighted least-squares gradients: the per-cell inverse normal matrices
+// (6 doubles per cell, built once), then the gradients of nf fields
+void launch_ls_setup(const DMesh& m, double* inv, hipStream_t s);
+void launch_gradient_ls(const DMesh& m, int nf, const double* const* phi, const double* const* bphi, double* const* g,
+                        const double* inv, hipStream_t s);
 void launch_bc_velocity(const DMesh& m, const DFields& F, bool has_outlet, double* partial, double* rate,
                         hipStream_t s);
 void launch_bc_copy(const DMesh& m, const double* cellv, double* bval, hipStream_t s);
 // scheme: 0 steady, 1 Euler, 2 BDF2 (time step dt, old levels in F)
+// fp32: the coefficients computed and accumulated in FP32 (stored in the
+// FP64 arrays); the pressure-correction diagonal and mass imbalance still
+// accumulate in FP64
 void launch_build_momentum(const DMesh& m, const DFields& F, double urf, const double* grav, int* bad, int scheme,
-                           double dt, hipStream_t s);
+                           double dt, bool fp32, hipStream_t s);
 void launch_ruface(const DMesh& m, const DFields& F, hipStream_t s);
-void launch_build_pressure(const DMesh& m, const DFields& F, hipStream_t s);
+void launch_build_pressure(const DMesh& m, const DFields& F, bool fp32, hipStream_t s);
 void launch_correct_fields(const DMesh& m, const DFields& F, int ref, double urf, double* partial, hipStream_t s);
 void launch_correct_ruface(const DMesh& m, const DFields& F, double urf, hipStream_t s);
 void launch_residual3(const DMesh& m, const double* u0, const double* v0, const double* w0, const double* u,
@@ -36,7 +44,7 @@ void launch_keps_wall(const DMesh& m, const DFields& F, const KepsWork& w, doubl
                       hipStream_t s);
 void launch_keps_copy_bc(const DMesh& m, const DFields& F, hipStream_t s);
 void launch_build_scalar(const DMesh& m, const DFields& F, double* phi, const double* bphi, const double* diff,
-                         const double* source, const double* app_in, double urf, int* bad, hipStream_t s);
+                         const double* source, const double* app_in, double urf, int* bad, bool fp32, hipStream_t s);
 void launch_clip(int n, double* v, double floor, hipStream_t s);
 void launch_eps_sources(const DMesh& m, const DFields& F, const KepsWork& w, hipStream_t s);
 void launch_dirichlet_rows(const DMesh& m, const DFields& F, hipStream_t s);

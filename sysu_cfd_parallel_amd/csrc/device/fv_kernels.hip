@@ -9,6 +9,8 @@
 // navier_velocity.cpp:125).  No atomics: every output element has exactly one
 // writer, so results are bitwise reproducible.  Built with
 // -ffp-contract=off; expressions keep the reference's evaluation order.
+#include <type_traits>
+
 #include "common.hpp"
 #include "fv.hpp"
 
@@ -83,6 +85,110 @@ void launch_gradient(const DMesh& m, int nf, const double* const* phi, const dou
   } else if (nf == 4) {
     GradArgs<4> a{{phi[0], phi[1], phi[2], phi[3]}, {bphi[0], bphi[1], bphi[2], bphi[3]}, {g[0], g[1], g[2], g[3]}};
     hipLaunchKernelGGL(k_gradient<4>, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, a);
+  } else {
+    throw std::invalid_argument("gradient: 1 or 4 fields");
+  }
+  S355_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- least-squares gradient
+// The offset d from cell i across face f (to the other cell's centre, or to
+// the face centre on a boundary face) and its weight 1 / |d|^2.
+__device__ __forceinline__ void ls_offset(const DMesh& m, int i, int f, double& d0, double& d1, double& d2,
+                                          double& w) {
+  const double* xn;
+  if (f < m.Nb) {
+    xn = &m.f_x[3 * f];
+  } else {
+    const int c1 = m.f_c1[f];
+    xn = &m.c_x[3 * (c1 == i ? m.f_c2[f] : c1)];
+  }
+  d0 = xn[0] - m.c_x[3 * i];
+  d1 = xn[1] - m.c_x[3 * i + 1];
+  d2 = xn[2] - m.c_x[3 * i + 2];
+  w = 1. / (d0 * d0 + d1 * d1 + d2 * d2);
+}
+
+// M = sum_f w d d^T over cell i's faces (pattern order), inverted by
+// cofactors: entries 00, 01, 02, 11, 12, 22 of M^-1, six per cell.
+__global__ void __launch_bounds__(kBlock) k_ls_setup(DMesh m, double* inv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.Nc) return;
+  double a = 0., b = 0., c = 0., d = 0., e = 0., f = 0.;
+  for (int k = m.cf_ptr[i]; k < m.cf_ptr[i + 1]; ++k) {
+    double r0, r1, r2, w;
+    ls_offset(m, i, m.cf_pat[k], r0, r1, r2, w);
+    const double w0 = w * r0, w1 = w * r1, w2 = w * r2;
+    a += w0 * r0;
+    b += w0 * r1;
+    c += w0 * r2;
+    d += w1 * r1;
+    e += w1 * r2;
+    f += w2 * r2;
+  }
+  const double c00 = d * f - e * e, c01 = c * e - b * f, c02 = b * e - c * d;
+  const double c11 = a * f - c * c, c12 = b * c - a * e, c22 = a * d - b * b;
+  const double det = a * c00 + b * c01 + c * c02;
+  double* o = inv + 6 * static_cast<size_t>(i);
+  o[0] = c00 / det;
+  o[1] = c01 / det;
+  o[2] = c02 / det;
+  o[3] = c11 / det;
+  o[4] = c12 / det;
+  o[5] = c22 / det;
+}
+
+// Weighted least-squares gradient of NF fields: g = M^-1 sum_f w d (phi_f -
+// phi_i) over the faces in pattern order, phi_f the other cell's value or
+// the boundary value.  Exact for linear fields on any mesh.
+template <int NF>
+__global__ void __launch_bounds__(kBlock) k_gradient_ls(DMesh m, GradArgs<NF> a, const double* __restrict__ inv) {
+  const int i = ordered_item(m.c_order, m.Nc);
+  if (i < 0) return;
+  double r[NF][3], pi[NF];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    r[k][0] = r[k][1] = r[k][2] = 0.;
+    pi[k] = a.phi[k][i];
+  }
+  for (int e = m.cf_ptr[i]; e < m.cf_ptr[i + 1]; ++e) {
+    const int f = m.cf_pat[e];
+    double d0, d1, d2, w;
+    ls_offset(m, i, f, d0, d1, d2, w);
+    const int c1 = f < m.Nb ? -1 : m.f_c1[f];
+    const int nb = c1 < 0 ? -1 : (c1 == i ? m.f_c2[f] : c1);
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      const double pn = nb < 0 ? a.bphi[k][f] : a.phi[k][nb];
+      const double t = w * (pn - pi[k]);
+      r[k][0] += t * d0;
+      r[k][1] += t * d1;
+      r[k][2] += t * d2;
+    }
+  }
+  const double* mi = inv + 6 * static_cast<size_t>(i);
+  const double m0 = mi[0], m1 = mi[1], m2 = mi[2], m3 = mi[3], m4 = mi[4], m5 = mi[5];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    a.g[k][3 * i] = m0 * r[k][0] + m1 * r[k][1] + m2 * r[k][2];
+    a.g[k][3 * i + 1] = m1 * r[k][0] + m3 * r[k][1] + m4 * r[k][2];
+    a.g[k][3 * i + 2] = m2 * r[k][0] + m4 * r[k][1] + m5 * r[k][2];
+  }
+}
+
+void launch_ls_setup(const DMesh& m, double* inv, hipStream_t s) {
+  hipLaunchKernelGGL(k_ls_setup, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, inv);
+  S355_LAUNCH_CHECK();
+}
+
+void launch_gradient_ls(const DMesh& m, int nf, const double* const* phi, const double* const* bphi, double* const* g,
+                        const double* inv, hipStream_t s) {
+  if (nf == 1) {
+    GradArgs<1> a{{phi[0]}, {bphi[0]}, {g[0]}};
+    hipLaunchKernelGGL(k_gradient_ls<1>, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, a, inv);
+  } else if (nf == 4) {
+    GradArgs<4> a{{phi[0], phi[1], phi[2], phi[3]}, {bphi[0], bphi[1], bphi[2], bphi[3]}, {g[0], g[1], g[2], g[3]}};
+    hipLaunchKernelGGL(k_gradient_ls<4>, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, a, inv);
   } else {
     throw std::invalid_argument("gradient: 1 or 4 fields");
   }
@@ -228,53 +334,54 @@ struct MomArgs {
 
 // BuildVelocityMatrix cell loop (navier_velocity.cpp:100-383); the time
 // term (107-123) comes first, as in the reference.
+template <class R>
 __global__ void __launch_bounds__(kBlock) k_build_momentum(DMesh m, DFields F, MomArgs a) {
   const int i = ordered_item(m.c_order, m.Nc);
   if (i < 0) return;
-  double app = 0., su = 0., sv = 0., sw = 0.;
+  R app = R(0), su = R(0), sv = R(0), sw = R(0);
   if (a.scheme == 1) {
-    const double coef = F.R[i] / a.dt * m.c_vol[i];
+    const R coef = R(F.R[i]) / R(a.dt) * R(m.c_vol[i]);
     app += coef;
-    su += coef * F.Unp[i];
-    sv += coef * F.Vnp[i];
-    sw += coef * F.Wnp[i];
+    su += coef * R(F.Unp[i]);
+    sv += coef * R(F.Vnp[i]);
+    sw += coef * R(F.Wnp[i]);
   } else if (a.scheme == 2) {
-    const double coef = F.R[i] / a.dt * m.c_vol[i];
-    app += 1.5 * coef;
-    su += coef * (2 * F.Unp[i] - 0.5 * F.Unp2[i]);
-    sv += coef * (2 * F.Vnp[i] - 0.5 * F.Vnp2[i]);
-    sw += coef * (2 * F.Wnp[i] - 0.5 * F.Wnp2[i]);
+    const R coef = R(F.R[i]) / R(a.dt) * R(m.c_vol[i]);
+    app += R(1.5) * coef;
+    su += coef * (R(2) * R(F.Unp[i]) - R(0.5) * R(F.Unp2[i]));
+    sv += coef * (R(2) * R(F.Vnp[i]) - R(0.5) * R(F.Vnp2[i]));
+    sw += coef * (R(2) * R(F.Wnp[i]) - R(0.5) * R(F.Wnp2[i]));
   }
-  const double xi0 = m.c_x[3 * i], xi1 = m.c_x[3 * i + 1], xi2 = m.c_x[3 * i + 2];
-  const double Ui = F.U[i], Vi = F.V[i], Wi = F.W[i];
-  const double mui = F.mu[i] + F.mut[i];
+  const R xi0 = R(m.c_x[3 * i]), xi1 = R(m.c_x[3 * i + 1]), xi2 = R(m.c_x[3 * i + 2]);
+  const R Ui = R(F.U[i]), Vi = R(F.V[i]), Wi = R(F.W[i]);
+  const R mui = R(F.mu[i]) + R(F.mut[i]);
   const double* gu = &F.dUdX[3 * i];
   const double* gv = &F.dVdX[3 * i];
   const double* gw = &F.dWdX[3 * i];
-  const double gu0 = gu[0], gu1 = gu[1], gu2 = gu[2];
-  const double gv0 = gv[0], gv1 = gv[1], gv2 = gv[2];
-  const double gw0 = gw[0], gw1 = gw[1], gw2 = gw[2];
+  const R gu0 = R(gu[0]), gu1 = R(gu[1]), gu2 = R(gu[2]);
+  const R gv0 = R(gv[0]), gv1 = R(gv[1]), gv2 = R(gv[2]);
+  const R gw0 = R(gw[0]), gw1 = R(gw[1]), gw2 = R(gw[2]);
   const int e0 = m.cf_ptr[i], e1 = m.cf_ptr[i + 1];
   for (int e = e0; e < e1; ++e) {
     const int f = m.cf_pat[e];
-    double fde0, fde1, fde2, fdi0, fdi1, fdi2, fcs0, fcs1, fcs2;
-    const double fx0 = m.f_x[3 * f], fx1 = m.f_x[3 * f + 1], fx2 = m.f_x[3 * f + 2];
+    R fde0, fde1, fde2, fdi0, fdi1, fdi2, fcs0, fcs1, fcs2;
+    const R fx0 = R(m.f_x[3 * f]), fx1 = R(m.f_x[3 * f + 1]), fx2 = R(m.f_x[3 * f + 2]);
     if (f < m.Nb) {
       const int t = m.b_type[f];
-      const double s1 = m.f_n[3 * f], s2 = m.f_n[3 * f + 1], s3 = m.f_n[3 * f + 2];
-      const double RUn = F.RUFace[f];
-      const double Visc = mui;
-      const double dxc0 = fx0 - xi0, dxc1 = fx1 - xi1, dxc2 = fx2 - xi2;
-      const double val = Visc * m.f_rlencos[f];
+      const R s1 = R(m.f_n[3 * f]), s2 = R(m.f_n[3 * f + 1]), s3 = R(m.f_n[3 * f + 2]);
+      const R RUn = R(F.RUFace[f]);
+      const R Visc = mui;
+      const R dxc0 = fx0 - xi0, dxc1 = fx1 - xi1, dxc2 = fx2 - xi2;
+      const R val = Visc * R(m.f_rlencos[f]);
       if (t == kWall) {
         app += val;
-        double sh0 = (gu0 + gu0) * s1 + (gu1 + gv0) * s2 + (gu2 + gw0) * s3;
-        double sh1 = (gv0 + gu1) * s1 + (gv1 + gv1) * s2 + (gv2 + gw1) * s3;
-        double sh2 = (gw0 + gu2) * s1 + (gw1 + gv2) * s2 + (gw2 + gw2) * s3;
+        R sh0 = (gu0 + gu0) * s1 + (gu1 + gv0) * s2 + (gu2 + gw0) * s3;
+        R sh1 = (gv0 + gu1) * s1 + (gv1 + gv1) * s2 + (gv2 + gw1) * s3;
+        R sh2 = (gw0 + gu2) * s1 + (gw1 + gv2) * s2 + (gw2 + gw2) * s3;
         sh0 *= Visc;
         sh1 *= Visc;
         sh2 *= Visc;
-        const double Pi = F.P[i];
+        const R Pi = R(F.P[i]);
         sh0 += Pi * s1;
         sh1 += Pi * s2;
         sh2 += Pi * s3;
@@ -295,70 +402,70 @@ __global__ void __launch_bounds__(kBlock) k_build_momentum(DMesh m, DFields F, M
         fde0 *= Visc;
         fde1 *= Visc;
         fde2 *= Visc;
-        fdi0 = val * (gu0 * dxc0 + gu1 * dxc1 + gu2 * dxc2 - F.BU[f]);
-        fdi1 = val * (gv0 * dxc0 + gv1 * dxc1 + gv2 * dxc2 - F.BV[f]);
-        fdi2 = val * (gw0 * dxc0 + gw1 * dxc1 + gw2 * dxc2 - F.BW[f]);
+        fdi0 = val * (gu0 * dxc0 + gu1 * dxc1 + gu2 * dxc2 - R(F.BU[f]));
+        fdi1 = val * (gv0 * dxc0 + gv1 * dxc1 + gv2 * dxc2 - R(F.BV[f]));
+        fdi2 = val * (gw0 * dxc0 + gw1 * dxc1 + gw2 * dxc2 - R(F.BW[f]));
       } else {
         *a.bad = 1;
-        fde0 = fde1 = fde2 = fdi0 = fdi1 = fdi2 = 0.;
+        fde0 = fde1 = fde2 = fdi0 = fdi1 = fdi2 = R(0);
       }
       if (t == kInlet || t == kOutlet) {
-        const double fl = fmin(RUn, 0.0);
+        const R fl = fmin(RUn, R(0));
         app -= fl;
-        fcs0 = fl * F.BU[f];
-        fcs1 = fl * F.BV[f];
-        fcs2 = fl * F.BW[f];
+        fcs0 = fl * R(F.BU[f]);
+        fcs1 = fl * R(F.BV[f]);
+        fcs2 = fl * R(F.BW[f]);
       } else {
-        fcs0 = fcs1 = fcs2 = 0.;
+        fcs0 = fcs1 = fcs2 = R(0);
       }
     } else {
       const int c1 = m.f_c1[f];
       const bool own = (c1 == i);
       const int in = own ? m.f_c2[f] : c1;
-      double s1 = m.f_n[3 * f], s2 = m.f_n[3 * f + 1], s3 = m.f_n[3 * f + 2];
-      double RUn = F.RUFace[f];
-      double lambda = m.f_lambda[f];
+      R s1 = R(m.f_n[3 * f]), s2 = R(m.f_n[3 * f + 1]), s3 = R(m.f_n[3 * f + 2]);
+      R RUn = R(F.RUFace[f]);
+      R lambda = R(m.f_lambda[f]);
       if (!own) {
         s1 = -s1;
         s2 = -s2;
         s3 = -s3;
         RUn = -RUn;
-        lambda = 1. - lambda;
+        lambda = R(1) - lambda;
       }
-      const double lambda2 = 1. - lambda;
+      const R lambda2 = R(1) - lambda;
       const double* hu = &F.dUdX[3 * in];
       const double* hv = &F.dVdX[3 * in];
       const double* hw = &F.dWdX[3 * in];
-      const double hu0 = hu[0], hu1 = hu[1], hu2 = hu[2];
-      const double hv0 = hv[0], hv1 = hv[1], hv2 = hv[2];
-      const double hw0 = hw[0], hw1 = hw[1], hw2 = hw[2];
-      const double Visc = lambda * mui + lambda2 * (F.mu[in] + F.mut[in]);
-      const double dudx = lambda * gu0 + lambda2 * hu0;
-      const double dudy = lambda * gu1 + lambda2 * hu1;
-      const double dudz = lambda * gu2 + lambda2 * hu2;
-      const double dvdx = lambda * gv0 + lambda2 * hv0;
-      const double dvdy = lambda * gv1 + lambda2 * hv1;
-      const double dvdz = lambda * gv2 + lambda2 * hv2;
-      const double dwdx = lambda * gw0 + lambda2 * hw0;
-      const double dwdy = lambda * gw1 + lambda2 * hw1;
-      const double dwdz = lambda * gw2 + lambda2 * hw2;
-      double apn = 0.;
-      if (RUn < 0.) {
+      const R hu0 = R(hu[0]), hu1 = R(hu[1]), hu2 = R(hu[2]);
+      const R hv0 = R(hv[0]), hv1 = R(hv[1]), hv2 = R(hv[2]);
+      const R hw0 = R(hw[0]), hw1 = R(hw[1]), hw2 = R(hw[2]);
+      const R Visc = lambda * mui + lambda2 * (R(F.mu[in]) + R(F.mut[in]));
+      const R dudx = lambda * gu0 + lambda2 * hu0;
+      const R dudy = lambda * gu1 + lambda2 * hu1;
+      const R dudz = lambda * gu2 + lambda2 * hu2;
+      const R dvdx = lambda * gv0 + lambda2 * hv0;
+      const R dvdy = lambda * gv1 + lambda2 * hv1;
+      const R dvdz = lambda * gv2 + lambda2 * hv2;
+      const R dwdx = lambda * gw0 + lambda2 * hw0;
+      const R dwdy = lambda * gw1 + lambda2 * hw1;
+      const R dwdz = lambda * gw2 + lambda2 * hw2;
+      R apn = R(0);
+      if (RUn < R(0)) {
         apn += RUn;
         app -= RUn;
       }
-      const double val = Visc * m.f_rlencos[f];
+      const R val = Visc * R(m.f_rlencos[f]);
       app += val;
       apn += -val;
       F.Au[m.cf_slot[e]] = apn;
-      const double xn0 = m.c_x[3 * in], xn1 = m.c_x[3 * in + 1], xn2 = m.c_x[3 * in + 2];
-      const double Un = F.U[in], Vn = F.V[in], Wn = F.W[in];
-      double ufi, vfi, wfi, ufh, vfh, wfh;
-      if (RUn > 0.) {
+      const R xn0 = R(m.c_x[3 * in]), xn1 = R(m.c_x[3 * in + 1]), xn2 = R(m.c_x[3 * in + 2]);
+      const R Un = R(F.U[in]), Vn = R(F.V[in]), Wn = R(F.W[in]);
+      R ufi, vfi, wfi, ufh, vfh, wfh;
+      if (RUn > R(0)) {
         ufi = Ui;
         vfi = Vi;
         wfi = Wi;
-        const double dx0 = fx0 - xi0, dx1 = fx1 - xi1, dx2 = fx2 - xi2;
+        const R dx0 = fx0 - xi0, dx1 = fx1 - xi1, dx2 = fx2 - xi2;
         ufh = Ui + (gu0 * dx0 + gu1 * dx1 + gu2 * dx2);
         vfh = Vi + (gv0 * dx0 + gv1 * dx1 + gv2 * dx2);
         wfh = Wi + (gw0 * dx0 + gw1 * dx1 + gw2 * dx2);
@@ -366,7 +473,7 @@ __global__ void __launch_bounds__(kBlock) k_build_momentum(DMesh m, DFields F, M
         ufi = Un;
         vfi = Vn;
         wfi = Wn;
-        const double dx0 = fx0 - xn0, dx1 = fx1 - xn1, dx2 = fx2 - xn2;
+        const R dx0 = fx0 - xn0, dx1 = fx1 - xn1, dx2 = fx2 - xn2;
         ufh = Un + (hu0 * dx0 + hu1 * dx1 + hu2 * dx2);
         vfh = Vn + (hv0 * dx0 + hv1 * dx1 + hv2 * dx2);
         wfh = Wn + (hw0 * dx0 + hw1 * dx1 + hw2 * dx2);
@@ -380,7 +487,7 @@ __global__ void __launch_bounds__(kBlock) k_build_momentum(DMesh m, DFields F, M
       fde0 *= Visc;
       fde1 *= Visc;
       fde2 *= Visc;
-      const double dxc0 = xn0 - xi0, dxc1 = xn1 - xi1, dxc2 = xn2 - xi2;
+      const R dxc0 = xn0 - xi0, dxc1 = xn1 - xi1, dxc2 = xn2 - xi2;
       fdi0 = val * (dudx * dxc0 + dudy * dxc1 + dudz * dxc2);
       fdi1 = val * (dvdx * dxc0 + dvdy * dxc1 + dvdz * dxc2);
       fdi2 = val * (dwdx * dxc0 + dwdy * dxc1 + dwdz * dxc2);
@@ -389,21 +496,24 @@ __global__ void __launch_bounds__(kBlock) k_build_momentum(DMesh m, DFields F, M
     sv += fde1 - fdi1 - fcs1;
     sw += fde2 - fdi2 - fcs2;
   }
-  app /= a.urf;
-  F.apr[i] = 1. / (app);
+  app /= R(a.urf);
+  F.apr[i] = R(1) / (app);
   F.Au[m.diag[i]] = app;
-  const double vol = m.c_vol[i];
-  const double Ri = F.R[i];
+  const R vol = R(m.c_vol[i]);
+  const R Ri = R(F.R[i]);
   const double* gp = &F.dPdX[3 * i];
-  F.bu[i] = su + (1. - a.urf) * app * Ui + (-gp[0] + Ri * a.g0) * vol;
-  F.bv[i] = sv + (1. - a.urf) * app * Vi + (-gp[1] + Ri * a.g1) * vol;
-  F.bw[i] = sw + (1. - a.urf) * app * Wi + (-gp[2] + Ri * a.g2) * vol;
+  F.bu[i] = su + (R(1) - R(a.urf)) * app * Ui + (-R(gp[0]) + Ri * R(a.g0)) * vol;
+  F.bv[i] = sv + (R(1) - R(a.urf)) * app * Vi + (-R(gp[1]) + Ri * R(a.g1)) * vol;
+  F.bw[i] = sw + (R(1) - R(a.urf)) * app * Wi + (-R(gp[2]) + Ri * R(a.g2)) * vol;
 }
 
 void launch_build_momentum(const DMesh& m, const DFields& F, double urf, const double* grav, int* bad, int scheme,
-                           double dt, hipStream_t s) {
+                           double dt, bool fp32, hipStream_t s) {
   MomArgs a{urf, grav[0], grav[1], grav[2], bad, scheme, dt};
-  hipLaunchKernelGGL(k_build_momentum, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, F, a);
+  if (fp32)
+    hipLaunchKernelGGL(k_build_momentum<float>, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, F, a);
+  else
+    hipLaunchKernelGGL(k_build_momentum<double>, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, F, a);
   S355_LAUNCH_CHECK();
 }
 
@@ -460,12 +570,16 @@ void launch_ruface(const DMesh& m, const DFields& F, hipStream_t s) {
 
 // ---------------------------------------------------------------- pressure-correction assembly
 // BuildPressureMatrix, DensityModel 0 (navier_pressure.cpp:106-213).
+template <class R>
 __global__ void __launch_bounds__(kBlock) k_build_pressure(DMesh m, DFields F) {
   const int i = ordered_item(m.c_order, m.Nc);
   if (i < 0) return;
+  // the diagonal and the mass imbalance accumulate in FP64 whatever R is:
+  // the diagonal then equals minus the sum of the stored off-diagonals, so
+  // A 1 = 0 holds for FP32 coefficients too
   double valcen = 0., bpv = 0.;
-  const double rai = F.R[i] * F.apr[i];
-  const double voli = m.c_vol[i];
+  const R rai = R(F.R[i]) * R(F.apr[i]);
+  const R voli = R(m.c_vol[i]);
   const int e0 = m.cf_ptr[i], e1 = m.cf_ptr[i + 1];
   for (int e = e0; e < e1; ++e) {
     const int f = m.cf_pat[e];
@@ -476,21 +590,37 @@ __global__ void __launch_bounds__(kBlock) k_build_pressure(DMesh m, DFields F) {
     else
       bpv -= ru;
     if (f < m.Nb) continue;
-    const int in = (i == c1) ? m.f_c2[f] : c1;
-    const double lambda = (i == c1) ? m.f_lambda[f] : 1. - m.f_lambda[f];
-    const double lambda2 = 1. - lambda;
-    const double roapf = rai * lambda + F.R[in] * F.apr[in] * lambda2;
-    const double vol = voli * lambda + m.c_vol[in] * lambda2;
-    const double tmp = roapf * vol * m.f_rlencos[f];
-    valcen += -tmp;
+    R roapf, vol;
+    if constexpr (std::is_same<R, float>::value) {
+      // FP32: evaluated from the face's c1 side in both of its rows, so the
+      // two rows store the same value.  The reference's per-row orientation
+      // rounds differently on the two sides; at FP32 that asymmetry (~1e-7)
+      // leaves the singular system of a closed domain inconsistent at ~1e-9
+      // of the right-hand side, and a 1e-9 solve can then not converge
+      const int c2 = m.f_c2[f];
+      const R l1 = R(m.f_lambda[f]), l2 = R(1) - l1;
+      roapf = R(F.R[c1]) * R(F.apr[c1]) * l1 + R(F.R[c2]) * R(F.apr[c2]) * l2;
+      vol = R(m.c_vol[c1]) * l1 + R(m.c_vol[c2]) * l2;
+    } else {
+      const int in = (i == c1) ? m.f_c2[f] : c1;
+      const R lambda = (i == c1) ? R(m.f_lambda[f]) : R(1) - R(m.f_lambda[f]);
+      const R lambda2 = R(1) - lambda;
+      roapf = rai * lambda + R(F.R[in]) * R(F.apr[in]) * lambda2;
+      vol = voli * lambda + R(m.c_vol[in]) * lambda2;
+    }
+    const R tmp = roapf * vol * R(m.f_rlencos[f]);
+    valcen += -static_cast<double>(tmp);
     F.Ap[m.cf_slot[e]] = tmp;
   }
   F.Ap[m.diag[i]] = valcen;
   F.bp[i] = bpv;
 }
 
-void launch_build_pressure(const DMesh& m, const DFields& F, hipStream_t s) {
-  hipLaunchKernelGGL(k_build_pressure, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, F);
+void launch_build_pressure(const DMesh& m, const DFields& F, bool fp32, hipStream_t s) {
+  if (fp32)
+    hipLaunchKernelGGL(k_build_pressure<float>, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, F);
+  else
+    hipLaunchKernelGGL(k_build_pressure<double>, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, F);
   S355_LAUNCH_CHECK();
 }
 

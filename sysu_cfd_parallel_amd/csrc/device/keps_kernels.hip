@@ -157,100 +157,105 @@ void launch_energy_coef(const DMesh& m, const DFields& F, const KepsWork& w, dou
 // BuildScalarMatrix cell loop (navier_scalar.cpp:421-589).  The gradient of
 // phi is in F.dPhidX.  bad[0] flags unknown boundary types, bad[1] an inlet
 // with outflow (the reference exits there, navier_scalar.cpp:476-479).
+template <class R>
 __global__ void __launch_bounds__(kBlock) k_build_scalar(DMesh m, DFields F, const double* phi, const double* bphi,
                                                          const double* diff, const double* source,
                                                          const double* app_in, double urf, int* bad) {
   const int i = ordered_item(m.c_order, m.Nc);
   if (i < 0) return;
-  double app = app_in[i];
-  double sphi = source[i];
-  const double xi0 = m.c_x[3 * i], xi1 = m.c_x[3 * i + 1], xi2 = m.c_x[3 * i + 2];
+  R app = R(app_in[i]);
+  R sphi = R(source[i]);
+  const R xi0 = R(m.c_x[3 * i]), xi1 = R(m.c_x[3 * i + 1]), xi2 = R(m.c_x[3 * i + 2]);
   const double* g = &F.dPhidX[3 * i];
-  const double g0 = g[0], g1 = g[1], g2 = g[2];
-  const double phii = phi[i];
-  const double difi = diff[i];
+  const R g0 = R(g[0]), g1 = R(g[1]), g2 = R(g[2]);
+  const R phii = R(phi[i]);
+  const R difi = R(diff[i]);
   const int e0 = m.cf_ptr[i], e1 = m.cf_ptr[i + 1];
   for (int e = e0; e < e1; ++e) {
     const int f = m.cf_pat[e];
-    const double fx0 = m.f_x[3 * f], fx1 = m.f_x[3 * f + 1], fx2 = m.f_x[3 * f + 2];
-    double fde, fdi, fcs;
+    const R fx0 = R(m.f_x[3 * f]), fx1 = R(m.f_x[3 * f + 1]), fx2 = R(m.f_x[3 * f + 2]);
+    R fde, fdi, fcs;
     if (f < m.Nb) {
       const int t = m.b_type[f];
-      const double s1 = m.f_n[3 * f], s2 = m.f_n[3 * f + 1], s3 = m.f_n[3 * f + 2];
-      const double RUn = F.RUFace[f];
-      const double Visc = difi;
-      const double dxc0 = fx0 - xi0, dxc1 = fx1 - xi1, dxc2 = fx2 - xi2;
+      const R s1 = R(m.f_n[3 * f]), s2 = R(m.f_n[3 * f + 1]), s3 = R(m.f_n[3 * f + 2]);
+      const R RUn = R(F.RUFace[f]);
+      const R Visc = difi;
+      const R dxc0 = fx0 - xi0, dxc1 = fx1 - xi1, dxc2 = fx2 - xi2;
       if (t < kWall || t > kSym) bad[0] = 1;
-      const double val = Visc * m.f_rlencos[f];
+      const R val = Visc * R(m.f_rlencos[f]);
       app += val;
       fde = Visc * (g0 * s1 + g1 * s2 + g2 * s3);
-      fdi = val * (g0 * dxc0 + g1 * dxc1 + g2 * dxc2 - bphi[f]);
+      fdi = val * (g0 * dxc0 + g1 * dxc1 + g2 * dxc2 - R(bphi[f]));
       if (t == kInlet || t == kOutlet) {
         if (t == kInlet && RUn > 0) bad[1] = 1;
-        const double fl = fmin(RUn, 0.0);
+        const R fl = fmin(RUn, R(0));
         app -= fl;
-        fcs = fl * bphi[f];
+        fcs = fl * R(bphi[f]);
       } else {
-        fcs = 0.;
+        fcs = R(0);
       }
     } else {
       const int c1 = m.f_c1[f];
       const bool own = (c1 == i);
       const int in = own ? m.f_c2[f] : c1;
-      double s1 = m.f_n[3 * f], s2 = m.f_n[3 * f + 1], s3 = m.f_n[3 * f + 2];
-      double RUn = F.RUFace[f];
-      double lambda = m.f_lambda[f];
+      R s1 = R(m.f_n[3 * f]), s2 = R(m.f_n[3 * f + 1]), s3 = R(m.f_n[3 * f + 2]);
+      R RUn = R(F.RUFace[f]);
+      R lambda = R(m.f_lambda[f]);
       if (!own) {
         s1 = -s1;
         s2 = -s2;
         s3 = -s3;
         RUn = -RUn;
-        lambda = 1. - lambda;
+        lambda = R(1) - lambda;
       }
-      const double lambda2 = 1. - lambda;
+      const R lambda2 = R(1) - lambda;
       const double* h = &F.dPhidX[3 * in];
-      const double h0 = h[0], h1 = h[1], h2 = h[2];
-      const double Visc = lambda * difi + lambda2 * diff[in];
-      const double dpx = lambda * g0 + lambda2 * h0;
-      const double dpy = lambda * g1 + lambda2 * h1;
-      const double dpz = lambda * g2 + lambda2 * h2;
-      double apn = 0.;
-      if (RUn < 0.) {
+      const R h0 = R(h[0]), h1 = R(h[1]), h2 = R(h[2]);
+      const R Visc = lambda * difi + lambda2 * R(diff[in]);
+      const R dpx = lambda * g0 + lambda2 * h0;
+      const R dpy = lambda * g1 + lambda2 * h1;
+      const R dpz = lambda * g2 + lambda2 * h2;
+      R apn = R(0);
+      if (RUn < R(0)) {
         apn += RUn;
         app -= RUn;
       }
-      const double val = Visc * m.f_rlencos[f];
+      const R val = Visc * R(m.f_rlencos[f]);
       app += val;
       apn += -val;
       F.As[m.cf_slot[e]] = apn;
-      const double xn0 = m.c_x[3 * in], xn1 = m.c_x[3 * in + 1], xn2 = m.c_x[3 * in + 2];
-      double pfi, pfh;
-      if (RUn > 0.) {
+      const R xn0 = R(m.c_x[3 * in]), xn1 = R(m.c_x[3 * in + 1]), xn2 = R(m.c_x[3 * in + 2]);
+      R pfi, pfh;
+      if (RUn > R(0)) {
         pfi = phii;
-        const double dx0 = fx0 - xi0, dx1 = fx1 - xi1, dx2 = fx2 - xi2;
+        const R dx0 = fx0 - xi0, dx1 = fx1 - xi1, dx2 = fx2 - xi2;
         pfh = phii + (g0 * dx0 + g1 * dx1 + g2 * dx2);
       } else {
-        const double pn = phi[in];
+        const R pn = R(phi[in]);
         pfi = pn;
-        const double dx0 = fx0 - xn0, dx1 = fx1 - xn1, dx2 = fx2 - xn2;
+        const R dx0 = fx0 - xn0, dx1 = fx1 - xn1, dx2 = fx2 - xn2;
         pfh = pn + (h0 * dx0 + h1 * dx1 + h2 * dx2);
       }
       fcs = RUn * (pfh - pfi);
       fde = Visc * (dpx * s1 + dpy * s2 + dpz * s3);
-      const double dxc0 = xn0 - xi0, dxc1 = xn1 - xi1, dxc2 = xn2 - xi2;
+      const R dxc0 = xn0 - xi0, dxc1 = xn1 - xi1, dxc2 = xn2 - xi2;
       fdi = val * (dpx * dxc0 + dpy * dxc1 + dpz * dxc2);
     }
     sphi += fde - fdi - fcs;
   }
-  app /= urf;
+  app /= R(urf);
   F.As[m.diag[i]] = app;
-  F.bs[i] = sphi + (1. - urf) * app * phii;
+  F.bs[i] = sphi + (R(1) - R(urf)) * app * phii;
 }
 
 void launch_build_scalar(const DMesh& m, const DFields& F, double* phi, const double* bphi, const double* diff,
-                         const double* source, const double* app_in, double urf, int* bad, hipStream_t s) {
-  hipLaunchKernelGGL(k_build_scalar, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, F, phi, bphi, diff, source, app_in,
-                     urf, bad);
+                         const double* source, const double* app_in, double urf, int* bad, bool fp32, hipStream_t s) {
+  if (fp32)
+    hipLaunchKernelGGL(k_build_scalar<float>, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, F, phi, bphi, diff, source,
+                       app_in, urf, bad);
+  else
+    hipLaunchKernelGGL(k_build_scalar<double>, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, F, phi, bphi, diff, source,
+                       app_in, urf, bad);
   S355_LAUNCH_CHECK();
 }
 

@@ -52,6 +52,34 @@ def test_gradient_bitwise(name):
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
+def test_least_squares_gradient_bitwise(name):
+    p = params()
+    p.grad_scheme = 1
+    d, o = pair(CASES[name](), p)
+    same_state(d, o, ["U", "V", "W", "P", "BU", "BV", "BW", "BPRE"], seed=3)
+    for f, b, g in [("P", "BPRE", "DPDX"), ("U", "BU", "DUDX")]:
+        d.gradient(f, b, g)
+        o.gradient(f, b, g)
+        assert np.array_equal(d.get(g), o.get(g)), f"{g}: rel err {rel_err(d.get(g), o.get(g)):.3e}"
+    # the four velocity/pressure gradients of the momentum assembly, fused
+    d.call("build_velocity_matrix")
+    o.call("build_velocity_matrix")
+    assert_close(d, o, ["DUDX", "DVDX", "DWDX", "DPDX", "AU", "BU_RHS"], 1e-13)
+
+
+@pytest.mark.parametrize("case_fn,steps", [
+    (lambda: s.cavity(8, re=100, jitter=0.1, seed=2), 15),
+    (lambda: s.channel(10, 6, 8, jitter=0.15, split=0.1, seed=1, re=200), 10),
+])
+def test_simple_loop_least_squares_gradients(case_fn, steps):
+    p = params(rtol=1e-12)
+    p.grad_scheme = 1
+    d, o, hd, ho = _loop_pair(case_fn(), p, steps)
+    assert np.allclose(hd, ho, rtol=1e-6, atol=1e-14)
+    assert_close(d, o, ["U", "V", "W", "P"], 1e-7)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
 def test_build_velocity_matrix(name):
     d, o = pair(CASES[name](), params())
     same_state(d, o, ["U", "V", "W", "P", "R", "VISLAM", "VISTUR", "RUFACE"], seed=4)
