@@ -80,6 +80,7 @@ class Channel:
     steps, warmup = 5, 1
     turbulent = False
     autotune, graphs = 1, 0
+    assembly_fp32 = 0
     # CPU baseline on the GPU case itself from the GPU's post-warm-up state:
     # "steps" times whole SIMPLE steps, "projected" times every phase but the
     # pressure solve in full and the pressure solve per Krylov iteration
@@ -408,6 +409,8 @@ def run_path(wl, case, prec: int, steps: int, warmup: int, label: str, snapshot:
     t = time.perf_counter()
     p = wl.params(prec)
     p.autotune, p.graphs = wl.autotune, wl.graphs
+    if prec == s.MIXED:
+        p.assembly_fp32 = wl.assembly_fp32
     if shapes:
         p.autotune = 0
     dev = s.BatchSolver(case, p) if isinstance(case, list) else s.Solver(case, p)
@@ -539,6 +542,8 @@ def main() -> int:
                     help="fixed GMRES kernel shapes instead of the autotuner, per precision: "
                          "'fp64=lanes,nt,rows,sigma;fp32=lanes,nt,rows,sigma' (lanes -2/-1/0: SELL-64 with two/one "
                          "slices per wave/by size), e.g. for repeatable counter passes")
+    ap.add_argument("--assembly", choices=["fp64", "fp32"], default="fp64",
+                    help="the mixed path's coefficient assembly: FP64 (default) or FP32 arithmetic")
     ap.add_argument("--graphs", type=int, choices=[0, 1], default=None,
                     help="replay GMRES cycles from captured graphs (default: per workload)")
     args = ap.parse_args()
@@ -552,6 +557,7 @@ def main() -> int:
     wl = WORKLOADS[args.workload](args.size)
     if args.graphs is not None:
         wl.graphs = args.graphs
+    wl.assembly_fp32 = int(args.assembly == "fp32")
     steps = args.steps if args.steps is not None else wl.steps
     warmup = args.warmup if args.warmup is not None else wl.warmup
     order = {"default": wl.precisions, "both": ["fp64", "mixed"], "fp64": ["fp64"], "mixed": ["mixed"]}[args.precision]
@@ -683,6 +689,7 @@ def main() -> int:
             "cells": z.n_cell, "faces": z.n_face, "nnz": z.nnz,
             "cells_iters_per_s": value * z.n_cell,
             "precision": main_name,
+            "assembly": "fp32" if main_name == "mixed" and wl.assembly_fp32 else "fp64",
         },
         "roofline": {
             "bound": "hbm", "kernel": dom["kernel"] + " on the pressure-correction matrix",

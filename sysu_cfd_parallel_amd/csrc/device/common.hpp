@@ -52,6 +52,10 @@ __device__ __forceinline__ int ordered_item(const int* __restrict__ order, int n
   return order ? order[t] : t;
 }
 
+// Block of a half-face kernel (entries [blk[b], blk[b + 1]) of a half-face
+// stream), with the same XCD renumbering.
+__device__ __forceinline__ int hf_block() { return xcd_block(blockIdx.x, gridDim.x); }
+
 // Rows [0, n) of one or several independent systems ("cases", stored one
 // after the other) split into tiles of at most `tile` rows that never
 // straddle two cases.  Block b of a tiled kernel works on tile b, so

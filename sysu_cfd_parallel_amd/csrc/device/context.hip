@@ -198,6 +198,29 @@ void Context::upload_mesh() {
     });
     dm.c_order = upload(allocs_, cord, stream);
     dm.f_order = upload(allocs_, ford, stream);
+    // the half-face streams in that order, packed into blocks of whole cells
+    auto stream_of = [&](const hvec<int>& faces, const hvec<int>* slots, const int*& blk, int& nblk) {
+      std::vector<int4> rec;
+      std::vector<int> b(1, 0);
+      rec.reserve(static_cast<size_t>(m.cf_ptr[m.n_cell]));
+      for (int t = 0; t < m.n_cell; ++t) {
+        const int i = cord[t], e0 = m.cf_ptr[i], e1 = m.cf_ptr[i + 1];
+        if (e1 - e0 > kBlock) throw std::runtime_error("mesh: a cell with more faces than a block has lanes");
+        if (static_cast<int>(rec.size()) - b.back() + (e1 - e0) > kBlock) b.push_back(static_cast<int>(rec.size()));
+        for (int e = e0; e < e1; ++e) {
+          const int f = faces[e];
+          const int nb = f < m.n_bnd ? 0 : (m.f_c1[f] == i ? m.f_c2[f] : ~m.f_c1[f]);
+          rec.push_back(make_int4(i, f, nb, slots ? (*slots)[e] : -1));
+        }
+      }
+      b.push_back(static_cast<int>(rec.size()));
+      nblk = static_cast<int>(b.size()) - 1;
+      blk = upload(allocs_, b, stream);
+      return upload(allocs_, rec, stream);
+    };
+    dm.hf = stream_of(m.cf_pat, &m.cf_slot, dm.hf_blk, dm.hf_nblk);
+    S355_HIP(hipStreamSynchronize(stream));
+    dm.hfs = stream_of(m.cf_sorted, nullptr, dm.hfs_blk, dm.hfs_nblk);
     S355_HIP(hipStreamSynchronize(stream));  // host staging vectors go out of scope
   }
   dm.ctile = make_tiles(lay.cell, kBlock);

@@ -25,47 +25,62 @@ struct GradArgs {
 };
 
 // Green-Gauss gradient (navier_gradient.cpp:12-48), NF fields fused so the
-// face geometry is read once for all of them.
+// face geometry is read once for all of them, over the half-face stream in
+// ascending face order: lane t of a block forms entry t's face terms
+// phi_f n (subtracted on the face's c2 side), with no load waiting on
+// another but the entry record; the cell's first lane then adds its entries'
+// terms in the cell's face order, so each sum is the reference's, term for
+// term.
 template <int NF>
 __global__ void __launch_bounds__(kBlock) k_gradient(DMesh m, GradArgs<NF> a) {
-  const int i = ordered_item(m.c_order, m.Nc);
-  if (i < 0) return;
-  double acc[NF][3];
-#pragma unroll
-  for (int k = 0; k < NF; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.;
-  const int e0 = m.cf_ptr[i], e1 = m.cf_ptr[i + 1];
-  for (int e = e0; e < e1; ++e) {
-    const int f = m.cf_sorted[e];
+  __shared__ int s_cell[kBlock];
+  __shared__ double s_t[3 * NF][kBlock];
+  const int b = hf_block();
+  const int E0 = m.hfs_blk[b], n = m.hfs_blk[b + 1] - E0;
+  const int t = threadIdx.x;
+  int i = -1;
+  if (t < n) {
+    const int4 r = m.hfs[E0 + t];
+    i = r.x;
+    const int f = r.y, nb = r.z;
     const double n0 = m.f_n[3 * f], n1 = m.f_n[3 * f + 1], n2 = m.f_n[3 * f + 2];
     if (f < m.Nb) {
 #pragma unroll
       for (int k = 0; k < NF; ++k) {
         const double pf = a.bphi[k][f];
-        acc[k][0] += pf * n0;
-        acc[k][1] += pf * n1;
-        acc[k][2] += pf * n2;
+        s_t[3 * k][t] = pf * n0;
+        s_t[3 * k + 1][t] = pf * n1;
+        s_t[3 * k + 2][t] = pf * n2;
       }
     } else {
-      const int c1 = m.f_c1[f], c2 = m.f_c2[f];
+      // phi_f = lambda phi[c1] + (1 - lambda) phi[c2]
+      const bool own = nb >= 0;
+      const int in = own ? nb : ~nb;
       const double lam = m.f_lambda[f];
       const double lam2 = 1. - lam;
-      if (c1 == i) {
 #pragma unroll
-        for (int k = 0; k < NF; ++k) {
-          const double pf = lam * a.phi[k][c1] + lam2 * a.phi[k][c2];
-          acc[k][0] += pf * n0;
-          acc[k][1] += pf * n1;
-          acc[k][2] += pf * n2;
-        }
-      } else {
-#pragma unroll
-        for (int k = 0; k < NF; ++k) {
-          const double pf = lam * a.phi[k][c1] + lam2 * a.phi[k][c2];
-          acc[k][0] -= pf * n0;
-          acc[k][1] -= pf * n1;
-          acc[k][2] -= pf * n2;
-        }
+      for (int k = 0; k < NF; ++k) {
+        const double pi = a.phi[k][i], pn = a.phi[k][in];
+        const double pf = own ? lam * pi + lam2 * pn : lam * pn + lam2 * pi;
+        const double t0 = pf * n0, t1 = pf * n1, t2 = pf * n2;
+        s_t[3 * k][t] = own ? t0 : -t0;
+        s_t[3 * k + 1][t] = own ? t1 : -t1;
+        s_t[3 * k + 2][t] = own ? t2 : -t2;
       }
+    }
+  }
+  s_cell[t] = i;
+  __syncthreads();
+  if (i < 0 || (t > 0 && s_cell[t - 1] == i)) return;
+  double acc[NF][3];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.;
+  for (int j = t; j < n && s_cell[j] == i; ++j) {
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      acc[k][0] += s_t[3 * k][j];
+      acc[k][1] += s_t[3 * k + 1][j];
+      acc[k][2] += s_t[3 * k + 2][j];
     }
   }
   const double vol = m.c_vol[i];
@@ -81,10 +96,10 @@ void launch_gradient(const DMesh& m, int nf, const double* const* phi, const dou
                      hipStream_t s) {
   if (nf == 1) {
     GradArgs<1> a{{phi[0]}, {bphi[0]}, {g[0]}};
-    hipLaunchKernelGGL(k_gradient<1>, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, a);
+    hipLaunchKernelGGL(k_gradient<1>, dim3(m.hfs_nblk), dim3(kBlock), 0, s, m, a);
   } else if (nf == 4) {
     GradArgs<4> a{{phi[0], phi[1], phi[2], phi[3]}, {bphi[0], bphi[1], bphi[2], bphi[3]}, {g[0], g[1], g[2], g[3]}};
-    hipLaunchKernelGGL(k_gradient<4>, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, a);
+    hipLaunchKernelGGL(k_gradient<4>, dim3(m.hfs_nblk), dim3(kBlock), 0, s, m, a);
   } else {
     throw std::invalid_argument("gradient: 1 or 4 fields");
   }
@@ -332,49 +347,50 @@ struct MomArgs {
   double dt;
 };
 
-// BuildVelocityMatrix cell loop (navier_velocity.cpp:100-383); the time
-// term (107-123) comes first, as in the reference.
+// BuildVelocityMatrix cell loop (navier_velocity.cpp:100-383) over the
+// half-face stream in pattern order.  Lane t of a block evaluates entry t's
+// face terms -- the two additions to the diagonal, the three source terms
+// fde - fdi - fcs and the off-diagonal -- with every load one step after the
+// entry record; the cell's first lane then adds them in the cell's face
+// order, after the time term (107-123), so every sum is the reference's, term
+// for term (an absent term is +0, which leaves the non-negative diagonal
+// unchanged).
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_build_momentum(DMesh m, DFields F, MomArgs a) {
-  const int i = ordered_item(m.c_order, m.Nc);
-  if (i < 0) return;
-  R app = R(0), su = R(0), sv = R(0), sw = R(0);
-  if (a.scheme == 1) {
-    const R coef = R(F.R[i]) / R(a.dt) * R(m.c_vol[i]);
-    app += coef;
-    su += coef * R(F.Unp[i]);
-    sv += coef * R(F.Vnp[i]);
-    sw += coef * R(F.Wnp[i]);
-  } else if (a.scheme == 2) {
-    const R coef = R(F.R[i]) / R(a.dt) * R(m.c_vol[i]);
-    app += R(1.5) * coef;
-    su += coef * (R(2) * R(F.Unp[i]) - R(0.5) * R(F.Unp2[i]));
-    sv += coef * (R(2) * R(F.Vnp[i]) - R(0.5) * R(F.Vnp2[i]));
-    sw += coef * (R(2) * R(F.Wnp[i]) - R(0.5) * R(F.Wnp2[i]));
-  }
-  const R xi0 = R(m.c_x[3 * i]), xi1 = R(m.c_x[3 * i + 1]), xi2 = R(m.c_x[3 * i + 2]);
-  const R Ui = R(F.U[i]), Vi = R(F.V[i]), Wi = R(F.W[i]);
-  const R mui = R(F.mu[i]) + R(F.mut[i]);
-  const double* gu = &F.dUdX[3 * i];
-  const double* gv = &F.dVdX[3 * i];
-  const double* gw = &F.dWdX[3 * i];
-  const R gu0 = R(gu[0]), gu1 = R(gu[1]), gu2 = R(gu[2]);
-  const R gv0 = R(gv[0]), gv1 = R(gv[1]), gv2 = R(gv[2]);
-  const R gw0 = R(gw[0]), gw1 = R(gw[1]), gw2 = R(gw[2]);
-  const int e0 = m.cf_ptr[i], e1 = m.cf_ptr[i + 1];
-  for (int e = e0; e < e1; ++e) {
-    const int f = m.cf_pat[e];
+  __shared__ int s_cell[kBlock];
+  __shared__ R s_t[5][kBlock];
+  const int b = hf_block();
+  const int E0 = m.hf_blk[b], n = m.hf_blk[b + 1] - E0;
+  const int t = threadIdx.x;
+  int i = -1;
+  R Ui = R(0), Vi = R(0), Wi = R(0);
+  if (t < n) {
+    const int4 r = m.hf[E0 + t];
+    i = r.x;
+    const int f = r.y, nb = r.z;
+    const R xi0 = R(m.c_x[3 * i]), xi1 = R(m.c_x[3 * i + 1]), xi2 = R(m.c_x[3 * i + 2]);
+    Ui = R(F.U[i]);
+    Vi = R(F.V[i]);
+    Wi = R(F.W[i]);
+    const R mui = R(F.mu[i]) + R(F.mut[i]);
+    const double* gu = &F.dUdX[3 * i];
+    const double* gv = &F.dVdX[3 * i];
+    const double* gw = &F.dWdX[3 * i];
+    const R gu0 = R(gu[0]), gu1 = R(gu[1]), gu2 = R(gu[2]);
+    const R gv0 = R(gv[0]), gv1 = R(gv[1]), gv2 = R(gv[2]);
+    const R gw0 = R(gw[0]), gw1 = R(gw[1]), gw2 = R(gw[2]);
+    R app1 = R(0), app2 = R(0);
     R fde0, fde1, fde2, fdi0, fdi1, fdi2, fcs0, fcs1, fcs2;
     const R fx0 = R(m.f_x[3 * f]), fx1 = R(m.f_x[3 * f + 1]), fx2 = R(m.f_x[3 * f + 2]);
     if (f < m.Nb) {
-      const int t = m.b_type[f];
+      const int bt = m.b_type[f];
       const R s1 = R(m.f_n[3 * f]), s2 = R(m.f_n[3 * f + 1]), s3 = R(m.f_n[3 * f + 2]);
       const R RUn = R(F.RUFace[f]);
       const R Visc = mui;
       const R dxc0 = fx0 - xi0, dxc1 = fx1 - xi1, dxc2 = fx2 - xi2;
       const R val = Visc * R(m.f_rlencos[f]);
-      if (t == kWall) {
-        app += val;
+      if (bt == kWall) {
+        app1 = val;
         R sh0 = (gu0 + gu0) * s1 + (gu1 + gv0) * s2 + (gu2 + gw0) * s3;
         R sh1 = (gv0 + gu1) * s1 + (gv1 + gv1) * s2 + (gv2 + gw1) * s3;
         R sh2 = (gw0 + gu2) * s1 + (gw1 + gv2) * s2 + (gw2 + gw2) * s3;
@@ -394,8 +410,8 @@ __global__ void __launch_bounds__(kBlock) k_build_momentum(DMesh m, DFields F, M
         fdi0 = val * Ui;
         fdi1 = val * Vi;
         fdi2 = val * Wi;
-      } else if (t >= kInlet && t <= kSym) {
-        app += val;
+      } else if (bt >= kInlet && bt <= kSym) {
+        app1 = val;
         fde0 = (gu0 + gu0) * s1 + (gu1 + gv0) * s2 + (gu2 + gw0) * s3;
         fde1 = (gv0 + gu1) * s1 + (gv1 + gv1) * s2 + (gv2 + gw1) * s3;
         fde2 = (gw0 + gu2) * s1 + (gw1 + gv2) * s2 + (gw2 + gw2) * s3;
@@ -409,9 +425,9 @@ __global__ void __launch_bounds__(kBlock) k_build_momentum(DMesh m, DFields F, M
         *a.bad = 1;
         fde0 = fde1 = fde2 = fdi0 = fdi1 = fdi2 = R(0);
       }
-      if (t == kInlet || t == kOutlet) {
+      if (bt == kInlet || bt == kOutlet) {
         const R fl = fmin(RUn, R(0));
-        app -= fl;
+        app2 = -fl;
         fcs0 = fl * R(F.BU[f]);
         fcs1 = fl * R(F.BV[f]);
         fcs2 = fl * R(F.BW[f]);
@@ -419,9 +435,8 @@ __global__ void __launch_bounds__(kBlock) k_build_momentum(DMesh m, DFields F, M
         fcs0 = fcs1 = fcs2 = R(0);
       }
     } else {
-      const int c1 = m.f_c1[f];
-      const bool own = (c1 == i);
-      const int in = own ? m.f_c2[f] : c1;
+      const bool own = nb >= 0;
+      const int in = own ? nb : ~nb;
       R s1 = R(m.f_n[3 * f]), s2 = R(m.f_n[3 * f + 1]), s3 = R(m.f_n[3 * f + 2]);
       R RUn = R(F.RUFace[f]);
       R lambda = R(m.f_lambda[f]);
@@ -452,32 +467,23 @@ __global__ void __launch_bounds__(kBlock) k_build_momentum(DMesh m, DFields F, M
       R apn = R(0);
       if (RUn < R(0)) {
         apn += RUn;
-        app -= RUn;
+        app1 = -RUn;
       }
       const R val = Visc * R(m.f_rlencos[f]);
-      app += val;
+      app2 = val;
       apn += -val;
-      F.Au[m.cf_slot[e]] = apn;
+      F.Au[r.w] = apn;
       const R xn0 = R(m.c_x[3 * in]), xn1 = R(m.c_x[3 * in + 1]), xn2 = R(m.c_x[3 * in + 2]);
       const R Un = R(F.U[in]), Vn = R(F.V[in]), Wn = R(F.W[in]);
-      R ufi, vfi, wfi, ufh, vfh, wfh;
-      if (RUn > R(0)) {
-        ufi = Ui;
-        vfi = Vi;
-        wfi = Wi;
-        const R dx0 = fx0 - xi0, dx1 = fx1 - xi1, dx2 = fx2 - xi2;
-        ufh = Ui + (gu0 * dx0 + gu1 * dx1 + gu2 * dx2);
-        vfh = Vi + (gv0 * dx0 + gv1 * dx1 + gv2 * dx2);
-        wfh = Wi + (gw0 * dx0 + gw1 * dx1 + gw2 * dx2);
-      } else {
-        ufi = Un;
-        vfi = Vn;
-        wfi = Wn;
-        const R dx0 = fx0 - xn0, dx1 = fx1 - xn1, dx2 = fx2 - xn2;
-        ufh = Un + (hu0 * dx0 + hu1 * dx1 + hu2 * dx2);
-        vfh = Vn + (hv0 * dx0 + hv1 * dx1 + hv2 * dx2);
-        wfh = Wn + (hw0 * dx0 + hw1 * dx1 + hw2 * dx2);
-      }
+      // both upwind face values evaluated and one selected, so the
+      // neighbour's velocity loads with its other data
+      const bool up = RUn > R(0);
+      const R dxi0 = fx0 - xi0, dxi1 = fx1 - xi1, dxi2 = fx2 - xi2;
+      const R dxn0 = fx0 - xn0, dxn1 = fx1 - xn1, dxn2 = fx2 - xn2;
+      const R ufi = up ? Ui : Un, vfi = up ? Vi : Vn, wfi = up ? Wi : Wn;
+      const R ufh = up ? Ui + (gu0 * dxi0 + gu1 * dxi1 + gu2 * dxi2) : Un + (hu0 * dxn0 + hu1 * dxn1 + hu2 * dxn2);
+      const R vfh = up ? Vi + (gv0 * dxi0 + gv1 * dxi1 + gv2 * dxi2) : Vn + (hv0 * dxn0 + hv1 * dxn1 + hv2 * dxn2);
+      const R wfh = up ? Wi + (gw0 * dxi0 + gw1 * dxi1 + gw2 * dxi2) : Wn + (hw0 * dxn0 + hw1 * dxn1 + hw2 * dxn2);
       fcs0 = RUn * (ufh - ufi);
       fcs1 = RUn * (vfh - vfi);
       fcs2 = RUn * (wfh - wfi);
@@ -492,9 +498,35 @@ __global__ void __launch_bounds__(kBlock) k_build_momentum(DMesh m, DFields F, M
       fdi1 = val * (dvdx * dxc0 + dvdy * dxc1 + dvdz * dxc2);
       fdi2 = val * (dwdx * dxc0 + dwdy * dxc1 + dwdz * dxc2);
     }
-    su += fde0 - fdi0 - fcs0;
-    sv += fde1 - fdi1 - fcs1;
-    sw += fde2 - fdi2 - fcs2;
+    s_t[0][t] = app1;
+    s_t[1][t] = app2;
+    s_t[2][t] = fde0 - fdi0 - fcs0;
+    s_t[3][t] = fde1 - fdi1 - fcs1;
+    s_t[4][t] = fde2 - fdi2 - fcs2;
+  }
+  s_cell[t] = i;
+  __syncthreads();
+  if (i < 0 || (t > 0 && s_cell[t - 1] == i)) return;
+  R app = R(0), su = R(0), sv = R(0), sw = R(0);
+  if (a.scheme == 1) {
+    const R coef = R(F.R[i]) / R(a.dt) * R(m.c_vol[i]);
+    app += coef;
+    su += coef * R(F.Unp[i]);
+    sv += coef * R(F.Vnp[i]);
+    sw += coef * R(F.Wnp[i]);
+  } else if (a.scheme == 2) {
+    const R coef = R(F.R[i]) / R(a.dt) * R(m.c_vol[i]);
+    app += R(1.5) * coef;
+    su += coef * (R(2) * R(F.Unp[i]) - R(0.5) * R(F.Unp2[i]));
+    sv += coef * (R(2) * R(F.Vnp[i]) - R(0.5) * R(F.Vnp2[i]));
+    sw += coef * (R(2) * R(F.Wnp[i]) - R(0.5) * R(F.Wnp2[i]));
+  }
+  for (int j = t; j < n && s_cell[j] == i; ++j) {
+    app += s_t[0][j];
+    app += s_t[1][j];
+    su += s_t[2][j];
+    sv += s_t[3][j];
+    sw += s_t[4][j];
   }
   app /= R(a.urf);
   F.apr[i] = R(1) / (app);
@@ -511,9 +543,9 @@ void launch_build_momentum(const DMesh& m, const DFields& F, double urf, const d
                            double dt, bool fp32, hipStream_t s) {
   MomArgs a{urf, grav[0], grav[1], grav[2], bad, scheme, dt};
   if (fp32)
-    hipLaunchKernelGGL(k_build_momentum<float>, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, F, a);
+    hipLaunchKernelGGL(k_build_momentum<float>, dim3(m.hf_nblk), dim3(kBlock), 0, s, m, F, a);
   else
-    hipLaunchKernelGGL(k_build_momentum<double>, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, F, a);
+    hipLaunchKernelGGL(k_build_momentum<double>, dim3(m.hf_nblk), dim3(kBlock), 0, s, m, F, a);
   S355_LAUNCH_CHECK();
 }
 
@@ -569,48 +601,61 @@ void launch_ruface(const DMesh& m, const DFields& F, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------- pressure-correction assembly
-// BuildPressureMatrix, DensityModel 0 (navier_pressure.cpp:106-213).
+// BuildPressureMatrix, DensityModel 0 (navier_pressure.cpp:106-213), over
+// the half-face stream in pattern order: lane t forms entry t's
+// off-diagonal and flux term, the cell's first lane sums them in face order.
+// The diagonal and the mass imbalance accumulate in FP64 whatever R is: the
+// diagonal then equals minus the sum of the stored off-diagonals, so A 1 = 0
+// holds for FP32 coefficients too.
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_build_pressure(DMesh m, DFields F) {
-  const int i = ordered_item(m.c_order, m.Nc);
-  if (i < 0) return;
-  // the diagonal and the mass imbalance accumulate in FP64 whatever R is:
-  // the diagonal then equals minus the sum of the stored off-diagonals, so
-  // A 1 = 0 holds for FP32 coefficients too
-  double valcen = 0., bpv = 0.;
-  const R rai = R(F.R[i]) * R(F.apr[i]);
-  const R voli = R(m.c_vol[i]);
-  const int e0 = m.cf_ptr[i], e1 = m.cf_ptr[i + 1];
-  for (int e = e0; e < e1; ++e) {
-    const int f = m.cf_pat[e];
-    const int c1 = m.f_c1[f];
+  __shared__ int s_cell[kBlock];
+  __shared__ double s_ru[kBlock], s_val[kBlock];
+  const int b = hf_block();
+  const int E0 = m.hf_blk[b], n = m.hf_blk[b + 1] - E0;
+  const int t = threadIdx.x;
+  int i = -1;
+  if (t < n) {
+    const int4 r = m.hf[E0 + t];
+    i = r.x;
+    const int f = r.y, nb = r.z;
+    const bool own = nb >= 0;  // the cell is the face's c1 (always on a boundary face)
+    const int in = own ? nb : ~nb;
     const double ru = F.RUFace[f];
-    if (i == c1)
-      bpv += ru;
-    else
-      bpv -= ru;
-    if (f < m.Nb) continue;
-    R roapf, vol;
-    if constexpr (std::is_same<R, float>::value) {
-      // FP32: evaluated from the face's c1 side in both of its rows, so the
-      // two rows store the same value.  The reference's per-row orientation
-      // rounds differently on the two sides; at FP32 that asymmetry (~1e-7)
-      // leaves the singular system of a closed domain inconsistent at ~1e-9
-      // of the right-hand side, and a 1e-9 solve can then not converge
-      const int c2 = m.f_c2[f];
-      const R l1 = R(m.f_lambda[f]), l2 = R(1) - l1;
-      roapf = R(F.R[c1]) * R(F.apr[c1]) * l1 + R(F.R[c2]) * R(F.apr[c2]) * l2;
-      vol = R(m.c_vol[c1]) * l1 + R(m.c_vol[c2]) * l2;
-    } else {
-      const int in = (i == c1) ? m.f_c2[f] : c1;
-      const R lambda = (i == c1) ? R(m.f_lambda[f]) : R(1) - R(m.f_lambda[f]);
-      const R lambda2 = R(1) - lambda;
-      roapf = rai * lambda + R(F.R[in]) * R(F.apr[in]) * lambda2;
-      vol = voli * lambda + R(m.c_vol[in]) * lambda2;
+    s_ru[t] = own ? ru : -ru;
+    double dv = 0.;  // boundary faces add nothing to the diagonal
+    if (f >= m.Nb) {
+      R roapf, vol;
+      if constexpr (std::is_same<R, float>::value) {
+        // FP32: evaluated from the face's c1 side in both of its rows, so the
+        // two rows store the same value.  The reference's per-row orientation
+        // rounds differently on the two sides; at FP32 that asymmetry (~1e-7)
+        // leaves the singular system of a closed domain inconsistent at ~1e-9
+        // of the right-hand side, and a 1e-9 solve can then not converge
+        const int c1 = own ? i : in, c2 = own ? in : i;
+        const R l1 = R(m.f_lambda[f]), l2 = R(1) - l1;
+        roapf = R(F.R[c1]) * R(F.apr[c1]) * l1 + R(F.R[c2]) * R(F.apr[c2]) * l2;
+        vol = R(m.c_vol[c1]) * l1 + R(m.c_vol[c2]) * l2;
+      } else {
+        const R rai = R(F.R[i]) * R(F.apr[i]);
+        const R lambda = own ? R(m.f_lambda[f]) : R(1) - R(m.f_lambda[f]);
+        const R lambda2 = R(1) - lambda;
+        roapf = rai * lambda + R(F.R[in]) * R(F.apr[in]) * lambda2;
+        vol = R(m.c_vol[i]) * lambda + R(m.c_vol[in]) * lambda2;
+      }
+      const R tmp = roapf * vol * R(m.f_rlencos[f]);
+      dv = -static_cast<double>(tmp);
+      F.Ap[r.w] = tmp;
     }
-    const R tmp = roapf * vol * R(m.f_rlencos[f]);
-    valcen += -static_cast<double>(tmp);
-    F.Ap[m.cf_slot[e]] = tmp;
+    s_val[t] = dv;
+  }
+  s_cell[t] = i;
+  __syncthreads();
+  if (i < 0 || (t > 0 && s_cell[t - 1] == i)) return;
+  double valcen = 0., bpv = 0.;
+  for (int j = t; j < n && s_cell[j] == i; ++j) {
+    bpv += s_ru[j];
+    valcen += s_val[j];
   }
   F.Ap[m.diag[i]] = valcen;
   F.bp[i] = bpv;
@@ -618,9 +663,9 @@ __global__ void __launch_bounds__(kBlock) k_build_pressure(DMesh m, DFields F) {
 
 void launch_build_pressure(const DMesh& m, const DFields& F, bool fp32, hipStream_t s) {
   if (fp32)
-    hipLaunchKernelGGL(k_build_pressure<float>, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, F);
+    hipLaunchKernelGGL(k_build_pressure<float>, dim3(m.hf_nblk), dim3(kBlock), 0, s, m, F);
   else
-    hipLaunchKernelGGL(k_build_pressure<double>, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, F);
+    hipLaunchKernelGGL(k_build_pressure<double>, dim3(m.hf_nblk), dim3(kBlock), 0, s, m, F);
   S355_LAUNCH_CHECK();
 }
 

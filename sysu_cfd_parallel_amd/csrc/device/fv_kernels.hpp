@@ -29,6 +29,19 @@ struct DMesh {
   const int* cf_pat = nullptr;
   const int* cf_slot = nullptr;
   const int* cf_sorted = nullptr;
+  // half-face streams: every cell's entries, cells in processing order
+  // (c_order), one record {cell, face, neighbour code, CSR slot (-1 on a
+  // boundary face)} per entry, the neighbour code being c2 (>= 0) where the
+  // cell is the face's c1, ~c1 (< 0) where it is c2, and 0 on boundary faces;
+  // hf in pattern order (the assembly loops), hfs in ascending face order
+  // (the gradient).  Block b of a half-face kernel takes entries
+  // [blk[b], blk[b + 1]): at most kBlock of them, whole cells only.
+  const int4* hf = nullptr;
+  const int* hf_blk = nullptr;
+  int hf_nblk = 0;
+  const int4* hfs = nullptr;
+  const int* hfs_blk = nullptr;
+  int hfs_nblk = 0;
   // CSR
   const int* rowptr = nullptr;
   const int* col = nullptr;

@@ -154,50 +154,56 @@ void launch_energy_coef(const DMesh& m, const DFields& F, const KepsWork& w, dou
   S355_LAUNCH_CHECK();
 }
 
-// BuildScalarMatrix cell loop (navier_scalar.cpp:421-589).  The gradient of
-// phi is in F.dPhidX.  bad[0] flags unknown boundary types, bad[1] an inlet
-// with outflow (the reference exits there, navier_scalar.cpp:476-479).
+// BuildScalarMatrix cell loop (navier_scalar.cpp:421-589) over the
+// half-face stream in pattern order: lane t evaluates entry t's face terms
+// (the two additions to the diagonal, fde - fdi - fcs and the off-diagonal),
+// the cell's first lane adds them in the cell's face order, so every sum is
+// the reference's, term for term.  The gradient of phi is in F.dPhidX.
+// bad[0] flags unknown boundary types, bad[1] an inlet with outflow (the
+// reference exits there, navier_scalar.cpp:476-479).
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_build_scalar(DMesh m, DFields F, const double* phi, const double* bphi,
                                                          const double* diff, const double* source,
                                                          const double* app_in, double urf, int* bad) {
-  const int i = ordered_item(m.c_order, m.Nc);
-  if (i < 0) return;
-  R app = R(app_in[i]);
-  R sphi = R(source[i]);
-  const R xi0 = R(m.c_x[3 * i]), xi1 = R(m.c_x[3 * i + 1]), xi2 = R(m.c_x[3 * i + 2]);
-  const double* g = &F.dPhidX[3 * i];
-  const R g0 = R(g[0]), g1 = R(g[1]), g2 = R(g[2]);
-  const R phii = R(phi[i]);
-  const R difi = R(diff[i]);
-  const int e0 = m.cf_ptr[i], e1 = m.cf_ptr[i + 1];
-  for (int e = e0; e < e1; ++e) {
-    const int f = m.cf_pat[e];
+  __shared__ int s_cell[kBlock];
+  __shared__ R s_t[3][kBlock];
+  const int b = hf_block();
+  const int E0 = m.hf_blk[b], n = m.hf_blk[b + 1] - E0;
+  const int t = threadIdx.x;
+  int i = -1;
+  if (t < n) {
+    const int4 r = m.hf[E0 + t];
+    i = r.x;
+    const int f = r.y, nb = r.z;
+    const R xi0 = R(m.c_x[3 * i]), xi1 = R(m.c_x[3 * i + 1]), xi2 = R(m.c_x[3 * i + 2]);
+    const double* g = &F.dPhidX[3 * i];
+    const R g0 = R(g[0]), g1 = R(g[1]), g2 = R(g[2]);
+    const R phii = R(phi[i]);
+    const R difi = R(diff[i]);
     const R fx0 = R(m.f_x[3 * f]), fx1 = R(m.f_x[3 * f + 1]), fx2 = R(m.f_x[3 * f + 2]);
-    R fde, fdi, fcs;
+    R fde, fdi, fcs, app1 = R(0), app2 = R(0);
     if (f < m.Nb) {
-      const int t = m.b_type[f];
+      const int bt = m.b_type[f];
       const R s1 = R(m.f_n[3 * f]), s2 = R(m.f_n[3 * f + 1]), s3 = R(m.f_n[3 * f + 2]);
       const R RUn = R(F.RUFace[f]);
       const R Visc = difi;
       const R dxc0 = fx0 - xi0, dxc1 = fx1 - xi1, dxc2 = fx2 - xi2;
-      if (t < kWall || t > kSym) bad[0] = 1;
+      if (bt < kWall || bt > kSym) bad[0] = 1;
       const R val = Visc * R(m.f_rlencos[f]);
-      app += val;
+      app1 = val;
       fde = Visc * (g0 * s1 + g1 * s2 + g2 * s3);
       fdi = val * (g0 * dxc0 + g1 * dxc1 + g2 * dxc2 - R(bphi[f]));
-      if (t == kInlet || t == kOutlet) {
-        if (t == kInlet && RUn > 0) bad[1] = 1;
+      if (bt == kInlet || bt == kOutlet) {
+        if (bt == kInlet && RUn > 0) bad[1] = 1;
         const R fl = fmin(RUn, R(0));
-        app -= fl;
+        app2 = -fl;
         fcs = fl * R(bphi[f]);
       } else {
         fcs = R(0);
       }
     } else {
-      const int c1 = m.f_c1[f];
-      const bool own = (c1 == i);
-      const int in = own ? m.f_c2[f] : c1;
+      const bool own = nb >= 0;
+      const int in = own ? nb : ~nb;
       R s1 = R(m.f_n[3 * f]), s2 = R(m.f_n[3 * f + 1]), s3 = R(m.f_n[3 * f + 2]);
       R RUn = R(F.RUFace[f]);
       R lambda = R(m.f_lambda[f]);
@@ -218,43 +224,51 @@ __global__ void __launch_bounds__(kBlock) k_build_scalar(DMesh m, DFields F, con
       R apn = R(0);
       if (RUn < R(0)) {
         apn += RUn;
-        app -= RUn;
+        app1 = -RUn;
       }
       const R val = Visc * R(m.f_rlencos[f]);
-      app += val;
+      app2 = val;
       apn += -val;
-      F.As[m.cf_slot[e]] = apn;
+      F.As[r.w] = apn;
       const R xn0 = R(m.c_x[3 * in]), xn1 = R(m.c_x[3 * in + 1]), xn2 = R(m.c_x[3 * in + 2]);
-      R pfi, pfh;
-      if (RUn > R(0)) {
-        pfi = phii;
-        const R dx0 = fx0 - xi0, dx1 = fx1 - xi1, dx2 = fx2 - xi2;
-        pfh = phii + (g0 * dx0 + g1 * dx1 + g2 * dx2);
-      } else {
-        const R pn = R(phi[in]);
-        pfi = pn;
-        const R dx0 = fx0 - xn0, dx1 = fx1 - xn1, dx2 = fx2 - xn2;
-        pfh = pn + (h0 * dx0 + h1 * dx1 + h2 * dx2);
-      }
+      // both upwind values evaluated and one selected (the neighbour's phi
+      // is loaded with its other data)
+      const R pn = R(phi[in]);
+      const bool up = RUn > R(0);
+      const R pfi = up ? phii : pn;
+      const R pfh = up ? phii + (g0 * (fx0 - xi0) + g1 * (fx1 - xi1) + g2 * (fx2 - xi2))
+                       : pn + (h0 * (fx0 - xn0) + h1 * (fx1 - xn1) + h2 * (fx2 - xn2));
       fcs = RUn * (pfh - pfi);
       fde = Visc * (dpx * s1 + dpy * s2 + dpz * s3);
       const R dxc0 = xn0 - xi0, dxc1 = xn1 - xi1, dxc2 = xn2 - xi2;
       fdi = val * (dpx * dxc0 + dpy * dxc1 + dpz * dxc2);
     }
-    sphi += fde - fdi - fcs;
+    s_t[0][t] = app1;
+    s_t[1][t] = app2;
+    s_t[2][t] = fde - fdi - fcs;
+  }
+  s_cell[t] = i;
+  __syncthreads();
+  if (i < 0 || (t > 0 && s_cell[t - 1] == i)) return;
+  R app = R(app_in[i]);
+  R sphi = R(source[i]);
+  for (int j = t; j < n && s_cell[j] == i; ++j) {
+    app += s_t[0][j];
+    app += s_t[1][j];
+    sphi += s_t[2][j];
   }
   app /= R(urf);
   F.As[m.diag[i]] = app;
-  F.bs[i] = sphi + (R(1) - R(urf)) * app * phii;
+  F.bs[i] = sphi + (R(1) - R(urf)) * app * R(phi[i]);
 }
 
 void launch_build_scalar(const DMesh& m, const DFields& F, double* phi, const double* bphi, const double* diff,
                          const double* source, const double* app_in, double urf, int* bad, bool fp32, hipStream_t s) {
   if (fp32)
-    hipLaunchKernelGGL(k_build_scalar<float>, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, F, phi, bphi, diff, source,
+    hipLaunchKernelGGL(k_build_scalar<float>, dim3(m.hf_nblk), dim3(kBlock), 0, s, m, F, phi, bphi, diff, source,
                        app_in, urf, bad);
   else
-    hipLaunchKernelGGL(k_build_scalar<double>, dim3(grid_for(m.Nc)), dim3(kBlock), 0, s, m, F, phi, bphi, diff, source,
+    hipLaunchKernelGGL(k_build_scalar<double>, dim3(m.hf_nblk), dim3(kBlock), 0, s, m, F, phi, bphi, diff, source,
                        app_in, urf, bad);
   S355_LAUNCH_CHECK();
 }
