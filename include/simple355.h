@@ -295,6 +295,25 @@ int s355_velocity_residual(s355_ctx* ctx);
  * reset != 0 zeroes them afterwards. Synchronous. */
 int s355_residuals(s355_ctx* ctx, double res[10], int reset);
 
+/* The device work between the solves of one SIMPLE pass (navier.cpp:60-101)
+ * in phases.  With params.graphs = 1 a phase is captured as one hipGraph on
+ * its second call and replayed after (s355_ctx_set_params drops the graphs);
+ * with 0 it runs kernel by kernel.  Either way it is exactly the member
+ * calls listed, in that order, so results are the same bits.  No reference
+ * counterpart: the reference calls the members one by one. */
+enum s355_phase {
+  S355_PHASE_MOMENTUM = 0,       /* build_velocity_matrix, save_velocity */
+  S355_PHASE_VELOCITY_RESIDUAL,  /* velocity_residual */
+  S355_PHASE_PRESSURE,           /* bc_velocity, cal_ru_face, build_pressure_matrix */
+  S355_PHASE_CORRECT,            /* correct_fields (res_p NULL), bc_velocity, correct_ru_face */
+  S355_PHASE_KEPS_K,             /* kepsilon_sources, bc_kepsilon, unsteady_sources(2)
+                                    (unsteady runs), build_scalar_matrix(2) */
+  S355_PHASE_KEPS_EPS,           /* kepsilon_post_k, build_scalar_matrix(3), eps_wall_rows */
+  S355_PHASE_KEPS_POST,          /* kepsilon_post_eps */
+  S355_PHASE_COUNT
+};
+int s355_run_phase(s355_ctx* ctx, int phase);
+
 /* k-epsilon pieces of UpdateTurKEpsilon (navier_scalar.cpp:11-197). */
 int s355_kepsilon_sources(s355_ctx* ctx);                  /* 33-64: mu_k, mu_eps, P_k, S_k, Ap_k */
 int s355_bc_kepsilon(s355_ctx* ctx);                       /* SetBCKEpsilon navier_bc.cpp:220-345 */

@@ -16,8 +16,9 @@ import numpy as np
 
 from . import _abi
 from ._abi import (FIELD_ID, FIELD_KIND, MESH_DBL_ID, MESH_INT_ID, FP64, KSP_CG, KSP_GMRES, KSP_PRESSURE,
-                   KSP_SCALAR, KSP_VELOCITY, MIXED, PC_ILU0, PC_JACOBI, PC_NONE, KspOpts, MeshDesc, Params, Region,
-                   Sizes, SolveInfo, field_len, mesh_dbl_len, mesh_int_len)
+                   KSP_SCALAR, KSP_VELOCITY, MIXED, PC_ILU0, PC_JACOBI, PC_NONE, PHASE_CORRECT, PHASE_KEPS_EPS,
+                   PHASE_KEPS_K, PHASE_KEPS_POST, PHASE_MOMENTUM, PHASE_PRESSURE, PHASE_VELOCITY_RESIDUAL, KspOpts,
+                   MeshDesc, Params, Region, Sizes, SolveInfo, field_len, mesh_dbl_len, mesh_int_len)
 
 LIB_PATH = Path(__file__).resolve().parent / "_build" / "libsimple355.so"
 
@@ -25,7 +26,8 @@ __all__ = [
     "LIB_PATH", "library", "CaseData", "cavity", "channel", "step", "read_mesh", "write_mesh", "default_params",
     "set_all_ksp", "HostMesh",
     "Solver", "BatchSolver", "S355Error", "NotConverged", "FP64", "MIXED", "KSP_GMRES", "KSP_CG", "PC_NONE", "PC_JACOBI",
-    "PC_ILU0", "KSP_VELOCITY", "KSP_PRESSURE", "KSP_SCALAR",
+    "PC_ILU0", "KSP_VELOCITY", "KSP_PRESSURE", "KSP_SCALAR", "PHASE_MOMENTUM", "PHASE_VELOCITY_RESIDUAL",
+    "PHASE_PRESSURE", "PHASE_CORRECT", "PHASE_KEPS_K", "PHASE_KEPS_EPS", "PHASE_KEPS_POST",
 ]
 
 
@@ -85,6 +87,7 @@ def library(path: Optional[Path] = None) -> C.CDLL:
         "synchronize": (I, [V]),
         "save_velocity": (I, [V]),
         "velocity_residual": (I, [V]),
+        "run_phase": (I, [V, I]),
         "residuals": (I, [V, pD, I]),
         "gradient_noalloc": (I, [V, I, I, I]),
         "kepsilon_sources": (I, [V]),

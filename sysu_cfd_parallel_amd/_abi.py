@@ -20,6 +20,9 @@ KSP_GMRES, KSP_CG = 0, 1
 PC_NONE, PC_JACOBI, PC_ILU0 = 0, 1, 2
 FP64, MIXED = 0, 1
 KSP_VELOCITY, KSP_PRESSURE, KSP_SCALAR = 0, 1, 2
+# s355_run_phase
+(PHASE_MOMENTUM, PHASE_VELOCITY_RESIDUAL, PHASE_PRESSURE, PHASE_CORRECT, PHASE_KEPS_K, PHASE_KEPS_EPS,
+ PHASE_KEPS_POST) = range(7)
 
 
 class Region(C.Structure):

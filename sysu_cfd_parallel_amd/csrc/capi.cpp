@@ -761,6 +761,9 @@ int s355_save_velocity(s355_ctx* ctx) { return guarded([&] { dev(ctx).save_veloc
 int s355_velocity_residual(s355_ctx* ctx) {
   return guarded([&] { dev(ctx).velocity_residual(); return S355_OK; });
 }
+int s355_run_phase(s355_ctx* ctx, int phase) {
+  return guarded([&] { dev(ctx).run_phase(phase); return S355_OK; });
+}
 
 int s355_residuals(s355_ctx* ctx, double res[10], int reset) {
   return guarded([&] {

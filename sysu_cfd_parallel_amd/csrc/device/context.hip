@@ -324,6 +324,9 @@ void Context::set_params(const s355_params& p) {
   par = p;
   if (par.limiter != 0) ensure_vertices();
   if (par.grad_scheme == 1) ensure_ls();
+  // captured phases hold the old parameters in their kernel arguments
+  phase_graphs_.reset();
+  phase_seen_ = 0;
 }
 
 FieldRef Context::field(int id) {
@@ -779,11 +782,60 @@ void Context::correct_fields() {
 
 void Context::correct_ru_face() { launch_correct_ruface(dm, df, par.urf[3], stream); }
 
-void Context::save_velocity() {
-  const size_t bytes = static_cast<size_t>(dm.Nc) * sizeof(double);
-  S355_HIP(hipMemcpyAsync(u0_, df.U, bytes, hipMemcpyDeviceToDevice, stream));
-  S355_HIP(hipMemcpyAsync(v0_, df.V, bytes, hipMemcpyDeviceToDevice, stream));
-  S355_HIP(hipMemcpyAsync(w0_, df.W, bytes, hipMemcpyDeviceToDevice, stream));
+void Context::save_velocity() { launch_copy3(dm.Nc, df.U, df.V, df.W, u0_, v0_, w0_, stream); }
+
+// The phases between the solves of a SIMPLE pass (navier.cpp:60-101).  Each
+// is the sequence of member calls the host driver makes there; captured, it
+// replays the same kernels with the same arguments, so both give the same
+// bits.  No phase allocates after its first call or reads anything back.
+void Context::phase_body(int phase) {
+  switch (phase) {
+    case S355_PHASE_MOMENTUM:
+      build_velocity_matrix();
+      save_velocity();
+      break;
+    case S355_PHASE_VELOCITY_RESIDUAL:
+      velocity_residual();
+      break;
+    case S355_PHASE_PRESSURE:
+      bc_velocity();
+      cal_ru_face();
+      build_pressure_matrix();
+      break;
+    case S355_PHASE_CORRECT:
+      correct_fields();
+      bc_velocity();
+      correct_ru_face();
+      break;
+    case S355_PHASE_KEPS_K:
+      kepsilon_sources();
+      bc_kepsilon();
+      unsteady_sources(2);  // nothing in steady runs
+      build_scalar_matrix(2);
+      break;
+    case S355_PHASE_KEPS_EPS:
+      kepsilon_post_k();
+      build_scalar_matrix(3);
+      eps_wall_rows();
+      break;
+    case S355_PHASE_KEPS_POST:
+      kepsilon_post_eps();
+      break;
+    default:
+      throw std::invalid_argument("run_phase: unknown phase");
+  }
+}
+
+void Context::run_phase(int phase) {
+  if (phase < 0 || phase >= S355_PHASE_COUNT) throw std::invalid_argument("run_phase: unknown phase");
+  if (!par.graphs || !(phase_seen_ >> phase & 1u)) {
+    phase_body(phase);
+    phase_seen_ |= 1u << phase;
+    return;
+  }
+  if (!phase_graphs_) phase_graphs_ = std::make_unique<GraphCache>();
+  hipGraphExec_t g = phase_graphs_->get({static_cast<uintptr_t>(phase)}, stream, [&] { phase_body(phase); });
+  S355_HIP(hipGraphLaunch(g, stream));
 }
 
 void Context::velocity_residual() {

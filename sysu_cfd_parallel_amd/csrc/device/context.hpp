@@ -69,6 +69,10 @@ class Context {
   void save_old_data();
   void unsteady_sources(int isca);
 
+  // one SIMPLE phase (S355_PHASE_*): the member calls it groups, launched
+  // from one captured hipGraph when params.graphs is 1
+  void run_phase(int phase);
+
   // residual slots and error flags on the device
   void zero_residuals();
   void read_residuals(double out[10]);          // element-wise maximum over the cases
@@ -149,6 +153,12 @@ class Context {
   float* osell32_[kSellVariants] = {nullptr, nullptr, nullptr};
   double* scratch_ = nullptr;  // this device's fixed scratch array (scratch.hpp)
   std::unique_ptr<GraphCache> graphs_;  // captured GMRES cycles (params.graphs)
+  // captured SIMPLE phases (params.graphs): a phase is captured on its second
+  // call (the first runs eagerly, so its lazy allocations happen outside a
+  // capture) and replayed after; set_params drops them
+  std::unique_ptr<GraphCache> phase_graphs_;
+  unsigned phase_seen_ = 0;
+  void phase_body(int phase);
   std::unique_ptr<SpmvTimer> timers_[2];  // in-solve SpMV timing, FP64 / FP32
   // SELL-64 copies of the matrix being solved, one per sorting window: patterns
   // and source maps built once, values filled before each GMRES solve that

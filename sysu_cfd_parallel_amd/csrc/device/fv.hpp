@@ -26,6 +26,9 @@ void launch_ruface(const DMesh& m, const DFields& F, hipStream_t s);
 void launch_build_pressure(const DMesh& m, const DFields& F, bool fp32, hipStream_t s);
 void launch_correct_fields(const DMesh& m, const DFields& F, int ref, double urf, double* partial, hipStream_t s);
 void launch_correct_ruface(const DMesh& m, const DFields& F, double urf, hipStream_t s);
+// b_k[i] = a_k[i], k = 0..2, i < n
+void launch_copy3(int n, const double* a0, const double* a1, const double* a2, double* b0, double* b1, double* b2,
+                  hipStream_t s);
 void launch_residual3(const DMesh& m, const double* u0, const double* v0, const double* w0, const double* u,
                       const double* v, const double* w, double* partial, hipStream_t s);
 // per-case sums of per-tile partials (ncomp interleaved columns) into

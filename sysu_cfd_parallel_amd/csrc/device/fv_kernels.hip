@@ -763,6 +763,23 @@ __global__ void __launch_bounds__(kBlock) k_residual3(DMesh m, const double* u0,
   }
 }
 
+// the velocity snapshot of CalculateVelocity (navier_velocity.cpp:19-24)
+__global__ void __launch_bounds__(kBlock) k_copy3(int n, const double* __restrict__ a0, const double* __restrict__ a1,
+                                                  const double* __restrict__ a2, double* __restrict__ b0,
+                                                  double* __restrict__ b1, double* __restrict__ b2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  b0[i] = a0[i];
+  b1[i] = a1[i];
+  b2[i] = a2[i];
+}
+
+void launch_copy3(int n, const double* a0, const double* a1, const double* a2, double* b0, double* b1, double* b2,
+                  hipStream_t s) {
+  hipLaunchKernelGGL(k_copy3, dim3(grid_for(n)), dim3(kBlock), 0, s, n, a0, a1, a2, b0, b1, b2);
+  S355_LAUNCH_CHECK();
+}
+
 void launch_residual3(const DMesh& m, const double* u0, const double* v0, const double* w0, const double* u,
                       const double* v, const double* w, double* partial, hipStream_t s) {
   hipLaunchKernelGGL(k_residual3, dim3(m.ctile.ntile), dim3(kBlock), 0, s, m, u0, v0, w0, u, v, w, partial);

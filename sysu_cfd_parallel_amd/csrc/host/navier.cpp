@@ -65,12 +65,18 @@ void NavierStokesSolver::BuildScalarMatrix(int iSca) {
   check(s355_build_scalar_matrix(ctx_, iSca), "BuildScalarMatrix");
 }
 
+// With params.graphs the device work between the solves runs as captured
+// phases (s355_run_phase): the same member calls in the same order.
 int NavierStokesSolver::CalculateVelocity() {
   check(s355_log_stage_push(ctx_, 1), "log");
-  BuildVelocityMatrix();
+  if (par_.graphs) {
+    check(s355_run_phase(ctx_, S355_PHASE_MOMENTUM), "BuildVelocityMatrix");
+  } else {
+    BuildVelocityMatrix();
+    check(s355_save_velocity(ctx_), "save velocity");
+  }
   check(s355_log_stage_pop(ctx_), "log");
   check(s355_log_stage_push(ctx_, 2), "log");
-  check(s355_save_velocity(ctx_), "save velocity");
   const s355_ksp_opts& o = par_.ksp[S355_KSP_VELOCITY];
   try {
     dataPartition.solveVelocity_GMRES(o.rtol, o.maxit, &last[0]);
@@ -80,16 +86,23 @@ int NavierStokesSolver::CalculateVelocity() {
   }
   check(s355_log_stage_pop(ctx_), "log");
   check(s355_log_stage_push(ctx_, 1), "log");
-  check(s355_velocity_residual(ctx_), "velocity residual");
+  if (par_.graphs)
+    check(s355_run_phase(ctx_, S355_PHASE_VELOCITY_RESIDUAL), "velocity residual");
+  else
+    check(s355_velocity_residual(ctx_), "velocity residual");
   check(s355_log_stage_pop(ctx_), "log");
   return 0;
 }
 
 int NavierStokesSolver::CalculatePressure() {
   check(s355_log_stage_push(ctx_, 3), "log");
-  SetBCVelocity();
-  CalRUFace2();
-  BuildPressureMatrix();
+  if (par_.graphs) {
+    check(s355_run_phase(ctx_, S355_PHASE_PRESSURE), "BuildPressureMatrix");
+  } else {
+    SetBCVelocity();
+    CalRUFace2();
+    BuildPressureMatrix();
+  }
   check(s355_log_stage_pop(ctx_), "log");
   check(s355_log_stage_push(ctx_, 4), "log");
   const s355_ksp_opts& o = par_.ksp[S355_KSP_PRESSURE];
@@ -102,9 +115,13 @@ int NavierStokesSolver::CalculatePressure() {
   check(s355_log_stage_pop(ctx_), "log");
   check(s355_log_stage_push(ctx_, 3), "log");
   // SetBCDeltaP, Gradient(deltaP) and the correction loop (navier_pressure.cpp:42-70)
-  check(s355_correct_fields(ctx_, nullptr), "correct fields");
-  SetBCVelocity();
-  CorrectRUFace2();
+  if (par_.graphs) {
+    check(s355_run_phase(ctx_, S355_PHASE_CORRECT), "correct fields");
+  } else {
+    check(s355_correct_fields(ctx_, nullptr), "correct fields");
+    SetBCVelocity();
+    CorrectRUFace2();
+  }
   check(s355_log_stage_pop(ctx_), "log");
   return 0;
 }
@@ -113,17 +130,28 @@ void NavierStokesSolver::UpdateTurKEpsilon() {
   check(s355_log_stage_push(ctx_, 5), "log");
   const s355_ksp_opts& o = par_.ksp[S355_KSP_SCALAR];
   try {
-    check(s355_kepsilon_sources(ctx_), "k-eps sources");
-    SetBCKEpsilon();
-    // time terms after the boundary treatment (navier_scalar.cpp:71-90)
-    if (!par_.steady) check(s355_unsteady_sources(ctx_, 2), "k-eps time terms");
-    BuildScalarMatrix(2);
+    if (par_.graphs) {
+      check(s355_run_phase(ctx_, S355_PHASE_KEPS_K), "k-eps sources / k system");
+    } else {
+      check(s355_kepsilon_sources(ctx_), "k-eps sources");
+      SetBCKEpsilon();
+      // time terms after the boundary treatment (navier_scalar.cpp:71-90)
+      if (!par_.steady) check(s355_unsteady_sources(ctx_, 2), "k-eps time terms");
+      BuildScalarMatrix(2);
+    }
     dataPartition.solveScarlar_GMRES(o.rtol, o.maxit, 2, &last[4]);
-    check(s355_kepsilon_post_k(ctx_), "k clip / eps sources");
-    BuildScalarMatrix(3);
-    check(s355_eps_wall_rows(ctx_), "eps wall rows");
+    if (par_.graphs) {
+      check(s355_run_phase(ctx_, S355_PHASE_KEPS_EPS), "eps system");
+    } else {
+      check(s355_kepsilon_post_k(ctx_), "k clip / eps sources");
+      BuildScalarMatrix(3);
+      check(s355_eps_wall_rows(ctx_), "eps wall rows");
+    }
     dataPartition.solveScarlar_GMRES(o.rtol, o.maxit, 3, &last[5]);
-    check(s355_kepsilon_post_eps(ctx_), "eps clip / mu_t");
+    if (par_.graphs)
+      check(s355_run_phase(ctx_, S355_PHASE_KEPS_POST), "eps clip / mu_t");
+    else
+      check(s355_kepsilon_post_eps(ctx_), "eps clip / mu_t");
   } catch (...) {
     s355_log_stage_pop(ctx_);
     throw;
