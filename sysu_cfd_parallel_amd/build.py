@@ -26,6 +26,7 @@ SOURCES = [
     "device/keps_kernels.hip",
     "device/transient_kernels.hip",
     "device/krylov.hip",
+    "device/geometry_kernels.hip",
     "device/scratch.hip",
     "device/context.hip",
     "host/navier.cpp",

@@ -1,6 +1,7 @@
 // Device context: owns the HBM copies of one case and runs the device-level
 // operations of the SIMPLE hot path on one HIP stream.
 #include "context.hpp"
+#include "geometry.hpp"
 #include "scratch.hpp"
 #include "../mesh/reorder.hpp"
 
@@ -53,7 +54,7 @@ Context::Context(const s355_mesh_desc* descs, int n_case, const s355_params& par
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw HipError("no HIP device available");
   if (ncase == 1) {
-    mesh = ingest_mesh(descs[0]);
+    mesh = ingest_mesh(descs[0], false);  // the geometry is computed on the device
     lay.cell = {0, mesh.n_cell};
     lay.bnd = {0, mesh.n_bnd};
     lay.iface = {0, mesh.n_face - mesh.n_bnd};
@@ -65,7 +66,7 @@ Context::Context(const s355_mesh_desc* descs, int n_case, const s355_params& par
   } else {
     std::vector<Mesh> parts;
     parts.reserve(ncase);
-    for (int k = 0; k < ncase; ++k) parts.push_back(ingest_mesh(descs[k]));
+    for (int k = 0; k < ncase; ++k) parts.push_back(ingest_mesh(descs[k], false));
     mesh = concat_meshes(parts, lay);
   }
   int min_cells = mesh.n_cell;
@@ -140,6 +141,93 @@ void Context::free_all() {
   stream = nullptr;
 }
 
+// CellFaceInfo (geometry.cpp:391-752) on the device, from the vertices and
+// the topology of the host ingest (which skipped the metrics), then copied
+// back into the host mesh for the host-side consumers (result files, the
+// monitor, mesh queries).
+void Context::compute_geometry() {
+  Mesh& m = mesh;
+  const size_t Nc = m.n_cell, Nf = m.n_face, Nb = m.n_bnd;
+  dm.Nv = m.n_vert;
+  dm.vert = upload(allocs_, m.vert, stream);
+  dm.c_vert = upload(allocs_, m.c_vert, stream);
+  // topology used by the geometry only: freed once it is done
+  uint8_t *poly = nullptr, *warped = nullptr;
+  int* fv = nullptr;
+  S355_HIP(hipMalloc(&poly, std::max<size_t>(Nc, 1)));
+  S355_HIP(hipMalloc(&warped, std::max<size_t>(Nf, 1)));
+  S355_HIP(hipMalloc(&fv, std::max<size_t>(4 * Nf, 1) * sizeof(int)));
+  std::vector<uint8_t> hw(Nf);
+  try {
+    S355_HIP(hipMemcpyAsync(poly, m.c_poly.data(), Nc, hipMemcpyHostToDevice, stream));
+    S355_HIP(hipMemcpyAsync(fv, m.f_v.data(), 4 * Nf * sizeof(int), hipMemcpyHostToDevice, stream));
+    GeomArgs g;
+    g.Nc = m.n_cell;
+    g.Nf = m.n_face;
+    g.Nb = m.n_bnd;
+    g.vert = dm.vert;
+    g.c_vert = dm.c_vert;
+    g.c_poly = poly;
+    g.cf_ptr = dm.cf_ptr;
+    g.cf_pat = dm.cf_pat;
+    g.f_v = fv;
+    g.f_c1 = dm.f_c1;
+    g.f_c2 = dm.f_c2;
+    g.c_vol = alloc<double>(Nc);
+    g.c_x = alloc<double>(3 * Nc);
+    g.f_x = alloc<double>(3 * Nf);
+    g.f_n = alloc<double>(3 * Nf);
+    g.f_xpac = alloc<double>(3 * Nf);
+    g.f_xnac = alloc<double>(3 * Nf);
+    g.f_area = alloc<double>(Nf);
+    g.f_lambda = alloc<double>(Nf);
+    g.f_rlencos = alloc<double>(Nf);
+    g.b_dist = alloc<double>(std::max<size_t>(Nb, 1));
+    g.warped = warped;
+    launch_geometry(g, stream);
+    dm.c_vol = g.c_vol;
+    dm.c_x = g.c_x;
+    dm.f_x = g.f_x;
+    dm.f_n = g.f_n;
+    dm.f_xpac = g.f_xpac;
+    dm.f_xnac = g.f_xnac;
+    dm.f_area = g.f_area;
+    dm.f_lambda = g.f_lambda;
+    dm.f_rlencos = g.f_rlencos;
+    dm.b_dist = g.b_dist;
+    auto down = [&](hvec<double>& h, const double* d, size_t n) {
+      h.resize(n);
+      if (n) S355_HIP(hipMemcpyAsync(h.data(), d, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+    };
+    down(m.c_vol, g.c_vol, Nc);
+    down(m.c_x, g.c_x, 3 * Nc);
+    down(m.f_x, g.f_x, 3 * Nf);
+    down(m.f_n, g.f_n, 3 * Nf);
+    down(m.f_xpac, g.f_xpac, 3 * Nf);
+    down(m.f_xnac, g.f_xnac, 3 * Nf);
+    down(m.f_area, g.f_area, Nf);
+    down(m.f_lambda, g.f_lambda, Nf);
+    down(m.f_rlencos, g.f_rlencos, Nf);
+    down(m.b_dist, g.b_dist, Nb);
+    if (Nf) S355_HIP(hipMemcpyAsync(hw.data(), warped, Nf, hipMemcpyDeviceToHost, stream));
+    S355_HIP(hipStreamSynchronize(stream));
+  } catch (...) {
+    hipStreamSynchronize(stream);
+    hipFree(poly);
+    hipFree(warped);
+    hipFree(fv);
+    throw;
+  }
+  S355_HIP(hipFree(poly));
+  S355_HIP(hipFree(warped));
+  S355_HIP(hipFree(fv));
+  m.n_warped = 0;
+  for (uint8_t w : hw) m.n_warped += w;
+  for (size_t b = 0; b < Nb; ++b)
+    if (m.b_dist[b] < 0.)
+      throw std::runtime_error("mesh: wall distance is negative at boundary " + std::to_string(b));
+}
+
 void Context::upload_mesh() {
   const Mesh& m = mesh;
   dm.Nc = m.n_cell;
@@ -148,15 +236,6 @@ void Context::upload_mesh() {
   dm.nnz = m.nnz();
   dm.f_c1 = upload(allocs_, m.f_c1, stream);
   dm.f_c2 = upload(allocs_, m.f_c2, stream);
-  dm.f_lambda = upload(allocs_, m.f_lambda, stream);
-  dm.f_rlencos = upload(allocs_, m.f_rlencos, stream);
-  dm.f_area = upload(allocs_, m.f_area, stream);
-  dm.f_n = upload(allocs_, m.f_n, stream);
-  dm.f_x = upload(allocs_, m.f_x, stream);
-  dm.f_xpac = upload(allocs_, m.f_xpac, stream);
-  dm.f_xnac = upload(allocs_, m.f_xnac, stream);
-  dm.c_vol = upload(allocs_, m.c_vol, stream);
-  dm.c_x = upload(allocs_, m.c_x, stream);
   dm.cf_ptr = upload(allocs_, m.cf_ptr, stream);
   dm.cf_pat = upload(allocs_, m.cf_pat, stream);
   dm.cf_slot = upload(allocs_, m.cf_slot, stream);
@@ -164,13 +243,13 @@ void Context::upload_mesh() {
   dm.rowptr = upload(allocs_, m.rowptr, stream);
   dm.col = upload(allocs_, m.col, stream);
   dm.diag = upload(allocs_, m.diag, stream);
+  compute_geometry();
   std::vector<int> btype(m.n_bnd);
   for (int b = 0; b < m.n_bnd; ++b) {
     btype[b] = m.type1_of_rid(m.b_rid[b]);
     if (btype[b] == kOutlet) has_outlet_ = true;
   }
   dm.b_type = upload(allocs_, btype, stream);
-  dm.b_dist = upload(allocs_, m.b_dist, stream);
   // wall-adjacent cells with their wall faces in ascending order
   std::vector<int> wcell, wptr(1, 0), wface;
   {
@@ -546,7 +625,7 @@ void Context::ensure_ls() {
 // vertex arrays of the limiters: each cell's 8 vertices, the coordinates, and
 // the cells around each vertex (each cell once, ascending)
 void Context::ensure_vertices() {
-  if (dm.c_vert) return;
+  if (dm.v2c) return;
   const int nv = mesh.n_vert, nc = mesh.n_cell;
   std::vector<int> ptr(nv + 1, 0);
   auto distinct = [&](int i, int j) {
@@ -563,13 +642,8 @@ void Context::ensure_vertices() {
   for (int i = 0; i < nc; ++i)
     for (int j = 0; j < 8; ++j)
       if (distinct(i, j)) v2c[fill[mesh.c_vert[8 * static_cast<size_t>(i) + j]]++] = i;
-  std::vector<int> cv(mesh.c_vert.begin(), mesh.c_vert.end());
-  std::vector<double> xyz(mesh.vert.begin(), mesh.vert.end());
-  dm.Nv = nv;
   dm.v2c_ptr = upload(allocs_, ptr, stream);
   dm.v2c = upload(allocs_, v2c, stream);
-  dm.c_vert = upload(allocs_, cv, stream);
-  dm.vert = upload(allocs_, xyz, stream);
   lim_vmax_ = alloc<double>(std::max(nv, 1));
   lim_vmin_ = alloc<double>(std::max(nv, 1));
   lim_tmp_ = alloc<double>(3 * static_cast<size_t>(dm.Nc));

@@ -101,6 +101,7 @@ class Context {
   template <class T>
   T* alloc(size_t n);
   void upload_mesh();
+  void compute_geometry();
   void allocate_fields();
   void free_all();
   int run_solve(int which_matrix, const double* b, double* x, const s355_ksp_opts& o, bool zero_guess,

@@ -71,22 +71,28 @@ Mesh concat_meshes(const std::vector<Mesh>& parts, BatchLayout& lay) {
     return f < p.n_bnd ? lay.bnd[k] + f : Nb + lay.iface[k] + (f - p.n_bnd);
   };
   const size_t Nf = m.n_face, Nc = m.n_cell, K = kMaxFace;
+  // the parts' geometry, when they carry it (ingest_mesh with metrics)
+  const bool met = !parts[0].c_vol.empty();
+  for (const Mesh& p : parts)
+    if (p.c_vol.empty() == met) throw std::invalid_argument("batch: parts with and without geometry");
   m.f_c1.resize(Nf);
   m.f_c2.resize(Nf);
   m.f_bnd.resize(Nf);
   m.f_v.resize(4 * Nf);
-  m.f_x.resize(3 * Nf);
-  m.f_n.resize(3 * Nf);
-  m.f_xpac.resize(3 * Nf);
-  m.f_xnac.resize(3 * Nf);
-  m.f_area.resize(Nf);
-  m.f_lambda.resize(Nf);
-  m.f_rlencos.resize(Nf);
+  if (met) {
+    m.f_x.resize(3 * Nf);
+    m.f_n.resize(3 * Nf);
+    m.f_xpac.resize(3 * Nf);
+    m.f_xnac.resize(3 * Nf);
+    m.f_area.resize(Nf);
+    m.f_lambda.resize(Nf);
+    m.f_rlencos.resize(Nf);
+    m.b_dist.resize(Nb);
+  }
   m.c_face.resize(Nc * K);
   m.c_cell.resize(Nc * K);
   m.slot.resize(Nc * K);
   m.b_rid.resize(Nb);
-  m.b_dist.resize(Nb);
   m.rowptr.resize(Nc + 1);
   m.rowptr[0] = 0;
   m.col.resize(lay.nnz[nc]);
@@ -111,6 +117,7 @@ Mesh concat_meshes(const std::vector<Mesh>& parts, BatchLayout& lay) {
         const int v = p.f_v[4 * static_cast<size_t>(f) + j];
         m.f_v[4 * g + j] = v >= 0 ? v + V0 : v;
       }
+      if (!met) continue;
       for (int j = 0; j < 3; ++j) {
         m.f_x[3 * g + j] = p.f_x[3 * static_cast<size_t>(f) + j];
         m.f_n[3 * g + j] = p.f_n[3 * static_cast<size_t>(f) + j];
@@ -123,7 +130,7 @@ Mesh concat_meshes(const std::vector<Mesh>& parts, BatchLayout& lay) {
     }
     for (int b = 0; b < p.n_bnd; ++b) {
       m.b_rid[B0 + b] = p.b_rid[b] + R0;
-      m.b_dist[B0 + b] = p.b_dist[b];
+      if (met) m.b_dist[B0 + b] = p.b_dist[b];
     }
     for (int i = 0; i < p.n_cell; ++i) {
       const size_t gi = static_cast<size_t>(C0 + i);

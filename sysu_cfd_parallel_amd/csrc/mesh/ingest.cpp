@@ -148,7 +148,7 @@ struct StageClock {
 };
 }  // namespace
 
-Mesh ingest_mesh(const s355_mesh_desc& d) {
+Mesh ingest_mesh(const s355_mesh_desc& d, bool metrics) {
   StageClock clk;
   if (d.n_vert <= 0 || d.n_elem <= 0 || !d.xyz || !d.elem_type || !d.elem_tag || !d.elem_vptr || !d.elem_vidx)
     throw std::runtime_error("mesh: empty or missing arrays");
@@ -433,49 +433,51 @@ Mesh ingest_mesh(const s355_mesh_desc& d) {
   // ---- cell metrics
   const double* X = m.vert.data();
   auto P = [&](int v) { return X + 3 * static_cast<size_t>(v); };
-  m.c_vol.resize(Nc);
-  m.c_x.resize(3 * static_cast<size_t>(Nc));
-  parallel_for(Nc, [&](long i) {
-    const int* h = &cv[8 * static_cast<size_t>(i)];
-    double* xc = &m.c_x[3 * static_cast<size_t>(i)];
-    for (int j = 0; j < 3; ++j)
-      xc[j] = 1. / 8. * (P(h[0])[j] + P(h[1])[j] + P(h[2])[j] + P(h[3])[j] + P(h[4])[j] + P(h[5])[j] +
-                         P(h[6])[j] + P(h[7])[j]);
-    if (m.c_poly[i]) {
-      // tets on the face triangles about the vertex mean
-      double xp[3] = {xc[0], xc[1], xc[2]};
-      double vs = 0., acc[3] = {0., 0., 0.};
-      for (int j = 0; j < m.c_nface[i]; ++j) {
-        const int* q = &m.f_v[4 * static_cast<size_t>(m.c_face[static_cast<size_t>(i) * K + j])];
-        const int tri[2][3] = {{q[0], q[1], q[3]}, {q[1], q[2], q[3]}};
-        for (const auto& t : tri) {
-          double v = tet_vol(xp, P(t[0]), P(t[1]), P(t[2]));
-          vs += v;
-          for (int k = 0; k < 3; ++k) acc[k] += v * 0.25 * (xp[k] + P(t[0])[k] + P(t[1])[k] + P(t[2])[k]);
+  if (metrics) {
+    m.c_vol.resize(Nc);
+    m.c_x.resize(3 * static_cast<size_t>(Nc));
+    parallel_for(Nc, [&](long i) {
+      const int* h = &cv[8 * static_cast<size_t>(i)];
+      double* xc = &m.c_x[3 * static_cast<size_t>(i)];
+      for (int j = 0; j < 3; ++j)
+        xc[j] = 1. / 8. * (P(h[0])[j] + P(h[1])[j] + P(h[2])[j] + P(h[3])[j] + P(h[4])[j] + P(h[5])[j] +
+                           P(h[6])[j] + P(h[7])[j]);
+      if (m.c_poly[i]) {
+        // tets on the face triangles about the vertex mean
+        double xp[3] = {xc[0], xc[1], xc[2]};
+        double vs = 0., acc[3] = {0., 0., 0.};
+        for (int j = 0; j < m.c_nface[i]; ++j) {
+          const int* q = &m.f_v[4 * static_cast<size_t>(m.c_face[static_cast<size_t>(i) * K + j])];
+          const int tri[2][3] = {{q[0], q[1], q[3]}, {q[1], q[2], q[3]}};
+          for (const auto& t : tri) {
+            double v = tet_vol(xp, P(t[0]), P(t[1]), P(t[2]));
+            vs += v;
+            for (int k = 0; k < 3; ++k) acc[k] += v * 0.25 * (xp[k] + P(t[0])[k] + P(t[1])[k] + P(t[2])[k]);
+          }
         }
+        m.c_vol[i] = vs;
+        for (int k = 0; k < 3; ++k) xc[k] = acc[k] / vs;
+        return;
       }
-      m.c_vol[i] = vs;
-      for (int k = 0; k < 3; ++k) xc[k] = acc[k] / vs;
-      return;
-    }
-    const double *x1 = P(h[0]), *x2 = P(h[1]), *x3 = P(h[2]), *x4 = P(h[3]), *x5 = P(h[4]), *x6 = P(h[5]),
-                 *x7 = P(h[6]), *x8 = P(h[7]);
-    double V1 = tet_vol(x1, x2, x4, x6), V2 = tet_vol(x1, x5, x4, x6), V3 = tet_vol(x4, x5, x8, x6);
-    double V4 = tet_vol(x2, x3, x4, x7), V5 = tet_vol(x2, x6, x4, x7), V6 = tet_vol(x8, x4, x6, x7);
-    m.c_vol[i] = V1 + V2 + V3 + V4 + V5 + V6;
-    double c[6][3];
-    for (int j = 0; j < 3; ++j) {
-      c[0][j] = 0.25 * (x1[j] + x2[j] + x4[j] + x6[j]);
-      c[1][j] = 0.25 * (x1[j] + x5[j] + x4[j] + x6[j]);
-      c[2][j] = 0.25 * (x4[j] + x5[j] + x8[j] + x6[j]);
-      c[3][j] = 0.25 * (x2[j] + x3[j] + x4[j] + x7[j]);
-      c[4][j] = 0.25 * (x2[j] + x6[j] + x4[j] + x7[j]);
-      c[5][j] = 0.25 * (x8[j] + x4[j] + x6[j] + x7[j]);
-    }
-    double vs = V1 + V2 + V3 + V4 + V5 + V6;
-    for (int j = 0; j < 3; ++j)
-      xc[j] = 1. / vs * (V1 * c[0][j] + V2 * c[1][j] + V3 * c[2][j] + V4 * c[3][j] + V5 * c[4][j] + V6 * c[5][j]);
-  });
+      const double *x1 = P(h[0]), *x2 = P(h[1]), *x3 = P(h[2]), *x4 = P(h[3]), *x5 = P(h[4]), *x6 = P(h[5]),
+                   *x7 = P(h[6]), *x8 = P(h[7]);
+      double V1 = tet_vol(x1, x2, x4, x6), V2 = tet_vol(x1, x5, x4, x6), V3 = tet_vol(x4, x5, x8, x6);
+      double V4 = tet_vol(x2, x3, x4, x7), V5 = tet_vol(x2, x6, x4, x7), V6 = tet_vol(x8, x4, x6, x7);
+      m.c_vol[i] = V1 + V2 + V3 + V4 + V5 + V6;
+      double c[6][3];
+      for (int j = 0; j < 3; ++j) {
+        c[0][j] = 0.25 * (x1[j] + x2[j] + x4[j] + x6[j]);
+        c[1][j] = 0.25 * (x1[j] + x5[j] + x4[j] + x6[j]);
+        c[2][j] = 0.25 * (x4[j] + x5[j] + x8[j] + x6[j]);
+        c[3][j] = 0.25 * (x2[j] + x3[j] + x4[j] + x7[j]);
+        c[4][j] = 0.25 * (x2[j] + x6[j] + x4[j] + x7[j]);
+        c[5][j] = 0.25 * (x8[j] + x4[j] + x6[j] + x7[j]);
+      }
+      double vs = V1 + V2 + V3 + V4 + V5 + V6;
+      for (int j = 0; j < 3; ++j)
+        xc[j] = 1. / vs * (V1 * c[0][j] + V2 * c[1][j] + V3 * c[2][j] + V4 * c[3][j] + V5 * c[4][j] + V6 * c[5][j]);
+    });
+  }
   m.c_cell.resize(static_cast<size_t>(Nc) * K);
   parallel_for(Nc, [&](long i) {
     for (int j = 0; j < K; ++j) {
@@ -489,75 +491,79 @@ Mesh ingest_mesh(const s355_mesh_desc& d) {
   });
   clk.mark("cell metrics");
   // ---- face metrics
-  m.f_x.resize(3 * static_cast<size_t>(Nf));
-  m.f_n.resize(3 * static_cast<size_t>(Nf));
-  m.f_xpac.resize(3 * static_cast<size_t>(Nf));
-  m.f_xnac.resize(3 * static_cast<size_t>(Nf));
-  m.f_area.resize(Nf);
-  m.f_lambda.resize(Nf);
-  m.f_rlencos.resize(Nf);
-  std::atomic<int> warped_sum{0};
-  parallel_chunks(Nf, [&](long f0, long f1) {
-    int warped = 0;
-    for (long f = f0; f < f1; ++f) {
-      const int* q = &m.f_v[4 * static_cast<size_t>(f)];
-      const double *p1 = P(q[0]), *p2 = P(q[1]), *p3 = P(q[2]), *p4 = P(q[3]);
-      double a1 = tri_area(p1, p2, p4), a2 = tri_area(p2, p3, p4);
-      double* fx = &m.f_x[3 * static_cast<size_t>(f)];
-      double* fn = &m.f_n[3 * static_cast<size_t>(f)];
-      double xc1[3], xc2[3];
-      for (int j = 0; j < 3; ++j) {
-        xc1[j] = 1. / 3 * (p1[j] + p2[j] + p4[j]);
-        xc2[j] = 1. / 3 * (p2[j] + p3[j] + p4[j]);
-      }
-      for (int j = 0; j < 3; ++j) fx[j] = (a1 * xc1[j] + a2 * xc2[j]) / (a1 + a2);
-      double d1[3], d2[3], dx[3];
-      sub3(d1, p1, p3);
-      sub3(d2, p2, p4);
-      cross3(fn, d1, d2);
-      for (int j = 0; j < 3; ++j) fn[j] = 0.5 * fn[j];
-      const double* x1 = &m.c_x[3 * static_cast<size_t>(m.f_c1[f])];
-      sub3(dx, fx, x1);
-      if (dot3(fn, dx) < 0.)
-        for (int j = 0; j < 3; ++j) fn[j] = -fn[j];
-      double area = len3(fn);
-      m.f_area[f] = area;
-      if (std::fabs((a1 + a2 - area) / area) > 1.e-4) warped++;
-      int c2 = m.f_c2[f];
-      if (c2 >= 0) {
-        const double* x2 = &m.c_x[3 * static_cast<size_t>(c2)];
-        sub3(d1, fx, x1);
-        sub3(d2, fx, x2);
-        double r1 = len3(d1), r2 = len3(d2);
-        m.f_lambda[f] = r2 / (r1 + r2);
-        sub3(dx, x2, x1);
-        m.f_rlencos[f] = area / std::fabs(dot3(dx, fn) / area);
-        double ss;
+  if (metrics) {
+    m.f_x.resize(3 * static_cast<size_t>(Nf));
+    m.f_n.resize(3 * static_cast<size_t>(Nf));
+    m.f_xpac.resize(3 * static_cast<size_t>(Nf));
+    m.f_xnac.resize(3 * static_cast<size_t>(Nf));
+    m.f_area.resize(Nf);
+    m.f_lambda.resize(Nf);
+    m.f_rlencos.resize(Nf);
+    std::atomic<int> warped_sum{0};
+    parallel_chunks(Nf, [&](long f0, long f1) {
+      int warped = 0;
+      for (long f = f0; f < f1; ++f) {
+        const int* q = &m.f_v[4 * static_cast<size_t>(f)];
+        const double *p1 = P(q[0]), *p2 = P(q[1]), *p3 = P(q[2]), *p4 = P(q[3]);
+        double a1 = tri_area(p1, p2, p4), a2 = tri_area(p2, p3, p4);
+        double* fx = &m.f_x[3 * static_cast<size_t>(f)];
+        double* fn = &m.f_n[3 * static_cast<size_t>(f)];
+        double xc1[3], xc2[3];
+        for (int j = 0; j < 3; ++j) {
+          xc1[j] = 1. / 3 * (p1[j] + p2[j] + p4[j]);
+          xc2[j] = 1. / 3 * (p2[j] + p3[j] + p4[j]);
+        }
+        for (int j = 0; j < 3; ++j) fx[j] = (a1 * xc1[j] + a2 * xc2[j]) / (a1 + a2);
+        double d1[3], d2[3], dx[3];
+        sub3(d1, p1, p3);
+        sub3(d2, p2, p4);
+        cross3(fn, d1, d2);
+        for (int j = 0; j < 3; ++j) fn[j] = 0.5 * fn[j];
+        const double* x1 = &m.c_x[3 * static_cast<size_t>(m.f_c1[f])];
         sub3(dx, fx, x1);
-        ss = dot3(dx, fn) / area;
-        for (int j = 0; j < 3; ++j) dx[j] = ss * fn[j] / area;
-        sub3(&m.f_xpac[3 * static_cast<size_t>(f)], fx, dx);
-        sub3(dx, fx, x2);
-        ss = dot3(dx, fn) / area;
-        for (int j = 0; j < 3; ++j) dx[j] = ss * fn[j] / area;
-        sub3(&m.f_xnac[3 * static_cast<size_t>(f)], fx, dx);
-      } else {
-        m.f_lambda[f] = 1.;
-        sub3(dx, fx, x1);
-        m.f_rlencos[f] = area / (dot3(dx, fn) / area);
-        for (int j = 0; j < 3; ++j) m.f_xpac[3 * static_cast<size_t>(f) + j] = m.f_xnac[3 * static_cast<size_t>(f) + j] = 0.0;
+        if (dot3(fn, dx) < 0.)
+          for (int j = 0; j < 3; ++j) fn[j] = -fn[j];
+        double area = len3(fn);
+        m.f_area[f] = area;
+        if (std::fabs((a1 + a2 - area) / area) > 1.e-4) warped++;
+        int c2 = m.f_c2[f];
+        if (c2 >= 0) {
+          const double* x2 = &m.c_x[3 * static_cast<size_t>(c2)];
+          sub3(d1, fx, x1);
+          sub3(d2, fx, x2);
+          double r1 = len3(d1), r2 = len3(d2);
+          m.f_lambda[f] = r2 / (r1 + r2);
+          sub3(dx, x2, x1);
+          m.f_rlencos[f] = area / std::fabs(dot3(dx, fn) / area);
+          double ss;
+          sub3(dx, fx, x1);
+          ss = dot3(dx, fn) / area;
+          for (int j = 0; j < 3; ++j) dx[j] = ss * fn[j] / area;
+          sub3(&m.f_xpac[3 * static_cast<size_t>(f)], fx, dx);
+          sub3(dx, fx, x2);
+          ss = dot3(dx, fn) / area;
+          for (int j = 0; j < 3; ++j) dx[j] = ss * fn[j] / area;
+          sub3(&m.f_xnac[3 * static_cast<size_t>(f)], fx, dx);
+        } else {
+          m.f_lambda[f] = 1.;
+          sub3(dx, fx, x1);
+          m.f_rlencos[f] = area / (dot3(dx, fn) / area);
+          for (int j = 0; j < 3; ++j)
+            m.f_xpac[3 * static_cast<size_t>(f) + j] = m.f_xnac[3 * static_cast<size_t>(f) + j] = 0.0;
+        }
       }
+      warped_sum += warped;
+    });
+    m.n_warped = warped_sum.load();
+    clk.mark("face metrics");
+    m.b_dist.resize(Nb);
+    for (int b = 0; b < Nb; ++b) {
+      double dx[3];
+      sub3(dx, &m.f_x[3 * static_cast<size_t>(b)], &m.c_x[3 * static_cast<size_t>(m.f_c1[b])]);
+      m.b_dist[b] = dot3(dx, &m.f_n[3 * static_cast<size_t>(b)]) / m.f_area[b];
+      if (m.b_dist[b] < 0.)
+        throw std::runtime_error("mesh: wall distance is negative at boundary " + std::to_string(b));
     }
-    warped_sum += warped;
-  });
-  m.n_warped = warped_sum.load();
-  clk.mark("face metrics");
-  m.b_dist.resize(Nb);
-  for (int b = 0; b < Nb; ++b) {
-    double dx[3];
-    sub3(dx, &m.f_x[3 * static_cast<size_t>(b)], &m.c_x[3 * static_cast<size_t>(m.f_c1[b])]);
-    m.b_dist[b] = dot3(dx, &m.f_n[3 * static_cast<size_t>(b)]) / m.f_area[b];
-    if (m.b_dist[b] < 0.) throw std::runtime_error("mesh: wall distance is negative at boundary " + std::to_string(b));
   }
   // ---- CSR pattern: count, prefix, fill
   m.rowptr.resize(Nc + 1);

@@ -76,7 +76,10 @@ struct Mesh {
 };
 
 // Builds the mesh; throws std::runtime_error with a message on bad input.
-Mesh ingest_mesh(const s355_mesh_desc& desc);
+// metrics = false leaves the geometry (cell volumes and centres, face
+// metrics, wall distances, the warped-face count) empty: the device context
+// computes it on the GPU from the same topology and vertices.
+Mesh ingest_mesh(const s355_mesh_desc& desc, bool metrics = true);
 
 // Offsets of the cases of a batch mesh (ncase + 1 entries each): case k owns
 // cells [cell[k], cell[k+1]), boundary records [bnd[k], bnd[k+1]), interior

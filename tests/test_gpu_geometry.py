@@ -1,0 +1,68 @@
+"""Mesh geometry on the device (CellFaceInfo, geometry.cpp:391-752): a device
+context computes cell volumes and centres, face centres, area vectors,
+areas, interpolation weights, rlencos, Xpac / Xnac and wall distances on the
+GPU from the ingest's topology and vertices, and copies them back for the
+host.  They must equal the host ingest's metrics (the same expressions in
+the same order) bit for bit -- hexes with 15 % jitter (warped quads),
+hexes split into tetrahedra with the polyhedral cells between, the
+tetrahedral step, a batch of cases, and the full-size channel -- and count
+the same warped faces."""
+import numpy as np
+import pytest
+
+import sysu_cfd_parallel_amd as s
+from helpers import params
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ("CELL_VOL", "CELL_X", "FACE_X", "FACE_N", "FACE_AREA", "FACE_LAMBDA", "FACE_RLENCOS", "FACE_XPAC",
+         "FACE_XNAC", "BND_DIST")
+
+CASES = {
+    "cavity_jitter": lambda: s.cavity(10, jitter=0.15, seed=1),
+    "channel_split": lambda: s.channel(12, 7, 9, jitter=0.15, split=0.2, seed=2),
+    "step_tets": lambda: s.step(3, jitter=0.10, tets=True, turbulent=True),
+    "cavity_tets": lambda: s.cavity(6, tets=True),
+}
+
+
+def compare(case, dev):
+    host = s.HostMesh(case)
+    try:
+        assert dev.sizes.n_warped == host.sizes.n_warped
+        for n in NAMES:
+            a, b = dev.dbls(n), host.dbls(n)
+            assert a.shape == b.shape, n
+            assert np.array_equal(a, b), f"{n}: {np.count_nonzero(a != b)} values differ, max {np.abs(a - b).max():.3e}"
+    finally:
+        host.close()
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_device_geometry_equals_host(name):
+    case = CASES[name]()
+    with s.Solver(case, s.default_params()) as d:
+        compare(case, d)
+        if name == "cavity_jitter" or name == "channel_split":
+            assert d.sizes.n_warped > 0  # jittered quads are warped
+
+
+def test_device_geometry_of_a_batch():
+    cases = [s.cavity(6, re=100.0), s.channel(8, 5, 6, jitter=0.15, split=0.2, seed=4), s.cavity(5, tets=True)]
+    host = s.HostMesh(cases)
+    d = s.BatchSolver(cases, params())  # batches solve with GMRES
+    try:
+        assert d.sizes.n_warped == host.sizes.n_warped
+        for n in NAMES:
+            assert np.array_equal(d.dbls(n), host.dbls(n)), n
+    finally:
+        d.close()
+        host.close()
+
+
+def test_device_geometry_full_size_channel():
+    case = s.channel(200, 100, 200, lx=2.0, ly=1.0, lz=2.0, jitter=0.15, split=0.10, seed=0, re=1000.0)
+    print("[geometry] 200x100x200 channel", flush=True)
+    with s.Solver(case, s.default_params()) as d:
+        assert d.sizes.n_cell == 5_995_990
+        compare(case, d)
