@@ -654,11 +654,23 @@ struct Vec16<double> {
   static constexpr int n = 2;
 };
 
-template <class T, int VW>
+// 16 bytes through a non-temporal load (the basis is streamed once per pass)
+template <class T>
+__device__ __forceinline__ typename Vec16<T>::type load16_nt(const T* p) {
+  using E = T __attribute__((ext_vector_type(Vec16<T>::n)));
+  const E e = __builtin_nontemporal_load(reinterpret_cast<const E*>(p));
+  typename Vec16<T>::type v;
+#pragma unroll
+  for (int u = 0; u < Vec16<T>::n; ++u) reinterpret_cast<T*>(&v)[u] = e[u];
+  return v;
+}
+
+template <class T, int VW, bool NT = false>
 __device__ __forceinline__ void load_chunk(const T* __restrict__ p, size_t i0, size_t end, bool vec, double* out) {
   if constexpr (VW == Vec16<T>::n) {
     if (vec && i0 + VW <= end) {
-      const typename Vec16<T>::type v = *reinterpret_cast<const typename Vec16<T>::type*>(p + i0);
+      const typename Vec16<T>::type v =
+          NT ? load16_nt<T>(p + i0) : *reinterpret_cast<const typename Vec16<T>::type*>(p + i0);
       const T* e = reinterpret_cast<const T*>(&v);
 #pragma unroll
       for (int u = 0; u < VW; ++u) out[u] = static_cast<double>(e[u]);
@@ -747,7 +759,7 @@ __global__ void __launch_bounds__(kBlock) k_mdot(const T* __restrict__ V, size_t
       for (int kk = 0; kk < 8; ++kk)
 #pragma unroll
         for (int q = 0; q < S::NC; ++q)
-          raw[kk][q] = load16<T>(V + static_cast<size_t>(k0 + kk) * ld + S::row0(a, q));
+          raw[kk][q] = load16_nt<T>(V + static_cast<size_t>(k0 + kk) * ld + S::row0(a, q));
 #pragma unroll
       for (int kk = 0; kk < 8; ++kk)
 #pragma unroll
@@ -859,7 +871,7 @@ __global__ void __launch_bounds__(kBlock) k_maxpy_norm(T* __restrict__ V, size_t
     load_chunk<T, S::VW>(w, i0, end, vec, sacc);
     for (int k = 0; k <= it; ++k) {
       double v[S::VW];
-      load_chunk<T, S::VW>(V + static_cast<size_t>(k) * ld, i0, end, vec, v);
+      load_chunk<T, S::VW, true>(V + static_cast<size_t>(k) * ld, i0, end, vec, v);
 #pragma unroll
       for (int u = 0; u < S::VW; ++u) sacc[u] -= cf[k] * v[u];
     }
@@ -1003,7 +1015,7 @@ __global__ void __launch_bounds__(kBlock) k_update_x(const T* __restrict__ V, si
     for (int u = 0; u < S::VW; ++u) sacc[u] = 0.0;
     for (int k = 0; k < nk; ++k) {
       double v[S::VW];
-      load_chunk<T, S::VW>(V + static_cast<size_t>(k) * ld, i0, end, vec, v);
+      load_chunk<T, S::VW, true>(V + static_cast<size_t>(k) * ld, i0, end, vec, v);
 #pragma unroll
       for (int u = 0; u < S::VW; ++u) sacc[u] += y[k] * v[u];
     }
