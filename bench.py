@@ -251,13 +251,16 @@ class Sweep(Channel):
         p = _gmres_all(s.default_params(), 30, s.PC_JACOBI, maxit=5000)
         p.residual_steady = 0.0
         p.precision = prec
+        # each cavity's FP32 cycle on its rows in the generator's cell order
+        p.inner_order = 2
         return p
 
     def describe(self, prec_name):
         cells = sum(e ** 3 for e in self.edges)
         return (f"{self.dims} lid-driven cavities as one batch, edges U{{16..64}} (sum {cells} cells), "
                 f"Re logU[100,1000], seed 2024, GMRES(30)+Jacobi, "
-                + ("FP32 inner solves in FP64 refinement" if prec_name == "mixed" else "FP64"))
+                + ("FP32 inner solves in FP64 refinement, each case's FP32 cycle on its rows in the input cell order"
+                   if prec_name == "mixed" else "FP64"))
 
     def cpu_sample(self):
         # the cases of the batch up to about 300K cells, in batch order

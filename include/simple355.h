@@ -102,12 +102,13 @@ typedef struct s355_params {
   int graphs;           /* 1: capture each GMRES restart cycle as a hipGraph
                            and replay it; 0: launch kernel by kernel */
   int solve_energy;     /* SolveEnergy: 1 = UpdateEnergy after the k-epsilon step */
-  int inner_order;      /* mixed GMRES on one case without ILU(0): the FP32
-                           Arnoldi cycle runs on the rows in another order,
-                           its SELL-64 copies holding 16-bit column offsets
+  int inner_order;      /* mixed GMRES without ILU(0): the FP32 Arnoldi
+                           cycle runs on the rows in another order, its
+                           SELL-64 copies holding 16-bit column offsets
                            (when every offset fits): 1 = reverse
-                           Cuthill-McKee, 2 = the cells' order in the input
-                           element stream; 0 = the natural order.  The
+                           Cuthill-McKee (one case), 2 = the cells' order in
+                           the input element stream (each case of a batch in
+                           its own range); 0 = the natural order.  The
                            order is built on the first solve that uses it;
                            switching to the other order later is refused */
   double cp, prl, prte; /* heat capacity, laminar / turbulent Prandtl numbers

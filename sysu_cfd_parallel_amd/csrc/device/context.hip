@@ -1010,6 +1010,24 @@ void Context::check_flags() {
   if (h[1]) throw std::runtime_error("reverse flow get out of inlet (scalar transport)");
 }
 
+// The case of each 64-row slice of a SELL pattern whose rows (slot rows
+// srow, which are row ids in the pattern's own numbering) all belong to one
+// case of a batch, else -1.
+std::vector<int> Context::slice_cases(const std::vector<int>& ptr, const std::vector<int>& srow) const {
+  const std::vector<int>& cr = lay.cell;
+  auto case_of = [&](int r) { return static_cast<int>(std::upper_bound(cr.begin(), cr.end(), r) - cr.begin()) - 1; };
+  const int nslice = static_cast<int>(ptr.size()) - 1;
+  std::vector<int> scase(nslice, -1);
+  for (int sl = 0; sl < nslice; ++sl) {
+    const int p0 = sl * 64, p1 = std::min(dm.Nc, p0 + 64);
+    const int c0 = case_of(srow[p0]);
+    bool one = true;
+    for (int p = p0 + 1; p < p1 && one; ++p) one = case_of(srow[p]) == c0;
+    if (one) scase[sl] = c0;
+  }
+  return scase;
+}
+
 void Context::ensure_sell() {
   if (have_sell_) return;
   for (int v = 0; v < kSellVariants; ++v) {
@@ -1023,20 +1041,7 @@ void Context::ensure_sell() {
     pt.src = upload(allocs_, src, stream);
     std::vector<int> scase;
     if (ncase > 1) {
-      // the case of each slice whose rows all belong to one case, else -1
-      const std::vector<int>& cr = lay.cell;
-      auto case_of = [&](int r) {
-        return static_cast<int>(std::upper_bound(cr.begin(), cr.end(), r) - cr.begin()) - 1;
-      };
-      const int nslice = static_cast<int>(ptr.size()) - 1;
-      scase.assign(nslice, -1);
-      for (int sl = 0; sl < nslice; ++sl) {
-        const int p0 = sl * 64, p1 = std::min(dm.Nc, p0 + 64);
-        const int c0 = case_of(row[p0]);
-        bool one = true;
-        for (int p = p0 + 1; p < p1 && one; ++p) one = case_of(row[p]) == c0;
-        if (one) scase[sl] = c0;
-      }
+      scase = slice_cases(ptr, row);
       pt.scase = upload(allocs_, scase, stream);
     }
     sync();  // host staging vectors go out of scope
@@ -1046,8 +1051,7 @@ void Context::ensure_sell() {
 }
 
 bool Context::use_order(const s355_ksp_opts& o) {
-  if (!par.inner_order || par.precision != S355_MIXED || ncase != 1 || o.type != S355_KSP_GMRES ||
-      o.pc == S355_PC_ILU0)
+  if (!par.inner_order || par.precision != S355_MIXED || o.type != S355_KSP_GMRES || o.pc == S355_PC_ILU0)
     return false;
   ensure_order();
   return order_ok_;
@@ -1073,17 +1077,23 @@ void Context::ensure_order() {
   const int n = dm.Nc;
   std::vector<int> row;
   if (par.inner_order == 2) {
+    // a batch's cases stay in their own position ranges: the element
+    // indices of case k follow those of case k - 1
     row.resize(n);
     for (int p = 0; p < n; ++p) row[p] = p;
     std::sort(row.begin(), row.end(), [&](int a, int b) { return mesh.c_elem[a] < mesh.c_elem[b]; });
   } else {
+    if (ncase > 1) return;  // the reverse Cuthill-McKee order is built for one case
     row = rcm_order(n, mesh.rowptr.data(), mesh.col.data());
   }
   auto len = [&](int r) { return mesh.rowptr[r + 1] - mesh.rowptr[r]; };
+  // windows start at each case's first row, so a case's windows (and so its
+  // rows' places) are the same in a batch as alone
   const int window = kSellSigma[1];
-  for (int w0 = 0; w0 < n; w0 += window)
-    std::stable_sort(row.begin() + w0, row.begin() + std::min(n, w0 + window),
-                     [&](int a, int b) { return len(a) > len(b); });
+  for (int k = 0; k < ncase; ++k)
+    for (int w0 = lay.cell[k]; w0 < lay.cell[k + 1]; w0 += window)
+      std::stable_sort(row.begin() + w0, row.begin() + std::min(lay.cell[k + 1], w0 + window),
+                       [&](int a, int b) { return len(a) > len(b); });
   std::vector<int> pos(n);
   for (int p = 0; p < n; ++p) pos[row[p]] = p;
   const size_t nnz = mesh.col.size();
@@ -1127,6 +1137,7 @@ void Context::ensure_order() {
     pt.row = ident[v] ? nullptr : upload(allocs_, srow[v], stream);
     pt.col16 = upload(allocs_, off[v], stream);
     pt.src = upload(allocs_, ssrc[v], stream);  // positions in the ordered CSR
+    if (ncase > 1) pt.scase = upload(allocs_, slice_cases(ptr[v], srow[v]), stream);
     osell_[v] = pt;
   }
   sync();  // host staging vectors go out of scope

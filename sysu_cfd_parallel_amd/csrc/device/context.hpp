@@ -101,6 +101,7 @@ class Context {
   template <class T>
   T* alloc(size_t n);
   void upload_mesh();
+  std::vector<int> slice_cases(const std::vector<int>& ptr, const std::vector<int>& srow) const;
   void compute_geometry();
   void allocate_fields();
   void free_all();

@@ -1487,12 +1487,13 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
   const Tiles& tl = w.tiles[log2_r(R)];
   if (tl.tile != kBlock * R || tl.n != n) throw std::runtime_error("gmres: tile table missing");
   const int nc = w.ncase;
-  // the inner order (mixed, one case, no ILU): the FP32 basis, the matrix A
-  // and its SELL copies are by position; the restart residual goes to its
-  // positions and the correction back to the natural rows
+  // the inner order (mixed, no ILU): the FP32 basis, the matrix A and its
+  // SELL copies are by position (each case of a batch keeps its own position
+  // range); the restart residual goes to its positions and the correction
+  // back to the natural rows
   const bool ordered = w.ord_pos != nullptr;
-  if (ordered && (!mixed || nc != 1 || o.pc == S355_PC_ILU0 || !w.ord_row))
-    throw std::invalid_argument("gmres: the inner order serves mixed solves of one case without ILU(0)");
+  if (ordered && (!mixed || o.pc == S355_PC_ILU0 || !w.ord_row))
+    throw std::invalid_argument("gmres: the inner order serves mixed solves without ILU(0)");
   // ILU sweeps skip their work only for a single system
   const KState* ilu_st = nc == 1 ? w.st : nullptr;
   T* V = static_cast<T*>(w.V);

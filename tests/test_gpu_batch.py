@@ -2,7 +2,8 @@
 uneven size and Reynolds number against one Solver per case.  Every case of
 the batch is reduced over the same tiles, relative to its own first row, as
 when it runs alone, so the batch reproduces the single-case runs bit for
-bit (FP64 and mixed, Jacobi and ILU(0))."""
+bit (FP64 and mixed, Jacobi and ILU(0), the mixed cycle in the natural and
+in the input cell order)."""
 import numpy as np
 import pytest
 
@@ -27,13 +28,17 @@ def run_alone(cases, p, steps):
     return out
 
 
-@pytest.mark.parametrize("prec,pc", [(s.FP64, s.PC_JACOBI), (s.MIXED, s.PC_JACOBI), (s.FP64, s.PC_ILU0)])
-def test_batch_reproduces_single_case_runs(prec, pc):
+@pytest.mark.parametrize("prec,pc,order", [(s.FP64, s.PC_JACOBI, 0), (s.MIXED, s.PC_JACOBI, 0), (s.FP64, s.PC_ILU0, 0),
+                                           (s.MIXED, s.PC_JACOBI, 2)])
+def test_batch_reproduces_single_case_runs(prec, pc, order):
+    # order 2: each case's FP32 cycle on its rows in the input cell order, in
+    # its own position range of the batch
     cases = sweep_cases()
     p = params(restart=30, pc=pc, rtol=1e-8, maxit=2000)
     p.ksp[s.KSP_PRESSURE].rtol = 1e-6
     p.residual_steady = 0.0
     p.precision = prec
+    p.inner_order = order
     steps = 6
     alone = run_alone(cases, p, steps)
     b = s.BatchSolver(cases, p)
@@ -49,6 +54,7 @@ def test_batch_reproduces_single_case_runs(prec, pc):
     assert np.array_equal(hb[:, :4], hmax[:, :4])
     info = b.case_solve_info()
     assert all(i["reason"] > 0 for i in info["P"])
+    assert b.kernel_shapes()["fp32"]["order"] == (1 if order else 0)
     b.close()
 
 
