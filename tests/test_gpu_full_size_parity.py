@@ -6,7 +6,8 @@ the Rhie-Chow face flux (425-476), the pressure-correction assembly
 (navier_pressure.cpp:99-232) and the correction block (39-78).  The half-face
 kernels add each cell's terms in the reference's order, so every array
 agrees bit for bit except where the outlet mass-flow rescale sums the
-outlet faces in another order (1e-13)."""
+outlet faces in another order (1e-13).  The k-epsilon pieces are checked
+the same way at one GPU's share of the 20 M-cell turbulent channel."""
 import numpy as np
 import pytest
 
@@ -79,3 +80,48 @@ def test_full_size_assembly_bitwise(pair):
     for x in (d, o):
         x.call("correct_ru_face")
     close(d, o, ["RUFACE"], 0.0)
+
+
+def test_kepsilon_pieces_at_one_gpus_share_of_the_20m_channel():
+    """The k-epsilon pieces (navier_scalar.cpp:11-197, navier_bc.cpp:220-345)
+    on the turbulent channel at one GPU's share of the 20 M-cell workload
+    (200 x 100 x 125 hexes, 10 % jitter, 2.5 M cells): sources, wall function,
+    the k and epsilon systems and the epsilon wall rows bit for bit, the wall
+    epsilon's k^1.5 to the ulp (the device rounds it correctly, the host's
+    pow is 1 ulp off for a few arguments)."""
+    case = s.channel(200, 100, 125, lx=2.0, ly=1.0, lz=2.0, jitter=0.10, split=0.0, seed=0, re=13750.0,
+                     turbulent=True)
+    p = params(turb=1)
+    log("turbulent channel 200x100x125: contexts")
+    d, o = s.Solver(case, p), Oracle(case, p)
+    z = d.sizes
+    assert z.n_cell == 2_500_000
+    rng = np.random.default_rng(5)
+    st = random_state(z.n_cell, z.n_bnd, z.n_face, 5)
+    st["VISLAM"] = 10.0 ** rng.uniform(-4.5, -1.0, z.n_cell)
+    st["TE"] = 10.0 ** rng.uniform(-3, -1, z.n_cell)
+    st["ED"] = 10.0 ** rng.uniform(-3, 0, z.n_cell)
+    ru = st["RUFACE"]
+    inlet = np.isin(d.ints("BND_RID"), [r.rid for r in case.regions if r.type1 == 2])
+    ru[: z.n_bnd][inlet] = -np.abs(ru[: z.n_bnd][inlet])
+    names = ("U", "V", "W", "R", "VISLAM", "VISTUR", "TE", "ED", "DUDX", "DVDX", "DWDX", "BTE", "BED", "RUFACE")
+    push(d, {k: st[k] for k in names})
+    push(o, {k: st[k] for k in names})
+    log("sources and wall function")
+    for x in (d, o):
+        x.call("kepsilon_sources")
+        x.call("bc_kepsilon")
+    close(d, o, ["KE_PROD", "KE_TESRC", "KE_EDSRC", "KE_APED", "KE_VISTE", "KE_VISED", "BTE"], 0.0)
+    for n in ("ED", "BED", "KE_APTE"):
+        a, b = d.get(n), o.get(n)
+        assert np.all(np.abs(a - b) <= 4 * np.spacing(np.abs(b))), n
+        d.set(n, b)
+    log("k and epsilon systems")
+    for isca in (2, 3):
+        for x in (d, o):
+            x.call("build_scalar_matrix", isca)
+        if isca == 3:
+            for x in (d, o):
+                x.call("eps_wall_rows")
+        close(d, o, ["DPHIDX", "AS", "BS_RHS"], 0.0)
+    d.close()
