@@ -294,3 +294,32 @@ def test_ilu_sweeps_replayed_from_graphs(ksp):
     for n in FIELDS:
         assert np.array_equal(out[0][0][n], out[1][0][n]), n
     assert np.array_equal(out[0][1], out[1][1])
+
+
+def shuffled(case, seed):
+    """The same mesh with its element stream in a random order."""
+    rng = np.random.default_rng(seed)
+    ne = case.etype.size
+    perm = rng.permutation(ne)
+    lens = np.diff(case.vptr)[perm]
+    vptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    vidx = np.concatenate([case.vidx[case.vptr[e]:case.vptr[e + 1]] for e in perm]).astype(np.int32)
+    return s.CaseData(case.xyz.copy(), case.etype[perm].copy(), case.tag[perm].copy(), vptr, vidx,
+                      list(case.regions), dict(case.meta))
+
+
+def test_inner_order_falls_back_when_offsets_do_not_fit():
+    # 48 K cells in a random input order: some column lies more than 32767
+    # positions from its row, so the FP32 cycle stays in the natural order,
+    # and the solves are those of the natural-order run
+    case = shuffled(s.channel(40, 20, 40, jitter=0.1, split=0.1, seed=3, re=300.0), 9)
+    out = {}
+    for order in (0, 2):
+        p = base(s.MIXED)
+        p.inner_order = order
+        d = s.Solver(case, p)
+        h, _ = d.iterate(2)
+        assert d.kernel_shapes()["fp32"]["order"] == 0
+        out[order] = (h, d.get("P"))
+        d.close()
+    assert np.array_equal(out[0][0], out[2][0]) and np.array_equal(out[0][1], out[2][1])
