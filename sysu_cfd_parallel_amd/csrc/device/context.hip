@@ -282,10 +282,15 @@ void Context::upload_mesh() {
       std::vector<int4> rec;
       std::vector<int> b(1, 0);
       rec.reserve(static_cast<size_t>(m.cf_ptr[m.n_cell]));
+      int cells = 0;  // in the current block: at most one wave's worth
       for (int t = 0; t < m.n_cell; ++t) {
         const int i = cord[t], e0 = m.cf_ptr[i], e1 = m.cf_ptr[i + 1];
         if (e1 - e0 > kBlock) throw std::runtime_error("mesh: a cell with more faces than a block has lanes");
-        if (static_cast<int>(rec.size()) - b.back() + (e1 - e0) > kBlock) b.push_back(static_cast<int>(rec.size()));
+        if (static_cast<int>(rec.size()) - b.back() + (e1 - e0) > kBlock || cells == kWave) {
+          b.push_back(static_cast<int>(rec.size()));
+          cells = 0;
+        }
+        ++cells;
         for (int e = e0; e < e1; ++e) {
           const int f = faces[e];
           const int nb = f < m.n_bnd ? 0 : (m.f_c1[f] == i ? m.f_c2[f] : ~m.f_c1[f]);

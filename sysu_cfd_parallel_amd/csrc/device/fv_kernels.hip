@@ -347,38 +347,85 @@ struct MomArgs {
   double dt;
 };
 
+// Cells of a half-face block: the block's entries cover at most kWave whole
+// cells; slot k is the block's k-th cell, its entries [beg[k], beg[k + 1]).
+// Every thread gets the slot of its entry (-1 past the block's entries).
+struct HfCells {
+  int n, ncell, slot;
+};
+__device__ __forceinline__ HfCells hf_cells(int n, int cell, int* s_cell, int* s_cid, int* s_beg, int* s_wcnt) {
+  const int t = threadIdx.x, lane = t & (kWave - 1), wid = t / kWave;
+  s_cell[t] = cell;
+  __syncthreads();
+  const bool lead = t < n && (t == 0 || s_cell[t - 1] != cell);
+  const unsigned long long bal = __ballot(lead);
+  if (lane == 0) s_wcnt[wid] = __popcll(bal);
+  __syncthreads();
+  int off = 0, ncell = 0;
+#pragma unroll
+  for (int q = 0; q < kBlock / kWave; ++q) {
+    if (q < wid) off += s_wcnt[q];
+    ncell += s_wcnt[q];
+  }
+  const unsigned long long upto = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1);
+  const int slot = t < n ? off + __popcll(bal & upto) - 1 : -1;
+  if (lead) {
+    s_cid[slot] = cell;
+    s_beg[slot] = t;
+  }
+  if (t == 0) s_beg[ncell] = n;
+  __syncthreads();
+  return {n, ncell, slot};
+}
+
 // BuildVelocityMatrix cell loop (navier_velocity.cpp:100-383) over the
-// half-face stream in pattern order.  Lane t of a block evaluates entry t's
-// face terms -- the two additions to the diagonal, the three source terms
-// fde - fdi - fcs and the off-diagonal -- with every load one step after the
-// entry record; the cell's first lane then adds them in the cell's face
-// order, after the time term (107-123), so every sum is the reference's, term
-// for term (an absent term is +0, which leaves the non-negative diagonal
-// unchanged).
+// half-face stream in pattern order.  One lane per cell of the block loads
+// the cell's own values into LDS; lane t then evaluates entry t's face terms
+// -- the two additions to the diagonal, the three source terms
+// fde - fdi - fcs and the off-diagonal -- with every global load one step
+// after the entry record; the cell's lane adds them in the cell's face
+// order, after the time term (107-123), so every sum is the reference's,
+// term for term (an absent term is +0, which leaves the non-negative
+// diagonal unchanged), and finishes the cell.  The cell's values and its
+// final sums taking one wave rather than a lane of every entry keeps the
+// vector memory instructions per entry down, which is what bounds this loop.
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_build_momentum(DMesh m, DFields F, MomArgs a) {
-  __shared__ int s_cell[kBlock];
+  __shared__ int s_cell[kBlock], s_cid[kWave], s_beg[kWave + 1], s_wcnt[kBlock / kWave];
+  __shared__ R s_own[16][kWave];
   __shared__ R s_t[5][kBlock];
   const int b = hf_block();
   const int E0 = m.hf_blk[b], n = m.hf_blk[b + 1] - E0;
   const int t = threadIdx.x;
-  int i = -1;
-  R Ui = R(0), Vi = R(0), Wi = R(0);
+  int4 r = make_int4(-1, 0, 0, -1);
+  if (t < n) r = m.hf[E0 + t];
+  const HfCells hc = hf_cells(n, t < n ? r.x : -1, s_cell, s_cid, s_beg, s_wcnt);
+  if (t < hc.ncell) {
+    // the cell's own values, one lane per cell
+    const int c = s_cid[t];
+    s_own[0][t] = R(m.c_x[3 * c]);
+    s_own[1][t] = R(m.c_x[3 * c + 1]);
+    s_own[2][t] = R(m.c_x[3 * c + 2]);
+    s_own[3][t] = R(F.U[c]);
+    s_own[4][t] = R(F.V[c]);
+    s_own[5][t] = R(F.W[c]);
+    s_own[6][t] = R(F.mu[c]) + R(F.mut[c]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      s_own[7 + k][t] = R(F.dUdX[3 * c + k]);
+      s_own[10 + k][t] = R(F.dVdX[3 * c + k]);
+      s_own[13 + k][t] = R(F.dWdX[3 * c + k]);
+    }
+  }
+  __syncthreads();
   if (t < n) {
-    const int4 r = m.hf[E0 + t];
-    i = r.x;
-    const int f = r.y, nb = r.z;
-    const R xi0 = R(m.c_x[3 * i]), xi1 = R(m.c_x[3 * i + 1]), xi2 = R(m.c_x[3 * i + 2]);
-    Ui = R(F.U[i]);
-    Vi = R(F.V[i]);
-    Wi = R(F.W[i]);
-    const R mui = R(F.mu[i]) + R(F.mut[i]);
-    const double* gu = &F.dUdX[3 * i];
-    const double* gv = &F.dVdX[3 * i];
-    const double* gw = &F.dWdX[3 * i];
-    const R gu0 = R(gu[0]), gu1 = R(gu[1]), gu2 = R(gu[2]);
-    const R gv0 = R(gv[0]), gv1 = R(gv[1]), gv2 = R(gv[2]);
-    const R gw0 = R(gw[0]), gw1 = R(gw[1]), gw2 = R(gw[2]);
+    const int i = r.x, f = r.y, nb = r.z, sl = hc.slot;
+    const R xi0 = s_own[0][sl], xi1 = s_own[1][sl], xi2 = s_own[2][sl];
+    const R Ui = s_own[3][sl], Vi = s_own[4][sl], Wi = s_own[5][sl];
+    const R mui = s_own[6][sl];
+    const R gu0 = s_own[7][sl], gu1 = s_own[8][sl], gu2 = s_own[9][sl];
+    const R gv0 = s_own[10][sl], gv1 = s_own[11][sl], gv2 = s_own[12][sl];
+    const R gw0 = s_own[13][sl], gw1 = s_own[14][sl], gw2 = s_own[15][sl];
     R app1 = R(0), app2 = R(0);
     R fde0, fde1, fde2, fdi0, fdi1, fdi2, fcs0, fcs1, fcs2;
     const R fx0 = R(m.f_x[3 * f]), fx1 = R(m.f_x[3 * f + 1]), fx2 = R(m.f_x[3 * f + 2]);
@@ -504,9 +551,10 @@ __global__ void __launch_bounds__(kBlock) k_build_momentum(DMesh m, DFields F, M
     s_t[3][t] = fde1 - fdi1 - fcs1;
     s_t[4][t] = fde2 - fdi2 - fcs2;
   }
-  s_cell[t] = i;
   __syncthreads();
-  if (i < 0 || (t > 0 && s_cell[t - 1] == i)) return;
+  if (t >= hc.ncell) return;
+  // the cell of slot t: its sums in face order and its row
+  const int i = s_cid[t];
   R app = R(0), su = R(0), sv = R(0), sw = R(0);
   if (a.scheme == 1) {
     const R coef = R(F.R[i]) / R(a.dt) * R(m.c_vol[i]);
@@ -521,13 +569,14 @@ __global__ void __launch_bounds__(kBlock) k_build_momentum(DMesh m, DFields F, M
     sv += coef * (R(2) * R(F.Vnp[i]) - R(0.5) * R(F.Vnp2[i]));
     sw += coef * (R(2) * R(F.Wnp[i]) - R(0.5) * R(F.Wnp2[i]));
   }
-  for (int j = t; j < n && s_cell[j] == i; ++j) {
+  for (int j = s_beg[t]; j < s_beg[t + 1]; ++j) {
     app += s_t[0][j];
     app += s_t[1][j];
     su += s_t[2][j];
     sv += s_t[3][j];
     sw += s_t[4][j];
   }
+  const R Ui = s_own[3][t], Vi = s_own[4][t], Wi = s_own[5][t];
   app /= R(a.urf);
   F.apr[i] = R(1) / (app);
   F.Au[m.diag[i]] = app;

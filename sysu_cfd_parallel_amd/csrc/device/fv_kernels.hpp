@@ -35,7 +35,8 @@ struct DMesh {
   // cell is the face's c1, ~c1 (< 0) where it is c2, and 0 on boundary faces;
   // hf in pattern order (the assembly loops), hfs in ascending face order
   // (the gradient).  Block b of a half-face kernel takes entries
-  // [blk[b], blk[b + 1]): at most kBlock of them, whole cells only.
+  // [blk[b], blk[b + 1]): at most kBlock of them, whole cells only, at most
+  // kWave cells.
   const int4* hf = nullptr;
   const int* hf_blk = nullptr;
   int hf_nblk = 0;

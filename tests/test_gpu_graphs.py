@@ -52,10 +52,14 @@ def test_graph_replays_match_eager_launches(prec, name, make):
         assert ia[k]["iters"] == ib[k]["iters"] and ia[k]["reason"] == ib[k]["reason"]
 
 
-def test_graph_replays_in_a_batch():
+@pytest.mark.parametrize("order", [0, 2])
+def test_graph_replays_in_a_batch(order):
+    # order 2 with graphs: the sweep workload's path (captured GMRES cycles
+    # and SIMPLE phases, each case's FP32 cycle in the input cell order)
     cases = [s.cavity(6, re=100.0), s.cavity(9, re=700.0, jitter=0.1, seed=5), s.cavity(5, re=300.0, tets=True)]
     p0 = base(s.MIXED)
     p1 = base(s.MIXED)
+    p0.inner_order = p1.inner_order = order
     p1.graphs = 1
     a, ha, _, _ = run(cases, p0, 5, batch=True)
     b, hb, _, _ = run(cases, p1, 5, batch=True)
