@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE passes scaled by a 1 GiB copy (profiles/hbm_traffic_channel_6M.md):
 # 276.4 MB read + 25.7 MB written.  Below the algorithmic figure, which prices
 # the CSR form (4-byte column ids): the FP32 copy stores 16-bit column offsets.
-MEASURED_TRAFFIC = {("k_spmv_sell2<float, false, false, true>", 5_995_990): 302_147_686}
+MEASURED_TRAFFIC = {("k_spmv_sell2<float, false, false, true>", 5_995_990): 302_145_434}
 METRIC = "SIMPLE iters/s @4M cells, FP64 vs mixed (<=1e-6 rel err); SpMV % HBM peak"
 STAGES = ["total", "build_velocity", "solve_velocity", "build_pressure", "solve_pressure", "other"]
 
