@@ -238,8 +238,6 @@ void Context::upload_mesh() {
   dm.f_c2 = upload(allocs_, m.f_c2, stream);
   dm.cf_ptr = upload(allocs_, m.cf_ptr, stream);
   dm.cf_pat = upload(allocs_, m.cf_pat, stream);
-  dm.cf_slot = upload(allocs_, m.cf_slot, stream);
-  dm.cf_sorted = upload(allocs_, m.cf_sorted, stream);
   dm.rowptr = upload(allocs_, m.rowptr, stream);
   dm.col = upload(allocs_, m.col, stream);
   dm.diag = upload(allocs_, m.diag, stream);

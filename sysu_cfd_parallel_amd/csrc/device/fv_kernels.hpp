@@ -27,8 +27,6 @@ struct DMesh {
   const double* c_x = nullptr;  // 3 per cell
   const int* cf_ptr = nullptr;
   const int* cf_pat = nullptr;
-  const int* cf_slot = nullptr;
-  const int* cf_sorted = nullptr;
   // half-face streams: every cell's entries, cells in processing order
   // (c_order), one record {cell, face, neighbour code, CSR slot (-1 on a
   // boundary face)} per entry, the neighbour code being c2 (>= 0) where the
