@@ -790,11 +790,12 @@ int Context::run_solve(int which, const double* b, double* x, const s355_ksp_opt
       kwork32_.ilu_zl = alloc<float>(dm.Nc);
       kwork32_.ilu_zu = alloc<float>(dm.Nc);
     }
-    convert_to_float(val, a32_, dm.nnz, stream);
     CsrView<float> A{dm.Nc, dm.rowptr, dm.col, dm.diag, a32_, dm.nnz};
     CsrView<double> A64{dm.Nc, dm.rowptr, dm.col, dm.diag, val, dm.nnz};
     const bool tune = par.autotune && o.type == S355_KSP_GMRES && !kwork32_.tuned;
-    if (use_order(o)) {
+    const bool ordered = use_order(o);
+    if (!ordered) convert_to_float(val, a32_, dm.nnz, stream);  // the inner order fills its own copy
+    if (ordered) {
       // the Arnoldi cycle on the rows in the inner order, tuned there
       fill_order(val, o.pc == S355_PC_JACOBI);
       CsrView<float> Ao{dm.Nc, ord_rowptr_, ord_col_, ord_diag_, a32o_, dm.nnz};

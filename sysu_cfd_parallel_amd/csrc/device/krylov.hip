@@ -1047,7 +1047,9 @@ __global__ void __launch_bounds__(kBlock) k_residual_pc(CsrView<X> A, Tiles t, c
   for (int i = a + threadIdx.x; i < e; i += kBlock) {
     double r = static_cast<double>(b[i]);
     if (!zero_guess) {
-      const X s = csr_dot<X, X, false>(A.val, A.col, A.rowptr[i], A.rowptr[i + 1], 1, x, X(0));
+      // the matrix streams once per restart: non-temporal, so it does not
+      // evict what the cycle's kernels reuse
+      const X s = csr_dot<X, X, true>(A.val, A.col, A.rowptr[i], A.rowptr[i + 1], 1, x, X(0));
       r -= static_cast<double>(s);
     }
     if (inv) r *= static_cast<double>(inv[i]);
