@@ -706,19 +706,93 @@ struct GsShape {
     return static_cast<size_t>(a) + (static_cast<size_t>(q) * kBlock + threadIdx.x) * VW;
   }
   __device__ static bool vec_ok(int a, size_t ld) { return (a % Vec16<T>::n) == 0 && (ld % Vec16<T>::n) == 0; }
-  // every chunk of this thread is whole and loads as one 16-byte vector
-  __device__ static bool full(int a, size_t end, size_t ld) {
-    return VW == Vec16<T>::n && vec_ok(a, ld) && row0(a, NC - 1) + VW <= end;
+  // every chunk of this thread lies inside the vector (and, when a chunk is
+  // 16 bytes, loads as one aligned 16-byte vector)
+  __device__ static bool whole(int a, size_t end, size_t ld) {
+    return (VW < Vec16<T>::n || vec_ok(a, ld)) && row0(a, NC - 1) + VW <= end;
   }
+  // ... and every chunk is 16 bytes
+  __device__ static bool full(int a, size_t end, size_t ld) { return VW == Vec16<T>::n && whole(a, end, ld); }
 };
 
-template <class T>
-__device__ __forceinline__ typename Vec16<T>::type load16(const T* p) {
-  return *reinterpret_cast<const typename Vec16<T>::type*>(p);
+// VW consecutive elements at p, all inside the vector, loaded non-temporally
+// as they are (one 16-byte load when VW fills it); nothing waits for them
+// until the values are used
+template <class T, int VW>
+__device__ __forceinline__ void load_raw(const T* p, T (&e)[VW]) {
+  if constexpr (VW == Vec16<T>::n) {
+    using E = T __attribute__((ext_vector_type(VW)));
+    const E v = __builtin_nontemporal_load(reinterpret_cast<const E*>(p));
+#pragma unroll
+    for (int u = 0; u < VW; ++u) e[u] = v[u];
+  } else {
+#pragma unroll
+    for (int u = 0; u < VW; ++u) e[u] = __builtin_nontemporal_load(p + u);
+  }
 }
-template <class T>
-__device__ __forceinline__ double lane_of(const typename Vec16<T>::type& v, int u) {
-  return static_cast<double>(reinterpret_cast<const T*>(&v)[u]);
+
+// f(integral_constant<int, c>) for c = rest in 1..7 (nothing for 0): a batch
+// of the last rest basis vectors with its size known at compile time
+template <class F>
+__device__ __forceinline__ void with_rest(int rest, F&& f) {
+  switch (rest) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 5: f(std::integral_constant<int, 5>{}); break;
+    case 6: f(std::integral_constant<int, 6>{}); break;
+    case 7: f(std::integral_constant<int, 7>{}); break;
+    default: break;
+  }
+}
+
+// acc[kk] += sum over the thread's whole chunks of w_i V_{k0+kk},i for
+// kk < NB: every load issued before any use, the products added chunk by
+// chunk and element by element, as the element path adds them
+template <class T, class S, int NB>
+__device__ __forceinline__ void dot_batch(const T* __restrict__ V, size_t ld, int k0, int a, const double* wr,
+                                          double* acc) {
+  T e[NB][S::NC][S::VW];
+#pragma unroll
+  for (int kk = 0; kk < NB; ++kk)
+#pragma unroll
+    for (int q = 0; q < S::NC; ++q) load_raw<T, S::VW>(V + static_cast<size_t>(k0 + kk) * ld + S::row0(a, q), e[kk][q]);
+#pragma unroll
+  for (int kk = 0; kk < NB; ++kk)
+#pragma unroll
+    for (int q = 0; q < S::NC; ++q)
+#pragma unroll
+      for (int u = 0; u < S::VW; ++u) acc[kk] += wr[q * S::VW + u] * static_cast<double>(e[kk][q][u]);
+}
+
+// sacc[u] += sgn * c[k] V_k,i0+u for k = k0 .. k0 + NB - 1 in that order
+// (sgn -1: the Gram-Schmidt update, +1: the solution update); the NB loads
+// issued before any use
+template <class T, int VW, int NB, bool Sub>
+__device__ __forceinline__ void axpy_batch(const T* __restrict__ V, size_t ld, int k0, size_t i0, const double* c,
+                                           double* sacc) {
+  T e[NB][VW];
+#pragma unroll
+  for (int kk = 0; kk < NB; ++kk) load_raw<T, VW>(V + static_cast<size_t>(k0 + kk) * ld + i0, e[kk]);
+#pragma unroll
+  for (int kk = 0; kk < NB; ++kk)
+#pragma unroll
+    for (int u = 0; u < VW; ++u) {
+      if constexpr (Sub)
+        sacc[u] -= c[k0 + kk] * static_cast<double>(e[kk][u]);
+      else
+        sacc[u] += c[k0 + kk] * static_cast<double>(e[kk][u]);
+    }
+}
+
+// the same over k = 0 .. nk - 1: batches of 8, then one of the rest
+template <class T, int VW, bool Sub>
+__device__ __forceinline__ void axpy_all(const T* __restrict__ V, size_t ld, int nk, size_t i0, const double* c,
+                                         double* sacc) {
+  int k = 0;
+  for (; k + 8 <= nk; k += 8) axpy_batch<T, VW, 8, Sub>(V, ld, k, i0, c, sacc);
+  with_rest(nk - k, [&](auto nb) { axpy_batch<T, VW, decltype(nb)::value, Sub>(V, ld, k, i0, c, sacc); });
 }
 
 // ---- partial[k][b] = sum over tile b of w_i V_k,i, k = 0..it (k-major, so
@@ -726,8 +800,11 @@ __device__ __forceinline__ double lane_of(const typename Vec16<T>::type& v, int 
 // is the tile count, pstride only bounds the capacity).  The k run in chunks
 // of 8; block (b, g) takes chunks g * cpb .. (g + 1) * cpb - 1 of tile b, so
 // a small system spreads its chunks over more blocks (each chunk's sums are
-// the same whichever block computes them).
-template <class T, int R>
+// the same whichever block computes them).  A thread whose chunks are 16
+// bytes issues the loads of a chunk of 8 vectors before it uses any; with
+// Batch (small grids, see k_maxpy_norm) it does so for every whole chunk and
+// for the last chunk's few vectors too.  The sums are the same either way.
+template <class T, int R, bool Batch>
 __global__ void __launch_bounds__(kBlock) k_mdot(const T* __restrict__ V, size_t ld, Tiles t, int it,
                                                  double* __restrict__ partial, int pstride, const KState* st,
                                                  int cpb) {
@@ -746,26 +823,15 @@ __global__ void __launch_bounds__(kBlock) k_mdot(const T* __restrict__ V, size_t
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
   const int kbeg = 8 * cpb * static_cast<int>(blockIdx.y);
   const int kend = min(it + 1, 8 * cpb * (static_cast<int>(blockIdx.y) + 1));
-  const bool full = S::full(a, end, ld);
+  const bool whole = Batch ? S::whole(a, end, ld) : S::full(a, end, ld);
   for (int k0 = kbeg; k0 < kend; k0 += 8) {
     double acc[8];
 #pragma unroll
     for (int kk = 0; kk < 8; ++kk) acc[kk] = 0.0;
-    if (full && k0 + 8 <= kend) {
-      // whole 16-byte chunks of 8 existing vectors: every load issued before
-      // any use; the products are added in the element path's order
-      typename Vec16<T>::type raw[8][S::NC];
-#pragma unroll
-      for (int kk = 0; kk < 8; ++kk)
-#pragma unroll
-        for (int q = 0; q < S::NC; ++q)
-          raw[kk][q] = load16_nt<T>(V + static_cast<size_t>(k0 + kk) * ld + S::row0(a, q));
-#pragma unroll
-      for (int kk = 0; kk < 8; ++kk)
-#pragma unroll
-        for (int q = 0; q < S::NC; ++q)
-#pragma unroll
-          for (int u = 0; u < S::VW; ++u) acc[kk] += wr[q * S::VW + u] * lane_of<T>(raw[kk][q], u);
+    if (whole && k0 + 8 <= kend) {
+      dot_batch<T, S, 8>(V, ld, k0, a, wr, acc);
+    } else if (Batch && whole) {
+      with_rest(kend - k0, [&](auto nb) { dot_batch<T, S, decltype(nb)::value>(V, ld, k0, a, wr, acc); });
     } else {
 #pragma unroll
       for (int kk = 0; kk < 8; ++kk) {
@@ -847,8 +913,13 @@ __global__ void __launch_bounds__(kBlock) k_mdot_final(const double* partial, Ti
   if (threadIdx.x == 0) s->hcol[k] = v * s->scale[k];
 }
 
-// ---- w -= sum_k hcol_k scale_k V_k ; partial ||w||^2 per tile
-template <class T, int R>
+// ---- w -= sum_k hcol_k scale_k V_k ; partial ||w||^2 per tile.  With Batch
+// (grids too small to hide memory latency with many waves) each thread
+// issues the loads of up to 8 basis vectors before it uses any; otherwise it
+// streams them one after the other and the resident waves hide the latency
+// (batched loads measured slower there).  The subtractions run in the same
+// order either way.
+template <class T, int R, bool Batch>
 __global__ void __launch_bounds__(kBlock) k_maxpy_norm(T* __restrict__ V, size_t ld, Tiles t, int it,
                                                        double* __restrict__ partial, const KState* st) {
   using S = GsShape<T, R>;
@@ -862,6 +933,7 @@ __global__ void __launch_bounds__(kBlock) k_maxpy_norm(T* __restrict__ V, size_t
   __syncthreads();
   const size_t end = static_cast<size_t>(e);
   const bool vec = S::vec_ok(a, ld);
+  const bool whole = Batch && S::whole(a, end, ld);
   T* w = V + static_cast<size_t>(it + 1) * ld;
   double nrm = 0.0;
 #pragma unroll
@@ -869,11 +941,15 @@ __global__ void __launch_bounds__(kBlock) k_maxpy_norm(T* __restrict__ V, size_t
     const size_t i0 = S::row0(a, q);
     double sacc[S::VW];
     load_chunk<T, S::VW>(w, i0, end, vec, sacc);
-    for (int k = 0; k <= it; ++k) {
-      double v[S::VW];
-      load_chunk<T, S::VW, true>(V + static_cast<size_t>(k) * ld, i0, end, vec, v);
+    if (Batch && whole) {
+      axpy_all<T, S::VW, true>(V, ld, it + 1, i0, cf, sacc);
+    } else {
+      for (int k = 0; k <= it; ++k) {
+        double v[S::VW];
+        load_chunk<T, S::VW, true>(V + static_cast<size_t>(k) * ld, i0, end, vec, v);
 #pragma unroll
-      for (int u = 0; u < S::VW; ++u) sacc[u] -= cf[k] * v[u];
+        for (int u = 0; u < S::VW; ++u) sacc[u] -= cf[k] * v[u];
+      }
     }
     T out[S::VW];
 #pragma unroll
@@ -993,7 +1069,8 @@ __global__ void __launch_bounds__(kWave) k_build_y(KState* st, int m) {
 }
 
 // x += sum_k y_k V_k; with xrow (inner order) position i updates x[xrow[i]]
-template <class T, class X, int R>
+// (Batch as in k_maxpy_norm)
+template <class T, class X, int R, bool Batch>
 __global__ void __launch_bounds__(kBlock) k_update_x(const T* __restrict__ V, size_t ld, Tiles t, X* __restrict__ x,
                                                      const KState* st, const int* __restrict__ xrow) {
   using S = GsShape<T, R>;
@@ -1007,17 +1084,22 @@ __global__ void __launch_bounds__(kBlock) k_update_x(const T* __restrict__ V, si
   __syncthreads();
   const size_t end = static_cast<size_t>(e);
   const bool vec = S::vec_ok(a, ld);
+  const bool whole = Batch && S::whole(a, end, ld);
 #pragma unroll
   for (int q = 0; q < S::NC; ++q) {
     const size_t i0 = S::row0(a, q);
     double sacc[S::VW];
 #pragma unroll
     for (int u = 0; u < S::VW; ++u) sacc[u] = 0.0;
-    for (int k = 0; k < nk; ++k) {
-      double v[S::VW];
-      load_chunk<T, S::VW, true>(V + static_cast<size_t>(k) * ld, i0, end, vec, v);
+    if (Batch && whole) {
+      axpy_all<T, S::VW, false>(V, ld, nk, i0, y, sacc);
+    } else {
+      for (int k = 0; k < nk; ++k) {
+        double v[S::VW];
+        load_chunk<T, S::VW, true>(V + static_cast<size_t>(k) * ld, i0, end, vec, v);
 #pragma unroll
-      for (int u = 0; u < S::VW; ++u) sacc[u] += y[k] * v[u];
+        for (int u = 0; u < S::VW; ++u) sacc[u] += y[k] * v[u];
+      }
     }
 #pragma unroll
     for (int u = 0; u < S::VW; ++u)
@@ -1353,6 +1435,11 @@ constexpr int kMdotBlocks = 1024;
 
 int log2_r(int R) { return R == 16 ? 4 : R == 8 ? 3 : R == 4 ? 2 : R == 2 ? 1 : 0; }
 
+// Below this many tiles (blocks of 256 threads) a Gram-Schmidt update or a
+// solution update has too few resident waves to hide its loads' latency, and
+// each thread batches its basis loads
+constexpr int kBatchTiles = 1024;
+
 template <class T>
 struct Dispatch {
   template <int R>
@@ -1363,17 +1450,27 @@ struct Dispatch {
     const int groups = std::max(1, std::min(chunks, (kMdotBlocks + t.ntile - 1) / t.ntile));
     const int cpb = (chunks + groups - 1) / groups;
     const int gy = (chunks + cpb - 1) / cpb;
-    hipLaunchKernelGGL((k_mdot<T, R>), dim3(t.ntile, gy), dim3(kBlock), 0, s, V, ld, t, it, partial, pstride, st,
-                       cpb);
+    if (t.ntile * gy < kBatchTiles)
+      hipLaunchKernelGGL((k_mdot<T, R, true>), dim3(t.ntile, gy), dim3(kBlock), 0, s, V, ld, t, it, partial, pstride,
+                         st, cpb);
+    else
+      hipLaunchKernelGGL((k_mdot<T, R, false>), dim3(t.ntile, gy), dim3(kBlock), 0, s, V, ld, t, it, partial, pstride,
+                         st, cpb);
   }
   template <int R>
   static void maxpy_r(T* V, size_t ld, const Tiles& t, int it, double* partial, const KState* st, hipStream_t s) {
-    hipLaunchKernelGGL((k_maxpy_norm<T, R>), dim3(t.ntile), dim3(kBlock), 0, s, V, ld, t, it, partial, st);
+    if (t.ntile < kBatchTiles)
+      hipLaunchKernelGGL((k_maxpy_norm<T, R, true>), dim3(t.ntile), dim3(kBlock), 0, s, V, ld, t, it, partial, st);
+    else
+      hipLaunchKernelGGL((k_maxpy_norm<T, R, false>), dim3(t.ntile), dim3(kBlock), 0, s, V, ld, t, it, partial, st);
   }
   template <int R, class X>
   static void update_r(const T* V, size_t ld, const Tiles& t, X* x, const KState* st, const int* xrow,
                        hipStream_t s) {
-    hipLaunchKernelGGL((k_update_x<T, X, R>), dim3(t.ntile), dim3(kBlock), 0, s, V, ld, t, x, st, xrow);
+    if (t.ntile < kBatchTiles)
+      hipLaunchKernelGGL((k_update_x<T, X, R, true>), dim3(t.ntile), dim3(kBlock), 0, s, V, ld, t, x, st, xrow);
+    else
+      hipLaunchKernelGGL((k_update_x<T, X, R, false>), dim3(t.ntile), dim3(kBlock), 0, s, V, ld, t, x, st, xrow);
   }
   static void mdot(int R, const T* V, size_t ld, const Tiles& t, int it, double* partial, int pstride,
                    const KState* st, hipStream_t s) {
