@@ -134,3 +134,66 @@ def parse_tecplot(path) -> dict:
     out["conn"] = np.array(tok[off:off + 8 * ne], dtype=np.int64).reshape(ne, 8)
     assert off + 8 * ne == len(tok)
     return out
+
+
+# hex corner bits a | b << 1 | c << 2 in gmsh hex order
+_HEX = (0, 1, 3, 2, 4, 5, 7, 6)
+# the six hex faces as quads whose right-hand normal points into the cell
+_INWARD = ((0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6), (0, 2, 6, 4), (1, 5, 7, 3))
+
+
+def hybrid_channel(nx, ny, nz, prism_frac=0.3, pyramid_frac=0.3, jitter=0.0, seed=0, re=1000.0, lx=2.0, ly=1.0,
+                   lz=1.0):
+    """The channel of ``s.channel`` (inlet x = 0, outlet x = lx, walls y = 0, ly,
+    symmetry z = 0, lz) on a hybrid mesh: whole z-columns of the hex lattice
+    split into two prisms along the (0,0)-(1,1) diagonal (their z faces
+    triangles, conforming from layer to layer), and a random share of the
+    other hexes split into six pyramids around a vertex at the cell's corner
+    average (their faces stay the hex's quads); nodes jittered by up to
+    jitter times the spacing, boundary nodes kept on their planes."""
+    rng = np.random.default_rng(seed)
+    h = np.array([lx / nx, ly / ny, lz / nz])
+    I, J, K = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    ijk = np.stack([I.ravel(), J.ravel(), K.ravel()], 1)
+    xyz = ijk * h
+    n = np.array([nx, ny, nz])
+    move = (ijk > 0) & (ijk < n)
+    xyz = xyz + move * rng.uniform(-jitter, jitter, xyz.shape) * h
+    nid = lambda i, j, k: (i * (ny + 1) + j) * (nz + 1) + k  # noqa: E731
+    pts = [tuple(p) for p in xyz]
+    prism_col = rng.uniform(size=(nx, ny)) < prism_frac
+    pyramid = rng.uniform(size=(nx, ny, nz)) < pyramid_frac
+    tag_of = {(0, 0): 2, (0, 1): 3, (1, 0): 1, (1, 1): 1, (2, 0): 4, (2, 1): 4}
+    bnd, vol = [], []
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                v = [nid(i + (c & 1), j + ((c >> 1) & 1), k + ((c >> 2) & 1)) for c in range(8)]
+                cell = (i, j, k)
+                for f, quad in enumerate(_INWARD):
+                    axis = (2, 2, 1, 1, 0, 0)[f]
+                    side = (0, 1, 0, 1, 0, 1)[f]
+                    if 0 <= cell[axis] + (2 * side - 1) < n[axis]:
+                        continue
+                    out = [v[c] for c in reversed(quad)]  # outward
+                    if axis == 2 and prism_col[i, j]:
+                        if side == 0:
+                            bnd += [(2, tag_of[(2, 0)], [v[0], v[3], v[1]]), (2, tag_of[(2, 0)], [v[0], v[2], v[3]])]
+                        else:
+                            bnd += [(2, tag_of[(2, 1)], [v[4], v[5], v[7]]), (2, tag_of[(2, 1)], [v[4], v[7], v[6]])]
+                    else:
+                        bnd.append((3, tag_of[(axis, side)], out))
+                if prism_col[i, j]:
+                    vol.append((6, 100, [v[0], v[1], v[3], v[4], v[5], v[7]]))
+                    vol.append((6, 100, [v[0], v[3], v[2], v[4], v[7], v[6]]))
+                elif pyramid[i, j, k]:
+                    apex = len(pts)
+                    pts.append(tuple(np.mean([pts[c] for c in v], axis=0)))
+                    for quad in _INWARD:
+                        vol.append((7, 100, [v[c] for c in quad] + [apex]))
+                else:
+                    vol.append((5, 100, [v[c] for c in _HEX]))
+    regions = s.channel(1, 1, 1, re=re).regions
+    case = manual_case(np.array(pts), bnd + vol, regions)
+    case.meta = {"kind": "channel", "mesh": "hybrid", "nx": nx, "ny": ny, "nz": nz, "jitter": jitter, "seed": seed}
+    return case

@@ -5,13 +5,13 @@ GPU from the ingest's topology and vertices, and copies them back for the
 host.  They must equal the host ingest's metrics (the same expressions in
 the same order) bit for bit -- hexes with 15 % jitter (warped quads),
 hexes split into tetrahedra with the polyhedral cells between, the
-tetrahedral step, a batch of cases, and the full-size channel -- and count
-the same warped faces."""
+tetrahedral step, prisms and pyramids among hexes, a batch of cases, and the
+full-size channel -- and count the same warped faces."""
 import numpy as np
 import pytest
 
 import sysu_cfd_parallel_amd as s
-from helpers import params
+from helpers import hybrid_channel, params
 
 pytestmark = pytest.mark.gpu
 
@@ -23,6 +23,7 @@ CASES = {
     "channel_split": lambda: s.channel(12, 7, 9, jitter=0.15, split=0.2, seed=2),
     "step_tets": lambda: s.step(3, jitter=0.10, tets=True, turbulent=True),
     "cavity_tets": lambda: s.cavity(6, tets=True),
+    "hybrid_prisms_pyramids": lambda: hybrid_channel(6, 4, 5, prism_frac=0.3, pyramid_frac=0.3, jitter=0.15, seed=2),
 }
 
 

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import sysu_cfd_parallel_amd as s
-from helpers import params, push, random_state, rel_err
+from helpers import hybrid_channel, params, push, random_state, rel_err
 from oracle.pyoracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -38,6 +38,7 @@ CASES = {
     "cavity_jitter": lambda: s.cavity(6, jitter=0.15, seed=1),
     "channel_split": lambda: s.channel(8, 5, 6, jitter=0.15, split=0.15, seed=2),
     "step_tets": lambda: s.step(2, upstream=2, downstream=4, span=2, turbulent=True),
+    "hybrid_prisms_pyramids": lambda: hybrid_channel(6, 4, 5, prism_frac=0.3, pyramid_frac=0.3, jitter=0.15, seed=3),
 }
 
 
@@ -176,6 +177,13 @@ def test_simple_loop_cavity_fp64():
 
 def test_simple_loop_channel_split_fp64():
     d, o, hd, ho = _loop_pair(s.channel(10, 6, 8, jitter=0.15, split=0.1, seed=1, re=200), params(rtol=1e-12), 10)
+    assert np.allclose(hd, ho, rtol=1e-6, atol=1e-14)
+    assert_close(d, o, ["U", "V", "W", "P"], 1e-7)
+
+
+def test_simple_loop_hybrid_prisms_pyramids_fp64():
+    case = hybrid_channel(8, 5, 6, prism_frac=0.3, pyramid_frac=0.3, jitter=0.1, seed=4, re=200.0)
+    d, o, hd, ho = _loop_pair(case, params(rtol=1e-12), 10)
     assert np.allclose(hd, ho, rtol=1e-6, atol=1e-14)
     assert_close(d, o, ["U", "V", "W", "P"], 1e-7)
 

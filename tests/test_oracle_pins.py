@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import sysu_cfd_parallel_amd as s
-from helpers import params, push, random_state, rel_err, tet_cases
+from helpers import hybrid_channel, params, push, random_state, rel_err, tet_cases
 from oracle.pyoracle import Oracle
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
@@ -64,6 +64,7 @@ def test_face_counts_tets():
 @pytest.mark.parametrize("case", [
     lambda: s.cavity(5), lambda: s.cavity(4, jitter=0.15, seed=2), lambda: s.cavity(3, tets=True),
     lambda: s.channel(5, 3, 4, jitter=0.12, split=0.3, seed=5), lambda: s.step(2, upstream=2, downstream=3, span=1),
+    lambda: hybrid_channel(4, 3, 3, prism_frac=0.4, pyramid_frac=0.4, jitter=0.15, seed=1),
 ])
 def test_face_incidence_invariants(case):
     o = Oracle(case(), params())
@@ -126,6 +127,10 @@ def test_tet_volume_closed_form():
     (lambda: s.cavity(5, jitter=0.2, seed=7), 1.0, 5e-3),
     (lambda: s.channel(6, 3, 5, lx=2, ly=1, lz=2, jitter=0.15, split=1.0, seed=9), 4.0, 1e-13),
     (lambda: s.channel(6, 3, 5, lx=2, ly=1, lz=2, jitter=0.15, split=0.25, seed=9), 4.0, 5e-3),
+    (lambda: hybrid_channel(4, 3, 3, prism_frac=0.4, pyramid_frac=0.4, seed=1), 2.0, 1e-13),
+    # the same with prisms and pyramids, whose degenerate-hex volume formula
+    # splits every warped quad along a diagonal of its own
+    (lambda: hybrid_channel(4, 3, 3, prism_frac=0.4, pyramid_frac=0.4, jitter=0.15, seed=1), 2.0, 3e-2),
 ])
 def test_volume_sum_and_closed_cells(case, vol, tol):
     o = Oracle(case(), params())
@@ -138,6 +143,35 @@ def test_volume_sum_and_closed_cells(case, vol, tol):
     np.add.at(acc, c1, nrm)
     np.subtract.at(acc, o.ints("FACE_C2")[z.n_bnd:], nrm[z.n_bnd:])
     assert np.max(np.abs(acc)) < 1e-13
+
+
+def test_prism_and_pyramid_closed_forms():
+    """Uniform hybrid lattice: every hex, prism (half a hex) and pyramid (a
+    sixth, apex at the centre) has its exact volume and centroid -- the
+    prism's at its vertex mean, the pyramid's a quarter of the way from the
+    base centre to the apex."""
+    nx, ny, nz, lx, ly, lz = 4, 3, 3, 2.0, 1.0, 1.0
+    case = hybrid_channel(nx, ny, nz, prism_frac=0.4, pyramid_frac=0.4, seed=1, lx=lx, ly=ly, lz=lz)
+    o = Oracle(case, params())
+    hv = lx * ly * lz / (nx * ny * nz)
+    xyz = case.xyz.reshape(-1, 3)
+    want = []
+    ncell = int(np.count_nonzero(case.etype >= 4))
+    for e in range(ncell):
+        k = len(case.etype) - ncell + e
+        v = xyz[case.vidx[case.vptr[k]:case.vptr[k + 1]]]
+        t = case.etype[k]
+        if t == 5:
+            want.append([hv, *v.mean(0)])
+        elif t == 6:
+            want.append([hv / 2, *v.mean(0)])
+        else:
+            base = v[:4].mean(0)
+            want.append([hv / 6, *(base + (v[4] - base) / 4)])
+    got = np.column_stack([o.dbls("CELL_VOL"), o.dbls("CELL_X").reshape(-1, 3)])
+    order = lambda a: a[np.lexsort(np.round(a, 9).T[::-1])]  # noqa: E731
+    assert {5, 6, 7} <= set(case.etype.tolist())
+    assert np.allclose(order(got), order(np.array(want)), rtol=1e-13, atol=1e-15)
 
 
 def test_affine_map_scales_volumes():
@@ -163,8 +197,12 @@ def test_gradient_exact_for_linear_field():
     assert np.allclose(o.get("DPDX").reshape(-1, 3), g, rtol=0, atol=1e-12)
 
 
-def test_gradient_telescoping_identity_jittered():
-    case = s.channel(5, 4, 4, jitter=0.15, split=0.2, seed=3)
+@pytest.mark.parametrize("case_fn", [
+    lambda: s.channel(5, 4, 4, jitter=0.15, split=0.2, seed=3),
+    lambda: hybrid_channel(4, 3, 3, prism_frac=0.4, pyramid_frac=0.4, jitter=0.15, seed=2),
+])
+def test_gradient_telescoping_identity_jittered(case_fn):
+    case = case_fn()
     o = Oracle(case, params())
     z = o.sizes
     rng = np.random.default_rng(1)
