@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <functional>
+#include <limits>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -1220,9 +1221,12 @@ __global__ void __launch_bounds__(kBlock) k_restart_finalize(const double* parti
 
 // ---- end of a GMRES cycle, after the update of x (one thread per case).
 // Plain solve: a set reason ends the case, then the iteration cap.  Mixed
-// solve: a passed estimate, a breakdown of the FP32 cycle or the cap met
-// inside the cycle is cleared and goes back to the FP64 residual, which
-// applies the cap after its own test; a divergence ends the case.
+// solve: a passed estimate, a breakdown of the FP32 cycle after which x was
+// still updated, or the cap met inside the cycle is cleared and goes back to
+// the FP64 residual, which applies the cap after its own test; a divergence
+// ends the case, and so does a breakdown of the back substitution (no
+// update: the next cycle would start from the same residual and break down
+// the same way).
 __global__ void k_cycle_end(KState* st, int ncase, int mixed) {
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncase; c += gridDim.x * blockDim.x) {
     KState* s = st + c;
@@ -1236,7 +1240,8 @@ __global__ void k_cycle_end(KState* st, int ncase, int mixed) {
         s->done = 1;
       }
     } else {
-      if (r < 0 && r != kDivergedBreakdown && r != kDivergedIts)
+      const int nupd = *reinterpret_cast<const int*>(&s->pad);
+      if ((r < 0 && r != kDivergedBreakdown && r != kDivergedIts) || (r == kDivergedBreakdown && nupd == 0))
         s->done = 1;
       else
         s->reason = 0;
@@ -1524,6 +1529,15 @@ bool all_stopped(const KWork& w) {
   return true;
 }
 
+// the fewest iterations a running case has completed (device count); int
+// max when none runs
+int running_its(const KWork& w) {
+  int its = std::numeric_limits<int>::max();
+  for (int c = 0; c < w.ncase; ++c)
+    if (!w.host[c].done) its = std::min(its, w.host[c].its);
+  return its;
+}
+
 void init_states(KWork& w, const s355_ksp_opts& o, hipStream_t s) {
   hipLaunchKernelGGL(k_init_states, dim3(grid_for(w.ncase)), dim3(kBlock), 0, s, w.st, w.ncase, o.maxit, o.rtol,
                      o.abstol, o.dtol);
@@ -1708,6 +1722,10 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
     return collect(w, mixed);
   }
 
+  // its: iterations launched, an upper bound on a running case's count that
+  // only saves launches; it goes back to the device count at every header
+  // read, so cycles that end early (the device applies the cap) never stop
+  // the launches before the device has ended the solve
   bool first = true, gz = zero_guess;
   int its = 0;
   const int check_every = n >= (1 << 20) ? 4 : 8;
@@ -1717,6 +1735,7 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
       // the FP64 residual decides whether the solve has ended
       read_headers(w, s);
       if (all_done(w)) break;
+      its = running_its(w);
     }
     first = false;
     gz = false;
@@ -1733,6 +1752,7 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
     enqueue_end();
     read_headers(w, s);
     if (all_done(w)) break;
+    its = running_its(w);
   }
   if (w.timer) w.timer->harvest();  // the header reads have synchronised the stream
   return collect(w, mixed);
