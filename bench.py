@@ -553,7 +553,7 @@ def main() -> int:
                     help="replay GMRES cycles from captured graphs (default: per workload)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus above 1 is not supported"}))
         log("only --gpus 1 is supported: every case runs on a single MI355X")
         return 2
 
