@@ -942,7 +942,7 @@ __global__ void __launch_bounds__(kBlock) k_maxpy_norm(T* __restrict__ V, size_t
     const size_t i0 = S::row0(a, q);
     double sacc[S::VW];
     load_chunk<T, S::VW>(w, i0, end, vec, sacc);
-    if (Batch && whole) {
+    if (whole) {
       axpy_all<T, S::VW, true>(V, ld, it + 1, i0, cf, sacc);
     } else {
       for (int k = 0; k <= it; ++k) {
@@ -1092,7 +1092,7 @@ __global__ void __launch_bounds__(kBlock) k_update_x(const T* __restrict__ V, si
     double sacc[S::VW];
 #pragma unroll
     for (int u = 0; u < S::VW; ++u) sacc[u] = 0.0;
-    if (Batch && whole) {
+    if (whole) {
       axpy_all<T, S::VW, false>(V, ld, nk, i0, y, sacc);
     } else {
       for (int k = 0; k < nk; ++k) {
