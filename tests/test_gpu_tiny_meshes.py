@@ -1,6 +1,6 @@
 """The smallest meshes through the device path, against the FP64 oracle: one
 hex cell (no interior face), two and three hexes in a row, the 2^3 cavity,
-one and two tetrahedra.  Every row kernel, half-face block, SELL slice and
+one and two tetrahedra, and k-epsilon on 12 and 30 cells.  Every row kernel, half-face block, SELL slice and
 reduction tile is then mostly padding: geometry equals the host's bit for
 bit, the assembly equals the oracle's (bitwise, outlet-rescaled arrays to
 rounding), and SIMPLE loops in FP64 and mixed precision follow the oracle.
@@ -85,6 +85,39 @@ def test_simple_loop(name, prec):
         for n in ("U", "V", "W", "P"):
             a, b = d.get(n), o.get(n)
             assert np.allclose(a, b, rtol=tol, atol=tol * max(1.0, float(np.abs(b).max()))), n
+    finally:
+        d.close()
+
+
+TURB = {
+    "step_30_cells": lambda: s.step(1, upstream=1, downstream=2, span=1, turbulent=True, re=5000.0),
+    "channel_12_cells": lambda: s.channel(3, 2, 2, turbulent=True, re=13750.0),
+}
+
+
+@pytest.mark.parametrize("prec", [s.FP64, s.MIXED])
+@pytest.mark.parametrize("name", sorted(TURB))
+def test_kepsilon_loop(name, prec):
+    """k-epsilon with wall functions on a few cells: every cell touches a
+    wall, an inlet or an outlet."""
+    log(f"k-epsilon {name} precision {prec}")
+    p = params(rtol=1e-12, turb=1)
+    p.precision = prec
+    case = TURB[name]()
+    d, o = s.Solver(case, p), Oracle(case, params(rtol=1e-12, turb=1))
+    try:
+        hd, _ = d.iterate(5)
+        ho, _, _ = o.iterate(5)
+        tol = 1e-7 if prec == s.FP64 else 1e-5
+        assert np.all(np.isfinite(hd))
+        assert np.allclose(hd, ho, rtol=tol, atol=1e-13)
+        # velocity components against the velocity scale (W is round-off on
+        # the channel, whose flow has no z component)
+        vref = max(float(np.abs(o.get(n)).max()) for n in ("U", "V", "W"))
+        for n in ("U", "V", "W", "P", "TE", "ED", "VISTUR"):
+            a, b = d.get(n), o.get(n)
+            ref = vref if n in ("U", "V", "W") else float(np.abs(b).max())
+            assert np.allclose(a, b, rtol=tol, atol=tol * ref), n
     finally:
         d.close()
 
