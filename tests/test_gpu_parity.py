@@ -144,6 +144,31 @@ def test_pressure_solve_matches_oracle(ksp, pc):
     assert_close(d, o, ["DELTAP"], 1e-8)
 
 
+@pytest.mark.parametrize("prec", [s.FP64, s.MIXED])
+@pytest.mark.parametrize("restart", [1, 2, 64])
+def test_restart_length_extremes_match_oracle(restart, prec):
+    """GMRES(1), GMRES(2) and GMRES(64), the longest cycle the workspace
+    holds (a 65-column Hessenberg matrix, one wavefront's worth of back
+    substitution), on a pressure correction that needs several cycles."""
+    case = s.cavity(10, jitter=0.1, seed=8)
+    p = params(restart=restart, pc=s.PC_JACOBI)
+    p.precision = prec
+    d, o = pair(case, p)
+    same_state(d, o, ["U", "V", "W", "P", "R", "APR", "DPDX", "RUFACE"], seed=9)
+    for k in ("bc_velocity", "cal_ru_face", "build_pressure_matrix"):
+        d.call(k)
+        o.call(k)
+    a = d.solve_pressure(1e-8, 20000)
+    b = o.solve_pressure(1e-8, 20000)
+    assert a["reason"] > 0 and b["reason"] > 0
+    assert b["iters"] > 64  # more than one full cycle of the longest restart
+    if prec == s.FP64:
+        assert abs(a["iters"] - b["iters"]) <= max(2, b["iters"] // 100)
+        assert_close(d, o, ["DELTAP"], 1e-6)
+    else:
+        assert_close(d, o, ["DELTAP"], 1e-5)
+
+
 def test_solve_cap_raises_not_converged():
     d = s.Solver(s.cavity(6), params(restart=5, pc=s.PC_NONE))
     st = random_state(d.sizes.n_cell, d.sizes.n_bnd, d.sizes.n_face, 1)
