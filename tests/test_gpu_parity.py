@@ -144,15 +144,18 @@ def test_pressure_solve_matches_oracle(ksp, pc):
     assert_close(d, o, ["DELTAP"], 1e-8)
 
 
+@pytest.mark.parametrize("graphs", [0, 1])
 @pytest.mark.parametrize("prec", [s.FP64, s.MIXED])
 @pytest.mark.parametrize("restart", [1, 2, 64])
-def test_restart_length_extremes_match_oracle(restart, prec):
+def test_restart_length_extremes_match_oracle(restart, prec, graphs):
     """GMRES(1), GMRES(2) and GMRES(64), the longest cycle the workspace
     holds (a 65-column Hessenberg matrix, one wavefront's worth of back
-    substitution), on a pressure correction that needs several cycles."""
+    substitution), on a pressure correction that needs several cycles;
+    launched kernel by kernel and as captured cycles."""
     case = s.cavity(10, jitter=0.1, seed=8)
     p = params(restart=restart, pc=s.PC_JACOBI)
     p.precision = prec
+    p.graphs = graphs
     d, o = pair(case, p)
     same_state(d, o, ["U", "V", "W", "P", "R", "APR", "DPDX", "RUFACE"], seed=9)
     for k in ("bc_velocity", "cal_ru_face", "build_pressure_matrix"):
