@@ -159,3 +159,28 @@ def test_inconsistent_one_cell_pressure_system_ends_with_breakdown(prec, graphs)
         d.set("BP_RHS", np.full(1, 1e-3))
         info = d.solve_pressure(1e-8, 1000)
         assert info["reason"] > 0 and d.get("DELTAP")[0] == pytest.approx(5e-4, rel=1e-8)
+
+
+@pytest.mark.parametrize("prec,order", [(s.FP64, 0), (s.MIXED, 0), (s.MIXED, 2)])
+def test_batch_of_many_tiny_cases(prec, order):
+    """40 two- and three-cell channels at different Reynolds numbers around a
+    3^3 cavity: SELL slices and reduction tiles hold many cases each (the
+    slices look their case up per lane), and every case must still equal its
+    run alone bit for bit."""
+    cases = [s.channel(2 + k % 2, 1, 1, re=50.0 + 10.0 * k) for k in range(20)]
+    cases += [s.cavity(3, re=200.0)]
+    cases += [s.channel(2 + k % 2, 1, 1, jitter=0.05, seed=k, re=300.0 + 5.0 * k) for k in range(20)]
+    p = params()
+    p.precision = prec
+    p.inner_order = order
+    log(f"batch of {len(cases)} tiny cases precision {prec} order {order}")
+    b = s.BatchSolver(cases, p)
+    try:
+        b.iterate(3)
+        for k, case in enumerate(cases):
+            with s.Solver(case, p) as d:
+                d.iterate(3)
+                for n in ("U", "V", "W", "P"):
+                    assert np.array_equal(b.case_field(k, n), d.get(n)), (k, n)
+    finally:
+        b.close()
