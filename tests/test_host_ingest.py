@@ -87,6 +87,36 @@ def test_bad_meshes_are_rejected(lib):
         s.HostMesh(s.CaseData(one.xyz, one.etype, one.tag, one.vptr, one.vidx, regs))
 
 
+def _elements(case, keep):
+    vp, vi = [0], []
+    for k in np.nonzero(keep)[0]:
+        vi += list(case.vidx[case.vptr[k]:case.vptr[k + 1]])
+        vp.append(len(vi))
+    return case.etype[keep], case.tag[keep], np.array(vp, np.int32), np.array(vi, np.int32)
+
+
+def test_malformed_element_streams_are_errors(lib):
+    """Empty streams, boundary elements without cells, vertex ids out of
+    range, unknown element types and short vertex lists are refused with an
+    error, never read past their arrays."""
+    one, _ = tet_cases()
+    z = np.zeros(0, np.int32)
+    cases = [s.CaseData(one.xyz, z, z, np.zeros(1, np.int32), z, one.regions),
+             s.CaseData(one.xyz, *_elements(one, one.etype < 4), one.regions)]
+    for bad_id in (99, -1):
+        vidx = one.vidx.copy()
+        vidx[-1] = bad_id
+        cases.append(s.CaseData(one.xyz, one.etype, one.tag, one.vptr, vidx, one.regions))
+    cases.append(s.CaseData(one.xyz, np.where(one.etype == 4, 9, one.etype).astype(np.int32), one.tag, one.vptr,
+                            one.vidx, one.regions))
+    vptr = one.vptr.copy()
+    vptr[-1] -= 1
+    cases.append(s.CaseData(one.xyz, one.etype, one.tag, vptr, one.vidx[:-1].copy(), one.regions))
+    for c in cases:
+        with pytest.raises(s.S355Error):
+            s.HostMesh(c)
+
+
 def test_ctx_create_fails_loudly_without_gpu(lib):
     from conftest import gpu_available
 
