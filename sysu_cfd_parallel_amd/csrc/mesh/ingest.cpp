@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <stdexcept>
 
 namespace s355 {
@@ -328,10 +329,17 @@ Mesh ingest_mesh(const s355_mesh_desc& d, bool metrics) {
     expand(i, [&](int, int, int, int) { ++c; });
     m.c_nface[i] = c;
   });
-  for (int i = 0; i < Nc; ++i) {
-    if (m.c_nface[i] > K) throw std::runtime_error("mesh: cell with more than 12 faces");
-    m.cf_ptr[i + 1] = m.cf_ptr[i] + m.c_nface[i];
+  {
+    // cell-face entries (and so half-face records) are indexed by 32 bits
+    long long total = 0;
+    for (int i = 0; i < Nc; ++i) {
+      if (m.c_nface[i] > K) throw std::runtime_error("mesh: cell with more than 12 faces");
+      total += m.c_nface[i];
+    }
+    if (total > std::numeric_limits<int>::max())
+      throw std::runtime_error("mesh: more cell faces than 32-bit indices address");
   }
+  for (int i = 0; i < Nc; ++i) m.cf_ptr[i + 1] = m.cf_ptr[i] + m.c_nface[i];
   const long H = m.cf_ptr[Nc];
   hvec<int> hv(4 * static_cast<size_t>(H));   // vertex order of each half-face
   hvec<int> hcell(H);                          // owning cell
@@ -584,6 +592,12 @@ Mesh ingest_mesh(const s355_mesh_desc& d, bool metrics) {
     int cols[kMaxFace + 1];
     m.rowptr[i + 1] = row_cols(i, cols);
   });
+  {
+    long long nnz = 0;
+    for (int i = 0; i < Nc; ++i) nnz += m.rowptr[i + 1];
+    if (nnz > std::numeric_limits<int>::max())
+      throw std::runtime_error("mesh: more matrix entries than 32-bit CSR indices address");
+  }
   for (int i = 0; i < Nc; ++i) m.rowptr[i + 1] += m.rowptr[i];
   m.col.resize(m.rowptr[Nc]);
   parallel_for(Nc, [&](long i) {
