@@ -368,18 +368,22 @@ __device__ __forceinline__ void sell_row(const SellView<T>& A, const Tiles& t, c
   int q0 = qa + (pos & 63);
   T v[B], xv[B];
   int c[B];
-#pragma unroll
-  for (int u = 0; u < B; ++u) {
-    v[u] = ld_stream<T, NT>(A.val + q0 + 64 * u);
-    c[u] = sell_col<NT, C16>(A, q0 + 64 * u, row);
-  }
   // the case: one case without a slice table; in batches per slice when the
-  // slice lies in one case, else by row
+  // slice lies in one case, else by row.  In a batch the state is read
+  // before the slots, so that the rows of cases that have finished load
+  // nothing (late in a batch's solve most rows are such); alone, the first
+  // batch of slots loads while the state does.
   const KState* sc = st;
   if (A.scase) {
     int cs = A.scase[sl];
     if (cs < 0) cs = t.case_of(row);
     sc = st + cs;
+    if (sc->reason) return;
+  }
+#pragma unroll
+  for (int u = 0; u < B; ++u) {
+    v[u] = ld_stream<T, NT>(A.val + q0 + 64 * u);
+    c[u] = sell_col<NT, C16>(A, q0 + 64 * u, row);
   }
   if (sc->reason) return;
   // used after the slots (PC: Jacobi, inv is set)
