@@ -143,7 +143,7 @@ _INWARD = ((0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6), (0, 2, 6, 4),
 
 
 def hybrid_channel(nx, ny, nz, prism_frac=0.3, pyramid_frac=0.3, jitter=0.0, seed=0, re=1000.0, lx=2.0, ly=1.0,
-                   lz=1.0):
+                   lz=1.0, turbulent=False):
     """The channel of ``s.channel`` (inlet x = 0, outlet x = lx, walls y = 0, ly,
     symmetry z = 0, lz) on a hybrid mesh: whole z-columns of the hex lattice
     split into two prisms along the (0,0)-(1,1) diagonal (their z faces
@@ -193,7 +193,7 @@ def hybrid_channel(nx, ny, nz, prism_frac=0.3, pyramid_frac=0.3, jitter=0.0, see
                         vol.append((7, 100, [v[c] for c in quad] + [apex]))
                 else:
                     vol.append((5, 100, [v[c] for c in _HEX]))
-    regions = s.channel(1, 1, 1, re=re).regions
+    regions = s.channel(1, 1, 1, re=re, turbulent=turbulent).regions
     case = manual_case(np.array(pts), bnd + vol, regions)
     case.meta = {"kind": "channel", "mesh": "hybrid", "nx": nx, "ny": ny, "nz": nz, "jitter": jitter, "seed": seed}
     return case

@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import sysu_cfd_parallel_amd as s
-from helpers import params, push, random_state, rel_err
+from helpers import hybrid_channel, params, push, random_state, rel_err
 from oracle.pyoracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -22,6 +22,8 @@ pytestmark = pytest.mark.gpu
 CASES = {
     "step_tets": lambda: s.step(2, upstream=2, downstream=4, span=2, turbulent=True),
     "channel_split": lambda: s.channel(8, 6, 5, jitter=0.15, split=0.2, seed=4, turbulent=True, re=8000.0),
+    "hybrid_prisms_pyramids": lambda: hybrid_channel(6, 4, 5, prism_frac=0.3, pyramid_frac=0.3, jitter=0.15, seed=5,
+                                                     re=8000.0, turbulent=True),
 }
 KE = ("KE_PROD", "KE_TESRC", "KE_EDSRC", "KE_APTE", "KE_APED", "KE_VISTE", "KE_VISED")
 

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import sysu_cfd_parallel_amd as s
-from helpers import heated, params, push, random_state, rel_err
+from helpers import heated, hybrid_channel, params, push, random_state, rel_err
 from oracle.pyoracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -16,6 +16,7 @@ CASES = {
     "cavity_jitter": lambda: s.cavity(6, jitter=0.15, seed=1),
     "channel_split": lambda: s.channel(8, 5, 6, jitter=0.15, split=0.15, seed=2),
     "step_tets": lambda: s.step(2, upstream=2, downstream=4, span=2, turbulent=True),
+    "hybrid_prisms_pyramids": lambda: hybrid_channel(6, 4, 5, prism_frac=0.3, pyramid_frac=0.3, jitter=0.15, seed=5),
 }
 
 
