@@ -347,8 +347,9 @@ constexpr int kSellBatch = 8;
 // before waiting on any), or of width > kSellBatch for W = 0 (batches of
 // kSellBatch slots, the guards wave-uniform).  Every column and value load of
 // a batch is issued, then every gather of x, then the products are added in
-// slot order.  The first batch loads before the state check, in parallel
-// with the row id and the state, which it does not depend on.
+// slot order.  With one case the first batch loads before the state check,
+// in parallel with the row id and the state, which it does not depend on; in
+// a batch the state of the row's case is checked first (see sell_row).
 // With C16 the columns are 16-bit offsets from the row (SellView::col16):
 // half the column bytes, added to the row id once both have loaded.
 template <bool NT, bool C16, class V>
