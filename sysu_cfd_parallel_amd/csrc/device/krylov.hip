@@ -363,7 +363,7 @@ __device__ __forceinline__ int sell_col(const V& A, size_t q, int row) {
 template <class T, bool NT, bool PC, int W, bool C16>
 __device__ __forceinline__ void sell_row(const SellView<T>& A, const Tiles& t, const T* __restrict__ x,
                                          const T* __restrict__ inv, T* __restrict__ y, const KState* st, int it,
-                                         int sl, int pos, int qa, int q1) {
+                                         int sl, int pos, int qa, int q1, double scl_in = -1.0) {
   constexpr int B = W > 0 ? W : kSellBatch;
   const int row = A.row ? A.row[pos] : pos;
   int q0 = qa + (pos & 63);
@@ -388,7 +388,8 @@ __device__ __forceinline__ void sell_row(const SellView<T>& A, const Tiles& t, c
   }
   if (sc->reason) return;
   // used after the slots (PC: Jacobi, inv is set)
-  const double scl = sc->scale[it];
+  // the basis vector's scale: from the state, or given (positive)
+  const double scl = scl_in > 0.0 ? scl_in : sc->scale[it];
   T invr = T(1);
   if constexpr (PC) invr = inv[row];
   T acc = T(0);
@@ -422,17 +423,17 @@ __device__ __forceinline__ void sell_row(const SellView<T>& A, const Tiles& t, c
 template <class T, bool NT, bool PC, bool C16>
 __device__ __forceinline__ void sell_row_any(const SellView<T>& A, const Tiles& t, const T* __restrict__ x,
                                              const T* __restrict__ inv, T* __restrict__ y, const KState* st, int it,
-                                             int sl, int pos, int qa, int q1) {
+                                             int sl, int pos, int qa, int q1, double scl = -1.0) {
   switch ((q1 - qa) >> 6) {
-    case 1: sell_row<T, NT, PC, 1, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
-    case 2: sell_row<T, NT, PC, 2, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
-    case 3: sell_row<T, NT, PC, 3, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
-    case 4: sell_row<T, NT, PC, 4, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
-    case 5: sell_row<T, NT, PC, 5, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
-    case 6: sell_row<T, NT, PC, 6, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
-    case 7: sell_row<T, NT, PC, 7, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
-    case 8: sell_row<T, NT, PC, 8, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
-    default: sell_row<T, NT, PC, 0, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1); break;
+    case 1: sell_row<T, NT, PC, 1, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1, scl); break;
+    case 2: sell_row<T, NT, PC, 2, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1, scl); break;
+    case 3: sell_row<T, NT, PC, 3, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1, scl); break;
+    case 4: sell_row<T, NT, PC, 4, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1, scl); break;
+    case 5: sell_row<T, NT, PC, 5, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1, scl); break;
+    case 6: sell_row<T, NT, PC, 6, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1, scl); break;
+    case 7: sell_row<T, NT, PC, 7, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1, scl); break;
+    case 8: sell_row<T, NT, PC, 8, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1, scl); break;
+    default: sell_row<T, NT, PC, 0, C16>(A, t, x, inv, y, st, it, sl, pos, qa, q1, scl); break;
   }
 }
 
@@ -969,25 +970,22 @@ __global__ void __launch_bounds__(kBlock) k_maxpy_norm(T* __restrict__ V, size_t
   if (threadIdx.x == 0) partial[blockIdx.x] = nrm;
 }
 
-// ---- Hessenberg column, Givens rotations, convergence test (one block per
-// case).  The column and the previous rotations are staged in LDS, so the
-// serial rotation sweep of thread 0 runs out of LDS rather than global memory.
-__global__ void __launch_bounds__(kBlock) k_arnoldi_finalize(const double* partial, Tiles t, int it, int m,
-                                                             KState* st) {
-  KState* s_ = st + blockIdx.x;
-  if (s_->reason) return;
-  __shared__ double lds[kBlock / kWave];
-  __shared__ double col[kMaxRestart + 2], cc[kMaxRestart + 1], ss[kMaxRestart + 1];
-  int t0, t1;
-  t.tiles_of(blockIdx.x, t0, t1);
-  double s = 0.0;
-  s = strided_sum(partial, t0, t1);
+// ---- Hessenberg column, Givens rotations, convergence test of step it of
+// one case, by one block.  The column and the previous rotations are staged
+// in LDS, so the serial rotation sweep of thread 0 runs out of LDS rather
+// than global memory.  Staging: every thread, before a barrier that
+// publishes col/cc/ss.
+__device__ __forceinline__ void arnoldi_stage(const KState* s_, int it, double* col, double* cc, double* ss) {
   for (int j = threadIdx.x; j <= it; j += kBlock) {
     col[j] = s_->hcol[j];
     cc[j] = s_->cc[j];
     ss[j] = s_->ss[j];
   }
-  s = block_sum(s, lds);  // contains the barrier that publishes col/cc/ss
+}
+
+// The rest, once staged and published: s (thread 0) is the squared norm of
+// the new basis vector.  Every thread of the block calls it.
+__device__ void arnoldi_close(double s, int it, int m, KState* s_, double* col, double* cc, double* ss) {
   const int ld = m + 1;
   if (threadIdx.x == 0) {
     const double tt = sqrt(s);
@@ -1034,6 +1032,91 @@ __global__ void __launch_bounds__(kBlock) k_arnoldi_finalize(const double* parti
   }
   __syncthreads();
   for (int j = threadIdx.x; j <= it + 1; j += kBlock) Hm(s_, ld, j, it) = col[j];
+}
+
+// one block per case, from the partial norms of the case's tiles
+__global__ void __launch_bounds__(kBlock) k_arnoldi_finalize(const double* partial, Tiles t, int it, int m,
+                                                             KState* st) {
+  KState* s_ = st + blockIdx.x;
+  if (s_->reason) return;
+  __shared__ double lds[kBlock / kWave];
+  __shared__ double col[kMaxRestart + 2], cc[kMaxRestart + 1], ss[kMaxRestart + 1];
+  int t0, t1;
+  t.tiles_of(blockIdx.x, t0, t1);
+  double s = strided_sum(partial, t0, t1);
+  arnoldi_stage(s_, it, col, cc, ss);
+  s = block_sum(s, lds);  // contains the barrier that publishes col/cc/ss
+  arnoldi_close(s, it, m, s_, col, cc, ss);
+}
+
+// The SELL-64 product of step it > 0 of one case together with the
+// finalisation of step it - 1, which would otherwise be a launch of its own
+// between them (small systems, where launches, not bytes, set the time).
+// Every block sums the partial norms of the basis vector it - 1 produced, in
+// the finalisation's order, for that vector's scale; block 0 also completes
+// step it - 1.  The other blocks may read the state before or after block 0
+// ends the solve there: the rows they then write belong to a basis vector
+// that is never used.  The partial norms stay valid until the next kernel.
+template <class T, bool NT, bool PC, bool C16>
+__global__ void __launch_bounds__(kBlock) k_spmv_sell_fin(SellView<T> A, Tiles t, const T* __restrict__ x,
+                                                          const T* __restrict__ inv, T* __restrict__ y, KState* st,
+                                                          const double* partial, int it, int m) {
+  __shared__ double lds[kBlock / kWave];
+  __shared__ double col[kMaxRestart + 2], cc[kMaxRestart + 1], ss[kMaxRestart + 1];
+  __shared__ double scl;
+  // block 0 alone writes the state in this launch, after its first barrier,
+  // so its threads all read the same value here
+  const bool fin = blockIdx.x == 0 && !st->reason;
+  int t0, t1;
+  t.tiles_of(0, t0, t1);
+  double s = strided_sum(partial, t0, t1);
+  if (fin) arnoldi_stage(st, it - 1, col, cc, ss);
+  s = block_sum(s, lds);
+  if (threadIdx.x == 0) {
+    const double tt = sqrt(s);
+    scl = tt != 0.0 ? 1.0 / tt : 1.0;
+  }
+  if (fin)
+    arnoldi_close(s, it - 1, m, st, col, cc, ss);
+  else
+    __syncthreads();
+  const int pos = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (pos >= A.n) return;
+  const int sl = __builtin_amdgcn_readfirstlane(pos >> 6);
+  sell_row_any<T, NT, PC, C16>(A, t, x, inv, y, st, it, sl, pos, A.ptr[sl], A.ptr[sl + 1], scl);
+}
+
+template <class T, bool NT, bool PC>
+void launch_sell_fin_c(const SellView<T>& sell, const Tiles& t, const T* x, const T* inv, T* y, KState* st,
+                       const double* partial, int it, int m, hipStream_t s) {
+  const int g = grid_for(sell.n);
+  if (sell.col16) {
+    if constexpr (std::is_same<T, float>::value)
+      hipLaunchKernelGGL((k_spmv_sell_fin<T, NT, PC, true>), dim3(g), dim3(kBlock), 0, s, sell, t, x, inv, y, st,
+                         partial, it, m);
+    else
+      throw std::logic_error("16-bit SELL columns in an FP64 product");
+  } else {
+    hipLaunchKernelGGL((k_spmv_sell_fin<T, NT, PC, false>), dim3(g), dim3(kBlock), 0, s, sell, t, x, inv, y, st,
+                       partial, it, m);
+  }
+}
+
+// y = scale M^-1 A x on the one-slice SELL-64 copy with the finalisation of
+// step it - 1 (one case, no ILU)
+template <class T>
+void launch_sell_fin(int nt, const SellView<T>& sell, const Tiles& t, const T* x, const T* inv, T* y, KState* st,
+                     const double* partial, int it, int m, hipStream_t s) {
+  if (!sell.ptr || sell.two || sell.scase) throw std::logic_error("fused SELL-64 product on another layout");
+  if (it < 1) throw std::logic_error("fused SELL-64 product without a previous step");
+  if (nt && inv)
+    launch_sell_fin_c<T, true, true>(sell, t, x, inv, y, st, partial, it, m, s);
+  else if (nt)
+    launch_sell_fin_c<T, true, false>(sell, t, x, inv, y, st, partial, it, m, s);
+  else if (inv)
+    launch_sell_fin_c<T, false, true>(sell, t, x, inv, y, st, partial, it, m, s);
+  else
+    launch_sell_fin_c<T, false, false>(sell, t, x, inv, y, st, partial, it, m, s);
 }
 
 // ---- back substitution R y = g on the completed columns of each case, y_j
@@ -1675,12 +1758,30 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
                        zero_guess ? 1 : 0, mixed ? 1 : 0);
     S355_LAUNCH_CHECK();
   };
+  // One case on a small system (at most kBlock tiles) without ILU, where
+  // launches rather than bytes set the time: the finalisation of step it - 1
+  // runs inside the SELL product of step it.  It is pending after each step
+  // and launched on its own when the product does not take it (before a
+  // state read, at the cycle's end, before a timed product).
+  const bool fuse_fin = nc == 1 && tl.ntile <= kBlock && !lu && L == 0 && !sell.two && !sell.scase;
+  int fin_pending = -1;
+  auto flush_fin = [&] {
+    if (fin_pending < 0) return;
+    hipLaunchKernelGGL(k_arnoldi_finalize, dim3(nc), dim3(kBlock), 0, s, w.partial, tl, fin_pending, m, w.st);
+    S355_LAUNCH_CHECK();
+    fin_pending = -1;
+  };
   auto enqueue_iteration = [&](int it, bool timed) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     const bool rec = timed && w.timer && (w.timer->seen++ % w.timer->every) == 0 && w.timer->take(e0, e1);
+    const bool fused = fuse_fin && !rec && it > 0 && fin_pending == it - 1;
+    if (!fused) flush_fin();
     if (rec) S355_HIP(hipEventRecord(e0, s));
     if (lu) {
       launch_spmv<T>(L, w.nt, A, sell, tl, V + it * ld, static_cast<const T*>(nullptr), tmp, w.st, it, s);
+    } else if (fused) {
+      launch_sell_fin<T>(w.nt, sell, tl, V + it * ld, inv, V + (it + 1) * ld, w.st, w.partial, it, m, s);
+      fin_pending = -1;
     } else {
       launch_spmv<T>(L, w.nt, A, sell, tl, V + it * ld, inv, V + (it + 1) * ld, w.st, it, s);
     }
@@ -1689,10 +1790,14 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
     Dispatch<T>::mdot(R, V, ld, tl, it, w.partial, pstride, w.st, s);
     hipLaunchKernelGGL(k_mdot_final, dim3(it + 1, nc), dim3(kBlock), 0, s, w.partial, tl, pstride, w.st);
     Dispatch<T>::maxpy(R, V, ld, tl, it, w.partial, w.st, s);
-    hipLaunchKernelGGL(k_arnoldi_finalize, dim3(nc), dim3(kBlock), 0, s, w.partial, tl, it, m, w.st);
+    if (fuse_fin)
+      fin_pending = it;
+    else
+      hipLaunchKernelGGL(k_arnoldi_finalize, dim3(nc), dim3(kBlock), 0, s, w.partial, tl, it, m, w.st);
     S355_LAUNCH_CHECK();
   };
   auto enqueue_end = [&] {
+    flush_fin();
     hipLaunchKernelGGL(k_build_y, dim3(nc), dim3(kWave), 0, s, w.st, m);
     Dispatch<T>::update(R, V, ld, tl, x, w.st, w.ord_row, s);
     hipLaunchKernelGGL(k_cycle_end, dim3(grid_for(nc)), dim3(kBlock), 0, s, w.st, nc, mixed ? 1 : 0);
@@ -1750,6 +1855,7 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
       ++it;
       ++its;
       if (it % check_every == 0 && it < m) {
+        flush_fin();
         read_headers(w, s);
         if (all_stopped(w)) break;
       }
@@ -1834,6 +1940,7 @@ __global__ void k_bench_init(KState* st, int ncase) {
   for (int c = threadIdx.x; c < ncase; c += blockDim.x) {
     st[c].reason = 0;
     st[c].scale[0] = 1.0;
+    st[c].scale[1] = 1.0;
   }
 }
 
@@ -2025,6 +2132,11 @@ void autotune(const CsrView<T>& A, KWork& w, int m, hipStream_t s) {
   hipLaunchKernelGGL(k_bench_init, dim3(1), dim3(kBlock), 0, s, w.st, w.ncase);
   S355_LAUNCH_CHECK();
   const T* none = nullptr;
+  // One small case: the solve runs the SELL product of step j together with
+  // the finalisation of step j - 1, and any other product beside a launch of
+  // its own, so every candidate is timed with that finalisation (and a reset
+  // of the state, the same for all)
+  const bool with_fin = w.ncase == 1 && w.tiles[0].ntile <= kBlock && m >= 2;
   TuneReport rep;
   int best_l = 8, best_nt = 0;
   double best = 1e300;
@@ -2040,8 +2152,23 @@ void autotune(const CsrView<T>& A, KWork& w, int m, hipStream_t s) {
                            pt.slots, A.val, static_cast<T*>(w.sell_vals[sv]));
       }
       for (int nt : {0, 1}) {
-        const double ms =
-            time_ms([&] { launch_spmv<T>(L, nt, A, sell, w.tiles[0], V, none, V + ld, w.st, 0, s); }, 5, s);
+        const double ms = time_ms(
+            [&] {
+              if (with_fin) {
+                // (a finalisation may end the timing solve: the state is reset first)
+                hipLaunchKernelGGL(k_bench_init, dim3(1), dim3(kBlock), 0, s, w.st, w.ncase);
+                if (L == 0 && !sell.two) {
+                  launch_sell_fin<T>(nt, sell, w.tiles[0], V, none, V + ld, w.st, w.partial, 1, m, s);
+                } else {
+                  launch_spmv<T>(L, nt, A, sell, w.tiles[0], V, none, V + ld, w.st, 1, s);
+                  hipLaunchKernelGGL(k_arnoldi_finalize, dim3(1), dim3(kBlock), 0, s, w.partial, w.tiles[0], 0, m,
+                                     w.st);
+                }
+              } else {
+                launch_spmv<T>(L, nt, A, sell, w.tiles[0], V, none, V + ld, w.st, 0, s);
+              }
+            },
+            5, s);
         rep.spmv_ms[spmv_variant(L, nt, sv)] = ms;
         if (ms < best) {
           best = ms;
@@ -2070,7 +2197,11 @@ void autotune(const CsrView<T>& A, KWork& w, int m, hipStream_t s) {
   }
   // the Gram-Schmidt launches of three Arnoldi steps across the cycle (a
   // quarter, half and three quarters of the restart length) at every tile
-  // size; the tile count also sets the work of the final reduction
+  // size; the tile count also sets the work of the final reduction.  The
+  // state is running again first (a finalisation above may have ended it,
+  // and the kernels of an ended solve return at once).
+  hipLaunchKernelGGL(k_bench_init, dim3(1), dim3(kBlock), 0, s, w.st, w.ncase);
+  S355_LAUNCH_CHECK();
   const int it = m / 2;
   int best_r = pick_r(n, sizeof(T));
   best = 1e300;

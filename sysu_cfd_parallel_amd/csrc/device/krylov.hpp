@@ -203,7 +203,9 @@ struct GraphCache {
 
 // Timings taken by autotune(): SpMV per variant v = 2 k + nt (k = 0..4: CSR
 // with 1, 2, 4, 8, 16 lanes per row, k = 5..7: SELL-64 with window 64, 512,
-// 4096; nt: non-temporal matrix loads) and one Gram-Schmidt step (multi-dot + update at iteration gs_it)
+// 4096; nt: non-temporal matrix loads; on one case of at most kBlock tiles
+// with the Arnoldi finalisation the solve runs beside or inside the product)
+// and one Gram-Schmidt step (multi-dot + update at iteration gs_it)
 // per rows-per-thread 1, 2, 4, 8, 16; -1 = not timed.
 struct TuneReport {
   // CSR lanes 1, 2, 4, 8, 16 then SELL-64 with windows 64, 512, 4096, each
