@@ -1052,11 +1052,12 @@ __global__ void __launch_bounds__(kBlock) k_arnoldi_finalize(const double* parti
 // The SELL-64 product of step it > 0 of one case together with the
 // finalisation of step it - 1, which would otherwise be a launch of its own
 // between them (small systems, where launches, not bytes, set the time).
-// Every block sums the partial norms of the basis vector it - 1 produced, in
-// the finalisation's order, for that vector's scale; block 0 also completes
-// step it - 1.  The other blocks may read the state before or after block 0
-// ends the solve there: the rows they then write belong to a basis vector
-// that is never used.  The partial norms stay valid until the next kernel.
+// Every block sums the tile norms of basis vector it (step it - 1 made it),
+// in the finalisation's order, for that vector's scale; block 0 also
+// completes step it - 1.  The other blocks may read the state before or
+// after block 0 ends the solve there: the rows they then write belong to a
+// basis vector that is never used.  The tile norms stay valid until the next
+// kernel (the multi-dot of step it writes over them).
 template <class T, bool NT, bool PC, bool C16>
 __global__ void __launch_bounds__(kBlock) k_spmv_sell_fin(SellView<T> A, Tiles t, const T* __restrict__ x,
                                                           const T* __restrict__ inv, T* __restrict__ y, KState* st,
