@@ -2136,7 +2136,9 @@ void autotune(const CsrView<T>& A, KWork& w, int m, hipStream_t s) {
   // One small case: the solve runs the SELL product of step j together with
   // the finalisation of step j - 1, and any other product beside a launch of
   // its own, so every candidate is timed with that finalisation (and a reset
-  // of the state, the same for all)
+  // of the state, the same for all).  The shapes are tuned once for every
+  // solve of the context: an ILU(0) solve, which keeps the two apart, then
+  // runs the shape timed with the saving it does not take.
   const bool with_fin = w.ncase == 1 && w.tiles[0].ntile <= kBlock && m >= 2;
   TuneReport rep;
   int best_l = 8, best_nt = 0;
