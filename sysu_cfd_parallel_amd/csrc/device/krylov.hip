@@ -1809,9 +1809,15 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
     // every restart cycle is one captured graph (the first cycle has its own:
     // it sets the reference norm and may skip the product with a zero
     // guess); a cycle always runs m iterations, the kernels of a finished
-    // case returning at once, and the iteration cap is applied on the device
+    // case returning at once, and the iteration cap is applied on the device.
+    // The first cycle runs as two graphs, its first ksplit steps and the
+    // rest, with a state read between: a solve that has ended by then (the
+    // momentum solves often do) runs only the cycle's end, not m - ksplit
+    // steps of kernels that would return at once.  The same kernels run in
+    // the same order either way.
+    const int ksplit = m >= 8 ? (2 * m) / 5 : m;
     for (bool first_c = true;; first_c = false) {
-      const std::vector<uintptr_t> key = {reinterpret_cast<uintptr_t>(A.val), reinterpret_cast<uintptr_t>(Ares.val),
+      std::vector<uintptr_t> key = {reinterpret_cast<uintptr_t>(A.val), reinterpret_cast<uintptr_t>(Ares.val),
                                           reinterpret_cast<uintptr_t>(b), reinterpret_cast<uintptr_t>(x),
                                           reinterpret_cast<uintptr_t>(inv), reinterpret_cast<uintptr_t>(invr),
                                           static_cast<uintptr_t>(m), static_cast<uintptr_t>(R),
@@ -1820,13 +1826,35 @@ KResult gmres(const CsrView<T>& A, const CsrView<X>& Ares, const X* b, X* x, con
                                           static_cast<uintptr_t>(w.sell2 + 1), sizeof(T), sizeof(X),
                                           reinterpret_cast<uintptr_t>(w.ord_row),
                                           reinterpret_cast<uintptr_t>(sell.col16),
-                                          static_cast<uintptr_t>(first_c), static_cast<uintptr_t>(zero_guess)};
-      hipGraphExec_t g = w.graphs->get(key, s, [&] {
-        enqueue_restart(first_c, first_c && zero_guess);
-        for (int it = 0; it < m; ++it) enqueue_iteration(it, false);
-        enqueue_end();
-      });
-      S355_HIP(hipGraphLaunch(g, s));
+                                          static_cast<uintptr_t>(first_c), static_cast<uintptr_t>(zero_guess), 0};
+      // segment: 0 a whole cycle, 1 the first ksplit steps, 2 the rest and
+      // the end, 3 the end alone
+      auto launch = [&](uintptr_t seg, const std::function<void()>& record) {
+        key.back() = seg;
+        S355_HIP(hipGraphLaunch(w.graphs->get(key, s, record), s));
+      };
+      if (first_c && ksplit < m) {
+        launch(1, [&] {
+          enqueue_restart(true, zero_guess);
+          for (int it = 0; it < ksplit; ++it) enqueue_iteration(it, false);
+          flush_fin();
+        });
+        read_headers(w, s);
+        if (all_stopped(w)) {
+          launch(3, [&] { enqueue_end(); });
+        } else {
+          launch(2, [&] {
+            for (int it = ksplit; it < m; ++it) enqueue_iteration(it, false);
+            enqueue_end();
+          });
+        }
+      } else {
+        launch(0, [&] {
+          enqueue_restart(first_c, first_c && zero_guess);
+          for (int it = 0; it < m; ++it) enqueue_iteration(it, false);
+          enqueue_end();
+        });
+      }
       read_headers(w, s);
       if (all_done(w)) break;
     }
