@@ -388,6 +388,92 @@ def test_gmres_reports_divergence_at_the_cap():
     assert info[0]["reason"] == -3 and info[0]["iters"] == 2
 
 
+def _minres_history(A, b, x0, B, m, kmax):
+    """||B (b - A x_k)|| for k = 0..kmax of GMRES(m) with left preconditioner
+    B, each cycle the minimal residual over its Krylov space by dense least
+    squares on a twice re-orthogonalised basis (no Hessenberg matrix, no
+    rotations); also the iterate after each step."""
+    BA = B @ A
+    x = x0.copy()
+    hist, xs = [np.linalg.norm(B @ (b - A @ x))], [x.copy()]
+    while len(hist) <= kmax:
+        r = B @ (b - A @ x)
+        Q = [r / np.linalg.norm(r)]
+        W = []
+        for _ in range(m):
+            W.append(BA @ Q[-1])
+            Wm = np.array(W).T
+            y = np.linalg.lstsq(Wm, r, rcond=None)[0]
+            xk = x + np.array(Q[:len(W)]).T @ y
+            hist.append(np.linalg.norm(r - Wm @ y))
+            xs.append(xk)
+            if len(hist) > kmax:
+                break
+            v = W[-1].copy()
+            for _ in range(2):
+                for q in Q:
+                    v -= (q @ v) * q
+            Q.append(v / np.linalg.norm(v))
+        x = xk
+    return np.array(hist), xs
+
+
+@pytest.mark.parametrize("restart", [30, 3])
+def test_gmres_iteration_count_and_residual_are_the_minimal_residual_ones(restart):
+    # PETSc's GMRES(m) with left Jacobi preconditioning and a nonzero initial
+    # guess stops at the first step whose minimal preconditioned residual is
+    # at most rtol * ||B b|| (KSPConvergedDefault); the tolerance is placed
+    # between two consecutive residuals of the dense restatement, so a dropped
+    # or mis-signed rotation term, a wrong reference norm or a restart from
+    # the wrong iterate changes the count or the residual
+    o = Oracle(s.cavity(4, jitter=0.1, seed=2), params(restart=restart))
+    z = o.sizes
+    rng = np.random.default_rng(11)
+    push(o, {k: v for k, v in random_state(z.n_cell, z.n_bnd, z.n_face, 4).items() if k in ("U", "V", "W", "P")})
+    o.set("R", np.ones(z.n_cell))
+    o.call("build_velocity_matrix")
+    A = _dense(o, "AU")
+    B = np.diag(1.0 / np.diag(A))
+    b = rng.normal(size=(3, z.n_cell))
+    x0 = rng.normal(size=(3, z.n_cell))
+    for k, n in enumerate(("BU_RHS", "BV_RHS", "BW_RHS")):
+        o.set(n, b[k])
+    for k, n in enumerate("UVW"):
+        o.set(n, x0[k])
+    hist, xs, kstar, rtol = [], [], [], []
+    for k in range(3):
+        h, x = _minres_history(A, b[k], x0[k], B, restart, 40)
+        # the first step from 6 on that beats its predecessor by 5 %
+        ks = next(j for j in range(6, 41) if h[j - 1] > 1.05 * h[j] and h[:j].min() > 1.05 * h[j])
+        rnorm0 = np.linalg.norm(B @ b[k])
+        hist.append(h)
+        xs.append(x)
+        kstar.append(ks)
+        rtol.append(np.sqrt(min(h[:ks].min(), h[ks - 1]) * h[ks]) / rnorm0)
+    # one tolerance for the three solves: each component's count is checked
+    # against its own first crossing of that tolerance
+    tol = rtol[0]
+    info = o.solve_velocity(tol, 1000)
+    for k, n in enumerate("UVW"):
+        rnorm0 = np.linalg.norm(B @ b[k])
+        h = hist[k]
+        crossing = int(np.argmax(h <= tol * rnorm0))
+        assert h[crossing] <= tol * rnorm0
+        margin = min(h[:crossing].min() / (tol * rnorm0), tol * rnorm0 / h[crossing])
+        i = info[k]
+        assert i["reason"] == 2, (n, i)
+        assert rel_err(np.array([i["rnorm0"]]), np.array([rnorm0])) < 1e-12, n
+        if margin > 1.01:
+            assert i["iters"] == crossing, (n, i["iters"], crossing)
+            assert abs(i["rnorm"] - h[crossing]) <= 1e-6 * h[crossing], n
+            # the returned iterate: its true preconditioned residual is the
+            # minimal one, and it is the dense restatement's iterate
+            u = o.get(n)
+            assert abs(np.linalg.norm(B @ (b[k] - A @ u)) - h[crossing]) <= 1e-6 * h[crossing], n
+            assert rel_err(u, xs[k][crossing]) < 1e-8, n
+    assert info[0]["iters"] == kstar[0]
+
+
 # ---------------------------------------------------------------- k-epsilon
 def test_kepsilon_wall_function_and_mut():
     case = s.channel(1, 1, 1, lx=1, ly=1, lz=1, turbulent=True, re=1000)
