@@ -8,7 +8,9 @@ closed forms and dense solves.
   minimum-norm least-squares solution (no preconditioner: the Krylov space of
   a consistent right-hand side stays in the range) or up to the null space
   (Jacobi, ILU(0)); the stopping test is checked with an explicit
-  ||M^-1 (b - A x)||.
+  ||M^-1 (b - A x)||; with a tolerance placed between the residuals of two
+  steps, CG stops at the step and with the iterate that the Galerkin
+  condition over an explicit Krylov basis gives.
 * Production P_k = mu_t (grad u + grad u^T) : grad u (navier_scalar.cpp:46-50)
   for an asymmetric velocity gradient, against numpy.
 * The wall function (navier_bc.cpp:262-285) in both y+ branches on two wall
@@ -107,6 +109,55 @@ def test_cg_on_the_singular_pressure_system(pc):
     # every preconditioner: x_min up to the null space (the constant)
     d = (x - x.mean()) - (xmin - xmin.mean())
     assert np.linalg.norm(d) <= 1e-7 * np.linalg.norm(xmin)
+
+
+def _galerkin_history(A, b, M_inv, kmax):
+    """Preconditioned CG from zero, restated densely: x_k is the Galerkin
+    solution over K_k(M^-1 A, M^-1 b) (Q^T (b - A x_k) = 0 on an explicit,
+    twice re-orthogonalised basis Q; no recurrences), returned with
+    ||M^-1 (b - A x_k)|| for k = 0..kmax."""
+    n = A.shape[0]
+    Q = []
+    v = M_inv @ b
+    xs, h = [np.zeros(n)], [np.linalg.norm(M_inv @ b)]
+    for _ in range(kmax):
+        for _ in range(2):
+            for q in Q:
+                v = v - (q @ v) * q
+        Q.append(v / np.linalg.norm(v))
+        Qm = np.array(Q).T
+        x = Qm @ np.linalg.solve(Qm.T @ A @ Qm, Qm.T @ b)
+        xs.append(x)
+        h.append(np.linalg.norm(M_inv @ (b - A @ x)))
+        v = M_inv @ (A @ Q[-1])
+    return np.array(h), xs
+
+
+def test_cg_stops_at_the_galerkin_step_with_its_iterate():
+    # PETSc's CG (preconditioned norm, zero guess) stops at the first step
+    # whose ||M^-1 r_k|| is at most rtol ||M^-1 b||; with the tolerance placed
+    # where every earlier residual is 5 % above it and that step's 5 % below,
+    # the oracle must stop there with the Galerkin iterate of the Krylov
+    # space (a wrong step length, search direction or norm changes the step
+    # or the iterate)
+    o, A = pressure_system(s.PC_JACOBI, seed=7)
+    n = A.shape[0]
+    A = A - np.diag(0.05 * np.abs(np.diag(A)))
+    A = 0.5 * (A + A.T)  # symmetric to rounding already; exactly, for the restatement
+    set_dense(o, "AP", A)
+    b = np.random.default_rng(8).normal(size=n)
+    o.set("BP_RHS", b)
+    M_inv = np.diag(1.0 / np.diag(A))
+    h, xs = _galerkin_history(A, b, M_inv, 40)
+    ks = next(j for j in range(5, 41) if h[:j].min() > 1.1025 * h[j])
+    tol = np.sqrt(h[:ks].min() * h[ks]) / h[0]
+    info = o.solve_pressure(tol, 2000)
+    assert info["reason"] == 2, info
+    assert info["iters"] == ks, (info["iters"], ks)
+    assert abs(info["rnorm0"] - h[0]) <= 1e-12 * h[0]
+    x = o.get("DELTAP")
+    assert np.linalg.norm(x - xs[ks]) <= 1e-8 * np.linalg.norm(xs[ks])
+    assert abs(info["rnorm"] - h[ks]) <= 1e-6 * h[ks]
 
 
 def test_cg_reports_an_indefinite_matrix():
