@@ -376,6 +376,41 @@ def test_ilu0_is_exact_on_tridiagonal():
     assert all(i["iters"] == 1 for i in info)
 
 
+def test_ilu0_is_the_pattern_restricted_factorisation():
+    # ILU(0) is the one factorisation M = L U with unit lower L and upper U
+    # both supported on the matrix pattern and (L U)_ij = a_ij on the pattern
+    # (fill off it is dropped); M is recovered from the preconditioner's
+    # action on the unit vectors and factored densely without pivoting
+    from oracle.pyoracle import pc_apply
+
+    # pyramids around a hexahedron's centre: three cells around each
+    # centre-corner edge are pairwise face neighbours, so earlier rows update
+    # later rows' upper entries (on hexahedra or these tetrahedra only the
+    # diagonal is ever updated)
+    o = Oracle(hybrid_channel(4, 3, 2, prism_frac=0.3, pyramid_frac=0.5, seed=3), params(pc=s.PC_ILU0))
+    n = o.sizes.n_cell
+    rp, col = o.ints("CSR_ROWPTR"), o.ints("CSR_COL")
+    rng = np.random.default_rng(9)
+    A = np.zeros((n, n))  # a nonsymmetric matrix on the mesh's pattern
+    for i in range(n):
+        for j in col[rp[i]:rp[i + 1]]:
+            A[i, j] = 8.0 + rng.uniform(0, 1) if i == j else rng.uniform(-1, 1)
+    _set_matrix(o, A)
+    val = o.get("AU")
+    Minv = np.column_stack([pc_apply(rp, col, val, s.PC_ILU0, e) for e in np.eye(n)])
+    M = np.linalg.inv(Minv)
+    L, U = np.eye(n), M.copy()
+    for k in range(n):  # Doolittle without pivoting
+        L[k + 1:, k] = U[k + 1:, k] / U[k, k]
+        U[k + 1:, :] -= np.outer(L[k + 1:, k], U[k, :])
+    on = A != 0.0
+    scale = np.abs(A).max()
+    assert np.abs(np.tril(L, -1)[~on]).max() < 1e-10
+    assert np.abs(np.triu(U)[~on]).max() < 1e-10 * scale
+    assert np.abs((M - A)[on]).max() < 1e-10 * scale
+    assert np.abs((M - A)[~on]).max() > 1e-6 * scale  # the dropped fill is real
+
+
 def test_gmres_reports_divergence_at_the_cap():
     o = Oracle(s.cavity(4, jitter=0.1, seed=2), params(restart=5, pc=s.PC_NONE))
     z = o.sizes
