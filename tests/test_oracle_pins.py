@@ -453,22 +453,36 @@ def _minres_history(A, b, x0, B, m, kmax):
     return np.array(hist), xs
 
 
-@pytest.mark.parametrize("restart", [30, 3])
-def test_gmres_iteration_count_and_residual_are_the_minimal_residual_ones(restart):
-    # PETSc's GMRES(m) with left Jacobi preconditioning and a nonzero initial
+def _dense_ilu0_inverse(A):
+    """(L U)^-1 of ILU(0) restated densely: IKJ elimination that updates only
+    the entries on the matrix pattern."""
+    n = A.shape[0]
+    on = A != 0.0
+    LU = A.copy()
+    for i in range(1, n):
+        for k in np.flatnonzero(on[i, :i]):
+            LU[i, k] /= LU[k, k]
+            m = on[i, k + 1:]
+            LU[i, k + 1:][m] -= LU[i, k] * LU[k, k + 1:][m]
+    return np.linalg.inv((np.tril(LU, -1) + np.eye(n)) @ np.triu(LU))
+
+
+@pytest.mark.parametrize("restart,pc", [(30, s.PC_JACOBI), (3, s.PC_JACOBI), (30, s.PC_ILU0)])
+def test_gmres_iteration_count_and_residual_are_the_minimal_residual_ones(restart, pc):
+    # PETSc's GMRES(m) with left Jacobi or ILU(0) preconditioning and a nonzero initial
     # guess stops at the first step whose minimal preconditioned residual is
     # at most rtol * ||B b|| (KSPConvergedDefault); the tolerance is placed
     # between two consecutive residuals of the dense restatement, so a dropped
     # or mis-signed rotation term, a wrong reference norm or a restart from
     # the wrong iterate changes the count or the residual
-    o = Oracle(s.cavity(4, jitter=0.1, seed=2), params(restart=restart))
+    o = Oracle(s.cavity(4, jitter=0.1, seed=2), params(restart=restart, pc=pc))
     z = o.sizes
     rng = np.random.default_rng(11)
     push(o, {k: v for k, v in random_state(z.n_cell, z.n_bnd, z.n_face, 4).items() if k in ("U", "V", "W", "P")})
     o.set("R", np.ones(z.n_cell))
     o.call("build_velocity_matrix")
     A = _dense(o, "AU")
-    B = np.diag(1.0 / np.diag(A))
+    B = np.diag(1.0 / np.diag(A)) if pc == s.PC_JACOBI else _dense_ilu0_inverse(A)
     b = rng.normal(size=(3, z.n_cell))
     x0 = rng.normal(size=(3, z.n_cell))
     for k, n in enumerate(("BU_RHS", "BV_RHS", "BW_RHS")):
